@@ -1,0 +1,524 @@
+// hcx_api.hip -- the C ABI of libhcx (declared in include/hcx.h).
+//
+// Device memory is owned by the library: an index keeps its iA/bA/pepEntry
+// arrays in HBM for its lifetime, a query batch keeps its spectra and result
+// buffers.  Host buffers passed in are only read or written during the call.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <new>
+
+#include "hcx_internal.h"
+
+namespace hcx {
+
+static thread_local std::string g_err;
+
+int set_error(const std::string &msg) {
+    g_err = msg;
+    return HCX_ERR_DEVICE;
+}
+int set_error_code(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+int DeviceBuf::ensure(size_t n) {
+    if (n <= bytes && p) return 0;
+    release();
+    if (n == 0) n = 16;
+    if (hipMalloc(&p, n) != hipSuccess) {
+        p = nullptr;
+        bytes = 0;
+        (void)hipGetLastError();
+        return 1;
+    }
+    bytes = n;
+    return 0;
+}
+void DeviceBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+size_t query_smem_bytes(const QueryArgs &a, bool replay);
+int launch_candidates(hcx_ctx *ctx, const QueryArgs &a);
+
+}  // namespace hcx
+
+using namespace hcx;
+
+extern "C" const char *hcx_last_error(void) { return g_err.c_str(); }
+
+extern "C" int hcx_heap_capacity(uint32_t topmatches) {
+    // dslim.cpp:422-428: 2^floor(log2(t) + 0.999) - 1
+    uint32_t num = topmatches == 0 ? 1 : topmatches;
+    num = (uint32_t)std::floor(std::log2((double)num) + 0.999);
+    return (1 << num) - 1;
+}
+
+extern "C" int hcx_create(const hcx_params *params, int device, hcx_ctx **out) {
+    if (!params || !out) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_create: null argument");
+    const hcx_params &p = *params;
+    if (p.maxz == 0 || p.maxz > 4 || p.scale == 0 || p.max_mass == 0 || p.max_mass <= p.min_mass)
+        return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_create: bad charge/scale/mass range");
+    if ((uint64_t)p.max_mass * p.scale >= (1u << 30))
+        return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_create: max_mass*scale too large");
+    if (p.max_hyperscore == 0 || p.max_hyperscore > 1000 || p.max_shdpeaks == 0 || p.max_shdpeaks > 4096)
+        return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_create: bad max_hyperscore/max_shdpeaks");
+    if (p.nmods > HCX_MAX_MOD_TYPES) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_create: too many mods");
+    if ((uint64_t)p.max_mass * p.scale <= 2ull * p.dF + 2)
+        return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_create: dF too large");
+    const int cap = hcx_heap_capacity(p.topmatches);
+    if (cap <= 0 || cap > 255)
+        return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_create: topmatches gives an empty or huge heap");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
+        (void)hipGetLastError();
+        return set_error_code(HCX_ERR_DEVICE, "hcx_create: no HIP device");
+    }
+    hcx_ctx *c = new (std::nothrow) hcx_ctx;
+    if (!c) return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_create: out of memory");
+    c->params = p;
+    c->device = device;
+    c->NB = p.max_mass * p.scale;
+    c->hbins = 10 * p.max_hyperscore + 2;
+    c->heap_cap = cap;
+    auto fail = [&](const char *what) {
+        int rc = set_error(std::string("hcx_create: ") + what);
+        hcx_destroy(c);
+        return rc;
+    };
+    if (hipSetDevice(device) != hipSuccess) return fail("hipSetDevice");
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail("hipGetDeviceProperties");
+    c->num_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail("stream");
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreate(&c->ev2) != hipSuccess)
+        return fail("events");
+    // utils.h:115-131 lgfact table: val[i] = log10(i) + val[i-1]
+    c->h_lgf.assign(p.max_shdpeaks + 1, 0.0);
+    for (uint32_t i = 2; i <= p.max_shdpeaks; i++) c->h_lgf[i] = std::log10((double)i) + c->h_lgf[i - 1];
+    if (hipMalloc(&c->d_lgf, sizeof(double) * c->h_lgf.size()) != hipSuccess) return fail("lgfact alloc");
+    if (hipMemcpy(c->d_lgf, c->h_lgf.data(), sizeof(double) * c->h_lgf.size(), hipMemcpyHostToDevice) !=
+        hipSuccess)
+        return fail("lgfact copy");
+    *out = c;
+    return HCX_SUCCESS;
+}
+
+extern "C" int hcx_destroy(hcx_ctx *c) {
+    if (!c) return HCX_SUCCESS;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    c->lens.release();
+    c->tie.release();
+    c->work.release();
+    c->sur.release();
+    c->sur_tmp.release();
+    if (c->d_lgf) (void)hipFree(c->d_lgf);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->ev2) (void)hipEventDestroy(c->ev2);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return HCX_SUCCESS;
+}
+
+extern "C" int hcx_lgfact_table(const hcx_ctx *c, double *out, uint32_t n) {
+    if (!c || !out) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_lgfact_table: null");
+    if (n > c->h_lgf.size()) return set_error_code(HCX_ERR_INVLD_SIZE, "hcx_lgfact_table: n too large");
+    std::memcpy(out, c->h_lgf.data(), sizeof(double) * n);
+    return HCX_SUCCESS;
+}
+
+extern "C" int hcx_index_build(hcx_ctx *c, uint32_t peplen, const char *seqs, uint32_t nseqs,
+                               const hcx_pep_entry *entries, uint32_t nentries, hcx_index **out) {
+    if (!c || !out || (nentries && (!seqs || !entries)))
+        return set_error_code(HCX_ERR_INVLD_PTR, "hcx_index_build: null argument");
+    if (peplen < 2 || peplen > 63) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_index_build: bad peptide length");
+    const hcx_params &p = c->params;
+    const uint32_t speclen = 2 * p.maxz * (peplen - 1);
+    const uint64_t nions = (uint64_t)nentries * speclen;
+    if (nions >= 0xFFFFFFF0ull)
+        return set_error_code(HCX_ERR_INVLD_SIZE, "hcx_index_build: more than 2^32 ions in one length");
+    for (uint64_t i = 0; i < (uint64_t)nseqs * peplen; i++)
+        if (seqs[i] < 'A' || seqs[i] > 'Z')
+            return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_index_build: sequence characters must be A-Z");
+    for (uint32_t k = 0; k < nentries; k++) {
+        const hcx_pep_entry &e = entries[k];
+        if (e.seq_id >= nseqs)
+            return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_index_build: entry seq_id out of range");
+        if (e.mod_num == 0) continue;
+        // modAA (slm_dsts.h:120-144): one nibble (type + 1) per modified site
+        int nib = 0;
+        for (int i = 0; i < 8; i++) {
+            uint32_t v = (e.mod_num >> (4 * i)) & 0xF;
+            if (v == 0) break;
+            if (v > p.nmods) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_index_build: mod type out of range");
+            nib++;
+        }
+        if ((nib < 8 && (e.mod_num >> (4 * nib)) != 0) || (e.sites >> peplen) != 0 ||
+            __builtin_popcountll(e.sites) != nib)
+            return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_index_build: mod sites and mod_num disagree");
+    }
+    for (uint32_t k = 1; k < nentries; k++)
+        if (!(entries[k - 1].mass <= entries[k].mass))
+            return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_index_build: entries not sorted by mass");
+    if (hipSetDevice(c->device) != hipSuccess) return set_error("hipSetDevice");
+    hcx_index *ix = new (std::nothrow) hcx_index;
+    if (!ix) return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_index_build: out of memory");
+    ix->ctx = c;
+    ix->peplen = peplen;
+    ix->speclen = speclen;
+    ix->n = nentries;
+    ix->nseqs = nseqs;
+    ix->nions = nions;
+    const uint32_t D = p.max_mass + 2;
+    if (ix->entries.ensure(sizeof(hcx_pep_entry) * (size_t)nentries + 16) ||
+        ix->mass.ensure(4ull * nentries + 16) || ix->iA.ensure(4 * nions + 16) ||
+        ix->bA.ensure(4ull * (c->NB + 1)) || ix->mdir.ensure(4ull * D) ||
+        ix->seqs.ensure((size_t)nseqs * peplen + 16)) {
+        delete ix;
+        return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_index_build: device allocation failed");
+    }
+    std::vector<float> mass(nentries);
+    for (uint32_t k = 0; k < nentries; k++) mass[k] = entries[k].mass;
+    auto fail = [&](int rc) {
+        delete ix;
+        return rc;
+    };
+    if (nentries) {
+        if (hipMemcpyAsync(ix->entries.p, entries, sizeof(hcx_pep_entry) * nentries, hipMemcpyHostToDevice,
+                           c->stream) != hipSuccess ||
+            hipMemcpyAsync(ix->mass.p, mass.data(), 4ull * nentries, hipMemcpyHostToDevice, c->stream) !=
+                hipSuccess ||
+            hipMemcpyAsync(ix->seqs.p, seqs, (size_t)nseqs * peplen, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+            return fail(set_error("hcx_index_build: upload failed"));
+    }
+    BuildArgs a;
+    a.seqs = (const char *)ix->seqs.p;
+    a.entries = (const hcx_pep_entry *)ix->entries.p;
+    a.n = nentries;
+    a.peplen = peplen;
+    a.speclen = speclen;
+    a.maxz = p.maxz;
+    a.scale = p.scale;
+    a.NB = c->NB;
+    a.min_mass = (double)p.min_mass;
+    a.max_mass = (double)p.max_mass;
+    std::memcpy(a.mod_mass, p.mod_mass, sizeof(a.mod_mass));
+    if (hipEventRecord(c->ev2, c->stream) != hipSuccess) return fail(set_error("event"));
+    int rc = launch_index_build(c, a, (uint32_t *)ix->iA.p, (uint32_t *)ix->bA.p, (const float *)ix->mass.p,
+                                (uint32_t *)ix->mdir.p, p.max_mass);
+    if (rc) return fail(rc);
+    if (hipEventRecord(c->ev1, c->stream) != hipSuccess || hipEventSynchronize(c->ev1) != hipSuccess)
+        return fail(set_error("event"));
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, c->ev2, c->ev1);
+    ix->build_ms = ms;
+    *out = ix;
+    return HCX_SUCCESS;
+}
+
+extern "C" int hcx_index_info(const hcx_index *ix, uint32_t *peplen, uint32_t *speclen, uint32_t *nentries,
+                              uint64_t *nions) {
+    if (!ix) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_index_info: null");
+    if (peplen) *peplen = ix->peplen;
+    if (speclen) *speclen = ix->speclen;
+    if (nentries) *nentries = ix->n;
+    if (nions) *nions = ix->nions;
+    return HCX_SUCCESS;
+}
+
+extern "C" int hcx_index_build_ms(const hcx_index *ix, float *ms) {
+    if (!ix || !ms) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_index_build_ms: null");
+    *ms = ix->build_ms;
+    return HCX_SUCCESS;
+}
+
+extern "C" int hcx_index_export(const hcx_index *ix, uint32_t *iA, uint32_t *bA) {
+    if (!ix) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_index_export: null");
+    hcx_ctx *c = ix->ctx;
+    HCX_HIP(hipSetDevice(c->device));
+    HCX_HIP(hipStreamSynchronize(c->stream));
+    if (iA && ix->nions) HCX_HIP(hipMemcpy(iA, ix->iA.p, 4 * ix->nions, hipMemcpyDeviceToHost));
+    if (bA) HCX_HIP(hipMemcpy(bA, ix->bA.p, 4ull * (c->NB + 1), hipMemcpyDeviceToHost));
+    return HCX_SUCCESS;
+}
+
+extern "C" int hcx_index_free(hcx_index *ix) {
+    if (!ix) return HCX_SUCCESS;
+    (void)hipSetDevice(ix->ctx->device);
+    (void)hipStreamSynchronize(ix->ctx->stream);
+    delete ix;
+    return HCX_SUCCESS;
+}
+
+extern "C" int hcx_queries_upload(hcx_ctx *c, const hcx_queries_view *q, hcx_qbatch **out) {
+    if (!c || !q || !out) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_queries_upload: null argument");
+    if (q->num_specs < 0) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_queries_upload: negative count");
+    const int32_t n = q->num_specs;
+    if (n > 0 && (!q->idx || !q->precurse || !q->charges))
+        return set_error_code(HCX_ERR_INVLD_PTR, "hcx_queries_upload: missing arrays");
+    const uint64_t npk = n > 0 ? q->idx[n] : 0;
+    if (npk && (!q->moz || !q->intensity))
+        return set_error_code(HCX_ERR_INVLD_PTR, "hcx_queries_upload: missing peaks");
+    uint32_t maxp = 0;
+    for (int32_t s = 0; s < n; s++) {
+        if (q->idx[s + 1] < q->idx[s]) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_queries_upload: idx not monotone");
+        maxp = std::max(maxp, q->idx[s + 1] - q->idx[s]);
+    }
+    if (n > 0 && q->idx[0] != 0) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_queries_upload: idx[0] != 0");
+    HCX_HIP(hipSetDevice(c->device));
+    hcx_qbatch *b = new (std::nothrow) hcx_qbatch;
+    if (!b) return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_queries_upload: out of memory");
+    b->ctx = c;
+    b->nspec = n;
+    b->file_num = q->file_num;
+    b->npeaks = npk;
+    b->max_peaks = maxp;
+    if (b->moz.ensure(4 * npk + 16) || b->inten.ensure(4 * npk + 16) || b->idx.ensure(4ull * (n + 1)) ||
+        b->prec.ensure(4ull * n + 16) || b->chg.ensure(4ull * n + 16) || b->rt.ensure(4ull * n + 16)) {
+        delete b;
+        return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_queries_upload: device allocation failed");
+    }
+    auto up = [&](hcx::DeviceBuf &d, const void *h, size_t bytes) {
+        if (!bytes || !h) return hipSuccess;
+        return hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, c->stream);
+    };
+    if (up(b->moz, q->moz, 4 * npk) || up(b->inten, q->intensity, 4 * npk) || up(b->idx, q->idx, 4ull * (n + 1)) ||
+        up(b->prec, q->precurse, 4ull * n) || up(b->chg, q->charges, 4ull * n) || up(b->rt, q->rtimes, 4ull * n) ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+        delete b;
+        return set_error("hcx_queries_upload: copy failed");
+    }
+    *out = b;
+    return HCX_SUCCESS;
+}
+
+extern "C" int hcx_queries_free(hcx_qbatch *b) {
+    if (!b) return HCX_SUCCESS;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    delete b;
+    return HCX_SUCCESS;
+}
+
+namespace {
+
+__global__ void k_zero_u32(uint32_t *p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// Fill QueryArgs for a search; also sizes the batch's result buffers.
+int prepare(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, const hcx_search_opts *opts,
+            QueryArgs *a) {
+    if (!c || !qb || (nidx > 0 && !idx)) return set_error_code(HCX_ERR_INVLD_PTR, "search: null argument");
+    if (qb->ctx != c) return set_error_code(HCX_ERR_INVLD_PARAM, "search: batch from another context");
+    if (nidx < 0 || nidx > 64) return set_error_code(HCX_ERR_INVLD_PARAM, "search: 0..64 indices");
+    const hcx_params &p = c->params;
+    std::vector<LenDesc> lens(nidx > 0 ? nidx : 1);
+    for (int i = 0; i < nidx; i++) {
+        const hcx_index *ix = idx[i];
+        if (!ix || ix->ctx != c) return set_error_code(HCX_ERR_INVLD_PARAM, "search: bad index");
+        LenDesc &L = lens[i];
+        L.mass = (const float *)ix->mass.p;
+        L.iA = (const uint32_t *)ix->iA.p;
+        L.bA = (const uint32_t *)ix->bA.p;
+        L.mdir = (const uint32_t *)ix->mdir.p;
+        L.n = ix->n;
+        L.speclen = ix->speclen;
+        L.half = ix->speclen / 2;
+        L.peplen = ix->peplen;
+        make_divider(ix->speclen, &L.div_m, &L.div_s);
+        L.pad0 = L.pad1 = 0;
+    }
+    if (c->lens.ensure(sizeof(LenDesc) * lens.size())) return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "search: alloc");
+    HCX_HIP(hipMemcpyAsync(c->lens.p, lens.data(), sizeof(LenDesc) * lens.size(), hipMemcpyHostToDevice, c->stream));
+    const uint64_t n = (uint64_t)qb->nspec;
+    if (qb->res.ensure(sizeof(hcx_result) * n + 16) || qb->hist.ensure(4ull * c->hbins * n + 16) ||
+        qb->stats.ensure(sizeof(hcx_stats) * n + 16) || c->tie.ensure(4 * (n + 1) + 16))
+        return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "search: result allocation failed");
+    std::memset(a, 0, sizeof(*a));
+    a->lens = (const LenDesc *)c->lens.p;
+    a->nlen = nidx;
+    a->moz = (const uint32_t *)qb->moz.p;
+    a->inten = (const uint32_t *)qb->inten.p;
+    a->qidx = (const uint32_t *)qb->idx.p;
+    a->prec = (const float *)qb->prec.p;
+    a->nspec = qb->nspec;
+    a->dF = p.dF;
+    a->NB = c->NB;
+    a->min_shp = p.min_shp;
+    a->hbins = c->hbins;
+    a->max_shdpeaks = p.max_shdpeaks;
+    a->max_mass = p.max_mass;
+    a->dM = p.dM;
+    a->lgf = c->d_lgf;
+    a->res = (hcx_result *)qb->res.p;
+    a->hist = (uint32_t *)qb->hist.p;
+    a->stats = (hcx_stats *)qb->stats.p;
+    a->tie = (uint32_t *)c->tie.p;
+    a->heap_cap = c->heap_cap;
+    a->nprobe_max = std::max<uint32_t>(1, qb->max_peaks) * (2 * p.dF + 1);
+    // tile: the largest power of two (<= 8192) whose LDS layout fits 80 KB,
+    // or the caller's choice if it fits the device limit
+    int dev_lds = 65536;
+    (void)hipDeviceGetAttribute(&dev_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
+    uint32_t tile = opts && opts->tile ? opts->tile : 0;
+    if (tile) {
+        a->tile = tile;
+        if (query_smem_bytes(*a, true) > (size_t)dev_lds)
+            return set_error_code(HCX_ERR_INVLD_PARAM, "search: tile does not fit in LDS");
+    } else {
+        const size_t budget = std::min<size_t>((size_t)dev_lds, 80 * 1024);
+        for (tile = 8192; tile >= 64; tile /= 2) {
+            a->tile = tile;
+            if (query_smem_bytes(*a, true) <= budget) break;
+        }
+        if (tile < 64) return set_error_code(HCX_ERR_INVLD_PARAM, "search: spectra too long for LDS");
+    }
+    return HCX_SUCCESS;
+}
+
+int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, const hcx_search_opts *opts,
+               bool timed, float *kernel_ms, float *total_ms) {
+    HCX_HIP(hipSetDevice(c->device));
+    QueryArgs a;
+    int rc = prepare(c, idx, nidx, qb, opts, &a);
+    if (rc) return rc;
+    const int kernel = opts ? opts->kernel : HCX_KERNEL_PLAIN;
+    if (timed) HCX_HIP(hipEventRecord(c->ev2, c->stream));
+    k_zero_u32<<<1, 64, 0, c->stream>>>(a.tie, 1);
+    HCX_HIP(hipGetLastError());
+    float kms = 0.0f;
+    rc = launch_query(c, a, kernel, false, &kms);
+    if (rc) return rc;
+    if (timed) HCX_HIP(hipEventRecord(c->ev0, c->stream));
+    rc = launch_replay(c, a);
+    if (rc) return rc;
+    rc = launch_tailfit(c, a.res, a.hist, a.hbins, a.nspec, c->params.min_cpsm, c->params.expect_max);
+    if (rc) return rc;
+    if (timed) {
+        HCX_HIP(hipEventRecord(c->ev1, c->stream));
+        HCX_HIP(hipEventSynchronize(c->ev1));
+        float q = 0.0f, t = 0.0f;
+        HCX_HIP(hipEventElapsedTime(&q, c->ev2, c->ev0));
+        HCX_HIP(hipEventElapsedTime(&t, c->ev2, c->ev1));
+        if (kernel_ms) *kernel_ms = q;
+        if (total_ms) *total_ms = t;
+    } else {
+        HCX_HIP(hipStreamSynchronize(c->stream));
+    }
+    return HCX_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" int hcx_search_device(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb,
+                                 const hcx_search_opts *opts, float *kernel_ms, float *total_ms) {
+    if (!c) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_search_device: null context");
+    return run_search(c, idx, nidx, qb, opts, true, kernel_ms, total_ms);
+}
+
+extern "C" int hcx_search_fetch(hcx_ctx *c, hcx_qbatch *qb, hcx_result *results, uint32_t *hist, hcx_stats *stats) {
+    if (!c || !qb) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_search_fetch: null argument");
+    HCX_HIP(hipSetDevice(c->device));
+    HCX_HIP(hipStreamSynchronize(c->stream));
+    const uint64_t n = (uint64_t)qb->nspec;
+    if (!qb->res.p) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_search_fetch: no search has run");
+    if (results && n) HCX_HIP(hipMemcpy(results, qb->res.p, sizeof(hcx_result) * n, hipMemcpyDeviceToHost));
+    if (hist && n) HCX_HIP(hipMemcpy(hist, qb->hist.p, 4ull * c->hbins * n, hipMemcpyDeviceToHost));
+    if (stats && n) HCX_HIP(hipMemcpy(stats, qb->stats.p, sizeof(hcx_stats) * n, hipMemcpyDeviceToHost));
+    return HCX_SUCCESS;
+}
+
+extern "C" int hcx_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb,
+                          const hcx_search_opts *opts, hcx_result *results, uint32_t *hist, hcx_stats *stats) {
+    if (!c || !results) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_search: null argument");
+    int rc = run_search(c, idx, nidx, qb, opts, false, nullptr, nullptr);
+    if (rc) return rc;
+    return hcx_search_fetch(c, qb, results, hist, stats);
+}
+
+extern "C" int hcx_search_candidates(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, hcx_cand *out,
+                                     uint64_t cap, uint64_t *n) {
+    if (!c || !n || (cap && !out)) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_search_candidates: null argument");
+    HCX_HIP(hipSetDevice(c->device));
+    QueryArgs a;
+    int rc = prepare(c, idx, nidx, qb, nullptr, &a);
+    if (rc) return rc;
+    hcx::DeviceBuf dc, dn;
+    if (dc.ensure(sizeof(hcx_cand) * cap + 16) || dn.ensure(16))
+        return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_search_candidates: alloc");
+    HCX_HIP(hipMemsetAsync(dn.p, 0, 8, c->stream));
+    a.cand = (hcx_cand *)dc.p;
+    a.ncand = (unsigned long long *)dn.p;
+    a.cand_cap = cap;
+    rc = launch_candidates(c, a);
+    if (rc) return rc;
+    HCX_HIP(hipStreamSynchronize(c->stream));
+    unsigned long long cnt = 0;
+    HCX_HIP(hipMemcpy(&cnt, dn.p, 8, hipMemcpyDeviceToHost));
+    *n = cnt;
+    if (cap && cnt) HCX_HIP(hipMemcpy(out, dc.p, sizeof(hcx_cand) * std::min<uint64_t>(cap, cnt), hipMemcpyDeviceToHost));
+    return HCX_SUCCESS;
+}
+
+extern "C" int hcx_tailfit_batch(hcx_ctx *c, const uint32_t *hist, uint32_t hbins, uint32_t n, const uint32_t *cpsms,
+                                 const int32_t *maxhyp, int32_t *mu, int32_t *beta, int32_t *minhyp, int32_t *nexthyp,
+                                 uint32_t *flags) {
+    if (!c || (n && (!hist || !cpsms || !maxhyp || !mu || !beta || !minhyp || !nexthyp)))
+        return set_error_code(HCX_ERR_INVLD_PTR, "hcx_tailfit_batch: null argument");
+    if (hbins < 2 || hbins > 20002) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_tailfit_batch: bad hbins");
+    for (uint32_t s = 0; s < n; s++)
+        if (maxhyp[s] >= (int32_t)hbins)
+            return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_tailfit_batch: maxhyp beyond histogram");
+    if (n == 0) return HCX_SUCCESS;
+    HCX_HIP(hipSetDevice(c->device));
+    hcx::DeviceBuf dh, dc, dm, dout, dfl;
+    if (dh.ensure(4ull * hbins * n) || dc.ensure(4ull * n) || dm.ensure(4ull * n) || dout.ensure(16ull * n) ||
+        dfl.ensure(4ull * n))
+        return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_tailfit_batch: alloc");
+    HCX_HIP(hipMemcpyAsync(dh.p, hist, 4ull * hbins * n, hipMemcpyHostToDevice, c->stream));
+    HCX_HIP(hipMemcpyAsync(dc.p, cpsms, 4ull * n, hipMemcpyHostToDevice, c->stream));
+    HCX_HIP(hipMemcpyAsync(dm.p, maxhyp, 4ull * n, hipMemcpyHostToDevice, c->stream));
+    int rc = launch_tailfit_raw(c, (const uint32_t *)dh.p, hbins, n, (const uint32_t *)dc.p, (const int32_t *)dm.p,
+                                (int32_t *)dout.p, (uint32_t *)dfl.p);
+    if (rc) return rc;
+    HCX_HIP(hipStreamSynchronize(c->stream));
+    std::vector<int32_t> o(4ull * n);
+    HCX_HIP(hipMemcpy(o.data(), dout.p, 16ull * n, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> fl(n);
+    HCX_HIP(hipMemcpy(fl.data(), dfl.p, 4ull * n, hipMemcpyDeviceToHost));
+    for (uint32_t s = 0; s < n; s++) {
+        mu[s] = o[4 * s];
+        beta[s] = o[4 * s + 1];
+        minhyp[s] = o[4 * s + 2];
+        nexthyp[s] = o[4 * s + 3];
+        if (flags) flags[s] = fl[s];
+    }
+    return HCX_SUCCESS;
+}
+
+// Host check of the divider the query kernels use for raw / speclen.
+extern "C" int hcx_selftest_divider(uint32_t d, const uint32_t *nums, uint32_t count, uint32_t *out) {
+    uint32_t m, s;
+    make_divider(d, &m, &s);
+    for (uint32_t i = 0; i < count; i++) {
+        const uint32_t n = nums[i];
+        if (s == 0) {
+            out[i] = n;
+            continue;
+        }
+        const uint32_t t = (uint32_t)(((uint64_t)m * n) >> 32);
+        out[i] = (t + ((n - t) >> 1)) >> (s - 1);
+    }
+    return HCX_SUCCESS;
+}

@@ -1,0 +1,134 @@
+// hcx_internal.h -- device data layout and kernel launchers of libhcx.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/hcx.h"
+
+namespace hcx {
+
+// Set the library's last error text and return HCX_ERR_DEVICE.
+int set_error(const std::string &msg);
+int set_error_code(int code, const std::string &msg);
+
+#define HCX_HIP(call)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return ::hcx::set_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Bins are (uint)(m/z * scale) < NB = max_mass * scale.
+// Per-length index as the query kernels read it.
+struct LenDesc {
+    const float *mass;      // n masses, ascending
+    const uint32_t *iA;     // n * speclen ion ids sorted by bin
+    const uint32_t *bA;     // NB + 1 bucket offsets
+    const uint32_t *mdir;   // max_mass + 2 entries: first entry with mass >= d (Da)
+    uint32_t n;
+    uint32_t speclen;
+    uint32_t half;
+    uint32_t peplen;
+    uint32_t div_m;         // magic multiplier for raw / speclen
+    uint32_t div_s;         // shift
+    uint32_t pad0, pad1;
+};
+
+struct DeviceBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t n);   // grow-only (hipMalloc); contents undefined after growth
+    void release();
+    ~DeviceBuf() { release(); }
+};
+
+}  // namespace hcx
+
+struct hcx_ctx {
+    hcx_params params;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    double *d_lgf = nullptr;          // max_shdpeaks + 1
+    std::vector<double> h_lgf;
+    uint32_t NB = 0;                  // max_mass * scale
+    uint32_t hbins = 0;               // 10 * max_hyperscore + 2
+    int heap_cap = 0;
+    int num_cu = 0;
+    // scratch
+    hcx::DeviceBuf lens;              // LenDesc[nidx]
+    hcx::DeviceBuf tie;               // tie list: count + spectrum ids
+    hcx::DeviceBuf work;              // batched kernel queue / counters
+    hcx::DeviceBuf sur;               // surrogate parameters + optimizer state
+    hcx::DeviceBuf sur_tmp;
+    bool sur_ready = false;
+    // last launch shape (for graph reuse)
+    hipGraphExec_t graph_exec = nullptr;
+    const void *graph_key = nullptr;
+};
+
+struct hcx_index {
+    hcx_ctx *ctx = nullptr;
+    uint32_t peplen = 0, speclen = 0, n = 0, nseqs = 0;
+    uint64_t nions = 0;
+    float build_ms = 0.0f;
+    hcx::DeviceBuf entries, mass, iA, bA, mdir, seqs;
+};
+
+struct hcx_qbatch {
+    hcx_ctx *ctx = nullptr;
+    int32_t nspec = 0, file_num = 0;
+    uint64_t npeaks = 0;
+    uint32_t max_peaks = 0;
+    hcx::DeviceBuf moz, inten, idx, prec, chg, rt;
+    hcx::DeviceBuf res, hist, stats, feat, pred;
+};
+
+namespace hcx {
+
+// Integer division n / d for all 32-bit n (round-up method with a 33-bit
+// effective multiplier): q = (t + ((n - t) >> 1)) >> (s - 1), t = mulhi(m, n).
+void make_divider(uint32_t d, uint32_t *m, uint32_t *s);
+
+// ---- index build (k_index.hip) ----
+struct BuildArgs {
+    const char *seqs;
+    const hcx_pep_entry *entries;
+    uint32_t n, peplen, speclen, maxz, scale, NB;
+    double min_mass, max_mass;
+    uint32_t mod_mass[HCX_MAX_MOD_TYPES];
+};
+int launch_index_build(hcx_ctx *ctx, const BuildArgs &a, uint32_t *d_iA, uint32_t *d_bA,
+                       const float *d_mass, uint32_t *d_mdir, uint32_t max_mass);
+
+// ---- query (k_query.hip) ----
+struct QueryArgs {
+    const LenDesc *lens;
+    int nlen;
+    const uint32_t *moz, *inten, *qidx;
+    const float *prec;
+    int32_t nspec;
+    uint32_t dF, NB, min_shp, hbins, max_shdpeaks, tile, nprobe_max, max_mass;
+    double dM;
+    const double *lgf;
+    hcx_result *res;
+    uint32_t *hist;
+    hcx_stats *stats;
+    uint32_t *tie;        // tie[0] = count, tie[1..] = spectrum ids
+    hcx_cand *cand;       // candidate dump (optional)
+    unsigned long long *ncand;
+    uint64_t cand_cap;
+    int heap_cap;
+    uint32_t *queue;      // batched kernel: work counter
+};
+int launch_query(hcx_ctx *ctx, const QueryArgs &a, int kernel, bool timed, float *ms);
+int launch_replay(hcx_ctx *ctx, const QueryArgs &a);
+int launch_tailfit(hcx_ctx *ctx, hcx_result *res, const uint32_t *hist, uint32_t hbins, int32_t n,
+                   uint32_t min_cpsm, double expect_max);
+int launch_tailfit_raw(hcx_ctx *ctx, const uint32_t *hist, uint32_t hbins, uint32_t n, const uint32_t *cpsms,
+                       const int32_t *maxhyp, int32_t *out4, uint32_t *flags);
+
+}  // namespace hcx

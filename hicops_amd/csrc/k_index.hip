@@ -1,0 +1,362 @@
+// k_index.hip -- DSLIM index build on the GPU (DSLIM_Construct, dslim.cpp:46-138).
+//
+// 1. k_frag: one thread per (peptide form, charge, series) walks the L-1
+//    fragment bins of that series (UTILS_GenerateSpectrum utils.cpp:231-276,
+//    UTILS_GenerateModSpectrum utils.cpp:378-505) and bumps the bin histogram
+//    (dslim.cpp:300-323).  FP64 with contraction off, so the truncated bins
+//    equal the host's.
+// 2. bA = exclusive scan of the histogram (dslim.cpp:89-126).
+// 3. iA = ion ids stably sorted by bin: LSD radix sort, 8-bit digits, tiles of
+//    4096 keys, wave multi-split ranking (ballots) so equal digits keep their
+//    input order.  A stable sort of ascending ids equals the reference's
+//    quicksort followed by the per-bin ascending sort (dslim.cpp:361-388,
+//    452-482).
+// 4. mdir[d] = first entry with mass >= d Da: a directory that shortens the
+//    precursor-window binary searches of the query.
+#include "hcx_internal.h"
+
+namespace hcx {
+namespace {
+
+constexpr int kBT = 256;            // threads per block
+constexpr int kIPT = 16;            // items per thread
+constexpr int kTile = kBT * kIPT;   // 4096
+constexpr int kRB = 8;              // radix bits per pass
+constexpr int kR = 1 << kRB;
+
+// utils.cpp:41-98 residue masses (float) + static C mod
+__constant__ float c_aa[26] = {
+    71.03712f, -20000.0f, 103.00919f, 115.030f, 129.0426f, 147.068f, 57.02146f,
+    137.060f, 113.084f, -20000.0f, 128.094f, 113.084f, 131.0405f, 114.043f, -20000.0f,
+    97.0527f, 128.05858f, 156.1012f, 87.032f, 101.0476f, -20000.0f, 99.06841f,
+    186.0793f, -20000.0f, 163.0633f, -20000.0f};
+
+__device__ inline float aa_f(char c) {
+    float m = c_aa[c - 'A'];
+    float s = (c == 'C') ? 57.021464f : 0.0f;
+    return m + s;
+}
+// GETAA(x, z) = float(AAMass + StatMods) + PROTON * z, in double (utils.cpp:101)
+__device__ inline double getaa(char c, uint32_t z) {
+    return (double)aa_f(c) + 1.00727647 * (double)z;
+}
+
+__device__ inline float pep_mass_dev(const char *seq, uint32_t L) {
+    float m = 18.015f;
+    for (uint32_t l = 0; l < L; l++) {
+        char c = seq[l];
+        m += c_aa[c - 'A'];
+        m += (c == 'C') ? 57.021464f : 0.0f;
+    }
+    return m;
+}
+
+__global__ __launch_bounds__(kBT) void k_frag(BuildArgs a, uint32_t *keys, uint32_t *cnt) {
+    const uint64_t t = (uint64_t)blockIdx.x * kBT + threadIdx.x;
+    const uint32_t per = a.maxz * 2;
+    if (t >= (uint64_t)a.n * per) return;
+    const uint32_t k = (uint32_t)(t / per);
+    const uint32_t r = (uint32_t)(t % per);
+    const uint32_t series = r / a.maxz;      // 0 = b, 1 = y
+    const uint32_t z = r % a.maxz;
+    const uint32_t c = z + 1;
+    const uint32_t L = a.peplen;
+    const uint32_t L1 = L - 1;
+    const hcx_pep_entry e = a.entries[k];
+    const char *seq = a.seqs + (size_t)e.seq_id * L;
+    uint32_t *out = keys + (uint64_t)k * a.speclen + series * a.maxz * L1 + z * L1;
+
+    const bool mod = e.mod_num != 0;
+    float mass;
+    if (!mod) {
+        mass = pep_mass_dev(seq, L);
+    } else if (e.sites == 0) {
+        mass = -20000.0f;
+    } else {
+        mass = pep_mass_dev(seq, L);
+        uint32_t first = e.mod_num & 0xF;     // first mod only (utils.cpp:350-358)
+        if (first) mass += (float)a.mod_mass[first - 1] / (float)a.scale;
+    }
+    const bool legal = (double)mass >= a.min_mass && (double)mass <= a.max_mass;
+    const bool filled = mod ? ((double)mass > a.min_mass && (double)mass < a.max_mass) : (mass > 0.0f);
+    if (!legal || !filled) {
+        // illegal form: speclen zeros into bin 0 (dslim.cpp:314-323); a legal
+        // mod form the reference leaves unfilled is zero-filled too
+        for (uint32_t l = 0; l < L1; l++) out[l] = 0;
+        atomicAdd(&cnt[0], L1);
+        return;
+    }
+    const double scale = (double)a.scale;
+    const uint32_t NB = a.NB;
+    if (!mod) {
+        uint32_t v;
+        if (series == 0) {
+            v = (uint32_t)((getaa(seq[0], c) * scale) / (double)c);
+        } else {
+            v = (uint32_t)(((getaa(seq[L - 1], c) + (double)18.015f) * scale) / (double)c);
+        }
+        for (uint32_t l = 0; l < L1; l++) {
+            if (l > 0) {
+                char ch = series == 0 ? seq[l] : seq[L - 1 - l];
+                v += (uint32_t)((getaa(ch, 0) * scale) / (double)c);
+            }
+            uint32_t b = v >= NB ? NB - 1 : v;
+            out[l] = b;
+            atomicAdd(&cnt[b], 1u);
+        }
+    } else {
+        // cumulative undivided sums + prefix of mod masses, then integer divide
+        int mods[8];
+        int seen = 0;
+        for (int i = 0; i < 8; i++) {
+            int nib = (int)((e.mod_num >> (4 * i)) & 0xF) - 1;
+            mods[i] = nib;
+            if (nib != -1) seen++;
+        }
+        uint32_t cum;
+        if (series == 0) cum = (uint32_t)(getaa(seq[0], c) * scale);
+        else cum = (uint32_t)((getaa(seq[L - 1], c) + (double)18.015f) * scale);
+        uint32_t counter = 0, madd = 0;
+        for (uint32_t l = 0; l < L1; l++) {
+            uint32_t pos = series == 0 ? l : L - 1 - l;
+            if (l > 0) cum += (uint32_t)(getaa(seq[pos], 0) * scale);
+            if ((e.sites >> pos) & 1ull) {
+                int mi = series == 0 ? mods[counter] : mods[seen - 1 - (int)counter];
+                counter++;
+                madd += a.mod_mass[mi];
+            }
+            uint32_t v = (cum + madd) / c;
+            uint32_t b = v >= NB ? NB - 1 : v;
+            out[l] = b;
+            atomicAdd(&cnt[b], 1u);
+        }
+    }
+}
+
+// ---- exclusive scan of u32 (tiles of 4096; padded LDS, no bank conflicts) ----
+__device__ inline uint32_t pad_idx(uint32_t i) { return i + (i >> 4); }
+
+__global__ __launch_bounds__(kBT) void k_scan_reduce(const uint32_t *in, uint64_t n, uint32_t *sums) {
+    __shared__ uint32_t red[kBT / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * kTile;
+    uint32_t s = 0;
+    for (int r = 0; r < kIPT; r++) {
+        uint64_t i = base + (uint64_t)r * kBT + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < kBT / 64; w++) t += red[w];
+        sums[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(kBT) void k_scan_block(const uint32_t *in, uint32_t *out, uint64_t n,
+                                                    const uint32_t *offs) {
+    __shared__ uint32_t s[kTile + kTile / 16];
+    __shared__ uint32_t wsum[kBT / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * kTile;
+    for (int r = 0; r < kIPT; r++) {
+        uint32_t li = r * kBT + threadIdx.x;
+        uint64_t i = base + li;
+        s[pad_idx(li)] = i < n ? in[i] : 0u;
+    }
+    __syncthreads();
+    uint32_t loc[kIPT];
+    uint32_t tsum = 0;
+    for (int j = 0; j < kIPT; j++) {
+        loc[j] = s[pad_idx(threadIdx.x * kIPT + j)];
+        tsum += loc[j];
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = tsum;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t run = offs ? offs[blockIdx.x] : 0u;
+    for (int ww = 0; ww < w; ww++) run += wsum[ww];
+    run += x - tsum;
+    for (int j = 0; j < kIPT; j++) {
+        s[pad_idx(threadIdx.x * kIPT + j)] = run;
+        run += loc[j];
+    }
+    __syncthreads();
+    for (int r = 0; r < kIPT; r++) {
+        uint32_t li = r * kBT + threadIdx.x;
+        uint64_t i = base + li;
+        if (i < n) out[i] = s[pad_idx(li)];
+    }
+}
+
+// ---- radix sort ----
+__global__ __launch_bounds__(kBT) void k_radix_hist(const uint32_t *keys, uint64_t n, int shift,
+                                                    uint32_t *tc, uint32_t ntiles) {
+    __shared__ uint32_t h[kR];
+    for (int d = threadIdx.x; d < kR; d += kBT) h[d] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * kTile;
+    for (int r = 0; r < kIPT; r++) {
+        uint64_t i = base + (uint64_t)r * kBT + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & (kR - 1)], 1u);
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < kR; d += kBT) tc[(uint64_t)d * ntiles + blockIdx.x] = h[d];
+}
+
+__global__ __launch_bounds__(kBT) void k_radix_scatter(const uint32_t *kin, const uint32_t *vin,
+                                                       uint32_t *kout, uint32_t *vout, uint64_t n,
+                                                       int shift, const uint32_t *offs, uint32_t ntiles) {
+    __shared__ uint32_t base[kR];
+    __shared__ uint32_t wc[kBT / 64][kR];
+    for (int d = threadIdx.x; d < kR; d += kBT) {
+        base[d] = offs[(uint64_t)d * ntiles + blockIdx.x];
+        for (int w = 0; w < kBT / 64; w++) wc[w][d] = 0;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint64_t tbase = (uint64_t)blockIdx.x * kTile;
+    for (int r = 0; r < kIPT; r++) {
+        const uint64_t i = tbase + (uint64_t)r * kBT + threadIdx.x;
+        const bool valid = i < n;
+        const uint32_t key = valid ? kin[i] : 0u;
+        const uint32_t val = valid ? (vin ? vin[i] : (uint32_t)i) : 0u;
+        const uint32_t d = (key >> shift) & (kR - 1);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < kRB; b++) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt);
+        if (valid && rank == 0) wc[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = base[d] + rank;
+            for (int ww = 0; ww < w; ww++) pos += wc[ww][d];
+            if (kout) kout[pos] = key;
+            vout[pos] = val;
+        }
+        __syncthreads();
+        for (int dd = threadIdx.x; dd < kR; dd += kBT) {
+            uint32_t add = 0;
+            for (int ww = 0; ww < kBT / 64; ww++) {
+                add += wc[ww][dd];
+                wc[ww][dd] = 0;
+            }
+            base[dd] += add;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_mdir(const float *mass, uint32_t n, uint32_t *mdir, uint32_t D) {
+    uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= D) return;
+    const float v = (float)d;
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint32_t m = lo + (hi - lo) / 2;
+        if (mass[m] < v) lo = m + 1;
+        else hi = m;
+    }
+    mdir[d] = lo;
+}
+
+__global__ void k_zero(uint32_t *p, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+}  // namespace
+
+// Exclusive scan of n u32 on the stream; tmp must hold scan_tmp_words(n) u32.
+static uint64_t scan_tmp_words(uint64_t n) {
+    uint64_t w = 0;
+    while (n > (uint64_t)kTile) {
+        n = (n + kTile - 1) / kTile;
+        w += n + 1;
+    }
+    return w + 1;
+}
+
+static int scan_exclusive(hipStream_t st, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *tmp) {
+    if (n == 0) return HCX_SUCCESS;
+    const uint64_t nb = (n + kTile - 1) / kTile;
+    if (nb == 1) {
+        k_scan_block<<<1, kBT, 0, st>>>(in, out, n, nullptr);
+        HCX_HIP(hipGetLastError());
+        return HCX_SUCCESS;
+    }
+    uint32_t *sums = tmp;
+    k_scan_reduce<<<(unsigned)nb, kBT, 0, st>>>(in, n, sums);
+    HCX_HIP(hipGetLastError());
+    int rc = scan_exclusive(st, sums, sums, nb, tmp + nb + 1);
+    if (rc) return rc;
+    k_scan_block<<<(unsigned)nb, kBT, 0, st>>>(in, out, n, sums);
+    HCX_HIP(hipGetLastError());
+    return HCX_SUCCESS;
+}
+
+int launch_index_build(hcx_ctx *ctx, const BuildArgs &a, uint32_t *d_iA, uint32_t *d_bA, const float *d_mass,
+                       uint32_t *d_mdir, uint32_t max_mass) {
+    hipStream_t st = ctx->stream;
+    const uint64_t nions = (uint64_t)a.n * a.speclen;
+    const uint32_t NB = a.NB;
+    uint32_t keybits = 1;
+    while (keybits < 32 && (1ull << keybits) < NB) keybits++;
+    const int passes = (int)((keybits + kRB - 1) / kRB);
+    const uint64_t ntiles = (nions + kTile - 1) / kTile;
+    const uint64_t tc_words = (uint64_t)kR * (ntiles ? ntiles : 1);
+    const uint64_t scan_words = scan_tmp_words(std::max<uint64_t>(tc_words, NB + 1));
+
+    DeviceBuf K0, K1, V1, V2, cnt, tc, tmp;
+    if (K0.ensure(nions * 4 + 4) || K1.ensure(nions * 4 + 4) || V1.ensure(nions * 4 + 4) ||
+        (passes > 2 && V2.ensure(nions * 4 + 4)) || cnt.ensure((uint64_t)(NB + 1) * 4) ||
+        tc.ensure(tc_words * 4) || tmp.ensure(scan_words * 4))
+        return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "index build: device allocation failed");
+    uint32_t *k0 = (uint32_t *)K0.p, *k1 = (uint32_t *)K1.p, *v1 = (uint32_t *)V1.p, *v2 = (uint32_t *)V2.p;
+    uint32_t *dcnt = (uint32_t *)cnt.p, *dtc = (uint32_t *)tc.p, *dtmp = (uint32_t *)tmp.p;
+
+    k_zero<<<(NB + 1 + 255) / 256, 256, 0, st>>>(dcnt, NB + 1);
+    HCX_HIP(hipGetLastError());
+    const uint64_t nthr = (uint64_t)a.n * a.maxz * 2;
+    if (nthr) {
+        k_frag<<<(unsigned)((nthr + kBT - 1) / kBT), kBT, 0, st>>>(a, k0, dcnt);
+        HCX_HIP(hipGetLastError());
+    }
+    int rc = scan_exclusive(st, dcnt, d_bA, (uint64_t)NB + 1, dtmp);
+    if (rc) return rc;
+    if (nions) {
+        const uint32_t *kin = k0, *vin = nullptr;
+        for (int p = 0; p < passes; p++) {
+            const bool last = p == passes - 1;
+            uint32_t *kout = last ? nullptr : (kin == k0 ? k1 : k0);
+            uint32_t *vout = last ? d_iA : (p % 2 == 0 ? v1 : v2);
+            k_radix_hist<<<(unsigned)ntiles, kBT, 0, st>>>(kin, nions, p * kRB, dtc, (uint32_t)ntiles);
+            HCX_HIP(hipGetLastError());
+            rc = scan_exclusive(st, dtc, dtc, tc_words, dtmp);
+            if (rc) return rc;
+            k_radix_scatter<<<(unsigned)ntiles, kBT, 0, st>>>(kin, vin, kout, vout, nions, p * kRB, dtc,
+                                                                (uint32_t)ntiles);
+            HCX_HIP(hipGetLastError());
+            kin = kout;
+            vin = vout;
+        }
+    }
+    const uint32_t D = max_mass + 2;
+    k_mdir<<<(D + 255) / 256, 256, 0, st>>>(d_mass, a.n, d_mdir, D);
+    HCX_HIP(hipGetLastError());
+    // temporaries are freed on return; finish the stream first
+    HCX_HIP(hipStreamSynchronize(st));
+    return HCX_SUCCESS;
+}
+
+}  // namespace hcx

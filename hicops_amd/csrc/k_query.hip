@@ -1,0 +1,797 @@
+// k_query.hip -- shared-peak scan, hyperscore, top hit and tail fit on the GPU.
+//
+// DSLIM_QuerySpectrum (dslim_query.cpp:515-842): for every spectrum, every
+// peptide length and every peak bin q-dF..q+dF, the iA entries of that bin
+// whose ion id lies inside the precursor window add one b- or y-count and the
+// peak intensity to their peptide's counters; every peptide with
+// bc+yc >= min_shp then gets the hyperscore
+//     hs = (float)(lgf[bc] + lgf[yc] + log10(1+ibc) + log10(1+iyc) - 6)
+// and, if hs > 0, enters the top-K heap and the 0.1-wide score histogram.
+//
+// Layout of one workgroup's LDS: the window of one length is tiled into T
+// peptides; each peptide has two packed u64 counters (b, y):
+//     count << 48 | intensity sum            (one ds_add_u64 per iA entry)
+// Every (peak, bin) probe keeps a cursor into its iA bin range, so each iA
+// entry is read once no matter how many tiles the window needs.
+//
+// Top hit: the reference keeps a min-heap (minheap.h) whose getMax() breaks
+// ties by heap position.  A unique maximum is the heap's maximum whatever the
+// insertion order, so the scan only tracks (max, #ties); spectra whose
+// maximum is tied are re-run by the replay kernel, which inserts every
+// candidate into an exact copy of the heap in the reference order (length,
+// then peptide id).
+//
+// Tail fit (expeRT.cpp:572-718) runs one wavefront per spectrum in FP64; the
+// prefix sums are of integer counts (exact in any order), the regression sums
+// run on one lane in the reference order.
+#include "hcx_internal.h"
+
+namespace hcx {
+
+void make_divider(uint32_t d, uint32_t *m, uint32_t *s) {
+    // smallest l with 2^l >= d; m = floor(2^32 * (2^l - d) / d) + 1
+    uint32_t l = 0;
+    while ((1ull << l) < d) l++;
+    uint64_t mm = ((((uint64_t)1 << l) - d) << 32) / d + 1;
+    *m = (uint32_t)mm;
+    *s = l;
+}
+
+namespace {
+
+constexpr int kBT = 256;
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr unsigned long long kCnt = 1ull << 48;
+constexpr unsigned long long kSumMask = kCnt - 1;
+
+__device__ inline uint32_t udiv(uint32_t n, uint32_t m, uint32_t s) {
+    if (s == 0) return n;   // d == 1
+    uint32_t t = __umulhi(m, n);
+    return (t + ((n - t) >> 1)) >> (s - 1);
+}
+
+struct Cell {
+    float hs;
+    int32_t psid;
+    uint32_t ixx, bc, yc, speclen;
+};
+
+// first index in [lo, hi) with mass >= v (v > 0 handled by the directory)
+__device__ inline uint32_t mass_lb(const float *mass, uint32_t lo, uint32_t hi, float v) {
+    while (lo < hi) {
+        uint32_t m = lo + (hi - lo) / 2;
+        if (mass[m] < v) lo = m + 1;
+        else hi = m;
+    }
+    return lo;
+}
+__device__ inline uint32_t mass_ub(const float *mass, uint32_t lo, uint32_t hi, float v) {
+    while (lo < hi) {
+        uint32_t m = lo + (hi - lo) / 2;
+        if (v < mass[m]) hi = m;
+        else lo = m + 1;
+    }
+    return lo;
+}
+
+// Directory-narrowed range for a value v: [mdir[d], mdir[d+1]] with d = floor(v).
+__device__ inline void dir_range(const uint32_t *mdir, uint32_t D, uint32_t n, float v, uint32_t *lo,
+                                 uint32_t *hi) {
+    if (!(v > 0.0f)) {
+        *lo = 0;
+        *hi = (D > 1) ? mdir[1] : n;
+        return;
+    }
+    uint32_t d = (v >= (float)(D - 1)) ? D - 1 : (uint32_t)v;
+    *lo = mdir[d];
+    *hi = (d + 1 < D) ? mdir[d + 1] : n;
+}
+
+// dslim_query.cpp:854-916 as lower/upper bounds on the float masses
+__device__ inline void window(const LenDesc &L, float prec, double dM, uint32_t D, int32_t *wlo, int32_t *whi) {
+    *wlo = 1;
+    *whi = 0;
+    if (L.n == 0 || dM < 0.0 || prec != prec) return;
+    const float p1 = (float)((double)prec - dM);
+    const float p2 = (float)((double)prec + dM);
+    uint32_t lo, hi;
+    dir_range(L.mdir, D, L.n, p1, &lo, &hi);
+    uint32_t a = mass_lb(L.mass, lo, hi, p1);
+    dir_range(L.mdir, D, L.n, p2, &lo, &hi);
+    uint32_t b = mass_ub(L.mass, lo, hi, p2);
+    *wlo = (int32_t)a;
+    *whi = (int32_t)b - 1;
+}
+
+__device__ inline uint32_t iA_lower(const uint32_t *iA, uint32_t lo, uint32_t hi, uint32_t key) {
+    while (lo < hi) {
+        uint32_t m = lo + (hi - lo) / 2;
+        if (iA[m] < key) lo = m + 1;
+        else hi = m;
+    }
+    return lo;
+}
+__device__ inline uint32_t iA_upper(const uint32_t *iA, uint32_t lo, uint32_t hi, uint32_t key) {
+    while (lo < hi) {
+        uint32_t m = lo + (hi - lo) / 2;
+        if (key < iA[m]) hi = m;
+        else lo = m + 1;
+    }
+    return lo;
+}
+
+// minheap.h:108-177 (insert, increase_key at the root, heapify)
+__device__ void heap_heapify(Cell *h, int size, int pos) {
+    for (;;) {
+        int r = (pos + 1) * 2, l = r - 1, s = pos;
+        float sv = h[s].hs;
+        if (l < size && h[l].hs < sv) { s = l; sv = h[l].hs; }
+        if (r < size && h[r].hs < sv) { s = r; sv = h[r].hs; }
+        if (s == pos) return;
+        Cell t = h[s];
+        h[s] = h[pos];
+        h[pos] = t;
+        pos = s;
+    }
+}
+__device__ void heap_insert(Cell *h, int *size, int cap, const Cell &e) {
+    if (*size == cap) {
+        if (e.hs < h[0].hs) return;
+        h[0] = e;
+        heap_heapify(h, *size, 0);
+        return;
+    }
+    h[(*size)++] = e;
+    for (int i = (*size - 1) / 2; i >= 0; i--) heap_heapify(h, *size, i);
+}
+// minheap.h:258-277
+__device__ Cell heap_getmax(const Cell *h, int size) {
+    if (size <= 1) return h[0];
+    Cell m = h[size - 1];
+    for (int k = size - 2; k >= size / 2; k--)
+        if (h[k].hs > m.hs) m = h[k];
+    return m;
+}
+
+struct Smem {
+    unsigned long long *sc;
+    uint32_t *hist, *pcur, *pend, *pnxt, *pk_moz, *pk_int;
+    int32_t *wlo, *whi;
+    double *lgf;
+    Cell *heap;
+    float *rb_hs;
+    uint32_t *rb_bc, *rb_yc;
+};
+
+__host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+__host__ __device__ inline size_t smem_layout(const QueryArgs &a, bool replay, unsigned char *base, Smem *sm) {
+    const uint32_t NP = a.nprobe_max;
+    const uint32_t PM = NP / (2 * a.dF + 1);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off = align16(off + bytes);
+        return base ? base + o : nullptr;
+    };
+    unsigned char *p;
+    p = take(sizeof(unsigned long long) * 2 * a.tile);
+    if (sm) sm->sc = (unsigned long long *)p;
+    p = take(4ull * a.hbins);
+    if (sm) sm->hist = (uint32_t *)p;
+    p = take(4ull * NP);
+    if (sm) sm->pcur = (uint32_t *)p;
+    p = take(4ull * NP);
+    if (sm) sm->pend = (uint32_t *)p;
+    p = take(4ull * NP);
+    if (sm) sm->pnxt = (uint32_t *)p;
+    p = take(4ull * PM);
+    if (sm) sm->pk_moz = (uint32_t *)p;
+    p = take(4ull * PM);
+    if (sm) sm->pk_int = (uint32_t *)p;
+    p = take(4ull * a.nlen);
+    if (sm) sm->wlo = (int32_t *)p;
+    p = take(4ull * a.nlen);
+    if (sm) sm->whi = (int32_t *)p;
+    p = take(8ull * (a.max_shdpeaks + 1));
+    if (sm) sm->lgf = (double *)p;
+    if (replay) {
+        p = take(sizeof(Cell) * (a.heap_cap + 1));
+        if (sm) sm->heap = (Cell *)p;
+        p = take(4ull * kBT);
+        if (sm) sm->rb_hs = (float *)p;
+        p = take(4ull * kBT);
+        if (sm) sm->rb_bc = (uint32_t *)p;
+        p = take(4ull * kBT);
+        if (sm) sm->rb_yc = (uint32_t *)p;
+    }
+    return off;
+}
+
+template <bool REPLAY>
+__global__ __launch_bounds__(kBT) void k_query(QueryArgs a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ uint32_t s_maxbits, s_ties, s_cpsms, s_flags;
+    __shared__ unsigned long long s_nia, s_nprobe;
+    __shared__ Cell s_top;
+    __shared__ int s_hsize;
+    Smem sm;
+    smem_layout(a, REPLAY, smem_raw, &sm);
+    const int tid = threadIdx.x;
+    const uint32_t W = 2 * a.dF + 1;
+    const uint32_t D = a.max_mass + 2;
+    for (uint32_t i = tid; i <= a.max_shdpeaks; i += kBT) sm.lgf[i] = a.lgf[i];
+    for (uint32_t i = tid; i < 2 * a.tile; i += kBT) sm.sc[i] = 0;
+
+    const uint32_t nwork = REPLAY ? a.tie[0] : (uint32_t)a.nspec;
+    for (uint32_t wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+        const int32_t s = REPLAY ? (int32_t)a.tie[1 + wi] : (int32_t)wi;
+        const uint32_t q0 = a.qidx[s];
+        const uint32_t P = a.qidx[s + 1] - q0;
+        const float prec = a.prec[s];
+        for (uint32_t k = tid; k < P; k += kBT) {
+            sm.pk_moz[k] = a.moz[q0 + k];
+            sm.pk_int[k] = a.inten[q0 + k];
+        }
+        for (uint32_t h = tid; h < a.hbins; h += kBT) sm.hist[h] = 0;
+        for (int l = tid; l < a.nlen; l += kBT) window(a.lens[l], prec, a.dM, D, &sm.wlo[l], &sm.whi[l]);
+        if (tid == 0) {
+            s_maxbits = 0;
+            s_ties = 0;
+            s_cpsms = 0;
+            s_flags = 0;
+            s_nia = 0;
+            s_nprobe = 0;
+            s_hsize = 0;
+        }
+        __syncthreads();
+
+        float best_hs = 0.0f;
+        uint32_t best_cnt = 0;
+        Cell best = {0.0f, 0, 0, 0, 0, 0};
+        uint32_t cpsms = 0, flags = 0;
+        unsigned long long nia = 0, nprobe = 0;
+        const uint32_t NP = P * W;
+
+        for (int ixx = 0; ixx < a.nlen; ixx++) {
+            const int32_t lo = sm.wlo[ixx], hi = sm.whi[ixx];
+            if (lo > hi) continue;
+            const LenDesc L = a.lens[ixx];
+            const uint32_t speclen = L.speclen;
+            const uint32_t lo_key = (uint32_t)lo * speclen;
+            const uint32_t hi_key = (uint32_t)(hi + 1) * speclen - 1;
+            for (uint32_t p = tid; p < NP; p += kBT) {
+                const uint32_t k = p / W, j = p - k * W;
+                const uint32_t q = sm.pk_moz[k];
+                uint32_t c = 0, e = 0, nx = kNone;
+                if (q > a.dF && q < a.NB - 1 - a.dF) {
+                    const uint32_t bin = q - a.dF + j;
+                    const uint32_t s0 = L.bA[bin], e0 = L.bA[bin + 1];
+                    nprobe++;
+                    if (e0 > s0) {
+                        c = iA_lower(L.iA, s0, e0, lo_key);
+                        e = iA_upper(L.iA, c, e0, hi_key);
+                        nia += e - c;
+                        if (c < e) nx = L.iA[c];
+                    }
+                }
+                sm.pcur[p] = c;
+                sm.pend[p] = e;
+                sm.pnxt[p] = nx;
+            }
+            __syncthreads();
+            for (int32_t t0 = lo; t0 <= hi; t0 += (int32_t)a.tile) {
+                const uint32_t tn = (uint32_t)min((int32_t)a.tile, hi - t0 + 1);
+                const uint32_t bound = (uint32_t)(t0 + (int32_t)tn) * speclen;
+                for (uint32_t p = tid; p < NP; p += kBT) {
+                    uint32_t nx = sm.pnxt[p];
+                    if (nx >= bound) continue;
+                    uint32_t c = sm.pcur[p];
+                    const uint32_t e = sm.pend[p];
+                    const unsigned long long add = kCnt + (unsigned long long)sm.pk_int[p / W];
+                    do {
+                        const uint32_t ppid = udiv(nx, L.div_m, L.div_s);
+                        const uint32_t rem = nx - ppid * speclen;
+                        const uint32_t isY = rem >= L.half ? 1u : 0u;
+                        const uint32_t local = ppid - (uint32_t)t0;
+                        atomicAdd(&sm.sc[2 * local + isY], add);
+                        c++;
+                        nx = (c < e) ? L.iA[c] : kNone;
+                    } while (nx < bound);
+                    sm.pcur[p] = c;
+                    sm.pnxt[p] = nx;
+                }
+                __syncthreads();
+                for (uint32_t cb = 0; cb < tn; cb += kBT) {
+                    const uint32_t cc = cb + tid;
+                    float hs_ins = -1.0f;
+                    uint32_t bc = 0, yc = 0;
+                    if (cc < tn) {
+                        const unsigned long long vb = sm.sc[2 * cc], vy = sm.sc[2 * cc + 1];
+                        if (vb | vy) {
+                            sm.sc[2 * cc] = 0;
+                            sm.sc[2 * cc + 1] = 0;
+                            bc = (uint32_t)(vb >> 48);
+                            yc = (uint32_t)(vy >> 48);
+                            const uint16_t shpk = (uint16_t)(bc + yc);
+                            if (shpk >= a.min_shp) {
+                                uint32_t bi = bc, yi = yc;
+                                if (bi > a.max_shdpeaks) { bi = a.max_shdpeaks; flags |= HCX_RES_LGF_CLAMP; }
+                                if (yi > a.max_shdpeaks) { yi = a.max_shdpeaks; flags |= HCX_RES_LGF_CLAMP; }
+                                double t = sm.lgf[bi] + sm.lgf[yi];
+                                t = t + log10((double)(1ull + (vb & kSumMask)));
+                                t = t + log10((double)(1ull + (vy & kSumMask)));
+                                t = t - 6.0;
+                                const float hs = (float)t;
+                                if (hs > 0.0f) {
+                                    cpsms++;
+                                    int bin = (int)((double)(hs * 10.0f) + 0.5);
+                                    if (bin >= (int)a.hbins) { bin = (int)a.hbins - 1; flags |= HCX_RES_HIST_CLAMP; }
+                                    atomicAdd(&sm.hist[bin], 1u);
+                                    hs_ins = hs;
+                                    if (hs > best_hs) {
+                                        best_hs = hs;
+                                        best_cnt = 1;
+                                        best = {hs, t0 + (int32_t)cc, (uint32_t)ixx, bc, yc, speclen};
+                                    } else if (hs == best_hs) {
+                                        best_cnt++;
+                                    }
+                                }
+                            }
+                        }
+                    }
+                    if (REPLAY) {
+                        sm.rb_hs[tid] = hs_ins;
+                        sm.rb_bc[tid] = bc;
+                        sm.rb_yc[tid] = yc;
+                        __syncthreads();
+                        if (tid == 0) {
+                            const uint32_t m = min((uint32_t)kBT, tn - cb);
+                            for (uint32_t i = 0; i < m; i++) {
+                                if (sm.rb_hs[i] > 0.0f) {
+                                    Cell e = {sm.rb_hs[i], t0 + (int32_t)(cb + i), (uint32_t)ixx, sm.rb_bc[i],
+                                              sm.rb_yc[i], speclen};
+                                    heap_insert(sm.heap, &s_hsize, a.heap_cap, e);
+                                }
+                            }
+                        }
+                        __syncthreads();
+                    }
+                }
+                __syncthreads();
+            }
+        }
+
+        // ---- block reductions ----
+        if (best_cnt) atomicMax(&s_maxbits, __float_as_uint(best_hs));
+        if (cpsms) atomicAdd(&s_cpsms, cpsms);
+        if (flags) atomicOr(&s_flags, flags);
+        if (nia) atomicAdd(&s_nia, nia);
+        if (nprobe) atomicAdd(&s_nprobe, nprobe);
+        __syncthreads();
+        const uint32_t maxb = s_maxbits;
+        const bool mine = best_cnt && __float_as_uint(best_hs) == maxb;
+        if (mine) atomicAdd(&s_ties, best_cnt);
+        __syncthreads();
+        if (!REPLAY && mine && s_ties == 1) s_top = best;
+        if (REPLAY && tid == 0 && s_hsize > 0) s_top = heap_getmax(sm.heap, s_hsize);
+        __syncthreads();
+        if (tid == 0) {
+            hcx_result r;
+            r.hyperscore = 0.0f;
+            r.psid = 0;
+            r.idxoffset = 0;
+            r.sharedions = 0;
+            r.totalions = 0;
+            r.bc = 0;
+            r.yc = 0;
+            r.flags = (uint16_t)s_flags;
+            r.cpsms = s_cpsms;
+            r.maxhypscore = r.mu = r.beta = r.minhypscore = r.nexthypscore = 0;
+            r.e_x = 0.0;
+            const bool have = REPLAY ? (s_hsize > 0) : (s_ties == 1);
+            if (have) {
+                r.hyperscore = s_top.hs;
+                r.psid = s_top.psid;
+                r.idxoffset = (uint16_t)s_top.ixx;
+                r.sharedions = (uint16_t)(s_top.bc + s_top.yc);
+                r.totalions = (uint16_t)s_top.speclen;
+                r.bc = s_top.bc;
+                r.yc = s_top.yc;
+            }
+            if (REPLAY) {
+                r.flags |= HCX_RES_TIE_REPLAYED;
+            } else if (s_ties > 1) {
+                // tied maximum: the replay kernel resolves the heap order
+                r.hyperscore = __uint_as_float(maxb);
+                const uint32_t slot = atomicAdd(&a.tie[0], 1u);
+                a.tie[1 + slot] = (uint32_t)s;
+            }
+            a.res[s] = r;
+            if (a.stats) {
+                a.stats[s].n_ia = s_nia;
+                a.stats[s].n_probe = s_nprobe;
+            }
+        }
+        if (!REPLAY) {
+            uint32_t *hout = a.hist + (uint64_t)s * a.hbins;
+            for (uint32_t h = tid; h < a.hbins; h += kBT) hout[h] = sm.hist[h];
+        }
+        __syncthreads();
+    }
+}
+
+// Candidate dump (parity tests): same scan, one block per spectrum, every
+// inserted candidate appended with its exact counters.
+__global__ __launch_bounds__(kBT) void k_candidates(QueryArgs a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    Smem sm;
+    smem_layout(a, false, smem_raw, &sm);
+    const int tid = threadIdx.x;
+    const uint32_t W = 2 * a.dF + 1;
+    const uint32_t D = a.max_mass + 2;
+    for (uint32_t i = tid; i <= a.max_shdpeaks; i += kBT) sm.lgf[i] = a.lgf[i];
+    for (uint32_t i = tid; i < 2 * a.tile; i += kBT) sm.sc[i] = 0;
+    const int32_t s = blockIdx.x;
+    const uint32_t q0 = a.qidx[s];
+    const uint32_t P = a.qidx[s + 1] - q0;
+    for (uint32_t k = tid; k < P; k += kBT) {
+        sm.pk_moz[k] = a.moz[q0 + k];
+        sm.pk_int[k] = a.inten[q0 + k];
+    }
+    for (int l = tid; l < a.nlen; l += kBT) window(a.lens[l], a.prec[s], a.dM, D, &sm.wlo[l], &sm.whi[l]);
+    __syncthreads();
+    const uint32_t NP = P * W;
+    for (int ixx = 0; ixx < a.nlen; ixx++) {
+        const int32_t lo = sm.wlo[ixx], hi = sm.whi[ixx];
+        if (lo > hi) continue;
+        const LenDesc L = a.lens[ixx];
+        const uint32_t speclen = L.speclen;
+        for (uint32_t p = tid; p < NP; p += kBT) {
+            const uint32_t k = p / W, j = p - k * W;
+            const uint32_t q = sm.pk_moz[k];
+            uint32_t c = 0, e = 0, nx = kNone;
+            if (q > a.dF && q < a.NB - 1 - a.dF) {
+                const uint32_t bin = q - a.dF + j;
+                const uint32_t s0 = L.bA[bin], e0 = L.bA[bin + 1];
+                if (e0 > s0) {
+                    c = iA_lower(L.iA, s0, e0, (uint32_t)lo * speclen);
+                    e = iA_upper(L.iA, c, e0, (uint32_t)(hi + 1) * speclen - 1);
+                    if (c < e) nx = L.iA[c];
+                }
+            }
+            sm.pcur[p] = c;
+            sm.pend[p] = e;
+            sm.pnxt[p] = nx;
+        }
+        __syncthreads();
+        for (int32_t t0 = lo; t0 <= hi; t0 += (int32_t)a.tile) {
+            const uint32_t tn = (uint32_t)min((int32_t)a.tile, hi - t0 + 1);
+            const uint32_t bound = (uint32_t)(t0 + (int32_t)tn) * speclen;
+            for (uint32_t p = tid; p < NP; p += kBT) {
+                uint32_t nx = sm.pnxt[p];
+                if (nx >= bound) continue;
+                uint32_t c = sm.pcur[p];
+                const uint32_t e = sm.pend[p];
+                const unsigned long long add = kCnt + (unsigned long long)sm.pk_int[p / W];
+                do {
+                    const uint32_t ppid = udiv(nx, L.div_m, L.div_s);
+                    const uint32_t isY = (nx - ppid * speclen) >= L.half ? 1u : 0u;
+                    atomicAdd(&sm.sc[2 * (ppid - (uint32_t)t0) + isY], add);
+                    c++;
+                    nx = (c < e) ? L.iA[c] : kNone;
+                } while (nx < bound);
+                sm.pcur[p] = c;
+                sm.pnxt[p] = nx;
+            }
+            __syncthreads();
+            for (uint32_t cc = tid; cc < tn; cc += kBT) {
+                const unsigned long long vb = sm.sc[2 * cc], vy = sm.sc[2 * cc + 1];
+                if (!(vb | vy)) continue;
+                sm.sc[2 * cc] = 0;
+                sm.sc[2 * cc + 1] = 0;
+                const uint32_t bc = (uint32_t)(vb >> 48), yc = (uint32_t)(vy >> 48);
+                if ((uint16_t)(bc + yc) < a.min_shp) continue;
+                const uint32_t bi = min(bc, a.max_shdpeaks), yi = min(yc, a.max_shdpeaks);
+                double t = sm.lgf[bi] + sm.lgf[yi];
+                t = t + log10((double)(1ull + (vb & kSumMask)));
+                t = t + log10((double)(1ull + (vy & kSumMask)));
+                t = t - 6.0;
+                const float hs = (float)t;
+                if (!(hs > 0.0f)) continue;
+                const unsigned long long slot = atomicAdd(a.ncand, 1ull);
+                if (slot < a.cand_cap) {
+                    hcx_cand o;
+                    o.spec = (uint32_t)s;
+                    o.idxoffset = (uint16_t)ixx;
+                    o.bc = (uint16_t)bc;
+                    o.psid = t0 + (int32_t)cc;
+                    o.yc = (uint16_t)yc;
+                    o.pad_ = 0;
+                    o.ibc = vb & kSumMask;
+                    o.iyc = vy & kSumMask;
+                    o.hyperscore = hs;
+                    o.pad2_ = 0;
+                    a.cand[slot] = o;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ---- expeRT::ModelTailFit (expeRT.cpp:572-718), one wavefront per spectrum ----
+struct FitOut {
+    int32_t mu, beta, stt1, end1;
+    uint32_t flags;
+};
+
+// sx: LDS scratch of this wave (>= hbins doubles).  Lane 0's return is valid.
+__device__ FitOut tailfit_wave(const uint32_t *H, uint32_t hbins, uint32_t cpsms, int32_t hyp, double *sx) {
+    const int lane = threadIdx.x & 63;
+    FitOut o = {0, 0, 0, 0, 0};
+    if (hyp < 1 || hyp >= (int)hbins) {
+        // the reference reads yy[-1] or past the histogram (UB): a zero fit
+        o.flags = HCX_RES_FIT_UB;
+        return o;
+    }
+    // end1 = rargmax(yy, 0, hyp-1, 1.0): last p <= hyp-1 with yy[p] >= 1, else hyp-1
+    int end1 = hyp - 1;
+    for (int top = hyp - 1; top >= 0; top -= 64) {
+        const int p = top - lane;
+        const bool ok = p >= 0 && H[p] >= 1u;
+        const unsigned long long m = __ballot(ok);
+        if (m) {
+            end1 = top - (__ffsll((long long)m) - 1);
+            break;
+        }
+    }
+    // stt1 = argmax(yy, 0, end1, 1.0): first p in [0, end1] with yy[p] >= 1, else 0
+    int stt1 = 0;
+    for (int b = 0; b <= end1; b += 64) {
+        const int p = b + lane;
+        const bool ok = p <= end1 && H[p] >= 1u;
+        const unsigned long long m = __ballot(ok);
+        if (m) {
+            stt1 = b + (__ffsll((long long)m) - 1);
+            break;
+        }
+    }
+    if (stt1 == end1) end1 += 1;
+    const int n = end1 - stt1 + 1;
+    o.stt1 = stt1;
+    o.end1 = end1;
+    double mu_t = 0.0, beta_t = 4.0;
+    if (hyp < 1 || n < 2 || end1 >= (int)hbins) {
+        o.flags |= HCX_RES_FIT_UB;
+        mu_t = 0.0;
+        beta_t = 0.0;
+    } else if ((int)cpsms < 1) {
+        mu_t = 0.0;
+        beta_t = 100.0;
+    } else {
+        const double vaa = (double)(int)cpsms;
+        // inclusive prefix sums of integer counts: exact in any order
+        double carry = 0.0;
+        for (int b = 0; b < n; b += 64) {
+            const int i = b + lane;
+            double x = (i < n) ? (double)H[stt1 + i] : 0.0;
+            for (int off = 1; off < 64; off <<= 1) {
+                double y = __shfl_up(x, off, 64);
+                if (lane >= off) x += y;
+            }
+            if (i < n) {
+                double t = (carry + x) / vaa;
+                t = t + (-1.0);
+                sx[i] = -t;
+            }
+            carry += __shfl(x, 63, 64);
+        }
+        __syncwarp();
+        for (int i = lane; i < n; i += 64)
+            if (sx[i] > 1.0) sx[i] = 0.999;
+        __syncwarp();
+        // replacement = rargmax(sx, 0, n-1, 1e-4)
+        int rep = n - 1;
+        for (int top = n - 1; top >= 0; top -= 64) {
+            const int p = top - lane;
+            const unsigned long long m = __ballot(p >= 0 && sx[p] >= 1e-4);
+            if (m) {
+                rep = top - (__ffsll((long long)m) - 1);
+                break;
+            }
+        }
+        const double rv = sx[rep];
+        __syncwarp();
+        for (int i = lane; i < n; i += 64) {
+            double v = sx[i];
+            if (v <= 0.0) v = rv;
+            sx[i] = log10(v);
+        }
+        __syncwarp();
+        const double s0 = sx[0];
+        const double hgt = sx[n - 1] - s0;
+        int mark = 0, mark2 = 0;
+        auto largmax = [&](double v) {   // first i with sx[i] <= v, else 0
+            for (int b = 0; b < n; b += 64) {
+                const int i = b + lane;
+                const unsigned long long m = __ballot(i < n && sx[i] <= v);
+                if (m) return b + (__ffsll((long long)m) - 1);
+            }
+            return 0;
+        };
+        auto rargmax = [&](double v) {   // last i with sx[i] >= v, else n-1
+            for (int top = n - 1; top >= 0; top -= 64) {
+                const int p = top - lane;
+                const unsigned long long m = __ballot(p >= 0 && sx[p] >= v);
+                if (m) return top - (__ffsll((long long)m) - 1);
+            }
+            return n - 1;
+        };
+        if (n > 3) {
+            mark = largmax(s0 + hgt * 0.22) - 1;
+            mark2 = rargmax(s0 + hgt * 0.87);
+            if (mark2 == n) mark2 -= 1;
+            if (mark >= mark2) mark = mark2 - 1;
+        } else if (n == 3) {
+            mark = largmax(s0 + hgt * 0.22) - 1;
+            mark2 = n - 1;
+            if (mark >= mark2) mark = mark2 - 1;
+        } else {
+            mark = 0;
+            mark2 = n - 1;
+        }
+        if (mark < 0) {
+            o.flags |= HCX_RES_FIT_UB;
+            mu_t = 0.0;
+            beta_t = 0.0;
+        } else if (lane == 0) {
+            const int m = mark2 - mark + 1;
+            const double *y = sx + mark;
+            if (m == 1) {
+                mu_t = 0.0;
+                beta_t = y[0];
+            } else {
+                double xbar = 0.0, ybar = 0.0, top = 0.0, bot = 0.0;
+                for (int i = 0; i < m; i++) {
+                    xbar = xbar + (double)(stt1 + mark + i);
+                    ybar = ybar + y[i];
+                }
+                xbar = xbar / (double)m;
+                ybar = ybar / (double)m;
+                for (int i = 0; i < m; i++) {
+                    const double dx = (double)(stt1 + mark + i) - xbar;
+                    top = top + dx * (y[i] - ybar);
+                    bot = bot + dx * dx;
+                }
+                mu_t = top / bot;
+                beta_t = ybar - mu_t * xbar;
+            }
+            if (!(fabs(mu_t) < 2.0e3) || !(fabs(beta_t) < 2.0e3)) {
+                o.flags |= HCX_RES_FIT_UB;
+                mu_t = 0.0;
+                beta_t = 0.0;
+            }
+        }
+    }
+    o.mu = (int32_t)(mu_t * 1e6);
+    o.beta = (int32_t)(beta_t * 1e6);
+    return o;
+}
+
+__global__ __launch_bounds__(kBT) void k_tailfit(hcx_result *res, const uint32_t *hist, uint32_t hbins, int32_t n,
+                                                 uint32_t min_cpsm, double expect_max) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int32_t s = blockIdx.x * (kBT / 64) + w;
+    if (s >= n) return;
+    double *sx = (double *)smem_raw + (size_t)w * hbins;
+    hcx_result r = res[s];
+    uint16_t flags = r.flags & ~(HCX_RES_VALID | HCX_RES_REPORTED);
+    if (r.cpsms < min_cpsm || r.cpsms == 0) {
+        if (lane == 0) res[s].flags = flags;
+        return;
+    }
+    const int32_t hyp = (int32_t)((double)(r.hyperscore * 10.0f) + 0.5);
+    FitOut f = tailfit_wave(hist + (size_t)s * hbins, hbins, r.cpsms, hyp, sx);
+    if (lane == 0) {
+        r.maxhypscore = hyp;
+        r.mu = f.mu;
+        r.beta = f.beta;
+        r.minhypscore = f.stt1;
+        r.nexthypscore = f.end1;
+        double w_ = r.mu, b_ = r.beta;
+        w_ /= 1e6;
+        b_ /= 1e6;
+        const double lgs = (w_ * r.maxhypscore) + b_;
+        double e = pow(10.0, lgs);
+        e *= (double)r.cpsms;
+        r.e_x = e;
+        flags |= (uint16_t)f.flags | HCX_RES_VALID;
+        if (e < expect_max) flags |= HCX_RES_REPORTED;
+        r.flags = flags;
+        res[s] = r;
+    }
+}
+
+__global__ __launch_bounds__(kBT) void k_tailfit_raw(const uint32_t *hist, uint32_t hbins, uint32_t n,
+                                                     const uint32_t *cpsms, const int32_t *maxhyp, int32_t *out4,
+                                                     uint32_t *flags) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t s = blockIdx.x * (kBT / 64) + w;
+    if (s >= n) return;
+    double *sx = (double *)smem_raw + (size_t)w * hbins;
+    FitOut f = tailfit_wave(hist + (size_t)s * hbins, hbins, cpsms[s], maxhyp[s], sx);
+    if (lane == 0) {
+        out4[4 * s + 0] = f.mu;
+        out4[4 * s + 1] = f.beta;
+        out4[4 * s + 2] = f.stt1;
+        out4[4 * s + 3] = f.end1;
+        if (flags) flags[s] = f.flags;
+    }
+}
+
+}  // namespace
+
+size_t query_smem_bytes(const QueryArgs &a, bool replay) { return smem_layout(a, replay, nullptr, nullptr); }
+
+int launch_query(hcx_ctx *ctx, const QueryArgs &a, int kernel, bool timed, float *ms) {
+    (void)kernel;
+    const size_t sm = query_smem_bytes(a, false);
+    HCX_HIP(hipFuncSetAttribute((const void *)k_query<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+    if (timed) HCX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    if (a.nspec > 0) {
+        k_query<false><<<a.nspec, kBT, sm, ctx->stream>>>(a);
+        HCX_HIP(hipGetLastError());
+    }
+    if (timed) {
+        HCX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+        HCX_HIP(hipEventSynchronize(ctx->ev1));
+        HCX_HIP(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    }
+    return HCX_SUCCESS;
+}
+
+int launch_replay(hcx_ctx *ctx, const QueryArgs &a) {
+    const size_t sm = query_smem_bytes(a, true);
+    HCX_HIP(hipFuncSetAttribute((const void *)k_query<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+    const int grid = std::max(1, std::min(a.nspec, ctx->num_cu * 2));
+    k_query<true><<<grid, kBT, sm, ctx->stream>>>(a);
+    HCX_HIP(hipGetLastError());
+    return HCX_SUCCESS;
+}
+
+int launch_candidates(hcx_ctx *ctx, const QueryArgs &a) {
+    const size_t sm = query_smem_bytes(a, false);
+    HCX_HIP(hipFuncSetAttribute((const void *)k_candidates, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+    if (a.nspec > 0) {
+        k_candidates<<<a.nspec, kBT, sm, ctx->stream>>>(a);
+        HCX_HIP(hipGetLastError());
+    }
+    return HCX_SUCCESS;
+}
+
+int launch_tailfit(hcx_ctx *ctx, hcx_result *res, const uint32_t *hist, uint32_t hbins, int32_t n,
+                   uint32_t min_cpsm, double expect_max) {
+    if (n <= 0) return HCX_SUCCESS;
+    const size_t sm = (size_t)(kBT / 64) * hbins * sizeof(double);
+    HCX_HIP(hipFuncSetAttribute((const void *)k_tailfit, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+    const int per = kBT / 64;
+    k_tailfit<<<(n + per - 1) / per, kBT, sm, ctx->stream>>>(res, hist, hbins, n, min_cpsm, expect_max);
+    HCX_HIP(hipGetLastError());
+    return HCX_SUCCESS;
+}
+
+int launch_tailfit_raw(hcx_ctx *ctx, const uint32_t *hist, uint32_t hbins, uint32_t n, const uint32_t *cpsms,
+                       const int32_t *maxhyp, int32_t *out4, uint32_t *flags) {
+    if (n == 0) return HCX_SUCCESS;
+    const size_t sm = (size_t)(kBT / 64) * hbins * sizeof(double);
+    HCX_HIP(hipFuncSetAttribute((const void *)k_tailfit_raw, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+    const uint32_t per = kBT / 64;
+    k_tailfit_raw<<<(n + per - 1) / per, kBT, sm, ctx->stream>>>(hist, hbins, n, cpsms, maxhyp, out4, flags);
+    HCX_HIP(hipGetLastError());
+    return HCX_SUCCESS;
+}
+
+}  // namespace hcx

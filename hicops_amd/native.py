@@ -1,0 +1,175 @@
+"""ctypes bindings of libhcx.so (include/hcx.h) and libhcx_host.so (include/hcx_datagen.h).
+
+The search path runs only through libhcx.so.  If the library is missing or a
+call fails, an exception is raised: there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIBHCX = PKG / "libhcx.so"
+LIBHOST = PKG / "libhcx_host.so"
+
+HCX_MAX_MOD_TYPES = 15
+
+# status codes (slmerr.h values)
+SUCCESS = 0
+ERR_INVLD_PTR = -3001
+ERR_INVLD_MEMORY = -3002
+ERR_INVLD_PARAM = -3003
+ERR_BAD_MEM_ALLOC = -3004
+ERR_INVLD_SIZE = -3006
+ERR_DEVICE = -3100
+
+RES_VALID = 0x1
+RES_REPORTED = 0x2
+RES_TIE_REPLAYED = 0x4
+RES_HIST_CLAMP = 0x8
+RES_LGF_CLAMP = 0x10
+RES_FIT_UB = 0x20
+RES_SURROGATE = 0x40
+RES_AUDITED = 0x80
+
+KERNEL_PLAIN = 0
+KERNEL_BATCHED = 1
+EVAL_TAILFIT = 0
+EVAL_SURROGATE = 1
+
+
+class HcxError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"hcx error {code}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("maxz", C.c_uint32), ("scale", C.c_uint32), ("min_mass", C.c_uint32),
+        ("max_mass", C.c_uint32), ("dF", C.c_uint32), ("min_shp", C.c_uint32),
+        ("min_cpsm", C.c_uint32), ("topmatches", C.c_uint32), ("max_hyperscore", C.c_uint32),
+        ("max_shdpeaks", C.c_uint32), ("nmods", C.c_uint32),
+        ("mod_mass", C.c_uint32 * HCX_MAX_MOD_TYPES),
+        ("dM", C.c_double), ("expect_max", C.c_double),
+    ]
+
+    @property
+    def hbins(self) -> int:
+        return 10 * self.max_hyperscore + 2
+
+
+class QueriesView(C.Structure):
+    _fields_ = [
+        ("moz", C.c_void_p), ("intensity", C.c_void_p), ("idx", C.c_void_p),
+        ("precurse", C.c_void_p), ("charges", C.c_void_p), ("rtimes", C.c_void_p),
+        ("num_specs", C.c_int32), ("file_num", C.c_int32),
+    ]
+
+
+class SearchOpts(C.Structure):
+    _fields_ = [
+        ("kernel", C.c_int32), ("evalue", C.c_int32), ("tile", C.c_uint32),
+        ("audit_every", C.c_uint32), ("audit_tol", C.c_double), ("use_graph", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class DbGenCfg(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64), ("npep", C.c_uint32), ("min_len", C.c_uint32),
+        ("max_len", C.c_uint32), ("min_mass", C.c_uint32), ("max_mass", C.c_uint32),
+        ("scale", C.c_uint32), ("nmods", C.c_uint32), ("max_mods", C.c_uint32),
+        ("mod_residues", (C.c_char * 4) * HCX_MAX_MOD_TYPES),
+        ("mod_max", C.c_uint32 * HCX_MAX_MOD_TYPES),
+        ("mod_mass", C.c_uint32 * HCX_MAX_MOD_TYPES),
+    ]
+
+
+class SpecGenCfg(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64), ("nspec", C.c_uint32), ("min_charge", C.c_uint32),
+        ("max_charge", C.c_uint32), ("frag_maxz", C.c_uint32), ("n_noise", C.c_uint32),
+        ("peaks_min", C.c_uint32), ("peaks_max", C.c_uint32), ("scale", C.c_uint32),
+        ("base_int", C.c_uint32), ("min_int", C.c_uint32), ("window_mode", C.c_uint32),
+        ("keep_prob", C.c_double), ("noise_lo", C.c_double), ("noise_hi", C.c_double),
+        ("jitter_sd", C.c_double), ("sig_mu", C.c_double), ("sig_sigma", C.c_double),
+        ("noise_mu", C.c_double), ("noise_sigma", C.c_double), ("dM", C.c_double),
+        ("wmin", C.c_double), ("wmax", C.c_double),
+    ]
+
+
+PEP_DTYPE = np.dtype([("mass", "<f4"), ("seq_id", "<u4"), ("sites", "<u8"),
+                      ("mod_num", "<u4"), ("pad_", "<u4")])
+RESULT_DTYPE = np.dtype([
+    ("hyperscore", "<f4"), ("psid", "<i4"), ("idxoffset", "<u2"), ("sharedions", "<u2"),
+    ("totalions", "<u2"), ("flags", "<u2"), ("cpsms", "<u4"), ("maxhypscore", "<i4"),
+    ("mu", "<i4"), ("beta", "<i4"), ("minhypscore", "<i4"), ("nexthypscore", "<i4"),
+    ("bc", "<u4"), ("yc", "<u4"), ("e_x", "<f8")])
+STATS_DTYPE = np.dtype([("n_ia", "<u8"), ("n_probe", "<u8")])
+CAND_DTYPE = np.dtype([
+    ("spec", "<u4"), ("idxoffset", "<u2"), ("bc", "<u2"), ("psid", "<i4"), ("yc", "<u2"),
+    ("pad_", "<u2"), ("ibc", "<u8"), ("iyc", "<u8"), ("hyperscore", "<f4"), ("pad2_", "<u4")])
+
+assert PEP_DTYPE.itemsize == 24 and RESULT_DTYPE.itemsize == 56
+assert CAND_DTYPE.itemsize == 40 and C.sizeof(Params) == 120
+
+
+def ptr(a: np.ndarray | None) -> C.c_void_p:
+    if a is None:
+        return C.c_void_p(0)
+    assert a.flags["C_CONTIGUOUS"], "arrays passed to the C ABI must be contiguous"
+    return C.c_void_p(a.ctypes.data)
+
+
+_hcx = None
+_host = None
+
+
+def _load(path: Path, what: str) -> C.CDLL:
+    if not path.exists():
+        raise RuntimeError(
+            f"{what} is not built ({path}); run `python -m hicops_amd.build` "
+            "(the search path has no fallback implementation)")
+    return C.CDLL(str(path))
+
+
+def hcx() -> C.CDLL:
+    """The HIP library.  Raises if it is not built."""
+    global _hcx
+    if _hcx is None:
+        lib = _load(LIBHCX, "libhcx.so")
+        lib.hcx_last_error.restype = C.c_char_p
+        for name in ["hcx_create", "hcx_destroy", "hcx_index_build", "hcx_index_info",
+                     "hcx_index_export", "hcx_index_build_ms", "hcx_index_free",
+                     "hcx_queries_upload", "hcx_queries_free", "hcx_search",
+                     "hcx_search_device", "hcx_search_fetch", "hcx_search_candidates",
+                     "hcx_tailfit_batch", "hcx_lgfact_table", "hcx_heap_capacity",
+                     "hcx_selftest_divider"]:
+            getattr(lib, name).restype = C.c_int
+        _hcx = lib
+    return _hcx
+
+
+def host() -> C.CDLL:
+    global _host
+    if _host is None:
+        lib = _load(LIBHOST, "libhcx_host.so")
+        for name in ["hcx_db_create", "hcx_db_free", "hcx_db_lengths", "hcx_db_length_info",
+                     "hcx_db_length_copy", "hcx_db_length_view", "hcx_spec_create",
+                     "hcx_spec_free", "hcx_spec_info", "hcx_spec_copy"]:
+            getattr(lib, name).restype = C.c_int
+        _host = lib
+    return _host
+
+
+def check(rc: int, lib: C.CDLL | None = None) -> None:
+    if rc != SUCCESS:
+        msg = ""
+        try:
+            msg = (lib or hcx()).hcx_last_error().decode()
+        except Exception:  # noqa: BLE001 - the host library has no error text
+            pass
+        raise HcxError(rc, msg)

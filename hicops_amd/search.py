@@ -1,0 +1,183 @@
+"""Python face of the search core, mirroring the reference's entry points.
+
+  Engine(params)                  gParams + DSLIM_InitializeScorecard (dslim.cpp:390-437)
+  Engine.construct(L, seqs, ents) DSLIM_Construct(Index*) for one peptide length (dslim.cpp:46-138)
+  Engine.query_spectrum(queries)  DSLIM_QuerySpectrum(Queries*, Index*, nIdx) (dslim_query.cpp:515-842)
+                                  including expeRT::ModelTailFit and the e-value block (:771-818)
+  Engine.model_tail_fit(...)      expeRT::ModelTailFit(Results*) (expeRT.cpp:572-718) on histograms
+
+All work runs in libhcx.so on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import native as N
+
+
+@dataclass
+class SearchOutput:
+    results: np.ndarray           # RESULT_DTYPE per spectrum
+    hist: np.ndarray | None       # (nspec, hbins) u32 score histograms
+    stats: np.ndarray | None      # STATS_DTYPE per spectrum
+
+
+class Index:
+    """One peptide length's DSLIM index, resident in HBM."""
+
+    def __init__(self, engine: "Engine", peplen: int, seqs: np.ndarray, entries: np.ndarray):
+        self.engine = engine
+        self.peplen = peplen
+        seqs = np.ascontiguousarray(seqs, dtype=np.uint8)
+        entries = np.ascontiguousarray(entries, dtype=N.PEP_DTYPE)
+        nseqs = len(seqs) // peplen
+        h = C.c_void_p()
+        N.check(N.hcx().hcx_index_build(engine._h, peplen, N.ptr(seqs), nseqs, N.ptr(entries),
+                                        len(entries), C.byref(h)))
+        self._h = h
+        pl, sl, ne, ni = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint64()
+        N.check(N.hcx().hcx_index_info(h, C.byref(pl), C.byref(sl), C.byref(ne), C.byref(ni)))
+        self.speclen, self.n, self.nions = sl.value, ne.value, ni.value
+
+    @property
+    def build_ms(self) -> float:
+        ms = C.c_float()
+        N.check(N.hcx().hcx_index_build_ms(self._h, C.byref(ms)))
+        return ms.value
+
+    def export(self) -> tuple[np.ndarray, np.ndarray]:
+        iA = np.zeros(self.nions, np.uint32)
+        bA = np.zeros(self.engine.NB + 1, np.uint32)
+        N.check(N.hcx().hcx_index_export(self._h, N.ptr(iA), N.ptr(bA)))
+        return iA, bA
+
+    def free(self) -> None:
+        if getattr(self, "_h", None):
+            N.hcx().hcx_index_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.free()
+
+
+class Batch:
+    """A Queries batch resident in HBM."""
+
+    def __init__(self, engine: "Engine", queries):
+        self.engine = engine
+        self.queries = queries
+        h = C.c_void_p()
+        v = queries.view()
+        N.check(N.hcx().hcx_queries_upload(engine._h, C.byref(v), C.byref(h)))
+        self._h = h
+        self.nspec = queries.nspec
+
+    def free(self) -> None:
+        if getattr(self, "_h", None):
+            N.hcx().hcx_queries_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.free()
+
+
+class Engine:
+    def __init__(self, params: N.Params, device: int = 0):
+        self.params = params
+        h = C.c_void_p()
+        N.check(N.hcx().hcx_create(C.byref(params), device, C.byref(h)))
+        self._h = h
+        self.NB = params.max_mass * params.scale
+        self.hbins = params.hbins
+        self.indices: list[Index] = []
+
+    # DSLIM_Construct, one length at a time (hicops.cpp:200-300 loop)
+    def construct(self, peplen: int, seqs: np.ndarray, entries: np.ndarray) -> Index:
+        ix = Index(self, peplen, seqs, entries)
+        self.indices.append(ix)
+        return ix
+
+    def construct_db(self, db) -> list[Index]:
+        return [self.construct(L, db.seqs(L), db.entries(L)) for L in db.lengths]
+
+    def upload(self, queries) -> Batch:
+        return Batch(self, queries)
+
+    def _idx_array(self, indices):
+        indices = self.indices if indices is None else indices
+        arr = (C.c_void_p * max(1, len(indices)))(*[ix._h for ix in indices])
+        return arr, len(indices)
+
+    @staticmethod
+    def _opts(kernel=N.KERNEL_PLAIN, evalue=N.EVAL_TAILFIT, tile=0, use_graph=0) -> N.SearchOpts:
+        o = N.SearchOpts()
+        o.kernel, o.evalue, o.tile, o.use_graph = kernel, evalue, tile, use_graph
+        o.audit_every, o.audit_tol = 0, 1e-3
+        return o
+
+    # DSLIM_QuerySpectrum + ModelTailFit + e-value
+    def query_spectrum(self, batch: Batch, indices=None, want_hist=True, want_stats=True,
+                       **opts) -> SearchOutput:
+        arr, n = self._idx_array(indices)
+        res = np.zeros(batch.nspec, N.RESULT_DTYPE)
+        hist = np.zeros((batch.nspec, self.hbins), np.uint32) if want_hist else None
+        stats = np.zeros(batch.nspec, N.STATS_DTYPE) if want_stats else None
+        o = self._opts(**opts)
+        N.check(N.hcx().hcx_search(self._h, arr, n, batch._h, C.byref(o), N.ptr(res),
+                                   N.ptr(hist), N.ptr(stats)))
+        return SearchOutput(res, hist, stats)
+
+    def query_device(self, batch: Batch, indices=None, **opts) -> tuple[float, float]:
+        """Run a search leaving results on the device; returns (query kernel ms, total ms)."""
+        arr, n = self._idx_array(indices)
+        o = self._opts(**opts)
+        k, t = C.c_float(), C.c_float()
+        N.check(N.hcx().hcx_search_device(self._h, arr, n, batch._h, C.byref(o), C.byref(k), C.byref(t)))
+        return k.value, t.value
+
+    def fetch(self, batch: Batch, want_hist=True, want_stats=True) -> SearchOutput:
+        res = np.zeros(batch.nspec, N.RESULT_DTYPE)
+        hist = np.zeros((batch.nspec, self.hbins), np.uint32) if want_hist else None
+        stats = np.zeros(batch.nspec, N.STATS_DTYPE) if want_stats else None
+        N.check(N.hcx().hcx_search_fetch(self._h, batch._h, N.ptr(res), N.ptr(hist), N.ptr(stats)))
+        return SearchOutput(res, hist, stats)
+
+    def candidates(self, batch: Batch, indices=None, cap: int = 1 << 20) -> np.ndarray:
+        arr, n = self._idx_array(indices)
+        out = np.zeros(cap, N.CAND_DTYPE)
+        cnt = C.c_uint64()
+        N.check(N.hcx().hcx_search_candidates(self._h, arr, n, batch._h, N.ptr(out), cap, C.byref(cnt)))
+        if cnt.value > cap:
+            raise N.HcxError(N.ERR_INVLD_SIZE, f"candidate dump needs {cnt.value} > {cap}")
+        return out[:cnt.value]
+
+    # expeRT::ModelTailFit over a batch of histograms
+    def model_tail_fit(self, hist: np.ndarray, cpsms: np.ndarray, maxhyp: np.ndarray):
+        hist = np.ascontiguousarray(hist, np.uint32)
+        n, hb = hist.shape
+        cpsms = np.ascontiguousarray(cpsms, np.uint32)
+        maxhyp = np.ascontiguousarray(maxhyp, np.int32)
+        out = {k: np.zeros(n, np.int32) for k in ("mu", "beta", "minhyp", "nexthyp")}
+        flags = np.zeros(n, np.uint32)
+        N.check(N.hcx().hcx_tailfit_batch(self._h, N.ptr(hist), hb, n, N.ptr(cpsms), N.ptr(maxhyp),
+                                          N.ptr(out["mu"]), N.ptr(out["beta"]), N.ptr(out["minhyp"]),
+                                          N.ptr(out["nexthyp"]), N.ptr(flags)))
+        out["flags"] = flags
+        return out
+
+    def close(self) -> None:
+        for ix in self.indices:
+            ix.free()
+        self.indices = []
+        if getattr(self, "_h", None):
+            N.hcx().hcx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass

@@ -1,0 +1,217 @@
+/*
+ * hcx.h -- C ABI of the MI355X-native one-rank HiCOPS search core.
+ *
+ * The reference exposes its search core as mangled C++ entry points in
+ * libhicops-core (source/core/include/dslim.h, expeRT.h).  This header is the
+ * device boundary underneath them: plain C, plain pointers and sizes, no
+ * C++ or torch types.  Every entry point returns a status code with the same
+ * meaning as the reference's status_t (source/core/include/slmerr.h:24-43).
+ *
+ * Reference interface each entry point replaces:
+ *   hcx_create / hcx_destroy      gParams set-up + DSLIM_InitializeScorecard
+ *                                 (source/core/dslim.cpp:390-437), DSLIM_DeallocateSC
+ *   hcx_index_build               DSLIM_Construct(Index*)            (dslim.h:49, dslim.cpp:46-138)
+ *   hcx_index_export              read-back of Index.ionIndex[0].{iA,bA} (slm_dsts.h:306-320)
+ *   hcx_index_free                DSLIM_DeallocateIonIndex(Index*)   (dslim.h:149, dslim.cpp:644-691)
+ *   hcx_queries_upload            hand-over of a Queries batch       (slm_dsts.h:422-559)
+ *   hcx_search                    DSLIM_QuerySpectrum(Queries*, Index*, uint_t) (dslim.h:184,
+ *                                 dslim_query.cpp:515-842) with expeRT::ModelTailFit(Results*)
+ *                                 (expeRT.h:143, expeRT.cpp:572-718) and the e-value block
+ *                                 (dslim_query.cpp:771-818); results in place of DFile_PrintScore
+ *                                 (dslim_fileout.h:29)
+ *   hcx_tailfit_batch             expeRT::ModelTailFit(Results*) over a batch of histograms
+ *   hcx_surrogate_*               no counterpart: learned tail-parameter model with audit fallback
+ */
+#ifndef HCX_H
+#define HCX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: same values as source/core/include/slmerr.h:24-43 ---- */
+#define HCX_SUCCESS            0
+#define HCX_ERR_INVLD_PTR      (-3001)
+#define HCX_ERR_INVLD_MEMORY   (-3002)
+#define HCX_ERR_INVLD_PARAM    (-3003)
+#define HCX_ERR_BAD_MEM_ALLOC  (-3004)
+#define HCX_ERR_INVLD_SIZE     (-3006)
+#define HCX_ERR_FILE_NOT_FOUND (-3007)
+#define HCX_ERR_NOT_ENOUGH_DATA (-3010)
+#define HCX_ERR_DEVICE         (-3100) /* a HIP call failed; hcx_last_error() has the text */
+
+/* ---- compile-time knobs of the reference, as run-time fields ----
+ * gParams (slm_dsts.h:359-418) plus the CMake knobs MAX_HYPERSCORE and
+ * MAX_SHDPEAKS (CMakeLists.txt:171-177).  */
+#define HCX_MAX_MOD_TYPES 15   /* slm_dsts.h:28 */
+
+typedef struct hcx_params {
+    uint32_t maxz;            /* max fragment charge                  (default 3)      */
+    uint32_t scale;           /* bins per Da                           (default 100)    */
+    uint32_t min_mass;        /* legal precursor mass range (Da)      (500)            */
+    uint32_t max_mass;        /*                                        (5000)           */
+    uint32_t dF;              /* fragment tolerance in bins             (tol*100, Q1)    */
+    uint32_t min_shp;         /* minimum shared peaks                   (4)              */
+    uint32_t min_cpsm;        /* minimum candidate PSMs for an e-value  (4)              */
+    uint32_t topmatches;      /* heap capacity = 2^floor(log2(t)+0.999)-1 (10 -> 15)     */
+    uint32_t max_hyperscore;  /* histogram has 10*max_hyperscore+2 bins (100)            */
+    uint32_t max_shdpeaks;    /* lgfact table has max_shdpeaks+1 entries (80)            */
+    uint32_t nmods;           /* number of variable-mod types in mod_mass[]                */
+    uint32_t mod_mass[HCX_MAX_MOD_TYPES]; /* (uint)(Da*scale), hicops.cpp:661          */
+    double   dM;              /* precursor tolerance in Da; < 0 disables the search (Q8)   */
+    double   expect_max;      /* report a PSM only if e < expect_max                        */
+} hcx_params;
+
+/* pepEntry (slm_dsts.h:146-282): 24 bytes, identical layout. */
+typedef struct hcx_pep_entry {
+    float    mass;      /* precursor mass (float, UTILS_CalculatePepMass / ModMass)  */
+    uint32_t seq_id;    /* row in the per-length sequence table                      */
+    uint64_t sites;     /* bit i set: residue i carries a variable mod               */
+    uint32_t mod_num;   /* 4 bits per mod: (mod type + 1), first mod in bits 0-3     */
+    uint32_t pad_;
+} hcx_pep_entry;
+
+/* Queries (slm_dsts.h:422-468): CSR spectra. */
+typedef struct hcx_queries_view {
+    const uint32_t *moz;        /* peak bins, (uint)(mz*scale)                          */
+    const uint32_t *intensity;  /* normalised intensities                              */
+    const uint32_t *idx;        /* num_specs+1 row offsets                             */
+    const float    *precurse;   /* precursor masses                                    */
+    const int32_t  *charges;    /* precursor charges                                   */
+    const float    *rtimes;     /* retention times (may be NULL)                       */
+    int32_t         num_specs;
+    int32_t         file_num;
+} hcx_queries_view;
+
+/* Per-spectrum search result: the top hCell (slm_dsts.h:562-662) plus the
+ * Results fields (slm_dsts.h:694-754) and the e-value of dslim_query.cpp:786-799. */
+typedef struct hcx_result {
+    float    hyperscore;    /* top hit hyperscore (float, as hCell.hyperscore)          */
+    int32_t  psid;          /* top hit: index into its length's sorted pepEntries      */
+    uint16_t idxoffset;     /* top hit: index of its length in the index list          */
+    uint16_t sharedions;    /* bc + yc of the top hit                                   */
+    uint16_t totalions;     /* speclen of the top hit's length                          */
+    uint16_t flags;         /* HCX_RES_* bits                                           */
+    uint32_t cpsms;         /* candidate PSMs (shpk >= min_shp and hs > 0)              */
+    int32_t  maxhypscore;   /* (int)(hs*10 + 0.5)                                       */
+    int32_t  mu;            /* tail-fit slope * 1e6, truncated (Q19)                    */
+    int32_t  beta;          /* tail-fit intercept * 1e6, truncated                       */
+    int32_t  minhypscore;   /* tail-fit stt1                                             */
+    int32_t  nexthypscore;  /* tail-fit end1                                             */
+    uint32_t bc;            /* b-ion count of the top hit                               */
+    uint32_t yc;            /* y-ion count of the top hit                               */
+    double   e_x;           /* e-value = 10^(mu/1e6*maxhyp + beta/1e6) * cpsms          */
+} hcx_result;
+
+#define HCX_RES_VALID        0x0001u /* cpsms >= min_cpsm: top hit + e-value valid     */
+#define HCX_RES_REPORTED     0x0002u /* e_x < expect_max (DFile_PrintScore would run)  */
+#define HCX_RES_TIE_REPLAYED 0x0004u /* max hyperscore tied: heap replayed in order    */
+#define HCX_RES_HIST_CLAMP   0x0008u /* a score bin exceeded the histogram (ref. UB)   */
+#define HCX_RES_LGF_CLAMP    0x0010u /* bc or yc exceeded max_shdpeaks (ref. UB)       */
+#define HCX_RES_FIT_UB       0x0020u /* tail fit hit an undefined path of the reference */
+#define HCX_RES_SURROGATE    0x0040u /* e-value from the surrogate model               */
+#define HCX_RES_AUDITED      0x0080u /* spectrum was in the surrogate audit sample      */
+
+/* Per-spectrum work counters (the algorithmic-bytes inputs). */
+typedef struct hcx_stats {
+    uint64_t n_ia;      /* iA entries visited inside the precursor windows         */
+    uint64_t n_probe;   /* (length, peak, bin) probes that read bA[bin], bA[bin+1]  */
+} hcx_stats;
+
+typedef struct hcx_ctx    hcx_ctx;
+typedef struct hcx_index  hcx_index;
+typedef struct hcx_qbatch hcx_qbatch;
+
+/* Query kernel choice. */
+#define HCX_KERNEL_PLAIN   0  /* one workgroup per spectrum                         */
+#define HCX_KERNEL_BATCHED 1  /* persistent, work-stealing over (spectrum, window)   */
+
+/* E-value model choice. */
+#define HCX_EVAL_TAILFIT   0  /* expeRT log-linear tail fit on every spectrum        */
+#define HCX_EVAL_SURROGATE 1  /* MLP prediction, tail fit on the audit sample        */
+
+typedef struct hcx_search_opts {
+    int32_t  kernel;          /* HCX_KERNEL_*                                       */
+    int32_t  evalue;          /* HCX_EVAL_*                                         */
+    uint32_t tile;            /* candidates per LDS tile (0 = library default)      */
+    uint32_t audit_every;     /* surrogate: tail-fit every n-th spectrum (0 = 64)   */
+    double   audit_tol;       /* surrogate: max |rel e error| before fallback       */
+    int32_t  use_graph;       /* capture the query launch sequence in a hipGraph    */
+    int32_t  reserved;
+} hcx_search_opts;
+
+/* ---- context ---- */
+int  hcx_create(const hcx_params *params, int device, hcx_ctx **out);
+int  hcx_destroy(hcx_ctx *ctx);
+const char *hcx_last_error(void);
+int  hcx_heap_capacity(uint32_t topmatches);   /* dslim.cpp:422-428 (Q14) */
+/* lgfact table the device uses (utils.h:115-131), max_shdpeaks+1 doubles */
+int  hcx_lgfact_table(const hcx_ctx *ctx, double *out, uint32_t n);
+
+/* ---- index (one per peptide length; DSLIM_Construct) ---- */
+int  hcx_index_build(hcx_ctx *ctx, uint32_t peplen,
+                     const char *seqs, uint32_t nseqs,
+                     const hcx_pep_entry *entries, uint32_t nentries,
+                     hcx_index **out);
+int  hcx_index_info(const hcx_index *idx, uint32_t *peplen, uint32_t *speclen,
+                    uint32_t *nentries, uint64_t *nions);
+int  hcx_index_export(const hcx_index *idx, uint32_t *iA, uint32_t *bA);
+/* last build's device time in ms (fragment generation + radix sort + bA scan) */
+int  hcx_index_build_ms(const hcx_index *idx, float *ms);
+int  hcx_index_free(hcx_index *idx);
+
+/* ---- query batches ---- */
+int  hcx_queries_upload(hcx_ctx *ctx, const hcx_queries_view *q, hcx_qbatch **out);
+int  hcx_queries_free(hcx_qbatch *qb);
+
+/* ---- search: DSLIM_QuerySpectrum + ModelTailFit + e-value ----
+ * results:  num_specs entries (host).            Required.
+ * hist:     num_specs * (10*max_hyperscore+2) u32 (host).  Optional (NULL).
+ * stats:    num_specs entries (host).            Optional (NULL).          */
+int  hcx_search(hcx_ctx *ctx, hcx_index *const *idx, int nidx, hcx_qbatch *qb,
+                const hcx_search_opts *opts, hcx_result *results,
+                uint32_t *hist, hcx_stats *stats);
+
+/* Device-resident variant for timing: runs the search and leaves results on
+ * the device; hcx_search_fetch copies them out.  Returns the query kernel's
+ * device time (HIP events on the library stream) in *kernel_ms. */
+int  hcx_search_device(hcx_ctx *ctx, hcx_index *const *idx, int nidx, hcx_qbatch *qb,
+                       const hcx_search_opts *opts, float *kernel_ms, float *total_ms);
+int  hcx_search_fetch(hcx_ctx *ctx, hcx_qbatch *qb, hcx_result *results,
+                      uint32_t *hist, hcx_stats *stats);
+
+/* Candidate dump for parity tests: every candidate with shpk >= min_shp
+ * and hs > 0, as (spectrum, idxoffset, psid, bc, yc, ibc, iyc, hs).
+ * Returns the count in *n (capacity cap).                                   */
+typedef struct hcx_cand {
+    uint32_t spec;
+    uint16_t idxoffset;
+    uint16_t bc;
+    int32_t  psid;
+    uint16_t yc;
+    uint16_t pad_;
+    uint64_t ibc;
+    uint64_t iyc;
+    float    hyperscore;
+    uint32_t pad2_;
+} hcx_cand;
+int  hcx_search_candidates(hcx_ctx *ctx, hcx_index *const *idx, int nidx, hcx_qbatch *qb,
+                           hcx_cand *out, uint64_t cap, uint64_t *n);
+
+/* ---- expeRT::ModelTailFit over a batch of histograms ----
+ * hist: n * hbins u32 counts; cpsms, maxhyp: n; out: mu, beta, minhyp, nexthyp. */
+int  hcx_tailfit_batch(hcx_ctx *ctx, const uint32_t *hist, uint32_t hbins, uint32_t n,
+                       const uint32_t *cpsms, const int32_t *maxhyp,
+                       int32_t *mu, int32_t *beta, int32_t *minhyp, int32_t *nexthyp,
+                       uint32_t *flags);
+
+/* ---- host self-test of the raw / speclen divider the query kernels use ---- */
+int  hcx_selftest_divider(uint32_t d, const uint32_t *nums, uint32_t count, uint32_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HCX_H */

@@ -1,0 +1,163 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker or the reported CPU baseline.
+The product package (hicops_amd) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from hicops_amd import native as N
+
+LIB = Path(__file__).resolve().parent / "liboracle.so"
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            raise RuntimeError(f"oracle not built ({LIB}); run `make -C oracle`")
+        L = C.CDLL(str(LIB))
+        L.orc_pep_mass.restype = C.c_float
+        L.orc_mod_mass.restype = C.c_float
+        L.orc_gen_spectrum.restype = C.c_float
+        L.orc_gen_mod_spectrum.restype = C.c_float
+        L.orc_aa_mass.restype = C.c_float
+        L.orc_stat_mod.restype = C.c_float
+        L.orc_aa_mass.argtypes = [C.c_char]
+        L.orc_stat_mod.argtypes = [C.c_char]
+        for f in ("orc_index_build", "orc_window", "orc_search", "orc_candidates", "orc_count_work",
+                  "orc_tailfit", "orc_heap_replay", "orc_heap_capacity"):
+            getattr(L, f).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+class OrcIndex(C.Structure):
+    _fields_ = [("peplen", C.c_uint32), ("n", C.c_uint32), ("mass", C.c_void_p),
+                ("iA", C.c_void_p), ("bA", C.c_void_p)]
+
+
+def lgfact(n: int) -> np.ndarray:
+    out = np.zeros(n, np.float64)
+    lib().orc_lgfact(N.ptr(out), C.c_uint32(n))
+    return out
+
+
+def pep_mass(seq: str) -> float:
+    b = seq.encode()
+    return lib().orc_pep_mass(b, C.c_uint32(len(b)))
+
+
+def mod_mass(seq: str, mod_num: int, params: N.Params) -> float:
+    b = seq.encode()
+    return lib().orc_mod_mass(b, C.c_uint32(len(b)), C.c_uint32(mod_num), C.byref(params))
+
+
+def gen_spectrum(seq: str, params: N.Params, sites: int = 0, mod_num: int = 0) -> tuple[float, np.ndarray]:
+    b = seq.encode()
+    out = np.zeros(2 * params.maxz * (len(b) - 1), np.uint32)
+    if mod_num == 0:
+        m = lib().orc_gen_spectrum(b, C.c_uint32(len(b)), C.byref(params), N.ptr(out))
+    else:
+        e = np.zeros(1, N.PEP_DTYPE)
+        e["sites"], e["mod_num"] = sites, mod_num
+        m = lib().orc_gen_mod_spectrum(b, C.c_uint32(len(b)), N.ptr(e), C.byref(params), N.ptr(out))
+    return m, out
+
+
+def index_build(params: N.Params, L: int, seqs: np.ndarray, entries: np.ndarray):
+    n = len(entries)
+    speclen = 2 * params.maxz * (L - 1)
+    iA = np.zeros(n * speclen, np.uint32)
+    bA = np.zeros(params.max_mass * params.scale + 1, np.uint32)
+    seqs = np.ascontiguousarray(seqs, np.uint8)
+    entries = np.ascontiguousarray(entries, N.PEP_DTYPE)
+    rc = lib().orc_index_build(C.byref(params), C.c_uint32(L), N.ptr(seqs), N.ptr(entries),
+                               C.c_uint32(n), N.ptr(iA), N.ptr(bA))
+    if rc != 0:
+        raise RuntimeError(f"orc_index_build failed: {rc}")
+    return iA, bA
+
+
+def window(mass: np.ndarray, prec: float, dM: float):
+    lo, hi = C.c_int32(), C.c_int32()
+    mass = np.ascontiguousarray(mass, np.float32)
+    ok = lib().orc_window(N.ptr(mass), C.c_uint32(len(mass)), C.c_float(prec), C.c_double(dM),
+                          C.byref(lo), C.byref(hi))
+    return bool(ok), lo.value, hi.value
+
+
+class OracleDB:
+    """The oracle's view of a database: per-length iA/bA built on the CPU."""
+
+    def __init__(self, params: N.Params, db, lengths=None):
+        self.params = params
+        self.lengths = list(db.lengths if lengths is None else lengths)
+        self.keep = []
+        self.idx = (OrcIndex * max(1, len(self.lengths)))()
+        self.iA, self.bA = {}, {}
+        for i, L in enumerate(self.lengths):
+            ents = db.entries(L)
+            iA, bA = index_build(params, L, db.seqs(L), ents)
+            mass = np.ascontiguousarray(ents["mass"])
+            self.iA[L], self.bA[L] = iA, bA
+            self.keep += [iA, bA, mass]
+            self.idx[i] = OrcIndex(L, len(ents), mass.ctypes.data, iA.ctypes.data, bA.ctypes.data)
+
+    def search(self, queries, threads: int = 0):
+        n = queries.nspec
+        res = np.zeros(n, N.RESULT_DTYPE)
+        hist = np.zeros((n, self.params.hbins), np.uint32)
+        stats = np.zeros(n, N.STATS_DTYPE)
+        v = queries.view()
+        rc = lib().orc_search(C.byref(self.params), self.idx, len(self.lengths), C.byref(v), N.ptr(res),
+                              N.ptr(hist), N.ptr(stats), threads)
+        if rc != 0:
+            raise RuntimeError(f"orc_search failed: {rc}")
+        return res, hist, stats
+
+    def candidates(self, queries, s: int, cap: int = 1 << 20) -> np.ndarray:
+        out = np.zeros(cap, N.CAND_DTYPE)
+        cnt = C.c_uint64()
+        v = queries.view()
+        rc = lib().orc_candidates(C.byref(self.params), self.idx, len(self.lengths), C.byref(v), s, N.ptr(out),
+                                  C.c_uint64(cap), C.byref(cnt))
+        if rc != 0:
+            raise RuntimeError(f"orc_candidates failed: {rc}")
+        return out[:min(cnt.value, cap)]
+
+    def count_work(self, queries, threads: int = 0) -> np.ndarray:
+        st = np.zeros(queries.nspec, N.STATS_DTYPE)
+        v = queries.view()
+        rc = lib().orc_count_work(C.byref(self.params), self.idx, len(self.lengths), C.byref(v), N.ptr(st), threads)
+        if rc != 0:
+            raise RuntimeError(f"orc_count_work failed: {rc}")
+        return st
+
+
+def tailfit(hist: np.ndarray, cpsms: int, maxhyp: int):
+    h = np.ascontiguousarray(hist, np.float64)
+    mu, beta, mn, nx = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+    fl = C.c_uint32(0)
+    lib().orc_tailfit(N.ptr(h), C.c_uint32(len(h)), C.c_uint32(cpsms), C.c_int32(maxhyp), C.byref(mu), C.byref(beta),
+                      C.byref(mn), C.byref(nx), C.byref(fl))
+    return mu.value, beta.value, mn.value, nx.value, fl.value
+
+
+def heap_replay(hs, cap: int):
+    a = np.ascontiguousarray(hs, np.float32)
+    am, sz = C.c_int32(), C.c_int32()
+    rc = lib().orc_heap_replay(N.ptr(a), C.c_uint32(len(a)), C.c_int32(cap), C.byref(am), C.byref(sz))
+    if rc != 0:
+        raise RuntimeError("heap replay failed")
+    return am.value, sz.value
+
+
+def heap_capacity(t: int) -> int:
+    return lib().orc_heap_capacity(C.c_uint32(t))

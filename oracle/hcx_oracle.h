@@ -1,0 +1,98 @@
+/*
+ * hcx_oracle.h -- CPU restatement of the reference's one-rank search core.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the
+ * checker or the reported CPU baseline.  The product path (libhcx.so) never
+ * links, loads or calls it.
+ *
+ * Every function restates the reference algorithm at the file:line cited
+ * beside it.  The reference build needs CMake-generated headers
+ * (config.hpp and common.hpp, generated from the templates in cmake/Templates), so it is not compiled
+ * here; each function is pinned by the checks listed in its comment.
+ */
+#ifndef HCX_ORACLE_H
+#define HCX_ORACLE_H
+#include <stdint.h>
+#include "../include/hcx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* utils.cpp:41-98 residue + static-mod tables (float). */
+float orc_aa_mass(char c);
+float orc_stat_mod(char c);
+
+/* utils.h:115-131: val[0]=val[1]=0, val[i]=log10(i)+val[i-1].
+ * Pinned: closed form lgamma(i+1)/ln(10) (tests/test_oracle_pins.py). */
+void orc_lgfact(double *out, uint32_t n);
+
+/* utils.cpp:290-303.  Pinned: published monoisotopic peptide masses. */
+float orc_pep_mass(const char *seq, uint32_t len);
+/* utils.cpp:337-362 (first mod only, Q4).  Pinned: unmod mass + mod mass. */
+float orc_mod_mass(const char *seq, uint32_t len, uint32_t mod_num, const hcx_params *p);
+
+/* utils.cpp:231-276 / 378-505.  Returns the precursor mass; fills speclen
+ * = 2*maxz*(len-1) bins laid out [b z1..zZ][y z1..zZ].  Pinned: textbook
+ * b/y ion m/z within the per-residue truncation bound, complementarity. */
+float orc_gen_spectrum(const char *seq, uint32_t len, const hcx_params *p, uint32_t *out);
+float orc_gen_mod_spectrum(const char *seq, uint32_t len, const hcx_pep_entry *e,
+                           const hcx_params *p, uint32_t *out);
+
+/* dslim.cpp:46-138, 216-347, 361-388, 452-482: bA = exclusive scan of the bin
+ * histogram (NB+1 entries, NB = max_mass*scale); iA = ion ids sorted by bin,
+ * ascending within a bin.  Pinned: prefix-sum / permutation invariants and a
+ * brute-force sort on tiny inputs. */
+int orc_index_build(const hcx_params *p, uint32_t peplen, const char *seqs,
+                    const hcx_pep_entry *entries, uint32_t n,
+                    uint32_t *iA, uint32_t *bA);
+
+/* dslim_query.cpp:854-994.  Window [minl, maxl] over masses sorted
+ * ascending; returns 1 if valid.  Pinned: brute-force linear scan. */
+int orc_window(const float *mass, uint32_t n, float precursor, double dM,
+               int32_t *minl, int32_t *maxl);
+
+/* One per-length index as the query sees it. */
+typedef struct orc_index {
+    uint32_t peplen;
+    uint32_t n;            /* entries */
+    const float *mass;     /* n masses (sorted) */
+    const uint32_t *iA;
+    const uint32_t *bA;
+} orc_index;
+
+/* dslim_query.cpp:515-842 + expeRT.cpp:572-718, serial per spectrum,
+ * OpenMP over spectra (threads <= 0: library default).  hist (optional) is
+ * num_specs * (10*max_hyperscore+2) u32; stats optional. */
+int orc_search(const hcx_params *p, const orc_index *idx, int nidx,
+               const hcx_queries_view *q, hcx_result *res, uint32_t *hist,
+               hcx_stats *stats, int threads);
+
+/* Candidate dump: every inserted candidate (shpk >= min_shp, hs > 0) of
+ * spectrum s, in the reference insertion order (length, psid). */
+int orc_candidates(const hcx_params *p, const orc_index *idx, int nidx,
+                   const hcx_queries_view *q, int32_t s,
+                   hcx_cand *out, uint64_t cap, uint64_t *n);
+
+/* Work counters only (window searches + bA probes + lower/upper bounds). */
+int orc_count_work(const hcx_params *p, const orc_index *idx, int nidx,
+                   const hcx_queries_view *q, hcx_stats *stats, int threads);
+
+/* expeRT.cpp:572-718 on one histogram (hbins doubles of counts). */
+int orc_tailfit(const double *survival, uint32_t hbins, uint32_t cpsms, int32_t maxhyp,
+                int32_t *mu, int32_t *beta, int32_t *minhyp, int32_t *nexthyp,
+                uint32_t *flags);
+
+/* dslim.cpp:422-428: heap capacity 2^floor(log2(t)+0.999)-1 (t=0 -> 1). */
+int orc_heap_capacity(uint32_t topmatches);
+
+/* minheap.h:86-277 exposed for the heap pin tests: replays hs[] (in order)
+ * into a heap of capacity cap and returns the getMax() index into hs[]. */
+int orc_heap_replay(const float *hs, uint32_t n, int32_t cap, int32_t *argmax,
+                    int32_t *final_size);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

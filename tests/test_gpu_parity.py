@@ -1,0 +1,218 @@
+"""HIP path vs the CPU oracle on the same seeded inputs (bit-exact counts,
+candidate sets, top hits, histograms; hyperscore <= 1e-5 rel; e-value <= 1e-3 rel)."""
+import sys
+
+import numpy as np
+import pytest
+
+from hicops_amd import native as N
+from hicops_amd import workload as W
+from hicops_amd.search import Engine
+from oracle import binding as O
+
+pytestmark = pytest.mark.gpu
+
+CHECK_EXACT = ["cpsms", "psid", "idxoffset", "sharedions", "totalions", "bc", "yc", "maxhypscore",
+               "mu", "beta", "minhypscore", "nexthypscore"]
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+class TwinDB:
+    """A database whose every sequence appears twice: every top hit is tied."""
+
+    def __init__(self, db):
+        self.lengths = db.lengths
+        self.tables = {}
+        for L in db.lengths:
+            seqs, ents = db.seqs(L), db.entries(L)
+            ns = len(seqs) // L
+            e2 = ents.copy()
+            e2["seq_id"] += ns
+            allents = np.concatenate([ents, e2])
+            order = np.argsort(allents["mass"], kind="stable")
+            self.tables[L] = (np.concatenate([seqs, seqs]), np.ascontiguousarray(allents[order]))
+
+    def seqs(self, L):
+        return self.tables[L][0]
+
+    def entries(self, L):
+        return self.tables[L][1]
+
+
+def compare(res_g, hist_g, st_g, res_o, hist_o, st_o, label):
+    valid = (res_o["flags"] & N.RES_VALID) != 0
+    # every flag the oracle sets (the GPU adds HCX_RES_TIE_REPLAYED on its own)
+    assert np.array_equal(res_g["flags"] & 0x3B, res_o["flags"] & 0x3B), label
+    assert np.array_equal(res_g["cpsms"], res_o["cpsms"]), label
+    for f in CHECK_EXACT:
+        g, o = res_g[f][valid], res_o[f][valid]
+        bad = np.nonzero(g != o)[0]
+        assert len(bad) == 0, (label, f, bad[:5], g[bad[:5]], o[bad[:5]])
+    hs_g, hs_o = res_g["hyperscore"][valid], res_o["hyperscore"][valid]
+    assert np.all(np.abs(hs_g - hs_o) <= 1e-5 * np.abs(hs_o)), label
+    e_g, e_o = res_g["e_x"][valid], res_o["e_x"][valid]
+    assert np.all(np.abs(e_g - e_o) <= 1e-3 * np.abs(e_o)), label
+    assert np.array_equal(hist_g, hist_o), label
+    assert np.array_equal(st_g["n_ia"], st_o["n_ia"]) and np.array_equal(st_g["n_probe"], st_o["n_probe"]), label
+    return int(valid.sum())
+
+
+@pytest.fixture(scope="module")
+def small_case():
+    p = W.scaled("small", npep=4000, nspec=300, dM=100.0)
+    db = W.Database(p)
+    q = W.make_queries(db, p)
+    prm = W.make_params(p)
+    return p, db, q, prm
+
+
+def test_index_parity_small(small_case):
+    p, db, q, prm = small_case
+    log("test_index_parity_small: building on GPU")
+    eng = Engine(prm)
+    for L in db.lengths:
+        ix = eng.construct(L, db.seqs(L), db.entries(L))
+        iA, bA = ix.export()
+        oiA, obA = O.index_build(prm, L, db.seqs(L), db.entries(L))
+        assert np.array_equal(bA, obA), L
+        assert np.array_equal(iA, oiA), L
+    eng.close()
+
+
+@pytest.mark.parametrize("maxz", [1, 2, 3])
+def test_index_parity_mods(maxz):
+    p = W.scaled("rank_share", npep=5000, nspec=10, maxz=maxz, min_len=7, max_len=25)
+    db = W.Database(p)
+    prm = W.make_params(p)
+    log(f"test_index_parity_mods maxz={maxz}")
+    eng = Engine(prm)
+    for L in db.lengths:
+        ix = eng.construct(L, db.seqs(L), db.entries(L))
+        iA, bA = ix.export()
+        oiA, obA = O.index_build(prm, L, db.seqs(L), db.entries(L))
+        assert np.array_equal(bA, obA), L
+        assert np.array_equal(iA, oiA), L
+    eng.close()
+
+
+def test_query_parity_small(small_case):
+    p, db, q, prm = small_case
+    log("test_query_parity_small")
+    odb = O.OracleDB(prm, db)
+    res_o, hist_o, st_o = odb.search(q)
+    eng = Engine(prm)
+    eng.construct_db(db)
+    b = eng.upload(q)
+    out = eng.query_spectrum(b)
+    n = compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, "small")
+    assert n > 100
+    eng.close()
+
+
+@pytest.mark.parametrize("tile", [64, 256, 4096])
+def test_query_parity_tiles(small_case, tile):
+    p, db, q, prm = small_case
+    log(f"test_query_parity_tiles tile={tile}")
+    odb = O.OracleDB(prm, db)
+    res_o, hist_o, st_o = odb.search(q)
+    eng = Engine(prm)
+    eng.construct_db(db)
+    out = eng.query_spectrum(eng.upload(q), tile=tile)
+    compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"tile{tile}")
+    eng.close()
+
+
+def test_candidate_sets_bitexact(small_case):
+    p, db, q, prm = small_case
+    log("test_candidate_sets_bitexact")
+    odb = O.OracleDB(prm, db)
+    eng = Engine(prm)
+    eng.construct_db(db)
+    b = eng.upload(q)
+    g = eng.candidates(b)
+    g = g[np.lexsort((g["psid"], g["idxoffset"], g["spec"]))]
+    o = np.concatenate([odb.candidates(q, s) for s in range(q.nspec)])
+    assert len(g) == len(o) and len(o) > 1000
+    for f in ["spec", "idxoffset", "psid", "bc", "yc", "ibc", "iyc"]:
+        assert np.array_equal(g[f], o[f]), f
+    assert np.all(np.abs(g["hyperscore"] - o["hyperscore"]) <= 1e-5 * np.abs(o["hyperscore"]))
+    eng.close()
+
+
+def test_ties_replayed_in_reference_order(small_case):
+    p, db, q, prm = small_case
+    log("test_ties_replayed_in_reference_order")
+    twin = TwinDB(db)
+    odb = O.OracleDB(prm, twin)
+    res_o, hist_o, st_o = odb.search(q)
+    eng = Engine(prm)
+    eng.construct_db(twin)
+    out = eng.query_spectrum(eng.upload(q))
+    valid = (res_o["flags"] & N.RES_VALID) != 0
+    assert np.all((out.results["flags"][valid] & N.RES_TIE_REPLAYED) != 0)
+    n = compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, "twin")
+    assert n > 100
+    eng.close()
+
+
+@pytest.mark.parametrize("maxz", [1, 2])
+def test_query_parity_open_search_mods(maxz):
+    p = W.scaled("rank_share", npep=30000, nspec=200, maxz=maxz)
+    db = W.Database(p)
+    q = W.make_queries(db, p)
+    prm = W.make_params(p)
+    log(f"test_query_parity_open_search_mods maxz={maxz}: {db.nforms} forms")
+    odb = O.OracleDB(prm, db)
+    res_o, hist_o, st_o = odb.search(q)
+    eng = Engine(prm)
+    eng.construct_db(db)
+    out = eng.query_spectrum(eng.upload(q))
+    n = compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"open{maxz}")
+    assert n > 150
+    eng.close()
+
+
+def test_tailfit_batch_hand_traced():
+    prm = W.make_params(W.PRESETS["small"])
+    eng = Engine(prm)
+    H = np.zeros((4, 1002), np.uint32)
+    H[0, 2:10] = [5, 10, 8, 6, 3, 1, 0, 1]
+    H[0, 10] = 1
+    H[1, 5], H[1, 8] = 4, 1
+    H[2, 7], H[2, 8] = 3, 1
+    H[3, 4], H[3, 6], H[3, 10] = 2, 1, 1
+    cps = np.array([35, 5, 4, 4], np.uint32)
+    hyp = np.array([10, 8, 8, 10], np.int32)
+    out = eng.model_tail_fit(H, cps, hyp)
+    for i in range(4):
+        mu, beta, mn, nx, fl = O.tailfit(H[i].astype(np.float64), int(cps[i]), int(hyp[i]))
+        assert (out["mu"][i], out["beta"][i], out["minhyp"][i], out["nexthyp"][i]) == (mu, beta, mn, nx), i
+    eng.close()
+
+
+def test_tailfit_batch_random_histograms():
+    rng = np.random.default_rng(11)
+    prm = W.make_params(W.PRESETS["small"])
+    eng = Engine(prm)
+    n = 2000
+    H = np.zeros((n, 1002), np.uint32)
+    hyp = rng.integers(20, 900, n).astype(np.int32)
+    for i in range(n):
+        k = int(rng.integers(3, 400))
+        centre = rng.uniform(5, hyp[i] * 0.6)
+        b = np.clip(rng.normal(centre, rng.uniform(3, 40), k), 0, hyp[i] - 1).astype(int)
+        np.add.at(H[i], b, 1)
+        H[i, hyp[i]] += 1
+    cps = H.sum(axis=1).astype(np.uint32)
+    out = eng.model_tail_fit(H, cps, hyp)
+    mism = 0
+    for i in range(n):
+        mu, beta, mn, nx, fl = O.tailfit(H[i].astype(np.float64), int(cps[i]), int(hyp[i]))
+        if (out["mu"][i], out["beta"][i], out["minhyp"][i], out["nexthyp"][i]) != (mu, beta, mn, nx):
+            mism += 1
+    assert mism == 0
+    eng.close()

@@ -1,0 +1,324 @@
+"""Pins of the CPU oracle (oracle/hcx_oracle.c) against what mathematics and
+published chemistry fix, plus hand-traced runs of the reference procedures.
+
+The reference cannot be compiled here (its headers are generated by CMake),
+so each oracle function is checked by something independent of its own code:
+closed forms, textbook ion masses, invariants, brute force on tiny inputs,
+and known answers traced by hand through the reference source.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from hicops_amd import native as N
+from hicops_amd import workload as W
+from oracle import binding as O
+
+# published monoisotopic residue masses (Da), carbamidomethyl C
+PUB = {"A": 71.03711, "C": 103.00919 + 57.02146, "D": 115.02694, "E": 129.04259, "F": 147.06841,
+       "G": 57.02146, "H": 137.05891, "I": 113.08406, "K": 128.09496, "L": 113.08406, "M": 131.04049,
+       "N": 114.04293, "P": 97.05276, "Q": 128.05858, "R": 156.10111, "S": 87.03203, "T": 101.04768,
+       "V": 99.06841, "W": 186.07931, "Y": 163.06333}
+H2O = 18.010565
+PROTON = 1.007276
+
+
+def params(maxz=1, dF=2, dM=10.0, mods=()):
+    p = W.make_params(W.scaled("small", maxz=maxz, dF=dF, dM=dM, mods=list(mods)))
+    return p
+
+
+def test_lgfact_closed_form():
+    v = O.lgfact(81)
+    for n in range(81):
+        ref = math.lgamma(n + 1) / math.log(10)
+        assert abs(v[n] - ref) <= 1e-12 * max(1.0, ref), n
+    assert v[0] == 0.0 and v[1] == 0.0
+
+
+def test_residue_table_vs_published():
+    for aa, m in PUB.items():
+        got = O.lib().orc_aa_mass(aa.encode()) + O.lib().orc_stat_mod(aa.encode())
+        assert abs(got - m) < 0.005, aa
+
+
+def test_peptide_mass_vs_published():
+    # PEPTIDE: 799.3600 Da neutral monoisotopic
+    assert abs(O.pep_mass("PEPTIDE") - 799.35994) < 0.02
+    for seq in ["SAMPLER", "GGGGGGGK", "WYWYWYWR", "CCDDEEFFK", "ACDEFGHIKLMNPQRSTVWY"]:
+        pub = sum(PUB[c] for c in seq) + H2O
+        assert abs(O.pep_mass(seq) - pub) < 0.02 + 0.001 * len(seq), seq
+
+
+def test_peptide_mass_is_float_accumulation():
+    # slm_dsts/utils types: float_t mass accumulated residue by residue
+    seq = "ACDEFGHIKLMNPQRSTVWY"
+    m = np.float32(18.015)
+    for c in seq:
+        m = np.float32(m + np.float32(O.lib().orc_aa_mass(c.encode())))
+        m = np.float32(m + np.float32(O.lib().orc_stat_mod(c.encode())))
+    assert O.pep_mass(seq) == m
+
+
+def textbook_ions(seq, z, mods=None):
+    """b_i and y_i (i = 1..L-1) m/z at charge z from published masses."""
+    mods = mods or {}
+    res = [PUB[c] + mods.get(i, 0.0) for i, c in enumerate(seq)]
+    L = len(seq)
+    b = [(sum(res[:i]) + z * PROTON) / z for i in range(1, L)]
+    y = [(sum(res[L - i:]) + H2O + z * PROTON) / z for i in range(1, L)]
+    return np.array(b), np.array(y)
+
+
+@pytest.mark.parametrize("seq", ["PEPTIDE", "SAMPLERK", "ACDEFGHIKLMNPQRSTVWYK", "MMGGMR"])
+@pytest.mark.parametrize("maxz", [1, 2, 3])
+def test_fragment_bins_vs_textbook(seq, maxz):
+    p = params(maxz=maxz)
+    _, bins = O.gen_spectrum(seq, p)
+    L = len(seq)
+    for z in range(1, maxz + 1):
+        b, y = textbook_ions(seq, z)
+        ob = bins[(z - 1) * (L - 1): z * (L - 1)].astype(np.int64)
+        oy = bins[maxz * (L - 1) + (z - 1) * (L - 1): maxz * (L - 1) + z * (L - 1)].astype(np.int64)
+        tb, ty = np.floor(b * 100).astype(np.int64), np.floor(y * 100).astype(np.int64)
+        # per-residue truncation loses < 1 bin per term; table precision < 0.005 Da/residue
+        assert np.all(ob <= tb + 2 + L // 2) and np.all(ob >= tb - L - 2), (z, ob - tb)
+        assert np.all(oy <= ty + 2 + L // 2) and np.all(oy >= ty - L - 2), (z, oy - ty)
+
+
+def test_fragment_complementarity():
+    p = params(maxz=1)
+    seq = "LLGNVLVCVLAHHFGK"
+    m, bins = O.gen_spectrum(seq, p)
+    L = len(seq)
+    b, y = bins[:L - 1].astype(np.int64), bins[L - 1:].astype(np.int64)
+    # b_i + y_{L-i} = M + 2 protons (in bins), up to truncation
+    total = (m + 2 * PROTON) * 100
+    for i in range(1, L):
+        s = b[i - 1] + y[L - i - 1]
+        assert total - 2 * L - 2 <= s <= total + 2, i
+
+
+def test_fragment_bins_hand_computed():
+    # "GK", maxz 1 (utils.cpp:257-259):
+    #   b1 = (uint)((float(57.02146) + 1.00727647) * 100) = 5802
+    #   y1 = (uint)(((float(128.094) + 1.00727647) + float(18.015)) * 100) = 14711
+    p = params(maxz=1)
+    _, bins = O.gen_spectrum("GK", p)
+    assert list(bins) == [5802, 14711]
+
+
+def test_mod_spectrum_shift():
+    # M-oxidation (+15.9949 -> 1599 after (uint)(Da*scale)) at residue 2 of MMGGMR:
+    # b ions from residue 2 on and y ions containing residue 2 shift by 1599/z bins
+    p = params(maxz=2, mods=[("M", 15.9949, 2)])
+    seq = "MMGGMR"
+    L = len(seq)
+    _, base = O.gen_spectrum(seq, p)
+    m, mod = O.gen_spectrum(seq, p, sites=1 << 1, mod_num=0x1)
+    assert abs(m - (O.pep_mass(seq) + 15.99)) < 1e-3
+    for z in (1, 2):
+        for i in range(1, L):          # b_i covers residues [0, i)
+            k = (z - 1) * (L - 1) + i - 1
+            shift = 1599 // z if i >= 2 else 0
+            assert abs(int(mod[k]) - int(base[k]) - shift) <= L, ("b", z, i)
+            ky = 2 * (L - 1) + (z - 1) * (L - 1) + i - 1   # y_i covers [L-i, L)
+            shift = 1599 // z if L - i <= 1 else 0
+            assert abs(int(mod[ky]) - int(base[ky]) - shift) <= L, ("y", z, i)
+
+
+def test_mod_mass_first_mod_only():
+    # utils.cpp:350-358 adds only the first mod nibble (Q4)
+    p = params(mods=[("M", 15.9949, 2)])
+    one = O.mod_mass("MMGGMR", 0x1, p)
+    two = O.mod_mass("MMGGMR", 0x11, p)
+    assert one == two
+    assert abs(one - (O.pep_mass("MMGGMR") + 15.99)) < 1e-3
+
+
+def test_index_invariants_and_bruteforce_sort():
+    p = W.scaled("small", npep=300, min_len=7, max_len=9)
+    db = W.Database(p)
+    prm = W.make_params(p)
+    NB = prm.max_mass * prm.scale
+    for L in db.lengths:
+        ents = db.entries(L)
+        iA, bA = O.index_build(prm, L, db.seqs(L), ents)
+        speclen = 2 * prm.maxz * (L - 1)
+        assert bA[0] == 0 and np.all(np.diff(bA.astype(np.int64)) >= 0)
+        assert bA[NB] == len(ents) * speclen
+        assert np.array_equal(np.sort(iA), np.arange(len(ents) * speclen))
+        # brute force: every ion's bin from its own spectrum, stable sort by bin
+        allbins = np.concatenate([O.gen_spectrum(db.sequence(L, k), prm)[1] for k in range(len(ents))])
+        allbins = np.minimum(allbins, NB - 1)
+        order = np.lexsort((np.arange(len(allbins)), allbins))
+        assert np.array_equal(iA, order.astype(np.uint32))
+        counts = np.bincount(allbins, minlength=NB)
+        assert np.array_equal(np.diff(bA.astype(np.int64)), counts)
+
+
+def test_window_bruteforce():
+    rng = np.random.default_rng(7)
+    for trial in range(200):
+        n = int(rng.integers(1, 60))
+        mass = np.sort(rng.choice(np.arange(500, 700, 0.5, dtype=np.float32), n)).astype(np.float32)
+        prec = np.float32(rng.uniform(480, 720))
+        dM = float(rng.choice([0.0, 0.5, 3.0, 10.0, 100.0]))
+        ok, lo, hi = O.window(mass, float(prec), dM)
+        p1 = np.float32(float(prec) - dM)
+        p2 = np.float32(float(prec) + dM)
+        inside = np.nonzero((mass >= p1) & (mass <= p2))[0]
+        if len(inside) == 0:
+            assert not ok or hi < lo
+        else:
+            assert ok and lo == inside[0] and hi == inside[-1], (trial, lo, hi, inside)
+    ok, _, _ = O.window(np.array([600.0], np.float32), 600.0, -1.0)
+    assert not ok
+
+
+def bruteforce_candidates(db, prm, q, s, lengths):
+    """Direct comparison of each candidate's ion bins with the query peaks."""
+    NB = prm.max_mass * prm.scale
+    lgf = [math.lgamma(n + 1) / math.log(10) for n in range(prm.max_shdpeaks + 1)]
+    moz = q.moz[q.idx[s]:q.idx[s + 1]].astype(np.int64)
+    inten = q.intensity[q.idx[s]:q.idx[s + 1]].astype(np.int64)
+    valid = (moz > prm.dF) & (moz < NB - 1 - prm.dF)
+    moz, inten = moz[valid], inten[valid]
+    out = []
+    p1 = np.float32(float(q.precurse[s]) - prm.dM)
+    p2 = np.float32(float(q.precurse[s]) + prm.dM)
+    for ixx, L in enumerate(lengths):
+        ents = db.entries(L)
+        for psid in np.nonzero((ents["mass"] >= p1) & (ents["mass"] <= p2))[0]:
+            e = ents[psid]
+            m, bins = O.gen_spectrum(db.sequence(L, psid), prm, int(e["sites"]), int(e["mod_num"]))
+            if not (prm.min_mass <= m <= prm.max_mass):
+                bins = np.zeros_like(bins)
+            bins = np.minimum(bins.astype(np.int64), NB - 1)
+            half = len(bins) // 2
+            hit = np.abs(bins[:, None] - moz[None, :]) <= prm.dF
+            bc, yc = int(hit[:half].sum()), int(hit[half:].sum())
+            ibc = int((hit[:half] * inten[None, :]).sum())
+            iyc = int((hit[half:] * inten[None, :]).sum())
+            if bc + yc < prm.min_shp:
+                continue
+            hs = lgf[bc] + lgf[yc] + math.log10(1 + ibc) + math.log10(1 + iyc) - 6
+            if hs > 0:
+                out.append((ixx, int(psid), bc, yc, ibc, iyc, hs))
+    return out
+
+
+@pytest.mark.parametrize("maxz,mods", [(2, []), (1, [("M", 15.9949, 2)])])
+def test_shared_peaks_bruteforce(maxz, mods):
+    p = W.scaled("small", npep=500, min_len=7, max_len=12, maxz=maxz, nspec=40, dM=150.0,
+                 mods=mods, max_mods=2 if mods else 0)
+    db = W.Database(p)
+    q = W.make_queries(db, p)
+    prm = W.make_params(p)
+    odb = O.OracleDB(prm, db)
+    seen = 0
+    for s in range(q.nspec):
+        bf = bruteforce_candidates(db, prm, q, s, odb.lengths)
+        oc = odb.candidates(q, s)
+        got = [(int(c["idxoffset"]), int(c["psid"]), int(c["bc"]), int(c["yc"]), int(c["ibc"]), int(c["iyc"]))
+               for c in oc]
+        assert got == [t[:6] for t in bf], s
+        for c, t in zip(oc, bf):
+            assert abs(float(c["hyperscore"]) - t[6]) <= 1e-5 * abs(t[6])
+        seen += len(bf)
+    assert seen > 50
+
+
+def test_heap_capacity_hand_values():
+    # 2^floor(log2(t)+0.999)-1 (dslim.cpp:422-428)
+    assert [O.heap_capacity(t) for t in (0, 1, 2, 3, 4, 5, 8, 10, 16, 17)] == [0, 0, 1, 3, 3, 7, 7, 15, 15, 31]
+
+
+@pytest.mark.parametrize("seq,cap,expect", [
+    # traced by hand through minheap.h insert / heapify / increase_key / getMax
+    ([5, 3, 3], 3, 0),        # [3a,5,3b] -> leaves {5,3b}: 5
+    ([4, 7, 7], 3, 2),        # [4,7a,7b]: scan from the right keeps 7b
+    ([7, 4, 7], 3, 2),        # [4,7a,7b]
+    ([7, 7, 4], 3, 0),        # heapify swaps 4 to the root: [4,7b,7a] -> 7a
+    ([7, 4, 7, 5], 3, 2),     # full: 5 >= root 4 replaces it: [5,7a,7b] -> 7b
+    ([5, 6, 6, 6], 3, 2),     # tie with the root replaces it: [6c,6a,6b] -> 6b
+    ([5, 6, 6, 6, 6], 3, 2),
+    ([5, 5], 3, 1),           # size 2: getMax returns the last slot
+    ([3, 5], 3, 1),
+    ([5, 3], 3, 0),
+    ([9], 3, 0),
+    ([2, 9, 1], 1, 1),        # capacity 1: 9 replaces 2, 1 < 9 rejected
+])
+def test_heap_hand_traced(seq, cap, expect):
+    am, _ = O.heap_replay(np.array(seq, np.float32), cap)
+    assert am == expect
+
+
+def test_heap_unique_max_is_max():
+    rng = np.random.default_rng(3)
+    for _ in range(300):
+        hs = rng.integers(1, 50, size=int(rng.integers(1, 80))).astype(np.float32)
+        am, _ = O.heap_replay(hs, 15)
+        assert hs[am] == hs.max()
+        if (hs == hs.max()).sum() == 1:
+            assert am == int(np.argmax(hs))
+
+
+def test_tailfit_hand_traced_general():
+    # y[2..9] = 5,10,8,6,3,1,0,1 and the top hit's bin 10 (hyp = 10), cpsms 35.
+    # By hand: end1 = 9, stt1 = 2, sx = 1 - cumsum/35, marks 1 and 6 ->
+    # OLS over x = 3..8, y = log10(sx[1..6]).
+    h = np.zeros(1002)
+    h[2:10] = [5, 10, 8, 6, 3, 1, 0, 1]
+    h[10] = 1
+    mu, beta, mn, nx, fl = O.tailfit(h, 35, 10)
+    assert (mn, nx, fl) == (2, 9, 0)
+    cum = np.cumsum([5, 10, 8, 6, 3, 1, 0, 1])
+    sx = np.log10(1 - cum / 35.0)
+    slope, icpt = np.polyfit(np.arange(3, 9), sx[1:7], 1)
+    assert abs(mu - slope * 1e6) <= 1.0 and abs(beta - icpt * 1e6) <= 1.0
+
+
+def test_tailfit_hand_traced_two_points():
+    # only y[5] = 4 below the top bin 8: stt1 == end1 = 5 -> end1 = 6, sx = [0.2, 0.2]
+    h = np.zeros(1002)
+    h[5], h[8] = 4, 1
+    mu, beta, mn, nx, fl = O.tailfit(h, 5, 8)
+    assert (mu, mn, nx, fl) == (0, 5, 6, 0)
+    assert beta == int(math.log10(0.2) * 1e6)
+
+
+def test_tailfit_hand_traced_zero_replacement():
+    # y[7] = 3, top bin 8: slice [7, 8] -> sx = [0.25, 0] -> 0 replaced by sx[0]
+    h = np.zeros(1002)
+    h[7], h[8] = 3, 1
+    mu, beta, mn, nx, fl = O.tailfit(h, 4, 8)
+    assert (mu, beta, mn, nx) == (0, int(math.log10(0.25) * 1e6), 7, 8)
+
+
+def test_tailfit_hand_traced_three_points():
+    # y[4] = 2, y[6] = 1, top bin 10, cpsms 4: n = 3, sx = [0.5, 0.5, 0.25],
+    # mark = 1, mark2 = 2 -> line through (5, log10 .5) and (6, log10 .25)
+    h = np.zeros(1002)
+    h[4], h[6], h[10] = 2, 1, 1
+    mu, beta, mn, nx, fl = O.tailfit(h, 4, 10)
+    slope = math.log10(0.25) - math.log10(0.5)
+    icpt = math.log10(0.5) - slope * 5
+    assert (mn, nx) == (4, 6)
+    assert mu == int(slope * 1e6) and abs(beta - icpt * 1e6) <= 1.0
+
+
+def test_evalue_closed_form(small_db):
+    p, db, q = small_db
+    prm = W.make_params(p)
+    odb = O.OracleDB(prm, db)
+    res, hist, _ = odb.search(q)
+    ok = (res["flags"] & N.RES_VALID) != 0
+    assert ok.sum() > 20
+    r = res[ok]
+    e = 10.0 ** ((r["mu"] / 1e6) * r["maxhypscore"] + r["beta"] / 1e6) * r["cpsms"]
+    assert np.allclose(r["e_x"], e, rtol=1e-12)
+    # the histogram holds every candidate PSM once
+    assert np.array_equal(hist.sum(axis=1), res["cpsms"])
+    assert np.array_equal(r["maxhypscore"], (np.float64(r["hyperscore"] * np.float32(10)) + 0.5).astype(np.int32))
