@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Benchmark: open search (+-500 Da) over a 10M-peptide index on one MI355X.
+
+One step = one full search of the query batch (scan + scoring + top hit +
+tail-fit e-values) with the index and spectra already resident in HBM.
+Prints one JSON line (see the fields below).
+
+Algorithmic bytes of one search (query kernel):
+    B_q = sum over spectra of 4*N_iA + 8*N_probe + 8*P + 8
+N_iA: iA entries inside the precursor windows, N_probe: (length, peak, bin)
+probes reading bA[bin], bA[bin+1], P: peaks (m/z + intensity), 8: the top hit.
+roofline.achieved = B_q / (average query-kernel time, HIP events).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "spectra/s/GPU open search 10M peptides, plain vs batched+surrogate; % HBM BW"
+HBM_PEAK = 8.0e12
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--preset", default="open10m")
+    ap.add_argument("--npep", type=int, default=0)
+    ap.add_argument("--nspec", type=int, default=0)
+    ap.add_argument("--kernel", default="plain", choices=["plain", "batched", "both"])
+    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}), flush=True)
+        return 2
+
+    from hicops_amd import native as N
+    from hicops_amd import workload as W
+    from hicops_amd.search import Engine
+
+    kw = {}
+    if args.npep:
+        kw["npep"] = args.npep
+    if args.nspec:
+        kw["nspec"] = args.nspec
+    p = W.scaled(args.preset, **kw)
+    t0 = time.perf_counter()
+    db = W.Database(p)
+    log(f"database: {db.nforms} forms over {len(db.lengths)} lengths ({time.perf_counter() - t0:.1f}s)")
+    t0 = time.perf_counter()
+    q = W.make_queries(db, p)
+    npeaks = int(q.idx[-1])
+    log(f"queries: {q.nspec} spectra, {npeaks} peaks ({time.perf_counter() - t0:.1f}s)")
+    prm = W.make_params(p)
+
+    eng = Engine(prm)
+    t0 = time.perf_counter()
+    idx = eng.construct_db(db)
+    build_wall = time.perf_counter() - t0
+    build_ms = sum(ix.build_ms for ix in idx)
+    nions = sum(ix.nions for ix in idx)
+    b_build = 4 * nions + 4 * (eng.NB + 1) * len(idx)
+    log(f"index: {nions} ions, device build {build_ms:.1f} ms, wall {build_wall:.1f}s")
+    batch = eng.upload(q)
+
+    forms = ["plain", "batched"] if args.kernel == "both" else [args.kernel]
+    kcode = {"plain": N.KERNEL_PLAIN, "batched": N.KERNEL_BATCHED}
+    runs = {}
+    for form in forms:
+        opts = dict(kernel=kcode[form], tile=args.tile)
+        for i in range(args.warmup):
+            k, t = eng.query_device(batch, **opts)
+            log(f"{form} warmup {i}: query kernel {k:.1f} ms, total {t:.1f} ms")
+        kms, tms = [], []
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            k, t = eng.query_device(batch, **opts)
+            kms.append(k)
+            tms.append(t)
+            log(f"{form} step {i}: query kernel {k:.1f} ms, total {t:.1f} ms")
+        wall = time.perf_counter() - t0
+        out = eng.fetch(batch, want_hist=False, want_stats=True)
+        runs[form] = dict(wall=wall, kms=float(np.mean(kms)), tms=float(np.mean(tms)), out=out)
+
+    head = runs[forms[-1]]
+    st = head["out"].stats
+    n_ia, n_probe = int(st["n_ia"].sum()), int(st["n_probe"].sum())
+    b_q = 4 * n_ia + 8 * n_probe + 8 * npeaks + 8 * q.nspec
+    ms_per_step = head["wall"] / args.steps * 1e3
+    value = q.nspec * args.steps / head["wall"]
+    achieved = b_q / (head["kms"] / 1e3)
+    res = head["out"].results
+    valid = int(((res["flags"] & N.RES_VALID) != 0).sum())
+
+    traffic = None
+    tfile = ROOT / "profiles" / f"traffic_{args.preset}_{forms[-1]}.json"
+    if tfile.exists() and not kw:
+        try:
+            traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
+        except Exception:  # noqa: BLE001
+            traffic = None
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(prm, db, idx, q, st, args.cpu_seconds)
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "spectra/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"open search +-{p.dM:g} Da, {db.nforms} peptides (lengths {p.min_len}-{p.max_len}, "
+                         f"fragment charge 1-{p.maxz}), {q.nspec} spectra ({p.peaks_min}-{p.peaks_max} peaks)"),
+            "preset": p.name, "peptides": db.nforms, "spectra": q.nspec, "ions": nions,
+            "form": forms[-1], "tile": args.tile,
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
+            "algorithmic_bytes": b_q, "kernel_ms": round(head["kms"], 3),
+        },
+        "cpu_baseline": cpu,
+        "forms": {f: {"spectra_per_s": round(q.nspec * args.steps / r["wall"], 2),
+                      "query_kernel_ms": round(r["kms"], 3), "search_ms": round(r["tms"], 3)}
+                  for f, r in runs.items()},
+        "index_build": {"ions": nions, "device_ms": round(build_ms, 2),
+                        "ions_per_s": round(nions / (build_ms / 1e3), 1),
+                        "frac_hbm": round(b_build / (build_ms / 1e3) / HBM_PEAK, 4)},
+        "psms_valid": valid,
+        "work": {"n_ia": n_ia, "n_probe": n_probe, "peaks": npeaks},
+    }
+    print(json.dumps(line), flush=True)
+    eng.close()
+    return 0
+
+
+def cpu_baseline(prm, db, idx, q, gpu_stats, seconds: float):
+    """The oracle (a C restatement of the reference loop, OpenMP over spectra)
+    on a bounded sample of the same spectra, against the same index."""
+    from oracle import binding as O
+
+    threads = len(os.sched_getaffinity(0))
+    lengths = db.lengths
+    orc = O.OracleDB.__new__(O.OracleDB)
+    orc.params, orc.lengths, orc.keep = prm, lengths, []
+    orc.idx = (O.OrcIndex * len(lengths))()
+    for i, (L, ix) in enumerate(zip(lengths, idx)):
+        iA, bA = ix.export()               # bit-identical to the oracle's own build (tests)
+        mass = np.ascontiguousarray(db.entries(L)["mass"])
+        orc.keep += [iA, bA, mass]
+        orc.idx[i] = O.OrcIndex(L, len(mass), mass.ctypes.data, iA.ctypes.data, bA.ctypes.data)
+    rng = np.random.default_rng(1)
+    order = rng.permutation(q.nspec)
+    probe = q.subset(np.sort(order[:64]))
+    t0 = time.perf_counter()
+    orc.search(probe, threads)
+    rate = probe.nspec / max(time.perf_counter() - t0, 1e-6)
+    n = int(min(q.nspec, max(64, rate * seconds)))
+    sel = np.sort(order[:n])
+    sample = q.subset(sel)
+    t0 = time.perf_counter()
+    _, _, st = orc.search(sample, threads)
+    dt = time.perf_counter() - t0
+    match = bool(np.array_equal(st["n_ia"], gpu_stats["n_ia"][sel]) and
+                 np.array_equal(st["n_probe"], gpu_stats["n_probe"][sel]))
+    log(f"cpu baseline: {n} spectra in {dt:.1f}s on {threads} threads; work counts match GPU: {match}")
+    return {"value": round(n / dt, 2), "unit": "spectra/s", "cores": threads, "kind": "port",
+            "sample": f"{n} random spectra of the same batch, same index, OpenMP over spectra",
+            "work_counts_match_gpu": match}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -95,9 +95,14 @@ extern "C" int hcx_create(const hcx_params *params, int device, hcx_ctx **out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail("hipGetDeviceProperties");
     c->num_cu = prop.multiProcessorCount;
+    if (hipDeviceGetAttribute(&c->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+        c->max_lds = 65536;
+    // 2 bytes of hit counter per peptide of a chunk; leave room for slots,
+    // peaks and the histogram
+    c->chunk = c->max_lds >= 96 * 1024 ? 16384 : 8192;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail("stream");
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-        hipEventCreate(&c->ev2) != hipSuccess)
+        hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess)
         return fail("events");
     // utils.h:115-131 lgfact table: val[i] = log10(i) + val[i-1]
     c->h_lgf.assign(p.max_shdpeaks + 1, 0.0);
@@ -124,8 +129,38 @@ extern "C" int hcx_destroy(hcx_ctx *c) {
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev2) (void)hipEventDestroy(c->ev2);
+    if (c->ev3) (void)hipEventDestroy(c->ev3);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+    return HCX_SUCCESS;
+}
+
+extern "C" int hcx_set_tuning(hcx_ctx *c, uint32_t chunk, uint32_t slots, uint32_t blocks_per_cu) {
+    if (!c) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_set_tuning: null context");
+    if (chunk) {
+        if (chunk < 256 || chunk > 65536 || (chunk & (chunk - 1)))
+            return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_set_tuning: chunk must be a power of two in [256, 65536]");
+        c->chunk = chunk;
+    }
+    if (slots) {
+        if (slots < 64 || slots > 32768) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_set_tuning: slots in [64, 32768]");
+        c->slots = slots;
+    }
+    if (blocks_per_cu) {
+        if (blocks_per_cu > 16) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_set_tuning: blocks_per_cu <= 16");
+        c->blocks_per_cu = blocks_per_cu;
+    }
+    return HCX_SUCCESS;
+}
+
+extern "C" int hcx_get_tuning(const hcx_ctx *c, uint32_t *chunk, uint32_t *slots, uint32_t *blocks_per_cu,
+                              int32_t *num_cu, int32_t *max_lds) {
+    if (!c) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_get_tuning: null context");
+    if (chunk) *chunk = c->chunk;
+    if (slots) *slots = c->slots;
+    if (blocks_per_cu) *blocks_per_cu = c->blocks_per_cu;
+    if (num_cu) *num_cu = c->num_cu;
+    if (max_lds) *max_lds = c->max_lds;
     return HCX_SUCCESS;
 }
 
@@ -216,6 +251,14 @@ extern "C" int hcx_index_build(hcx_ctx *c, uint32_t peplen, const char *seqs, ui
     int rc = launch_index_build(c, a, (uint32_t *)ix->iA.p, (uint32_t *)ix->bA.p, (const float *)ix->mass.p,
                                 (uint32_t *)ix->mdir.p, p.max_mass);
     if (rc) return fail(rc);
+    // (bin, chunk) refinement of bA for the chunked query
+    ix->chunk = c->chunk;
+    ix->nch = std::max<uint32_t>(1, (nentries + c->chunk - 1) / c->chunk);
+    if (ix->bA2.ensure(4ull * c->NB * (ix->nch + 1)))
+        return fail(set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_index_build: chunk offsets allocation failed"));
+    rc = launch_chunk_offsets(c, (const uint32_t *)ix->iA.p, (const uint32_t *)ix->bA.p, c->NB, ix->nch, ix->chunk,
+                              speclen, (uint32_t *)ix->bA2.p);
+    if (rc) return fail(rc);
     if (hipEventRecord(c->ev1, c->stream) != hipSuccess || hipEventSynchronize(c->ev1) != hipSuccess)
         return fail(set_error("event"));
     float ms = 0.0f;
@@ -274,6 +317,22 @@ extern "C" int hcx_queries_upload(hcx_ctx *c, const hcx_queries_view *q, hcx_qba
         maxp = std::max(maxp, q->idx[s + 1] - q->idx[s]);
     }
     if (n > 0 && q->idx[0] != 0) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_queries_upload: idx[0] != 0");
+    // largest number of one spectrum's peaks inside any 2*dF+1 bins: bounds
+    // how often one fragment bin can be probed
+    uint32_t mult = 0;
+    {
+        std::vector<uint32_t> t;
+        const uint32_t span = 2 * c->params.dF;
+        for (int32_t s = 0; s < n; s++) {
+            t.assign(q->moz + q->idx[s], q->moz + q->idx[s + 1]);
+            std::sort(t.begin(), t.end());
+            size_t i = 0;
+            for (size_t j = 0; j < t.size(); j++) {
+                while (t[j] - t[i] > span) i++;
+                mult = std::max<uint32_t>(mult, (uint32_t)(j - i + 1));
+            }
+        }
+    }
     HCX_HIP(hipSetDevice(c->device));
     hcx_qbatch *b = new (std::nothrow) hcx_qbatch;
     if (!b) return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_queries_upload: out of memory");
@@ -282,6 +341,7 @@ extern "C" int hcx_queries_upload(hcx_ctx *c, const hcx_queries_view *q, hcx_qba
     b->file_num = q->file_num;
     b->npeaks = npk;
     b->max_peaks = maxp;
+    b->max_mult = mult;
     if (b->moz.ensure(4 * npk + 16) || b->inten.ensure(4 * npk + 16) || b->idx.ensure(4ull * (n + 1)) ||
         b->prec.ensure(4ull * n + 16) || b->chg.ensure(4ull * n + 16) || b->rt.ensure(4ull * n + 16)) {
         delete b;
@@ -337,7 +397,12 @@ int prepare(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, const h
         L.half = ix->speclen / 2;
         L.peplen = ix->peplen;
         make_divider(ix->speclen, &L.div_m, &L.div_s);
-        L.pad0 = L.pad1 = 0;
+        L.nch = ix->nch;
+        L.pad0 = 0;
+        L.bA2 = (const uint32_t *)ix->bA2.p;
+        if (ix->chunk != c->chunk)
+            return set_error_code(HCX_ERR_INVLD_PARAM, "search: index built with another chunk size");
+        if (ix->n >= (1u << 25)) return set_error_code(HCX_ERR_INVLD_SIZE, "search: more than 2^25 entries in a length");
     }
     if (c->lens.ensure(sizeof(LenDesc) * lens.size())) return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "search: alloc");
     HCX_HIP(hipMemcpyAsync(c->lens.p, lens.data(), sizeof(LenDesc) * lens.size(), hipMemcpyHostToDevice, c->stream));
@@ -387,30 +452,85 @@ int prepare(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, const h
     return HCX_SUCCESS;
 }
 
+uint64_t items_key(hcx_index *const *idx, int nidx, uint32_t chunk) {
+    uint64_t h = 1469598103934665603ull ^ chunk;
+    for (int i = 0; i < nidx; i++) {
+        h ^= (uint64_t)(uintptr_t)idx[i];
+        h *= 1099511628211ull;
+    }
+    return h ^ (uint64_t)nidx;
+}
+
 int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, const hcx_search_opts *opts,
                bool timed, float *kernel_ms, float *total_ms) {
     HCX_HIP(hipSetDevice(c->device));
-    QueryArgs a;
-    int rc = prepare(c, idx, nidx, qb, opts, &a);
+    ChunkArgs ca;
+    std::memset(&ca, 0, sizeof ca);
+    int rc = prepare(c, idx, nidx, qb, opts, &ca.q);
     if (rc) return rc;
+    QueryArgs &a = ca.q;
     const int kernel = opts ? opts->kernel : HCX_KERNEL_PLAIN;
+    const uint64_t n = (uint64_t)qb->nspec;
+    ca.tc = c->chunk;
+    ca.slots = c->slots;
+    ca.max_peaks = std::max<uint32_t>(1, qb->max_peaks);
+    ca.grid = (uint32_t)std::max(1, c->num_cu) * c->blocks_per_cu;
+    if (chunk_smem_bytes(ca) > (size_t)c->max_lds)
+        return set_error_code(HCX_ERR_INVLD_PARAM, "search: chunk/slots do not fit in LDS");
+    // hits per peptide must fit the 15-bit counters: speclen * peaks within +-dF
+    if (qb->max_mult) {
+        uint32_t maxspec = 0;
+        for (int i = 0; i < nidx; i++) maxspec = std::max(maxspec, idx[i]->speclen);
+        if ((uint64_t)maxspec * qb->max_mult >= 0x8000)
+            return set_error_code(HCX_ERR_INVLD_PARAM, "search: too many peaks within +-dF of one bin");
+    }
+    if (qb->acc.ensure(sizeof(SpecAcc) * n + 16) || c->work.ensure(64) ||
+        qb->counts.ensure(4 * (n + 1) + 16) || c->scan_tmp.ensure(4 * scan_tmp_words(n + 1) + 64))
+        return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "search: scratch allocation failed");
+    ca.acc = (SpecAcc *)qb->acc.p;
+    ca.queue = (uint32_t *)c->work.p;
+    if (kernel == HCX_KERNEL_BATCHED) {
+        const uint64_t key = items_key(idx, nidx, c->chunk);
+        if (qb->items_key != key || !qb->items.p) {
+            rc = launch_items(c, ca, (uint32_t *)qb->counts.p, (uint32_t *)c->scan_tmp.p, false);
+            if (rc) return rc;
+            uint32_t total = 0;
+            HCX_HIP(hipMemcpyAsync(&total, (uint32_t *)qb->counts.p + n, 4, hipMemcpyDeviceToHost, c->stream));
+            HCX_HIP(hipStreamSynchronize(c->stream));
+            if (qb->items.ensure(sizeof(Item) * (size_t)total + 16))
+                return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "search: item list allocation failed");
+            ca.items_w = (Item *)qb->items.p;
+            rc = launch_items(c, ca, (uint32_t *)qb->counts.p, (uint32_t *)c->scan_tmp.p, true);
+            if (rc) return rc;
+            qb->nitems = total;
+            qb->items_key = key;
+        }
+        ca.items = (const Item *)qb->items.p;
+        ca.nitems = qb->nitems;
+    }
     if (timed) HCX_HIP(hipEventRecord(c->ev2, c->stream));
-    k_zero_u32<<<1, 64, 0, c->stream>>>(a.tie, 1);
-    HCX_HIP(hipGetLastError());
-    float kms = 0.0f;
-    rc = launch_query(c, a, kernel, false, &kms);
+    // zero the per-spectrum accumulators, histograms, tie list and queue with
+    // fill kernels (no memset nodes: the sequence may be graph-captured)
+    rc = launch_fill_u32(c, (uint32_t *)qb->acc.p, n * (sizeof(SpecAcc) / 4), 0);
+    if (!rc) rc = launch_fill_u32(c, a.hist, n * c->hbins, 0);
+    if (!rc) rc = launch_fill_u32(c, a.tie, 1, 0);
+    if (!rc) rc = launch_fill_u32(c, ca.queue, 1, 0);
     if (rc) return rc;
-    if (timed) HCX_HIP(hipEventRecord(c->ev0, c->stream));
+    if (kernel == HCX_KERNEL_CURSOR)
+        rc = launch_cursor(c, a);          // writes results + tie list itself
+    else
+        rc = launch_chunked(c, ca, kernel);   // records ev0 / ev1 around the scan kernel
+    if (rc) return rc;
     rc = launch_replay(c, a);
     if (rc) return rc;
     rc = launch_tailfit(c, a.res, a.hist, a.hbins, a.nspec, c->params.min_cpsm, c->params.expect_max);
     if (rc) return rc;
     if (timed) {
-        HCX_HIP(hipEventRecord(c->ev1, c->stream));
-        HCX_HIP(hipEventSynchronize(c->ev1));
+        HCX_HIP(hipEventRecord(c->ev3, c->stream));
+        HCX_HIP(hipEventSynchronize(c->ev3));
         float q = 0.0f, t = 0.0f;
-        HCX_HIP(hipEventElapsedTime(&q, c->ev2, c->ev0));
-        HCX_HIP(hipEventElapsedTime(&t, c->ev2, c->ev1));
+        HCX_HIP(hipEventElapsedTime(&q, c->ev0, c->ev1));
+        HCX_HIP(hipEventElapsedTime(&t, c->ev2, c->ev3));
         if (kernel_ms) *kernel_ms = q;
         if (total_ms) *total_ms = t;
     } else {

@@ -34,7 +34,9 @@ struct LenDesc {
     uint32_t peplen;
     uint32_t div_m;         // magic multiplier for raw / speclen
     uint32_t div_s;         // shift
-    uint32_t pad0, pad1;
+    uint32_t nch;           // peptide chunks (of ctx->chunk peptides)
+    uint32_t pad0;
+    const uint32_t *bA2;    // NB * (nch + 1): start of chunk c inside bin b
 };
 
 struct DeviceBuf {
@@ -51,7 +53,7 @@ struct hcx_ctx {
     hcx_params params;
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     double *d_lgf = nullptr;          // max_shdpeaks + 1
     std::vector<double> h_lgf;
     uint32_t NB = 0;                  // max_mass * scale
@@ -64,7 +66,12 @@ struct hcx_ctx {
     hcx::DeviceBuf work;              // batched kernel queue / counters
     hcx::DeviceBuf sur;               // surrogate parameters + optimizer state
     hcx::DeviceBuf sur_tmp;
+    hcx::DeviceBuf scan_tmp;          // device scan scratch
     bool sur_ready = false;
+    uint32_t chunk = 16384;           // peptides per query chunk (power of two)
+    uint32_t slots = 1024;            // scored-peptide slots per chunk pass
+    uint32_t blocks_per_cu = 2;       // persistent grid of the batched kernel
+    int max_lds = 65536;              // device limit per workgroup
     // last launch shape (for graph reuse)
     hipGraphExec_t graph_exec = nullptr;
     const void *graph_key = nullptr;
@@ -75,7 +82,8 @@ struct hcx_index {
     uint32_t peplen = 0, speclen = 0, n = 0, nseqs = 0;
     uint64_t nions = 0;
     float build_ms = 0.0f;
-    hcx::DeviceBuf entries, mass, iA, bA, mdir, seqs;
+    uint32_t chunk = 0, nch = 0;
+    hcx::DeviceBuf entries, mass, iA, bA, mdir, seqs, bA2;
 };
 
 struct hcx_qbatch {
@@ -83,8 +91,12 @@ struct hcx_qbatch {
     int32_t nspec = 0, file_num = 0;
     uint64_t npeaks = 0;
     uint32_t max_peaks = 0;
+    uint32_t max_mult = 0;   // most peaks of one spectrum within +-dF of a single bin
     hcx::DeviceBuf moz, inten, idx, prec, chg, rt;
     hcx::DeviceBuf res, hist, stats, feat, pred;
+    hcx::DeviceBuf acc, items, counts;   // chunked-search state
+    uint32_t nitems = 0;
+    uint64_t items_key = 0;              // index set + chunk the items were built for
 };
 
 namespace hcx {
@@ -124,8 +136,38 @@ struct QueryArgs {
     int heap_cap;
     uint32_t *queue;      // batched kernel: work counter
 };
-int launch_query(hcx_ctx *ctx, const QueryArgs &a, int kernel, bool timed, float *ms);
+int launch_cursor(hcx_ctx *ctx, const QueryArgs &a);
 int launch_replay(hcx_ctx *ctx, const QueryArgs &a);
+// ---- chunked search (k_chunk.hip) ----
+struct SpecAcc {
+    unsigned long long top_a, top_b, top_n;
+    uint32_t cpsms, flags;
+    unsigned long long n_ia, n_probe;
+};
+struct Item {
+    uint32_t spec;
+    uint16_t ixx;
+    uint16_t chunk;
+    int32_t lo, hi;
+};
+struct ChunkArgs {
+    QueryArgs q;
+    uint32_t tc, slots, max_peaks, grid;
+    SpecAcc *acc;
+    const Item *items;
+    Item *items_w;
+    uint32_t nitems;
+    uint32_t *queue;
+};
+size_t chunk_smem_bytes(const ChunkArgs &a);
+int launch_chunk_offsets(hcx_ctx *ctx, const uint32_t *iA, const uint32_t *bA, uint32_t NB, uint32_t nch, uint32_t TC,
+                         uint32_t speclen, uint32_t *bA2);
+int launch_fill_u32(hcx_ctx *ctx, uint32_t *p, uint64_t n, uint32_t v);
+int launch_items(hcx_ctx *ctx, ChunkArgs &a, uint32_t *counts, uint32_t *scan_tmp, bool fill);
+int launch_chunked(hcx_ctx *ctx, ChunkArgs &a, int kernel);
+int scan_exclusive_u32(hipStream_t st, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *tmp);
+uint64_t scan_tmp_words(uint64_t n);
+
 int launch_tailfit(hcx_ctx *ctx, hcx_result *res, const uint32_t *hist, uint32_t hbins, int32_t n,
                    uint32_t min_cpsm, double expect_max);
 int launch_tailfit_raw(hcx_ctx *ctx, const uint32_t *hist, uint32_t hbins, uint32_t n, const uint32_t *cpsms,

@@ -1,0 +1,758 @@
+// k_chunk.hip -- the fast shared-peak scan over a (bin, peptide-chunk) bucketed index.
+//
+// Index refinement: each length's peptides (sorted by mass) are cut into
+// chunks of TC peptides.  Inside a bin the reference iA is ascending in ion
+// id, hence in peptide id, so the entries of one (bin, chunk) pair are a
+// contiguous run; bA2[bin * (nch+1) + c] is where chunk c starts inside bin
+// `bin` (bA2[..+0] = bA[bin], bA2[..+nch] = bA[bin+1]).  iA itself is the
+// reference array, unchanged.
+//
+// Work item = (spectrum, length, chunk).  For one item:
+//   R   every (peak, bin) probe reads its run bounds [bA2[c], bA2[c+1]) and a
+//       block scan of the run lengths gives each thread an equal contiguous
+//       share of the item's entries;
+//   P1  8 independent iA loads per lane at a time; each entry inside the
+//       precursor window adds 1 to its peptide's 16-bit hit counter (two
+//       counters per LDS word);
+//   Q   peptides with hits >= min_shp get a slot (the counter becomes
+//       0x8000|slot), the others are cleared;
+//   P2  the entries are read again (L2-resident) and only slotted peptides
+//       accumulate exact b/y counts and intensity sums;
+//   S   slots are scored: hs = lgf[bc] + lgf[yc] + log10(1+ibc) + log10(1+iyc) - 6
+//       (dslim_query.cpp:679-722), histogram + top hit of the spectrum.
+// Per-spectrum results from several items merge through global atomics:
+// cpsms, histogram bins, and three 64-bit keys whose high half is the
+// hyperscore's bit pattern (positive floats order like unsigned ints):
+//   top_a = hs | length << 25 | peptide id    (atomicMax)
+//   top_b = hs | bc << 16 | yc                (atomicMax)
+//   top_n = hs | number of peptides at hs     (CAS merge of (max, count))
+// A spectrum whose maximum is shared by more than one peptide goes to the
+// in-order heap replay (k_query.hip), as the reference's getMax tie rules
+// require.
+//
+// Every LDS pointer is address-space-3 qualified and every global pointer on
+// the scan path address-space-1 qualified, so no access compiles to a flat
+// instruction (a flat load counts against both the vector-memory and the LDS
+// wait counters and would serialise the LDS atomics behind HBM reads).
+#include "hcx_internal.h"
+
+namespace hcx {
+namespace {
+
+#define AS_LDS __attribute__((address_space(3)))
+#define AS_GLB __attribute__((address_space(1)))
+typedef AS_LDS uint32_t lu32;
+typedef AS_LDS int32_t li32;
+typedef AS_LDS unsigned long long lu64;
+typedef AS_LDS double lf64;
+typedef const AS_GLB uint32_t gcu32;
+typedef AS_GLB uint32_t gu32;
+typedef AS_GLB unsigned long long gu64;
+
+constexpr int kBT = 512;
+constexpr uint32_t kNoProbe = 0xFFFFFFFFu;
+constexpr uint32_t kGrab = 4;        // items taken per queue atomic
+constexpr unsigned long long kLow25 = (1ull << 25) - 1;
+
+__device__ __forceinline__ uint32_t ldg(const uint32_t *p, size_t i) { return ((gcu32 *)p)[i]; }
+__device__ __forceinline__ void lds_add(lu32 *p, uint32_t v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ uint32_t lds_add_rtn(lu32 *p, uint32_t v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_add64(lu64 *p, unsigned long long v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_and(lu32 *p, uint32_t v) {
+    __hip_atomic_fetch_and(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_max(lu32 *p, uint32_t v) {
+    __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_or(lu32 *p, uint32_t v) {
+    __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <typename T>
+__device__ __forceinline__ void glb_add(T *p, T v) {
+    __hip_atomic_fetch_add((AS_GLB T *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void glb_or(uint32_t *p, uint32_t v) {
+    __hip_atomic_fetch_or((gu32 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void glb_max64(unsigned long long *p, unsigned long long v) {
+    __hip_atomic_fetch_max((gu64 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t udiv(uint32_t n, uint32_t m, uint32_t s) {
+    if (s == 0) return n;
+    uint32_t t = __umulhi(m, n);
+    return (t + ((n - t) >> 1)) >> (s - 1);
+}
+
+__device__ __forceinline__ uint32_t mass_lb(const float *mass, uint32_t lo, uint32_t hi, float v) {
+    while (lo < hi) {
+        uint32_t m = lo + (hi - lo) / 2;
+        if (mass[m] < v) lo = m + 1;
+        else hi = m;
+    }
+    return lo;
+}
+__device__ __forceinline__ uint32_t mass_ub(const float *mass, uint32_t lo, uint32_t hi, float v) {
+    while (lo < hi) {
+        uint32_t m = lo + (hi - lo) / 2;
+        if (v < mass[m]) hi = m;
+        else lo = m + 1;
+    }
+    return lo;
+}
+__device__ __forceinline__ void dir_range(const uint32_t *mdir, uint32_t D, uint32_t n, float v, uint32_t *lo,
+                                          uint32_t *hi) {
+    if (!(v > 0.0f)) {
+        *lo = 0;
+        *hi = mdir[1];
+        return;
+    }
+    uint32_t d = (v >= (float)(D - 1)) ? D - 1 : (uint32_t)v;
+    *lo = mdir[d];
+    *hi = (d + 1 < D) ? mdir[d + 1] : n;
+}
+// dslim_query.cpp:854-916 as lower/upper bounds over float masses
+__device__ __forceinline__ void window(const LenDesc &L, float prec, double dM, uint32_t D, int32_t *wlo,
+                                       int32_t *whi) {
+    *wlo = 1;
+    *whi = 0;
+    if (L.n == 0 || dM < 0.0 || prec != prec) return;
+    const float p1 = (float)((double)prec - dM);
+    const float p2 = (float)((double)prec + dM);
+    uint32_t lo, hi;
+    dir_range(L.mdir, D, L.n, p1, &lo, &hi);
+    const uint32_t a = mass_lb(L.mass, lo, hi, p1);
+    dir_range(L.mdir, D, L.n, p2, &lo, &hi);
+    const uint32_t b = mass_ub(L.mass, lo, hi, p2);
+    *wlo = (int32_t)a;
+    *whi = (int32_t)b - 1;
+}
+
+struct Acc {
+    float best_hs;
+    uint32_t best_cnt;
+    uint32_t best_ixx, best_psid, best_bc, best_yc;
+    uint32_t cpsms, flags;
+    unsigned long long nia, nprobe;
+};
+
+__device__ __forceinline__ void acc_reset(Acc &a) {
+    a.best_hs = 0.0f;
+    a.best_cnt = 0;
+    a.best_ixx = a.best_psid = a.best_bc = a.best_yc = 0;
+    a.cpsms = a.flags = 0;
+    a.nia = a.nprobe = 0;
+}
+
+// Byte offsets of the dynamic LDS arrays (host and device agree on them).
+struct Layout {
+    uint32_t map;       // tc/2 words: two 16-bit counters each
+    uint32_t slot_cnt;  // slots: bc | yc << 16
+    uint32_t slot_sum;  // 2*slots u64: ibc, iyc
+    uint32_t slot_loc;  // slots: local peptide index
+    uint32_t pk_moz, pk_int;   // max_peaks
+    uint32_t prs;       // NP: run start of each probe in iA
+    uint32_t pofs;      // NP + 1: exclusive prefix of the run lengths
+    uint32_t wsum;      // kBT / 64 wave partial sums
+    uint32_t hist;      // hbins
+    uint32_t lgf;       // max_shdpeaks + 1 doubles
+    uint32_t wlo, whi;  // nlen
+    uint32_t total;
+};
+
+__host__ __device__ inline uint32_t al16(uint32_t x) { return (x + 15) & ~15u; }
+
+__host__ __device__ inline Layout make_layout(const ChunkArgs &a) {
+    Layout l;
+    uint32_t off = 0;
+    auto take = [&](uint32_t bytes) {
+        uint32_t o = off;
+        off = al16(off + bytes);
+        return o;
+    };
+    const uint32_t np = a.max_peaks * (2 * a.q.dF + 1);
+    l.map = take(4u * (a.tc / 2));
+    l.slot_cnt = take(4u * a.slots);
+    l.slot_sum = take(16u * a.slots);
+    l.slot_loc = take(4u * a.slots);
+    l.pk_moz = take(4u * a.max_peaks);
+    l.pk_int = take(4u * a.max_peaks);
+    l.prs = take(4u * np);
+    l.pofs = take(4u * (np + 1));
+    l.wsum = take(4u * (kBT / 64 + 1));
+    l.hist = take(4u * a.q.hbins);
+    l.lgf = take(8u * (a.q.max_shdpeaks + 1));
+    l.wlo = take(4u * a.q.nlen);
+    l.whi = take(4u * a.q.nlen);
+    l.total = off;
+    return l;
+}
+
+struct Smem {
+    lu32 *map, *slot_cnt, *slot_loc, *pk_moz, *pk_int, *prs, *pofs, *wsum, *hist;
+    lu64 *slot_sum;
+    lf64 *lgf;
+    li32 *wlo, *whi;
+};
+
+__device__ __forceinline__ Smem bind(unsigned char *raw, const Layout &l) {
+    AS_LDS unsigned char *b = (AS_LDS unsigned char *)raw;
+    Smem s;
+    s.map = (lu32 *)(b + l.map);
+    s.slot_cnt = (lu32 *)(b + l.slot_cnt);
+    s.slot_sum = (lu64 *)(b + l.slot_sum);
+    s.slot_loc = (lu32 *)(b + l.slot_loc);
+    s.pk_moz = (lu32 *)(b + l.pk_moz);
+    s.pk_int = (lu32 *)(b + l.pk_int);
+    s.prs = (lu32 *)(b + l.prs);
+    s.pofs = (lu32 *)(b + l.pofs);
+    s.wsum = (lu32 *)(b + l.wsum);
+    s.hist = (lu32 *)(b + l.hist);
+    s.lgf = (lf64 *)(b + l.lgf);
+    s.wlo = (li32 *)(b + l.wlo);
+    s.whi = (li32 *)(b + l.whi);
+    return s;
+}
+
+// Probe set-up of one item: run start and length of every (peak, bin) probe
+// in chunk c, then an exclusive block scan of the run lengths.  Returns the
+// item's entry total E (sm.pofs[NP] == E).
+__device__ __forceinline__ uint32_t setup_runs(const ChunkArgs &a, const Smem &sm, const LenDesc &L, uint32_t P,
+                                               uint32_t c, bool count_probes, Acc &acc) {
+    const uint32_t W = 2 * a.q.dF + 1;
+    const uint32_t NP = P * W;
+    const uint32_t stride = L.nch + 1;
+    // each thread owns a contiguous group of probes, so its sums feed the scan
+    const uint32_t per = (NP + kBT - 1) / kBT;
+    const uint32_t p0 = threadIdx.x * per;
+    const uint32_t p1 = min(NP, p0 + per);
+    uint32_t tsum = 0;
+    for (uint32_t p = p0; p < p1; p++) {
+        const uint32_t k = p / W, j = p - k * W;
+        const uint32_t q = sm.pk_moz[k];
+        uint32_t rs = 0, len = 0;
+        if (q > a.q.dF && q < a.q.NB - 1 - a.q.dF) {
+            const size_t o = (size_t)(q - a.q.dF + j) * stride + c;
+            rs = ldg(L.bA2, o);
+            len = ldg(L.bA2, o + 1) - rs;
+            if (count_probes) acc.nprobe++;
+        }
+        sm.prs[p] = rs;
+        sm.pofs[p] = len;   // replaced by the prefix below
+        tsum += len;
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = tsum;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) sm.wsum[w] = x;
+    __syncthreads();
+    uint32_t run = x - tsum;
+    for (int ww = 0; ww < w; ww++) run += sm.wsum[ww];
+    for (uint32_t p = p0; p < p1; p++) {
+        const uint32_t len = sm.pofs[p];
+        sm.pofs[p] = run;
+        run += len;
+    }
+    if (threadIdx.x == kBT - 1) sm.pofs[NP] = run;
+    __syncthreads();
+    return sm.pofs[NP];
+}
+
+// Load-balanced walk over the item's E entries: thread t takes the flat range
+// [t*E/kBT, (t+1)*E/kBT), 8 independent iA loads at a time.  fn(raw, probe).
+template <typename F>
+__device__ __forceinline__ void walk_flat(const Smem &sm, const uint32_t *iA_, uint32_t NP, uint32_t E, F &&fn) {
+    const uint32_t f0 = (uint32_t)(((uint64_t)E * threadIdx.x) / kBT);
+    const uint32_t f1 = (uint32_t)(((uint64_t)E * (threadIdx.x + 1)) / kBT);
+    if (f0 >= f1) return;
+    uint32_t lo = 0, hi = NP;   // last p with pofs[p] <= f0
+    while (hi - lo > 1) {
+        const uint32_t m = (lo + hi) >> 1;
+        if (sm.pofs[m] <= f0) lo = m;
+        else hi = m;
+    }
+    // probe p covers flat indices [pbeg, pend) and iA indices [prun, prun + pend - pbeg);
+    // f >= pbeg always, so prun + (f - pbeg) never wraps
+    uint32_t p = lo, pbeg = sm.pofs[p], pend = sm.pofs[p + 1], prun = sm.prs[p];
+    uint32_t f = f0;
+    gcu32 *iA = (gcu32 *)iA_;
+    while (f < f1) {
+#define HCX_GATHER(i)                          \
+    uint32_t r##i = 0, q##i = kNoProbe;        \
+    if (f < f1) {                              \
+        while (f >= pend) {                    \
+            p++;                               \
+            pbeg = pend;                       \
+            pend = sm.pofs[p + 1];             \
+            prun = sm.prs[p];                  \
+        }                                      \
+        r##i = iA[prun + (f - pbeg)];          \
+        q##i = p;                              \
+        f++;                                   \
+    }
+        HCX_GATHER(0) HCX_GATHER(1) HCX_GATHER(2) HCX_GATHER(3)
+        HCX_GATHER(4) HCX_GATHER(5) HCX_GATHER(6) HCX_GATHER(7)
+#undef HCX_GATHER
+        if (q0 != kNoProbe) fn(r0, q0);
+        if (q1 != kNoProbe) fn(r1, q1);
+        if (q2 != kNoProbe) fn(r2, q2);
+        if (q3 != kNoProbe) fn(r3, q3);
+        if (q4 != kNoProbe) fn(r4, q4);
+        if (q5 != kNoProbe) fn(r5, q5);
+        if (q6 != kNoProbe) fn(r6, q6);
+        if (q7 != kNoProbe) fn(r7, q7);
+    }
+}
+
+__device__ __forceinline__ LenDesc load_len(const ChunkArgs &a, int ixx) { return a.q.lens[ixx]; }
+
+// Process one (spectrum, length, chunk) item; peaks are already in LDS.
+__device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm, lu32 *s_ctl, uint32_t P, int ixx,
+                                             uint32_t c, int32_t lo, int32_t hi, Acc &acc) {
+    // s_ctl[0] = slots taken, s_ctl[1] = another round needed
+    const LenDesc L = load_len(a, ixx);
+    const uint32_t TC = a.tc;
+    const uint32_t base = c * TC;
+    const uint32_t ilo = max((uint32_t)lo, base);
+    const uint32_t ihi = min((uint32_t)hi, base + TC - 1);
+    const uint32_t speclen = L.speclen, half = L.half;
+    const bool first = (uint32_t)lo / TC == c;
+    const uint32_t W = 2 * a.q.dF + 1;
+    const uint32_t NP = P * W;
+    const uint32_t E = setup_runs(a, sm, L, P, c, first, acc);
+    // P1: hit counts
+    walk_flat(sm, L.iA, NP, E, [&](uint32_t raw, uint32_t) {
+        const uint32_t ppid = udiv(raw, L.div_m, L.div_s);
+        if (ppid >= ilo && ppid <= ihi) {
+            const uint32_t local = ppid - base;
+            lds_add(&sm.map[local >> 1], 1u << ((local & 1u) << 4));
+            acc.nia++;
+        }
+    });
+    __syncthreads();
+    const uint32_t nwords = (ihi - base) / 2 + 1;
+    const uint32_t w0 = (ilo - base) / 2;
+    for (;;) {
+        // Q: slot the peptides with hits >= min_shp, clear the others
+        for (uint32_t w = w0 + threadIdx.x; w < nwords; w += kBT) {
+            const uint32_t v = sm.map[w];
+            if (v == 0) continue;
+            uint32_t nv = 0;
+            for (int h = 0; h < 2; h++) {
+                const uint32_t cnt = (v >> (16 * h)) & 0xFFFFu;
+                if (cnt == 0) continue;
+                uint32_t nh = 0;
+                if (cnt >= a.q.min_shp) {
+                    const uint32_t sl = lds_add_rtn(&s_ctl[0], 1u);
+                    if (sl < a.slots) {
+                        nh = 0x8000u | sl;
+                        sm.slot_loc[sl] = 2 * w + h;
+                        sm.slot_cnt[sl] = 0;
+                        sm.slot_sum[2 * sl] = 0;
+                        sm.slot_sum[2 * sl + 1] = 0;
+                    } else {
+                        nh = cnt & 0x7FFFu;   // next round
+                        s_ctl[1] = 1;
+                    }
+                }
+                nv |= nh << (16 * h);
+            }
+            sm.map[w] = nv;
+        }
+        __syncthreads();
+        const uint32_t ns = min(s_ctl[0], a.slots);
+        if (ns > 0) {
+            // P2: exact counts and sums of the slotted peptides
+            walk_flat(sm, L.iA, NP, E, [&](uint32_t raw, uint32_t prb) {
+                const uint32_t ppid = udiv(raw, L.div_m, L.div_s);
+                if (ppid >= ilo && ppid <= ihi) {
+                    const uint32_t local = ppid - base;
+                    const uint32_t h = (sm.map[local >> 1] >> ((local & 1u) << 4)) & 0xFFFFu;
+                    if (h & 0x8000u) {
+                        const uint32_t sl = h & 0x7FFFu;
+                        const uint32_t isY = (raw - ppid * speclen) >= half ? 1u : 0u;
+                        lds_add(&sm.slot_cnt[sl], isY ? 0x10000u : 1u);
+                        lds_add64(&sm.slot_sum[2 * sl + isY], (unsigned long long)sm.pk_int[prb / W]);
+                    }
+                }
+            });
+            __syncthreads();
+            // S: score the slots
+            for (uint32_t sl = threadIdx.x; sl < ns; sl += kBT) {
+                const uint32_t loc = sm.slot_loc[sl];
+                const uint32_t cw = sm.slot_cnt[sl];
+                const uint32_t bc = cw & 0xFFFFu, yc = cw >> 16;
+                const unsigned long long ib = sm.slot_sum[2 * sl], iy = sm.slot_sum[2 * sl + 1];
+                lds_and(&sm.map[loc >> 1], ~(0xFFFFu << ((loc & 1u) << 4)));
+                if ((uint16_t)(bc + yc) < a.q.min_shp) continue;
+                uint32_t bi = bc, yi = yc;
+                if (bi > a.q.max_shdpeaks) { bi = a.q.max_shdpeaks; acc.flags |= HCX_RES_LGF_CLAMP; }
+                if (yi > a.q.max_shdpeaks) { yi = a.q.max_shdpeaks; acc.flags |= HCX_RES_LGF_CLAMP; }
+                double t = sm.lgf[bi] + sm.lgf[yi];
+                t = t + log10((double)(1ull + ib));
+                t = t + log10((double)(1ull + iy));
+                t = t - 6.0;
+                const float hs = (float)t;
+                if (!(hs > 0.0f)) continue;
+                acc.cpsms++;
+                int bin = (int)((double)(hs * 10.0f) + 0.5);
+                if (bin >= (int)a.q.hbins) { bin = (int)a.q.hbins - 1; acc.flags |= HCX_RES_HIST_CLAMP; }
+                lds_add(&sm.hist[bin], 1u);
+                if (hs > acc.best_hs) {
+                    acc.best_hs = hs;
+                    acc.best_cnt = 1;
+                    acc.best_ixx = (uint32_t)ixx;
+                    acc.best_psid = base + loc;
+                    acc.best_bc = bc;
+                    acc.best_yc = yc;
+                } else if (hs == acc.best_hs) {
+                    acc.best_cnt++;
+                }
+            }
+            __syncthreads();
+        }
+        const bool more = s_ctl[1] != 0;
+        __syncthreads();
+        if (!more) break;
+        if (threadIdx.x == 0) {
+            s_ctl[0] = 0;
+            s_ctl[1] = 0;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        s_ctl[0] = 0;
+        s_ctl[1] = 0;
+    }
+    __syncthreads();
+}
+
+// Merge a block's accumulators for spectrum s into the global SpecAcc.
+// s_red: [0] max bits, [1] count at max, [2] cpsms, [3] flags; s_red64: n_ia, n_probe
+__device__ __forceinline__ void flush_spectrum(const ChunkArgs &a, const Smem &sm, uint32_t s, Acc &acc, lu32 *s_red,
+                                               lu64 *s_red64) {
+    if (threadIdx.x == 0) {
+        s_red[0] = 0;
+        s_red[1] = 0;
+        s_red[2] = 0;
+        s_red[3] = 0;
+        s_red64[0] = 0;
+        s_red64[1] = 0;
+    }
+    __syncthreads();
+    if (acc.best_cnt) lds_max(&s_red[0], __float_as_uint(acc.best_hs));
+    if (acc.cpsms) lds_add(&s_red[2], acc.cpsms);
+    if (acc.flags) lds_or(&s_red[3], acc.flags);
+    if (acc.nia) lds_add64(&s_red64[0], acc.nia);
+    if (acc.nprobe) lds_add64(&s_red64[1], acc.nprobe);
+    __syncthreads();
+    SpecAcc *g = a.acc + s;
+    const uint32_t mb = s_red[0];
+    if (acc.best_cnt && __float_as_uint(acc.best_hs) == mb) {
+        lds_add(&s_red[1], acc.best_cnt);
+        glb_max64(&g->top_a, ((unsigned long long)mb << 32) | ((unsigned long long)acc.best_ixx << 25) |
+                                 (acc.best_psid & kLow25));
+        glb_max64(&g->top_b, ((unsigned long long)mb << 32) | ((unsigned long long)acc.best_bc << 16) | acc.best_yc);
+    }
+    // histogram: non-zero bins to global, then clear
+    uint32_t *hg = a.q.hist + (size_t)s * a.q.hbins;
+    for (uint32_t h = threadIdx.x; h < a.q.hbins; h += kBT) {
+        const uint32_t v = sm.hist[h];
+        if (v) {
+            glb_add(&hg[h], v);
+            sm.hist[h] = 0;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_red[2]) glb_add(&g->cpsms, (uint32_t)s_red[2]);
+        if (s_red[3]) glb_or(&g->flags, s_red[3]);
+        if (s_red64[0]) glb_add(&g->n_ia, (unsigned long long)s_red64[0]);
+        if (s_red64[1]) glb_add(&g->n_probe, (unsigned long long)s_red64[1]);
+        const uint32_t cnt = s_red[1];
+        if (cnt) {
+            gu64 *tn = (gu64 *)&g->top_n;
+            const unsigned long long mine = ((unsigned long long)mb << 32) | cnt;
+            unsigned long long old = *tn;
+            for (;;) {
+                const uint32_t oh = (uint32_t)(old >> 32);
+                if (mb < oh) break;
+                const unsigned long long want = (mb > oh) ? mine : old + cnt;
+                if (__hip_atomic_compare_exchange_strong(tn, &old, want, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT))
+                    break;
+            }
+        }
+    }
+    __syncthreads();
+    acc_reset(acc);
+}
+
+__device__ __forceinline__ uint32_t load_peaks(const ChunkArgs &a, const Smem &sm, uint32_t s) {
+    gcu32 *qidx = (gcu32 *)a.q.qidx, *moz = (gcu32 *)a.q.moz, *inten = (gcu32 *)a.q.inten;
+    const uint32_t q0 = qidx[s];
+    const uint32_t P = qidx[s + 1] - q0;
+    for (uint32_t k = threadIdx.x; k < P; k += kBT) {
+        sm.pk_moz[k] = moz[q0 + k];
+        sm.pk_int[k] = inten[q0 + k];
+    }
+    return P;
+}
+
+__device__ __forceinline__ void init_smem(const ChunkArgs &a, const Smem &sm) {
+    for (uint32_t i = threadIdx.x; i <= a.q.max_shdpeaks; i += kBT) sm.lgf[i] = a.q.lgf[i];
+    for (uint32_t i = threadIdx.x; i < a.tc / 2; i += kBT) sm.map[i] = 0;
+    for (uint32_t i = threadIdx.x; i < a.q.hbins; i += kBT) sm.hist[i] = 0;
+}
+
+// Plain form: one workgroup per spectrum, its items in order.
+__global__ __launch_bounds__(kBT) void k_spec_chunked(ChunkArgs a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ uint32_t s_ctl_[2];
+    __shared__ uint32_t s_red_[4];
+    __shared__ unsigned long long s_red64_[2];
+    lu32 *s_ctl = (lu32 *)s_ctl_, *s_red = (lu32 *)s_red_;
+    lu64 *s_red64 = (lu64 *)s_red64_;
+    const Smem sm = bind(smem_raw, make_layout(a));
+    init_smem(a, sm);
+    if (threadIdx.x == 0) {
+        s_ctl[0] = 0;
+        s_ctl[1] = 0;
+    }
+    const uint32_t s = blockIdx.x;
+    const uint32_t P = load_peaks(a, sm, s);
+    const uint32_t D = a.q.max_mass + 2;
+    const float prec = ((const AS_GLB float *)a.q.prec)[s];
+    for (int l = threadIdx.x; l < a.q.nlen; l += kBT) {
+        const LenDesc L = load_len(a, l);
+        int32_t lo, hi;
+        window(L, prec, a.q.dM, D, &lo, &hi);
+        sm.wlo[l] = lo;
+        sm.whi[l] = hi;
+    }
+    __syncthreads();
+    Acc acc;
+    acc_reset(acc);
+    for (int ixx = 0; ixx < a.q.nlen; ixx++) {
+        const int32_t lo = sm.wlo[ixx], hi = sm.whi[ixx];
+        if (lo > hi) continue;
+        for (uint32_t c = (uint32_t)lo / a.tc; c <= (uint32_t)hi / a.tc; c++)
+            process_item(a, sm, s_ctl, P, ixx, c, lo, hi, acc);
+    }
+    flush_spectrum(a, sm, s, acc, s_red, s_red64);
+}
+
+// Batched form: persistent workgroups pull (spectrum, length, chunk) items.
+__global__ __launch_bounds__(kBT) void k_items_chunked(ChunkArgs a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ uint32_t s_ctl_[2];
+    __shared__ uint32_t s_item_;
+    __shared__ uint32_t s_red_[4];
+    __shared__ unsigned long long s_red64_[2];
+    lu32 *s_ctl = (lu32 *)s_ctl_, *s_red = (lu32 *)s_red_, *s_item = (lu32 *)&s_item_;
+    lu64 *s_red64 = (lu64 *)s_red64_;
+    const Smem sm = bind(smem_raw, make_layout(a));
+    init_smem(a, sm);
+    if (threadIdx.x == 0) {
+        s_ctl[0] = 0;
+        s_ctl[1] = 0;
+    }
+    Acc acc;
+    acc_reset(acc);
+    uint32_t cur = 0xFFFFFFFFu, P = 0;
+    const Item *items = a.items;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) *s_item = __hip_atomic_fetch_add((gu32 *)a.queue, kGrab, __ATOMIC_RELAXED,
+                                                               __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const uint32_t it0 = *s_item;
+        if (it0 >= a.nitems) break;
+        const uint32_t it1 = min(a.nitems, it0 + kGrab);
+        for (uint32_t it = it0; it < it1; it++) {
+            const Item item = items[it];
+            if (item.spec != cur) {
+                if (cur != 0xFFFFFFFFu) flush_spectrum(a, sm, cur, acc, s_red, s_red64);
+                cur = item.spec;
+                P = load_peaks(a, sm, cur);
+                __syncthreads();
+            }
+            process_item(a, sm, s_ctl, P, item.ixx, item.chunk, item.lo, item.hi, acc);
+        }
+    }
+    if (cur != 0xFFFFFFFFu) flush_spectrum(a, sm, cur, acc, s_red, s_red64);
+}
+
+// Items: per spectrum and length with a valid window, one item per chunk.
+__global__ void k_items_count(ChunkArgs a, uint32_t *counts) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= (uint32_t)a.q.nspec) return;
+    const uint32_t D = a.q.max_mass + 2;
+    uint32_t n = 0;
+    for (int l = 0; l < a.q.nlen; l++) {
+        int32_t lo, hi;
+        window(a.q.lens[l], a.q.prec[s], a.q.dM, D, &lo, &hi);
+        if (lo <= hi) n += (uint32_t)hi / a.tc - (uint32_t)lo / a.tc + 1;
+    }
+    counts[s] = n;
+}
+
+__global__ void k_items_fill(ChunkArgs a, const uint32_t *offs, Item *items) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= (uint32_t)a.q.nspec) return;
+    const uint32_t D = a.q.max_mass + 2;
+    uint32_t o = offs[s];
+    for (int l = 0; l < a.q.nlen; l++) {
+        int32_t lo, hi;
+        window(a.q.lens[l], a.q.prec[s], a.q.dM, D, &lo, &hi);
+        if (lo > hi) continue;
+        for (uint32_t c = (uint32_t)lo / a.tc; c <= (uint32_t)hi / a.tc; c++) {
+            Item it;
+            it.spec = s;
+            it.ixx = (uint16_t)l;
+            it.chunk = (uint16_t)c;
+            it.lo = lo;
+            it.hi = hi;
+            items[o++] = it;
+        }
+    }
+}
+
+// SpecAcc -> hcx_result; tied maxima go to the replay list.
+__global__ void k_finalize(ChunkArgs a) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= (uint32_t)a.q.nspec) return;
+    const SpecAcc g = a.acc[s];
+    hcx_result r;
+    r.hyperscore = 0.0f;
+    r.psid = 0;
+    r.idxoffset = 0;
+    r.sharedions = 0;
+    r.totalions = 0;
+    r.bc = r.yc = 0;
+    r.cpsms = g.cpsms;
+    r.flags = (uint16_t)g.flags;
+    r.maxhypscore = r.mu = r.beta = r.minhypscore = r.nexthypscore = 0;
+    r.e_x = 0.0;
+    const uint32_t cnt = (uint32_t)g.top_n;
+    if (cnt >= 1) {
+        r.hyperscore = __uint_as_float((uint32_t)(g.top_n >> 32));
+        if (cnt == 1) {
+            const uint32_t ixx = (uint32_t)((g.top_a >> 25) & 0x7F);
+            r.psid = (int32_t)(g.top_a & kLow25);
+            r.idxoffset = (uint16_t)ixx;
+            r.bc = (uint32_t)((g.top_b >> 16) & 0xFFFF);
+            r.yc = (uint32_t)(g.top_b & 0xFFFF);
+            r.sharedions = (uint16_t)(r.bc + r.yc);
+            r.totalions = (uint16_t)a.q.lens[ixx].speclen;
+        } else {
+            const uint32_t slot = atomicAdd(&a.q.tie[0], 1u);
+            a.q.tie[1 + slot] = s;
+        }
+    }
+    a.q.res[s] = r;
+    if (a.q.stats) {
+        a.q.stats[s].n_ia = g.n_ia;
+        a.q.stats[s].n_probe = g.n_probe;
+    }
+}
+
+__global__ void k_chunk_offsets(const uint32_t *iA, const uint32_t *bA, uint32_t NB, uint32_t nch, uint32_t TC,
+                                uint32_t speclen, uint32_t *bA2) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)nch + 1;
+    if (t >= (uint64_t)NB * stride) return;
+    const uint32_t bin = (uint32_t)(t / stride), c = (uint32_t)(t % stride);
+    const uint32_t s0 = bA[bin], e0 = bA[bin + 1];
+    uint32_t r;
+    if (c == 0) {
+        r = s0;
+    } else if (c == nch) {
+        r = e0;
+    } else {
+        const uint64_t key = (uint64_t)c * TC * speclen;
+        uint32_t lo = s0, hi = e0;
+        while (lo < hi) {
+            const uint32_t m = lo + (hi - lo) / 2;
+            if ((uint64_t)iA[m] < key) lo = m + 1;
+            else hi = m;
+        }
+        r = lo;
+    }
+    bA2[t] = r;
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+size_t chunk_smem_bytes(const ChunkArgs &a) { return make_layout(a).total; }
+
+int launch_chunk_offsets(hcx_ctx *ctx, const uint32_t *iA, const uint32_t *bA, uint32_t NB, uint32_t nch, uint32_t TC,
+                         uint32_t speclen, uint32_t *bA2) {
+    const uint64_t n = (uint64_t)NB * (nch + 1);
+    k_chunk_offsets<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(iA, bA, NB, nch, TC, speclen, bA2);
+    HCX_HIP(hipGetLastError());
+    return HCX_SUCCESS;
+}
+
+int launch_fill_u32(hcx_ctx *ctx, uint32_t *p, uint64_t n, uint32_t v) {
+    if (n == 0) return HCX_SUCCESS;
+    k_fill_u32<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(p, n, v);
+    HCX_HIP(hipGetLastError());
+    return HCX_SUCCESS;
+}
+
+// counts: nspec + 1 words.  fill == false: count and scan only (counts[nspec]
+// then holds the item total); fill == true: write the items.
+int launch_items(hcx_ctx *ctx, ChunkArgs &a, uint32_t *counts, uint32_t *scan_tmp, bool fill) {
+    hipStream_t st = ctx->stream;
+    const int n = a.q.nspec;
+    if (n <= 0) return HCX_SUCCESS;
+    if (!fill) {
+        int rc = launch_fill_u32(ctx, counts, (uint64_t)n + 1, 0);
+        if (rc) return rc;
+        k_items_count<<<(n + 255) / 256, 256, 0, st>>>(a, counts);
+        HCX_HIP(hipGetLastError());
+        return scan_exclusive_u32(st, counts, counts, (uint64_t)n + 1, scan_tmp);
+    }
+    k_items_fill<<<(n + 255) / 256, 256, 0, st>>>(a, counts, a.items_w);
+    HCX_HIP(hipGetLastError());
+    return HCX_SUCCESS;
+}
+
+int launch_chunked(hcx_ctx *ctx, ChunkArgs &a, int kernel) {
+    hipStream_t st = ctx->stream;
+    const size_t sm = chunk_smem_bytes(a);
+    const int n = a.q.nspec;
+    if (n <= 0) return HCX_SUCCESS;
+    if (kernel == HCX_KERNEL_BATCHED) {
+        HCX_HIP(hipFuncSetAttribute((const void *)k_items_chunked, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+        HCX_HIP(hipEventRecord(ctx->ev0, st));
+        k_items_chunked<<<a.grid, kBT, sm, st>>>(a);
+        HCX_HIP(hipGetLastError());
+    } else {
+        HCX_HIP(hipFuncSetAttribute((const void *)k_spec_chunked, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+        HCX_HIP(hipEventRecord(ctx->ev0, st));
+        k_spec_chunked<<<n, kBT, sm, st>>>(a);
+        HCX_HIP(hipGetLastError());
+    }
+    HCX_HIP(hipEventRecord(ctx->ev1, st));
+    k_finalize<<<(n + 255) / 256, 256, 0, st>>>(a);
+    HCX_HIP(hipGetLastError());
+    return HCX_SUCCESS;
+}
+
+}  // namespace hcx

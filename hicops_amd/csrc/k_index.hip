@@ -278,7 +278,7 @@ __global__ void k_zero(uint32_t *p, uint64_t n) {
 }  // namespace
 
 // Exclusive scan of n u32 on the stream; tmp must hold scan_tmp_words(n) u32.
-static uint64_t scan_tmp_words(uint64_t n) {
+uint64_t scan_tmp_words(uint64_t n) {
     uint64_t w = 0;
     while (n > (uint64_t)kTile) {
         n = (n + kTile - 1) / kTile;
@@ -287,7 +287,11 @@ static uint64_t scan_tmp_words(uint64_t n) {
     return w + 1;
 }
 
+int scan_exclusive_u32(hipStream_t st, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *tmp);
 static int scan_exclusive(hipStream_t st, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *tmp) {
+    return scan_exclusive_u32(st, in, out, n, tmp);
+}
+int scan_exclusive_u32(hipStream_t st, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *tmp) {
     if (n == 0) return HCX_SUCCESS;
     const uint64_t nb = (n + kTile - 1) / kTile;
     if (nb == 1) {
@@ -298,7 +302,7 @@ static int scan_exclusive(hipStream_t st, const uint32_t *in, uint32_t *out, uin
     uint32_t *sums = tmp;
     k_scan_reduce<<<(unsigned)nb, kBT, 0, st>>>(in, n, sums);
     HCX_HIP(hipGetLastError());
-    int rc = scan_exclusive(st, sums, sums, nb, tmp + nb + 1);
+    int rc = scan_exclusive_u32(st, sums, sums, nb, tmp + nb + 1);
     if (rc) return rc;
     k_scan_block<<<(unsigned)nb, kBT, 0, st>>>(in, out, n, sums);
     HCX_HIP(hipGetLastError());

@@ -736,20 +736,18 @@ __global__ __launch_bounds__(kBT) void k_tailfit_raw(const uint32_t *hist, uint3
 
 size_t query_smem_bytes(const QueryArgs &a, bool replay) { return smem_layout(a, replay, nullptr, nullptr); }
 
-int launch_query(hcx_ctx *ctx, const QueryArgs &a, int kernel, bool timed, float *ms) {
-    (void)kernel;
+// Cursor form: one workgroup per spectrum over the reference bA/iA only
+// (per-probe binary searches, then cursors across LDS tiles).  Records
+// ctx->ev0 / ev1 around the kernel.
+int launch_cursor(hcx_ctx *ctx, const QueryArgs &a) {
     const size_t sm = query_smem_bytes(a, false);
     HCX_HIP(hipFuncSetAttribute((const void *)k_query<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
-    if (timed) HCX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    HCX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     if (a.nspec > 0) {
         k_query<false><<<a.nspec, kBT, sm, ctx->stream>>>(a);
         HCX_HIP(hipGetLastError());
     }
-    if (timed) {
-        HCX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
-        HCX_HIP(hipEventSynchronize(ctx->ev1));
-        HCX_HIP(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
-    }
+    HCX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     return HCX_SUCCESS;
 }
 

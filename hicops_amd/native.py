@@ -36,6 +36,7 @@ RES_AUDITED = 0x80
 
 KERNEL_PLAIN = 0
 KERNEL_BATCHED = 1
+KERNEL_CURSOR = 2
 EVAL_TAILFIT = 0
 EVAL_SURROGATE = 1
 
@@ -147,7 +148,7 @@ def hcx() -> C.CDLL:
                      "hcx_queries_upload", "hcx_queries_free", "hcx_search",
                      "hcx_search_device", "hcx_search_fetch", "hcx_search_candidates",
                      "hcx_tailfit_batch", "hcx_lgfact_table", "hcx_heap_capacity",
-                     "hcx_selftest_divider"]:
+                     "hcx_selftest_divider", "hcx_set_tuning", "hcx_get_tuning"]:
             getattr(lib, name).restype = C.c_int
         _hcx = lib
     return _hcx

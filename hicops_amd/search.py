@@ -85,7 +85,8 @@ class Batch:
 
 
 class Engine:
-    def __init__(self, params: N.Params, device: int = 0):
+    def __init__(self, params: N.Params, device: int = 0, chunk: int = 0, slots: int = 0,
+                 blocks_per_cu: int = 0):
         self.params = params
         h = C.c_void_p()
         N.check(N.hcx().hcx_create(C.byref(params), device, C.byref(h)))
@@ -93,6 +94,18 @@ class Engine:
         self.NB = params.max_mass * params.scale
         self.hbins = params.hbins
         self.indices: list[Index] = []
+        self.set_tuning(chunk, slots, blocks_per_cu)
+
+    def set_tuning(self, chunk: int = 0, slots: int = 0, blocks_per_cu: int = 0) -> None:
+        """chunk must be set before indices are built (they record it)."""
+        N.check(N.hcx().hcx_set_tuning(self._h, C.c_uint32(chunk), C.c_uint32(slots), C.c_uint32(blocks_per_cu)))
+
+    def tuning(self) -> dict:
+        ch, sl, bp = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        cu, lds = C.c_int32(), C.c_int32()
+        N.check(N.hcx().hcx_get_tuning(self._h, C.byref(ch), C.byref(sl), C.byref(bp), C.byref(cu), C.byref(lds)))
+        return {"chunk": ch.value, "slots": sl.value, "blocks_per_cu": bp.value, "num_cu": cu.value,
+                "max_lds": lds.value}
 
     # DSLIM_Construct, one length at a time (hicops.cpp:200-300 loop)
     def construct(self, peplen: int, seqs: np.ndarray, entries: np.ndarray) -> Index:

@@ -126,8 +126,11 @@ typedef struct hcx_index  hcx_index;
 typedef struct hcx_qbatch hcx_qbatch;
 
 /* Query kernel choice. */
-#define HCX_KERNEL_PLAIN   0  /* one workgroup per spectrum                         */
-#define HCX_KERNEL_BATCHED 1  /* persistent, work-stealing over (spectrum, window)   */
+#define HCX_KERNEL_PLAIN   0  /* one workgroup per spectrum, its chunks in order     */
+#define HCX_KERNEL_BATCHED 1  /* persistent, work-stealing over (spectrum, length,
+                                 chunk) items                                       */
+#define HCX_KERNEL_CURSOR  2  /* one workgroup per spectrum on bA/iA alone: binary
+                                 searches + cursors (diagnostic, slow)              */
 
 /* E-value model choice. */
 #define HCX_EVAL_TAILFIT   0  /* expeRT log-linear tail fit on every spectrum        */
@@ -148,6 +151,13 @@ int  hcx_create(const hcx_params *params, int device, hcx_ctx **out);
 int  hcx_destroy(hcx_ctx *ctx);
 const char *hcx_last_error(void);
 int  hcx_heap_capacity(uint32_t topmatches);   /* dslim.cpp:422-428 (Q14) */
+/* Query tuning (no reference counterpart).  chunk: peptides per LDS chunk
+ * (power of two, 256..65536; set before building indices, which record it);
+ * slots: scored-peptide slots per chunk pass; blocks_per_cu: persistent grid
+ * of the batched kernel.  0 keeps the current value. */
+int  hcx_set_tuning(hcx_ctx *ctx, uint32_t chunk, uint32_t slots, uint32_t blocks_per_cu);
+int  hcx_get_tuning(const hcx_ctx *ctx, uint32_t *chunk, uint32_t *slots, uint32_t *blocks_per_cu,
+                    int32_t *num_cu, int32_t *max_lds);
 /* lgfact table the device uses (utils.h:115-131), max_shdpeaks+1 doubles */
 int  hcx_lgfact_table(const hcx_ctx *ctx, double *out, uint32_t n);
 

@@ -99,30 +99,56 @@ def test_index_parity_mods(maxz):
     eng.close()
 
 
-def test_query_parity_small(small_case):
+KERNELS = {"plain": N.KERNEL_PLAIN, "batched": N.KERNEL_BATCHED, "cursor": N.KERNEL_CURSOR}
+
+
+@pytest.mark.parametrize("kernel", ["plain", "batched", "cursor"])
+def test_query_parity_small(small_case, kernel):
     p, db, q, prm = small_case
-    log("test_query_parity_small")
+    log(f"test_query_parity_small kernel={kernel}")
     odb = O.OracleDB(prm, db)
     res_o, hist_o, st_o = odb.search(q)
     eng = Engine(prm)
     eng.construct_db(db)
     b = eng.upload(q)
-    out = eng.query_spectrum(b)
-    n = compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, "small")
+    out = eng.query_spectrum(b, kernel=KERNELS[kernel])
+    n = compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"small-{kernel}")
     assert n > 100
+    # a second search on the same batch (cached work items) gives the same answer
+    out2 = eng.query_spectrum(b, kernel=KERNELS[kernel])
+    assert np.array_equal(out2.results, out.results) and np.array_equal(out2.hist, out.hist)
     eng.close()
 
 
 @pytest.mark.parametrize("tile", [64, 256, 4096])
-def test_query_parity_tiles(small_case, tile):
+def test_cursor_kernel_tiles(small_case, tile):
     p, db, q, prm = small_case
-    log(f"test_query_parity_tiles tile={tile}")
+    log(f"test_cursor_kernel_tiles tile={tile}")
     odb = O.OracleDB(prm, db)
     res_o, hist_o, st_o = odb.search(q)
     eng = Engine(prm)
     eng.construct_db(db)
-    out = eng.query_spectrum(eng.upload(q), tile=tile)
+    out = eng.query_spectrum(eng.upload(q), tile=tile, kernel=N.KERNEL_CURSOR)
     compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"tile{tile}")
+    eng.close()
+
+
+@pytest.mark.parametrize("kernel", ["plain", "batched"])
+@pytest.mark.parametrize("chunk,slots", [(256, 64), (1024, 64), (8192, 256)])
+def test_chunked_edges_and_slot_rounds(kernel, chunk, slots):
+    # small chunks: windows span many chunks and start/end inside chunks;
+    # few slots: peptides reaching min_shp overflow the slot table (several rounds)
+    p = W.scaled("rank_share", npep=20000, nspec=120, maxz=2)
+    db = W.Database(p)
+    q = W.make_queries(db, p)
+    prm = W.make_params(p)
+    log(f"test_chunked_edges_and_slot_rounds kernel={kernel} chunk={chunk} slots={slots}")
+    res_o, hist_o, st_o = O.OracleDB(prm, db).search(q)
+    eng = Engine(prm, chunk=chunk, slots=slots)
+    eng.construct_db(db)
+    out = eng.query_spectrum(eng.upload(q), kernel=KERNELS[kernel])
+    n = compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"{kernel}-{chunk}-{slots}")
+    assert n > 50
     eng.close()
 
 
@@ -143,15 +169,16 @@ def test_candidate_sets_bitexact(small_case):
     eng.close()
 
 
-def test_ties_replayed_in_reference_order(small_case):
+@pytest.mark.parametrize("kernel", ["plain", "batched"])
+def test_ties_replayed_in_reference_order(small_case, kernel):
     p, db, q, prm = small_case
-    log("test_ties_replayed_in_reference_order")
+    log(f"test_ties_replayed_in_reference_order kernel={kernel}")
     twin = TwinDB(db)
     odb = O.OracleDB(prm, twin)
     res_o, hist_o, st_o = odb.search(q)
     eng = Engine(prm)
     eng.construct_db(twin)
-    out = eng.query_spectrum(eng.upload(q))
+    out = eng.query_spectrum(eng.upload(q), kernel=KERNELS[kernel])
     valid = (res_o["flags"] & N.RES_VALID) != 0
     assert np.all((out.results["flags"][valid] & N.RES_TIE_REPLAYED) != 0)
     n = compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, "twin")
@@ -170,9 +197,29 @@ def test_query_parity_open_search_mods(maxz):
     res_o, hist_o, st_o = odb.search(q)
     eng = Engine(prm)
     eng.construct_db(db)
-    out = eng.query_spectrum(eng.upload(q))
-    n = compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"open{maxz}")
-    assert n > 150
+    b = eng.upload(q)
+    for kernel in ("plain", "batched"):
+        out = eng.query_spectrum(b, kernel=KERNELS[kernel])
+        n = compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"open{maxz}-{kernel}")
+        assert n > 100
+    eng.close()
+
+
+def test_open_search_uneven_windows():
+    # the benchmark's shape at 1/100 scale: 20-500 peaks, windows of 1e2-1e6 peptides
+    p = W.scaled("open10m", npep=100000, nspec=300)
+    db = W.Database(p)
+    q = W.make_queries(db, p)
+    prm = W.make_params(p)
+    log(f"test_open_search_uneven_windows: {db.nforms} peptides")
+    res_o, hist_o, st_o = O.OracleDB(prm, db).search(q)
+    eng = Engine(prm)
+    eng.construct_db(db)
+    b = eng.upload(q)
+    for kernel in ("plain", "batched"):
+        out = eng.query_spectrum(b, kernel=KERNELS[kernel])
+        n = compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"uneven-{kernel}")
+        assert n > 200
     eng.close()
 
 
