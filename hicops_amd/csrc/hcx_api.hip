@@ -521,7 +521,8 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
     else
         rc = launch_chunked(c, ca, kernel);   // records ev0 / ev1 around the scan kernel
     if (rc) return rc;
-    rc = launch_replay(c, a);
+    // the cursor path keeps its own (independent) replay; the chunked forms share one
+    rc = kernel == HCX_KERNEL_CURSOR ? launch_replay_cursor(c, a) : launch_replay_chunked(c, ca);
     if (rc) return rc;
     rc = launch_tailfit(c, a.res, a.hist, a.hbins, a.nspec, c->params.min_cpsm, c->params.expect_max);
     if (rc) return rc;

@@ -137,7 +137,7 @@ struct QueryArgs {
     uint32_t *queue;      // batched kernel: work counter
 };
 int launch_cursor(hcx_ctx *ctx, const QueryArgs &a);
-int launch_replay(hcx_ctx *ctx, const QueryArgs &a);
+int launch_replay_cursor(hcx_ctx *ctx, const QueryArgs &a);
 // ---- chunked search (k_chunk.hip) ----
 struct SpecAcc {
     unsigned long long top_a, top_b, top_n;
@@ -165,6 +165,7 @@ int launch_chunk_offsets(hcx_ctx *ctx, const uint32_t *iA, const uint32_t *bA, u
 int launch_fill_u32(hcx_ctx *ctx, uint32_t *p, uint64_t n, uint32_t v);
 int launch_items(hcx_ctx *ctx, ChunkArgs &a, uint32_t *counts, uint32_t *scan_tmp, bool fill);
 int launch_chunked(hcx_ctx *ctx, ChunkArgs &a, int kernel);
+int launch_replay_chunked(hcx_ctx *ctx, ChunkArgs &a);
 int scan_exclusive_u32(hipStream_t st, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *tmp);
 uint64_t scan_tmp_words(uint64_t n);
 

@@ -156,6 +156,7 @@ struct Layout {
     uint32_t slot_cnt;  // slots: bc | yc << 16
     uint32_t slot_sum;  // 2*slots u64: ibc, iyc
     uint32_t slot_loc;  // slots: local peptide index
+    uint32_t slot_hs;   // slots: hyperscore (replay)
     uint32_t pk_moz, pk_int;   // max_peaks
     uint32_t prs;       // NP: run start of each probe in iA
     uint32_t pofs;      // NP + 1: exclusive prefix of the run lengths
@@ -181,6 +182,7 @@ __host__ __device__ inline Layout make_layout(const ChunkArgs &a) {
     l.slot_cnt = take(4u * a.slots);
     l.slot_sum = take(16u * a.slots);
     l.slot_loc = take(4u * a.slots);
+    l.slot_hs = take(4u * a.slots);
     l.pk_moz = take(4u * a.max_peaks);
     l.pk_int = take(4u * a.max_peaks);
     l.prs = take(4u * np);
@@ -195,7 +197,7 @@ __host__ __device__ inline Layout make_layout(const ChunkArgs &a) {
 }
 
 struct Smem {
-    lu32 *map, *slot_cnt, *slot_loc, *pk_moz, *pk_int, *prs, *pofs, *wsum, *hist;
+    lu32 *map, *slot_cnt, *slot_loc, *slot_hs, *pk_moz, *pk_int, *prs, *pofs, *wsum, *hist;
     lu64 *slot_sum;
     lf64 *lgf;
     li32 *wlo, *whi;
@@ -208,6 +210,7 @@ __device__ __forceinline__ Smem bind(unsigned char *raw, const Layout &l) {
     s.slot_cnt = (lu32 *)(b + l.slot_cnt);
     s.slot_sum = (lu64 *)(b + l.slot_sum);
     s.slot_loc = (lu32 *)(b + l.slot_loc);
+    s.slot_hs = (lu32 *)(b + l.slot_hs);
     s.pk_moz = (lu32 *)(b + l.pk_moz);
     s.pk_int = (lu32 *)(b + l.pk_int);
     s.prs = (lu32 *)(b + l.prs);
@@ -315,9 +318,95 @@ __device__ __forceinline__ void walk_flat(const Smem &sm, const uint32_t *iA_, u
 
 __device__ __forceinline__ LenDesc load_len(const ChunkArgs &a, int ixx) { return a.q.lens[ixx]; }
 
+// ---- the reference top-K heap (minheap.h:108-177, 258-277), in LDS ----
+struct Cell {
+    float hs;
+    int32_t psid;
+    uint32_t ixx, bc, yc;
+};
+// structure of arrays in LDS: hs (float bits), psid, ixx, bc, yc
+struct Heap {
+    lu32 *hs, *psid, *ixx, *bc, *yc;
+};
+__device__ __forceinline__ void heap_put(const Heap &h, int i, const Cell &e) {
+    h.hs[i] = __float_as_uint(e.hs);
+    h.psid[i] = (uint32_t)e.psid;
+    h.ixx[i] = e.ixx;
+    h.bc[i] = e.bc;
+    h.yc[i] = e.yc;
+}
+__device__ __forceinline__ Cell heap_get(const Heap &h, int i) {
+    Cell e;
+    e.hs = __uint_as_float(h.hs[i]);
+    e.psid = (int32_t)h.psid[i];
+    e.ixx = h.ixx[i];
+    e.bc = h.bc[i];
+    e.yc = h.yc[i];
+    return e;
+}
+__device__ __forceinline__ float heap_hs(const Heap &h, int i) { return __uint_as_float(h.hs[i]); }
+
+__device__ __forceinline__ void heap_heapify(const Heap &h, int size, int pos) {
+    for (;;) {
+        const int r = (pos + 1) * 2, l = r - 1;
+        int s = pos;
+        float sv = heap_hs(h, s);
+        if (l < size && heap_hs(h, l) < sv) { s = l; sv = heap_hs(h, l); }
+        if (r < size && heap_hs(h, r) < sv) { s = r; sv = heap_hs(h, r); }
+        if (s == pos) return;
+        const Cell t = heap_get(h, s);
+        heap_put(h, s, heap_get(h, pos));
+        heap_put(h, pos, t);
+        pos = s;
+    }
+}
+__device__ __forceinline__ void heap_insert(const Heap &h, int *size, int cap, const Cell &e) {
+    if (*size == cap) {                 // increase_key(0, e)
+        if (e.hs < heap_hs(h, 0)) return;
+        heap_put(h, 0, e);
+        heap_heapify(h, *size, 0);
+        return;
+    }
+    heap_put(h, (*size)++, e);
+    for (int i = (*size - 1) / 2; i >= 0; i--) heap_heapify(h, *size, i);
+}
+__device__ __forceinline__ Cell heap_getmax(const Heap &h, int size) {
+    if (size <= 1) return heap_get(h, 0);
+    Cell m = heap_get(h, size - 1);
+    for (int k = size - 2; k >= size / 2; k--)
+        if (heap_hs(h, k) > m.hs) m = heap_get(h, k);
+    return m;
+}
+
+// Block-wide exclusive scan of one value per thread; returns the prefix and
+// writes the total to *total.  Uses sm.wsum; contains two barriers.
+__device__ __forceinline__ uint32_t block_scan(const Smem &sm, uint32_t v, uint32_t *total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) sm.wsum[w] = x;
+    __syncthreads();
+    uint32_t pre = x - v, tot = 0;
+    for (int ww = 0; ww < kBT / 64; ww++) {
+        const uint32_t s = sm.wsum[ww];
+        if (ww < w) pre += s;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return pre;
+}
+
 // Process one (spectrum, length, chunk) item; peaks are already in LDS.
+// ORDERED: slots follow peptide order and lane 0 inserts every scored peptide
+// into the heap in that order (the reference insertion order within an item).
+template <bool ORDERED>
 __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm, lu32 *s_ctl, uint32_t P, int ixx,
-                                             uint32_t c, int32_t lo, int32_t hi, Acc &acc) {
+                                             uint32_t c, int32_t lo, int32_t hi, Acc &acc, const Heap *heap = nullptr,
+                                             int *hsize = nullptr) {
     // s_ctl[0] = slots taken, s_ctl[1] = another round needed
     const LenDesc L = load_len(a, ixx);
     const uint32_t TC = a.tc;
@@ -343,30 +432,65 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
     const uint32_t w0 = (ilo - base) / 2;
     for (;;) {
         // Q: slot the peptides with hits >= min_shp, clear the others
-        for (uint32_t w = w0 + threadIdx.x; w < nwords; w += kBT) {
-            const uint32_t v = sm.map[w];
-            if (v == 0) continue;
-            uint32_t nv = 0;
-            for (int h = 0; h < 2; h++) {
-                const uint32_t cnt = (v >> (16 * h)) & 0xFFFFu;
-                if (cnt == 0) continue;
-                uint32_t nh = 0;
-                if (cnt >= a.q.min_shp) {
-                    const uint32_t sl = lds_add_rtn(&s_ctl[0], 1u);
-                    if (sl < a.slots) {
-                        nh = 0x8000u | sl;
-                        sm.slot_loc[sl] = 2 * w + h;
-                        sm.slot_cnt[sl] = 0;
-                        sm.slot_sum[2 * sl] = 0;
-                        sm.slot_sum[2 * sl + 1] = 0;
-                    } else {
-                        nh = cnt & 0x7FFFu;   // next round
-                        s_ctl[1] = 1;
+        if (!ORDERED) {
+            for (uint32_t w = w0 + threadIdx.x; w < nwords; w += kBT) {
+                const uint32_t v = sm.map[w];
+                if (v == 0) continue;
+                uint32_t nv = 0;
+                for (int h = 0; h < 2; h++) {
+                    const uint32_t cnt = (v >> (16 * h)) & 0xFFFFu;
+                    if (cnt == 0) continue;
+                    uint32_t nh = 0;
+                    if (cnt >= a.q.min_shp) {
+                        const uint32_t sl = lds_add_rtn(&s_ctl[0], 1u);
+                        if (sl < a.slots) {
+                            nh = 0x8000u | sl;
+                            sm.slot_loc[sl] = 2 * w + h;
+                            sm.slot_cnt[sl] = 0;
+                            sm.slot_sum[2 * sl] = 0;
+                            sm.slot_sum[2 * sl + 1] = 0;
+                        } else {
+                            nh = cnt & 0x7FFFu;   // next round
+                            s_ctl[1] = 1;
+                        }
                     }
+                    nv |= nh << (16 * h);
                 }
-                nv |= nh << (16 * h);
+                sm.map[w] = nv;
             }
-            sm.map[w] = nv;
+        } else {
+            uint32_t taken = 0;
+            for (uint32_t wb = w0; wb < nwords; wb += kBT) {
+                const uint32_t w = wb + threadIdx.x;
+                const uint32_t v = w < nwords ? sm.map[w] : 0u;
+                const uint32_t c0 = v & 0xFFFFu, c1 = v >> 16;
+                const uint32_t q0 = c0 >= a.q.min_shp ? 1u : 0u, q1 = c1 >= a.q.min_shp ? 1u : 0u;
+                uint32_t tot;
+                const uint32_t pre = taken + block_scan(sm, q0 + q1, &tot);
+                if (v) {
+                    uint32_t nv = 0;
+                    for (int h = 0; h < 2; h++) {
+                        const uint32_t cnt = h ? c1 : c0;
+                        if (!(h ? q1 : q0)) continue;
+                        const uint32_t sl = pre + (h ? q0 : 0u);
+                        uint32_t nh;
+                        if (sl < a.slots) {
+                            nh = 0x8000u | sl;
+                            sm.slot_loc[sl] = 2 * w + h;
+                            sm.slot_cnt[sl] = 0;
+                            sm.slot_sum[2 * sl] = 0;
+                            sm.slot_sum[2 * sl + 1] = 0;
+                        } else {
+                            nh = cnt & 0x7FFFu;
+                            s_ctl[1] = 1;
+                        }
+                        nv |= nh << (16 * h);
+                    }
+                    sm.map[w] = nv;
+                }
+                taken += tot;
+            }
+            if (threadIdx.x == 0) s_ctl[0] = taken;
         }
         __syncthreads();
         const uint32_t ns = min(s_ctl[0], a.slots);
@@ -393,6 +517,7 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
                 const uint32_t bc = cw & 0xFFFFu, yc = cw >> 16;
                 const unsigned long long ib = sm.slot_sum[2 * sl], iy = sm.slot_sum[2 * sl + 1];
                 lds_and(&sm.map[loc >> 1], ~(0xFFFFu << ((loc & 1u) << 4)));
+                if (ORDERED) sm.slot_hs[sl] = 0u;   // +0.0f: not inserted
                 if ((uint16_t)(bc + yc) < a.q.min_shp) continue;
                 uint32_t bi = bc, yi = yc;
                 if (bi > a.q.max_shdpeaks) { bi = a.q.max_shdpeaks; acc.flags |= HCX_RES_LGF_CLAMP; }
@@ -403,6 +528,10 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
                 t = t - 6.0;
                 const float hs = (float)t;
                 if (!(hs > 0.0f)) continue;
+                if (ORDERED) {
+                    sm.slot_hs[sl] = __float_as_uint(hs);
+                    continue;
+                }
                 acc.cpsms++;
                 int bin = (int)((double)(hs * 10.0f) + 0.5);
                 if (bin >= (int)a.q.hbins) { bin = (int)a.q.hbins - 1; acc.flags |= HCX_RES_HIST_CLAMP; }
@@ -419,6 +548,18 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
                 }
             }
             __syncthreads();
+            if (ORDERED) {
+                if (threadIdx.x == 0) {
+                    for (uint32_t sl = 0; sl < ns; sl++) {
+                        const float hs = __uint_as_float(sm.slot_hs[sl]);
+                        if (!(hs > 0.0f)) continue;
+                        const uint32_t cw = sm.slot_cnt[sl];
+                        const Cell e = {hs, (int32_t)(base + sm.slot_loc[sl]), (uint32_t)ixx, cw & 0xFFFFu, cw >> 16};
+                        heap_insert(*heap, hsize, a.q.heap_cap, e);
+                    }
+                }
+                __syncthreads();
+            }
         }
         const bool more = s_ctl[1] != 0;
         __syncthreads();
@@ -546,7 +687,7 @@ __global__ __launch_bounds__(kBT) void k_spec_chunked(ChunkArgs a) {
         const int32_t lo = sm.wlo[ixx], hi = sm.whi[ixx];
         if (lo > hi) continue;
         for (uint32_t c = (uint32_t)lo / a.tc; c <= (uint32_t)hi / a.tc; c++)
-            process_item(a, sm, s_ctl, P, ixx, c, lo, hi, acc);
+            process_item<false>(a, sm, s_ctl, P, ixx, c, lo, hi, acc);
     }
     flush_spectrum(a, sm, s, acc, s_red, s_red64);
 }
@@ -586,10 +727,67 @@ __global__ __launch_bounds__(kBT) void k_items_chunked(ChunkArgs a) {
                 P = load_peaks(a, sm, cur);
                 __syncthreads();
             }
-            process_item(a, sm, s_ctl, P, item.ixx, item.chunk, item.lo, item.hi, acc);
+            process_item<false>(a, sm, s_ctl, P, item.ixx, item.chunk, item.lo, item.hi, acc);
         }
     }
     if (cur != 0xFFFFFFFFu) flush_spectrum(a, sm, cur, acc, s_red, s_red64);
+}
+
+// Tie replay: spectra whose maximum hyperscore is shared by several peptides.
+// One workgroup per listed spectrum runs its items in the reference order
+// (length, then peptide id) and inserts every scored peptide into the heap,
+// whose getMax() then picks the top hit exactly as the reference does.
+__global__ __launch_bounds__(kBT) void k_replay_chunked(ChunkArgs a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ uint32_t s_ctl_[2];
+    __shared__ uint32_t s_heap_[5][256];
+    lu32 *s_ctl = (lu32 *)s_ctl_;
+    const Heap heap = {(lu32 *)s_heap_[0], (lu32 *)s_heap_[1], (lu32 *)s_heap_[2], (lu32 *)s_heap_[3],
+                       (lu32 *)s_heap_[4]};
+    const Smem sm = bind(smem_raw, make_layout(a));
+    init_smem(a, sm);
+    if (threadIdx.x == 0) {
+        s_ctl[0] = 0;
+        s_ctl[1] = 0;
+    }
+    const uint32_t nwork = ((const AS_GLB uint32_t *)a.q.tie)[0];
+    const uint32_t D = a.q.max_mass + 2;
+    for (uint32_t wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+        const uint32_t s = ((const AS_GLB uint32_t *)a.q.tie)[1 + wi];
+        __syncthreads();
+        const uint32_t P = load_peaks(a, sm, s);
+        const float prec = ((const AS_GLB float *)a.q.prec)[s];
+        for (int l = threadIdx.x; l < a.q.nlen; l += kBT) {
+            const LenDesc L = load_len(a, l);
+            int32_t lo, hi;
+            window(L, prec, a.q.dM, D, &lo, &hi);
+            sm.wlo[l] = lo;
+            sm.whi[l] = hi;
+        }
+        __syncthreads();
+        int hsize = 0;   // meaningful on lane 0 only
+        Acc acc;
+        acc_reset(acc);
+        for (int ixx = 0; ixx < a.q.nlen; ixx++) {
+            const int32_t lo = sm.wlo[ixx], hi = sm.whi[ixx];
+            if (lo > hi) continue;
+            for (uint32_t c = (uint32_t)lo / a.tc; c <= (uint32_t)hi / a.tc; c++)
+                process_item<true>(a, sm, s_ctl, P, ixx, c, lo, hi, acc, &heap, &hsize);
+        }
+        if (threadIdx.x == 0 && hsize > 0) {
+            const Cell m = heap_getmax(heap, hsize);
+            hcx_result r = a.q.res[s];
+            r.hyperscore = m.hs;
+            r.psid = m.psid;
+            r.idxoffset = (uint16_t)m.ixx;
+            r.bc = m.bc;
+            r.yc = m.yc;
+            r.sharedions = (uint16_t)(m.bc + m.yc);
+            r.totalions = (uint16_t)load_len(a, (int)m.ixx).speclen;
+            r.flags |= HCX_RES_TIE_REPLAYED;
+            a.q.res[s] = r;
+        }
+    }
 }
 
 // Items: per spectrum and length with a valid window, one item per chunk.
@@ -751,6 +949,16 @@ int launch_chunked(hcx_ctx *ctx, ChunkArgs &a, int kernel) {
     }
     HCX_HIP(hipEventRecord(ctx->ev1, st));
     k_finalize<<<(n + 255) / 256, 256, 0, st>>>(a);
+    HCX_HIP(hipGetLastError());
+    return HCX_SUCCESS;
+}
+
+int launch_replay_chunked(hcx_ctx *ctx, ChunkArgs &a) {
+    const size_t sm = chunk_smem_bytes(a);
+    if (a.q.nspec <= 0) return HCX_SUCCESS;
+    HCX_HIP(hipFuncSetAttribute((const void *)k_replay_chunked, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+    const int grid = std::max(1, std::min(a.q.nspec, ctx->num_cu * (int)ctx->blocks_per_cu));
+    k_replay_chunked<<<grid, kBT, sm, ctx->stream>>>(a);
     HCX_HIP(hipGetLastError());
     return HCX_SUCCESS;
 }

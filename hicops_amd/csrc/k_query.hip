@@ -751,7 +751,9 @@ int launch_cursor(hcx_ctx *ctx, const QueryArgs &a) {
     return HCX_SUCCESS;
 }
 
-int launch_replay(hcx_ctx *ctx, const QueryArgs &a) {
+// Reference-order replay of tied spectra on the cursor path (kept as an
+// independent second implementation for the parity tests).
+int launch_replay_cursor(hcx_ctx *ctx, const QueryArgs &a) {
     const size_t sm = query_smem_bytes(a, true);
     HCX_HIP(hipFuncSetAttribute((const void *)k_query<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
     const int grid = std::max(1, std::min(a.nspec, ctx->num_cu * 2));
