@@ -97,9 +97,9 @@ extern "C" int hcx_create(const hcx_params *params, int device, hcx_ctx **out) {
     c->num_cu = prop.multiProcessorCount;
     if (hipDeviceGetAttribute(&c->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
         c->max_lds = 65536;
-    // 2 bytes of hit counter per peptide of a chunk; leave room for slots,
-    // peaks and the histogram
-    c->chunk = c->max_lds >= 96 * 1024 ? 16384 : 8192;
+    // one byte of hit counter per peptide of a chunk; leave room for slots,
+    // peaks, run table and the histogram
+    c->chunk = c->max_lds >= 96 * 1024 ? 32768 : 16384;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail("stream");
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess)
@@ -469,11 +469,13 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
     int rc = prepare(c, idx, nidx, qb, opts, &ca.q);
     if (rc) return rc;
     QueryArgs &a = ca.q;
-    const int kernel = opts ? opts->kernel : HCX_KERNEL_PLAIN;
+    int kernel = opts ? opts->kernel : HCX_KERNEL_PLAIN;
+    if (c->params.min_shp > kMaxChunkShp) kernel = HCX_KERNEL_CURSOR;
     const uint64_t n = (uint64_t)qb->nspec;
     ca.tc = c->chunk;
     ca.slots = c->slots;
     ca.max_peaks = std::max<uint32_t>(1, qb->max_peaks);
+    ca.group = std::min<uint32_t>(1024, ca.max_peaks * (2 * c->params.dF + 1));
     ca.grid = (uint32_t)std::max(1, c->num_cu) * c->blocks_per_cu;
     if (chunk_smem_bytes(ca) > (size_t)c->max_lds)
         return set_error_code(HCX_ERR_INVLD_PARAM, "search: chunk/slots do not fit in LDS");

@@ -68,8 +68,8 @@ struct hcx_ctx {
     hcx::DeviceBuf sur_tmp;
     hcx::DeviceBuf scan_tmp;          // device scan scratch
     bool sur_ready = false;
-    uint32_t chunk = 16384;           // peptides per query chunk (power of two)
-    uint32_t slots = 1024;            // scored-peptide slots per chunk pass
+    uint32_t chunk = 32768;           // peptides per query chunk (power of two)
+    uint32_t slots = 512;             // scored-peptide slots per chunk pass
     uint32_t blocks_per_cu = 2;       // persistent grid of the batched kernel
     int max_lds = 65536;              // device limit per workgroup
     // last launch shape (for graph reuse)
@@ -150,9 +150,14 @@ struct Item {
     uint16_t chunk;
     int32_t lo, hi;
 };
+// Largest min_shp the chunked kernels' 8-bit saturating counters can test;
+// a larger threshold runs on the cursor kernel.
+constexpr uint32_t kMaxChunkShp = 64;
 struct ChunkArgs {
     QueryArgs q;
     uint32_t tc, slots, max_peaks, grid;
+    uint32_t group;       // probes per run-table group
+    uint32_t pad_;
     SpecAcc *acc;
     const Item *items;
     Item *items_w;

@@ -7,17 +7,21 @@
 // `bin` (bA2[..+0] = bA[bin], bA2[..+nch] = bA[bin+1]).  iA itself is the
 // reference array, unchanged.
 //
-// Work item = (spectrum, length, chunk).  For one item:
-//   R   every (peak, bin) probe reads its run bounds [bA2[c], bA2[c+1]) and a
-//       block scan of the run lengths gives each thread an equal contiguous
-//       share of the item's entries;
+// Work item = (spectrum, length, chunk).  The item's (peak, bin) probes are
+// taken in groups of at most `group` probes; for one group:
+//   R   every probe reads its run bounds [bA2[c], bA2[c+1]) and a block scan
+//       of the run lengths gives each thread an equal contiguous share of the
+//       group's entries;
 //   P1  8 independent iA loads per lane at a time; each entry inside the
-//       precursor window adds 1 to its peptide's 16-bit hit counter (two
-//       counters per LDS word);
-//   Q   peptides with hits >= min_shp get a slot (the counter becomes
-//       0x8000|slot), the others are cleared;
-//   P2  the entries are read again (L2-resident) and only slotted peptides
-//       accumulate exact b/y counts and intensity sums;
+//       precursor window adds 1 to its peptide's 8-bit hit counter (four
+//       counters per LDS word, saturating so that no byte carries);
+// then, once per item:
+//   Q   a bitmap of the peptides with hits >= min_shp and, per 32-peptide
+//       bitmap word, the number of qualifying peptides before it: a
+//       qualifying peptide's slot is its rank, so slots are in peptide order;
+//   P2  the entries are read again (L2-resident; a one-group item keeps its
+//       run table) and peptides whose rank falls in the current round of
+//       `slots` ranks accumulate exact b/y counts and intensity sums;
 //   S   slots are scored: hs = lgf[bc] + lgf[yc] + log10(1+ibc) + log10(1+iyc) - 6
 //       (dslim_query.cpp:679-722), histogram + top hit of the spectrum.
 // Per-spectrum results from several items merge through global atomics:
@@ -27,8 +31,7 @@
 //   top_b = hs | bc << 16 | yc                (atomicMax)
 //   top_n = hs | number of peptides at hs     (CAS merge of (max, count))
 // A spectrum whose maximum is shared by more than one peptide goes to the
-// in-order heap replay (k_query.hip), as the reference's getMax tie rules
-// require.
+// in-order heap replay below, as the reference's getMax tie rules require.
 //
 // Every LDS pointer is address-space-3 qualified and every global pointer on
 // the scan path address-space-1 qualified, so no access compiles to a flat
@@ -152,14 +155,16 @@ __device__ __forceinline__ void acc_reset(Acc &a) {
 
 // Byte offsets of the dynamic LDS arrays (host and device agree on them).
 struct Layout {
-    uint32_t map;       // tc/2 words: two 16-bit counters each
+    uint32_t map;       // tc/4 words: four 8-bit saturating hit counters each
+    uint32_t bmp;       // tc/32 words: qualification bitmap
+    uint32_t pre;       // tc/32 words: qualifying peptides before each bitmap word
     uint32_t slot_cnt;  // slots: bc | yc << 16
-    uint32_t slot_sum;  // 2*slots u64: ibc, iyc
     uint32_t slot_loc;  // slots: local peptide index
-    uint32_t slot_hs;   // slots: hyperscore (replay)
+    uint32_t slot_hs;   // slots: hyperscore bits (replay)
+    uint32_t slot_sum;  // 2*slots u64: ibc, iyc
     uint32_t pk_moz, pk_int;   // max_peaks
-    uint32_t prs;       // NP: run start of each probe in iA
-    uint32_t pofs;      // NP + 1: exclusive prefix of the run lengths
+    uint32_t prs;       // group: run start of each probe in iA
+    uint32_t pofs;      // group + 1: exclusive prefix of the run lengths
     uint32_t wsum;      // kBT / 64 wave partial sums
     uint32_t hist;      // hbins
     uint32_t lgf;       // max_shdpeaks + 1 doubles
@@ -177,16 +182,17 @@ __host__ __device__ inline Layout make_layout(const ChunkArgs &a) {
         off = al16(off + bytes);
         return o;
     };
-    const uint32_t np = a.max_peaks * (2 * a.q.dF + 1);
-    l.map = take(4u * (a.tc / 2));
+    l.map = take(a.tc);
+    l.bmp = take(a.tc / 8);
+    l.pre = take(a.tc / 8);
     l.slot_cnt = take(4u * a.slots);
-    l.slot_sum = take(16u * a.slots);
     l.slot_loc = take(4u * a.slots);
     l.slot_hs = take(4u * a.slots);
+    l.slot_sum = take(16u * a.slots);
     l.pk_moz = take(4u * a.max_peaks);
     l.pk_int = take(4u * a.max_peaks);
-    l.prs = take(4u * np);
-    l.pofs = take(4u * (np + 1));
+    l.prs = take(4u * a.group);
+    l.pofs = take(4u * (a.group + 1));
     l.wsum = take(4u * (kBT / 64 + 1));
     l.hist = take(4u * a.q.hbins);
     l.lgf = take(8u * (a.q.max_shdpeaks + 1));
@@ -197,7 +203,7 @@ __host__ __device__ inline Layout make_layout(const ChunkArgs &a) {
 }
 
 struct Smem {
-    lu32 *map, *slot_cnt, *slot_loc, *slot_hs, *pk_moz, *pk_int, *prs, *pofs, *wsum, *hist;
+    lu32 *map, *bmp, *pre, *slot_cnt, *slot_loc, *slot_hs, *pk_moz, *pk_int, *prs, *pofs, *wsum, *hist;
     lu64 *slot_sum;
     lf64 *lgf;
     li32 *wlo, *whi;
@@ -207,10 +213,12 @@ __device__ __forceinline__ Smem bind(unsigned char *raw, const Layout &l) {
     AS_LDS unsigned char *b = (AS_LDS unsigned char *)raw;
     Smem s;
     s.map = (lu32 *)(b + l.map);
+    s.bmp = (lu32 *)(b + l.bmp);
+    s.pre = (lu32 *)(b + l.pre);
     s.slot_cnt = (lu32 *)(b + l.slot_cnt);
-    s.slot_sum = (lu64 *)(b + l.slot_sum);
     s.slot_loc = (lu32 *)(b + l.slot_loc);
     s.slot_hs = (lu32 *)(b + l.slot_hs);
+    s.slot_sum = (lu64 *)(b + l.slot_sum);
     s.pk_moz = (lu32 *)(b + l.pk_moz);
     s.pk_int = (lu32 *)(b + l.pk_int);
     s.prs = (lu32 *)(b + l.prs);
@@ -223,33 +231,9 @@ __device__ __forceinline__ Smem bind(unsigned char *raw, const Layout &l) {
     return s;
 }
 
-// Probe set-up of one item: run start and length of every (peak, bin) probe
-// in chunk c, then an exclusive block scan of the run lengths.  Returns the
-// item's entry total E (sm.pofs[NP] == E).
-__device__ __forceinline__ uint32_t setup_runs(const ChunkArgs &a, const Smem &sm, const LenDesc &L, uint32_t P,
-                                               uint32_t c, bool count_probes, Acc &acc) {
-    const uint32_t W = 2 * a.q.dF + 1;
-    const uint32_t NP = P * W;
-    const uint32_t stride = L.nch + 1;
-    // each thread owns a contiguous group of probes, so its sums feed the scan
-    const uint32_t per = (NP + kBT - 1) / kBT;
-    const uint32_t p0 = threadIdx.x * per;
-    const uint32_t p1 = min(NP, p0 + per);
-    uint32_t tsum = 0;
-    for (uint32_t p = p0; p < p1; p++) {
-        const uint32_t k = p / W, j = p - k * W;
-        const uint32_t q = sm.pk_moz[k];
-        uint32_t rs = 0, len = 0;
-        if (q > a.q.dF && q < a.q.NB - 1 - a.q.dF) {
-            const size_t o = (size_t)(q - a.q.dF + j) * stride + c;
-            rs = ldg(L.bA2, o);
-            len = ldg(L.bA2, o + 1) - rs;
-            if (count_probes) acc.nprobe++;
-        }
-        sm.prs[p] = rs;
-        sm.pofs[p] = len;   // replaced by the prefix below
-        tsum += len;
-    }
+// Block-wide exclusive scan where thread t holds tsum, the sum of its own
+// contiguous group; returns t's prefix.  Uses sm.wsum; one barrier.
+__device__ __forceinline__ uint32_t block_prefix(const Smem &sm, uint32_t tsum) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint32_t x = tsum;
     for (int off = 1; off < 64; off <<= 1) {
@@ -260,18 +244,49 @@ __device__ __forceinline__ uint32_t setup_runs(const ChunkArgs &a, const Smem &s
     __syncthreads();
     uint32_t run = x - tsum;
     for (int ww = 0; ww < w; ww++) run += sm.wsum[ww];
-    for (uint32_t p = p0; p < p1; p++) {
-        const uint32_t len = sm.pofs[p];
-        sm.pofs[p] = run;
-        run += len;
-    }
-    if (threadIdx.x == kBT - 1) sm.pofs[NP] = run;
-    __syncthreads();
-    return sm.pofs[NP];
+    return run;
 }
 
-// Load-balanced walk over the item's E entries: thread t takes the flat range
-// [t*E/kBT, (t+1)*E/kBT), 8 independent iA loads at a time.  fn(raw, probe).
+// Run set-up for the probes [g0, g1) of one item: start and length of each
+// (peak, bin) probe's run in chunk c, then an exclusive scan of the lengths.
+// Returns the group's entry total E (sm.pofs[g1 - g0] == E).
+__device__ __forceinline__ uint32_t setup_runs(const ChunkArgs &a, const Smem &sm, const LenDesc &L, uint32_t c,
+                                               uint32_t g0, uint32_t g1, bool count_probes, Acc &acc) {
+    const uint32_t W = 2 * a.q.dF + 1;
+    const uint32_t n = g1 - g0;
+    const uint32_t stride = L.nch + 1;
+    const uint32_t per = (n + kBT - 1) / kBT;
+    const uint32_t p0 = threadIdx.x * per;
+    const uint32_t p1 = min(n, p0 + per);
+    uint32_t tsum = 0;
+    for (uint32_t lp = p0; lp < p1; lp++) {
+        const uint32_t p = g0 + lp;
+        const uint32_t k = p / W, j = p - k * W;
+        const uint32_t q = sm.pk_moz[k];
+        uint32_t rs = 0, len = 0;
+        if (q > a.q.dF && q < a.q.NB - 1 - a.q.dF) {
+            const size_t o = (size_t)(q - a.q.dF + j) * stride + c;
+            rs = ldg(L.bA2, o);
+            len = ldg(L.bA2, o + 1) - rs;
+            if (count_probes) acc.nprobe++;
+        }
+        sm.prs[lp] = rs;
+        sm.pofs[lp] = len;   // replaced by the prefix below
+        tsum += len;
+    }
+    uint32_t run = block_prefix(sm, tsum);
+    for (uint32_t lp = p0; lp < p1; lp++) {
+        const uint32_t len = sm.pofs[lp];
+        sm.pofs[lp] = run;
+        run += len;
+    }
+    if (threadIdx.x == kBT - 1) sm.pofs[n] = run;
+    __syncthreads();
+    return sm.pofs[n];
+}
+
+// Load-balanced walk over a group's E entries: thread t takes the flat range
+// [t*E/kBT, (t+1)*E/kBT), 8 independent iA loads at a time.  fn(raw, local probe).
 template <typename F>
 __device__ __forceinline__ void walk_flat(const Smem &sm, const uint32_t *iA_, uint32_t NP, uint32_t E, F &&fn) {
     const uint32_t f0 = (uint32_t)(((uint64_t)E * threadIdx.x) / kBT);
@@ -378,36 +393,13 @@ __device__ __forceinline__ Cell heap_getmax(const Heap &h, int size) {
     return m;
 }
 
-// Block-wide exclusive scan of one value per thread; returns the prefix and
-// writes the total to *total.  Uses sm.wsum; contains two barriers.
-__device__ __forceinline__ uint32_t block_scan(const Smem &sm, uint32_t v, uint32_t *total) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint32_t x = v;
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
-    }
-    if (lane == 63) sm.wsum[w] = x;
-    __syncthreads();
-    uint32_t pre = x - v, tot = 0;
-    for (int ww = 0; ww < kBT / 64; ww++) {
-        const uint32_t s = sm.wsum[ww];
-        if (ww < w) pre += s;
-        tot += s;
-    }
-    __syncthreads();
-    *total = tot;
-    return pre;
-}
-
 // Process one (spectrum, length, chunk) item; peaks are already in LDS.
-// ORDERED: slots follow peptide order and lane 0 inserts every scored peptide
-// into the heap in that order (the reference insertion order within an item).
+// ORDERED: lane 0 inserts every scored peptide into the heap in peptide order
+// (slots are ranks in the qualification bitmap, so slot order is peptide order).
 template <bool ORDERED>
 __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm, lu32 *s_ctl, uint32_t P, int ixx,
                                              uint32_t c, int32_t lo, int32_t hi, Acc &acc, const Heap *heap = nullptr,
                                              int *hsize = nullptr) {
-    // s_ctl[0] = slots taken, s_ctl[1] = another round needed
     const LenDesc L = load_len(a, ixx);
     const uint32_t TC = a.tc;
     const uint32_t base = c * TC;
@@ -417,164 +409,147 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
     const bool first = (uint32_t)lo / TC == c;
     const uint32_t W = 2 * a.q.dF + 1;
     const uint32_t NP = P * W;
-    const uint32_t E = setup_runs(a, sm, L, P, c, first, acc);
-    // P1: hit counts
-    walk_flat(sm, L.iA, NP, E, [&](uint32_t raw, uint32_t) {
-        const uint32_t ppid = udiv(raw, L.div_m, L.div_s);
-        if (ppid >= ilo && ppid <= ihi) {
-            const uint32_t local = ppid - base;
-            lds_add(&sm.map[local >> 1], 1u << ((local & 1u) << 4));
-            acc.nia++;
+    const uint32_t GP = a.group;
+    const uint32_t ng = (NP + GP - 1) / GP;
+
+    // P1: 8-bit hit counters.  A counter that reaches 0x80 is pulled back to
+    // 0x40: only "hits >= min_shp" (min_shp <= kMaxChunkShp) matters here, and
+    // the byte never carries into its neighbour unless more than 0x7F other
+    // increments of the same peptide land between one lane's add and its
+    // pull-back; a byte seen at >= 0xC0 (halfway there) flags the spectrum.
+    uint32_t E = 0;
+    for (uint32_t g = 0; g < ng; g++) {
+        const uint32_t g0 = g * GP, g1 = min(NP, g0 + GP);
+        E = setup_runs(a, sm, L, c, g0, g1, first, acc);
+        walk_flat(sm, L.iA, g1 - g0, E, [&](uint32_t raw, uint32_t) {
+            const uint32_t ppid = udiv(raw, L.div_m, L.div_s);
+            if (ppid >= ilo && ppid <= ihi) {
+                const uint32_t local = ppid - base;
+                const uint32_t sh = (local & 3u) << 3;
+                const uint32_t old = lds_add_rtn(&sm.map[local >> 2], 1u << sh);
+                const uint32_t ob = (old >> sh) & 0xFFu;
+                if (ob == 0x7Fu) lds_add(&sm.map[local >> 2], 0u - (0x40u << sh));
+                else if (ob >= 0xC0u) acc.flags |= HCX_RES_COUNT_OVF;
+                acc.nia++;
+            }
+        });
+        __syncthreads();
+    }
+
+    // Q: qualification bitmap, prefix of qualifying peptides per bitmap word,
+    // counters cleared for the next item
+    const uint32_t bw0 = (ilo - base) >> 5, bw1 = ((ihi - base) >> 5) + 1;
+    const uint32_t nbw = bw1 - bw0;
+    {
+        const uint32_t per = (nbw + kBT - 1) / kBT;
+        const uint32_t t0 = bw0 + threadIdx.x * per;
+        const uint32_t t1 = min(bw1, t0 + per);
+        uint32_t tsum = 0;
+        for (uint32_t bw = t0; bw < t1; bw++) {
+            uint32_t bits = 0;
+            for (uint32_t j = 0; j < 8; j++) {
+                const uint32_t v = sm.map[bw * 8 + j];
+                if (!v) continue;
+                for (uint32_t b = 0; b < 4; b++)
+                    if (((v >> (8 * b)) & 0xFFu) >= a.q.min_shp) bits |= 1u << (4 * j + b);
+                sm.map[bw * 8 + j] = 0;
+            }
+            sm.bmp[bw] = bits;
+            sm.pre[bw] = __popc(bits);
+            tsum += __popc(bits);
         }
-    });
-    __syncthreads();
-    const uint32_t nwords = (ihi - base) / 2 + 1;
-    const uint32_t w0 = (ilo - base) / 2;
-    for (;;) {
-        // Q: slot the peptides with hits >= min_shp, clear the others
-        if (!ORDERED) {
-            for (uint32_t w = w0 + threadIdx.x; w < nwords; w += kBT) {
-                const uint32_t v = sm.map[w];
-                if (v == 0) continue;
-                uint32_t nv = 0;
-                for (int h = 0; h < 2; h++) {
-                    const uint32_t cnt = (v >> (16 * h)) & 0xFFFFu;
-                    if (cnt == 0) continue;
-                    uint32_t nh = 0;
-                    if (cnt >= a.q.min_shp) {
-                        const uint32_t sl = lds_add_rtn(&s_ctl[0], 1u);
-                        if (sl < a.slots) {
-                            nh = 0x8000u | sl;
-                            sm.slot_loc[sl] = 2 * w + h;
-                            sm.slot_cnt[sl] = 0;
-                            sm.slot_sum[2 * sl] = 0;
-                            sm.slot_sum[2 * sl + 1] = 0;
-                        } else {
-                            nh = cnt & 0x7FFFu;   // next round
-                            s_ctl[1] = 1;
-                        }
-                    }
-                    nv |= nh << (16 * h);
-                }
-                sm.map[w] = nv;
-            }
-        } else {
-            uint32_t taken = 0;
-            for (uint32_t wb = w0; wb < nwords; wb += kBT) {
-                const uint32_t w = wb + threadIdx.x;
-                const uint32_t v = w < nwords ? sm.map[w] : 0u;
-                const uint32_t c0 = v & 0xFFFFu, c1 = v >> 16;
-                const uint32_t q0 = c0 >= a.q.min_shp ? 1u : 0u, q1 = c1 >= a.q.min_shp ? 1u : 0u;
-                uint32_t tot;
-                const uint32_t pre = taken + block_scan(sm, q0 + q1, &tot);
-                if (v) {
-                    uint32_t nv = 0;
-                    for (int h = 0; h < 2; h++) {
-                        const uint32_t cnt = h ? c1 : c0;
-                        if (!(h ? q1 : q0)) continue;
-                        const uint32_t sl = pre + (h ? q0 : 0u);
-                        uint32_t nh;
-                        if (sl < a.slots) {
-                            nh = 0x8000u | sl;
-                            sm.slot_loc[sl] = 2 * w + h;
-                            sm.slot_cnt[sl] = 0;
-                            sm.slot_sum[2 * sl] = 0;
-                            sm.slot_sum[2 * sl + 1] = 0;
-                        } else {
-                            nh = cnt & 0x7FFFu;
-                            s_ctl[1] = 1;
-                        }
-                        nv |= nh << (16 * h);
-                    }
-                    sm.map[w] = nv;
-                }
-                taken += tot;
-            }
-            if (threadIdx.x == 0) s_ctl[0] = taken;
+        uint32_t run = block_prefix(sm, tsum);
+        for (uint32_t bw = t0; bw < t1; bw++) {
+            const uint32_t cnt = sm.pre[bw];
+            sm.pre[bw] = run;
+            run += cnt;
+        }
+        if (threadIdx.x == kBT - 1) s_ctl[0] = run;
+        __syncthreads();
+    }
+    const uint32_t Qn = s_ctl[0];
+
+    for (uint32_t r0 = 0; r0 < Qn; r0 += a.slots) {
+        const uint32_t ns = min(a.slots, Qn - r0);
+        for (uint32_t sl = threadIdx.x; sl < ns; sl += kBT) {
+            sm.slot_cnt[sl] = 0;
+            sm.slot_sum[2 * sl] = 0;
+            sm.slot_sum[2 * sl + 1] = 0;
         }
         __syncthreads();
-        const uint32_t ns = min(s_ctl[0], a.slots);
-        if (ns > 0) {
-            // P2: exact counts and sums of the slotted peptides
-            walk_flat(sm, L.iA, NP, E, [&](uint32_t raw, uint32_t prb) {
+        // P2: exact b/y counts and intensity sums of slots [r0, r0 + ns)
+        for (uint32_t g = 0; g < ng; g++) {
+            const uint32_t g0 = g * GP, g1 = min(NP, g0 + GP);
+            if (ng > 1) E = setup_runs(a, sm, L, c, g0, g1, false, acc);
+            walk_flat(sm, L.iA, g1 - g0, E, [&](uint32_t raw, uint32_t lp) {
                 const uint32_t ppid = udiv(raw, L.div_m, L.div_s);
                 if (ppid >= ilo && ppid <= ihi) {
                     const uint32_t local = ppid - base;
-                    const uint32_t h = (sm.map[local >> 1] >> ((local & 1u) << 4)) & 0xFFFFu;
-                    if (h & 0x8000u) {
-                        const uint32_t sl = h & 0x7FFFu;
-                        const uint32_t isY = (raw - ppid * speclen) >= half ? 1u : 0u;
-                        lds_add(&sm.slot_cnt[sl], isY ? 0x10000u : 1u);
-                        lds_add64(&sm.slot_sum[2 * sl + isY], (unsigned long long)sm.pk_int[prb / W]);
+                    const uint32_t w = local >> 5, bit = local & 31u;
+                    const uint32_t bits = sm.bmp[w];
+                    if ((bits >> bit) & 1u) {
+                        const uint32_t sl = sm.pre[w] + __popc(bits & ((1u << bit) - 1u)) - r0;
+                        if (sl < ns) {
+                            const uint32_t isY = (raw - ppid * speclen) >= half ? 1u : 0u;
+                            lds_add(&sm.slot_cnt[sl], isY ? 0x10000u : 1u);
+                            lds_add64(&sm.slot_sum[2 * sl + isY], (unsigned long long)sm.pk_int[(g0 + lp) / W]);
+                            sm.slot_loc[sl] = local;
+                        }
                     }
                 }
             });
             __syncthreads();
-            // S: score the slots
-            for (uint32_t sl = threadIdx.x; sl < ns; sl += kBT) {
-                const uint32_t loc = sm.slot_loc[sl];
-                const uint32_t cw = sm.slot_cnt[sl];
-                const uint32_t bc = cw & 0xFFFFu, yc = cw >> 16;
-                const unsigned long long ib = sm.slot_sum[2 * sl], iy = sm.slot_sum[2 * sl + 1];
-                lds_and(&sm.map[loc >> 1], ~(0xFFFFu << ((loc & 1u) << 4)));
-                if (ORDERED) sm.slot_hs[sl] = 0u;   // +0.0f: not inserted
-                if ((uint16_t)(bc + yc) < a.q.min_shp) continue;
-                uint32_t bi = bc, yi = yc;
-                if (bi > a.q.max_shdpeaks) { bi = a.q.max_shdpeaks; acc.flags |= HCX_RES_LGF_CLAMP; }
-                if (yi > a.q.max_shdpeaks) { yi = a.q.max_shdpeaks; acc.flags |= HCX_RES_LGF_CLAMP; }
-                double t = sm.lgf[bi] + sm.lgf[yi];
-                t = t + log10((double)(1ull + ib));
-                t = t + log10((double)(1ull + iy));
-                t = t - 6.0;
-                const float hs = (float)t;
-                if (!(hs > 0.0f)) continue;
-                if (ORDERED) {
-                    sm.slot_hs[sl] = __float_as_uint(hs);
-                    continue;
-                }
-                acc.cpsms++;
-                int bin = (int)((double)(hs * 10.0f) + 0.5);
-                if (bin >= (int)a.q.hbins) { bin = (int)a.q.hbins - 1; acc.flags |= HCX_RES_HIST_CLAMP; }
-                lds_add(&sm.hist[bin], 1u);
-                if (hs > acc.best_hs) {
-                    acc.best_hs = hs;
-                    acc.best_cnt = 1;
-                    acc.best_ixx = (uint32_t)ixx;
-                    acc.best_psid = base + loc;
-                    acc.best_bc = bc;
-                    acc.best_yc = yc;
-                } else if (hs == acc.best_hs) {
-                    acc.best_cnt++;
+        }
+        // S: score the slots
+        for (uint32_t sl = threadIdx.x; sl < ns; sl += kBT) {
+            const uint32_t loc = sm.slot_loc[sl];
+            const uint32_t cw = sm.slot_cnt[sl];
+            const uint32_t bc = cw & 0xFFFFu, yc = cw >> 16;
+            const unsigned long long ib = sm.slot_sum[2 * sl], iy = sm.slot_sum[2 * sl + 1];
+            if (ORDERED) sm.slot_hs[sl] = 0u;   // +0.0f: not inserted
+            if ((uint16_t)(bc + yc) < a.q.min_shp) continue;
+            uint32_t bi = bc, yi = yc;
+            if (bi > a.q.max_shdpeaks) { bi = a.q.max_shdpeaks; acc.flags |= HCX_RES_LGF_CLAMP; }
+            if (yi > a.q.max_shdpeaks) { yi = a.q.max_shdpeaks; acc.flags |= HCX_RES_LGF_CLAMP; }
+            double t = sm.lgf[bi] + sm.lgf[yi];
+            t = t + log10((double)(1ull + ib));
+            t = t + log10((double)(1ull + iy));
+            t = t - 6.0;
+            const float hs = (float)t;
+            if (!(hs > 0.0f)) continue;
+            if (ORDERED) {
+                sm.slot_hs[sl] = __float_as_uint(hs);
+                continue;
+            }
+            acc.cpsms++;
+            int bin = (int)((double)(hs * 10.0f) + 0.5);
+            if (bin >= (int)a.q.hbins) { bin = (int)a.q.hbins - 1; acc.flags |= HCX_RES_HIST_CLAMP; }
+            lds_add(&sm.hist[bin], 1u);
+            if (hs > acc.best_hs) {
+                acc.best_hs = hs;
+                acc.best_cnt = 1;
+                acc.best_ixx = (uint32_t)ixx;
+                acc.best_psid = base + loc;
+                acc.best_bc = bc;
+                acc.best_yc = yc;
+            } else if (hs == acc.best_hs) {
+                acc.best_cnt++;
+            }
+        }
+        __syncthreads();
+        if (ORDERED) {
+            if (threadIdx.x == 0) {
+                for (uint32_t sl = 0; sl < ns; sl++) {
+                    const float hs = __uint_as_float(sm.slot_hs[sl]);
+                    if (!(hs > 0.0f)) continue;
+                    const uint32_t cw = sm.slot_cnt[sl];
+                    const Cell e = {hs, (int32_t)(base + sm.slot_loc[sl]), (uint32_t)ixx, cw & 0xFFFFu, cw >> 16};
+                    heap_insert(*heap, hsize, a.q.heap_cap, e);
                 }
             }
             __syncthreads();
-            if (ORDERED) {
-                if (threadIdx.x == 0) {
-                    for (uint32_t sl = 0; sl < ns; sl++) {
-                        const float hs = __uint_as_float(sm.slot_hs[sl]);
-                        if (!(hs > 0.0f)) continue;
-                        const uint32_t cw = sm.slot_cnt[sl];
-                        const Cell e = {hs, (int32_t)(base + sm.slot_loc[sl]), (uint32_t)ixx, cw & 0xFFFFu, cw >> 16};
-                        heap_insert(*heap, hsize, a.q.heap_cap, e);
-                    }
-                }
-                __syncthreads();
-            }
         }
-        const bool more = s_ctl[1] != 0;
-        __syncthreads();
-        if (!more) break;
-        if (threadIdx.x == 0) {
-            s_ctl[0] = 0;
-            s_ctl[1] = 0;
-        }
-        __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        s_ctl[0] = 0;
-        s_ctl[1] = 0;
-    }
-    __syncthreads();
 }
 
 // Merge a block's accumulators for spectrum s into the global SpecAcc.
@@ -651,7 +626,7 @@ __device__ __forceinline__ uint32_t load_peaks(const ChunkArgs &a, const Smem &s
 
 __device__ __forceinline__ void init_smem(const ChunkArgs &a, const Smem &sm) {
     for (uint32_t i = threadIdx.x; i <= a.q.max_shdpeaks; i += kBT) sm.lgf[i] = a.q.lgf[i];
-    for (uint32_t i = threadIdx.x; i < a.tc / 2; i += kBT) sm.map[i] = 0;
+    for (uint32_t i = threadIdx.x; i < a.tc / 4; i += kBT) sm.map[i] = 0;
     for (uint32_t i = threadIdx.x; i < a.q.hbins; i += kBT) sm.hist[i] = 0;
 }
 
@@ -667,7 +642,6 @@ __global__ __launch_bounds__(kBT) void k_spec_chunked(ChunkArgs a) {
     init_smem(a, sm);
     if (threadIdx.x == 0) {
         s_ctl[0] = 0;
-        s_ctl[1] = 0;
     }
     const uint32_t s = blockIdx.x;
     const uint32_t P = load_peaks(a, sm, s);
@@ -705,7 +679,6 @@ __global__ __launch_bounds__(kBT) void k_items_chunked(ChunkArgs a) {
     init_smem(a, sm);
     if (threadIdx.x == 0) {
         s_ctl[0] = 0;
-        s_ctl[1] = 0;
     }
     Acc acc;
     acc_reset(acc);
@@ -748,7 +721,6 @@ __global__ __launch_bounds__(kBT) void k_replay_chunked(ChunkArgs a) {
     init_smem(a, sm);
     if (threadIdx.x == 0) {
         s_ctl[0] = 0;
-        s_ctl[1] = 0;
     }
     const uint32_t nwork = ((const AS_GLB uint32_t *)a.q.tie)[0];
     const uint32_t D = a.q.max_mass + 2;

@@ -114,6 +114,8 @@ typedef struct hcx_result {
 #define HCX_RES_FIT_UB       0x0020u /* tail fit hit an undefined path of the reference */
 #define HCX_RES_SURROGATE    0x0040u /* e-value from the surrogate model               */
 #define HCX_RES_AUDITED      0x0080u /* spectrum was in the surrogate audit sample      */
+#define HCX_RES_COUNT_OVF    0x0100u /* a chunk hit counter neared overflow: rerun the
+                                        spectrum with HCX_KERNEL_CURSOR                  */
 
 /* Per-spectrum work counters (the algorithmic-bytes inputs). */
 typedef struct hcx_stats {
