@@ -97,9 +97,10 @@ extern "C" int hcx_create(const hcx_params *params, int device, hcx_ctx **out) {
     c->num_cu = prop.multiProcessorCount;
     if (hipDeviceGetAttribute(&c->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
         c->max_lds = 65536;
-    // one byte of hit counter per peptide of a chunk; leave room for slots,
-    // peaks, run table and the histogram
-    c->chunk = c->max_lds >= 96 * 1024 ? 32768 : 16384;
+    // one byte of hit counter per peptide of a chunk; the rest of a
+    // workgroup's LDS share holds slots, peaks, run table, histogram and the
+    // hit list (16384 measured best on a 160 KB CU with two workgroups each)
+    c->chunk = c->max_lds >= 96 * 1024 ? 16384 : 8192;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail("stream");
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess)
@@ -138,8 +139,8 @@ extern "C" int hcx_destroy(hcx_ctx *c) {
 extern "C" int hcx_set_tuning(hcx_ctx *c, uint32_t chunk, uint32_t slots, uint32_t blocks_per_cu) {
     if (!c) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_set_tuning: null context");
     if (chunk) {
-        if (chunk < 256 || chunk > 65536 || (chunk & (chunk - 1)))
-            return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_set_tuning: chunk must be a power of two in [256, 65536]");
+        if (chunk < 256 || chunk > 32768 || (chunk & (chunk - 1)))
+            return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_set_tuning: chunk must be a power of two in [256, 32768]");
         c->chunk = chunk;
     }
     if (slots) {
@@ -151,6 +152,30 @@ extern "C" int hcx_set_tuning(hcx_ctx *c, uint32_t chunk, uint32_t slots, uint32
         c->blocks_per_cu = blocks_per_cu;
     }
     return HCX_SUCCESS;
+}
+
+extern "C" int hcx_set_option(hcx_ctx *c, int32_t option, int64_t value) {
+    if (!c) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_set_option: null context");
+    switch (option) {
+    case HCX_OPT_HITLIST:
+        c->hitlist = (int32_t)std::max<int64_t>(-1, std::min<int64_t>(value, 1 << 24));
+        return HCX_SUCCESS;
+    case HCX_OPT_ITEM_ORDER:
+        if (value != 0 && value != 1) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_set_option: item order is 0 or 1");
+        if (c->item_order != (int32_t)value) c->item_order = (int32_t)value, ++c->item_gen;
+        return HCX_SUCCESS;
+    default:
+        return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_set_option: unknown option");
+    }
+}
+
+extern "C" int hcx_get_option(const hcx_ctx *c, int32_t option, int64_t *value) {
+    if (!c || !value) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_get_option: null argument");
+    switch (option) {
+    case HCX_OPT_HITLIST: *value = c->hitlist; return HCX_SUCCESS;
+    case HCX_OPT_ITEM_ORDER: *value = c->item_order; return HCX_SUCCESS;
+    default: return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_get_option: unknown option");
+    }
 }
 
 extern "C" int hcx_get_tuning(const hcx_ctx *c, uint32_t *chunk, uint32_t *slots, uint32_t *blocks_per_cu,
@@ -258,6 +283,10 @@ extern "C" int hcx_index_build(hcx_ctx *c, uint32_t peplen, const char *seqs, ui
         return fail(set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_index_build: chunk offsets allocation failed"));
     rc = launch_chunk_offsets(c, (const uint32_t *)ix->iA.p, (const uint32_t *)ix->bA.p, c->NB, ix->nch, ix->chunk,
                               speclen, (uint32_t *)ix->bA2.p);
+    if (rc) return fail(rc);
+    if (ix->iAc.ensure(2 * nions + 16))
+        return fail(set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_index_build: chunk-local entry allocation failed"));
+    rc = launch_chunk_local(c, (const uint32_t *)ix->iA.p, nions, speclen, ix->chunk, (uint16_t *)ix->iAc.p);
     if (rc) return fail(rc);
     if (hipEventRecord(c->ev1, c->stream) != hipSuccess || hipEventSynchronize(c->ev1) != hipSuccess)
         return fail(set_error("event"));
@@ -371,11 +400,6 @@ extern "C" int hcx_queries_free(hcx_qbatch *b) {
 
 namespace {
 
-__global__ void k_zero_u32(uint32_t *p, uint32_t n) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
-}
-
 // Fill QueryArgs for a search; also sizes the batch's result buffers.
 int prepare(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, const hcx_search_opts *opts,
             QueryArgs *a) {
@@ -384,6 +408,7 @@ int prepare(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, const h
     if (nidx < 0 || nidx > 64) return set_error_code(HCX_ERR_INVLD_PARAM, "search: 0..64 indices");
     const hcx_params &p = c->params;
     std::vector<LenDesc> lens(nidx > 0 ? nidx : 1);
+    uint32_t nbk = 0;
     for (int i = 0; i < nidx; i++) {
         const hcx_index *ix = idx[i];
         if (!ix || ix->ctx != c) return set_error_code(HCX_ERR_INVLD_PARAM, "search: bad index");
@@ -398,8 +423,12 @@ int prepare(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, const h
         L.peplen = ix->peplen;
         make_divider(ix->speclen, &L.div_m, &L.div_s);
         L.nch = ix->nch;
-        L.pad0 = 0;
+        L.cbase = nbk;
+        nbk += ix->nch;
+        if (ix->nch > 0xFFFF || nbk >= (1u << 30))
+            return set_error_code(HCX_ERR_INVLD_SIZE, "search: too many peptide chunks for the chunk size");
         L.bA2 = (const uint32_t *)ix->bA2.p;
+        L.iAc = (const uint16_t *)ix->iAc.p;
         if (ix->chunk != c->chunk)
             return set_error_code(HCX_ERR_INVLD_PARAM, "search: index built with another chunk size");
         if (ix->n >= (1u << 25)) return set_error_code(HCX_ERR_INVLD_SIZE, "search: more than 2^25 entries in a length");
@@ -452,8 +481,8 @@ int prepare(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, const h
     return HCX_SUCCESS;
 }
 
-uint64_t items_key(hcx_index *const *idx, int nidx, uint32_t chunk) {
-    uint64_t h = 1469598103934665603ull ^ chunk;
+uint64_t items_key(hcx_index *const *idx, int nidx, uint32_t chunk, uint32_t gen) {
+    uint64_t h = (1469598103934665603ull ^ chunk) * 1099511628211ull ^ gen;
     for (int i = 0; i < nidx; i++) {
         h ^= (uint64_t)(uintptr_t)idx[i];
         h *= 1099511628211ull;
@@ -476,6 +505,17 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
     ca.slots = c->slots;
     ca.max_peaks = std::max<uint32_t>(1, qb->max_peaks);
     ca.group = std::min<uint32_t>(1024, ca.max_peaks * (2 * c->params.dF + 1));
+    // the hit list takes the LDS left in a workgroup's share of the CU once
+    // blocks_per_cu workgroups are resident (less a reserve for the kernels'
+    // static LDS); its records hold the 16-bit entry and a 16-bit peak index
+    ca.hl_cap = 0;
+    if (ca.max_peaks < 0xFFFF) {
+        const size_t base = chunk_smem_bytes(ca);
+        const size_t share = (size_t)c->max_lds / std::max<uint32_t>(1, c->blocks_per_cu);
+        const size_t reserve = 8192;
+        if (share > base + reserve + 4096) ca.hl_cap = (uint32_t)(((share - base - reserve) / 4) & ~(size_t)63);
+        if (c->hitlist >= 0) ca.hl_cap = std::min<uint32_t>(ca.hl_cap, (uint32_t)c->hitlist & ~63u);
+    }
     ca.grid = (uint32_t)std::max(1, c->num_cu) * c->blocks_per_cu;
     if (chunk_smem_bytes(ca) > (size_t)c->max_lds)
         return set_error_code(HCX_ERR_INVLD_PARAM, "search: chunk/slots do not fit in LDS");
@@ -486,18 +526,22 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
         if ((uint64_t)maxspec * qb->max_mult >= 0x8000)
             return set_error_code(HCX_ERR_INVLD_PARAM, "search: too many peaks within +-dF of one bin");
     }
+    for (int i = 0; i < nidx; i++) ca.nbk += idx[i]->nch;
+    ca.by_chunk = c->item_order == 1 ? 1u : 0u;
     if (qb->acc.ensure(sizeof(SpecAcc) * n + 16) || c->work.ensure(64) ||
-        qb->counts.ensure(4 * (n + 1) + 16) || c->scan_tmp.ensure(4 * scan_tmp_words(n + 1) + 64))
+        qb->counts.ensure(8 * (std::max<uint64_t>(n, ca.nbk) + 1) + 16) ||
+        c->scan_tmp.ensure(4 * scan_tmp_words(std::max<uint64_t>(n, ca.nbk) + 1) + 64))
         return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "search: scratch allocation failed");
     ca.acc = (SpecAcc *)qb->acc.p;
     ca.queue = (uint32_t *)c->work.p;
     if (kernel == HCX_KERNEL_BATCHED) {
-        const uint64_t key = items_key(idx, nidx, c->chunk);
+        const uint64_t key = items_key(idx, nidx, c->chunk, c->item_gen);
         if (qb->items_key != key || !qb->items.p) {
             rc = launch_items(c, ca, (uint32_t *)qb->counts.p, (uint32_t *)c->scan_tmp.p, false);
             if (rc) return rc;
             uint32_t total = 0;
-            HCX_HIP(hipMemcpyAsync(&total, (uint32_t *)qb->counts.p + n, 4, hipMemcpyDeviceToHost, c->stream));
+            const uint64_t nbuckets = ca.by_chunk ? ca.nbk : n;
+            HCX_HIP(hipMemcpyAsync(&total, (uint32_t *)qb->counts.p + nbuckets, 4, hipMemcpyDeviceToHost, c->stream));
             HCX_HIP(hipStreamSynchronize(c->stream));
             if (qb->items.ensure(sizeof(Item) * (size_t)total + 16))
                 return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "search: item list allocation failed");

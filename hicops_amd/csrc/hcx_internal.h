@@ -35,8 +35,9 @@ struct LenDesc {
     uint32_t div_m;         // magic multiplier for raw / speclen
     uint32_t div_s;         // shift
     uint32_t nch;           // peptide chunks (of ctx->chunk peptides)
-    uint32_t pad0;
+    uint32_t cbase;         // first (length, chunk) work bucket of this length
     const uint32_t *bA2;    // NB * (nch + 1): start of chunk c inside bin b
+    const uint16_t *iAc;    // n * speclen: (peptide mod chunk) << 1 | isY, iA order
 };
 
 struct DeviceBuf {
@@ -68,9 +69,12 @@ struct hcx_ctx {
     hcx::DeviceBuf sur_tmp;
     hcx::DeviceBuf scan_tmp;          // device scan scratch
     bool sur_ready = false;
-    uint32_t chunk = 32768;           // peptides per query chunk (power of two)
-    uint32_t slots = 512;             // scored-peptide slots per chunk pass
+    uint32_t chunk = 16384;           // peptides per query chunk (power of two)
+    uint32_t slots = 1024;            // scored-peptide slots per chunk pass
     uint32_t blocks_per_cu = 2;       // persistent grid of the batched kernel
+    int32_t hitlist = -1;             // LDS hit-list entries: < 0 as many as fit
+    int32_t item_order = 0;           // batched work items: 0 by spectrum, 1 by (length, chunk)
+    uint32_t item_gen = 0;            // bumped when item_order changes (cached item lists)
     int max_lds = 65536;              // device limit per workgroup
     // last launch shape (for graph reuse)
     hipGraphExec_t graph_exec = nullptr;
@@ -83,7 +87,7 @@ struct hcx_index {
     uint64_t nions = 0;
     float build_ms = 0.0f;
     uint32_t chunk = 0, nch = 0;
-    hcx::DeviceBuf entries, mass, iA, bA, mdir, seqs, bA2;
+    hcx::DeviceBuf entries, mass, iA, bA, mdir, seqs, bA2, iAc;
 };
 
 struct hcx_qbatch {
@@ -157,17 +161,20 @@ struct ChunkArgs {
     QueryArgs q;
     uint32_t tc, slots, max_peaks, grid;
     uint32_t group;       // probes per run-table group
-    uint32_t pad_;
+    uint32_t hl_cap;      // LDS hit-list entries (0: none); needs fewer than 65535 peaks
     SpecAcc *acc;
     const Item *items;
     Item *items_w;
     uint32_t nitems;
     uint32_t *queue;
+    uint32_t nbk;         // (length, chunk) buckets: sum of nch
+    uint32_t by_chunk;    // items bucketed by (length, chunk) instead of spectrum
 };
 size_t chunk_smem_bytes(const ChunkArgs &a);
 int launch_chunk_offsets(hcx_ctx *ctx, const uint32_t *iA, const uint32_t *bA, uint32_t NB, uint32_t nch, uint32_t TC,
                          uint32_t speclen, uint32_t *bA2);
 int launch_fill_u32(hcx_ctx *ctx, uint32_t *p, uint64_t n, uint32_t v);
+int launch_chunk_local(hcx_ctx *ctx, const uint32_t *iA, uint64_t n, uint32_t speclen, uint32_t TC, uint16_t *iAc);
 int launch_items(hcx_ctx *ctx, ChunkArgs &a, uint32_t *counts, uint32_t *scan_tmp, bool fill);
 int launch_chunked(hcx_ctx *ctx, ChunkArgs &a, int kernel);
 int launch_replay_chunked(hcx_ctx *ctx, ChunkArgs &a);

@@ -5,14 +5,16 @@
 // id, hence in peptide id, so the entries of one (bin, chunk) pair are a
 // contiguous run; bA2[bin * (nch+1) + c] is where chunk c starts inside bin
 // `bin` (bA2[..+0] = bA[bin], bA2[..+nch] = bA[bin+1]).  iA itself is the
-// reference array, unchanged.
+// reference array, unchanged; the scan reads iAc, its 16-bit chunk-local
+// form (peptide mod TC) << 1 | isY, which needs neither a division nor the
+// ion offset at query time and halves the bytes read.
 //
 // Work item = (spectrum, length, chunk).  The item's (peak, bin) probes are
 // taken in groups of at most `group` probes; for one group:
 //   R   every probe reads its run bounds [bA2[c], bA2[c+1]) and a block scan
 //       of the run lengths gives each thread an equal contiguous share of the
 //       group's entries;
-//   P1  8 independent iA loads per lane at a time; each entry inside the
+//   P1  8 independent iAc loads per lane at a time; each entry inside the
 //       precursor window adds 1 to its peptide's 8-bit hit counter (four
 //       counters per LDS word, saturating so that no byte carries);
 // then, once per item:
@@ -53,7 +55,10 @@ typedef AS_GLB uint32_t gu32;
 typedef AS_GLB unsigned long long gu64;
 
 constexpr int kBT = 512;
+// at most 128 VGPRs: four waves per SIMD, i.e. two kBT workgroups per CU
+constexpr int kWavesPerSimd = 4;
 constexpr uint32_t kNoProbe = 0xFFFFFFFFu;
+constexpr uint32_t kNoHit = 0xFFFFFFFFu;      // hit-list entry outside the window
 constexpr uint32_t kGrab = 4;        // items taken per queue atomic
 constexpr unsigned long long kLow25 = (1ull << 25) - 1;
 
@@ -137,18 +142,28 @@ __device__ __forceinline__ void window(const LenDesc &L, float prec, double dM, 
     *whi = (int32_t)b - 1;
 }
 
+// Values every lane of the workgroup holds alike, moved to scalar registers
+// (an LDS or global read lands in a vector register even when uniform).
+__device__ __forceinline__ uint32_t ufl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int32_t ufl(int32_t v) { return (int32_t)__builtin_amdgcn_readfirstlane((uint32_t)v); }
+template <typename T>
+__device__ __forceinline__ T *ufl(T *p) {
+    const uint64_t v = (uint64_t)(uintptr_t)p;
+    return (T *)(uintptr_t)(((uint64_t)ufl((uint32_t)(v >> 32)) << 32) | ufl((uint32_t)v));
+}
 struct Acc {
     float best_hs;
     uint32_t best_cnt;
-    uint32_t best_ixx, best_psid, best_bc, best_yc;
+    uint32_t best_id;    // length << 25 | peptide id
+    uint32_t best_by;    // bc << 16 | yc
     uint32_t cpsms, flags;
-    unsigned long long nia, nprobe;
+    uint32_t nia, nprobe;   // per thread and spectrum: far below 2^32
 };
 
 __device__ __forceinline__ void acc_reset(Acc &a) {
     a.best_hs = 0.0f;
     a.best_cnt = 0;
-    a.best_ixx = a.best_psid = a.best_bc = a.best_yc = 0;
+    a.best_id = a.best_by = 0;
     a.cpsms = a.flags = 0;
     a.nia = a.nprobe = 0;
 }
@@ -163,12 +178,14 @@ struct Layout {
     uint32_t slot_hs;   // slots: hyperscore bits (replay)
     uint32_t slot_sum;  // 2*slots u64: ibc, iyc
     uint32_t pk_moz, pk_int;   // max_peaks
-    uint32_t prs;       // group: run start of each probe in iA
-    uint32_t pofs;      // group + 1: exclusive prefix of the run lengths
-    uint32_t wsum;      // kBT / 64 wave partial sums
+    uint32_t crs;       // group: iA start of each non-empty run
+    uint32_t cofs;      // group + 1: flat offset of each non-empty run (exclusive prefix)
+    uint32_t ck;        // group: peak index of each non-empty run
+    uint32_t wsum;      // 2 * (kBT / 64) wave partial sums + 2 totals
     uint32_t hist;      // hbins
     uint32_t lgf;       // max_shdpeaks + 1 doubles
     uint32_t wlo, whi;  // nlen
+    uint32_t hl;        // hl_cap: hit list of the current item
     uint32_t total;
 };
 
@@ -191,19 +208,21 @@ __host__ __device__ inline Layout make_layout(const ChunkArgs &a) {
     l.slot_sum = take(16u * a.slots);
     l.pk_moz = take(4u * a.max_peaks);
     l.pk_int = take(4u * a.max_peaks);
-    l.prs = take(4u * a.group);
-    l.pofs = take(4u * (a.group + 1));
-    l.wsum = take(4u * (kBT / 64 + 1));
+    l.crs = take(4u * a.group);
+    l.cofs = take(4u * (a.group + 1));
+    l.ck = take(4u * a.group);
+    l.wsum = take(4u * (2 * (kBT / 64) + 2));
     l.hist = take(4u * a.q.hbins);
     l.lgf = take(8u * (a.q.max_shdpeaks + 1));
     l.wlo = take(4u * a.q.nlen);
     l.whi = take(4u * a.q.nlen);
+    l.hl = take(4u * a.hl_cap);
     l.total = off;
     return l;
 }
 
 struct Smem {
-    lu32 *map, *bmp, *pre, *slot_cnt, *slot_loc, *slot_hs, *pk_moz, *pk_int, *prs, *pofs, *wsum, *hist;
+    lu32 *map, *bmp, *pre, *slot_cnt, *slot_loc, *slot_hs, *pk_moz, *pk_int, *crs, *cofs, *ck, *wsum, *hist, *hl;
     lu64 *slot_sum;
     lf64 *lgf;
     li32 *wlo, *whi;
@@ -221,13 +240,15 @@ __device__ __forceinline__ Smem bind(unsigned char *raw, const Layout &l) {
     s.slot_sum = (lu64 *)(b + l.slot_sum);
     s.pk_moz = (lu32 *)(b + l.pk_moz);
     s.pk_int = (lu32 *)(b + l.pk_int);
-    s.prs = (lu32 *)(b + l.prs);
-    s.pofs = (lu32 *)(b + l.pofs);
+    s.crs = (lu32 *)(b + l.crs);
+    s.cofs = (lu32 *)(b + l.cofs);
+    s.ck = (lu32 *)(b + l.ck);
     s.wsum = (lu32 *)(b + l.wsum);
     s.hist = (lu32 *)(b + l.hist);
     s.lgf = (lf64 *)(b + l.lgf);
     s.wlo = (li32 *)(b + l.wlo);
     s.whi = (li32 *)(b + l.whi);
+    s.hl = (lu32 *)(b + l.hl);
     return s;
 }
 
@@ -247,91 +268,171 @@ __device__ __forceinline__ uint32_t block_prefix(const Smem &sm, uint32_t tsum) 
     return run;
 }
 
-// Run set-up for the probes [g0, g1) of one item: start and length of each
-// (peak, bin) probe's run in chunk c, then an exclusive scan of the lengths.
-// Returns the group's entry total E (sm.pofs[g1 - g0] == E).
-__device__ __forceinline__ uint32_t setup_runs(const ChunkArgs &a, const Smem &sm, const LenDesc &L, uint32_t c,
-                                               uint32_t g0, uint32_t g1, bool count_probes, Acc &acc) {
+// Block-wide exclusive scan of two values at once (thread t holds the sums
+// of its own contiguous group); returns t's prefixes, totals in *tot.
+__device__ __forceinline__ uint2 block_prefix2(const Smem &sm, uint32_t a0, uint32_t a1, uint2 *tot) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    constexpr int NW = kBT / 64;
+    uint32_t x = a0, y = a1;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t u = __shfl_up(x, off, 64), v = __shfl_up(y, off, 64);
+        if (lane >= off) {
+            x += u;
+            y += v;
+        }
+    }
+    if (lane == 63) {
+        sm.wsum[w] = x;
+        sm.wsum[NW + w] = y;
+    }
+    __syncthreads();
+    uint32_t rx = x - a0, ry = y - a1, tx = 0, ty = 0;
+    for (int ww = 0; ww < NW; ww++) {
+        const uint32_t sx = sm.wsum[ww], sy = sm.wsum[NW + ww];
+        if (ww < w) {
+            rx += sx;
+            ry += sy;
+        }
+        tx += sx;
+        ty += sy;
+    }
+    *tot = make_uint2(tx, ty);
+    return make_uint2(rx, ry);
+}
+
+// Run table of the probes [g0, g1) of one item (at most 2 * kBT probes):
+// each (peak, bin) probe's run [bA2[c], bA2[c+1]) in chunk c; the non-empty
+// runs are compacted, in probe order, into crs (iA start), cofs (flat offset,
+// cofs[nc] = E) and ck (the probe's peak).  Returns (E, nc), uniform.
+__device__ __forceinline__ uint2 setup_runs(const ChunkArgs &a, const Smem &sm, const LenDesc &L, uint32_t c,
+                                            uint32_t g0, uint32_t g1, bool count_probes, Acc &acc) {
     const uint32_t W = 2 * a.q.dF + 1;
     const uint32_t n = g1 - g0;
     const uint32_t stride = L.nch + 1;
-    const uint32_t per = (n + kBT - 1) / kBT;
-    const uint32_t p0 = threadIdx.x * per;
-    const uint32_t p1 = min(n, p0 + per);
-    uint32_t tsum = 0;
-    for (uint32_t lp = p0; lp < p1; lp++) {
-        const uint32_t p = g0 + lp;
-        const uint32_t k = p / W, j = p - k * W;
-        const uint32_t q = sm.pk_moz[k];
-        uint32_t rs = 0, len = 0;
-        if (q > a.q.dF && q < a.q.NB - 1 - a.q.dF) {
-            const size_t o = (size_t)(q - a.q.dF + j) * stride + c;
-            rs = ldg(L.bA2, o);
-            len = ldg(L.bA2, o + 1) - rs;
-            if (count_probes) acc.nprobe++;
+    uint32_t rs[2] = {0, 0}, len[2] = {0, 0}, pk[2] = {0, 0};
+#pragma unroll
+    for (uint32_t h = 0; h < 2; h++) {
+        const uint32_t lp = 2 * threadIdx.x + h;
+        if (lp < n) {
+            const uint32_t p = g0 + lp;
+            const uint32_t k = p / W, j = p - k * W;
+            pk[h] = k;
+            const uint32_t q = sm.pk_moz[k];
+            if (q > a.q.dF && q < a.q.NB - 1 - a.q.dF) {
+                const size_t o = (size_t)(q - a.q.dF + j) * stride + c;
+                rs[h] = ldg(L.bA2, o);
+                len[h] = ldg(L.bA2, o + 1) - rs[h];
+                if (count_probes) acc.nprobe++;
+            }
         }
-        sm.prs[lp] = rs;
-        sm.pofs[lp] = len;   // replaced by the prefix below
-        tsum += len;
     }
-    uint32_t run = block_prefix(sm, tsum);
-    for (uint32_t lp = p0; lp < p1; lp++) {
-        const uint32_t len = sm.pofs[lp];
-        sm.pofs[lp] = run;
-        run += len;
+    uint2 tot;
+    const uint2 pre = block_prefix2(sm, (len[0] ? 1u : 0u) + (len[1] ? 1u : 0u), len[0] + len[1], &tot);
+    uint32_t ci = pre.x, fo = pre.y;
+#pragma unroll
+    for (uint32_t h = 0; h < 2; h++) {
+        if (len[h]) {
+            sm.crs[ci] = rs[h];
+            sm.cofs[ci] = fo;
+            sm.ck[ci] = pk[h];
+            ci++;
+            fo += len[h];
+        }
     }
-    if (threadIdx.x == kBT - 1) sm.pofs[n] = run;
+    if (threadIdx.x == 0) sm.cofs[tot.x] = tot.y;
     __syncthreads();
-    return sm.pofs[n];
+    return make_uint2(ufl(tot.y), ufl(tot.x));
 }
 
-// Load-balanced walk over a group's E entries: thread t takes the flat range
-// [t*E/kBT, (t+1)*E/kBT), 8 independent iA loads at a time.  fn(raw, local probe).
-template <typename F>
-__device__ __forceinline__ void walk_flat(const Smem &sm, const uint32_t *iA_, uint32_t NP, uint32_t E, F &&fn) {
-    const uint32_t f0 = (uint32_t)(((uint64_t)E * threadIdx.x) / kBT);
-    const uint32_t f1 = (uint32_t)(((uint64_t)E * (threadIdx.x + 1)) / kBT);
-    if (f0 >= f1) return;
-    uint32_t lo = 0, hi = NP;   // last p with pofs[p] <= f0
+// Walk over the flat entries [fb, E) of a group's (compacted, non-empty)
+// runs in the chunk-local entry array: thread t takes an equal contiguous
+// share and reads it kStep entries at a time.  A step takes entries from at
+// most two consecutive runs, whose bounds and peaks are held in registers;
+// its loads are unconditional (lanes past the step's end reload its first
+// entry), so they issue back to back.
+// body(v[kStep], kA, kB, nA, cnt, f): entries i < nA belong to peak kA,
+// entries nA <= i < cnt to peak kB; entry i has flat index f + i.
+constexpr uint32_t kStep = 8;
+template <typename Body>
+__device__ __forceinline__ void walk_steps(const Smem &sm, const uint16_t *src, uint32_t nc, uint32_t fb, uint32_t E,
+                                           Body &&body) {
+    const uint32_t n = E - fb;
+    uint32_t f = fb + (uint32_t)(((uint64_t)n * threadIdx.x) / kBT);
+    const uint32_t f1 = fb + (uint32_t)(((uint64_t)n * (threadIdx.x + 1)) / kBT);
+    if (f >= f1) return;
+    uint32_t lo = 0, hi = nc;   // run holding f: cofs[lo] <= f < cofs[lo + 1]
     while (hi - lo > 1) {
         const uint32_t m = (lo + hi) >> 1;
-        if (sm.pofs[m] <= f0) lo = m;
+        if (sm.cofs[m] <= f) lo = m;
         else hi = m;
     }
-    // probe p covers flat indices [pbeg, pend) and iA indices [prun, prun + pend - pbeg);
-    // f >= pbeg always, so prun + (f - pbeg) never wraps
-    uint32_t p = lo, pbeg = sm.pofs[p], pend = sm.pofs[p + 1], prun = sm.prs[p];
-    uint32_t f = f0;
-    gcu32 *iA = (gcu32 *)iA_;
-    while (f < f1) {
-#define HCX_GATHER(i)                          \
-    uint32_t r##i = 0, q##i = kNoProbe;        \
-    if (f < f1) {                              \
-        while (f >= pend) {                    \
-            p++;                               \
-            pbeg = pend;                       \
-            pend = sm.pofs[p + 1];             \
-            prun = sm.prs[p];                  \
-        }                                      \
-        r##i = iA[prun + (f - pbeg)];          \
-        q##i = p;                              \
-        f++;                                   \
+    // run r: flat [cofs[r], pend), entry index abase + flat (32-bit
+    // wrap-around arithmetic; the sum is the exact index)
+    uint32_t r = lo;
+    uint32_t pend = sm.cofs[r + 1], abase = sm.crs[r] - sm.cofs[r], kA = sm.ck[r];
+    uint32_t pend2 = pend, abase2 = 0, kB = 0;
+    if (r + 1 < nc) {
+        pend2 = sm.cofs[r + 2];
+        abase2 = sm.crs[r + 1] - pend;
+        kB = sm.ck[r + 1];
     }
-        HCX_GATHER(0) HCX_GATHER(1) HCX_GATHER(2) HCX_GATHER(3)
-        HCX_GATHER(4) HCX_GATHER(5) HCX_GATHER(6) HCX_GATHER(7)
-#undef HCX_GATHER
-        if (q0 != kNoProbe) fn(r0, q0);
-        if (q1 != kNoProbe) fn(r1, q1);
-        if (q2 != kNoProbe) fn(r2, q2);
-        if (q3 != kNoProbe) fn(r3, q3);
-        if (q4 != kNoProbe) fn(r4, q4);
-        if (q5 != kNoProbe) fn(r5, q5);
-        if (q6 != kNoProbe) fn(r6, q6);
-        if (q7 != kNoProbe) fn(r7, q7);
+    const AS_GLB uint16_t *e = (const AS_GLB uint16_t *)src;
+    while (f < f1) {
+        const uint32_t m = min(kStep, f1 - f);
+        const uint32_t nA = min(m, pend - f);
+        const uint32_t cnt = nA + min(m - nA, pend2 - pend);
+        uint32_t v[kStep];
+#pragma unroll
+        for (uint32_t i = 0; i < kStep; i++) {
+            const uint32_t j = i < cnt ? i : 0u;
+            v[i] = e[(j < nA ? abase : abase2) + f + j];
+        }
+        const uint32_t fs = f, ka = kA, kb = kB;
+        f += cnt;
+        if (f < f1 && f >= pend) {
+            // runs are non-empty: at most one more run was consumed whole
+            r++;
+            pend = pend2;
+            abase = abase2;
+            kA = kB;
+            if (f >= pend) {
+                r++;
+                abase = sm.crs[r] - pend;
+                kA = sm.ck[r];
+                pend = sm.cofs[r + 1];
+            }
+            pend2 = pend;
+            if (r + 1 < nc) {
+                pend2 = sm.cofs[r + 2];
+                abase2 = sm.crs[r + 1] - pend;
+                kB = sm.ck[r + 1];
+            }
+        }
+        body(v, ka, kb, nA, cnt, fs);
     }
 }
 
 __device__ __forceinline__ LenDesc load_len(const ChunkArgs &a, int ixx) { return a.q.lens[ixx]; }
+
+// the descriptor of length ixx, ixx uniform
+__device__ __forceinline__ LenDesc load_len_u(const ChunkArgs &a, uint32_t ixx) {
+    LenDesc L = a.q.lens[ufl(ixx)];
+    L.mass = ufl(L.mass);
+    L.iA = ufl(L.iA);
+    L.bA = ufl(L.bA);
+    L.mdir = ufl(L.mdir);
+    L.n = ufl(L.n);
+    L.speclen = ufl(L.speclen);
+    L.half = ufl(L.half);
+    L.peplen = ufl(L.peplen);
+    L.div_m = ufl(L.div_m);
+    L.div_s = ufl(L.div_s);
+    L.nch = ufl(L.nch);
+    L.cbase = ufl(L.cbase);
+    L.bA2 = ufl(L.bA2);
+    L.iAc = ufl(L.iAc);
+    return L;
+}
 
 // ---- the reference top-K heap (minheap.h:108-177, 258-277), in LDS ----
 struct Cell {
@@ -400,12 +501,12 @@ template <bool ORDERED>
 __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm, lu32 *s_ctl, uint32_t P, int ixx,
                                              uint32_t c, int32_t lo, int32_t hi, Acc &acc, const Heap *heap = nullptr,
                                              int *hsize = nullptr) {
-    const LenDesc L = load_len(a, ixx);
+    const LenDesc L = load_len_u(a, (uint32_t)ixx);
     const uint32_t TC = a.tc;
     const uint32_t base = c * TC;
     const uint32_t ilo = max((uint32_t)lo, base);
     const uint32_t ihi = min((uint32_t)hi, base + TC - 1);
-    const uint32_t speclen = L.speclen, half = L.half;
+    const uint32_t wlo = ilo - base, wspan = ihi - ilo;   // window in chunk-local ids
     const bool first = (uint32_t)lo / TC == c;
     const uint32_t W = 2 * a.q.dF + 1;
     const uint32_t NP = P * W;
@@ -417,22 +518,51 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
     // the byte never carries into its neighbour unless more than 0x7F other
     // increments of the same peptide land between one lane's add and its
     // pull-back; a byte seen at >= 0xC0 (halfway there) flags the spectrum.
-    uint32_t E = 0;
+    // Entries are chunk-local: local << 1 | isY.  The first hl_cap flat
+    // entries of the item (in group order) are also recorded in the LDS hit
+    // list as entry | peak << 16, or kNoHit outside the window, so that P2
+    // need not read them again.
+    uint32_t E = 0, nc = 0, hl_n = 0;
+    uint32_t g_part = ng, l_part = 0;   // first group not wholly listed, its listed prefix
+    const uint32_t lane = threadIdx.x & 63u;
     for (uint32_t g = 0; g < ng; g++) {
         const uint32_t g0 = g * GP, g1 = min(NP, g0 + GP);
-        E = setup_runs(a, sm, L, c, g0, g1, first, acc);
-        walk_flat(sm, L.iA, g1 - g0, E, [&](uint32_t raw, uint32_t) {
-            const uint32_t ppid = udiv(raw, L.div_m, L.div_s);
-            if (ppid >= ilo && ppid <= ihi) {
-                const uint32_t local = ppid - base;
-                const uint32_t sh = (local & 3u) << 3;
-                const uint32_t old = lds_add_rtn(&sm.map[local >> 2], 1u << sh);
-                const uint32_t ob = (old >> sh) & 0xFFu;
-                if (ob == 0x7Fu) lds_add(&sm.map[local >> 2], 0u - (0x40u << sh));
-                else if (ob >= 0xC0u) acc.flags |= HCX_RES_COUNT_OVF;
-                acc.nia++;
+        const uint2 en = setup_runs(a, sm, L, c, g0, g1, first, acc);
+        E = en.x;
+        nc = en.y;
+        const uint32_t lim = min(E, a.hl_cap - hl_n);
+        lu32 *hl = sm.hl + hl_n;
+        walk_steps(sm, L.iAc, nc, 0, E,
+                   [&](const uint32_t (&v)[kStep], uint32_t ka, uint32_t kb, uint32_t nA, uint32_t cnt, uint32_t fs) {
+            uint32_t old[kStep], win = 0;
+#pragma unroll
+            for (uint32_t i = 0; i < kStep; i++)
+                if (i < cnt && (v[i] >> 1) - wlo <= wspan) win |= 1u << i;
+            // every lane adds (0 outside the window, to a word of its own)
+#pragma unroll
+            for (uint32_t i = 0; i < kStep; i++) {
+                const bool k = (win >> i) & 1u;
+                const uint32_t local = v[i] >> 1;
+                old[i] = lds_add_rtn(&sm.map[k ? local >> 2 : lane], k ? 1u << ((local & 3u) << 3) : 0u);
             }
+#pragma unroll
+            for (uint32_t i = 0; i < kStep; i++) {
+                const uint32_t local = v[i] >> 1;
+                if ((win >> i) & 1u) {
+                    const uint32_t sh = (local & 3u) << 3;
+                    const uint32_t ob = (old[i] >> sh) & 0xFFu;
+                    if (ob == 0x7Fu) lds_add(&sm.map[local >> 2], 0u - (0x40u << sh));
+                    else if (ob >= 0xC0u) acc.flags |= HCX_RES_COUNT_OVF;
+                }
+                if (i < cnt && fs + i < lim) hl[fs + i] = ((win >> i) & 1u) ? v[i] | ((i < nA ? ka : kb) << 16) : kNoHit;
+            }
+            acc.nia += __popc(win);
         });
+        if (lim < E && g_part == ng) {
+            g_part = g;
+            l_part = lim;
+        }
+        hl_n += lim;
         __syncthreads();
     }
 
@@ -467,7 +597,7 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
         if (threadIdx.x == kBT - 1) s_ctl[0] = run;
         __syncthreads();
     }
-    const uint32_t Qn = s_ctl[0];
+    const uint32_t Qn = ufl((uint32_t)s_ctl[0]);
 
     for (uint32_t r0 = 0; r0 < Qn; r0 += a.slots) {
         const uint32_t ns = min(a.slots, Qn - r0);
@@ -477,29 +607,40 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
             sm.slot_sum[2 * sl + 1] = 0;
         }
         __syncthreads();
-        // P2: exact b/y counts and intensity sums of slots [r0, r0 + ns)
-        for (uint32_t g = 0; g < ng; g++) {
-            const uint32_t g0 = g * GP, g1 = min(NP, g0 + GP);
-            if (ng > 1) E = setup_runs(a, sm, L, c, g0, g1, false, acc);
-            walk_flat(sm, L.iA, g1 - g0, E, [&](uint32_t raw, uint32_t lp) {
-                const uint32_t ppid = udiv(raw, L.div_m, L.div_s);
-                if (ppid >= ilo && ppid <= ihi) {
-                    const uint32_t local = ppid - base;
-                    const uint32_t w = local >> 5, bit = local & 31u;
-                    const uint32_t bits = sm.bmp[w];
-                    if ((bits >> bit) & 1u) {
-                        const uint32_t sl = sm.pre[w] + __popc(bits & ((1u << bit) - 1u)) - r0;
-                        if (sl < ns) {
-                            const uint32_t isY = (raw - ppid * speclen) >= half ? 1u : 0u;
-                            lds_add(&sm.slot_cnt[sl], isY ? 0x10000u : 1u);
-                            lds_add64(&sm.slot_sum[2 * sl + isY], (unsigned long long)sm.pk_int[(g0 + lp) / W]);
-                            sm.slot_loc[sl] = local;
-                        }
-                    }
+        // P2: exact b/y counts and intensity sums of slots [r0, r0 + ns),
+        // first from the hit list, then from the entries not listed
+        auto tally = [&](uint32_t local, uint32_t isY, uint32_t k) {
+            const uint32_t w = local >> 5, bit = local & 31u;
+            const uint32_t bits = sm.bmp[w];
+            if ((bits >> bit) & 1u) {
+                const uint32_t sl = sm.pre[w] + __popc(bits & ((1u << bit) - 1u)) - r0;
+                if (sl < ns) {
+                    lds_add(&sm.slot_cnt[sl], isY ? 0x10000u : 1u);
+                    lds_add64(&sm.slot_sum[2 * sl + isY], (unsigned long long)sm.pk_int[k]);
+                    sm.slot_loc[sl] = local;
                 }
-            });
-            __syncthreads();
+            }
+        };
+        for (uint32_t i = threadIdx.x; i < hl_n; i += kBT) {
+            const uint32_t rec = sm.hl[i];
+            if (rec != kNoHit) tally((rec & 0xFFFFu) >> 1, rec & 1u, rec >> 16);
         }
+        for (uint32_t g = g_part; g < ng; g++) {
+            const uint32_t g0 = g * GP, g1 = min(NP, g0 + GP);
+            if (ng > 1) {
+                const uint2 en = setup_runs(a, sm, L, c, g0, g1, false, acc);
+                E = en.x;
+                nc = en.y;
+            }
+            walk_steps(sm, L.iAc, nc, g == g_part ? l_part : 0u, E,
+                       [&](const uint32_t (&v)[kStep], uint32_t ka, uint32_t kb, uint32_t nA, uint32_t cnt, uint32_t) {
+#pragma unroll
+                for (uint32_t i = 0; i < kStep; i++)
+                    if (i < cnt && (v[i] >> 1) - wlo <= wspan) tally(v[i] >> 1, v[i] & 1u, i < nA ? ka : kb);
+            });
+            if (ng > 1) __syncthreads();
+        }
+        __syncthreads();
         // S: score the slots
         for (uint32_t sl = threadIdx.x; sl < ns; sl += kBT) {
             const uint32_t loc = sm.slot_loc[sl];
@@ -528,10 +669,8 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
             if (hs > acc.best_hs) {
                 acc.best_hs = hs;
                 acc.best_cnt = 1;
-                acc.best_ixx = (uint32_t)ixx;
-                acc.best_psid = base + loc;
-                acc.best_bc = bc;
-                acc.best_yc = yc;
+                acc.best_id = ((uint32_t)ixx << 25) | (base + loc);
+                acc.best_by = (bc << 16) | yc;
             } else if (hs == acc.best_hs) {
                 acc.best_cnt++;
             }
@@ -568,16 +707,15 @@ __device__ __forceinline__ void flush_spectrum(const ChunkArgs &a, const Smem &s
     if (acc.best_cnt) lds_max(&s_red[0], __float_as_uint(acc.best_hs));
     if (acc.cpsms) lds_add(&s_red[2], acc.cpsms);
     if (acc.flags) lds_or(&s_red[3], acc.flags);
-    if (acc.nia) lds_add64(&s_red64[0], acc.nia);
-    if (acc.nprobe) lds_add64(&s_red64[1], acc.nprobe);
+    if (acc.nia) lds_add64(&s_red64[0], (unsigned long long)acc.nia);
+    if (acc.nprobe) lds_add64(&s_red64[1], (unsigned long long)acc.nprobe);
     __syncthreads();
     SpecAcc *g = a.acc + s;
     const uint32_t mb = s_red[0];
     if (acc.best_cnt && __float_as_uint(acc.best_hs) == mb) {
         lds_add(&s_red[1], acc.best_cnt);
-        glb_max64(&g->top_a, ((unsigned long long)mb << 32) | ((unsigned long long)acc.best_ixx << 25) |
-                                 (acc.best_psid & kLow25));
-        glb_max64(&g->top_b, ((unsigned long long)mb << 32) | ((unsigned long long)acc.best_bc << 16) | acc.best_yc);
+        glb_max64(&g->top_a, ((unsigned long long)mb << 32) | acc.best_id);
+        glb_max64(&g->top_b, ((unsigned long long)mb << 32) | acc.best_by);
     }
     // histogram: non-zero bins to global, then clear
     uint32_t *hg = a.q.hist + (size_t)s * a.q.hbins;
@@ -616,7 +754,7 @@ __device__ __forceinline__ void flush_spectrum(const ChunkArgs &a, const Smem &s
 __device__ __forceinline__ uint32_t load_peaks(const ChunkArgs &a, const Smem &sm, uint32_t s) {
     gcu32 *qidx = (gcu32 *)a.q.qidx, *moz = (gcu32 *)a.q.moz, *inten = (gcu32 *)a.q.inten;
     const uint32_t q0 = qidx[s];
-    const uint32_t P = qidx[s + 1] - q0;
+    const uint32_t P = ufl(qidx[s + 1] - q0);
     for (uint32_t k = threadIdx.x; k < P; k += kBT) {
         sm.pk_moz[k] = moz[q0 + k];
         sm.pk_int[k] = inten[q0 + k];
@@ -631,7 +769,7 @@ __device__ __forceinline__ void init_smem(const ChunkArgs &a, const Smem &sm) {
 }
 
 // Plain form: one workgroup per spectrum, its items in order.
-__global__ __launch_bounds__(kBT) void k_spec_chunked(ChunkArgs a) {
+__global__ __launch_bounds__(kBT, kWavesPerSimd) void k_spec_chunked(ChunkArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ uint32_t s_ctl_[2];
     __shared__ uint32_t s_red_[4];
@@ -658,7 +796,7 @@ __global__ __launch_bounds__(kBT) void k_spec_chunked(ChunkArgs a) {
     Acc acc;
     acc_reset(acc);
     for (int ixx = 0; ixx < a.q.nlen; ixx++) {
-        const int32_t lo = sm.wlo[ixx], hi = sm.whi[ixx];
+        const int32_t lo = ufl((int32_t)sm.wlo[ixx]), hi = ufl((int32_t)sm.whi[ixx]);
         if (lo > hi) continue;
         for (uint32_t c = (uint32_t)lo / a.tc; c <= (uint32_t)hi / a.tc; c++)
             process_item<false>(a, sm, s_ctl, P, ixx, c, lo, hi, acc);
@@ -667,7 +805,7 @@ __global__ __launch_bounds__(kBT) void k_spec_chunked(ChunkArgs a) {
 }
 
 // Batched form: persistent workgroups pull (spectrum, length, chunk) items.
-__global__ __launch_bounds__(kBT) void k_items_chunked(ChunkArgs a) {
+__global__ __launch_bounds__(kBT, kWavesPerSimd) void k_items_chunked(ChunkArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ uint32_t s_ctl_[2];
     __shared__ uint32_t s_item_;
@@ -689,11 +827,16 @@ __global__ __launch_bounds__(kBT) void k_items_chunked(ChunkArgs a) {
         if (threadIdx.x == 0) *s_item = __hip_atomic_fetch_add((gu32 *)a.queue, kGrab, __ATOMIC_RELAXED,
                                                                __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
-        const uint32_t it0 = *s_item;
+        const uint32_t it0 = ufl((uint32_t)*s_item);
         if (it0 >= a.nitems) break;
         const uint32_t it1 = min(a.nitems, it0 + kGrab);
         for (uint32_t it = it0; it < it1; it++) {
-            const Item item = items[it];
+            Item item = items[it];
+            item.spec = ufl(item.spec);
+            item.lo = ufl(item.lo);
+            item.hi = ufl(item.hi);
+            item.ixx = (uint16_t)ufl((uint32_t)item.ixx);
+            item.chunk = (uint16_t)ufl((uint32_t)item.chunk);
             if (item.spec != cur) {
                 if (cur != 0xFFFFFFFFu) flush_spectrum(a, sm, cur, acc, s_red, s_red64);
                 cur = item.spec;
@@ -741,7 +884,7 @@ __global__ __launch_bounds__(kBT) void k_replay_chunked(ChunkArgs a) {
         Acc acc;
         acc_reset(acc);
         for (int ixx = 0; ixx < a.q.nlen; ixx++) {
-            const int32_t lo = sm.wlo[ixx], hi = sm.whi[ixx];
+            const int32_t lo = ufl((int32_t)sm.wlo[ixx]), hi = ufl((int32_t)sm.whi[ixx]);
             if (lo > hi) continue;
             for (uint32_t c = (uint32_t)lo / a.tc; c <= (uint32_t)hi / a.tc; c++)
                 process_item<true>(a, sm, s_ctl, P, ixx, c, lo, hi, acc, &heap, &hsize);
@@ -762,29 +905,35 @@ __global__ __launch_bounds__(kBT) void k_replay_chunked(ChunkArgs a) {
     }
 }
 
-// Items: per spectrum and length with a valid window, one item per chunk.
-__global__ void k_items_count(ChunkArgs a, uint32_t *counts) {
+// Items: per spectrum and length with a valid window, one item per chunk,
+// bucketed by spectrum, or by (length, chunk) -- bucket LenDesc::cbase + c --
+// so that the workgroups running at any moment share a few chunks of iA in L2.
+__global__ void k_items_count(ChunkArgs a, uint32_t *bcount) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= (uint32_t)a.q.nspec) return;
     const uint32_t D = a.q.max_mass + 2;
-    uint32_t n = 0;
-    for (int l = 0; l < a.q.nlen; l++) {
-        int32_t lo, hi;
-        window(a.q.lens[l], a.q.prec[s], a.q.dM, D, &lo, &hi);
-        if (lo <= hi) n += (uint32_t)hi / a.tc - (uint32_t)lo / a.tc + 1;
-    }
-    counts[s] = n;
-}
-
-__global__ void k_items_fill(ChunkArgs a, const uint32_t *offs, Item *items) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= (uint32_t)a.q.nspec) return;
-    const uint32_t D = a.q.max_mass + 2;
-    uint32_t o = offs[s];
     for (int l = 0; l < a.q.nlen; l++) {
         int32_t lo, hi;
         window(a.q.lens[l], a.q.prec[s], a.q.dM, D, &lo, &hi);
         if (lo > hi) continue;
+        if (!a.by_chunk) {
+            bcount[s] += (uint32_t)hi / a.tc - (uint32_t)lo / a.tc + 1;
+            continue;
+        }
+        const uint32_t cb = a.q.lens[l].cbase;
+        for (uint32_t c = (uint32_t)lo / a.tc; c <= (uint32_t)hi / a.tc; c++) atomicAdd(&bcount[cb + c], 1u);
+    }
+}
+
+__global__ void k_items_fill(ChunkArgs a, uint32_t *cursor, Item *items) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= (uint32_t)a.q.nspec) return;
+    const uint32_t D = a.q.max_mass + 2;
+    for (int l = 0; l < a.q.nlen; l++) {
+        int32_t lo, hi;
+        window(a.q.lens[l], a.q.prec[s], a.q.dM, D, &lo, &hi);
+        if (lo > hi) continue;
+        const uint32_t cb = a.q.lens[l].cbase;
         for (uint32_t c = (uint32_t)lo / a.tc; c <= (uint32_t)hi / a.tc; c++) {
             Item it;
             it.spec = s;
@@ -792,9 +941,14 @@ __global__ void k_items_fill(ChunkArgs a, const uint32_t *offs, Item *items) {
             it.chunk = (uint16_t)c;
             it.lo = lo;
             it.hi = hi;
-            items[o++] = it;
+            items[a.by_chunk ? atomicAdd(&cursor[cb + c], 1u) : cursor[s]++] = it;
         }
     }
+}
+
+__global__ void k_copy_u32(const uint32_t *src, uint32_t *dst, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
 // SpecAcc -> hcx_result; tied maxima go to the replay list.
@@ -861,6 +1015,17 @@ __global__ void k_chunk_offsets(const uint32_t *iA, const uint32_t *bA, uint32_t
     bA2[t] = r;
 }
 
+// Chunk-local copy of iA: entry e = (peptide mod TC) << 1 | isY, where the
+// ion id iA[e] = peptide * speclen + ion and isY = ion >= speclen / 2.
+__global__ void k_chunk_local(const uint32_t *iA, uint64_t n, uint32_t speclen, uint32_t TC, uint16_t *iAc) {
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const uint32_t raw = iA[e];
+    const uint32_t ppid = raw / speclen;
+    const uint32_t isY = (raw - ppid * speclen) >= speclen / 2 ? 1u : 0u;
+    iAc[e] = (uint16_t)(((ppid & (TC - 1)) << 1) | isY);
+}
+
 __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -878,6 +1043,13 @@ int launch_chunk_offsets(hcx_ctx *ctx, const uint32_t *iA, const uint32_t *bA, u
     return HCX_SUCCESS;
 }
 
+int launch_chunk_local(hcx_ctx *ctx, const uint32_t *iA, uint64_t n, uint32_t speclen, uint32_t TC, uint16_t *iAc) {
+    if (n == 0) return HCX_SUCCESS;
+    k_chunk_local<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(iA, n, speclen, TC, iAc);
+    HCX_HIP(hipGetLastError());
+    return HCX_SUCCESS;
+}
+
 int launch_fill_u32(hcx_ctx *ctx, uint32_t *p, uint64_t n, uint32_t v) {
     if (n == 0) return HCX_SUCCESS;
     k_fill_u32<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(p, n, v);
@@ -885,20 +1057,26 @@ int launch_fill_u32(hcx_ctx *ctx, uint32_t *p, uint64_t n, uint32_t v) {
     return HCX_SUCCESS;
 }
 
-// counts: nspec + 1 words.  fill == false: count and scan only (counts[nspec]
-// then holds the item total); fill == true: write the items.
+// counts: 2 * (nb + 1) words, nb = number of buckets (spectra, or (length,
+// chunk) pairs).  fill == false: bucket counts and their exclusive scan
+// (counts[nb] then holds the item total); fill == true: write the items,
+// bucket by bucket.
 int launch_items(hcx_ctx *ctx, ChunkArgs &a, uint32_t *counts, uint32_t *scan_tmp, bool fill) {
     hipStream_t st = ctx->stream;
     const int n = a.q.nspec;
     if (n <= 0) return HCX_SUCCESS;
+    const uint64_t nb = (uint64_t)(a.by_chunk ? a.nbk : (uint32_t)n) + 1;
     if (!fill) {
-        int rc = launch_fill_u32(ctx, counts, (uint64_t)n + 1, 0);
+        int rc = launch_fill_u32(ctx, counts, nb, 0);
         if (rc) return rc;
         k_items_count<<<(n + 255) / 256, 256, 0, st>>>(a, counts);
         HCX_HIP(hipGetLastError());
-        return scan_exclusive_u32(st, counts, counts, (uint64_t)n + 1, scan_tmp);
+        return scan_exclusive_u32(st, counts, counts, nb, scan_tmp);
     }
-    k_items_fill<<<(n + 255) / 256, 256, 0, st>>>(a, counts, a.items_w);
+    uint32_t *cursor = counts + nb;
+    k_copy_u32<<<(unsigned)((nb + 255) / 256), 256, 0, st>>>(counts, cursor, nb);
+    HCX_HIP(hipGetLastError());
+    k_items_fill<<<(n + 255) / 256, 256, 0, st>>>(a, cursor, a.items_w);
     HCX_HIP(hipGetLastError());
     return HCX_SUCCESS;
 }

@@ -33,10 +33,13 @@ RES_LGF_CLAMP = 0x10
 RES_FIT_UB = 0x20
 RES_SURROGATE = 0x40
 RES_AUDITED = 0x80
+RES_COUNT_OVF = 0x100
 
 KERNEL_PLAIN = 0
 KERNEL_BATCHED = 1
 KERNEL_CURSOR = 2
+OPT_HITLIST = 1
+OPT_ITEM_ORDER = 2
 EVAL_TAILFIT = 0
 EVAL_SURROGATE = 1
 
@@ -148,7 +151,7 @@ def hcx() -> C.CDLL:
                      "hcx_queries_upload", "hcx_queries_free", "hcx_search",
                      "hcx_search_device", "hcx_search_fetch", "hcx_search_candidates",
                      "hcx_tailfit_batch", "hcx_lgfact_table", "hcx_heap_capacity",
-                     "hcx_selftest_divider", "hcx_set_tuning", "hcx_get_tuning"]:
+                     "hcx_selftest_divider", "hcx_set_tuning", "hcx_get_tuning", "hcx_set_option", "hcx_get_option"]:
             getattr(lib, name).restype = C.c_int
         _hcx = lib
     return _hcx

@@ -86,7 +86,7 @@ class Batch:
 
 class Engine:
     def __init__(self, params: N.Params, device: int = 0, chunk: int = 0, slots: int = 0,
-                 blocks_per_cu: int = 0):
+                 blocks_per_cu: int = 0, hitlist: int = -1):
         self.params = params
         h = C.c_void_p()
         N.check(N.hcx().hcx_create(C.byref(params), device, C.byref(h)))
@@ -95,10 +95,19 @@ class Engine:
         self.hbins = params.hbins
         self.indices: list[Index] = []
         self.set_tuning(chunk, slots, blocks_per_cu)
+        self.set_hitlist(hitlist)
 
     def set_tuning(self, chunk: int = 0, slots: int = 0, blocks_per_cu: int = 0) -> None:
         """chunk must be set before indices are built (they record it)."""
         N.check(N.hcx().hcx_set_tuning(self._h, C.c_uint32(chunk), C.c_uint32(slots), C.c_uint32(blocks_per_cu)))
+
+    def set_hitlist(self, entries: int = -1) -> None:
+        """LDS hit-list entries per work item: -1 as many as fit, 0 none."""
+        N.check(N.hcx().hcx_set_option(self._h, N.OPT_HITLIST, C.c_int64(entries)))
+
+    def set_item_order(self, by_chunk: bool) -> None:
+        """Batched work items by spectrum (False) or by (length, chunk) (True)."""
+        N.check(N.hcx().hcx_set_option(self._h, N.OPT_ITEM_ORDER, C.c_int64(1 if by_chunk else 0)))
 
     def tuning(self) -> dict:
         ch, sl, bp = C.c_uint32(), C.c_uint32(), C.c_uint32()

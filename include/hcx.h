@@ -154,10 +154,22 @@ int  hcx_destroy(hcx_ctx *ctx);
 const char *hcx_last_error(void);
 int  hcx_heap_capacity(uint32_t topmatches);   /* dslim.cpp:422-428 (Q14) */
 /* Query tuning (no reference counterpart).  chunk: peptides per LDS chunk
- * (power of two, 256..65536; set before building indices, which record it);
+ * (power of two, 256..32768; set before building indices, which record it);
  * slots: scored-peptide slots per chunk pass; blocks_per_cu: persistent grid
  * of the batched kernel.  0 keeps the current value. */
 int  hcx_set_tuning(hcx_ctx *ctx, uint32_t chunk, uint32_t slots, uint32_t blocks_per_cu);
+/* Further query tuning options (no reference counterpart).
+ * HCX_OPT_HITLIST: LDS hit list of the chunked kernels -- the first entries
+ *   of each work item are kept in LDS so the exact-count pass need not read
+ *   iA again.  < 0: as many as fit (default); 0: none; n > 0: at most n
+ *   (rounded down to a multiple of 64).
+ * HCX_OPT_ITEM_ORDER: order of the batched kernel's work items.
+ *   0: by spectrum (default); 1: by (length, chunk), so that concurrent
+ *   workgroups share chunks of iA in L2. */
+#define HCX_OPT_HITLIST    1
+#define HCX_OPT_ITEM_ORDER 2
+int  hcx_set_option(hcx_ctx *ctx, int32_t option, int64_t value);
+int  hcx_get_option(const hcx_ctx *ctx, int32_t option, int64_t *value);
 int  hcx_get_tuning(const hcx_ctx *ctx, uint32_t *chunk, uint32_t *slots, uint32_t *blocks_per_cu,
                     int32_t *num_cu, int32_t *max_lds);
 /* lgfact table the device uses (utils.h:115-131), max_shdpeaks+1 doubles */

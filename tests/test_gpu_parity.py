@@ -120,6 +120,23 @@ def test_query_parity_small(small_case, kernel):
     eng.close()
 
 
+def test_batched_items_by_chunk(small_case):
+    # work items bucketed by (length, chunk): many workgroups merge into each spectrum
+    p, db, q, prm = small_case
+    log("test_batched_items_by_chunk")
+    res_o, hist_o, st_o = O.OracleDB(prm, db).search(q)
+    eng = Engine(prm, chunk=1024)
+    eng.set_item_order(True)
+    eng.construct_db(db)
+    b = eng.upload(q)
+    out = eng.query_spectrum(b, kernel=N.KERNEL_BATCHED)
+    compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, "by-chunk")
+    eng.set_item_order(False)   # the cached item list is rebuilt
+    out = eng.query_spectrum(b, kernel=N.KERNEL_BATCHED)
+    compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, "by-spectrum")
+    eng.close()
+
+
 @pytest.mark.parametrize("tile", [64, 256, 4096])
 def test_cursor_kernel_tiles(small_case, tile):
     p, db, q, prm = small_case
@@ -133,22 +150,48 @@ def test_cursor_kernel_tiles(small_case, tile):
     eng.close()
 
 
-@pytest.mark.parametrize("kernel", ["plain", "batched"])
-@pytest.mark.parametrize("chunk,slots", [(256, 64), (1024, 64), (8192, 256)])
-def test_chunked_edges_and_slot_rounds(kernel, chunk, slots):
-    # small chunks: windows span many chunks and start/end inside chunks;
-    # few slots: peptides reaching min_shp overflow the slot table (several rounds)
+@pytest.fixture(scope="module")
+def edge_case():
     p = W.scaled("rank_share", npep=20000, nspec=120, maxz=2)
     db = W.Database(p)
     q = W.make_queries(db, p)
     prm = W.make_params(p)
-    log(f"test_chunked_edges_and_slot_rounds kernel={kernel} chunk={chunk} slots={slots}")
-    res_o, hist_o, st_o = O.OracleDB(prm, db).search(q)
-    eng = Engine(prm, chunk=chunk, slots=slots)
+    return db, q, prm, O.OracleDB(prm, db).search(q)
+
+
+@pytest.mark.parametrize("kernel", ["plain", "batched"])
+@pytest.mark.parametrize("chunk,slots", [(256, 64), (1024, 64), (8192, 256)])
+@pytest.mark.parametrize("hitlist", [-1, 0, 64])
+def test_chunked_edges_and_slot_rounds(edge_case, kernel, chunk, slots, hitlist):
+    # small chunks: windows span many chunks and start/end inside chunks;
+    # few slots: peptides reaching min_shp overflow the slot table (several rounds);
+    # hit list off / 64 entries: the exact-count pass reads iA for all or most entries
+    db, q, prm, (res_o, hist_o, st_o) = edge_case
+    log(f"test_chunked_edges_and_slot_rounds kernel={kernel} chunk={chunk} slots={slots} hitlist={hitlist}")
+    eng = Engine(prm, chunk=chunk, slots=slots, hitlist=hitlist)
     eng.construct_db(db)
     out = eng.query_spectrum(eng.upload(q), kernel=KERNELS[kernel])
-    n = compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"{kernel}-{chunk}-{slots}")
+    n = compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"{kernel}-{chunk}-{slots}-{hitlist}")
     assert n > 50
+    eng.close()
+
+
+@pytest.mark.parametrize("hitlist", [0, 256])
+def test_open_search_groups_without_hit_list(hitlist):
+    # 250-400 peaks x 5 bins (dF = 2): two run-table groups per item
+    p = W.scaled("rank_share", npep=20000, nspec=100, maxz=2, peaks_min=250, peaks_max=400)
+    db = W.Database(p)
+    q = W.make_queries(db, p)
+    prm = W.make_params(p)
+    log(f"test_open_search_groups_without_hit_list hitlist={hitlist}")
+    res_o, hist_o, st_o = O.OracleDB(prm, db).search(q)
+    eng = Engine(prm, hitlist=hitlist)
+    eng.construct_db(db)
+    b = eng.upload(q)
+    for kernel in ("plain", "batched"):
+        out = eng.query_spectrum(b, kernel=KERNELS[kernel])
+        n = compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"groups-{hitlist}-{kernel}")
+        assert n > 50
     eng.close()
 
 
