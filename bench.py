@@ -41,11 +41,18 @@ def main() -> int:
     ap.add_argument("--preset", default="open10m")
     ap.add_argument("--npep", type=int, default=0)
     ap.add_argument("--nspec", type=int, default=0)
-    ap.add_argument("--kernel", default="plain", choices=["plain", "batched", "both"])
+    ap.add_argument("--kernel", default="both", choices=["plain", "batched", "both"])
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="one batched search, then a 1 GiB device copy (run under rocprofv3 --pmc)")
+    ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="turn two rocprofv3 --pmc runs of --pmc-probe into the traffic figure")
     args = ap.parse_args()
+
+    if args.traffic:
+        return traffic_from_pmc(args.traffic[0], args.traffic[1], args.preset)
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -80,6 +87,14 @@ def main() -> int:
     b_build = 4 * nions + 4 * (eng.NB + 1) * len(idx)
     log(f"index: {nions} ions, device build {build_ms:.1f} ms, wall {build_wall:.1f}s")
     batch = eng.upload(q)
+
+    if args.pmc_probe:
+        eng.query_device(batch, kernel=N.KERNEL_BATCHED)
+        k, t = eng.query_device(batch, kernel=N.KERNEL_BATCHED)
+        log(f"pmc probe: batched query kernel {k:.1f} ms")
+        calibration_copy(1 << 30)
+        eng.close()
+        return 0
 
     forms = ["plain", "batched"] if args.kernel == "both" else [args.kernel]
     kcode = {"plain": N.KERNEL_PLAIN, "batched": N.KERNEL_BATCHED}
@@ -158,6 +173,64 @@ def main() -> int:
     }
     print(json.dumps(line), flush=True)
     eng.close()
+    return 0
+
+
+SCAN_KERNEL = "k_items_chunked"
+COPY_KERNEL = "copyBuffer"
+
+
+def calibration_copy(nbytes: int) -> None:
+    """A plain device-to-device copy of known size, for scaling FETCH_SIZE /
+    WRITE_SIZE (they count in KiB per dispatch with a hardware-dependent
+    granularity)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) == 0   # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log(f"calibration copy: {nbytes} bytes")
+
+
+def traffic_from_pmc(fetch_csv: str, write_csv: str, preset: str) -> int:
+    """HBM bytes per launch of the scan kernel from FETCH_SIZE and WRITE_SIZE
+    (rocprofv3 counter_collection CSVs of a --pmc-probe run), each scaled by
+    what the same counter reads for the 1 GiB calibration copy."""
+    import csv
+
+    def read(path, counter):
+        scan, copy = [], []
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            if SCAN_KERNEL in r["Kernel_Name"]:
+                scan.append(float(r["Counter_Value"]))
+            elif COPY_KERNEL in r["Kernel_Name"]:
+                copy.append(float(r["Counter_Value"]))
+        return scan, copy
+
+    fs, fc = read(fetch_csv, "FETCH_SIZE")
+    ws, wc = read(write_csv, "WRITE_SIZE")
+    if not fs or not ws or not fc or not wc:
+        print(json.dumps({"error": "scan or copy kernel missing from the counter files"}))
+        return 1
+    gib_kib = float(1 << 20)
+    f_factor = gib_kib / max(fc)      # the copy reads 1 GiB
+    w_factor = gib_kib / max(wc)      # and writes 1 GiB
+    fetch = fs[-1] * 1024 * f_factor
+    write = ws[-1] * 1024 * w_factor
+    out = {"kernel": SCAN_KERNEL, "preset": preset, "fetch_kib_raw": fs[-1], "write_kib_raw": ws[-1],
+           "copy_fetch_kib": max(fc), "copy_write_kib": max(wc), "fetch_factor": round(f_factor, 4),
+           "write_factor": round(w_factor, 4), "fetch_bytes": round(fetch), "write_bytes": round(write),
+           "hbm_bytes_per_launch": round(fetch + write)}
+    dst = ROOT / "profiles" / f"traffic_{preset}_batched.json"
+    dst.write_text(json.dumps(out, indent=1) + "\n")
+    print(json.dumps(out), flush=True)
     return 0
 
 
