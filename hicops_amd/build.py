@@ -1,8 +1,11 @@
 """Build the native libraries in-tree.
 
   hicops_amd/libhcx.so       HIP kernels + C ABI (include/hcx.h), gfx950
+  hicops_amd/libhcx_dslim.so the reference's C++ entry points over the C ABI
+                             (include/hcx_dslim.hpp)
   hicops_amd/libhcx_host.so  synthetic data generator (include/hcx_datagen.h)
   oracle/liboracle.so        CPU restatement used by tests and the CPU baseline
+  tests/cpp/dslim_driver     test program driving include/hcx_dslim.hpp
 
 Run as ``python -m hicops_amd.build`` or through ``__graft_entry__.build()``.
 """
@@ -61,13 +64,28 @@ def build_hip() -> Path:
     return out
 
 
+def build_mirror() -> Path:
+    out = PKG / "libhcx_dslim.so"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra", "-I", str(ROOT / "include"),
+          "-o", str(out), str(PKG / "host" / "dslim_hcx.cpp"),
+          "-L", str(PKG), "-lhcx", "-Wl,-rpath,$ORIGIN"])
+    drv = ROOT / "tests" / "cpp" / "dslim_driver"
+    # -rdynamic: the driver's DFile_PrintScore takes precedence over the library's
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-rdynamic", "-I", str(ROOT / "include"),
+          "-o", str(drv), str(ROOT / "tests" / "cpp" / "dslim_driver.cpp"),
+          "-L", str(PKG), "-lhcx_dslim", "-lhcx", "-Wl,-rpath,$ORIGIN/../../hicops_amd"])
+    return out
+
+
 def build_all() -> None:
     build_oracle()
     build_host()
     build_hip()
+    build_mirror()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "oracle": build_oracle, "host": build_host, "hip": build_hip}[w]()
+        {"all": build_all, "oracle": build_oracle, "host": build_host, "hip": build_hip,
+         "mirror": build_mirror}[w]()

@@ -1,0 +1,284 @@
+// dslim_hcx.cpp -- the reference's DSLIM construct / query / e-value entry
+// points (include/hcx_dslim.hpp) over the C ABI of libhcx.
+//
+// One device context per process, created from `params` on first use, as
+// the reference builds its scorecard from the global parameter block
+// (dslim.cpp:390-437).  Device indices live in a side table keyed by Index*.
+#include "../../include/hcx_dslim.hpp"
+
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+struct State {
+    std::mutex mu;
+    hcx_ctx *ctx = nullptr;
+    std::unordered_map<const Index *, hcx_index *> idx;
+    uint_t max_hyperscore = MAX_HYPERSCORE;
+    uint_t max_shdpeaks = MAX_SHDPEAKS;
+    int_t kernel = HCX_KERNEL_BATCHED;
+    uint_t spectrumID = 0;   // running spectrum number across batches (dslim_query.cpp:808-812)
+};
+
+State &st() {
+    static State s;
+    return s;
+}
+
+// gParams -> hcx_params (the fields the search core reads)
+status_t make_params(hcx_params *p) {
+    *p = hcx_params{};
+    p->maxz = params.maxz;
+    p->scale = params.scale;
+    p->min_mass = params.min_mass;
+    p->max_mass = params.max_mass;
+    p->dF = params.dF;
+    p->min_shp = params.min_shp;
+    p->min_cpsm = params.min_cpsm;
+    p->topmatches = params.topmatches;
+    p->max_hyperscore = st().max_hyperscore;
+    p->max_shdpeaks = st().max_shdpeaks;
+    if (params.vModInfo.num_vars > HCX_MAX_MOD_TYPES) return ERR_INVLD_PARAM;
+    p->nmods = params.vModInfo.num_vars;
+    for (uint_t i = 0; i < p->nmods; i++) p->mod_mass[i] = params.vModInfo.vmods[i].modMass;
+    p->dM = params.dM;
+    p->expect_max = params.expect_max;
+    return SLM_SUCCESS;
+}
+
+// caller holds st().mu
+status_t ensure_ctx() {
+    State &s = st();
+    if (s.ctx) return SLM_SUCCESS;
+    hcx_params p;
+    status_t rc = make_params(&p);
+    if (rc != SLM_SUCCESS) return rc;
+    return hcx_create(&p, 0, &s.ctx);
+}
+
+}  // namespace
+
+// The library's own sink discards the PSM; an application that defines
+// DFile_PrintScore receives every reported PSM instead.
+__attribute__((weak)) status_t DFile_PrintScore(Index *, uint_t, float, hCell *, double, uint_t) {
+    return SLM_SUCCESS;
+}
+
+status_t DSLIM_SetLimits(uint_t max_hyperscore, uint_t max_shdpeaks) {
+    State &s = st();
+    std::lock_guard<std::mutex> g(s.mu);
+    if (s.ctx) return ERR_INVLD_PARAM;   // fixed once the context exists
+    if (max_hyperscore == 0 || max_shdpeaks == 0) return ERR_INVLD_PARAM;
+    s.max_hyperscore = max_hyperscore;
+    s.max_shdpeaks = max_shdpeaks;
+    return SLM_SUCCESS;
+}
+
+status_t DSLIM_SetKernel(int_t kernel) {
+    if (kernel != HCX_KERNEL_BATCHED && kernel != HCX_KERNEL_PLAIN && kernel != HCX_KERNEL_CURSOR)
+        return ERR_INVLD_PARAM;
+    st().kernel = kernel;
+    return SLM_SUCCESS;
+}
+
+hcx_ctx *DSLIM_Context() { return st().ctx; }
+
+// dslim.cpp:46-138: build the ion index of one peptide length from its
+// sequences and mass-sorted pepEntries.
+status_t DSLIM_Construct(Index *index) {
+    if (!index) return ERR_INVLD_PTR;
+    State &s = st();
+    std::lock_guard<std::mutex> g(s.mu);
+    status_t rc = ensure_ctx();
+    if (rc != SLM_SUCCESS) return rc;
+    const uint_t L = index->pepIndex.peplen;
+    const uint_t n = index->lcltotCnt;
+    if (L == 0) return ERR_INVLD_PARAM;
+    if (n && (!index->pepEntries || !index->pepIndex.seqs)) return ERR_INVLD_PTR;
+    std::vector<hcx_pep_entry> e(n);
+    for (uint_t i = 0; i < n; i++) {
+        e[i].mass = index->pepEntries[i].Mass;
+        e[i].seq_id = index->pepEntries[i].seqID;
+        e[i].sites = index->pepEntries[i].sites.sites;
+        e[i].mod_num = index->pepEntries[i].sites.modNum;
+        e[i].pad_ = 0;
+    }
+    auto it = s.idx.find(index);
+    if (it != s.idx.end()) {
+        hcx_index_free(it->second);
+        s.idx.erase(it);
+    }
+    hcx_index *ix = nullptr;
+    rc = hcx_index_build(s.ctx, L, index->pepIndex.seqs, index->pepIndex.AAs / L, e.data(), n, &ix);
+    if (rc != SLM_SUCCESS) return rc;
+    s.idx[index] = ix;
+    return SLM_SUCCESS;
+}
+
+// dslim.cpp:390-437: the scorecard (LDS counters, histograms, heaps) lives on
+// the device and is sized per search; only the context is needed here.
+status_t DSLIM_InitializeScorecard(Index *index, uint_t idxs) {
+    if (!index && idxs) return ERR_INVLD_PTR;
+    State &s = st();
+    std::lock_guard<std::mutex> g(s.mu);
+    return ensure_ctx();
+}
+
+// dslim_query.cpp:515-842: search one batch against the indices
+// index[0 .. idxchunk) and report each spectrum's top PSM through
+// DFile_PrintScore when cpsms >= min_cpsm and e < expect_max.
+status_t DSLIM_QuerySpectrum(Queries *ss, Index *index, uint_t idxchunk) {
+    if (!ss || (!index && idxchunk)) return ERR_INVLD_PTR;
+    State &s = st();
+    std::vector<hcx_index *> ix(idxchunk);
+    hcx_ctx *ctx;
+    int_t kernel;
+    uint_t first;
+    {
+        std::lock_guard<std::mutex> g(s.mu);
+        status_t rc = ensure_ctx();
+        if (rc != SLM_SUCCESS) return rc;
+        for (uint_t i = 0; i < idxchunk; i++) {
+            auto it = s.idx.find(&index[i]);
+            if (it == s.idx.end()) return ERR_INVLD_PARAM;   // DSLIM_Construct not called
+            ix[i] = it->second;
+        }
+        ctx = s.ctx;
+        kernel = s.kernel;
+        first = s.spectrumID;
+        s.spectrumID += (uint_t)(ss->numSpecs > 0 ? ss->numSpecs : 0);
+    }
+    if (ss->numSpecs <= 0) return SLM_SUCCESS;
+    hcx_queries_view v;
+    v.moz = ss->moz;
+    v.intensity = ss->intensity;
+    v.idx = ss->idx;
+    v.precurse = ss->precurse;
+    v.charges = ss->charges;
+    v.rtimes = ss->rtimes;
+    v.num_specs = ss->numSpecs;
+    v.file_num = ss->fileNum;
+    hcx_qbatch *qb = nullptr;
+    status_t rc = hcx_queries_upload(ctx, &v, &qb);
+    if (rc != SLM_SUCCESS) return rc;
+    std::vector<hcx_result> res((size_t)ss->numSpecs);
+    hcx_search_opts o = {};
+    o.kernel = kernel;
+    o.evalue = HCX_EVAL_TAILFIT;
+    rc = hcx_search(ctx, ix.data(), (int)idxchunk, qb, &o, res.data(), nullptr, nullptr);
+    hcx_queries_free(qb);
+    if (rc != SLM_SUCCESS) return rc;
+    for (int_t q = 0; q < ss->numSpecs; q++) {
+        const hcx_result &r = res[q];
+        if (!(r.flags & HCX_RES_VALID)) continue;      // cpsms < min_cpsm
+        if (!(r.e_x < params.expect_max)) continue;
+        hCell psm;
+        psm.idxoffset = r.idxoffset;
+        psm.sharedions = r.sharedions;
+        psm.totalions = r.totalions;
+        psm.fileIndex = (ushort_t)ss->fileNum;
+        psm.psid = r.psid;
+        psm.pmass = ss->precurse[q];
+        psm.pchg = ss->charges ? ss->charges[q] : 0;
+        psm.rtime = ss->rtimes ? ss->rtimes[q] : 0.0f;
+        psm.hyperscore = r.hyperscore;
+        rc = DFile_PrintScore(index, first + (uint_t)q, ss->precurse[q], &psm, r.e_x, r.cpsms);
+        if (rc != SLM_SUCCESS) return rc;
+    }
+    return SLM_SUCCESS;
+}
+
+status_t DSLIM_ExportIonIndex(Index *index) {
+    if (!index) return ERR_INVLD_PTR;
+    State &s = st();
+    std::lock_guard<std::mutex> g(s.mu);
+    auto it = s.idx.find(index);
+    if (it == s.idx.end()) return ERR_INVLD_PARAM;
+    uint32_t peplen, speclen, n;
+    uint64_t nions;
+    status_t rc = hcx_index_info(it->second, &peplen, &speclen, &n, &nions);
+    if (rc != SLM_SUCCESS) return rc;
+    if (index->ionIndex) return ERR_INVLD_MEMORY;   // already holds host arrays
+    SLMchunk *ch = new SLMchunk[1];
+    ch->iA = new uint_t[nions ? nions : 1];
+    ch->bA = new uint_t[(size_t)params.max_mass * params.scale + 1];
+    rc = hcx_index_export(it->second, ch->iA, ch->bA);
+    if (rc != SLM_SUCCESS) {
+        delete[] ch->iA;
+        delete[] ch->bA;
+        delete[] ch;
+        return rc;
+    }
+    index->ionIndex = ch;
+    index->nChunks = 1;
+    return SLM_SUCCESS;
+}
+
+// dslim.cpp:657-691: release one length's ion index (device, and the host
+// copy if DSLIM_ExportIonIndex made one).
+status_t DSLIM_DeallocateIonIndex(Index *index) {
+    if (!index) return ERR_INVLD_PTR;
+    State &s = st();
+    std::lock_guard<std::mutex> g(s.mu);
+    auto it = s.idx.find(index);
+    if (it != s.idx.end()) {
+        hcx_index_free(it->second);
+        s.idx.erase(it);
+    }
+    if (index->ionIndex) {
+        delete[] index->ionIndex[0].iA;
+        delete[] index->ionIndex[0].bA;
+        delete[] index->ionIndex;
+        index->ionIndex = nullptr;
+    }
+    return SLM_SUCCESS;
+}
+
+// dslim.cpp:744-770: release the scorecard; here the whole device context,
+// with every index still held.
+status_t DSLIM_DeallocateSC() {
+    State &s = st();
+    std::lock_guard<std::mutex> g(s.mu);
+    for (auto &kv : s.idx) hcx_index_free(kv.second);
+    s.idx.clear();
+    if (s.ctx) hcx_destroy(s.ctx);
+    s.ctx = nullptr;
+    s.spectrumID = 0;
+    return SLM_SUCCESS;
+}
+
+// expeRT.cpp:572-718: fit the survival-function tail of one spectrum's
+// histogram (rPtr->survival holds counts) and store mu, beta, minhypscore
+// and nexthypscore.  rPtr->cpsms and rPtr->maxhypscore are inputs.
+status_t expeRT::ModelTailFit(Results *rPtr) {
+    if (!rPtr || !rPtr->survival) return ERR_INVLD_PTR;
+    State &s = st();
+    hcx_ctx *ctx;
+    uint_t hb;
+    {
+        std::lock_guard<std::mutex> g(s.mu);
+        status_t rc = ensure_ctx();
+        if (rc != SLM_SUCCESS) return rc;
+        ctx = s.ctx;
+        hb = 10 * s.max_hyperscore + 2;
+    }
+    std::vector<uint32_t> h(hb);
+    for (uint_t i = 0; i < hb; i++) {
+        const double v = rPtr->survival[i];
+        if (!(v >= 0.0) || v > 4294967295.0 || v != (double)(uint32_t)v) return ERR_INVLD_PARAM;   // counts only
+        h[i] = (uint32_t)v;
+    }
+    const uint32_t cps = rPtr->cpsms;
+    const int32_t hyp = rPtr->maxhypscore;
+    int32_t mu = 0, beta = 0, mn = 0, nx = 0;
+    uint32_t fl = 0;
+    status_t rc = hcx_tailfit_batch(ctx, h.data(), hb, 1, &cps, &hyp, &mu, &beta, &mn, &nx, &fl);
+    if (rc != SLM_SUCCESS) return rc;
+    rPtr->mu = mu;
+    rPtr->beta = beta;
+    rPtr->minhypscore = mn;
+    rPtr->nexthypscore = nx;
+    return SLM_SUCCESS;
+}

@@ -21,6 +21,9 @@ def _ensure_cpu_libs():
         build.build_oracle()
     if not (ROOT / "hicops_amd" / "libhcx_host.so").exists():
         build.build_host()
+    # the C++ entry-point library links the HIP library; g++ only
+    if (ROOT / "hicops_amd" / "libhcx.so").exists() and not (ROOT / "hicops_amd" / "libhcx_dslim.so").exists():
+        build.build_mirror()
 
 
 _ensure_cpu_libs()

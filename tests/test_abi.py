@@ -146,3 +146,32 @@ def test_datagen_mod_variants():
             cnt = int(((ents["seq_id"] == sid) & (ents["mod_num"] != 0)).sum())
             assert cnt == m + m * (m - 1) // 2
     assert nmod > 100
+
+
+# The reference's own C++ entry points, by their mangled names, so that an
+# application built against source/core/include/dslim.h and expeRT.h links to
+# libhcx_dslim.so (dslim.h:49,53,149,164,184; expeRT.h:143; dslim_fileout.h:29).
+REFERENCE_SYMBOLS = [
+    "_Z15DSLIM_ConstructP6_Index",
+    "_Z25DSLIM_InitializeScorecardP6_Indexj",
+    "_Z19DSLIM_QuerySpectrumP8_queriesP6_Indexj",
+    "_Z24DSLIM_DeallocateIonIndexP6_Index",
+    "_Z18DSLIM_DeallocateSCv",
+    "_ZN6expeRT12ModelTailFitEP8_Results",
+    "_Z16DFile_PrintScoreP6_IndexjfP10_heapEntrydj",
+]
+
+
+def test_mirror_exports_reference_entry_points():
+    # like the reference core, the library leaves `gParams params` to the
+    # application, so it is inspected rather than loaded
+    so = Path(__file__).resolve().parent.parent / "hicops_amd" / "libhcx_dslim.so"
+    out = subprocess.run(["nm", "-D", str(so)], capture_output=True, text=True, check=True).stdout
+    kinds = {}
+    for ln in out.splitlines():
+        parts = ln.split()
+        if len(parts) >= 2:
+            kinds[parts[-1]] = parts[-2]
+    for name in REFERENCE_SYMBOLS:
+        assert kinds.get(name) in ("T", "W"), name
+    assert kinds.get("params") == "U"

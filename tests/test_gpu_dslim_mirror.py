@@ -1,0 +1,84 @@
+"""The reference-shaped C++ entry points (include/hcx_dslim.hpp) driven the
+way the reference application drives its own (tests/cpp/dslim_driver.cpp):
+DSLIM_Construct per length, DSLIM_InitializeScorecard, DSLIM_QuerySpectrum
+per batch, every reported PSM through DFile_PrintScore, ion-index export,
+expeRT::ModelTailFit.  Checked against the oracle on the same inputs."""
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from hicops_amd import native as N
+from hicops_amd import workload as W
+from oracle import binding as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = Path(__file__).resolve().parent.parent
+DRIVER = ROOT / "tests" / "cpp" / "dslim_driver"
+
+
+def write_case(d: Path, p, db, q, prm, kernel: int) -> None:
+    kv = {"maxz": prm.maxz, "scale": prm.scale, "min_mass": prm.min_mass, "max_mass": prm.max_mass,
+          "dF": prm.dF, "min_shp": prm.min_shp, "min_cpsm": prm.min_cpsm, "topmatches": prm.topmatches,
+          "dM": prm.dM, "expect_max": 1e300, "nmods": prm.nmods, "max_hyperscore": prm.max_hyperscore,
+          "max_shdpeaks": prm.max_shdpeaks, "kernel": kernel}
+    for i in range(prm.nmods):
+        kv[f"mod{i}"] = prm.mod_mass[i]
+    (d / "params.txt").write_text("".join(f"{k} {v!r}\n" for k, v in kv.items()))
+    (d / "lengths.txt").write_text("".join(f"{L}\n" for L in db.lengths))
+    for L in db.lengths:
+        np.ascontiguousarray(db.seqs(L)).tofile(d / f"seqs_{L}.bin")
+        np.ascontiguousarray(db.entries(L)).tofile(d / f"ents_{L}.bin")
+    q.moz.astype(np.uint32).tofile(d / "moz.bin")
+    q.intensity.astype(np.uint32).tofile(d / "inten.bin")
+    q.idx.astype(np.uint32).tofile(d / "idx.bin")
+    q.precurse.astype(np.float32).tofile(d / "prec.bin")
+    q.charges.astype(np.int32).tofile(d / "chg.bin")
+
+
+@pytest.mark.parametrize("kernel", ["batched", "plain"])
+def test_dslim_entry_points_match_oracle(tmp_path, kernel):
+    assert DRIVER.exists(), "tests/cpp/dslim_driver is built by __graft_entry__.build()"
+    p = W.scaled("small", npep=4000, nspec=300, dM=100.0)
+    db = W.Database(p)
+    q = W.make_queries(db, p)
+    prm = W.make_params(p)
+    kc = N.KERNEL_BATCHED if kernel == "batched" else N.KERNEL_PLAIN
+    write_case(tmp_path, p, db, q, prm, kc)
+    print(f"test_dslim_entry_points_match_oracle kernel={kernel}", flush=True)
+    r = subprocess.run([str(DRIVER), str(tmp_path), "3"], capture_output=True, text=True, timeout=300)
+    sys.stdout.write(r.stdout)
+    sys.stderr.write(r.stderr)
+    assert r.returncode == 0, r.stderr
+
+    res_o, _, _ = O.OracleDB(prm, db).search(q)
+    reported = np.nonzero((res_o["flags"] & N.RES_VALID) != 0)[0]
+    rows = [ln.split() for ln in (tmp_path / "psms.txt").read_text().splitlines()]
+    assert len(rows) == len(reported) and len(rows) > 100
+    got = {int(x[0]): x for x in rows}
+    assert sorted(got) == [int(s) for s in reported]      # running spectrum ids across 3 batches
+    for s in reported:
+        x = got[int(s)]
+        o = res_o[s]
+        assert int(x[1]) == o["psid"] and int(x[2]) == o["idxoffset"], s
+        assert int(x[3]) == o["sharedions"] and int(x[4]) == o["totalions"], s
+        hs = np.frombuffer(bytes.fromhex(x[5])[::-1], np.float32)[0]
+        assert abs(hs - o["hyperscore"]) <= 1e-5 * abs(o["hyperscore"]), s
+        assert abs(float(x[6]) - o["e_x"]) <= 1e-3 * abs(o["e_x"]), s
+        assert int(x[7]) == o["cpsms"], s
+        assert int(x[8]) == int(q.charges[s]) and int(x[9]) == 3, s
+
+    # DSLIM_ExportIonIndex: the host copy of the first length's iA / bA
+    L0 = db.lengths[0]
+    oiA, obA = O.index_build(prm, L0, db.seqs(L0), db.entries(L0))
+    raw = np.fromfile(tmp_path / "ion0.bin", np.uint32)
+    assert np.array_equal(raw[:len(oiA)], oiA) and np.array_equal(raw[len(oiA):], obA)
+
+    # expeRT::ModelTailFit(Results*) on a hand-traced histogram
+    H = np.zeros(10 * prm.max_hyperscore + 2)
+    H[2:11] = [5, 10, 8, 6, 3, 1, 0, 1, 1]
+    mu, beta, mn, nx, _ = O.tailfit(H, 35, 10)
+    assert [int(v) for v in (tmp_path / "tailfit.txt").read_text().split()] == [mu, beta, mn, nx]
