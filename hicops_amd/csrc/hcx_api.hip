@@ -498,6 +498,8 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
     int rc = prepare(c, idx, nidx, qb, opts, &ca.q);
     if (rc) return rc;
     QueryArgs &a = ca.q;
+    if (opts && opts->evalue != HCX_EVAL_TAILFIT)
+        return set_error_code(HCX_ERR_INVLD_PARAM, "search: only HCX_EVAL_TAILFIT e-values are available");
     int kernel = opts ? opts->kernel : HCX_KERNEL_PLAIN;
     if (c->params.min_shp > kMaxChunkShp) kernel = HCX_KERNEL_CURSOR;
     const uint64_t n = (uint64_t)qb->nspec;

@@ -6,9 +6,17 @@
 // (dslim.cpp:390-437).  Device indices live in a side table keyed by Index*.
 #include "../../include/hcx_dslim.hpp"
 
+#include <cstddef>
 #include <mutex>
 #include <unordered_map>
 #include <vector>
+
+// layouts an application built against source/core/include/slm_dsts.h relies on
+static_assert(sizeof(pepEntry) == 24, "pepEntry layout");
+static_assert(sizeof(hCell) == 28, "hCell layout");
+static_assert(sizeof(Results) == 56 && offsetof(Results, mu) == 24 && offsetof(Results, survival) == 48,
+              "Results layout");
+static_assert(sizeof(pepEntry) == sizeof(hcx_pep_entry), "pepEntry and hcx_pep_entry");
 
 namespace {
 

@@ -20,7 +20,6 @@
  *                                 (dslim_query.cpp:771-818); results in place of DFile_PrintScore
  *                                 (dslim_fileout.h:29)
  *   hcx_tailfit_batch             expeRT::ModelTailFit(Results*) over a batch of histograms
- *   hcx_surrogate_*               no counterpart: learned tail-parameter model with audit fallback
  */
 #ifndef HCX_H
 #define HCX_H
@@ -136,7 +135,8 @@ typedef struct hcx_qbatch hcx_qbatch;
 
 /* E-value model choice. */
 #define HCX_EVAL_TAILFIT   0  /* expeRT log-linear tail fit on every spectrum        */
-#define HCX_EVAL_SURROGATE 1  /* MLP prediction, tail fit on the audit sample        */
+#define HCX_EVAL_SURROGATE 1  /* MLP prediction, tail fit on the audit sample (not
+                                 available yet: hcx_search returns HCX_ERR_INVLD_PARAM) */
 
 typedef struct hcx_search_opts {
     int32_t  kernel;          /* HCX_KERNEL_*                                       */

@@ -176,9 +176,18 @@ typedef struct _heapEntry {
     float    hyperscore;
 } hCell;
 
-/* slm_dsts.h:694-754 (the top-K heap lives on the device, not here) */
+/* the data members of minHeap<hCell> (minheap.h:33-36); the top-K heap of
+ * this path lives on the device, so the library never touches these */
+typedef struct _hcx_heap_storage {
+    int    size;
+    int    capacity;
+    hCell *array;
+} hcx_heap_storage;
+
+/* slm_dsts.h:694-754 (56 bytes, as in the reference) */
 typedef struct _Results {
     uint_t  cpsms;
+    hcx_heap_storage topK;
     int_t   mu;
     int_t   beta;
     int_t   minhypscore;
