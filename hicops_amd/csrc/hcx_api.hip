@@ -490,6 +490,39 @@ uint64_t items_key(hcx_index *const *idx, int nidx, uint32_t chunk, uint32_t gen
     return h ^ (uint64_t)nidx;
 }
 
+uint64_t fnv1a(const void *p, size_t n, uint64_t h) {
+    const unsigned char *b = (const unsigned char *)p;
+    h ^= 1469598103934665603ull;
+    for (size_t i = 0; i < n; i++) {
+        h ^= b[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+// The device work of one search, all on the context stream with no host
+// synchronisation: zero the per-spectrum state with fill kernels (a memset
+// node is not used, so the sequence can be captured into a graph), scan,
+// replay tied spectra, tail fit.  ev0/ev1 bracket the scan kernel.
+int launch_sequence(hcx_ctx *c, ChunkArgs &ca, int kernel) {
+    QueryArgs &a = ca.q;
+    const uint64_t n = (uint64_t)a.nspec;
+    int rc = launch_fill_u32(c, (uint32_t *)ca.acc, n * (sizeof(SpecAcc) / 4), 0);
+    if (!rc) rc = launch_fill_u32(c, a.hist, n * c->hbins, 0);
+    if (!rc) rc = launch_fill_u32(c, a.tie, 1, 0);
+    if (!rc) rc = launch_fill_u32(c, ca.queue, 1, 0);
+    if (rc) return rc;
+    if (kernel == HCX_KERNEL_CURSOR)
+        rc = launch_cursor(c, a);          // writes results + tie list itself
+    else
+        rc = launch_chunked(c, ca, kernel);
+    if (rc) return rc;
+    // the cursor path keeps its own (independent) replay; the chunked forms share one
+    rc = kernel == HCX_KERNEL_CURSOR ? launch_replay_cursor(c, a) : launch_replay_chunked(c, ca);
+    if (rc) return rc;
+    return launch_tailfit(c, a.res, a.hist, a.hbins, a.nspec, c->params.min_cpsm, c->params.expect_max);
+}
+
 int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, const hcx_search_opts *opts,
                bool timed, float *kernel_ms, float *total_ms) {
     HCX_HIP(hipSetDevice(c->device));
@@ -555,25 +588,38 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
         }
         ca.items = (const Item *)qb->items.p;
         ca.nitems = qb->nitems;
+        ca.items_w = nullptr;
     }
     if (timed) HCX_HIP(hipEventRecord(c->ev2, c->stream));
-    // zero the per-spectrum accumulators, histograms, tie list and queue with
-    // fill kernels (no memset nodes: the sequence may be graph-captured)
-    rc = launch_fill_u32(c, (uint32_t *)qb->acc.p, n * (sizeof(SpecAcc) / 4), 0);
-    if (!rc) rc = launch_fill_u32(c, a.hist, n * c->hbins, 0);
-    if (!rc) rc = launch_fill_u32(c, a.tie, 1, 0);
-    if (!rc) rc = launch_fill_u32(c, ca.queue, 1, 0);
-    if (rc) return rc;
-    if (kernel == HCX_KERNEL_CURSOR)
-        rc = launch_cursor(c, a);          // writes results + tie list itself
-    else
-        rc = launch_chunked(c, ca, kernel);   // records ev0 / ev1 around the scan kernel
-    if (rc) return rc;
-    // the cursor path keeps its own (independent) replay; the chunked forms share one
-    rc = kernel == HCX_KERNEL_CURSOR ? launch_replay_cursor(c, a) : launch_replay_chunked(c, ca);
-    if (rc) return rc;
-    rc = launch_tailfit(c, a.res, a.hist, a.hbins, a.nspec, c->params.min_cpsm, c->params.expect_max);
-    if (rc) return rc;
+    if (opts && opts->use_graph) {
+        // one graph per launch shape: every pointer and size the kernels see
+        // is in ca, so its bytes (zero-padded above) key the captured graph
+        const uint64_t key = fnv1a(&ca, sizeof ca, (uint64_t)kernel * 0x9E3779B97F4A7C15ull + 1);
+        if (!c->graph_exec || c->graph_key != key) {
+            if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+            c->graph_exec = nullptr;
+            c->graph_key = 0;
+            HCX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            rc = launch_sequence(c, ca, kernel);
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (ec != hipSuccess) return set_error(std::string("search: stream capture: ") + hipGetErrorString(ec));
+            hipGraphExec_t ex = nullptr;
+            const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) return set_error(std::string("search: graph instantiate: ") + hipGetErrorString(ei));
+            c->graph_exec = ex;
+            c->graph_key = key;
+        }
+        HCX_HIP(hipGraphLaunch(c->graph_exec, c->stream));
+    } else {
+        rc = launch_sequence(c, ca, kernel);
+        if (rc) return rc;
+    }
     if (timed) {
         HCX_HIP(hipEventRecord(c->ev3, c->stream));
         HCX_HIP(hipEventSynchronize(c->ev3));
@@ -589,6 +635,61 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
 }
 
 }  // namespace
+
+extern "C" int hcx_autotune(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, int32_t kernel,
+                            hcx_tuning *best) {
+    if (!c || !qb || !best) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_autotune: null argument");
+    if (kernel != HCX_KERNEL_BATCHED && kernel != HCX_KERNEL_PLAIN)
+        return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_autotune: tunes the batched or plain form");
+    const uint32_t slots0 = c->slots, bpc0 = c->blocks_per_cu;
+    const int32_t order0 = c->item_order;
+    const uint32_t slot_c[] = {256, 512, 1024};
+    const uint32_t bpc_c[] = {1, 2, 3};
+    const int norder = kernel == HCX_KERNEL_BATCHED ? 2 : 1;
+    hcx_search_opts o = {};
+    o.kernel = kernel;
+    o.evalue = HCX_EVAL_TAILFIT;
+    hcx_tuning b = {};
+    b.kernel_ms = -1.0f;
+    for (int order = 0; order < norder; order++) {
+        if (c->item_order != order) c->item_order = order, ++c->item_gen;
+        for (uint32_t sl : slot_c) {
+            for (uint32_t bp : bpc_c) {
+                c->slots = sl;
+                c->blocks_per_cu = bp;
+                float k0 = 0, k1 = 0, t = 0;
+                // the first run builds (and caches) the work items; keep the better
+                int rc = run_search(c, idx, nidx, qb, &o, true, &k0, &t);
+                if (rc == HCX_ERR_INVLD_PARAM) continue;   // does not fit in LDS
+                if (!rc) rc = run_search(c, idx, nidx, qb, &o, true, &k1, &t);
+                if (rc) {
+                    c->slots = slots0, c->blocks_per_cu = bpc0;
+                    if (c->item_order != order0) c->item_order = order0, ++c->item_gen;
+                    return rc;
+                }
+                const float k = std::min(k0, k1);
+                if (b.kernel_ms < 0.0f || k < b.kernel_ms) {
+                    b.chunk = c->chunk;
+                    b.slots = sl;
+                    b.blocks_per_cu = bp;
+                    b.hitlist = c->hitlist;
+                    b.item_order = order;
+                    b.kernel_ms = k;
+                }
+            }
+        }
+    }
+    if (b.kernel_ms < 0.0f) {
+        c->slots = slots0, c->blocks_per_cu = bpc0;
+        if (c->item_order != order0) c->item_order = order0, ++c->item_gen;
+        return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_autotune: no candidate fits");
+    }
+    c->slots = b.slots;
+    c->blocks_per_cu = b.blocks_per_cu;
+    if (c->item_order != b.item_order) c->item_order = b.item_order, ++c->item_gen;
+    *best = b;
+    return HCX_SUCCESS;
+}
 
 extern "C" int hcx_search_device(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb,
                                  const hcx_search_opts *opts, float *kernel_ms, float *total_ms) {

@@ -76,9 +76,9 @@ struct hcx_ctx {
     int32_t item_order = 0;           // batched work items: 0 by spectrum, 1 by (length, chunk)
     uint32_t item_gen = 0;            // bumped when item_order changes (cached item lists)
     int max_lds = 65536;              // device limit per workgroup
-    // last launch shape (for graph reuse)
+    // the captured search sequence and the launch shape it was captured for
     hipGraphExec_t graph_exec = nullptr;
-    const void *graph_key = nullptr;
+    uint64_t graph_key = 0;
 };
 
 struct hcx_index {

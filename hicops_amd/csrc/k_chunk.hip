@@ -54,8 +54,11 @@ typedef const AS_GLB uint32_t gcu32;
 typedef AS_GLB uint32_t gu32;
 typedef AS_GLB unsigned long long gu64;
 
-constexpr int kBT = 512;
-// at most 128 VGPRs: four waves per SIMD, i.e. two kBT workgroups per CU
+#ifndef HCX_CHUNK_THREADS
+#define HCX_CHUNK_THREADS 512
+#endif
+constexpr int kBT = HCX_CHUNK_THREADS;   // threads per workgroup of the chunked kernels
+// at most 128 VGPRs: four waves per SIMD (two 512-thread workgroups per CU)
 constexpr int kWavesPerSimd = 4;
 constexpr uint32_t kNoProbe = 0xFFFFFFFFu;
 constexpr uint32_t kNoHit = 0xFFFFFFFFu;      // hit-list entry outside the window

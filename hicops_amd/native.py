@@ -81,6 +81,13 @@ class SearchOpts(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [
+        ("chunk", C.c_uint32), ("slots", C.c_uint32), ("blocks_per_cu", C.c_uint32),
+        ("hitlist", C.c_int32), ("item_order", C.c_int32), ("kernel_ms", C.c_float),
+    ]
+
+
 class DbGenCfg(C.Structure):
     _fields_ = [
         ("seed", C.c_uint64), ("npep", C.c_uint32), ("min_len", C.c_uint32),
@@ -151,7 +158,7 @@ def hcx() -> C.CDLL:
                      "hcx_queries_upload", "hcx_queries_free", "hcx_search",
                      "hcx_search_device", "hcx_search_fetch", "hcx_search_candidates",
                      "hcx_tailfit_batch", "hcx_lgfact_table", "hcx_heap_capacity",
-                     "hcx_selftest_divider", "hcx_set_tuning", "hcx_get_tuning", "hcx_set_option", "hcx_get_option"]:
+                     "hcx_selftest_divider", "hcx_set_tuning", "hcx_get_tuning", "hcx_set_option", "hcx_get_option", "hcx_autotune"]:
             getattr(lib, name).restype = C.c_int
         _hcx = lib
     return _hcx

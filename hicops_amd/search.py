@@ -160,6 +160,13 @@ class Engine:
         N.check(N.hcx().hcx_search_device(self._h, arr, n, batch._h, C.byref(o), C.byref(k), C.byref(t)))
         return k.value, t.value
 
+    def autotune(self, batch: Batch, kernel: int = N.KERNEL_BATCHED, indices=None) -> dict:
+        """Time the scan for each candidate tuning on this batch; keep the fastest."""
+        arr, n = self._idx_array(indices)
+        t = N.Tuning()
+        N.check(N.hcx().hcx_autotune(self._h, arr, n, batch._h, kernel, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in N.Tuning._fields_}
+
     def fetch(self, batch: Batch, want_hist=True, want_stats=True) -> SearchOutput:
         res = np.zeros(batch.nspec, N.RESULT_DTYPE)
         hist = np.zeros((batch.nspec, self.hbins), np.uint32) if want_hist else None

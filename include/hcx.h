@@ -144,7 +144,8 @@ typedef struct hcx_search_opts {
     uint32_t tile;            /* candidates per LDS tile (0 = library default)      */
     uint32_t audit_every;     /* surrogate: tail-fit every n-th spectrum (0 = 64)   */
     double   audit_tol;       /* surrogate: max |rel e error| before fallback       */
-    int32_t  use_graph;       /* capture the query launch sequence in a hipGraph    */
+    int32_t  use_graph;       /* capture the search sequence in a hipGraph, replayed
+                                 while the launch shape is unchanged                 */
     int32_t  reserved;
 } hcx_search_opts;
 
@@ -198,6 +199,22 @@ int  hcx_queries_free(hcx_qbatch *qb);
 int  hcx_search(hcx_ctx *ctx, hcx_index *const *idx, int nidx, hcx_qbatch *qb,
                 const hcx_search_opts *opts, hcx_result *results,
                 uint32_t *hist, hcx_stats *stats);
+
+/* Autotuner (no reference counterpart): times the scan kernel of `kernel`
+ * (HCX_KERNEL_BATCHED or _PLAIN) on this batch for each candidate -- slots
+ * 256/512/1024, workgroups per CU 1/2/3 and, for the batched form, both item
+ * orders -- keeps the fastest in the context and returns it.  The chunk size
+ * is the indices' own (they are built for it).  Runs 2 searches per candidate. */
+typedef struct hcx_tuning {
+    uint32_t chunk;
+    uint32_t slots;
+    uint32_t blocks_per_cu;
+    int32_t  hitlist;
+    int32_t  item_order;
+    float    kernel_ms;     /* the chosen candidate's scan-kernel time */
+} hcx_tuning;
+int  hcx_autotune(hcx_ctx *ctx, hcx_index *const *idx, int nidx, hcx_qbatch *qb, int32_t kernel,
+                  hcx_tuning *best);
 
 /* Device-resident variant for timing: runs the search and leaves results on
  * the device; hcx_search_fetch copies them out.  Returns the query kernel's

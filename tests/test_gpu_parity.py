@@ -306,3 +306,44 @@ def test_tailfit_batch_random_histograms():
             mism += 1
     assert mism == 0
     eng.close()
+
+
+def test_graph_replays_match_eager(small_case):
+    # the captured search sequence, replayed several times with eager searches
+    # and synchronisations in between; every replay equals the eager result
+    p, db, q, prm = small_case
+    log("test_graph_replays_match_eager")
+    eng = Engine(prm)
+    eng.construct_db(db)
+    b = eng.upload(q)
+    for kernel in (N.KERNEL_BATCHED, N.KERNEL_PLAIN):
+        ref = eng.query_spectrum(b, kernel=kernel)
+        for i in range(3):
+            g = eng.query_spectrum(b, kernel=kernel, use_graph=1)
+            assert np.array_equal(g.results, ref.results), (kernel, i)
+            assert np.array_equal(g.hist, ref.hist) and np.array_equal(g.stats, ref.stats), (kernel, i)
+            e = eng.query_spectrum(b, kernel=kernel)
+            assert np.array_equal(e.results, ref.results), (kernel, i)
+        k, t = eng.query_device(b, kernel=kernel, use_graph=1)
+        assert k > 0 and t >= k
+    eng.close()
+
+
+def test_autotune_keeps_parity():
+    p = W.scaled("open10m", npep=100000, nspec=300)
+    db = W.Database(p)
+    q = W.make_queries(db, p)
+    prm = W.make_params(p)
+    log("test_autotune_keeps_parity")
+    res_o, hist_o, st_o = O.OracleDB(prm, db).search(q)
+    eng = Engine(prm)
+    eng.construct_db(db)
+    b = eng.upload(q)
+    best = eng.autotune(b, N.KERNEL_BATCHED)
+    log(f"autotune: {best}")
+    assert best["kernel_ms"] > 0 and best["slots"] in (256, 512, 1024) and best["blocks_per_cu"] in (1, 2, 3)
+    t = eng.tuning()
+    assert (t["slots"], t["blocks_per_cu"]) == (best["slots"], best["blocks_per_cu"])
+    out = eng.query_spectrum(b, kernel=N.KERNEL_BATCHED)
+    compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, "autotuned")
+    eng.close()
