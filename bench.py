@@ -45,6 +45,8 @@ def main() -> int:
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batched form with the library defaults instead of the autotuned tuning")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one batched search, then a 1 GiB device copy (run under rocprofv3 --pmc)")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -99,8 +101,17 @@ def main() -> int:
     forms = ["plain", "batched"] if args.kernel == "both" else [args.kernel]
     kcode = {"plain": N.KERNEL_PLAIN, "batched": N.KERNEL_BATCHED}
     runs = {}
+    tuned = None
     for form in forms:
         opts = dict(kernel=kcode[form], tile=args.tile)
+        if form == "batched":
+            # the batched form: autotuned (outside the timed region) and replayed
+            # from a captured hipGraph
+            if not args.no_autotune:
+                t0 = time.perf_counter()
+                tuned = eng.autotune(batch, N.KERNEL_BATCHED)
+                log(f"autotune ({time.perf_counter() - t0:.1f}s): {tuned}")
+            opts["use_graph"] = 1
         for i in range(args.warmup):
             k, t = eng.query_device(batch, **opts)
             log(f"{form} warmup {i}: query kernel {k:.1f} ms, total {t:.1f} ms")
@@ -154,7 +165,8 @@ def main() -> int:
             "workload": (f"open search +-{p.dM:g} Da, {db.nforms} peptides (lengths {p.min_len}-{p.max_len}, "
                          f"fragment charge 1-{p.maxz}), {q.nspec} spectra ({p.peaks_min}-{p.peaks_max} peaks)"),
             "preset": p.name, "peptides": db.nforms, "spectra": q.nspec, "ions": nions,
-            "form": forms[-1], "tile": args.tile,
+            "form": forms[-1], "tile": args.tile, "tuning": eng.tuning(),
+            "autotuned": tuned is not None, "graph": forms[-1] == "batched",
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9, "unit": "GB/s",

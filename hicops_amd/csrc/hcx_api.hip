@@ -126,6 +126,7 @@ extern "C" int hcx_destroy(hcx_ctx *c) {
     c->work.release();
     c->sur.release();
     c->sur_tmp.release();
+    c->mlp_buf.release();
     if (c->d_lgf) (void)hipFree(c->d_lgf);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -504,7 +505,7 @@ uint64_t fnv1a(const void *p, size_t n, uint64_t h) {
 // synchronisation: zero the per-spectrum state with fill kernels (a memset
 // node is not used, so the sequence can be captured into a graph), scan,
 // replay tied spectra, tail fit.  ev0/ev1 bracket the scan kernel.
-int launch_sequence(hcx_ctx *c, ChunkArgs &ca, int kernel) {
+int launch_sequence(hcx_ctx *c, ChunkArgs &ca, int kernel, bool tailfit = true) {
     QueryArgs &a = ca.q;
     const uint64_t n = (uint64_t)a.nspec;
     int rc = launch_fill_u32(c, (uint32_t *)ca.acc, n * (sizeof(SpecAcc) / 4), 0);
@@ -519,7 +520,7 @@ int launch_sequence(hcx_ctx *c, ChunkArgs &ca, int kernel) {
     if (rc) return rc;
     // the cursor path keeps its own (independent) replay; the chunked forms share one
     rc = kernel == HCX_KERNEL_CURSOR ? launch_replay_cursor(c, a) : launch_replay_chunked(c, ca);
-    if (rc) return rc;
+    if (rc || !tailfit) return rc;
     return launch_tailfit(c, a.res, a.hist, a.hbins, a.nspec, c->params.min_cpsm, c->params.expect_max);
 }
 
@@ -531,8 +532,10 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
     int rc = prepare(c, idx, nidx, qb, opts, &ca.q);
     if (rc) return rc;
     QueryArgs &a = ca.q;
-    if (opts && opts->evalue != HCX_EVAL_TAILFIT)
-        return set_error_code(HCX_ERR_INVLD_PARAM, "search: only HCX_EVAL_TAILFIT e-values are available");
+    const bool sur = opts && opts->evalue == HCX_EVAL_SURROGATE;
+    if (opts && opts->evalue != HCX_EVAL_TAILFIT && !sur)
+        return set_error_code(HCX_ERR_INVLD_PARAM, "search: unknown e-value method");
+    if (sur && !c->sur_ready) return set_error_code(HCX_ERR_INVLD_PARAM, "search: the surrogate is not trained");
     int kernel = opts ? opts->kernel : HCX_KERNEL_PLAIN;
     if (c->params.min_shp > kMaxChunkShp) kernel = HCX_KERNEL_CURSOR;
     const uint64_t n = (uint64_t)qb->nspec;
@@ -591,7 +594,34 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
         ca.items_w = nullptr;
     }
     if (timed) HCX_HIP(hipEventRecord(c->ev2, c->stream));
-    if (opts && opts->use_graph) {
+    if (sur) {
+        // scan, then predicted e-values, the audit sample's plain fits, and
+        // the plain fit for everything if the audit finds a disagreement
+        rc = launch_sequence(c, ca, kernel, false);
+        if (!rc) rc = mlp_infer(c, qb);
+        uint32_t *cnt = (uint32_t *)c->work.p + 4;
+        if (!rc) rc = launch_fill_u32(c, cnt, 4, 0);
+        const uint32_t every = opts->audit_every ? opts->audit_every : 64;
+        const double tol = opts->audit_tol > 0.0 ? opts->audit_tol : 1e-3;
+        if (!rc)
+            rc = launch_tailfit_audit(c, a.res, a.hist, a.hbins, a.nspec, c->params.min_cpsm, c->params.expect_max,
+                                      every, tol, cnt);
+        if (rc) return rc;
+        uint32_t h[4];
+        HCX_HIP(hipMemcpyAsync(h, cnt, 16, hipMemcpyDeviceToHost, c->stream));
+        HCX_HIP(hipStreamSynchronize(c->stream));
+        hcx_audit au = {};
+        au.audited = h[0];
+        au.disagreed = h[1];
+        uint64_t eb = (uint64_t)h[2] | ((uint64_t)h[3] << 32);
+        std::memcpy(&au.max_rel_err, &eb, 8);
+        au.fell_back = h[1] > 0 ? 1 : 0;
+        c->last_audit = au;
+        if (au.fell_back) {
+            rc = launch_tailfit(c, a.res, a.hist, a.hbins, a.nspec, c->params.min_cpsm, c->params.expect_max);
+            if (rc) return rc;
+        }
+    } else if (opts && opts->use_graph) {
         // one graph per launch shape: every pointer and size the kernels see
         // is in ca, so its bytes (zero-padded above) key the captured graph
         const uint64_t key = fnv1a(&ca, sizeof ca, (uint64_t)kernel * 0x9E3779B97F4A7C15ull + 1);
@@ -635,6 +665,78 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
 }
 
 }  // namespace
+
+extern "C" int hcx_surrogate_param_count(void) { return (int)mlp_param_count(); }
+
+extern "C" int hcx_surrogate_init(hcx_ctx *c, uint64_t seed) {
+    if (!c) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_surrogate_init: null context");
+    HCX_HIP(hipSetDevice(c->device));
+    // He-uniform weights (fan-in), zero biases; layout W1 b1 W2 b2 W3 b3
+    const uint32_t P = mlp_param_count();
+    std::vector<float> p(P, 0.0f);
+    uint64_t st = seed ? seed : 0x5EED;
+    auto uni = [&]() {
+        st += 0x9E3779B97F4A7C15ull;
+        uint64_t z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (double)(z >> 11) / 9007199254740992.0 * 2.0 - 1.0;
+    };
+    const uint32_t H = 128, F = 128, O = 16;
+    size_t o = 0;
+    const double a1 = std::sqrt(6.0 / F), a2 = std::sqrt(6.0 / H);
+    for (uint32_t i = 0; i < H * F; i++) p[o++] = (float)(a1 * uni());
+    o += H;
+    for (uint32_t i = 0; i < H * H; i++) p[o++] = (float)(a2 * uni());
+    o += H;
+    for (uint32_t r = 0; r < O; r++)
+        for (uint32_t k = 0; k < H; k++) p[o++] = r < 3 ? (float)(a2 * uni()) : 0.0f;
+    const float norm[6] = {0, 0, 0, 1, 1, 1};
+    return mlp_set(c, p.data(), norm);
+}
+
+extern "C" int hcx_surrogate_set(hcx_ctx *c, const float *params, uint32_t n, const float *norm6) {
+    if (!c || !params || !norm6) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_surrogate_set: null argument");
+    if (n != mlp_param_count()) return set_error_code(HCX_ERR_INVLD_SIZE, "hcx_surrogate_set: wrong parameter count");
+    HCX_HIP(hipSetDevice(c->device));
+    return mlp_set(c, params, norm6);
+}
+
+extern "C" int hcx_surrogate_get(hcx_ctx *c, float *params, uint32_t n, float *norm6) {
+    if (!c || !params || !norm6) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_surrogate_get: null argument");
+    if (n != mlp_param_count()) return set_error_code(HCX_ERR_INVLD_SIZE, "hcx_surrogate_get: wrong parameter count");
+    HCX_HIP(hipSetDevice(c->device));
+    return mlp_get(c, params, norm6);
+}
+
+extern "C" int hcx_surrogate_train(hcx_ctx *c, hcx_qbatch *qb, uint32_t steps, uint32_t batch, float lr, uint64_t seed,
+                                   float *loss) {
+    if (!c || !qb) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_surrogate_train: null argument");
+    if (qb->ctx != c) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_surrogate_train: batch from another context");
+    if (steps == 0 || !(lr > 0.0f)) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_surrogate_train: steps, lr");
+    HCX_HIP(hipSetDevice(c->device));
+    return mlp_train(c, qb, steps, batch, lr, seed, loss);
+}
+
+extern "C" int hcx_surrogate_gradient(hcx_ctx *c, const float *X, const float *T, uint32_t n, float *grad,
+                                      float *loss) {
+    if (!c || !X || !T || !grad || !loss) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_surrogate_gradient: null");
+    HCX_HIP(hipSetDevice(c->device));
+    return mlp_gradient(c, X, T, n, grad, loss);
+}
+
+extern "C" int hcx_surrogate_last_audit(const hcx_ctx *c, hcx_audit *out) {
+    if (!c || !out) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_surrogate_last_audit: null argument");
+    *out = c->last_audit;
+    return HCX_SUCCESS;
+}
+
+extern "C" int hcx_selftest_mfma(hcx_ctx *c, const float *A, const float *B, float *D) {
+    if (!c || !A || !B || !D) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_selftest_mfma: null argument");
+    HCX_HIP(hipSetDevice(c->device));
+    return selftest_mfma(c, A, B, D);
+}
 
 extern "C" int hcx_autotune(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, int32_t kernel,
                             hcx_tuning *best) {

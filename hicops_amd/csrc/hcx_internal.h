@@ -66,7 +66,12 @@ struct hcx_ctx {
     hcx::DeviceBuf tie;               // tie list: count + spectrum ids
     hcx::DeviceBuf work;              // batched kernel queue / counters
     hcx::DeviceBuf sur;               // surrogate parameters + optimizer state
-    hcx::DeviceBuf sur_tmp;
+    hcx::DeviceBuf sur_tmp;           // surrogate features / targets / outputs of a batch
+    hcx::DeviceBuf mlp_buf;           // training activations
+    float *mlp_loss = nullptr, *mlp_Y = nullptr;
+    uint16_t *mlp_H1 = nullptr, *mlp_H2 = nullptr;
+    uint32_t sur_step = 0;            // Adam steps taken
+    hcx_audit last_audit = {};        // audit of the last surrogate search
     hcx::DeviceBuf scan_tmp;          // device scan scratch
     bool sur_ready = false;
     uint32_t chunk = 16384;           // peptides per query chunk (power of two)
@@ -183,6 +188,21 @@ uint64_t scan_tmp_words(uint64_t n);
 
 int launch_tailfit(hcx_ctx *ctx, hcx_result *res, const uint32_t *hist, uint32_t hbins, int32_t n,
                    uint32_t min_cpsm, double expect_max);
+// audit of surrogate e-values: the plain fit for every stride-th spectrum,
+// compared with the surrogate's e-value; cnt[0] audited, cnt[1] disagreeing
+// (relative error > tol), cnt[2..3] largest relative error (double bits).
+int launch_tailfit_audit(hcx_ctx *ctx, hcx_result *res, const uint32_t *hist, uint32_t hbins, int32_t n,
+                         uint32_t min_cpsm, double expect_max, uint32_t stride, double tol, uint32_t *cnt);
+// ---- surrogate (k_mlp.hip) ----
+uint32_t mlp_param_count();
+int mlp_alloc(hcx_ctx *c);
+int mlp_set(hcx_ctx *c, const float *params, const float *norm6);
+int mlp_get(hcx_ctx *c, float *params, float *norm6);
+int mlp_infer(hcx_ctx *c, hcx_qbatch *qb);
+int mlp_gradient(hcx_ctx *c, const float *X, const float *T, uint32_t n, float *grad, float *loss);
+int mlp_train(hcx_ctx *c, hcx_qbatch *qb, uint32_t steps, uint32_t batch, float lr, uint64_t seed, float *loss_out);
+int selftest_mfma(hcx_ctx *c, const float *A, const float *B, float *D);
+
 int launch_tailfit_raw(hcx_ctx *ctx, const uint32_t *hist, uint32_t hbins, uint32_t n, const uint32_t *cpsms,
                        const int32_t *maxhyp, int32_t *out4, uint32_t *flags);
 

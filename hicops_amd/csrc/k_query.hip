@@ -687,7 +687,7 @@ __global__ __launch_bounds__(kBT) void k_tailfit(hcx_result *res, const uint32_t
     if (s >= n) return;
     double *sx = (double *)smem_raw + (size_t)w * hbins;
     hcx_result r = res[s];
-    uint16_t flags = r.flags & ~(HCX_RES_VALID | HCX_RES_REPORTED);
+    uint16_t flags = r.flags & ~(HCX_RES_VALID | HCX_RES_REPORTED | HCX_RES_SURROGATE);
     if (r.cpsms < min_cpsm || r.cpsms == 0) {
         if (lane == 0) res[s].flags = flags;
         return;
@@ -708,6 +708,45 @@ __global__ __launch_bounds__(kBT) void k_tailfit(hcx_result *res, const uint32_t
         e *= (double)r.cpsms;
         r.e_x = e;
         flags |= (uint16_t)f.flags | HCX_RES_VALID;
+        if (e < expect_max) flags |= HCX_RES_REPORTED;
+        r.flags = flags;
+        res[s] = r;
+    }
+}
+
+// The plain fit of every stride-th spectrum with cpsms >= min_cpsm, compared
+// with the surrogate e-value already in res[s]; the plain values replace it
+// (flag HCX_RES_AUDITED).
+__global__ __launch_bounds__(kBT) void k_tailfit_audit(hcx_result *res, const uint32_t *hist, uint32_t hbins,
+                                                       int32_t n, uint32_t min_cpsm, double expect_max,
+                                                       uint32_t stride, double tol, uint32_t *cnt) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t s = ((int64_t)blockIdx.x * (kBT / 64) + w) * stride;
+    if (s >= n) return;
+    double *sx = (double *)smem_raw + (size_t)w * hbins;
+    hcx_result r = res[s];
+    if (r.cpsms < min_cpsm || r.cpsms == 0) return;
+    const int32_t hyp = (int32_t)((double)(r.hyperscore * 10.0f) + 0.5);
+    FitOut f = tailfit_wave(hist + (size_t)s * hbins, hbins, r.cpsms, hyp, sx);
+    if (lane == 0) {
+        double w_ = f.mu, b_ = f.beta;
+        w_ /= 1e6;
+        b_ /= 1e6;
+        double e = pow(10.0, (w_ * hyp) + b_);
+        e *= (double)r.cpsms;
+        const double err = fabs(r.e_x - e) / fmax(fabs(e), 1e-300);
+        atomicAdd(&cnt[0], 1u);
+        if (!(err <= tol)) atomicAdd(&cnt[1], 1u);
+        atomicMax((unsigned long long *)&cnt[2], (unsigned long long)__double_as_longlong(err == err ? err : 1e300));
+        r.maxhypscore = hyp;
+        r.mu = f.mu;
+        r.beta = f.beta;
+        r.minhypscore = f.stt1;
+        r.nexthypscore = f.end1;
+        r.e_x = e;
+        uint16_t flags = r.flags & ~(HCX_RES_REPORTED | HCX_RES_SURROGATE);
+        flags |= (uint16_t)f.flags | HCX_RES_VALID | HCX_RES_AUDITED;
         if (e < expect_max) flags |= HCX_RES_REPORTED;
         r.flags = flags;
         res[s] = r;
@@ -779,6 +818,19 @@ int launch_tailfit(hcx_ctx *ctx, hcx_result *res, const uint32_t *hist, uint32_t
     HCX_HIP(hipFuncSetAttribute((const void *)k_tailfit, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
     const int per = kBT / 64;
     k_tailfit<<<(n + per - 1) / per, kBT, sm, ctx->stream>>>(res, hist, hbins, n, min_cpsm, expect_max);
+    HCX_HIP(hipGetLastError());
+    return HCX_SUCCESS;
+}
+
+int launch_tailfit_audit(hcx_ctx *ctx, hcx_result *res, const uint32_t *hist, uint32_t hbins, int32_t n,
+                         uint32_t min_cpsm, double expect_max, uint32_t stride, double tol, uint32_t *cnt) {
+    if (n <= 0) return HCX_SUCCESS;
+    const size_t sm = (size_t)(kBT / 64) * hbins * sizeof(double);
+    HCX_HIP(hipFuncSetAttribute((const void *)k_tailfit_audit, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+    const int per = kBT / 64;
+    const int64_t m = ((int64_t)n + stride - 1) / stride;
+    k_tailfit_audit<<<(unsigned)((m + per - 1) / per), kBT, sm, ctx->stream>>>(res, hist, hbins, n, min_cpsm,
+                                                                             expect_max, stride, tol, cnt);
     HCX_HIP(hipGetLastError());
     return HCX_SUCCESS;
 }

@@ -88,6 +88,13 @@ class Tuning(C.Structure):
     ]
 
 
+class Audit(C.Structure):
+    _fields_ = [
+        ("audited", C.c_uint32), ("disagreed", C.c_uint32), ("max_rel_err", C.c_double),
+        ("fell_back", C.c_int32), ("pad_", C.c_int32),
+    ]
+
+
 class DbGenCfg(C.Structure):
     _fields_ = [
         ("seed", C.c_uint64), ("npep", C.c_uint32), ("min_len", C.c_uint32),
@@ -158,7 +165,10 @@ def hcx() -> C.CDLL:
                      "hcx_queries_upload", "hcx_queries_free", "hcx_search",
                      "hcx_search_device", "hcx_search_fetch", "hcx_search_candidates",
                      "hcx_tailfit_batch", "hcx_lgfact_table", "hcx_heap_capacity",
-                     "hcx_selftest_divider", "hcx_set_tuning", "hcx_get_tuning", "hcx_set_option", "hcx_get_option", "hcx_autotune"]:
+                     "hcx_selftest_divider", "hcx_set_tuning", "hcx_get_tuning", "hcx_set_option", "hcx_get_option", "hcx_autotune",
+                     "hcx_surrogate_param_count", "hcx_surrogate_init", "hcx_surrogate_set",
+                     "hcx_surrogate_get", "hcx_surrogate_train", "hcx_surrogate_gradient",
+                     "hcx_surrogate_last_audit", "hcx_selftest_mfma"]:
             getattr(lib, name).restype = C.c_int
         _hcx = lib
     return _hcx

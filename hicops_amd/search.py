@@ -134,10 +134,11 @@ class Engine:
         return arr, len(indices)
 
     @staticmethod
-    def _opts(kernel=N.KERNEL_PLAIN, evalue=N.EVAL_TAILFIT, tile=0, use_graph=0) -> N.SearchOpts:
+    def _opts(kernel=N.KERNEL_PLAIN, evalue=N.EVAL_TAILFIT, tile=0, use_graph=0, audit_every=0,
+              audit_tol=1e-3) -> N.SearchOpts:
         o = N.SearchOpts()
         o.kernel, o.evalue, o.tile, o.use_graph = kernel, evalue, tile, use_graph
-        o.audit_every, o.audit_tol = 0, 1e-3
+        o.audit_every, o.audit_tol = audit_every, audit_tol
         return o
 
     # DSLIM_QuerySpectrum + ModelTailFit + e-value
@@ -166,6 +167,52 @@ class Engine:
         t = N.Tuning()
         N.check(N.hcx().hcx_autotune(self._h, arr, n, batch._h, kernel, C.byref(t)))
         return {f: getattr(t, f) for f, _ in N.Tuning._fields_}
+
+    # ---- learned tail-parameter surrogate ----
+    def surrogate_init(self, seed: int = 1) -> None:
+        N.check(N.hcx().hcx_surrogate_init(self._h, C.c_uint64(seed)))
+
+    def surrogate_train(self, batch: Batch, steps: int = 2000, batch_size: int = 1024, lr: float = 1e-3,
+                        seed: int = 7) -> float:
+        """Train on `batch`, which must have been searched with the plain fit."""
+        loss = C.c_float()
+        N.check(N.hcx().hcx_surrogate_train(self._h, batch._h, C.c_uint32(steps), C.c_uint32(batch_size),
+                                            C.c_float(lr), C.c_uint64(seed), C.byref(loss)))
+        return loss.value
+
+    def surrogate_params(self) -> tuple[np.ndarray, np.ndarray]:
+        n = N.hcx().hcx_surrogate_param_count()
+        p = np.zeros(n, np.float32)
+        nrm = np.zeros(6, np.float32)
+        N.check(N.hcx().hcx_surrogate_get(self._h, N.ptr(p), C.c_uint32(n), N.ptr(nrm)))
+        return p, nrm
+
+    def surrogate_set(self, params: np.ndarray, norm6: np.ndarray) -> None:
+        p = np.ascontiguousarray(params, np.float32)
+        nrm = np.ascontiguousarray(norm6, np.float32)
+        N.check(N.hcx().hcx_surrogate_set(self._h, N.ptr(p), C.c_uint32(len(p)), N.ptr(nrm)))
+
+    def surrogate_gradient(self, X: np.ndarray, T: np.ndarray) -> tuple[np.ndarray, float]:
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        g = np.zeros(N.hcx().hcx_surrogate_param_count(), np.float32)
+        loss = C.c_float()
+        N.check(N.hcx().hcx_surrogate_gradient(self._h, N.ptr(X), N.ptr(T), C.c_uint32(len(X)), N.ptr(g),
+                                               C.byref(loss)))
+        return g, loss.value
+
+    def last_audit(self) -> dict:
+        a = N.Audit()
+        N.check(N.hcx().hcx_surrogate_last_audit(self._h, C.byref(a)))
+        return {"audited": a.audited, "disagreed": a.disagreed, "max_rel_err": a.max_rel_err,
+                "fell_back": bool(a.fell_back)}
+
+    def selftest_mfma(self, A: np.ndarray, B: np.ndarray) -> np.ndarray:
+        A = np.ascontiguousarray(A, np.float32)
+        B = np.ascontiguousarray(B, np.float32)
+        D = np.zeros((16, 16), np.float32)
+        N.check(N.hcx().hcx_selftest_mfma(self._h, N.ptr(A), N.ptr(B), N.ptr(D)))
+        return D
 
     def fetch(self, batch: Batch, want_hist=True, want_stats=True) -> SearchOutput:
         res = np.zeros(batch.nspec, N.RESULT_DTYPE)

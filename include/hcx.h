@@ -20,6 +20,7 @@
  *                                 (dslim_query.cpp:771-818); results in place of DFile_PrintScore
  *                                 (dslim_fileout.h:29)
  *   hcx_tailfit_batch             expeRT::ModelTailFit(Results*) over a batch of histograms
+ *   hcx_surrogate_*               no counterpart: learned tail-parameter model with audit fallback
  */
 #ifndef HCX_H
 #define HCX_H
@@ -135,8 +136,7 @@ typedef struct hcx_qbatch hcx_qbatch;
 
 /* E-value model choice. */
 #define HCX_EVAL_TAILFIT   0  /* expeRT log-linear tail fit on every spectrum        */
-#define HCX_EVAL_SURROGATE 1  /* MLP prediction, tail fit on the audit sample (not
-                                 available yet: hcx_search returns HCX_ERR_INVLD_PARAM) */
+#define HCX_EVAL_SURROGATE 1  /* MLP prediction, tail fit on the audit sample        */
 
 typedef struct hcx_search_opts {
     int32_t  kernel;          /* HCX_KERNEL_*                                       */
@@ -248,6 +248,44 @@ int  hcx_tailfit_batch(hcx_ctx *ctx, const uint32_t *hist, uint32_t hbins, uint3
                        const uint32_t *cpsms, const int32_t *maxhyp,
                        int32_t *mu, int32_t *beta, int32_t *minhyp, int32_t *nexthyp,
                        uint32_t *flags);
+
+/* ---- learned tail-parameter surrogate (no reference counterpart) ----
+ * A 128-128-128-3 MLP (BF16 MFMA, FP32 accumulate) maps a spectrum's score
+ * histogram to the tail-fit slope, intercept and log10 s(maxhyp); it is
+ * trained on pairs the plain fit (expeRT::ModelTailFit) produced.  With
+ * opts.evalue = HCX_EVAL_SURROGATE, hcx_search predicts every e-value, runs
+ * the plain fit on every audit_every-th spectrum (0: 64), reports the plain
+ * values for those (HCX_RES_AUDITED), and falls back to the plain fit for the
+ * whole batch when any audited e-value differs by more than audit_tol
+ * (relative; 0: 1e-3).  hcx_surrogate_last_audit reports what happened.   */
+typedef struct hcx_audit {
+    uint32_t audited;       /* spectra with a plain fit in the sample          */
+    uint32_t disagreed;     /* of those, relative e-value error > audit_tol     */
+    double   max_rel_err;   /* largest relative error in the sample            */
+    int32_t  fell_back;     /* 1: the batch got plain e-values throughout      */
+    int32_t  pad_;
+} hcx_audit;
+/* number of float parameters (weights + biases) of the model */
+int  hcx_surrogate_param_count(void);
+/* He-uniform initial weights from a seed; targets unstandardised */
+int  hcx_surrogate_init(hcx_ctx *ctx, uint64_t seed);
+/* parameters in and out: n = hcx_surrogate_param_count() floats, plus the
+ * target standardisation (mean[3], std[3]) */
+int  hcx_surrogate_set(hcx_ctx *ctx, const float *params, uint32_t n, const float *norm6);
+int  hcx_surrogate_get(hcx_ctx *ctx, float *params, uint32_t n, float *norm6);
+/* train on a batch last searched with HCX_EVAL_TAILFIT (its histograms and
+ * fits are on the device): `steps` Adam steps over mini-batches of `batch`
+ * spectra.  *loss: mean standardised squared error over the last 10 % of steps. */
+int  hcx_surrogate_train(hcx_ctx *ctx, hcx_qbatch *qb, uint32_t steps, uint32_t batch, float lr,
+                         uint64_t seed, float *loss);
+/* one forward + backward pass with the current parameters: X n x 128
+ * features (rounded to bf16 on the device), T n x 4 (three standardised
+ * targets + weight), n a multiple of 16; grad receives dL/dparams.       */
+int  hcx_surrogate_gradient(hcx_ctx *ctx, const float *X, const float *T, uint32_t n,
+                            float *grad, float *loss);
+int  hcx_surrogate_last_audit(const hcx_ctx *ctx, hcx_audit *out);
+/* lane-layout self-check of the BF16 MFMA tile: D = A B, 16 x 16 row-major */
+int  hcx_selftest_mfma(hcx_ctx *ctx, const float *A, const float *B, float *D);
 
 /* ---- host self-test of the raw / speclen divider the query kernels use ---- */
 int  hcx_selftest_divider(uint32_t d, const uint32_t *nums, uint32_t count, uint32_t *out);
