@@ -91,6 +91,8 @@ def main() -> int:
     batch = eng.upload(q)
 
     if args.pmc_probe:
+        if not args.no_autotune:
+            log(f"pmc probe autotune: {eng.autotune(batch, N.KERNEL_BATCHED)}")
         eng.query_device(batch, kernel=N.KERNEL_BATCHED)
         k, t = eng.query_device(batch, kernel=N.KERNEL_BATCHED)
         log(f"pmc probe: batched query kernel {k:.1f} ms")
@@ -98,33 +100,54 @@ def main() -> int:
         eng.close()
         return 0
 
-    forms = ["plain", "batched"] if args.kernel == "both" else [args.kernel]
-    kcode = {"plain": N.KERNEL_PLAIN, "batched": N.KERNEL_BATCHED}
+    # forms: plain (one workgroup per spectrum, plain tail fit); batched (the
+    # persistent work-stealing kernel, autotuned, plain tail fit);
+    # batched+surrogate (the same scan, e-values from the learned model with
+    # the sampled audit and its fallback).  `value` is the last form run.
+    # Each step is a separate search launch so that the scan kernel is timed
+    # on its own (a graph replay, hcx_search_opts.use_graph, is timed whole).
+    forms = {"plain": ["plain"], "batched": ["batched", "batched+surrogate"],
+             "both": ["plain", "batched", "batched+surrogate"]}[args.kernel]
     runs = {}
     tuned = None
+    sur_info = None
     for form in forms:
-        opts = dict(kernel=kcode[form], tile=args.tile)
+        opts = dict(kernel=N.KERNEL_PLAIN if form == "plain" else N.KERNEL_BATCHED, tile=args.tile)
         if form == "batched":
-            # the batched form: autotuned (outside the timed region) and replayed
-            # from a captured hipGraph
             if not args.no_autotune:
                 t0 = time.perf_counter()
                 tuned = eng.autotune(batch, N.KERNEL_BATCHED)
                 log(f"autotune ({time.perf_counter() - t0:.1f}s): {tuned}")
-            opts["use_graph"] = 1
+        if form == "batched+surrogate":
+            # trained on plain fits of a separate batch of 20 000 spectra
+            t0 = time.perf_counter()
+            qt = W.make_queries(db, p, nspec=min(20000, q.nspec), seed=p.seed ^ 0x7A11)
+            bt = eng.upload(qt)
+            eng.query_device(bt, kernel=N.KERNEL_BATCHED)
+            eng.surrogate_init(seed=3)
+            loss = eng.surrogate_train(bt, steps=3000, batch_size=256, lr=2e-3)
+            bt.free()
+            sur_info = {"train_spectra": qt.nspec, "train_steps": 3000, "train_loss": round(loss, 4),
+                        "train_s": round(time.perf_counter() - t0, 2)}
+            log(f"surrogate trained: {sur_info}")
+            opts.update(evalue=N.EVAL_SURROGATE, audit_every=64, audit_tol=1e-3)
         for i in range(args.warmup):
             k, t = eng.query_device(batch, **opts)
             log(f"{form} warmup {i}: query kernel {k:.1f} ms, total {t:.1f} ms")
-        kms, tms = [], []
+        kms, tms, ems = [], [], []
         t0 = time.perf_counter()
         for i in range(args.steps):
             k, t = eng.query_device(batch, **opts)
             kms.append(k)
             tms.append(t)
-            log(f"{form} step {i}: query kernel {k:.1f} ms, total {t:.1f} ms")
+            ems.append(eng.last_timing()["evalue_ms"])
+            log(f"{form} step {i}: query kernel {k:.1f} ms, total {t:.1f} ms, e-values {ems[-1]:.2f} ms")
         wall = time.perf_counter() - t0
         out = eng.fetch(batch, want_hist=False, want_stats=True)
-        runs[form] = dict(wall=wall, kms=float(np.mean(kms)), tms=float(np.mean(tms)), out=out)
+        runs[form] = dict(wall=wall, kms=float(np.mean(kms)), tms=float(np.mean(tms)), ems=float(np.mean(ems)),
+                          out=out)
+        if form == "batched+surrogate":
+            sur_info["audit"] = eng.last_audit()
 
     head = runs[forms[-1]]
     st = head["out"].stats
@@ -137,7 +160,8 @@ def main() -> int:
     valid = int(((res["flags"] & N.RES_VALID) != 0).sum())
 
     traffic = None
-    tfile = ROOT / "profiles" / f"traffic_{args.preset}_{forms[-1]}.json"
+    tname = "plain" if forms[-1] == "plain" else "batched"   # both batched forms run the same scan
+    tfile = ROOT / "profiles" / f"traffic_{args.preset}_{tname}.json"
     if tfile.exists() and not kw:
         try:
             traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
@@ -148,6 +172,10 @@ def main() -> int:
     if not args.no_cpu:
         cpu = cpu_baseline(prm, db, idx, q, st, args.cpu_seconds)
 
+    if sur_info is not None:
+        sur_info["surrogate_evalue_ms"] = round(runs["batched+surrogate"]["ems"], 3)
+        if "batched" in runs:
+            sur_info["plain_fit_ms"] = round(runs["batched"]["ems"], 3)
     line = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -166,7 +194,8 @@ def main() -> int:
                          f"fragment charge 1-{p.maxz}), {q.nspec} spectra ({p.peaks_min}-{p.peaks_max} peaks)"),
             "preset": p.name, "peptides": db.nforms, "spectra": q.nspec, "ions": nions,
             "form": forms[-1], "tile": args.tile, "tuning": eng.tuning(),
-            "autotuned": tuned is not None, "graph": forms[-1] == "batched",
+            "autotuned": tuned is not None,
+            "evalue": "surrogate+audit" if forms[-1] == "batched+surrogate" else "tailfit",
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
@@ -175,8 +204,10 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "forms": {f: {"spectra_per_s": round(q.nspec * args.steps / r["wall"], 2),
-                      "query_kernel_ms": round(r["kms"], 3), "search_ms": round(r["tms"], 3)}
+                      "query_kernel_ms": round(r["kms"], 3), "search_ms": round(r["tms"], 3),
+                      "evalue_ms": round(r["ems"], 3)}
                   for f, r in runs.items()},
+        "surrogate": sur_info,
         "index_build": {"ions": nions, "device_ms": round(build_ms, 2),
                         "ions_per_s": round(nions / (build_ms / 1e3), 1),
                         "frac_hbm": round(b_build / (build_ms / 1e3) / HBM_PEAK, 4)},

@@ -103,7 +103,8 @@ extern "C" int hcx_create(const hcx_params *params, int device, hcx_ctx **out) {
     c->chunk = c->max_lds >= 96 * 1024 ? 16384 : 8192;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail("stream");
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-        hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess)
+        hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess ||
+        hipEventCreate(&c->ev4) != hipSuccess || hipEventCreate(&c->ev5) != hipSuccess)
         return fail("events");
     // utils.h:115-131 lgfact table: val[i] = log10(i) + val[i-1]
     c->h_lgf.assign(p.max_shdpeaks + 1, 0.0);
@@ -132,6 +133,8 @@ extern "C" int hcx_destroy(hcx_ctx *c) {
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev2) (void)hipEventDestroy(c->ev2);
     if (c->ev3) (void)hipEventDestroy(c->ev3);
+    if (c->ev4) (void)hipEventDestroy(c->ev4);
+    if (c->ev5) (void)hipEventDestroy(c->ev5);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return HCX_SUCCESS;
@@ -521,7 +524,11 @@ int launch_sequence(hcx_ctx *c, ChunkArgs &ca, int kernel, bool tailfit = true) 
     // the cursor path keeps its own (independent) replay; the chunked forms share one
     rc = kernel == HCX_KERNEL_CURSOR ? launch_replay_cursor(c, a) : launch_replay_chunked(c, ca);
     if (rc || !tailfit) return rc;
-    return launch_tailfit(c, a.res, a.hist, a.hbins, a.nspec, c->params.min_cpsm, c->params.expect_max);
+    if (!c->capturing) HCX_HIP(hipEventRecord(c->ev4, c->stream));
+    rc = launch_tailfit(c, a.res, a.hist, a.hbins, a.nspec, c->params.min_cpsm, c->params.expect_max);
+    if (rc) return rc;
+    if (!c->capturing) HCX_HIP(hipEventRecord(c->ev5, c->stream));
+    return HCX_SUCCESS;
 }
 
 int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, const hcx_search_opts *opts,
@@ -598,7 +605,9 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
         // scan, then predicted e-values, the audit sample's plain fits, and
         // the plain fit for everything if the audit finds a disagreement
         rc = launch_sequence(c, ca, kernel, false);
+        if (!rc) HCX_HIP(hipEventRecord(c->ev4, c->stream));
         if (!rc) rc = mlp_infer(c, qb);
+        if (!rc) HCX_HIP(hipEventRecord(c->ev5, c->stream));
         uint32_t *cnt = (uint32_t *)c->work.p + 4;
         if (!rc) rc = launch_fill_u32(c, cnt, 4, 0);
         const uint32_t every = opts->audit_every ? opts->audit_every : 64;
@@ -630,7 +639,9 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
             c->graph_exec = nullptr;
             c->graph_key = 0;
             HCX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            c->capturing = true;   // no event records inside the graph
             rc = launch_sequence(c, ca, kernel);
+            c->capturing = false;
             hipGraph_t g = nullptr;
             const hipError_t ec = hipStreamEndCapture(c->stream, &g);
             if (rc) {
@@ -645,7 +656,13 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
             c->graph_exec = ex;
             c->graph_key = key;
         }
+        // a replayed graph is timed as a whole: scan and e-value times both
+        // cover the captured sequence
+        HCX_HIP(hipEventRecord(c->ev0, c->stream));
+        HCX_HIP(hipEventRecord(c->ev4, c->stream));
         HCX_HIP(hipGraphLaunch(c->graph_exec, c->stream));
+        HCX_HIP(hipEventRecord(c->ev1, c->stream));
+        HCX_HIP(hipEventRecord(c->ev5, c->stream));
     } else {
         rc = launch_sequence(c, ca, kernel);
         if (rc) return rc;
@@ -658,6 +675,7 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
         HCX_HIP(hipEventElapsedTime(&t, c->ev2, c->ev3));
         if (kernel_ms) *kernel_ms = q;
         if (total_ms) *total_ms = t;
+        c->timed_ok = true;
     } else {
         HCX_HIP(hipStreamSynchronize(c->stream));
     }
@@ -665,6 +683,19 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
 }
 
 }  // namespace
+
+extern "C" int hcx_last_timing(hcx_ctx *c, float *scan_ms, float *evalue_ms, float *total_ms) {
+    if (!c) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_last_timing: null context");
+    if (!c->timed_ok) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_last_timing: no timed search yet");
+    float q = 0, e = 0, t = 0;
+    HCX_HIP(hipEventElapsedTime(&q, c->ev0, c->ev1));
+    HCX_HIP(hipEventElapsedTime(&e, c->ev4, c->ev5));
+    HCX_HIP(hipEventElapsedTime(&t, c->ev2, c->ev3));
+    if (scan_ms) *scan_ms = q;
+    if (evalue_ms) *evalue_ms = e;
+    if (total_ms) *total_ms = t;
+    return HCX_SUCCESS;
+}
 
 extern "C" int hcx_surrogate_param_count(void) { return (int)mlp_param_count(); }
 

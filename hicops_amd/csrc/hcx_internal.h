@@ -55,6 +55,9 @@ struct hcx_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+    hipEvent_t ev4 = nullptr, ev5 = nullptr;   // around the e-value stage
+    bool timed_ok = false;                     // the last search recorded all six events
+    bool capturing = false;                    // launchers record no events (graph capture)
     double *d_lgf = nullptr;          // max_shdpeaks + 1
     std::vector<double> h_lgf;
     uint32_t NB = 0;                  // max_mass * scale

@@ -1091,16 +1091,16 @@ int launch_chunked(hcx_ctx *ctx, ChunkArgs &a, int kernel) {
     if (n <= 0) return HCX_SUCCESS;
     if (kernel == HCX_KERNEL_BATCHED) {
         HCX_HIP(hipFuncSetAttribute((const void *)k_items_chunked, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
-        HCX_HIP(hipEventRecord(ctx->ev0, st));
+        if (!ctx->capturing) HCX_HIP(hipEventRecord(ctx->ev0, st));
         k_items_chunked<<<a.grid, kBT, sm, st>>>(a);
         HCX_HIP(hipGetLastError());
     } else {
         HCX_HIP(hipFuncSetAttribute((const void *)k_spec_chunked, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
-        HCX_HIP(hipEventRecord(ctx->ev0, st));
+        if (!ctx->capturing) HCX_HIP(hipEventRecord(ctx->ev0, st));
         k_spec_chunked<<<n, kBT, sm, st>>>(a);
         HCX_HIP(hipGetLastError());
     }
-    HCX_HIP(hipEventRecord(ctx->ev1, st));
+    if (!ctx->capturing) HCX_HIP(hipEventRecord(ctx->ev1, st));
     k_finalize<<<(n + 255) / 256, 256, 0, st>>>(a);
     HCX_HIP(hipGetLastError());
     return HCX_SUCCESS;

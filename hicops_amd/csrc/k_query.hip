@@ -781,12 +781,12 @@ size_t query_smem_bytes(const QueryArgs &a, bool replay) { return smem_layout(a,
 int launch_cursor(hcx_ctx *ctx, const QueryArgs &a) {
     const size_t sm = query_smem_bytes(a, false);
     HCX_HIP(hipFuncSetAttribute((const void *)k_query<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
-    HCX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    if (!ctx->capturing) HCX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     if (a.nspec > 0) {
         k_query<false><<<a.nspec, kBT, sm, ctx->stream>>>(a);
         HCX_HIP(hipGetLastError());
     }
-    HCX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    if (!ctx->capturing) HCX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     return HCX_SUCCESS;
 }
 

@@ -168,7 +168,7 @@ def hcx() -> C.CDLL:
                      "hcx_selftest_divider", "hcx_set_tuning", "hcx_get_tuning", "hcx_set_option", "hcx_get_option", "hcx_autotune",
                      "hcx_surrogate_param_count", "hcx_surrogate_init", "hcx_surrogate_set",
                      "hcx_surrogate_get", "hcx_surrogate_train", "hcx_surrogate_gradient",
-                     "hcx_surrogate_last_audit", "hcx_selftest_mfma"]:
+                     "hcx_surrogate_last_audit", "hcx_selftest_mfma", "hcx_last_timing"]:
             getattr(lib, name).restype = C.c_int
         _hcx = lib
     return _hcx

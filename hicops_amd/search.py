@@ -201,6 +201,12 @@ class Engine:
                                                C.byref(loss)))
         return g, loss.value
 
+    def last_timing(self) -> dict:
+        """Device ms of the last query_device: scan kernel, e-value stage, whole search."""
+        q, e, t = C.c_float(), C.c_float(), C.c_float()
+        N.check(N.hcx().hcx_last_timing(self._h, C.byref(q), C.byref(e), C.byref(t)))
+        return {"scan_ms": q.value, "evalue_ms": e.value, "total_ms": t.value}
+
     def last_audit(self) -> dict:
         a = N.Audit()
         N.check(N.hcx().hcx_surrogate_last_audit(self._h, C.byref(a)))

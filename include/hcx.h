@@ -218,11 +218,18 @@ int  hcx_autotune(hcx_ctx *ctx, hcx_index *const *idx, int nidx, hcx_qbatch *qb,
 
 /* Device-resident variant for timing: runs the search and leaves results on
  * the device; hcx_search_fetch copies them out.  Returns the query kernel's
- * device time (HIP events on the library stream) in *kernel_ms. */
+ * device time (HIP events on the library stream) in *kernel_ms (with
+ * use_graph: the replayed graph's time). */
 int  hcx_search_device(hcx_ctx *ctx, hcx_index *const *idx, int nidx, hcx_qbatch *qb,
                        const hcx_search_opts *opts, float *kernel_ms, float *total_ms);
 int  hcx_search_fetch(hcx_ctx *ctx, hcx_qbatch *qb, hcx_result *results,
                       uint32_t *hist, hcx_stats *stats);
+/* Device times (HIP events) of the last hcx_search_device: the scan kernel,
+ * the e-value stage (the plain tail fit, or the surrogate's features +
+ * network + e-values, without its audit), and the whole search.  A search
+ * replayed from a hipGraph is timed as a whole: its scan and e-value times
+ * are the graph's.                                                          */
+int  hcx_last_timing(hcx_ctx *ctx, float *scan_ms, float *evalue_ms, float *total_ms);
 
 /* Candidate dump for parity tests: every candidate with shpk >= min_shp
  * and hs > 0, as (spectrum, idxoffset, psid, bc, yc, ibc, iyc, hs).
