@@ -344,7 +344,8 @@ extern "C" int hcx_queries_upload(hcx_ctx *c, const hcx_queries_view *q, hcx_qba
     const uint64_t npk = n > 0 ? q->idx[n] : 0;
     if (npk && (!q->moz || !q->intensity))
         return set_error_code(HCX_ERR_INVLD_PTR, "hcx_queries_upload: missing peaks");
-    uint32_t maxp = 0;
+    uint32_t maxp = 0, maxi = 0;
+    for (uint64_t i = 0; i < npk; i++) maxi = std::max(maxi, q->intensity[i]);
     for (int32_t s = 0; s < n; s++) {
         if (q->idx[s + 1] < q->idx[s]) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_queries_upload: idx not monotone");
         maxp = std::max(maxp, q->idx[s + 1] - q->idx[s]);
@@ -375,6 +376,7 @@ extern "C" int hcx_queries_upload(hcx_ctx *c, const hcx_queries_view *q, hcx_qba
     b->npeaks = npk;
     b->max_peaks = maxp;
     b->max_mult = mult;
+    b->max_int = maxi;
     if (b->moz.ensure(4 * npk + 16) || b->inten.ensure(4 * npk + 16) || b->idx.ensure(4ull * (n + 1)) ||
         b->prec.ensure(4ull * n + 16) || b->chg.ensure(4ull * n + 16) || b->rt.ensure(4ull * n + 16)) {
         delete b;
@@ -550,15 +552,25 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
     ca.slots = c->slots;
     ca.max_peaks = std::max<uint32_t>(1, qb->max_peaks);
     ca.group = std::min<uint32_t>(1024, ca.max_peaks * (2 * c->params.dF + 1));
-    // the hit list takes the LDS left in a workgroup's share of the CU once
-    // blocks_per_cu workgroups are resident (less a reserve for the kernels'
-    // static LDS); its records hold the 16-bit entry and a 16-bit peak index
+    // the hit list (16-bit entries) takes the LDS left in a workgroup's share
+    // of the CU once blocks_per_cu workgroups are resident
+    // 32-bit slot sums when no peptide's sum can reach 2^32: at most
+    // speclen * max_mult hits per peptide, each at most the batch's largest
+    // intensity
+    {
+        uint32_t maxspec = 0;
+        for (int i = 0; i < nidx; i++) maxspec = std::max(maxspec, idx[i]->speclen);
+        const uint64_t hits = (uint64_t)maxspec * std::max<uint32_t>(1, qb->max_mult);
+        ca.wide = (uint64_t)qb->max_int * hits >= (1ull << 32) ? 1u : 0u;
+    }
     ca.hl_cap = 0;
-    if (ca.max_peaks < 0xFFFF) {
+    {
         const size_t base = chunk_smem_bytes(ca);
         const size_t share = (size_t)c->max_lds / std::max<uint32_t>(1, c->blocks_per_cu);
-        const size_t reserve = 8192;
-        if (share > base + reserve + 4096) ca.hl_cap = (uint32_t)(((share - base - reserve) / 4) & ~(size_t)63);
+        // the replay kernel's extras: its slots' hyperscores and the static
+        // LDS heap (5 x 256 words), plus the kernels' other static LDS
+        const size_t reserve = 4ull * ca.slots + 5 * 256 * 4 + 1024;
+        if (share > base + reserve + 4096) ca.hl_cap = (uint32_t)(((share - base - reserve) / 2) & ~(size_t)63);
         if (c->hitlist >= 0) ca.hl_cap = std::min<uint32_t>(ca.hl_cap, (uint32_t)c->hitlist & ~63u);
     }
     ca.grid = (uint32_t)std::max(1, c->num_cu) * c->blocks_per_cu;

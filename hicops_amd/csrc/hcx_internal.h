@@ -104,6 +104,7 @@ struct hcx_qbatch {
     uint64_t npeaks = 0;
     uint32_t max_peaks = 0;
     uint32_t max_mult = 0;   // most peaks of one spectrum within +-dF of a single bin
+    uint32_t max_int = 0;    // largest peak intensity of the batch
     hcx::DeviceBuf moz, inten, idx, prec, chg, rt;
     hcx::DeviceBuf res, hist, stats, feat, pred;
     hcx::DeviceBuf acc, items, counts;   // chunked-search state
@@ -177,6 +178,8 @@ struct ChunkArgs {
     uint32_t *queue;
     uint32_t nbk;         // (length, chunk) buckets: sum of nch
     uint32_t by_chunk;    // items bucketed by (length, chunk) instead of spectrum
+    uint32_t wide;        // 64-bit intensity sums in the slots
+    uint32_t replay;      // layout of the in-order replay (slots keep their hyperscores)
 };
 size_t chunk_smem_bytes(const ChunkArgs &a);
 int launch_chunk_offsets(hcx_ctx *ctx, const uint32_t *iA, const uint32_t *bA, uint32_t NB, uint32_t nch, uint32_t TC,

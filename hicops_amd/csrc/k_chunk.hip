@@ -47,6 +47,7 @@ namespace {
 #define AS_LDS __attribute__((address_space(3)))
 #define AS_GLB __attribute__((address_space(1)))
 typedef AS_LDS uint32_t lu32;
+typedef AS_LDS uint16_t lu16;
 typedef AS_LDS int32_t li32;
 typedef AS_LDS unsigned long long lu64;
 typedef AS_LDS double lf64;
@@ -61,7 +62,6 @@ constexpr int kBT = HCX_CHUNK_THREADS;   // threads per workgroup of the chunked
 // at most 128 VGPRs: four waves per SIMD (two 512-thread workgroups per CU)
 constexpr int kWavesPerSimd = 4;
 constexpr uint32_t kNoProbe = 0xFFFFFFFFu;
-constexpr uint32_t kNoHit = 0xFFFFFFFFu;      // hit-list entry outside the window
 constexpr uint32_t kGrab = 4;        // items taken per queue atomic
 constexpr unsigned long long kLow25 = (1ull << 25) - 1;
 
@@ -179,7 +179,7 @@ struct Layout {
     uint32_t slot_cnt;  // slots: bc | yc << 16
     uint32_t slot_loc;  // slots: local peptide index
     uint32_t slot_hs;   // slots: hyperscore bits (replay)
-    uint32_t slot_sum;  // 2*slots u64: ibc, iyc
+    uint32_t slot_sum;  // 2*slots u64 (wide) or u32: ibc, iyc
     uint32_t pk_moz, pk_int;   // max_peaks
     uint32_t crs;       // group: iA start of each non-empty run
     uint32_t cofs;      // group + 1: flat offset of each non-empty run (exclusive prefix)
@@ -188,7 +188,7 @@ struct Layout {
     uint32_t hist;      // hbins
     uint32_t lgf;       // max_shdpeaks + 1 doubles
     uint32_t wlo, whi;  // nlen
-    uint32_t hl;        // hl_cap: hit list of the current item
+    uint32_t hl;        // hl_cap u16: hit list of the current item
     uint32_t total;
 };
 
@@ -207,8 +207,8 @@ __host__ __device__ inline Layout make_layout(const ChunkArgs &a) {
     l.pre = take(a.tc / 8);
     l.slot_cnt = take(4u * a.slots);
     l.slot_loc = take(4u * a.slots);
-    l.slot_hs = take(4u * a.slots);
-    l.slot_sum = take(16u * a.slots);
+    l.slot_hs = take(a.replay ? 4u * a.slots : 0u);
+    l.slot_sum = take((a.wide ? 16u : 8u) * a.slots);
     l.pk_moz = take(4u * a.max_peaks);
     l.pk_int = take(4u * a.max_peaks);
     l.crs = take(4u * a.group);
@@ -219,14 +219,16 @@ __host__ __device__ inline Layout make_layout(const ChunkArgs &a) {
     l.lgf = take(8u * (a.q.max_shdpeaks + 1));
     l.wlo = take(4u * a.q.nlen);
     l.whi = take(4u * a.q.nlen);
-    l.hl = take(4u * a.hl_cap);
+    l.hl = take(2u * a.hl_cap);
     l.total = off;
     return l;
 }
 
 struct Smem {
-    lu32 *map, *bmp, *pre, *slot_cnt, *slot_loc, *slot_hs, *pk_moz, *pk_int, *crs, *cofs, *ck, *wsum, *hist, *hl;
-    lu64 *slot_sum;
+    lu32 *map, *bmp, *pre, *slot_cnt, *slot_loc, *slot_hs, *pk_moz, *pk_int, *crs, *cofs, *ck, *wsum, *hist;
+    lu16 *hl;
+    lu64 *slot_sum64;
+    lu32 *slot_sum32;
     lf64 *lgf;
     li32 *wlo, *whi;
 };
@@ -240,7 +242,8 @@ __device__ __forceinline__ Smem bind(unsigned char *raw, const Layout &l) {
     s.slot_cnt = (lu32 *)(b + l.slot_cnt);
     s.slot_loc = (lu32 *)(b + l.slot_loc);
     s.slot_hs = (lu32 *)(b + l.slot_hs);
-    s.slot_sum = (lu64 *)(b + l.slot_sum);
+    s.slot_sum64 = (lu64 *)(b + l.slot_sum);
+    s.slot_sum32 = (lu32 *)(b + l.slot_sum);
     s.pk_moz = (lu32 *)(b + l.pk_moz);
     s.pk_int = (lu32 *)(b + l.pk_int);
     s.crs = (lu32 *)(b + l.crs);
@@ -251,7 +254,7 @@ __device__ __forceinline__ Smem bind(unsigned char *raw, const Layout &l) {
     s.lgf = (lf64 *)(b + l.lgf);
     s.wlo = (li32 *)(b + l.wlo);
     s.whi = (li32 *)(b + l.whi);
-    s.hl = (lu32 *)(b + l.hl);
+    s.hl = (lu16 *)(b + l.hl);
     return s;
 }
 
@@ -347,19 +350,23 @@ __device__ __forceinline__ uint2 setup_runs(const ChunkArgs &a, const Smem &sm, 
     return make_uint2(ufl(tot.y), ufl(tot.x));
 }
 
-// Walk over the flat entries [fb, E) of a group's (compacted, non-empty)
-// runs in the chunk-local entry array: thread t takes an equal contiguous
-// share and reads it kStep entries at a time.  A step takes entries from at
-// most two consecutive runs, whose bounds and peaks are held in registers;
-// its loads are unconditional (lanes past the step's end reload its first
-// entry), so they issue back to back.
+// Walk over the flat entries [fb, fe) of a group's (compacted, non-empty)
+// runs: thread t takes an equal contiguous share and reads it kStep entries
+// at a time.  A step takes entries from at most two consecutive runs, whose
+// bounds and peaks are held in registers; its loads are unconditional (lanes
+// past the step's end reload its first entry), so they issue back to back,
+// and the next step's loads are issued before the current step is processed.
+// load(gi, f): the entry of flat index f, whose index in the entry array is gi.
 // body(v[kStep], kA, kB, nA, cnt, f): entries i < nA belong to peak kA,
 // entries nA <= i < cnt to peak kB; entry i has flat index f + i.
 constexpr uint32_t kStep = 8;
-template <typename Body>
-__device__ __forceinline__ void walk_steps(const Smem &sm, const uint16_t *src, uint32_t nc, uint32_t fb, uint32_t E,
+struct StepInfo {
+    uint32_t fs, nA, cnt, ka, kb;
+};
+template <typename Load, typename Body>
+__device__ __forceinline__ void walk_steps(const Smem &sm, uint32_t nc, uint32_t fb, uint32_t fe, Load &&load,
                                            Body &&body) {
-    const uint32_t n = E - fb;
+    const uint32_t n = fe - fb;
     uint32_t f = fb + (uint32_t)(((uint64_t)n * threadIdx.x) / kBT);
     const uint32_t f1 = fb + (uint32_t)(((uint64_t)n * (threadIdx.x + 1)) / kBT);
     if (f >= f1) return;
@@ -379,18 +386,16 @@ __device__ __forceinline__ void walk_steps(const Smem &sm, const uint16_t *src, 
         abase2 = sm.crs[r + 1] - pend;
         kB = sm.ck[r + 1];
     }
-    const AS_GLB uint16_t *e = (const AS_GLB uint16_t *)src;
-    while (f < f1) {
+    auto issue = [&](uint32_t (&v)[kStep], StepInfo &st) {
         const uint32_t m = min(kStep, f1 - f);
         const uint32_t nA = min(m, pend - f);
         const uint32_t cnt = nA + min(m - nA, pend2 - pend);
-        uint32_t v[kStep];
 #pragma unroll
         for (uint32_t i = 0; i < kStep; i++) {
             const uint32_t j = i < cnt ? i : 0u;
-            v[i] = e[(j < nA ? abase : abase2) + f + j];
+            v[i] = load((j < nA ? abase : abase2) + f + j, f + j);
         }
-        const uint32_t fs = f, ka = kA, kb = kB;
+        st = StepInfo{f, nA, cnt, kA, kB};
         f += cnt;
         if (f < f1 && f >= pend) {
             // runs are non-empty: at most one more run was consumed whole
@@ -411,7 +416,18 @@ __device__ __forceinline__ void walk_steps(const Smem &sm, const uint16_t *src, 
                 kB = sm.ck[r + 1];
             }
         }
-        body(v, ka, kb, nA, cnt, fs);
+    };
+    uint32_t va[kStep], vb[kStep];
+    StepInfo sa, sb;
+    issue(va, sa);
+    for (;;) {
+        const bool more = f < f1;
+        if (more) issue(vb, sb);
+        body(va, sa.ka, sa.kb, sa.nA, sa.cnt, sa.fs);
+        if (!more) break;
+#pragma unroll
+        for (uint32_t i = 0; i < kStep; i++) va[i] = vb[i];
+        sa = sb;
     }
 }
 
@@ -500,7 +516,9 @@ __device__ __forceinline__ Cell heap_getmax(const Heap &h, int size) {
 // Process one (spectrum, length, chunk) item; peaks are already in LDS.
 // ORDERED: lane 0 inserts every scored peptide into the heap in peptide order
 // (slots are ranks in the qualification bitmap, so slot order is peptide order).
-template <bool ORDERED>
+// WIDE: 64-bit intensity sums (otherwise 32-bit, chosen by the host when the
+// batch's largest intensity times the most hits a peptide can take fits).
+template <bool ORDERED, bool WIDE>
 __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm, lu32 *s_ctl, uint32_t P, int ixx,
                                              uint32_t c, int32_t lo, int32_t hi, Acc &acc, const Heap *heap = nullptr,
                                              int *hsize = nullptr) {
@@ -515,6 +533,8 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
     const uint32_t NP = P * W;
     const uint32_t GP = a.group;
     const uint32_t ng = (NP + GP - 1) / GP;
+    const AS_GLB uint16_t *iAc = (const AS_GLB uint16_t *)L.iAc;
+    auto load_g = [&](uint32_t gi, uint32_t) { return (uint32_t)iAc[gi]; };
 
     // P1: 8-bit hit counters.  A counter that reaches 0x80 is pulled back to
     // 0x40: only "hits >= min_shp" (min_shp <= kMaxChunkShp) matters here, and
@@ -522,11 +542,9 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
     // increments of the same peptide land between one lane's add and its
     // pull-back; a byte seen at >= 0xC0 (halfway there) flags the spectrum.
     // Entries are chunk-local: local << 1 | isY.  The first hl_cap flat
-    // entries of the item (in group order) are also recorded in the LDS hit
-    // list as entry | peak << 16, or kNoHit outside the window, so that P2
-    // need not read them again.
+    // entries of the item (in group order) are also copied to the LDS hit
+    // list, so that P2 need not read them again.
     uint32_t E = 0, nc = 0, hl_n = 0;
-    uint32_t g_part = ng, l_part = 0;   // first group not wholly listed, its listed prefix
     const uint32_t lane = threadIdx.x & 63u;
     for (uint32_t g = 0; g < ng; g++) {
         const uint32_t g0 = g * GP, g1 = min(NP, g0 + GP);
@@ -534,9 +552,9 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
         E = en.x;
         nc = en.y;
         const uint32_t lim = min(E, a.hl_cap - hl_n);
-        lu32 *hl = sm.hl + hl_n;
-        walk_steps(sm, L.iAc, nc, 0, E,
-                   [&](const uint32_t (&v)[kStep], uint32_t ka, uint32_t kb, uint32_t nA, uint32_t cnt, uint32_t fs) {
+        lu16 *hl = sm.hl + hl_n;
+        walk_steps(sm, nc, 0, E, load_g,
+                   [&](const uint32_t (&v)[kStep], uint32_t, uint32_t, uint32_t, uint32_t cnt, uint32_t fs) {
             uint32_t old[kStep], win = 0;
 #pragma unroll
             for (uint32_t i = 0; i < kStep; i++)
@@ -550,21 +568,17 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
             }
 #pragma unroll
             for (uint32_t i = 0; i < kStep; i++) {
-                const uint32_t local = v[i] >> 1;
                 if ((win >> i) & 1u) {
+                    const uint32_t local = v[i] >> 1;
                     const uint32_t sh = (local & 3u) << 3;
                     const uint32_t ob = (old[i] >> sh) & 0xFFu;
                     if (ob == 0x7Fu) lds_add(&sm.map[local >> 2], 0u - (0x40u << sh));
                     else if (ob >= 0xC0u) acc.flags |= HCX_RES_COUNT_OVF;
                 }
-                if (i < cnt && fs + i < lim) hl[fs + i] = ((win >> i) & 1u) ? v[i] | ((i < nA ? ka : kb) << 16) : kNoHit;
+                if (i < cnt && fs + i < lim) hl[fs + i] = (uint16_t)v[i];
             }
             acc.nia += __popc(win);
         });
-        if (lim < E && g_part == ng) {
-            g_part = g;
-            l_part = lim;
-        }
         hl_n += lim;
         __syncthreads();
     }
@@ -606,41 +620,53 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
         const uint32_t ns = min(a.slots, Qn - r0);
         for (uint32_t sl = threadIdx.x; sl < ns; sl += kBT) {
             sm.slot_cnt[sl] = 0;
-            sm.slot_sum[2 * sl] = 0;
-            sm.slot_sum[2 * sl + 1] = 0;
+            if (WIDE) {
+                sm.slot_sum64[2 * sl] = 0;
+                sm.slot_sum64[2 * sl + 1] = 0;
+            } else {
+                sm.slot_sum32[2 * sl] = 0;
+                sm.slot_sum32[2 * sl + 1] = 0;
+            }
         }
         __syncthreads();
         // P2: exact b/y counts and intensity sums of slots [r0, r0 + ns),
-        // first from the hit list, then from the entries not listed
-        auto tally = [&](uint32_t local, uint32_t isY, uint32_t k) {
-            const uint32_t w = local >> 5, bit = local & 31u;
-            const uint32_t bits = sm.bmp[w];
-            if ((bits >> bit) & 1u) {
-                const uint32_t sl = sm.pre[w] + __popc(bits & ((1u << bit) - 1u)) - r0;
-                if (sl < ns) {
-                    lds_add(&sm.slot_cnt[sl], isY ? 0x10000u : 1u);
-                    lds_add64(&sm.slot_sum[2 * sl + isY], (unsigned long long)sm.pk_int[k]);
-                    sm.slot_loc[sl] = local;
+        // group by group: the listed entries from the hit list, the rest
+        // from the entry array again
+        auto tally = [&](const uint32_t (&v)[kStep], uint32_t ka, uint32_t kb, uint32_t nA, uint32_t cnt,
+                         uint32_t) {
+#pragma unroll
+            for (uint32_t i = 0; i < kStep; i++) {
+                const uint32_t local = v[i] >> 1;
+                if (i < cnt && local - wlo <= wspan) {
+                    const uint32_t w = local >> 5, bit = local & 31u;
+                    const uint32_t bits = sm.bmp[w];
+                    if ((bits >> bit) & 1u) {
+                        const uint32_t sl = sm.pre[w] + __popc(bits & ((1u << bit) - 1u)) - r0;
+                        if (sl < ns) {
+                            const uint32_t isY = v[i] & 1u;
+                            const uint32_t it = sm.pk_int[i < nA ? ka : kb];
+                            lds_add(&sm.slot_cnt[sl], isY ? 0x10000u : 1u);
+                            if (WIDE) lds_add64(&sm.slot_sum64[2 * sl + isY], (unsigned long long)it);
+                            else lds_add(&sm.slot_sum32[2 * sl + isY], it);
+                            sm.slot_loc[sl] = local;
+                        }
+                    }
                 }
             }
         };
-        for (uint32_t i = threadIdx.x; i < hl_n; i += kBT) {
-            const uint32_t rec = sm.hl[i];
-            if (rec != kNoHit) tally((rec & 0xFFFFu) >> 1, rec & 1u, rec >> 16);
-        }
-        for (uint32_t g = g_part; g < ng; g++) {
+        uint32_t hb = 0;
+        for (uint32_t g = 0; g < ng; g++) {
             const uint32_t g0 = g * GP, g1 = min(NP, g0 + GP);
             if (ng > 1) {
                 const uint2 en = setup_runs(a, sm, L, c, g0, g1, false, acc);
                 E = en.x;
                 nc = en.y;
             }
-            walk_steps(sm, L.iAc, nc, g == g_part ? l_part : 0u, E,
-                       [&](const uint32_t (&v)[kStep], uint32_t ka, uint32_t kb, uint32_t nA, uint32_t cnt, uint32_t) {
-#pragma unroll
-                for (uint32_t i = 0; i < kStep; i++)
-                    if (i < cnt && (v[i] >> 1) - wlo <= wspan) tally(v[i] >> 1, v[i] & 1u, i < nA ? ka : kb);
-            });
+            const uint32_t lim = min(E, a.hl_cap - hb);
+            const lu16 *hl = sm.hl + hb;
+            if (lim) walk_steps(sm, nc, 0, lim, [&](uint32_t, uint32_t f) { return (uint32_t)hl[f]; }, tally);
+            if (lim < E) walk_steps(sm, nc, lim, E, load_g, tally);
+            hb += lim;
             if (ng > 1) __syncthreads();
         }
         __syncthreads();
@@ -649,7 +675,8 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
             const uint32_t loc = sm.slot_loc[sl];
             const uint32_t cw = sm.slot_cnt[sl];
             const uint32_t bc = cw & 0xFFFFu, yc = cw >> 16;
-            const unsigned long long ib = sm.slot_sum[2 * sl], iy = sm.slot_sum[2 * sl + 1];
+            const unsigned long long ib = WIDE ? sm.slot_sum64[2 * sl] : sm.slot_sum32[2 * sl];
+            const unsigned long long iy = WIDE ? sm.slot_sum64[2 * sl + 1] : sm.slot_sum32[2 * sl + 1];
             if (ORDERED) sm.slot_hs[sl] = 0u;   // +0.0f: not inserted
             if ((uint16_t)(bc + yc) < a.q.min_shp) continue;
             uint32_t bi = bc, yi = yc;
@@ -772,6 +799,7 @@ __device__ __forceinline__ void init_smem(const ChunkArgs &a, const Smem &sm) {
 }
 
 // Plain form: one workgroup per spectrum, its items in order.
+template <bool WIDE>
 __global__ __launch_bounds__(kBT, kWavesPerSimd) void k_spec_chunked(ChunkArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ uint32_t s_ctl_[2];
@@ -802,12 +830,13 @@ __global__ __launch_bounds__(kBT, kWavesPerSimd) void k_spec_chunked(ChunkArgs a
         const int32_t lo = ufl((int32_t)sm.wlo[ixx]), hi = ufl((int32_t)sm.whi[ixx]);
         if (lo > hi) continue;
         for (uint32_t c = (uint32_t)lo / a.tc; c <= (uint32_t)hi / a.tc; c++)
-            process_item<false>(a, sm, s_ctl, P, ixx, c, lo, hi, acc);
+            process_item<false, WIDE>(a, sm, s_ctl, P, ixx, c, lo, hi, acc);
     }
     flush_spectrum(a, sm, s, acc, s_red, s_red64);
 }
 
 // Batched form: persistent workgroups pull (spectrum, length, chunk) items.
+template <bool WIDE>
 __global__ __launch_bounds__(kBT, kWavesPerSimd) void k_items_chunked(ChunkArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ uint32_t s_ctl_[2];
@@ -846,7 +875,7 @@ __global__ __launch_bounds__(kBT, kWavesPerSimd) void k_items_chunked(ChunkArgs 
                 P = load_peaks(a, sm, cur);
                 __syncthreads();
             }
-            process_item<false>(a, sm, s_ctl, P, item.ixx, item.chunk, item.lo, item.hi, acc);
+            process_item<false, WIDE>(a, sm, s_ctl, P, item.ixx, item.chunk, item.lo, item.hi, acc);
         }
     }
     if (cur != 0xFFFFFFFFu) flush_spectrum(a, sm, cur, acc, s_red, s_red64);
@@ -856,6 +885,7 @@ __global__ __launch_bounds__(kBT, kWavesPerSimd) void k_items_chunked(ChunkArgs 
 // One workgroup per listed spectrum runs its items in the reference order
 // (length, then peptide id) and inserts every scored peptide into the heap,
 // whose getMax() then picks the top hit exactly as the reference does.
+template <bool WIDE>
 __global__ __launch_bounds__(kBT) void k_replay_chunked(ChunkArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ uint32_t s_ctl_[2];
@@ -890,7 +920,7 @@ __global__ __launch_bounds__(kBT) void k_replay_chunked(ChunkArgs a) {
             const int32_t lo = ufl((int32_t)sm.wlo[ixx]), hi = ufl((int32_t)sm.whi[ixx]);
             if (lo > hi) continue;
             for (uint32_t c = (uint32_t)lo / a.tc; c <= (uint32_t)hi / a.tc; c++)
-                process_item<true>(a, sm, s_ctl, P, ixx, c, lo, hi, acc, &heap, &hsize);
+                process_item<true, WIDE>(a, sm, s_ctl, P, ixx, c, lo, hi, acc, &heap, &hsize);
         }
         if (threadIdx.x == 0 && hsize > 0) {
             const Cell m = heap_getmax(heap, hsize);
@@ -1084,20 +1114,22 @@ int launch_items(hcx_ctx *ctx, ChunkArgs &a, uint32_t *counts, uint32_t *scan_tm
     return HCX_SUCCESS;
 }
 
-int launch_chunked(hcx_ctx *ctx, ChunkArgs &a, int kernel) {
+template <bool WIDE>
+static int launch_chunked_t(hcx_ctx *ctx, ChunkArgs &a, int kernel) {
     hipStream_t st = ctx->stream;
     const size_t sm = chunk_smem_bytes(a);
     const int n = a.q.nspec;
-    if (n <= 0) return HCX_SUCCESS;
     if (kernel == HCX_KERNEL_BATCHED) {
-        HCX_HIP(hipFuncSetAttribute((const void *)k_items_chunked, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+        HCX_HIP(hipFuncSetAttribute((const void *)k_items_chunked<WIDE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)sm));
         if (!ctx->capturing) HCX_HIP(hipEventRecord(ctx->ev0, st));
-        k_items_chunked<<<a.grid, kBT, sm, st>>>(a);
+        k_items_chunked<WIDE><<<a.grid, kBT, sm, st>>>(a);
         HCX_HIP(hipGetLastError());
     } else {
-        HCX_HIP(hipFuncSetAttribute((const void *)k_spec_chunked, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+        HCX_HIP(hipFuncSetAttribute((const void *)k_spec_chunked<WIDE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)sm));
         if (!ctx->capturing) HCX_HIP(hipEventRecord(ctx->ev0, st));
-        k_spec_chunked<<<n, kBT, sm, st>>>(a);
+        k_spec_chunked<WIDE><<<n, kBT, sm, st>>>(a);
         HCX_HIP(hipGetLastError());
     }
     if (!ctx->capturing) HCX_HIP(hipEventRecord(ctx->ev1, st));
@@ -1106,14 +1138,27 @@ int launch_chunked(hcx_ctx *ctx, ChunkArgs &a, int kernel) {
     return HCX_SUCCESS;
 }
 
-int launch_replay_chunked(hcx_ctx *ctx, ChunkArgs &a) {
-    const size_t sm = chunk_smem_bytes(a);
+int launch_chunked(hcx_ctx *ctx, ChunkArgs &a, int kernel) {
     if (a.q.nspec <= 0) return HCX_SUCCESS;
-    HCX_HIP(hipFuncSetAttribute((const void *)k_replay_chunked, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+    return a.wide ? launch_chunked_t<true>(ctx, a, kernel) : launch_chunked_t<false>(ctx, a, kernel);
+}
+
+template <bool WIDE>
+static int launch_replay_t(hcx_ctx *ctx, const ChunkArgs &a) {
+    const size_t sm = chunk_smem_bytes(a);
+    HCX_HIP(hipFuncSetAttribute((const void *)k_replay_chunked<WIDE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)sm));
     const int grid = std::max(1, std::min(a.q.nspec, ctx->num_cu * (int)ctx->blocks_per_cu));
-    k_replay_chunked<<<grid, kBT, sm, ctx->stream>>>(a);
+    k_replay_chunked<WIDE><<<grid, kBT, sm, ctx->stream>>>(a);
     HCX_HIP(hipGetLastError());
     return HCX_SUCCESS;
+}
+
+int launch_replay_chunked(hcx_ctx *ctx, ChunkArgs &a) {
+    if (a.q.nspec <= 0) return HCX_SUCCESS;
+    ChunkArgs r = a;
+    r.replay = 1;   // the replay layout adds the slots' hyperscores
+    return r.wide ? launch_replay_t<true>(ctx, r) : launch_replay_t<false>(ctx, r);
 }
 
 }  // namespace hcx
