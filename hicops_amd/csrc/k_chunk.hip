@@ -14,7 +14,7 @@
 //   R   every probe reads its run bounds [bA2[c], bA2[c+1]) and a block scan
 //       of the run lengths gives each thread an equal contiguous share of the
 //       group's entries;
-//   P1  8 independent iAc loads per lane at a time; each entry inside the
+//   P1  kStep independent iAc loads per lane at a time; each entry inside the
 //       precursor window adds 1 to its peptide's 8-bit hit counter (four
 //       counters per LDS word, saturating so that no byte carries);
 // then, once per item:
@@ -354,12 +354,11 @@ __device__ __forceinline__ uint2 setup_runs(const ChunkArgs &a, const Smem &sm, 
 // runs: thread t takes an equal contiguous share and reads it kStep entries
 // at a time.  A step takes entries from at most two consecutive runs, whose
 // bounds and peaks are held in registers; its loads are unconditional (lanes
-// past the step's end reload its first entry), so they issue back to back,
-// and the next step's loads are issued before the current step is processed.
+// past the step's end reload its first entry), so they issue back to back.
 // load(gi, f): the entry of flat index f, whose index in the entry array is gi.
 // body(v[kStep], kA, kB, nA, cnt, f): entries i < nA belong to peak kA,
 // entries nA <= i < cnt to peak kB; entry i has flat index f + i.
-constexpr uint32_t kStep = 8;
+constexpr uint32_t kStep = 4;   // measured best of 2-16 (runs average ~17 entries)
 struct StepInfo {
     uint32_t fs, nA, cnt, ka, kb;
 };
@@ -417,17 +416,11 @@ __device__ __forceinline__ void walk_steps(const Smem &sm, uint32_t nc, uint32_t
             }
         }
     };
-    uint32_t va[kStep], vb[kStep];
-    StepInfo sa, sb;
-    issue(va, sa);
-    for (;;) {
-        const bool more = f < f1;
-        if (more) issue(vb, sb);
+    uint32_t va[kStep];
+    StepInfo sa;
+    while (f < f1) {
+        issue(va, sa);
         body(va, sa.ka, sa.kb, sa.nA, sa.cnt, sa.fs);
-        if (!more) break;
-#pragma unroll
-        for (uint32_t i = 0; i < kStep; i++) va[i] = vb[i];
-        sa = sb;
     }
 }
 
