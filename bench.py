@@ -158,6 +158,7 @@ def main() -> int:
     achieved = b_q / (head["kms"] / 1e3)
     res = head["out"].results
     valid = int(((res["flags"] & N.RES_VALID) != 0).sum())
+    tied = int(((res["flags"] & N.RES_TIE_REPLAYED) != 0).sum())
 
     traffic = None
     tname = "plain" if forms[-1] == "plain" else "batched"   # both batched forms run the same scan
@@ -212,6 +213,7 @@ def main() -> int:
                         "ions_per_s": round(nions / (build_ms / 1e3), 1),
                         "frac_hbm": round(b_build / (build_ms / 1e3) / HBM_PEAK, 4)},
         "psms_valid": valid,
+        "tied_spectra": tied,
         "work": {"n_ia": n_ia, "n_probe": n_probe, "peaks": npeaks},
     }
     print(json.dumps(line), flush=True)

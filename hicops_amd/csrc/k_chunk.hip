@@ -700,13 +700,31 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
         }
         __syncthreads();
         if (ORDERED) {
-            if (threadIdx.x == 0) {
-                for (uint32_t sl = 0; sl < ns; sl++) {
-                    const float hs = __uint_as_float(sm.slot_hs[sl]);
-                    if (!(hs > 0.0f)) continue;
-                    const uint32_t cw = sm.slot_cnt[sl];
-                    const Cell e = {hs, (int32_t)(base + sm.slot_loc[sl]), (uint32_t)ixx, cw & 0xFFFFu, cw >> 16};
-                    heap_insert(*heap, hsize, a.q.heap_cap, e);
+            // wave 0 feeds the round's scored slots to the heap in slot order;
+            // once the heap is full, a slot below the root would be rejected
+            // (increase_key, minheap.h:218-225), and the root only grows, so a
+            // ballot skips every such slot up to the next one that is taken
+            if (threadIdx.x < 64) {
+                const int cap = a.q.heap_cap;
+                for (uint32_t b0 = 0; b0 < ns; b0 += 64) {
+                    const uint32_t sl = b0 + threadIdx.x;
+                    const float hs = sl < ns ? __uint_as_float(sm.slot_hs[sl]) : 0.0f;
+                    unsigned long long mask = __ballot(hs > 0.0f);
+                    while (mask) {
+                        const int hsz = __shfl(*hsize, 0, 64);
+                        const float root = hsz == cap ? heap_hs(*heap, 0) : -1.0f;
+                        const unsigned long long take = mask & __ballot(hs >= root);
+                        if (!take) break;
+                        const int first = __ffsll((long long)take) - 1;
+                        if (threadIdx.x == 0) {
+                            const uint32_t s2 = b0 + (uint32_t)first;
+                            const uint32_t cw = sm.slot_cnt[s2];
+                            const Cell e = {__uint_as_float(sm.slot_hs[s2]), (int32_t)(base + sm.slot_loc[s2]),
+                                            (uint32_t)ixx, cw & 0xFFFFu, cw >> 16};
+                            heap_insert(*heap, hsize, cap, e);
+                        }
+                        mask &= ~((2ull << first) - 1ull);   // slots up to `first` are done
+                    }
                 }
             }
             __syncthreads();
