@@ -438,6 +438,9 @@ int prepare(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, const h
         if (ix->chunk != c->chunk)
             return set_error_code(HCX_ERR_INVLD_PARAM, "search: index built with another chunk size");
         if (ix->n >= (1u << 25)) return set_error_code(HCX_ERR_INVLD_SIZE, "search: more than 2^25 entries in a length");
+        // the scan reads iAc through 32-bit byte offsets
+        if ((uint64_t)ix->n * ix->speclen * 2 > 0xFFFFFFFFull)
+            return set_error_code(HCX_ERR_INVLD_SIZE, "search: a length's ion array exceeds 4 GiB");
     }
     if (c->lens.ensure(sizeof(LenDesc) * lens.size())) return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "search: alloc");
     HCX_HIP(hipMemcpyAsync(c->lens.p, lens.data(), sizeof(LenDesc) * lens.size(), hipMemcpyHostToDevice, c->stream));
@@ -567,9 +570,9 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
     {
         const size_t base = chunk_smem_bytes(ca);
         const size_t share = (size_t)c->max_lds / std::max<uint32_t>(1, c->blocks_per_cu);
-        // the replay kernel's extras: its slots' hyperscores and the static
-        // LDS heap (5 x 256 words), plus the kernels' other static LDS
-        const size_t reserve = 4ull * ca.slots + 5 * 256 * 4 + 1024;
+        // the kernels' static LDS (control words) is under 256 bytes; the
+        // replay kernel sizes its own hit list
+        const size_t reserve = 256;
         if (share > base + reserve + 4096) ca.hl_cap = (uint32_t)(((share - base - reserve) / 2) & ~(size_t)63);
         if (c->hitlist >= 0) ca.hl_cap = std::min<uint32_t>(ca.hl_cap, (uint32_t)c->hitlist & ~63u);
     }

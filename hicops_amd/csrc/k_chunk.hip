@@ -11,19 +11,25 @@
 //
 // Work item = (spectrum, length, chunk).  The item's (peak, bin) probes are
 // taken in groups of at most `group` probes; for one group:
-//   R   every probe reads its run bounds [bA2[c], bA2[c+1]) and a block scan
-//       of the run lengths gives each thread an equal contiguous share of the
-//       group's entries;
-//   P1  kStep independent iAc loads per lane at a time; each entry inside the
-//       precursor window adds 1 to its peptide's 8-bit hit counter (four
-//       counters per LDS word, saturating so that no byte carries);
+//   R   every probe reads its run bounds [bA2[c], bA2[c+1]); the non-empty
+//       runs are compacted by a block scan into one flat sequence of the
+//       group's entries, and a bitmap marks where each run starts (one
+//       64-bit word per 64-entry tile);
+//   P1  each wavefront takes a contiguous share of the tiles; lane j of a
+//       tile reads flat entry F + j, whose run is the tile's first run plus
+//       the run starts at or below bit j (mbcnt), so consecutive lanes read
+//       consecutive iAc entries.  Each entry inside the precursor window adds
+//       1 to its peptide's 8-bit hit counter (four counters per LDS word,
+//       saturating so that no byte carries); the entry that lifts a counter
+//       to min_shp sets the peptide's bit in the qualification bitmap;
 // then, once per item:
-//   Q   a bitmap of the peptides with hits >= min_shp and, per 32-peptide
-//       bitmap word, the number of qualifying peptides before it: a
-//       qualifying peptide's slot is its rank, so slots are in peptide order;
-//   P2  the entries are read again (L2-resident; a one-group item keeps its
-//       run table) and peptides whose rank falls in the current round of
-//       `slots` ranks accumulate exact b/y counts and intensity sums;
+//   Q   per 32-peptide bitmap word, the number of qualifying peptides before
+//       it: a qualifying peptide's slot is its rank, so slots are in peptide
+//       order;
+//   P2  the entries are read again (from the LDS hit list, past it from iAc;
+//       a one-group item keeps its run table) and peptides whose rank falls
+//       in the current round of `slots` ranks accumulate exact b/y counts and
+//       intensity sums;
 //   S   slots are scored: hs = lgf[bc] + lgf[yc] + log10(1+ibc) + log10(1+iyc) - 6
 //       (dslim_query.cpp:679-722), histogram + top hit of the spectrum.
 // Per-spectrum results from several items merge through global atomics:
@@ -41,6 +47,8 @@
 // wait counters and would serialise the LDS atomics behind HBM reads).
 #include "hcx_internal.h"
 
+#include <type_traits>
+
 namespace hcx {
 namespace {
 
@@ -51,6 +59,8 @@ typedef AS_LDS uint16_t lu16;
 typedef AS_LDS int32_t li32;
 typedef AS_LDS unsigned long long lu64;
 typedef AS_LDS double lf64;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef AS_LDS u32x4 lu32x4;
 typedef const AS_GLB uint32_t gcu32;
 typedef AS_GLB uint32_t gu32;
 typedef AS_GLB unsigned long long gu64;
@@ -64,8 +74,20 @@ constexpr int kWavesPerSimd = 4;
 constexpr uint32_t kNoProbe = 0xFFFFFFFFu;
 constexpr uint32_t kGrab = 4;        // items taken per queue atomic
 constexpr unsigned long long kLow25 = (1ull << 25) - 1;
+// A group's flat entries are walked in segments of kSegEntries; the run
+// starts of the current segment are a bitmap of kSegWords 64-bit words, one
+// per 64-entry tile.
+constexpr uint32_t kSegWords = 256;
+constexpr uint32_t kSegEntries = 64u * kSegWords;
+constexpr uint32_t kTileU = 4;       // tiles per wavefront step (loads in flight)
 
 __device__ __forceinline__ uint32_t ldg(const uint32_t *p, size_t i) { return ((gcu32 *)p)[i]; }
+// raw buffer resource word 3 on gfx9-family parts (gfx950 included): byte
+// offsets, range-checked against num_records
+constexpr int kBufDword3 = 0x00020000;
+__device__ __forceinline__ uint32_t iac_load(__amdgpu_buffer_rsrc_t r, uint32_t entry) {
+    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, (int)(entry << 1), 0, 0);
+}
 __device__ __forceinline__ void lds_add(lu32 *p, uint32_t v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
@@ -177,13 +199,14 @@ struct Layout {
     uint32_t bmp;       // tc/32 words: qualification bitmap
     uint32_t pre;       // tc/32 words: qualifying peptides before each bitmap word
     uint32_t slot_cnt;  // slots: bc | yc << 16
-    uint32_t slot_loc;  // slots: local peptide index
+    uint32_t slot_loc;  // slots u16: local peptide index (chunks hold at most 32768 peptides)
     uint32_t slot_hs;   // slots: hyperscore bits (replay)
     uint32_t slot_sum;  // 2*slots u64 (wide) or u32: ibc, iyc
     uint32_t pk_moz, pk_int;   // max_peaks
-    uint32_t crs;       // group: iA start of each non-empty run
+    uint32_t cab;       // group + 1: entry index minus flat offset of each non-empty run
     uint32_t cofs;      // group + 1: flat offset of each non-empty run (exclusive prefix)
-    uint32_t ck;        // group: peak index of each non-empty run
+    uint32_t cint;      // group: intensity of each non-empty run's peak
+    uint32_t seg;       // kSegWords u64: run ends of the current segment, a bit per entry
     uint32_t wsum;      // 2 * (kBT / 64) wave partial sums + 2 totals
     uint32_t hist;      // hbins
     uint32_t lgf;       // max_shdpeaks + 1 doubles
@@ -206,14 +229,15 @@ __host__ __device__ inline Layout make_layout(const ChunkArgs &a) {
     l.bmp = take(a.tc / 8);
     l.pre = take(a.tc / 8);
     l.slot_cnt = take(4u * a.slots);
-    l.slot_loc = take(4u * a.slots);
+    l.slot_loc = take(2u * a.slots);
     l.slot_hs = take(a.replay ? 4u * a.slots : 0u);
     l.slot_sum = take((a.wide ? 16u : 8u) * a.slots);
     l.pk_moz = take(4u * a.max_peaks);
     l.pk_int = take(4u * a.max_peaks);
-    l.crs = take(4u * a.group);
+    l.cab = take(4u * (a.group + 1));
     l.cofs = take(4u * (a.group + 1));
-    l.ck = take(4u * a.group);
+    l.cint = take(4u * a.group);
+    l.seg = take(8u * kSegWords);
     l.wsum = take(4u * (2 * (kBT / 64) + 2));
     l.hist = take(4u * a.q.hbins);
     l.lgf = take(8u * (a.q.max_shdpeaks + 1));
@@ -225,8 +249,10 @@ __host__ __device__ inline Layout make_layout(const ChunkArgs &a) {
 }
 
 struct Smem {
-    lu32 *map, *bmp, *pre, *slot_cnt, *slot_loc, *slot_hs, *pk_moz, *pk_int, *crs, *cofs, *ck, *wsum, *hist;
-    lu16 *hl;
+    lu32 *map, *bmp, *pre, *slot_cnt, *slot_hs, *pk_moz, *pk_int, *cab, *cofs, *cint, *wsum, *hist;
+    lu32 *seg32;
+    lu64 *seg64;
+    lu16 *slot_loc, *hl;
     lu64 *slot_sum64;
     lu32 *slot_sum32;
     lf64 *lgf;
@@ -240,15 +266,17 @@ __device__ __forceinline__ Smem bind(unsigned char *raw, const Layout &l) {
     s.bmp = (lu32 *)(b + l.bmp);
     s.pre = (lu32 *)(b + l.pre);
     s.slot_cnt = (lu32 *)(b + l.slot_cnt);
-    s.slot_loc = (lu32 *)(b + l.slot_loc);
+    s.slot_loc = (lu16 *)(b + l.slot_loc);
     s.slot_hs = (lu32 *)(b + l.slot_hs);
     s.slot_sum64 = (lu64 *)(b + l.slot_sum);
     s.slot_sum32 = (lu32 *)(b + l.slot_sum);
     s.pk_moz = (lu32 *)(b + l.pk_moz);
     s.pk_int = (lu32 *)(b + l.pk_int);
-    s.crs = (lu32 *)(b + l.crs);
+    s.cab = (lu32 *)(b + l.cab);
     s.cofs = (lu32 *)(b + l.cofs);
-    s.ck = (lu32 *)(b + l.ck);
+    s.cint = (lu32 *)(b + l.cint);
+    s.seg32 = (lu32 *)(b + l.seg);
+    s.seg64 = (lu64 *)(b + l.seg);
     s.wsum = (lu32 *)(b + l.wsum);
     s.hist = (lu32 *)(b + l.hist);
     s.lgf = (lf64 *)(b + l.lgf);
@@ -308,8 +336,12 @@ __device__ __forceinline__ uint2 block_prefix2(const Smem &sm, uint32_t a0, uint
 
 // Run table of the probes [g0, g1) of one item (at most 2 * kBT probes):
 // each (peak, bin) probe's run [bA2[c], bA2[c+1]) in chunk c; the non-empty
-// runs are compacted, in probe order, into crs (iA start), cofs (flat offset,
-// cofs[nc] = E) and ck (the probe's peak).  Returns (E, nc), uniform.
+// runs are compacted, in probe order, into cab (entry index minus flat
+// offset: 32-bit wrap-around, the sum with a flat index is exact; cab[nc]
+// = 0), cofs (flat offset, cofs[nc] = E) and cint (the probe's peak
+// intensity), and the run ends (last entries) of segment 0 go into the
+// bitmap.  Returns (E, nc), uniform.  Every reader of the bitmap and of the
+// run table must be past a barrier.
 __device__ __forceinline__ uint2 setup_runs(const ChunkArgs &a, const Smem &sm, const LenDesc &L, uint32_t c,
                                             uint32_t g0, uint32_t g1, bool count_probes, Acc &acc) {
     const uint32_t W = 2 * a.q.dF + 1;
@@ -332,95 +364,99 @@ __device__ __forceinline__ uint2 setup_runs(const ChunkArgs &a, const Smem &sm, 
             }
         }
     }
+    // the bitmap is cleared before the scan's barrier and filled after it
+    for (uint32_t w = threadIdx.x; w < 2 * kSegWords; w += kBT) sm.seg32[w] = 0;
     uint2 tot;
     const uint2 pre = block_prefix2(sm, (len[0] ? 1u : 0u) + (len[1] ? 1u : 0u), len[0] + len[1], &tot);
     uint32_t ci = pre.x, fo = pre.y;
 #pragma unroll
     for (uint32_t h = 0; h < 2; h++) {
         if (len[h]) {
-            sm.crs[ci] = rs[h];
+            sm.cab[ci] = rs[h] - fo;
             sm.cofs[ci] = fo;
-            sm.ck[ci] = pk[h];
-            ci++;
+            sm.cint[ci] = sm.pk_int[pk[h]];
             fo += len[h];
+            if (fo - 1 < kSegEntries) lds_or(&sm.seg32[(fo - 1) >> 5], 1u << ((fo - 1) & 31u));
+            ci++;
         }
     }
-    if (threadIdx.x == 0) sm.cofs[tot.x] = tot.y;
+    if (threadIdx.x == 0) {
+        sm.cofs[tot.x] = tot.y;
+        sm.cab[tot.x] = 0;
+    }
     __syncthreads();
     return make_uint2(ufl(tot.y), ufl(tot.x));
 }
 
-// Walk over the flat entries [fb, fe) of a group's (compacted, non-empty)
-// runs: thread t takes an equal contiguous share and reads it kStep entries
-// at a time.  A step takes entries from at most two consecutive runs, whose
-// bounds and peaks are held in registers; its loads are unconditional (lanes
-// past the step's end reload its first entry), so they issue back to back.
-// load(gi, f): the entry of flat index f, whose index in the entry array is gi.
-// body(v[kStep], kA, kB, nA, cnt, f): entries i < nA belong to peak kA,
-// entries nA <= i < cnt to peak kB; entry i has flat index f + i.
-constexpr uint32_t kStep = 4;   // measured best of 2-16 (runs average ~17 entries)
-struct StepInfo {
-    uint32_t fs, nA, cnt, ka, kb;
-};
-template <typename Load, typename Body>
-__device__ __forceinline__ void walk_steps(const Smem &sm, uint32_t nc, uint32_t fb, uint32_t fe, Load &&load,
-                                           Body &&body) {
-    const uint32_t n = fe - fb;
-    uint32_t f = fb + (uint32_t)(((uint64_t)n * threadIdx.x) / kBT);
-    const uint32_t f1 = fb + (uint32_t)(((uint64_t)n * (threadIdx.x + 1)) / kBT);
-    if (f >= f1) return;
-    uint32_t lo = 0, hi = nc;   // run holding f: cofs[lo] <= f < cofs[lo + 1]
-    while (hi - lo > 1) {
+// The bitmap for segment [s0, s0 + kSegEntries) of the current run table.
+__device__ __forceinline__ void build_seg(const Smem &sm, uint32_t nc, uint32_t s0) {
+    __syncthreads();
+    for (uint32_t w = threadIdx.x; w < 2 * kSegWords; w += kBT) sm.seg32[w] = 0;
+    __syncthreads();
+    for (uint32_t r = threadIdx.x; r < nc; r += kBT) {
+        const uint32_t f = sm.cofs[r + 1] - 1 - s0;
+        if (f < kSegEntries) lds_or(&sm.seg32[f >> 5], 1u << (f & 31u));
+    }
+    __syncthreads();
+}
+
+// The run holding flat index x: the number of runs that end before x.
+__device__ __forceinline__ uint32_t run_at(const Smem &sm, uint32_t nc, uint32_t x) {
+    uint32_t lo = 0, hi = nc;
+    while (lo < hi) {
         const uint32_t m = (lo + hi) >> 1;
-        if (sm.cofs[m] <= f) lo = m;
+        if (sm.cofs[m + 1] <= x) lo = m + 1;
         else hi = m;
     }
-    // run r: flat [cofs[r], pend), entry index abase + flat (32-bit
-    // wrap-around arithmetic; the sum is the exact index)
-    uint32_t r = lo;
-    uint32_t pend = sm.cofs[r + 1], abase = sm.crs[r] - sm.cofs[r], kA = sm.ck[r];
-    uint32_t pend2 = pend, abase2 = 0, kB = 0;
-    if (r + 1 < nc) {
-        pend2 = sm.cofs[r + 2];
-        abase2 = sm.crs[r + 1] - pend;
-        kB = sm.ck[r + 1];
-    }
-    auto issue = [&](uint32_t (&v)[kStep], StepInfo &st) {
-        const uint32_t m = min(kStep, f1 - f);
-        const uint32_t nA = min(m, pend - f);
-        const uint32_t cnt = nA + min(m - nA, pend2 - pend);
+    return lo;
+}
+
+// Walk the flat entries [s0, se) of the segment in the bitmap: wavefront w
+// takes a contiguous share of its 64-entry tiles, kTileU tiles per step, lane
+// j of a tile the entry F + j.  The entry's run is the number of runs that
+// end before it: the count before the tile (C, carried from tile to tile)
+// plus the tile word's bits below j -- tile_run(t) = mbcnt(t.m, t.C), at
+// most nc.  Per step, load(t, x, ok, v, fs) fills v[] for the kTileU tiles
+// (ok[u]: x[u] is in the wavefront's share; fs: the step's first flat index)
+// and body(t, v, x, ok, fs) consumes them, so that each phase's memory
+// operations of the kTileU tiles issue back to back.
+struct Tile {
+    unsigned long long m;   // run ends in the tile, uniform
+    uint32_t C;             // runs ending before the tile, uniform
+};
+__device__ __forceinline__ uint32_t tile_run(const Tile &t) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(t.m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)t.m, t.C));
+}
+template <typename Load, typename Body>
+__device__ __forceinline__ void walk_tiles(const Smem &sm, uint32_t nc, uint32_t s0, uint32_t se, Load &&load,
+                                           Body &&body) {
+    constexpr uint32_t NW = kBT / 64;
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint32_t T = (se - s0 + 63) >> 6;
+    const uint32_t k0 = ufl(T * w / NW), k1 = ufl(T * (w + 1) / NW);
+    if (k0 >= k1) return;
+    const uint32_t xe = min(se, s0 + 64 * k1);   // end of this wavefront's share
+    uint32_t C = ufl(run_at(sm, nc, s0 + 64 * k0));
+    for (uint32_t k = k0; k < k1; k += kTileU) {
+        Tile t[kTileU];
+        uint32_t x[kTileU], v[kTileU];
+        bool ok[kTileU];
+        unsigned long long mv[kTileU];
 #pragma unroll
-        for (uint32_t i = 0; i < kStep; i++) {
-            const uint32_t j = i < cnt ? i : 0u;
-            v[i] = load((j < nA ? abase : abase2) + f + j, f + j);
+        for (uint32_t u = 0; u < kTileU; u++) mv[u] = sm.seg64[min(k + u, kSegWords - 1)];
+        const uint32_t x0 = s0 + 64 * k + lane;
+#pragma unroll
+        for (uint32_t u = 0; u < kTileU; u++) {
+            unsigned long long m = ((unsigned long long)ufl((uint32_t)(mv[u] >> 32)) << 32) | ufl((uint32_t)mv[u]);
+            m = k + u < k1 ? m : 0ull;
+            t[u].m = m;
+            t[u].C = C;
+            C += (uint32_t)__popcll(m);
+            x[u] = x0 + 64 * u;
+            ok[u] = x[u] < xe;
         }
-        st = StepInfo{f, nA, cnt, kA, kB};
-        f += cnt;
-        if (f < f1 && f >= pend) {
-            // runs are non-empty: at most one more run was consumed whole
-            r++;
-            pend = pend2;
-            abase = abase2;
-            kA = kB;
-            if (f >= pend) {
-                r++;
-                abase = sm.crs[r] - pend;
-                kA = sm.ck[r];
-                pend = sm.cofs[r + 1];
-            }
-            pend2 = pend;
-            if (r + 1 < nc) {
-                pend2 = sm.cofs[r + 2];
-                abase2 = sm.crs[r + 1] - pend;
-                kB = sm.ck[r + 1];
-            }
-        }
-    };
-    uint32_t va[kStep];
-    StepInfo sa;
-    while (f < f1) {
-        issue(va, sa);
-        body(va, sa.ka, sa.kb, sa.nA, sa.cnt, sa.fs);
+        load(t, x, ok, v, s0 + 64 * k);
+        body(t, v, x, ok, s0 + 64 * k);
     }
 }
 
@@ -526,81 +562,108 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
     const uint32_t NP = P * W;
     const uint32_t GP = a.group;
     const uint32_t ng = (NP + GP - 1) / GP;
-    const AS_GLB uint16_t *iAc = (const AS_GLB uint16_t *)L.iAc;
-    auto load_g = [&](uint32_t gi, uint32_t) { return (uint32_t)iAc[gi]; };
+    // iAc through a buffer resource: 32-bit offsets, and reads past the
+    // array return 0 instead of faulting
+    const __amdgpu_buffer_rsrc_t rs_iac =
+        __builtin_amdgcn_make_buffer_rsrc((void *)L.iAc, 0, (int)(L.n * L.speclen * 2u), kBufDword3);
+    const uint32_t lane = threadIdx.x & 63u;
+    // the count at which a peptide qualifies (a peptide without hits scores
+    // -6 and is never a candidate, so min_shp 0 behaves as 1)
+    const uint32_t thr1 = max(a.q.min_shp, 1u) - 1u;
 
     // P1: 8-bit hit counters.  A counter that reaches 0x80 is pulled back to
     // 0x40: only "hits >= min_shp" (min_shp <= kMaxChunkShp) matters here, and
     // the byte never carries into its neighbour unless more than 0x7F other
     // increments of the same peptide land between one lane's add and its
     // pull-back; a byte seen at >= 0xC0 (halfway there) flags the spectrum.
-    // Entries are chunk-local: local << 1 | isY.  The first hl_cap flat
-    // entries of the item (in group order) are also copied to the LDS hit
-    // list, so that P2 need not read them again.
-    uint32_t E = 0, nc = 0, hl_n = 0;
-    const uint32_t lane = threadIdx.x & 63u;
-    for (uint32_t g = 0; g < ng; g++) {
-        const uint32_t g0 = g * GP, g1 = min(NP, g0 + GP);
-        const uint2 en = setup_runs(a, sm, L, c, g0, g1, first, acc);
-        E = en.x;
-        nc = en.y;
-        const uint32_t lim = min(E, a.hl_cap - hl_n);
-        lu16 *hl = sm.hl + hl_n;
-        walk_steps(sm, nc, 0, E, load_g,
-                   [&](const uint32_t (&v)[kStep], uint32_t, uint32_t, uint32_t, uint32_t cnt, uint32_t fs) {
-            uint32_t old[kStep], win = 0;
-#pragma unroll
-            for (uint32_t i = 0; i < kStep; i++)
-                if (i < cnt && (v[i] >> 1) - wlo <= wspan) win |= 1u << i;
-            // every lane adds (0 outside the window, to a word of its own)
-#pragma unroll
-            for (uint32_t i = 0; i < kStep; i++) {
-                const bool k = (win >> i) & 1u;
-                const uint32_t local = v[i] >> 1;
-                old[i] = lds_add_rtn(&sm.map[k ? local >> 2 : lane], k ? 1u << ((local & 3u) << 3) : 0u);
-            }
-#pragma unroll
-            for (uint32_t i = 0; i < kStep; i++) {
-                if ((win >> i) & 1u) {
-                    const uint32_t local = v[i] >> 1;
-                    const uint32_t sh = (local & 3u) << 3;
-                    const uint32_t ob = (old[i] >> sh) & 0xFFu;
-                    if (ob == 0x7Fu) lds_add(&sm.map[local >> 2], 0u - (0x40u << sh));
-                    else if (ob >= 0xC0u) acc.flags |= HCX_RES_COUNT_OVF;
+    // Exactly one increment of a qualifying peptide returns min_shp - 1 (the
+    // pull-back never takes a byte below 0x40 > min_shp - 1): that lane sets
+    // the peptide's qualification bit.  Entries are chunk-local: local << 1 |
+    // isY.  The first hl_cap flat entries of the item (in group order) are
+    // also copied to the LDS hit list, so that P2 need not read them again.
+    // An item whose window covers its whole chunk (FULL) skips the window
+    // test and counts its entries in bulk.
+    const bool full = wlo == 0 && wspan == TC - 1;
+    uint32_t E = 0, nc = 0, hl_n = 0, seg_cur = 0, nia_w = 0;
+    auto p1 = [&](auto full_tag) {
+        constexpr bool FULL = decltype(full_tag)::value;
+        for (uint32_t g = 0; g < ng; g++) {
+            const uint32_t g0 = g * GP, g1 = min(NP, g0 + GP);
+            const uint2 en = setup_runs(a, sm, L, c, g0, g1, first, acc);
+            E = en.x;
+            nc = en.y;
+            seg_cur = 0;
+            if (FULL && threadIdx.x == 0) nia_w += E;   // per wavefront otherwise
+            const uint32_t lim = min(E, a.hl_cap - hl_n);
+            lu16 *hl = sm.hl + hl_n;
+            for (uint32_t s0 = 0; s0 < E; s0 += kSegEntries) {
+                if (s0 != seg_cur) {
+                    build_seg(sm, nc, s0);
+                    seg_cur = s0;
                 }
-                if (i < cnt && fs + i < lim) hl[fs + i] = (uint16_t)v[i];
+                walk_tiles(
+                    sm, nc, s0, min(E, s0 + kSegEntries),
+                    [&](const Tile (&t)[kTileU], const uint32_t (&x)[kTileU], const bool (&)[kTileU],
+                        uint32_t (&v)[kTileU], uint32_t) {
+                        uint32_t gi[kTileU];
+#pragma unroll
+                        for (uint32_t u = 0; u < kTileU; u++) gi[u] = sm.cab[tile_run(t[u])] + x[u];
+                        // lanes outside the share read some entry or, past
+                        // the array, 0 (buffer range check); they add 0
+#pragma unroll
+                        for (uint32_t u = 0; u < kTileU; u++) v[u] = iac_load(rs_iac, gi[u]);
+                    },
+                    [&](const Tile (&)[kTileU], const uint32_t (&v)[kTileU], const uint32_t (&x)[kTileU],
+                        const bool (&ok)[kTileU], uint32_t fs) {
+                        uint32_t old[kTileU];
+                        bool win[kTileU];
+#pragma unroll
+                        for (uint32_t u = 0; u < kTileU; u++) {
+                            const uint32_t local = v[u] >> 1;
+                            win[u] = FULL ? ok[u] : ok[u] && local - wlo <= wspan;
+                            if (!FULL) nia_w += (uint32_t)__popcll(__ballot(win[u]));
+                            // every lane adds: 0 outside the window or the share
+                            old[u] = lds_add_rtn(&sm.map[local >> 2], win[u] ? 1u << ((v[u] & 6u) << 2) : 0u);
+                        }
+#pragma unroll
+                        for (uint32_t u = 0; u < kTileU; u++) {
+                            const uint32_t local = v[u] >> 1;
+                            const uint32_t sh = (v[u] & 6u) << 2;
+                            const uint32_t ob = (old[u] >> sh) & 0xFFu;
+                            if (win[u] && (ob == thr1 || ob >= 0x7Fu)) {
+                                if (ob == thr1) lds_or(&sm.bmp[local >> 5], 1u << (local & 31u));
+                                else if (ob == 0x7Fu) lds_add(&sm.map[local >> 2], 0u - (0x40u << sh));
+                                else if (ob >= 0xC0u) acc.flags |= HCX_RES_COUNT_OVF;
+                            }
+                        }
+                        if (fs < lim) {
+#pragma unroll
+                            for (uint32_t u = 0; u < kTileU; u++)
+                                if (ok[u] && x[u] < lim) hl[x[u]] = (uint16_t)v[u];
+                        }
+                    });
             }
-            acc.nia += __popc(win);
-        });
-        hl_n += lim;
-        __syncthreads();
-    }
-
-    // Q: qualification bitmap, prefix of qualifying peptides per bitmap word,
-    // counters cleared for the next item
-    const uint32_t bw0 = (ilo - base) >> 5, bw1 = ((ihi - base) >> 5) + 1;
-    const uint32_t nbw = bw1 - bw0;
-    {
-        const uint32_t per = (nbw + kBT - 1) / kBT;
-        const uint32_t t0 = bw0 + threadIdx.x * per;
-        const uint32_t t1 = min(bw1, t0 + per);
-        uint32_t tsum = 0;
-        for (uint32_t bw = t0; bw < t1; bw++) {
-            uint32_t bits = 0;
-            for (uint32_t j = 0; j < 8; j++) {
-                const uint32_t v = sm.map[bw * 8 + j];
-                if (!v) continue;
-                for (uint32_t b = 0; b < 4; b++)
-                    if (((v >> (8 * b)) & 0xFFu) >= a.q.min_shp) bits |= 1u << (4 * j + b);
-                sm.map[bw * 8 + j] = 0;
-            }
-            sm.bmp[bw] = bits;
-            sm.pre[bw] = __popc(bits);
-            tsum += __popc(bits);
+            hl_n += lim;
+            __syncthreads();
         }
+    };
+    if (full) p1(std::true_type{});
+    else p1(std::false_type{});
+    if (lane == 0) acc.nia += nia_w;
+
+    // Q: qualifying peptides before each bitmap word; the counters are done
+    // with and are cleared for the next item
+    {
+        const uint32_t nbw = TC >> 5;
+        const uint32_t per = (nbw + kBT - 1) / kBT;
+        const uint32_t t0 = min(nbw, threadIdx.x * per);
+        const uint32_t t1 = min(nbw, t0 + per);
+        uint32_t tsum = 0;
+        for (uint32_t bw = t0; bw < t1; bw++) tsum += __popc(sm.bmp[bw]);
+        for (uint32_t i = threadIdx.x; i < TC / 16; i += kBT) ((lu32x4 *)sm.map)[i] = (u32x4){0u, 0u, 0u, 0u};
         uint32_t run = block_prefix(sm, tsum);
         for (uint32_t bw = t0; bw < t1; bw++) {
-            const uint32_t cnt = sm.pre[bw];
+            const uint32_t cnt = __popc(sm.bmp[bw]);
             sm.pre[bw] = run;
             run += cnt;
         }
@@ -624,29 +687,8 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
         __syncthreads();
         // P2: exact b/y counts and intensity sums of slots [r0, r0 + ns),
         // group by group: the listed entries from the hit list, the rest
-        // from the entry array again
-        auto tally = [&](const uint32_t (&v)[kStep], uint32_t ka, uint32_t kb, uint32_t nA, uint32_t cnt,
-                         uint32_t) {
-#pragma unroll
-            for (uint32_t i = 0; i < kStep; i++) {
-                const uint32_t local = v[i] >> 1;
-                if (i < cnt && local - wlo <= wspan) {
-                    const uint32_t w = local >> 5, bit = local & 31u;
-                    const uint32_t bits = sm.bmp[w];
-                    if ((bits >> bit) & 1u) {
-                        const uint32_t sl = sm.pre[w] + __popc(bits & ((1u << bit) - 1u)) - r0;
-                        if (sl < ns) {
-                            const uint32_t isY = v[i] & 1u;
-                            const uint32_t it = sm.pk_int[i < nA ? ka : kb];
-                            lds_add(&sm.slot_cnt[sl], isY ? 0x10000u : 1u);
-                            if (WIDE) lds_add64(&sm.slot_sum64[2 * sl + isY], (unsigned long long)it);
-                            else lds_add(&sm.slot_sum32[2 * sl + isY], it);
-                            sm.slot_loc[sl] = local;
-                        }
-                    }
-                }
-            }
-        };
+        // from the entry array again.  Only the bitmap matters here: a
+        // peptide outside the window never has its bit set.
         uint32_t hb = 0;
         for (uint32_t g = 0; g < ng; g++) {
             const uint32_t g0 = g * GP, g1 = min(NP, g0 + GP);
@@ -654,11 +696,56 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
                 const uint2 en = setup_runs(a, sm, L, c, g0, g1, false, acc);
                 E = en.x;
                 nc = en.y;
+                seg_cur = 0;
             }
             const uint32_t lim = min(E, a.hl_cap - hb);
             const lu16 *hl = sm.hl + hb;
-            if (lim) walk_steps(sm, nc, 0, lim, [&](uint32_t, uint32_t f) { return (uint32_t)hl[f]; }, tally);
-            if (lim < E) walk_steps(sm, nc, lim, E, load_g, tally);
+            for (uint32_t s0 = 0; s0 < E; s0 += kSegEntries) {
+                if (s0 != seg_cur) {
+                    build_seg(sm, nc, s0);
+                    seg_cur = s0;
+                }
+                walk_tiles(
+                    sm, nc, s0, min(E, s0 + kSegEntries),
+                    [&](const Tile (&t)[kTileU], const uint32_t (&x)[kTileU], const bool (&ok)[kTileU],
+                        uint32_t (&v)[kTileU], uint32_t fs) {
+                        if (fs + 64 * kTileU <= lim) {   // the whole step is in the hit list
+#pragma unroll
+                            for (uint32_t u = 0; u < kTileU; u++) v[u] = hl[x[u]];
+                            return;
+                        }
+                        uint32_t gi[kTileU];
+#pragma unroll
+                        for (uint32_t u = 0; u < kTileU; u++) gi[u] = sm.cab[tile_run(t[u])] + x[u];
+#pragma unroll
+                        for (uint32_t u = 0; u < kTileU; u++) v[u] = iac_load(rs_iac, gi[u]);
+                        if (fs < lim) {
+#pragma unroll
+                            for (uint32_t u = 0; u < kTileU; u++)
+                                if (x[u] < lim) v[u] = hl[x[u]];
+                        }
+                    },
+                    [&](const Tile (&t)[kTileU], const uint32_t (&v)[kTileU], const uint32_t (&)[kTileU],
+                        const bool (&ok)[kTileU], uint32_t) {
+                        uint32_t bits[kTileU];
+#pragma unroll
+                        for (uint32_t u = 0; u < kTileU; u++) bits[u] = sm.bmp[v[u] >> 6];
+#pragma unroll
+                        for (uint32_t u = 0; u < kTileU; u++) {
+                            const uint32_t bit = (v[u] >> 1) & 31u;
+                            if (!(ok[u] && ((bits[u] >> bit) & 1u))) continue;
+                            const uint32_t local = v[u] >> 1;
+                            const uint32_t sl = sm.pre[local >> 5] + __popc(bits[u] & ((1u << bit) - 1u)) - r0;
+                            if (sl >= ns) continue;
+                            const uint32_t isY = v[u] & 1u;
+                            const uint32_t it = sm.cint[tile_run(t[u])];
+                            lds_add(&sm.slot_cnt[sl], isY ? 0x10000u : 1u);
+                            if (WIDE) lds_add64(&sm.slot_sum64[2 * sl + isY], (unsigned long long)it);
+                            else lds_add(&sm.slot_sum32[2 * sl + isY], it);
+                            sm.slot_loc[sl] = (uint16_t)local;
+                        }
+                    });
+            }
             hb += lim;
             if (ng > 1) __syncthreads();
         }
@@ -730,6 +817,10 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
             __syncthreads();
         }
     }
+    // the qualification bitmap's last reader (P2) is past a barrier; with no
+    // qualifying peptide it is already clear
+    if (Qn)
+        for (uint32_t bw = threadIdx.x; bw < (TC >> 5); bw += kBT) sm.bmp[bw] = 0;
 }
 
 // Merge a block's accumulators for spectrum s into the global SpecAcc.
@@ -806,6 +897,7 @@ __device__ __forceinline__ uint32_t load_peaks(const ChunkArgs &a, const Smem &s
 __device__ __forceinline__ void init_smem(const ChunkArgs &a, const Smem &sm) {
     for (uint32_t i = threadIdx.x; i <= a.q.max_shdpeaks; i += kBT) sm.lgf[i] = a.q.lgf[i];
     for (uint32_t i = threadIdx.x; i < a.tc / 4; i += kBT) sm.map[i] = 0;
+    for (uint32_t i = threadIdx.x; i < a.tc / 32; i += kBT) sm.bmp[i] = 0;
     for (uint32_t i = threadIdx.x; i < a.q.hbins; i += kBT) sm.hist[i] = 0;
 }
 
@@ -1169,6 +1261,16 @@ int launch_replay_chunked(hcx_ctx *ctx, ChunkArgs &a) {
     if (a.q.nspec <= 0) return HCX_SUCCESS;
     ChunkArgs r = a;
     r.replay = 1;   // the replay layout adds the slots' hyperscores
+    // its hit list takes what is left of a workgroup's share beside the
+    // static heap (5 x 256 words) and control words
+    r.hl_cap = 0;
+    const size_t base = chunk_smem_bytes(r);
+    const size_t share = (size_t)ctx->max_lds / std::max<uint32_t>(1, ctx->blocks_per_cu);
+    const size_t fixed = 5 * 256 * 4 + 256;
+    if (share > base + fixed + 4096)
+        r.hl_cap = std::min<uint32_t>(a.hl_cap, (uint32_t)(((share - base - fixed) / 2) & ~(size_t)63));
+    if (chunk_smem_bytes(r) + fixed > (size_t)ctx->max_lds)
+        return set_error_code(HCX_ERR_INVLD_PARAM, "search: the replay layout does not fit in LDS");
     return r.wide ? launch_replay_t<true>(ctx, r) : launch_replay_t<false>(ctx, r);
 }
 
