@@ -168,6 +168,10 @@ extern "C" int hcx_set_option(hcx_ctx *c, int32_t option, int64_t value) {
         if (value != 0 && value != 1) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_set_option: item order is 0 or 1");
         if (c->item_order != (int32_t)value) c->item_order = (int32_t)value, ++c->item_gen;
         return HCX_SUCCESS;
+    case HCX_OPT_GROUP:
+        if (value < 64 || value > 1024) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_set_option: group is 64..1024");
+        c->group = (uint32_t)value;
+        return HCX_SUCCESS;
     default:
         return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_set_option: unknown option");
     }
@@ -178,6 +182,7 @@ extern "C" int hcx_get_option(const hcx_ctx *c, int32_t option, int64_t *value) 
     switch (option) {
     case HCX_OPT_HITLIST: *value = c->hitlist; return HCX_SUCCESS;
     case HCX_OPT_ITEM_ORDER: *value = c->item_order; return HCX_SUCCESS;
+    case HCX_OPT_GROUP: *value = c->group; return HCX_SUCCESS;
     default: return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_get_option: unknown option");
     }
 }
@@ -554,7 +559,7 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
     ca.tc = c->chunk;
     ca.slots = c->slots;
     ca.max_peaks = std::max<uint32_t>(1, qb->max_peaks);
-    ca.group = std::min<uint32_t>(1024, ca.max_peaks * (2 * c->params.dF + 1));
+    ca.group = std::min<uint32_t>(c->group, ca.max_peaks * (2 * c->params.dF + 1));
     // the hit list (16-bit entries) takes the LDS left in a workgroup's share
     // of the CU once blocks_per_cu workgroups are resident
     // 32-bit slot sums when no peptide's sum can reach 2^32: at most

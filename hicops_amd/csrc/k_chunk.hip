@@ -79,7 +79,13 @@ constexpr unsigned long long kLow25 = (1ull << 25) - 1;
 // per 64-entry tile.
 constexpr uint32_t kSegWords = 256;
 constexpr uint32_t kSegEntries = 64u * kSegWords;
-constexpr uint32_t kTileU = 4;       // tiles per wavefront step (loads in flight)
+#ifndef HCX_TILE_U
+#define HCX_TILE_U 8
+#endif
+constexpr uint32_t kTileU = HCX_TILE_U;   // tiles per wavefront step (loads in flight)
+// histogram bins kept in LDS (hyperscores below 51.15); the rare higher
+// bins go to the global histogram directly
+constexpr uint32_t kHistLds = 512;
 
 __device__ __forceinline__ uint32_t ldg(const uint32_t *p, size_t i) { return ((gcu32 *)p)[i]; }
 // raw buffer resource word 3 on gfx9-family parts (gfx950 included): byte
@@ -208,7 +214,7 @@ struct Layout {
     uint32_t cint;      // group: intensity of each non-empty run's peak
     uint32_t seg;       // kSegWords u64: run ends of the current segment, a bit per entry
     uint32_t wsum;      // 2 * (kBT / 64) wave partial sums + 2 totals
-    uint32_t hist;      // hbins
+    uint32_t hist;      // min(hbins, kHistLds): the low bins of the spectrum's histogram
     uint32_t lgf;       // max_shdpeaks + 1 doubles
     uint32_t wlo, whi;  // nlen
     uint32_t hl;        // hl_cap u16: hit list of the current item
@@ -239,7 +245,7 @@ __host__ __device__ inline Layout make_layout(const ChunkArgs &a) {
     l.cint = take(4u * a.group);
     l.seg = take(8u * kSegWords);
     l.wsum = take(4u * (2 * (kBT / 64) + 2));
-    l.hist = take(4u * a.q.hbins);
+    l.hist = take(4u * min(a.q.hbins, kHistLds));
     l.lgf = take(8u * (a.q.max_shdpeaks + 1));
     l.wlo = take(4u * a.q.nlen);
     l.whi = take(4u * a.q.nlen);
@@ -548,7 +554,8 @@ __device__ __forceinline__ Cell heap_getmax(const Heap &h, int size) {
 // WIDE: 64-bit intensity sums (otherwise 32-bit, chosen by the host when the
 // batch's largest intensity times the most hits a peptide can take fits).
 template <bool ORDERED, bool WIDE>
-__device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm, lu32 *s_ctl, uint32_t P, int ixx,
+__device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm, lu32 *s_ctl, uint32_t spec,
+                                             uint32_t P, int ixx,
                                              uint32_t c, int32_t lo, int32_t hi, Acc &acc, const Heap *heap = nullptr,
                                              int *hsize = nullptr) {
     const LenDesc L = load_len_u(a, (uint32_t)ixx);
@@ -775,7 +782,8 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
             acc.cpsms++;
             int bin = (int)((double)(hs * 10.0f) + 0.5);
             if (bin >= (int)a.q.hbins) { bin = (int)a.q.hbins - 1; acc.flags |= HCX_RES_HIST_CLAMP; }
-            lds_add(&sm.hist[bin], 1u);
+            if ((uint32_t)bin < kHistLds) lds_add(&sm.hist[bin], 1u);
+            else glb_add(a.q.hist + (size_t)spec * a.q.hbins + bin, 1u);
             if (hs > acc.best_hs) {
                 acc.best_hs = hs;
                 acc.best_cnt = 1;
@@ -851,7 +859,7 @@ __device__ __forceinline__ void flush_spectrum(const ChunkArgs &a, const Smem &s
     }
     // histogram: non-zero bins to global, then clear
     uint32_t *hg = a.q.hist + (size_t)s * a.q.hbins;
-    for (uint32_t h = threadIdx.x; h < a.q.hbins; h += kBT) {
+    for (uint32_t h = threadIdx.x; h < min(a.q.hbins, kHistLds); h += kBT) {
         const uint32_t v = sm.hist[h];
         if (v) {
             glb_add(&hg[h], v);
@@ -898,7 +906,7 @@ __device__ __forceinline__ void init_smem(const ChunkArgs &a, const Smem &sm) {
     for (uint32_t i = threadIdx.x; i <= a.q.max_shdpeaks; i += kBT) sm.lgf[i] = a.q.lgf[i];
     for (uint32_t i = threadIdx.x; i < a.tc / 4; i += kBT) sm.map[i] = 0;
     for (uint32_t i = threadIdx.x; i < a.tc / 32; i += kBT) sm.bmp[i] = 0;
-    for (uint32_t i = threadIdx.x; i < a.q.hbins; i += kBT) sm.hist[i] = 0;
+    for (uint32_t i = threadIdx.x; i < min(a.q.hbins, kHistLds); i += kBT) sm.hist[i] = 0;
 }
 
 // Plain form: one workgroup per spectrum, its items in order.
@@ -933,7 +941,7 @@ __global__ __launch_bounds__(kBT, kWavesPerSimd) void k_spec_chunked(ChunkArgs a
         const int32_t lo = ufl((int32_t)sm.wlo[ixx]), hi = ufl((int32_t)sm.whi[ixx]);
         if (lo > hi) continue;
         for (uint32_t c = (uint32_t)lo / a.tc; c <= (uint32_t)hi / a.tc; c++)
-            process_item<false, WIDE>(a, sm, s_ctl, P, ixx, c, lo, hi, acc);
+            process_item<false, WIDE>(a, sm, s_ctl, s, P, ixx, c, lo, hi, acc);
     }
     flush_spectrum(a, sm, s, acc, s_red, s_red64);
 }
@@ -978,7 +986,7 @@ __global__ __launch_bounds__(kBT, kWavesPerSimd) void k_items_chunked(ChunkArgs 
                 P = load_peaks(a, sm, cur);
                 __syncthreads();
             }
-            process_item<false, WIDE>(a, sm, s_ctl, P, item.ixx, item.chunk, item.lo, item.hi, acc);
+            process_item<false, WIDE>(a, sm, s_ctl, cur, P, item.ixx, item.chunk, item.lo, item.hi, acc);
         }
     }
     if (cur != 0xFFFFFFFFu) flush_spectrum(a, sm, cur, acc, s_red, s_red64);
@@ -1023,7 +1031,7 @@ __global__ __launch_bounds__(kBT) void k_replay_chunked(ChunkArgs a) {
             const int32_t lo = ufl((int32_t)sm.wlo[ixx]), hi = ufl((int32_t)sm.whi[ixx]);
             if (lo > hi) continue;
             for (uint32_t c = (uint32_t)lo / a.tc; c <= (uint32_t)hi / a.tc; c++)
-                process_item<true, WIDE>(a, sm, s_ctl, P, ixx, c, lo, hi, acc, &heap, &hsize);
+                process_item<true, WIDE>(a, sm, s_ctl, s, P, ixx, c, lo, hi, acc, &heap, &hsize);
         }
         if (threadIdx.x == 0 && hsize > 0) {
             const Cell m = heap_getmax(heap, hsize);

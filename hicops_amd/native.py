@@ -40,6 +40,7 @@ KERNEL_BATCHED = 1
 KERNEL_CURSOR = 2
 OPT_HITLIST = 1
 OPT_ITEM_ORDER = 2
+OPT_GROUP = 3
 EVAL_TAILFIT = 0
 EVAL_SURROGATE = 1
 

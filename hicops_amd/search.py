@@ -109,6 +109,10 @@ class Engine:
         """Batched work items by spectrum (False) or by (length, chunk) (True)."""
         N.check(N.hcx().hcx_set_option(self._h, N.OPT_ITEM_ORDER, C.c_int64(1 if by_chunk else 0)))
 
+    def set_group(self, probes: int) -> None:
+        """(peak, bin) probes per run table of the chunked kernels (64..1024)."""
+        N.check(N.hcx().hcx_set_option(self._h, N.OPT_GROUP, C.c_int64(probes)))
+
     def tuning(self) -> dict:
         ch, sl, bp = C.c_uint32(), C.c_uint32(), C.c_uint32()
         cu, lds = C.c_int32(), C.c_int32()

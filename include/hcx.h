@@ -166,9 +166,13 @@ int  hcx_set_tuning(hcx_ctx *ctx, uint32_t chunk, uint32_t slots, uint32_t block
  *   (rounded down to a multiple of 64).
  * HCX_OPT_ITEM_ORDER: order of the batched kernel's work items.
  *   0: by spectrum (default); 1: by (length, chunk), so that concurrent
- *   workgroups share chunks of iA in L2. */
+ *   workgroups share chunks of iA in L2.
+ * HCX_OPT_GROUP: (peak, bin) probes per run table of the chunked kernels,
+ *   64..1024 (default 1024); a work item with more probes walks its entries
+ *   group by group.  Smaller groups take less LDS (12 bytes per probe). */
 #define HCX_OPT_HITLIST    1
 #define HCX_OPT_ITEM_ORDER 2
+#define HCX_OPT_GROUP      3
 int  hcx_set_option(hcx_ctx *ctx, int32_t option, int64_t value);
 int  hcx_get_option(const hcx_ctx *ctx, int32_t option, int64_t *value);
 int  hcx_get_tuning(const hcx_ctx *ctx, uint32_t *chunk, uint32_t *slots, uint32_t *blocks_per_cu,
