@@ -72,7 +72,13 @@ constexpr int kBT = HCX_CHUNK_THREADS;   // threads per workgroup of the chunked
 // at most 128 VGPRs: four waves per SIMD (two 512-thread workgroups per CU)
 constexpr int kWavesPerSimd = 4;
 constexpr uint32_t kNoProbe = 0xFFFFFFFFu;
-constexpr uint32_t kGrab = 4;        // items taken per queue atomic
+#ifndef HCX_GRAB
+#define HCX_GRAB 32
+#endif
+// items taken per queue atomic, at most: a grab of consecutive items is
+// mostly one spectrum's, so fewer grabs mean fewer per-spectrum merges; near
+// the end of the queue grabs shrink so the workgroups finish together
+constexpr uint32_t kGrab = HCX_GRAB;
 constexpr unsigned long long kLow25 = (1ull << 25) - 1;
 // A group's flat entries are walked in segments of kSegEntries; the run
 // starts of the current segment are a bitmap of kSegWords 64-bit words, one
@@ -965,14 +971,17 @@ __global__ __launch_bounds__(kBT, kWavesPerSimd) void k_items_chunked(ChunkArgs 
     acc_reset(acc);
     uint32_t cur = 0xFFFFFFFFu, P = 0;
     const Item *items = a.items;
+    uint32_t grab = max(1u, min(kGrab, a.nitems / (4u * gridDim.x)));
     for (;;) {
         __syncthreads();
-        if (threadIdx.x == 0) *s_item = __hip_atomic_fetch_add((gu32 *)a.queue, kGrab, __ATOMIC_RELAXED,
+        if (threadIdx.x == 0) *s_item = __hip_atomic_fetch_add((gu32 *)a.queue, grab, __ATOMIC_RELAXED,
                                                                __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
         const uint32_t it0 = ufl((uint32_t)*s_item);
         if (it0 >= a.nitems) break;
-        const uint32_t it1 = min(a.nitems, it0 + kGrab);
+        const uint32_t it1 = min(a.nitems, it0 + grab);
+        // guided: about a quarter of each workgroup's share of what is left
+        grab = max(1u, min(kGrab, (a.nitems - it1) / (4u * gridDim.x)));
         for (uint32_t it = it0; it < it1; it++) {
             Item item = items[it];
             item.spec = ufl(item.spec);
