@@ -169,7 +169,8 @@ extern "C" int hcx_set_option(hcx_ctx *c, int32_t option, int64_t value) {
         if (c->item_order != (int32_t)value) c->item_order = (int32_t)value, ++c->item_gen;
         return HCX_SUCCESS;
     case HCX_OPT_GROUP:
-        if (value < 64 || value > 1024) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_set_option: group is 64..1024");
+        if (value < 64 || value > 2 * (int64_t)kChunkThreads)
+            return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_set_option: group is 64..2 x the workgroup's threads");
         c->group = (uint32_t)value;
         return HCX_SUCCESS;
     default:

@@ -65,10 +65,7 @@ typedef const AS_GLB uint32_t gcu32;
 typedef AS_GLB uint32_t gu32;
 typedef AS_GLB unsigned long long gu64;
 
-#ifndef HCX_CHUNK_THREADS
-#define HCX_CHUNK_THREADS 512
-#endif
-constexpr int kBT = HCX_CHUNK_THREADS;   // threads per workgroup of the chunked kernels
+constexpr int kBT = (int)kChunkThreads;   // threads per workgroup of the chunked kernels
 // at most 128 VGPRs: four waves per SIMD (two 512-thread workgroups per CU)
 constexpr int kWavesPerSimd = 4;
 constexpr uint32_t kNoProbe = 0xFFFFFFFFu;
@@ -687,17 +684,21 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
 
     for (uint32_t r0 = 0; r0 < Qn; r0 += a.slots) {
         const uint32_t ns = min(a.slots, Qn - r0);
-        for (uint32_t sl = threadIdx.x; sl < ns; sl += kBT) {
-            sm.slot_cnt[sl] = 0;
-            if (WIDE) {
-                sm.slot_sum64[2 * sl] = 0;
-                sm.slot_sum64[2 * sl + 1] = 0;
-            } else {
-                sm.slot_sum32[2 * sl] = 0;
-                sm.slot_sum32[2 * sl + 1] = 0;
+        // the slots start at zero: S clears what it has read, except in the
+        // replay, whose heap feed reads them after S
+        if (ORDERED) {
+            for (uint32_t sl = threadIdx.x; sl < ns; sl += kBT) {
+                sm.slot_cnt[sl] = 0;
+                if (WIDE) {
+                    sm.slot_sum64[2 * sl] = 0;
+                    sm.slot_sum64[2 * sl + 1] = 0;
+                } else {
+                    sm.slot_sum32[2 * sl] = 0;
+                    sm.slot_sum32[2 * sl + 1] = 0;
+                }
             }
+            __syncthreads();
         }
-        __syncthreads();
         // P2: exact b/y counts and intensity sums of slots [r0, r0 + ns),
         // group by group: the listed entries from the hit list, the rest
         // from the entry array again.  Only the bitmap matters here: a
@@ -770,7 +771,18 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
             const uint32_t bc = cw & 0xFFFFu, yc = cw >> 16;
             const unsigned long long ib = WIDE ? sm.slot_sum64[2 * sl] : sm.slot_sum32[2 * sl];
             const unsigned long long iy = WIDE ? sm.slot_sum64[2 * sl + 1] : sm.slot_sum32[2 * sl + 1];
-            if (ORDERED) sm.slot_hs[sl] = 0u;   // +0.0f: not inserted
+            if (ORDERED) {
+                sm.slot_hs[sl] = 0u;   // +0.0f: not inserted
+            } else {
+                sm.slot_cnt[sl] = 0;
+                if (WIDE) {
+                    sm.slot_sum64[2 * sl] = 0;
+                    sm.slot_sum64[2 * sl + 1] = 0;
+                } else {
+                    sm.slot_sum32[2 * sl] = 0;
+                    sm.slot_sum32[2 * sl + 1] = 0;
+                }
+            }
             if ((uint16_t)(bc + yc) < a.q.min_shp) continue;
             uint32_t bi = bc, yi = yc;
             if (bi > a.q.max_shdpeaks) { bi = a.q.max_shdpeaks; acc.flags |= HCX_RES_LGF_CLAMP; }
@@ -912,6 +924,16 @@ __device__ __forceinline__ void init_smem(const ChunkArgs &a, const Smem &sm) {
     for (uint32_t i = threadIdx.x; i <= a.q.max_shdpeaks; i += kBT) sm.lgf[i] = a.q.lgf[i];
     for (uint32_t i = threadIdx.x; i < a.tc / 4; i += kBT) sm.map[i] = 0;
     for (uint32_t i = threadIdx.x; i < a.tc / 32; i += kBT) sm.bmp[i] = 0;
+    for (uint32_t i = threadIdx.x; i < a.slots; i += kBT) {
+        sm.slot_cnt[i] = 0;
+        if (a.wide) {
+            sm.slot_sum64[2 * i] = 0;
+            sm.slot_sum64[2 * i + 1] = 0;
+        } else {
+            sm.slot_sum32[2 * i] = 0;
+            sm.slot_sum32[2 * i + 1] = 0;
+        }
+    }
     for (uint32_t i = threadIdx.x; i < min(a.q.hbins, kHistLds); i += kBT) sm.hist[i] = 0;
 }
 
