@@ -169,8 +169,8 @@ extern "C" int hcx_set_option(hcx_ctx *c, int32_t option, int64_t value) {
         if (c->item_order != (int32_t)value) c->item_order = (int32_t)value, ++c->item_gen;
         return HCX_SUCCESS;
     case HCX_OPT_GROUP:
-        if (value < 64 || value > 2 * (int64_t)kChunkThreads)
-            return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_set_option: group is 64..2 x the workgroup's threads");
+        if (value != 0 && (value < 64 || value > (int64_t)(kProbesPerThread * kChunkThreads)))
+            return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_set_option: group is 0 (automatic) or 64..5 x the workgroup threads");
         c->group = (uint32_t)value;
         return HCX_SUCCESS;
     default:
@@ -560,9 +560,6 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
     ca.tc = c->chunk;
     ca.slots = c->slots;
     ca.max_peaks = std::max<uint32_t>(1, qb->max_peaks);
-    ca.group = std::min<uint32_t>(c->group, ca.max_peaks * (2 * c->params.dF + 1));
-    // the hit list (16-bit entries) takes the LDS left in a workgroup's share
-    // of the CU once blocks_per_cu workgroups are resident
     // 32-bit slot sums when no peptide's sum can reach 2^32: at most
     // speclen * max_mult hits per peptide, each at most the batch's largest
     // intensity
@@ -574,11 +571,26 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
     }
     ca.hl_cap = 0;
     {
-        const size_t base = chunk_smem_bytes(ca);
         const size_t share = (size_t)c->max_lds / std::max<uint32_t>(1, c->blocks_per_cu);
         // the kernels' static LDS (control words) is under 256 bytes; the
         // replay kernel sizes its own hit list
         const size_t reserve = 256;
+        // run-table group: an item walks its probes group by group, and every
+        // extra group costs the exact-count pass another run setup, so by
+        // default the group is as large as the workgroup's LDS share allows
+        // (12 bytes per probe), but not below 1024 probes
+        const uint32_t need = std::max<uint32_t>(1, ca.max_peaks * (2 * c->params.dF + 1));
+        if (c->group) {
+            ca.group = std::min<uint32_t>(c->group, need);
+        } else {
+            const uint32_t floor = std::min<uint32_t>(need, 1024);
+            ca.group = std::min<uint32_t>(need, kProbesPerThread * kChunkThreads);
+            while (ca.group > floor && chunk_smem_bytes(ca) + reserve > share)
+                ca.group = std::max<uint32_t>(floor, ca.group - 128);
+        }
+        // the hit list (16-bit entries) takes the LDS left in a workgroup's
+        // share of the CU once blocks_per_cu workgroups are resident
+        const size_t base = chunk_smem_bytes(ca);
         if (share > base + reserve + 4096) ca.hl_cap = (uint32_t)(((share - base - reserve) / 2) & ~(size_t)63);
         if (c->hitlist >= 0) ca.hl_cap = std::min<uint32_t>(ca.hl_cap, (uint32_t)c->hitlist & ~63u);
     }

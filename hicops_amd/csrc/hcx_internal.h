@@ -26,8 +26,9 @@ int set_error_code(int code, const std::string &msg);
 #define HCX_CHUNK_THREADS 512
 #endif
 // threads per workgroup of the chunked scan kernels; a run-table group holds
-// at most two probes per thread
+// at most kProbesPerThread probes per thread
 constexpr uint32_t kChunkThreads = HCX_CHUNK_THREADS;
+constexpr uint32_t kProbesPerThread = 5;
 
 // Per-length index as the query kernels read it.
 struct LenDesc {
@@ -89,7 +90,7 @@ struct hcx_ctx {
     uint32_t blocks_per_cu = 2;       // persistent grid of the batched kernel
     int32_t hitlist = -1;             // LDS hit-list entries: < 0 as many as fit
     int32_t item_order = 0;           // batched work items: 0 by spectrum, 1 by (length, chunk)
-    uint32_t group = 2 * hcx::kChunkThreads;   // probes per run table (64 .. 2 x threads)
+    uint32_t group = 0;               // probes per run table: 0 as many as fit, else 64..kProbesPerThread x threads
     uint32_t item_gen = 0;            // bumped when item_order changes (cached item lists)
     int max_lds = 65536;              // device limit per workgroup
     // the captured search sequence and the launch shape it was captured for

@@ -66,8 +66,12 @@ typedef AS_GLB uint32_t gu32;
 typedef AS_GLB unsigned long long gu64;
 
 constexpr int kBT = (int)kChunkThreads;   // threads per workgroup of the chunked kernels
+constexpr uint32_t kPPT = kProbesPerThread;   // run-table probes per thread
 // at most 128 VGPRs: four waves per SIMD (two 512-thread workgroups per CU)
-constexpr int kWavesPerSimd = 4;
+#ifndef HCX_WAVES_PER_SIMD
+#define HCX_WAVES_PER_SIMD 4
+#endif
+constexpr int kWavesPerSimd = HCX_WAVES_PER_SIMD;
 constexpr uint32_t kNoProbe = 0xFFFFFFFFu;
 #ifndef HCX_GRAB
 #define HCX_GRAB 32
@@ -343,7 +347,7 @@ __device__ __forceinline__ uint2 block_prefix2(const Smem &sm, uint32_t a0, uint
     return make_uint2(rx, ry);
 }
 
-// Run table of the probes [g0, g1) of one item (at most 2 * kBT probes):
+// Run table of the probes [g0, g1) of one item (at most kPPT * kBT probes):
 // each (peak, bin) probe's run [bA2[c], bA2[c+1]) in chunk c; the non-empty
 // runs are compacted, in probe order, into cab (entry index minus flat
 // offset: 32-bit wrap-around, the sum with a flat index is exact; cab[nc]
@@ -356,10 +360,12 @@ __device__ __forceinline__ uint2 setup_runs(const ChunkArgs &a, const Smem &sm, 
     const uint32_t W = 2 * a.q.dF + 1;
     const uint32_t n = g1 - g0;
     const uint32_t stride = L.nch + 1;
-    uint32_t rs[2] = {0, 0}, len[2] = {0, 0}, pk[2] = {0, 0};
+    uint32_t rs[kPPT], len[kPPT], pk[kPPT];
+    uint32_t nr = 0, ne = 0;
 #pragma unroll
-    for (uint32_t h = 0; h < 2; h++) {
-        const uint32_t lp = 2 * threadIdx.x + h;
+    for (uint32_t h = 0; h < kPPT; h++) {
+        rs[h] = len[h] = pk[h] = 0;
+        const uint32_t lp = h * kBT + threadIdx.x;   // any fixed order: P1 and P2 share it
         if (lp < n) {
             const uint32_t p = g0 + lp;
             const uint32_t k = p / W, j = p - k * W;
@@ -372,14 +378,16 @@ __device__ __forceinline__ uint2 setup_runs(const ChunkArgs &a, const Smem &sm, 
                 if (count_probes) acc.nprobe++;
             }
         }
+        nr += len[h] ? 1u : 0u;
+        ne += len[h];
     }
     // the bitmap is cleared before the scan's barrier and filled after it
     for (uint32_t w = threadIdx.x; w < 2 * kSegWords; w += kBT) sm.seg32[w] = 0;
     uint2 tot;
-    const uint2 pre = block_prefix2(sm, (len[0] ? 1u : 0u) + (len[1] ? 1u : 0u), len[0] + len[1], &tot);
+    const uint2 pre = block_prefix2(sm, nr, ne, &tot);
     uint32_t ci = pre.x, fo = pre.y;
 #pragma unroll
-    for (uint32_t h = 0; h < 2; h++) {
+    for (uint32_t h = 0; h < kPPT; h++) {
         if (len[h]) {
             sm.cab[ci] = rs[h] - fo;
             sm.cofs[ci] = fo;

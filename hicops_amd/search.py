@@ -110,7 +110,7 @@ class Engine:
         N.check(N.hcx().hcx_set_option(self._h, N.OPT_ITEM_ORDER, C.c_int64(1 if by_chunk else 0)))
 
     def set_group(self, probes: int) -> None:
-        """(peak, bin) probes per run table of the chunked kernels (64..1024)."""
+        """(peak, bin) probes per run table of the chunked kernels: 0 automatic, else 64..2560."""
         N.check(N.hcx().hcx_set_option(self._h, N.OPT_GROUP, C.c_int64(probes)))
 
     def tuning(self) -> dict:

@@ -167,9 +167,10 @@ int  hcx_set_tuning(hcx_ctx *ctx, uint32_t chunk, uint32_t slots, uint32_t block
  * HCX_OPT_ITEM_ORDER: order of the batched kernel's work items.
  *   0: by spectrum (default); 1: by (length, chunk), so that concurrent
  *   workgroups share chunks of iA in L2.
- * HCX_OPT_GROUP: (peak, bin) probes per run table of the chunked kernels,
- *   64..1024 (default 1024); a work item with more probes walks its entries
- *   group by group.  Smaller groups take less LDS (12 bytes per probe). */
+ * HCX_OPT_GROUP: (peak, bin) probes per run table of the chunked kernels.
+ *   0 (default): as many as the workgroup's LDS share holds beside the other
+ *   arrays, up to 2560 and not below 1024; n: 64..2560.  A work item with
+ *   more probes walks its entries group by group (12 bytes of LDS per probe). */
 #define HCX_OPT_HITLIST    1
 #define HCX_OPT_ITEM_ORDER 2
 #define HCX_OPT_GROUP      3
