@@ -560,6 +560,7 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
     ca.tc = c->chunk;
     ca.slots = c->slots;
     ca.max_peaks = std::max<uint32_t>(1, qb->max_peaks);
+    make_divider(2 * c->params.dF + 1, &ca.wdiv_m, &ca.wdiv_s);
     // 32-bit slot sums when no peptide's sum can reach 2^32: at most
     // speclen * max_mult hits per peptide, each at most the batch's largest
     // intensity

@@ -179,6 +179,7 @@ struct ChunkArgs {
     QueryArgs q;
     uint32_t tc, slots, max_peaks, grid;
     uint32_t group;       // probes per run-table group
+    uint32_t wdiv_m, wdiv_s;   // magic divider for 2 * dF + 1 (probes per peak)
     uint32_t hl_cap;      // LDS hit-list entries (0: none); needs fewer than 65535 peaks
     SpecAcc *acc;
     const Item *items;

@@ -368,7 +368,7 @@ __device__ __forceinline__ uint2 setup_runs(const ChunkArgs &a, const Smem &sm, 
         const uint32_t lp = h * kBT + threadIdx.x;   // any fixed order: P1 and P2 share it
         if (lp < n) {
             const uint32_t p = g0 + lp;
-            const uint32_t k = p / W, j = p - k * W;
+            const uint32_t k = udiv(p, a.wdiv_m, a.wdiv_s), j = p - k * W;
             pk[h] = k;
             const uint32_t q = sm.pk_moz[k];
             if (q > a.q.dF && q < a.q.NB - 1 - a.q.dF) {
@@ -432,7 +432,7 @@ __device__ __forceinline__ uint32_t run_at(const Smem &sm, uint32_t nc, uint32_t
 // takes a contiguous share of its 64-entry tiles, kTileU tiles per step, lane
 // j of a tile the entry F + j.  The entry's run is the number of runs that
 // end before it: the count before the tile (C, carried from tile to tile)
-// plus the tile word's bits below j -- tile_run(t) = mbcnt(t.m, t.C), at
+// plus the tile word's bits below j -- t.C + mbcnt(t.m), at
 // most nc.  Per step, load(t, x, ok, v, fs) fills v[] for the kTileU tiles
 // (ok[u]: x[u] is in the wavefront's share; fs: the step's first flat index)
 // and body(t, v, x, ok, fs) consumes them, so that each phase's memory
@@ -441,9 +441,12 @@ struct Tile {
     unsigned long long m;   // run ends in the tile, uniform
     uint32_t C;             // runs ending before the tile, uniform
 };
-__device__ __forceinline__ uint32_t tile_run(const Tile &t) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(t.m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)t.m, t.C));
+__device__ __forceinline__ uint32_t tile_below(const Tile &t) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(t.m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)t.m, 0u));
 }
+// arr[run of this lane's entry]: the tile's base run is uniform, so the lane
+// part is a scaled add to a scalar base
+__device__ __forceinline__ uint32_t tile_at(const lu32 *arr, const Tile &t) { return (arr + t.C)[tile_below(t)]; }
 template <typename Load, typename Body>
 __device__ __forceinline__ void walk_tiles(const Smem &sm, uint32_t nc, uint32_t s0, uint32_t se, Load &&load,
                                            Body &&body) {
@@ -454,27 +457,33 @@ __device__ __forceinline__ void walk_tiles(const Smem &sm, uint32_t nc, uint32_t
     if (k0 >= k1) return;
     const uint32_t xe = min(se, s0 + 64 * k1);   // end of this wavefront's share
     uint32_t C = ufl(run_at(sm, nc, s0 + 64 * k0));
-    for (uint32_t k = k0; k < k1; k += kTileU) {
+    // a step whose kTileU tiles lie wholly in the share (all but the last)
+    // runs with ok known true
+    auto step = [&](uint32_t k, auto whole_tag) {
+        constexpr bool WHOLE = decltype(whole_tag)::value;
         Tile t[kTileU];
         uint32_t x[kTileU], v[kTileU];
         bool ok[kTileU];
         unsigned long long mv[kTileU];
 #pragma unroll
-        for (uint32_t u = 0; u < kTileU; u++) mv[u] = sm.seg64[min(k + u, kSegWords - 1)];
+        for (uint32_t u = 0; u < kTileU; u++) mv[u] = sm.seg64[WHOLE ? k + u : min(k + u, kSegWords - 1)];
         const uint32_t x0 = s0 + 64 * k + lane;
 #pragma unroll
         for (uint32_t u = 0; u < kTileU; u++) {
             unsigned long long m = ((unsigned long long)ufl((uint32_t)(mv[u] >> 32)) << 32) | ufl((uint32_t)mv[u]);
-            m = k + u < k1 ? m : 0ull;
+            if (!WHOLE) m = k + u < k1 ? m : 0ull;
             t[u].m = m;
             t[u].C = C;
             C += (uint32_t)__popcll(m);
             x[u] = x0 + 64 * u;
-            ok[u] = x[u] < xe;
+            ok[u] = WHOLE ? true : x[u] < xe;
         }
         load(t, x, ok, v, s0 + 64 * k);
         body(t, v, x, ok, s0 + 64 * k);
-    }
+    };
+    uint32_t k = k0;
+    for (; k + kTileU <= k1 && s0 + 64 * (k + kTileU) <= xe; k += kTileU) step(k, std::true_type{});
+    for (; k < k1; k += kTileU) step(k, std::false_type{});
 }
 
 __device__ __forceinline__ LenDesc load_len(const ChunkArgs &a, int ixx) { return a.q.lens[ixx]; }
@@ -625,7 +634,7 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
                         uint32_t (&v)[kTileU], uint32_t) {
                         uint32_t gi[kTileU];
 #pragma unroll
-                        for (uint32_t u = 0; u < kTileU; u++) gi[u] = sm.cab[tile_run(t[u])] + x[u];
+                        for (uint32_t u = 0; u < kTileU; u++) gi[u] = tile_at(sm.cab, t[u]) + x[u];
                         // lanes outside the share read some entry or, past
                         // the array, 0 (buffer range check); they add 0
 #pragma unroll
@@ -738,7 +747,7 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
                         }
                         uint32_t gi[kTileU];
 #pragma unroll
-                        for (uint32_t u = 0; u < kTileU; u++) gi[u] = sm.cab[tile_run(t[u])] + x[u];
+                        for (uint32_t u = 0; u < kTileU; u++) gi[u] = tile_at(sm.cab, t[u]) + x[u];
 #pragma unroll
                         for (uint32_t u = 0; u < kTileU; u++) v[u] = iac_load(rs_iac, gi[u]);
                         if (fs < lim) {
@@ -760,7 +769,7 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
                             const uint32_t sl = sm.pre[local >> 5] + __popc(bits[u] & ((1u << bit) - 1u)) - r0;
                             if (sl >= ns) continue;
                             const uint32_t isY = v[u] & 1u;
-                            const uint32_t it = sm.cint[tile_run(t[u])];
+                            const uint32_t it = tile_at(sm.cint, t[u]);
                             lds_add(&sm.slot_cnt[sl], isY ? 0x10000u : 1u);
                             if (WIDE) lds_add64(&sm.slot_sum64[2 * sl + isY], (unsigned long long)it);
                             else lds_add(&sm.slot_sum32[2 * sl + isY], it);
