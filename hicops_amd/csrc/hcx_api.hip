@@ -573,9 +573,9 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
     ca.hl_cap = 0;
     {
         const size_t share = (size_t)c->max_lds / std::max<uint32_t>(1, c->blocks_per_cu);
-        // the kernels' static LDS (control words) is under 256 bytes; the
-        // replay kernel sizes its own hit list
-        const size_t reserve = 256;
+        // the kernels' LDS is all in the layout; the replay kernel sizes its
+        // own hit list
+        const size_t reserve = 0;
         // run-table group: an item walks its probes group by group, and every
         // extra group costs the exact-count pass another run setup, so by
         // default the group is as large as the workgroup's LDS share allows

@@ -225,6 +225,9 @@ struct Layout {
     uint32_t lgf;       // max_shdpeaks + 1 doubles
     uint32_t wlo, whi;  // nlen
     uint32_t hl;        // hl_cap u16: hit list of the current item
+    uint32_t ctl;       // 8 words: [0..1] control, [2..5] spectrum merge, [6] grabbed item
+    uint32_t red64;     // 2 u64: spectrum merge (entries, probes)
+    uint32_t heap;      // replay only: 5 x 256 words, the reference heap (hs, psid, ixx, bc, yc)
     uint32_t total;
 };
 
@@ -257,6 +260,9 @@ __host__ __device__ inline Layout make_layout(const ChunkArgs &a) {
     l.wlo = take(4u * a.q.nlen);
     l.whi = take(4u * a.q.nlen);
     l.hl = take(2u * a.hl_cap);
+    l.ctl = take(32u);
+    l.red64 = take(16u);
+    l.heap = take(a.replay ? 5u * 256u * 4u : 0u);
     l.total = off;
     return l;
 }
@@ -266,6 +272,8 @@ struct Smem {
     lu32 *seg32;
     lu64 *seg64;
     lu16 *slot_loc, *hl;
+    lu32 *ctl, *heap;
+    lu64 *red64;
     lu64 *slot_sum64;
     lu32 *slot_sum32;
     lf64 *lgf;
@@ -296,6 +304,9 @@ __device__ __forceinline__ Smem bind(unsigned char *raw, const Layout &l) {
     s.wlo = (li32 *)(b + l.wlo);
     s.whi = (li32 *)(b + l.whi);
     s.hl = (lu16 *)(b + l.hl);
+    s.ctl = (lu32 *)(b + l.ctl);
+    s.red64 = (lu64 *)(b + l.red64);
+    s.heap = (lu32 *)(b + l.heap);
     return s;
 }
 
@@ -957,13 +968,11 @@ __device__ __forceinline__ void init_smem(const ChunkArgs &a, const Smem &sm) {
 // Plain form: one workgroup per spectrum, its items in order.
 template <bool WIDE>
 __global__ __launch_bounds__(kBT, kWavesPerSimd) void k_spec_chunked(ChunkArgs a) {
+    // all LDS is dynamic, so the hit counters start at address 0
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    __shared__ uint32_t s_ctl_[2];
-    __shared__ uint32_t s_red_[4];
-    __shared__ unsigned long long s_red64_[2];
-    lu32 *s_ctl = (lu32 *)s_ctl_, *s_red = (lu32 *)s_red_;
-    lu64 *s_red64 = (lu64 *)s_red64_;
     const Smem sm = bind(smem_raw, make_layout(a));
+    lu32 *s_ctl = sm.ctl, *s_red = sm.ctl + 2;
+    lu64 *s_red64 = sm.red64;
     init_smem(a, sm);
     if (threadIdx.x == 0) {
         s_ctl[0] = 0;
@@ -995,13 +1004,9 @@ __global__ __launch_bounds__(kBT, kWavesPerSimd) void k_spec_chunked(ChunkArgs a
 template <bool WIDE>
 __global__ __launch_bounds__(kBT, kWavesPerSimd) void k_items_chunked(ChunkArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    __shared__ uint32_t s_ctl_[2];
-    __shared__ uint32_t s_item_;
-    __shared__ uint32_t s_red_[4];
-    __shared__ unsigned long long s_red64_[2];
-    lu32 *s_ctl = (lu32 *)s_ctl_, *s_red = (lu32 *)s_red_, *s_item = (lu32 *)&s_item_;
-    lu64 *s_red64 = (lu64 *)s_red64_;
     const Smem sm = bind(smem_raw, make_layout(a));
+    lu32 *s_ctl = sm.ctl, *s_red = sm.ctl + 2, *s_item = sm.ctl + 6;
+    lu64 *s_red64 = sm.red64;
     init_smem(a, sm);
     if (threadIdx.x == 0) {
         s_ctl[0] = 0;
@@ -1047,12 +1052,9 @@ __global__ __launch_bounds__(kBT, kWavesPerSimd) void k_items_chunked(ChunkArgs 
 template <bool WIDE>
 __global__ __launch_bounds__(kBT) void k_replay_chunked(ChunkArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    __shared__ uint32_t s_ctl_[2];
-    __shared__ uint32_t s_heap_[5][256];
-    lu32 *s_ctl = (lu32 *)s_ctl_;
-    const Heap heap = {(lu32 *)s_heap_[0], (lu32 *)s_heap_[1], (lu32 *)s_heap_[2], (lu32 *)s_heap_[3],
-                       (lu32 *)s_heap_[4]};
     const Smem sm = bind(smem_raw, make_layout(a));
+    lu32 *s_ctl = sm.ctl;
+    const Heap heap = {sm.heap, sm.heap + 256, sm.heap + 512, sm.heap + 768, sm.heap + 1024};
     init_smem(a, sm);
     if (threadIdx.x == 0) {
         s_ctl[0] = 0;
@@ -1317,12 +1319,12 @@ int launch_replay_chunked(hcx_ctx *ctx, ChunkArgs &a) {
     if (a.q.nspec <= 0) return HCX_SUCCESS;
     ChunkArgs r = a;
     r.replay = 1;   // the replay layout adds the slots' hyperscores
-    // its hit list takes what is left of a workgroup's share beside the
-    // static heap (5 x 256 words) and control words
+    // its hit list takes what is left of a workgroup's share (the layout
+    // holds the heap)
     r.hl_cap = 0;
     const size_t base = chunk_smem_bytes(r);
     const size_t share = (size_t)ctx->max_lds / std::max<uint32_t>(1, ctx->blocks_per_cu);
-    const size_t fixed = 5 * 256 * 4 + 256;
+    const size_t fixed = 0;
     if (share > base + fixed + 4096)
         r.hl_cap = std::min<uint32_t>(a.hl_cap, (uint32_t)(((share - base - fixed) / 2) & ~(size_t)63));
     if (chunk_smem_bytes(r) + fixed > (size_t)ctx->max_lds)
