@@ -810,7 +810,7 @@ extern "C" int hcx_autotune(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qba
         return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_autotune: tunes the batched or plain form");
     const uint32_t slots0 = c->slots, bpc0 = c->blocks_per_cu;
     const int32_t order0 = c->item_order;
-    const uint32_t slot_c[] = {512, 1024, 2048};
+    const uint32_t slot_c[] = {512, 1024, 1536, 2048};
     const uint32_t bpc_c[] = {1, 2, 3};
     const int norder = kernel == HCX_KERNEL_BATCHED ? 2 : 1;
     hcx_search_opts o = {};

@@ -207,7 +207,7 @@ int  hcx_search(hcx_ctx *ctx, hcx_index *const *idx, int nidx, hcx_qbatch *qb,
 
 /* Autotuner (no reference counterpart): times the scan kernel of `kernel`
  * (HCX_KERNEL_BATCHED or _PLAIN) on this batch for each candidate -- slots
- * 512/1024/2048, workgroups per CU 1/2/3 and, for the batched form, both item
+ * 512/1024/1536/2048, workgroups per CU 1/2/3 and, for the batched form, both item
  * orders -- keeps the fastest in the context and returns it.  The chunk size
  * is the indices' own (they are built for it).  Runs 2 searches per candidate. */
 typedef struct hcx_tuning {
