@@ -310,15 +310,26 @@ __device__ __forceinline__ Smem bind(unsigned char *raw, const Layout &l) {
     return s;
 }
 
+// Inclusive scan over the 64 lanes of a wavefront with DPP moves (no LDS
+// round trips): shifts by 1, 2, 4, 8 within each 16-lane row (lanes shifted
+// in from outside the row read 0), then row 0's last lane is added to row 1
+// and row 2's to row 3 (row_bcast:15), and lane 31's to rows 2 and 3
+// (row_bcast:31).
+__device__ __forceinline__ uint32_t wave_scan(uint32_t x) {
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, true);   // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, true);   // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, true);   // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, true);   // row_shr:8
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);  // row_bcast:31
+    return x;
+}
+
 // Block-wide exclusive scan where thread t holds tsum, the sum of its own
 // contiguous group; returns t's prefix.  Uses sm.wsum; one barrier.
 __device__ __forceinline__ uint32_t block_prefix(const Smem &sm, uint32_t tsum) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint32_t x = tsum;
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
-    }
+    const uint32_t x = wave_scan(tsum);
     if (lane == 63) sm.wsum[w] = x;
     __syncthreads();
     uint32_t run = x - tsum;
@@ -331,14 +342,7 @@ __device__ __forceinline__ uint32_t block_prefix(const Smem &sm, uint32_t tsum) 
 __device__ __forceinline__ uint2 block_prefix2(const Smem &sm, uint32_t a0, uint32_t a1, uint2 *tot) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     constexpr int NW = kBT / 64;
-    uint32_t x = a0, y = a1;
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t u = __shfl_up(x, off, 64), v = __shfl_up(y, off, 64);
-        if (lane >= off) {
-            x += u;
-            y += v;
-        }
-    }
+    const uint32_t x = wave_scan(a0), y = wave_scan(a1);
     if (lane == 63) {
         sm.wsum[w] = x;
         sm.wsum[NW + w] = y;
