@@ -471,7 +471,16 @@ __device__ __forceinline__ void walk_tiles(const Smem &sm, uint32_t nc, uint32_t
     const uint32_t k0 = ufl(T * w / NW), k1 = ufl(T * (w + 1) / NW);
     if (k0 >= k1) return;
     const uint32_t xe = min(se, s0 + 64 * k1);   // end of this wavefront's share
-    uint32_t C = ufl(run_at(sm, nc, s0 + 64 * k0));
+    // runs ending before the share: in segment 0, the bitmap words before it
+    // (popcounts summed across the wavefront), else a binary search
+    uint32_t C;
+    if (s0 == 0) {
+        uint32_t cnt = 0;
+        for (uint32_t i = lane; i < k0; i += 64) cnt += (uint32_t)__popcll(sm.seg64[i]);
+        C = ufl((uint32_t)__builtin_amdgcn_readlane((int)wave_scan(cnt), 63));
+    } else {
+        C = ufl(run_at(sm, nc, s0 + 64 * k0));
+    }
     // a step whose kTileU tiles lie wholly in the share (all but the last)
     // runs with ok known true
     auto step = [&](uint32_t k, auto whole_tag) {
