@@ -44,7 +44,7 @@ struct LenDesc {
     uint32_t div_s;         // shift
     uint32_t nch;           // peptide chunks (of ctx->chunk peptides)
     uint32_t cbase;         // first (length, chunk) work bucket of this length
-    const uint32_t *bA2;    // NB * (nch + 1): start of chunk c inside bin b
+    const uint32_t *bA2;    // (nch + 1) * NB, chunk-major: bA2[c * NB + b] = start of chunk c inside bin b
     const uint16_t *iAc;    // n * speclen: (peptide mod chunk) << 1 | isY, iA order
 };
 

@@ -3,8 +3,9 @@
 // Index refinement: each length's peptides (sorted by mass) are cut into
 // chunks of TC peptides.  Inside a bin the reference iA is ascending in ion
 // id, hence in peptide id, so the entries of one (bin, chunk) pair are a
-// contiguous run; bA2[bin * (nch+1) + c] is where chunk c starts inside bin
-// `bin` (bA2[..+0] = bA[bin], bA2[..+nch] = bA[bin+1]).  iA itself is the
+// contiguous run; bA2[c * NB + bin] is where chunk c starts inside bin `bin`
+// (bA2[0 * NB + bin] = bA[bin], bA2[nch * NB + bin] = bA[bin+1]): chunk-major,
+// so one item's probes of neighbouring bins share cache lines.  iA itself is the
 // reference array, unchanged; the scan reads iAc, its 16-bit chunk-local
 // form (peptide mod TC) << 1 | isY, which needs neither a division nor the
 // ion offset at query time and halves the bytes read.
@@ -374,7 +375,6 @@ __device__ __forceinline__ uint2 setup_runs(const ChunkArgs &a, const Smem &sm, 
                                             uint32_t g0, uint32_t g1, bool count_probes, Acc &acc) {
     const uint32_t W = 2 * a.q.dF + 1;
     const uint32_t n = g1 - g0;
-    const uint32_t stride = L.nch + 1;
     uint32_t rs[kPPT], len[kPPT], pk[kPPT];
     uint32_t nr = 0, ne = 0;
 #pragma unroll
@@ -387,9 +387,9 @@ __device__ __forceinline__ uint2 setup_runs(const ChunkArgs &a, const Smem &sm, 
             pk[h] = k;
             const uint32_t q = sm.pk_moz[k];
             if (q > a.q.dF && q < a.q.NB - 1 - a.q.dF) {
-                const size_t o = (size_t)(q - a.q.dF + j) * stride + c;
+                const size_t o = (size_t)c * a.q.NB + (q - a.q.dF + j);
                 rs[h] = ldg(L.bA2, o);
-                len[h] = ldg(L.bA2, o + 1) - rs[h];
+                len[h] = ldg(L.bA2, o + a.q.NB) - rs[h];
                 if (count_probes) acc.nprobe++;
             }
         }
@@ -1202,7 +1202,7 @@ __global__ void k_chunk_offsets(const uint32_t *iA, const uint32_t *bA, uint32_t
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t stride = (uint64_t)nch + 1;
     if (t >= (uint64_t)NB * stride) return;
-    const uint32_t bin = (uint32_t)(t / stride), c = (uint32_t)(t % stride);
+    const uint32_t c = (uint32_t)(t / NB), bin = (uint32_t)(t % NB);   // chunk-major
     const uint32_t s0 = bA[bin], e0 = bA[bin + 1];
     uint32_t r;
     if (c == 0) {
