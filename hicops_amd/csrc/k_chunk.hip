@@ -75,7 +75,7 @@ constexpr uint32_t kPPT = kProbesPerThread;   // run-table probes per thread
 constexpr int kWavesPerSimd = HCX_WAVES_PER_SIMD;
 constexpr uint32_t kNoProbe = 0xFFFFFFFFu;
 #ifndef HCX_GRAB
-#define HCX_GRAB 32
+#define HCX_GRAB 64
 #endif
 // items taken per queue atomic, at most: a grab of consecutive items is
 // mostly one spectrum's, so fewer grabs mean fewer per-spectrum merges; near
