@@ -633,6 +633,14 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
         ca.items = (const Item *)qb->items.p;
         ca.nitems = qb->nitems;
         ca.items_w = nullptr;
+        // items listed by spectrum: the scan records each item's best score,
+        // which lets the tie replay skip items the full heap would reject
+        if (!ca.by_chunk) {
+            if (qb->item_max.ensure(4ull * qb->nitems + 16))
+                return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "search: item score allocation failed");
+            ca.item_max = (uint32_t *)qb->item_max.p;
+            ca.item_base = (const uint32_t *)qb->counts.p;
+        }
     }
     if (timed) HCX_HIP(hipEventRecord(c->ev2, c->stream));
     if (sur) {

@@ -116,7 +116,7 @@ struct hcx_qbatch {
     uint32_t max_int = 0;    // largest peak intensity of the batch
     hcx::DeviceBuf moz, inten, idx, prec, chg, rt;
     hcx::DeviceBuf res, hist, stats, feat, pred;
-    hcx::DeviceBuf acc, items, counts;   // chunked-search state
+    hcx::DeviceBuf acc, items, counts, item_max;   // chunked-search state
     uint32_t nitems = 0;
     uint64_t items_key = 0;              // index set + chunk the items were built for
 };
@@ -190,6 +190,8 @@ struct ChunkArgs {
     uint32_t by_chunk;    // items bucketed by (length, chunk) instead of spectrum
     uint32_t wide;        // 64-bit intensity sums in the slots
     uint32_t replay;      // layout of the in-order replay (slots keep their hyperscores)
+    uint32_t *item_max;         // batched, by spectrum: each item's best hyperscore (float bits), or null
+    const uint32_t *item_base;  // with item_max: first item of each spectrum
 };
 size_t chunk_smem_bytes(const ChunkArgs &a);
 int launch_chunk_offsets(hcx_ctx *ctx, const uint32_t *iA, const uint32_t *bA, uint32_t NB, uint32_t nch, uint32_t TC,

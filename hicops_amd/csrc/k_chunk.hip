@@ -227,6 +227,7 @@ struct Layout {
     uint32_t wlo, whi;  // nlen
     uint32_t hl;        // hl_cap u16: hit list of the current item
     uint32_t ctl;       // 8 words: [0..1] control, [2..5] spectrum merge, [6] grabbed item
+    uint32_t imax;      // kGrab words: best score of each item of the current grab
     uint32_t red64;     // 2 u64: spectrum merge (entries, probes)
     uint32_t heap;      // replay only: 5 x 256 words, the reference heap (hs, psid, ixx, bc, yc)
     uint32_t total;
@@ -262,6 +263,7 @@ __host__ __device__ inline Layout make_layout(const ChunkArgs &a) {
     l.whi = take(4u * a.q.nlen);
     l.hl = take(2u * a.hl_cap);
     l.ctl = take(32u);
+    l.imax = take(4u * kGrab);
     l.red64 = take(16u);
     l.heap = take(a.replay ? 5u * 256u * 4u : 0u);
     l.total = off;
@@ -273,7 +275,7 @@ struct Smem {
     lu32 *seg32;
     lu64 *seg64;
     lu16 *slot_loc, *hl;
-    lu32 *ctl, *heap;
+    lu32 *ctl, *heap, *imax;
     lu64 *red64;
     lu64 *slot_sum64;
     lu32 *slot_sum32;
@@ -306,6 +308,7 @@ __device__ __forceinline__ Smem bind(unsigned char *raw, const Layout &l) {
     s.whi = (li32 *)(b + l.whi);
     s.hl = (lu16 *)(b + l.hl);
     s.ctl = (lu32 *)(b + l.ctl);
+    s.imax = (lu32 *)(b + l.imax);
     s.red64 = (lu64 *)(b + l.red64);
     s.heap = (lu32 *)(b + l.heap);
     return s;
@@ -323,6 +326,18 @@ __device__ __forceinline__ uint32_t wave_scan(uint32_t x) {
     x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, true);   // row_shr:8
     x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);  // row_bcast:15
     x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);  // row_bcast:31
+    return x;
+}
+
+// Maximum over the 64 lanes of a wavefront, in lane 63 (the same DPP
+// pattern as wave_scan, with max for add).
+__device__ __forceinline__ uint32_t wave_max_last(uint32_t x) {
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, true));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, true));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, true));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, true));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false));
     return x;
 }
 
@@ -599,7 +614,7 @@ __device__ __forceinline__ Cell heap_getmax(const Heap &h, int size) {
 // batch's largest intensity times the most hits a peptide can take fits).
 template <bool ORDERED, bool WIDE>
 __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm, lu32 *s_ctl, uint32_t spec,
-                                             uint32_t P, int ixx,
+                                             lu32 *imax_out, uint32_t P, int ixx,
                                              uint32_t c, int32_t lo, int32_t hi, Acc &acc, const Heap *heap = nullptr,
                                              int *hsize = nullptr) {
     const LenDesc L = load_len_u(a, (uint32_t)ixx);
@@ -722,6 +737,7 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
         __syncthreads();
     }
     const uint32_t Qn = ufl((uint32_t)s_ctl[0]);
+    float imax = 0.0f;   // this thread's best score in the item (batched item list)
 
     for (uint32_t r0 = 0; r0 < Qn; r0 += a.slots) {
         const uint32_t ns = min(a.slots, Qn - r0);
@@ -839,6 +855,7 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
                 continue;
             }
             acc.cpsms++;
+            imax = fmaxf(imax, hs);
             int bin = (int)((double)(hs * 10.0f) + 0.5);
             if (bin >= (int)a.q.hbins) { bin = (int)a.q.hbins - 1; acc.flags |= HCX_RES_HIST_CLAMP; }
             if ((uint32_t)bin < kHistLds) lds_add(&sm.hist[bin], 1u);
@@ -851,6 +868,12 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
             } else if (hs == acc.best_hs) {
                 acc.best_cnt++;
             }
+        }
+        // the item's best score, for the replay (positive floats order as uints)
+        // (one LDS atomic per wavefront: same-address atomics serialise)
+        if (!ORDERED && imax_out) {
+            const uint32_t wm = wave_max_last(__float_as_uint(imax));
+            if ((threadIdx.x & 63u) == 63u && wm) lds_max(imax_out, wm);
         }
         __syncthreads();
         if (ORDERED) {
@@ -965,6 +988,7 @@ __device__ __forceinline__ void init_smem(const ChunkArgs &a, const Smem &sm) {
     for (uint32_t i = threadIdx.x; i <= a.q.max_shdpeaks; i += kBT) sm.lgf[i] = a.q.lgf[i];
     for (uint32_t i = threadIdx.x; i < a.tc / 4; i += kBT) sm.map[i] = 0;
     for (uint32_t i = threadIdx.x; i < a.tc / 32; i += kBT) sm.bmp[i] = 0;
+    if (threadIdx.x < 8) sm.ctl[threadIdx.x] = 0;
     for (uint32_t i = threadIdx.x; i < a.slots; i += kBT) {
         sm.slot_cnt[i] = 0;
         if (a.wide) {
@@ -1008,7 +1032,7 @@ __global__ __launch_bounds__(kBT, kWavesPerSimd) void k_spec_chunked(ChunkArgs a
         const int32_t lo = ufl((int32_t)sm.wlo[ixx]), hi = ufl((int32_t)sm.whi[ixx]);
         if (lo > hi) continue;
         for (uint32_t c = (uint32_t)lo / a.tc; c <= (uint32_t)hi / a.tc; c++)
-            process_item<false, WIDE>(a, sm, s_ctl, s, P, ixx, c, lo, hi, acc);
+            process_item<false, WIDE>(a, sm, s_ctl, s, nullptr, P, ixx, c, lo, hi, acc);
     }
     flush_spectrum(a, sm, s, acc, s_red, s_red64);
 }
@@ -1039,6 +1063,10 @@ __global__ __launch_bounds__(kBT, kWavesPerSimd) void k_items_chunked(ChunkArgs 
         const uint32_t it1 = min(a.nitems, it0 + grab);
         // guided: about a quarter of each workgroup's share of what is left
         grab = max(1u, min(kGrab, (a.nitems - it1) / (4u * gridDim.x)));
+        // the grab's items' best scores collect in LDS and go out together
+        // (a global store per item would make the next barrier wait for it)
+        const bool rec = a.item_max != nullptr;
+        if (rec && threadIdx.x < it1 - it0) sm.imax[threadIdx.x] = 0;
         for (uint32_t it = it0; it < it1; it++) {
             Item item = items[it];
             item.spec = ufl(item.spec);
@@ -1052,8 +1080,10 @@ __global__ __launch_bounds__(kBT, kWavesPerSimd) void k_items_chunked(ChunkArgs 
                 P = load_peaks(a, sm, cur);
                 __syncthreads();
             }
-            process_item<false, WIDE>(a, sm, s_ctl, cur, P, item.ixx, item.chunk, item.lo, item.hi, acc);
+            process_item<false, WIDE>(a, sm, s_ctl, cur, rec ? sm.imax + (it - it0) : nullptr, P, item.ixx,
+                                      item.chunk, item.lo, item.hi, acc);
         }
+        if (rec && threadIdx.x < it1 - it0) ((AS_GLB uint32_t *)a.item_max)[it0 + threadIdx.x] = sm.imax[threadIdx.x];
     }
     if (cur != 0xFFFFFFFFu) flush_spectrum(a, sm, cur, acc, s_red, s_red64);
 }
@@ -1086,6 +1116,14 @@ __global__ __launch_bounds__(kBT) void k_replay_chunked(ChunkArgs a) {
             sm.wlo[l] = lo;
             sm.whi[l] = hi;
         }
+        // the heap's size and root as thread 0 last published them; the
+        // spectrum's items follow in the item list in this loop's order
+        const bool skip_ok = a.item_max != nullptr && a.item_base != nullptr;
+        uint32_t ir = skip_ok ? ufl(((const AS_GLB uint32_t *)a.item_base)[s]) : 0u;
+        if (threadIdx.x == 0) {
+            sm.ctl[6] = 0;
+            sm.ctl[7] = 0;
+        }
         __syncthreads();
         int hsize = 0;   // meaningful on lane 0 only
         Acc acc;
@@ -1093,8 +1131,24 @@ __global__ __launch_bounds__(kBT) void k_replay_chunked(ChunkArgs a) {
         for (int ixx = 0; ixx < a.q.nlen; ixx++) {
             const int32_t lo = ufl((int32_t)sm.wlo[ixx]), hi = ufl((int32_t)sm.whi[ixx]);
             if (lo > hi) continue;
-            for (uint32_t c = (uint32_t)lo / a.tc; c <= (uint32_t)hi / a.tc; c++)
-                process_item<true, WIDE>(a, sm, s_ctl, s, P, ixx, c, lo, hi, acc, &heap, &hsize);
+            for (uint32_t c = (uint32_t)lo / a.tc; c <= (uint32_t)hi / a.tc; c++) {
+                if (skip_ok) {
+                    // every candidate of an item whose best score is below the
+                    // full heap's root is rejected (minheap.h:111-114): skip it
+                    const uint32_t m = ufl(((const AS_GLB uint32_t *)a.item_max)[ir++]);
+                    const uint32_t hsz = ufl((uint32_t)sm.ctl[6]), rootb = ufl((uint32_t)sm.ctl[7]);
+                    if (m == 0u || (hsz == (uint32_t)a.q.heap_cap && __uint_as_float(m) < __uint_as_float(rootb)))
+                        continue;
+                }
+                process_item<true, WIDE>(a, sm, s_ctl, s, nullptr, P, ixx, c, lo, hi, acc, &heap, &hsize);
+                if (skip_ok) {
+                    if (threadIdx.x == 0) {
+                        sm.ctl[6] = (uint32_t)hsize;
+                        sm.ctl[7] = __float_as_uint(heap_hs(heap, 0));
+                    }
+                    __syncthreads();
+                }
+            }
         }
         if (threadIdx.x == 0 && hsize > 0) {
             const Cell m = heap_getmax(heap, hsize);
