@@ -329,16 +329,20 @@ __device__ __forceinline__ uint32_t wave_scan(uint32_t x) {
     return x;
 }
 
-// Maximum over the 64 lanes of a wavefront, in lane 63 (the same DPP
-// pattern as wave_scan, with max for add).
-__device__ __forceinline__ uint32_t wave_max_last(uint32_t x) {
-    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, true));
-    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, true));
-    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, true));
-    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, true));
-    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false));
-    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false));
+// Reduction over the 64 lanes of a wavefront, complete in lane 63 (the
+// DPP pattern of wave_scan with any associative op whose identity is 0).
+template <typename Op>
+__device__ __forceinline__ uint32_t wave_reduce_last(uint32_t x, Op op) {
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, true));
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, true));
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, true));
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, true));
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false));
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false));
     return x;
+}
+__device__ __forceinline__ uint32_t wave_max_last(uint32_t x) {
+    return wave_reduce_last(x, [](uint32_t u, uint32_t v) { return max(u, v); });
 }
 
 // Block-wide exclusive scan where thread t holds tsum, the sum of its own
@@ -926,11 +930,23 @@ __device__ __forceinline__ void flush_spectrum(const ChunkArgs &a, const Smem &s
         s_red64[1] = 0;
     }
     __syncthreads();
-    if (acc.best_cnt) lds_max(&s_red[0], __float_as_uint(acc.best_hs));
-    if (acc.cpsms) lds_add(&s_red[2], acc.cpsms);
-    if (acc.flags) lds_or(&s_red[3], acc.flags);
-    if (acc.nia) lds_add64(&s_red64[0], (unsigned long long)acc.nia);
-    if (acc.nprobe) lds_add64(&s_red64[1], (unsigned long long)acc.nprobe);
+    // wavefront reductions first: one LDS atomic per wavefront and field
+    // (same-address atomics serialise)
+    {
+        auto add = [](uint32_t u, uint32_t v) { return u + v; };
+        const uint32_t hm = wave_max_last(acc.best_cnt ? __float_as_uint(acc.best_hs) : 0u);
+        const uint32_t cp = wave_reduce_last(acc.cpsms, add);
+        const uint32_t fl = wave_reduce_last(acc.flags, [](uint32_t u, uint32_t v) { return u | v; });
+        const uint32_t ni = wave_reduce_last(acc.nia, add);
+        const uint32_t np = wave_reduce_last(acc.nprobe, add);
+        if ((threadIdx.x & 63u) == 63u) {
+            if (hm) lds_max(&s_red[0], hm);
+            if (cp) lds_add(&s_red[2], cp);
+            if (fl) lds_or(&s_red[3], fl);
+            if (ni) lds_add64(&s_red64[0], (unsigned long long)ni);
+            if (np) lds_add64(&s_red64[1], (unsigned long long)np);
+        }
+    }
     __syncthreads();
     SpecAcc *g = a.acc + s;
     const uint32_t mb = s_red[0];
