@@ -86,7 +86,7 @@ struct hcx_ctx {
     hcx::DeviceBuf scan_tmp;          // device scan scratch
     bool sur_ready = false;
     uint32_t chunk = 16384;           // peptides per query chunk (power of two)
-    uint32_t slots = 1024;            // scored-peptide slots per chunk pass
+    uint32_t slots = 1536;            // scored-peptide slots per chunk pass (measured best)
     uint32_t blocks_per_cu = 2;       // persistent grid of the batched kernel
     int32_t hitlist = -1;             // LDS hit-list entries: < 0 as many as fit
     int32_t item_order = 0;           // batched work items: 0 by spectrum, 1 by (length, chunk)

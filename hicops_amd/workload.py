@@ -8,6 +8,8 @@ Each preset is one of the benchmark configurations, written out with its sizes:
                  50,000 spectra (charge 2-4, 50% retention, top 150), +-500 Da
   open10m        10M peptides (7-40), charge 1-2 fragments, 200,000 spectra with
                  20-500 peaks (log-uniform) and windows of 1e2-1e6 candidates, +-500 Da
+  large40m       12.7M peptides (7-40) + Met-oxidation (<= 3 mods): ~40M forms, ~4.2G
+                 ions; 1,000,000 spectra as open10m, 0.01 Da fragments, +-500 Da
 
 ``scaled(name, npep=..., nspec=...)`` keeps a preset's shape at other sizes.
 """
@@ -59,6 +61,11 @@ PRESETS = {
     "open10m": Preset("open10m", 10_000_000, 7, 40, 2, 200000, 500.0, 2, 2, 4, 0.5, 100, 20, 500,
                       window_mode=1, wmin=1e2, wmax=1e6, seed=0x48C095 ^ 2,
                       max_hyperscore=200, max_shdpeaks=160),
+    # 12.7M base peptides with Met-oxidation (<= 3 per peptide): ~40M forms,
+    # ~4.2G fragment ions; 1M spectra at 0.01 Da fragment bins (dF = 1)
+    "large40m": Preset("large40m", 12_700_000, 7, 40, 2, 1_000_000, 500.0, 1, 2, 4, 0.5, 100, 20, 500,
+                       window_mode=1, wmin=1e2, wmax=4e6, mods=[("M", 15.9949, 3)], max_mods=3,
+                       seed=0x48C095 ^ 4, max_hyperscore=200, max_shdpeaks=160),
 }
 
 
