@@ -1,0 +1,54 @@
+"""One rank's share at full size (2M tryptic peptides with Met-oxidation <= 2,
+~5.6M forms, charge-1 fragments; 50,000 spectra, open search +-500 Da): the
+GPU index and both GPU query forms against the oracle, bit-exact counts,
+candidates' top hits, histograms and work counters."""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+from hicops_amd import native as N
+from hicops_amd import workload as W
+from hicops_amd.search import Engine
+from oracle import binding as O
+from tests.test_gpu_parity import compare
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def test_rank_share_full_size_bitexact():
+    p = W.PRESETS["rank_share"]
+    t0 = time.perf_counter()
+    db = W.Database(p)
+    q = W.make_queries(db, p)
+    prm = W.make_params(p)
+    log(f"test_rank_share_full_size_bitexact: {db.nforms} forms, {q.nspec} spectra "
+        f"({time.perf_counter() - t0:.1f}s)")
+    assert db.nforms > 5_000_000 and q.nspec == 50_000
+
+    t0 = time.perf_counter()
+    odb = O.OracleDB(prm, db)
+    log(f"  oracle index {time.perf_counter() - t0:.1f}s")
+    eng = Engine(prm)
+    idx = eng.construct_db(db)
+    for L, ix in zip(db.lengths, idx):
+        iA, bA = ix.export()
+        assert np.array_equal(iA, odb.iA[L]) and np.array_equal(bA, odb.bA[L]), L
+    log("  index: iA and bA bit-exact for every length")
+
+    t0 = time.perf_counter()
+    res_o, hist_o, st_o = odb.search(q)
+    log(f"  oracle search {time.perf_counter() - t0:.1f}s")
+    b = eng.upload(q)
+    for kernel in (N.KERNEL_BATCHED, N.KERNEL_PLAIN):
+        out = eng.query_spectrum(b, kernel=kernel)
+        n = compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"rank_share-full-{kernel}")
+        log(f"  kernel {kernel}: {n} spectra with e-values; counts, top hits, histograms, work match")
+        assert n > 40_000
+    eng.close()

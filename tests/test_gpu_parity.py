@@ -13,7 +13,13 @@ from oracle import binding as O
 pytestmark = pytest.mark.gpu
 
 CHECK_EXACT = ["cpsms", "psid", "idxoffset", "sharedions", "totalions", "bc", "yc", "maxhypscore",
-               "mu", "beta", "minhypscore", "nexthypscore"]
+               "minhypscore", "nexthypscore"]
+# the tail fit's slope and intercept are FP64 results truncated to ints (x 1e6,
+# expeRT.cpp:700-718): where the exact value sits within an ulp of an integer,
+# the GPU's log10 and the host libm's (the oracle's, like the reference's) can
+# round to neighbouring ints -- seen once in 50,000 spectra at full size; the
+# e-value then still agrees within 1e-3
+CHECK_INT_1 = ["mu", "beta"]
 
 
 def log(msg):
@@ -52,6 +58,12 @@ def compare(res_g, hist_g, st_g, res_o, hist_o, st_o, label):
         g, o = res_g[f][valid], res_o[f][valid]
         bad = np.nonzero(g != o)[0]
         assert len(bad) == 0, (label, f, bad[:5], g[bad[:5]], o[bad[:5]])
+    for f in CHECK_INT_1:
+        g, o = res_g[f][valid].astype(np.int64), res_o[f][valid].astype(np.int64)
+        d = np.abs(g - o)
+        assert d.max(initial=0) <= 1, (label, f, np.nonzero(d > 1)[0][:5])
+        if d.any():
+            log(f"  {label}: {f} differs by one unit in {int((d > 0).sum())} of {len(d)} spectra")
     hs_g, hs_o = res_g["hyperscore"][valid], res_o["hyperscore"][valid]
     assert np.all(np.abs(hs_g - hs_o) <= 1e-5 * np.abs(hs_o)), label
     e_g, e_o = res_g["e_x"][valid], res_o["e_x"][valid]
