@@ -52,3 +52,54 @@ def test_rank_share_full_size_bitexact():
         log(f"  kernel {kernel}: {n} spectra with e-values; counts, top hits, histograms, work match")
         assert n > 40_000
     eng.close()
+
+
+def test_small_config_full_size():
+    # 20,000 tryptic peptides (7-30), charge 1-2 fragments, 1,000 spectra, +-10 Da
+    p = W.PRESETS["small"]
+    db = W.Database(p)
+    q = W.make_queries(db, p)
+    prm = W.make_params(p)
+    log(f"test_small_config_full_size: {db.nforms} forms, {q.nspec} spectra")
+    odb = O.OracleDB(prm, db)
+    res_o, hist_o, st_o = odb.search(q)
+    eng = Engine(prm)
+    idx = eng.construct_db(db)
+    for L, ix in zip(db.lengths, idx):
+        iA, bA = ix.export()
+        assert np.array_equal(iA, odb.iA[L]) and np.array_equal(bA, odb.bA[L]), L
+    b = eng.upload(q)
+    for kernel in (N.KERNEL_BATCHED, N.KERNEL_PLAIN):
+        out = eng.query_spectrum(b, kernel=kernel)
+        n = compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"small-full-{kernel}")
+        log(f"  kernel {kernel}: {n} spectra with e-values")
+        assert n > 300   # narrow windows: about 430 of the 1,000 spectra have >= min_cpsm candidates
+    eng.close()
+
+
+def test_open10m_benchmark_workload_full_size():
+    # the benchmark's workload itself: 10M peptides (890M ions), all 200,000
+    # spectra, +-500 Da
+    p = W.PRESETS["open10m"]
+    db = W.Database(p)
+    q = W.make_queries(db, p)
+    prm = W.make_params(p)
+    log(f"test_open10m_benchmark_workload_full_size: {db.nforms} forms, {q.nspec} spectra")
+    t0 = time.perf_counter()
+    odb = O.OracleDB(prm, db)
+    log(f"  oracle index {time.perf_counter() - t0:.1f}s")
+    eng = Engine(prm)
+    idx = eng.construct_db(db)
+    for L, ix in zip(db.lengths, idx):
+        iA, bA = ix.export()
+        assert np.array_equal(iA, odb.iA[L]) and np.array_equal(bA, odb.bA[L]), L
+    t0 = time.perf_counter()
+    res_o, hist_o, st_o = odb.search(q)
+    log(f"  oracle search {time.perf_counter() - t0:.1f}s")
+    b = eng.upload(q)
+    for kernel in (N.KERNEL_BATCHED, N.KERNEL_PLAIN):
+        out = eng.query_spectrum(b, kernel=kernel)
+        n = compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"open10m-{kernel}")
+        log(f"  kernel {kernel}: {n} spectra with e-values")
+        assert n > 190_000
+    eng.close()
