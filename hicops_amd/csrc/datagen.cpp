@@ -101,6 +101,8 @@ struct hcx_specset {
     std::vector<uint32_t> moz, inten, idx, src_len;
     std::vector<float> prec, rt;
     std::vector<int32_t> chg, src_psid;
+    // raw peaks before pickpeaks (hcx_spec_create_raw only)
+    std::vector<uint32_t> raw_mz, raw_in, raw_idx;
 };
 
 namespace {
@@ -281,7 +283,7 @@ struct Peak { uint32_t mz; uint32_t in; };
 // Fragment m/z of one b or y ion: neutral residue sum (+ mods, + H2O for y),
 // plus z protons, over z.
 void spectrum_for(const hcx_db *db, const hcx_specgen_cfg *cfg, uint32_t L, int32_t psid, uint32_t s,
-                  std::vector<Peak> &peaks_out, uint32_t qalen) {
+                  std::vector<Peak> &peaks_out, uint32_t qalen, std::vector<Peak> *raw_out) {
     const LenTable *T = db->get(L);
     const hcx_pep_entry &e = T->entries[psid];
     const char *seq = &T->seqs[(size_t)e.seq_id * L];
@@ -330,6 +332,11 @@ void spectrum_for(const hcx_db *db, const hcx_specgen_cfg *cfg, uint32_t L, int3
         pk[i].mz = (uint32_t)(raw[i].first * cfg->scale);
         pk[i].in = (uint32_t)(raw[i].second * 1000);
     }
+    if (raw_out) {
+        // as an MS2 file lists them: ascending m/z
+        *raw_out = pk;
+        std::stable_sort(raw_out->begin(), raw_out->end(), [](const Peak &a, const Peak &b) { return a.mz < b.mz; });
+    }
     std::stable_sort(pk.begin(), pk.end(), [](const Peak &a, const Peak &b) { return a.in < b.in; });
     const int S = (int)pk.size();
     uint32_t speclen = 0;
@@ -348,7 +355,9 @@ void spectrum_for(const hcx_db *db, const hcx_specgen_cfg *cfg, uint32_t L, int3
 
 }  // namespace
 
-extern "C" int hcx_spec_create(const hcx_db *db, const hcx_specgen_cfg *cfg, hcx_specset **out) {
+namespace {
+
+int spec_create(const hcx_db *db, const hcx_specgen_cfg *cfg, bool keep_raw, hcx_specset **out) {
     if (!db || !cfg || !out) return HCX_ERR_INVLD_PTR;
     if (cfg->peaks_min == 0 || cfg->peaks_max < cfg->peaks_min || cfg->frag_maxz == 0)
         return HCX_ERR_INVLD_PARAM;
@@ -382,7 +391,7 @@ extern "C" int hcx_spec_create(const hcx_db *db, const hcx_specgen_cfg *cfg, hcx
     hcx_specset *ss = new (std::nothrow) hcx_specset;
     if (!ss) return HCX_ERR_BAD_MEM_ALLOC;
     const uint32_t n = cfg->nspec;
-    std::vector<std::vector<Peak>> per(n);
+    std::vector<std::vector<Peak>> per(n), per_raw(keep_raw ? n : 0);
     ss->prec.resize(n);
     ss->chg.resize(n);
     ss->rt.resize(n);
@@ -420,7 +429,8 @@ extern "C" int hcx_spec_create(const hcx_db *db, const hcx_specgen_cfg *cfg, hcx
         ss->rt[s] = (float)s * 0.01f;
         ss->src_len[s] = forms[c].len;
         ss->src_psid[s] = forms[c].psid;
-        spectrum_for(db, cfg, forms[c].len, forms[c].psid, (uint32_t)s, per[s], qalen);
+        spectrum_for(db, cfg, forms[c].len, forms[c].psid, (uint32_t)s, per[s], qalen,
+                     keep_raw ? &per_raw[s] : nullptr);
     }
     ss->idx.resize(n + 1);
     ss->idx[0] = 0;
@@ -432,7 +442,51 @@ extern "C" int hcx_spec_create(const hcx_db *db, const hcx_specgen_cfg *cfg, hcx
             ss->moz[ss->idx[s] + k] = per[s][k].mz;
             ss->inten[ss->idx[s] + k] = per[s][k].in;
         }
+    if (keep_raw) {
+        ss->raw_idx.resize(n + 1);
+        ss->raw_idx[0] = 0;
+        for (uint32_t s = 0; s < n; s++) ss->raw_idx[s + 1] = ss->raw_idx[s] + (uint32_t)per_raw[s].size();
+        ss->raw_mz.resize(ss->raw_idx[n]);
+        ss->raw_in.resize(ss->raw_idx[n]);
+        for (uint32_t s = 0; s < n; s++)
+            for (size_t k = 0; k < per_raw[s].size(); k++) {
+                ss->raw_mz[ss->raw_idx[s] + k] = per_raw[s][k].mz;
+                ss->raw_in[ss->raw_idx[s] + k] = per_raw[s][k].in;
+            }
+    }
     *out = ss;
+    return HCX_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" int hcx_spec_create(const hcx_db *db, const hcx_specgen_cfg *cfg, hcx_specset **out) {
+    return spec_create(db, cfg, false, out);
+}
+
+extern "C" int hcx_spec_create_raw(const hcx_db *db, const hcx_specgen_cfg *cfg, hcx_specset **out) {
+    return spec_create(db, cfg, true, out);
+}
+
+extern "C" int hcx_spec_raw_info(const hcx_specset *ss, uint64_t *npeaks) {
+    if (!ss) return HCX_ERR_INVLD_PTR;
+    if (npeaks) *npeaks = ss->raw_mz.size();
+    return ss->raw_idx.empty() ? HCX_ERR_INVLD_PARAM : HCX_SUCCESS;
+}
+
+extern "C" int hcx_spec_raw_copy(const hcx_specset *ss, uint32_t *mz, uint32_t *intensity, uint32_t *idx,
+                                 double *prec_mz, int32_t *z, double *rtime) {
+    if (!ss) return HCX_ERR_INVLD_PTR;
+    if (ss->raw_idx.empty()) return HCX_ERR_INVLD_PARAM;
+    const size_t n = ss->prec.size();
+    if (mz) std::memcpy(mz, ss->raw_mz.data(), ss->raw_mz.size() * 4);
+    if (intensity) std::memcpy(intensity, ss->raw_in.data(), ss->raw_in.size() * 4);
+    if (idx) std::memcpy(idx, ss->raw_idx.data(), ss->raw_idx.size() * 4);
+    for (size_t s = 0; s < n; s++) {
+        if (prec_mz) prec_mz[s] = (double)ss->prec[s];
+        if (z) z[s] = ss->chg[s];
+        if (rtime) rtime[s] = (double)ss->rt[s];
+    }
     return HCX_SUCCESS;
 }
 

@@ -1,0 +1,465 @@
+// k_pick.hip -- the Queries producer: MSQuery::pickpeaks (msquery.cpp:509-569)
+// over a batch of raw spectra, as MSQuery::extractbatch (msquery.cpp:238-298)
+// calls it once per spectrum, and the C ABI around it (include/hcx.h).
+//
+// One wavefront per spectrum.  The spectrum's raw (intensity, m/z) pairs are
+// staged in LDS (up to kPickLdsMax peaks; larger spectra in a device scratch
+// copy) and sorted by one lane with the reference's own Hoare quicksort
+// (keyval.h:48-132): the sort is unstable, and which of several equal
+// intensities survive the QALEN cut, and in what order they are stored, is the
+// order that partition scheme leaves.  Sub-ranges lying wholly below the
+// kept top-QALEN suffix are not sorted further: partitions only move elements
+// inside their own range, so the suffix ends exactly as in a full sort.  The
+// whole wavefront then scales, counts and writes the kept peaks, and computes
+// the batch facts the search needs (largest kept count, largest intensity,
+// most peaks of one spectrum within 2*dF+1 bins).
+//
+// Device data per pass: staging rows at ub[s] = sum over t < s of
+// min(S_t, QALEN); a one-workgroup scan of the kept counts gives Queries.idx;
+// a copy kernel packs the rows into the dense CSR arrays of the batch.
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "hcx_internal.h"
+
+struct hcx_rawbatch {
+    hcx_ctx *ctx = nullptr;
+    int32_t nspec = 0, file_num = 0;
+    uint64_t npeaks = 0;
+    uint32_t max_len = 0;            // most raw peaks of one spectrum
+    uint32_t lds_cap = 0;            // spectra up to this many peaks sort in LDS
+    bool has_rt = false;
+    std::vector<uint32_t> h_idx;     // host copy of the row offsets
+    hcx::DeviceBuf mz, in, idx, prec, z, rt;
+    hcx::DeviceBuf big_off, scr;     // scratch rows of spectra above lds_cap
+    hcx::DeviceBuf ub, cnt, st, summary;   // per-pass staging
+};
+
+namespace hcx {
+namespace {
+
+constexpr uint32_t kPickLdsMax = 1024;   // raw peaks of one spectrum staged in LDS
+constexpr uint32_t kQalenMax = 4096;
+constexpr int kStackMax = 40;            // > log2(2^32): ranges pushed by the sort
+
+struct PickArgs {
+    const uint32_t *raw_mz, *raw_in, *raw_idx;
+    const double *prec_mz;
+    const int32_t *z;
+    const double *rtime;         // may be null: 0
+    const uint32_t *big_off;     // scratch row of a spectrum above lds_cap
+    uint32_t *scr_key, *scr_val;
+    const uint32_t *ub;          // staging row of each spectrum
+    uint32_t *st_moz, *st_in, *cnt;
+    float *prec;
+    int32_t *chg;
+    float *rt;
+    uint32_t *summary;           // [0] most kept peaks, [1] largest intensity, [2] largest multiplicity
+    int32_t n;
+    uint32_t qalen, base_int, min_int, span, lds_cap, p2;
+};
+
+// keyval.h:48-105 on key[0, n), one lane.  Both halves of every partition are
+// sorted (keyval.h:82-105), the larger pushed, so the stack stays below
+// log2(n) + 1 ranges; ranges ending below `cut` are dropped.
+__device__ void keyval_sort_lane(uint32_t *key, uint32_t *val, int n, int cut, int *stack) {
+    if (n < 2) return;
+    int sp = 0, l = 0, r = n - 1;
+    for (;;) {
+        // keyval.h:50-80: pivot key[(left+right)/2]; the scans stop at the
+        // pivot or at a swapped element, never outside [l, r] (the bounds in
+        // the loop conditions only make that explicit)
+        const uint32_t p = key[(l + r) >> 1];
+        int i = l, j = r;
+        while (i <= j) {
+            while (i < r && key[i] < p) i++;
+            while (j > l && key[j] > p) j--;
+            if (i <= j) {
+                const uint32_t tk = key[i], tv = val[i];
+                key[i] = key[j];
+                val[i] = val[j];
+                key[j] = tk;
+                val[j] = tv;
+                i++;
+                j--;
+            }
+        }
+        const bool go_l = l < j && j >= cut;
+        const bool go_r = i < r;
+        if (go_l && go_r) {
+            if (j - l > r - i) {
+                stack[2 * sp] = l;
+                stack[2 * sp + 1] = j;
+                l = i;
+            } else {
+                stack[2 * sp] = i;
+                stack[2 * sp + 1] = r;
+                r = j;
+            }
+            sp++;
+        } else if (go_l) {
+            r = j;
+        } else if (go_r) {
+            l = i;
+        } else {
+            if (sp == 0) break;
+            sp--;
+            l = stack[2 * sp];
+            r = stack[2 * sp + 1];
+        }
+    }
+}
+
+// msquery.cpp:537: dIntArr[j] *= factor, a uint times a double truncated back
+// to uint.  0 * inf (a spectrum whose top intensity is 0) is undefined in C;
+// x86-64 converts it to 0, and so does this.
+__device__ __forceinline__ uint32_t scale_int(uint32_t k, double factor) {
+    const double v = (double)k * factor;
+    return v == v ? (uint32_t)v : 0u;
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+    for (int d = 32; d > 0; d >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, d, 64));
+    return v;
+}
+
+__global__ __launch_bounds__(64) void k_pickpeaks(PickArgs a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    uint32_t *lds = reinterpret_cast<uint32_t *>(smem_raw);
+    const int s = blockIdx.x;
+    const int lane = threadIdx.x;
+    const uint32_t b = a.raw_idx[s];
+    const int S = (int)(a.raw_idx[s + 1] - b);
+    uint32_t *tmp = lds + 2 * a.lds_cap;
+    int *stack = reinterpret_cast<int *>(tmp + a.p2);
+    uint32_t *key, *val;
+    if ((uint32_t)S <= a.lds_cap) {
+        key = lds;
+        val = lds + a.lds_cap;
+    } else {
+        key = a.scr_key + a.big_off[s];
+        val = a.scr_val + a.big_off[s];
+    }
+    for (int t = lane; t < S; t += 64) {
+        key[t] = a.raw_in[b + t];
+        val[t] = a.raw_mz[b + t];
+    }
+    __syncthreads();
+    // msquery.cpp:519, only the part the kept suffix depends on
+    const int cut = S > (int)a.qalen ? S - (int)a.qalen : 0;
+    if (lane == 0) keyval_sort_lane(key, val, S, cut, stack);
+    __syncthreads();
+
+    // msquery.cpp:524-545: factor, top -> base_int, count of scaled >= min_int
+    uint32_t speclen = 0;
+    double factor = 0.0;
+    if (S > 0) {
+        factor = (double)a.base_int / (double)key[S - 1];
+        uint32_t c = 0;
+        for (int j0 = cut; j0 <= S - 2; j0 += 64) {
+            const int j = j0 + lane;
+            const bool f = j <= S - 2 && scale_int(key[j], factor) >= a.min_int;
+            c += (uint32_t)__popcll(__ballot(f));
+        }
+        speclen = 1 + c;
+    }
+    // msquery.cpp:548-562: the last min(speclen, QALEN) peaks, ascending intensity
+    const int keep = (int)min(speclen, a.qalen);
+    const uint32_t o = a.ub[s];
+    uint32_t imax = 0;
+    for (int t = lane; t < keep; t += 64) {
+        const int pos = S - keep + t;
+        const uint32_t in = pos == S - 1 ? a.base_int : scale_int(key[pos], factor);
+        const uint32_t mz = val[pos];
+        a.st_in[o + t] = in;
+        a.st_moz[o + t] = mz;
+        tmp[t] = mz;
+        imax = max(imax, in);
+    }
+    // most kept peaks within span+1 consecutive bins: bitonic sort of the kept
+    // bins (padded to a power of two), then an upper bound per peak
+    int p = 1;
+    while (p < keep) p <<= 1;
+    for (int t = keep + lane; t < p; t += 64) tmp[t] = 0xFFFFFFFFu;
+    for (int k = 2; k <= p; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            __syncthreads();
+            for (int t = lane; t < p; t += 64) {
+                const int u = t ^ j;
+                if (u > t) {
+                    const uint32_t x = tmp[t], y = tmp[u];
+                    const bool up = (t & k) == 0;
+                    if ((x > y) == up) {
+                        tmp[t] = y;
+                        tmp[u] = x;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    uint32_t mult = 0;
+    for (int t = lane; t < keep; t += 64) {
+        const uint64_t lim = (uint64_t)tmp[t] + a.span;
+        int lo = t + 1, hi = keep;
+        while (lo < hi) {
+            const int m = (lo + hi) >> 1;
+            if ((uint64_t)tmp[m] <= lim) lo = m + 1;
+            else hi = m;
+        }
+        mult = max(mult, (uint32_t)(lo - t));
+    }
+    imax = wave_max_u32(imax);
+    mult = wave_max_u32(mult);
+    if (lane == 0) {
+        a.cnt[s] = (uint32_t)keep;
+        // msquery.cpp:515-517
+        a.prec[s] = (float)a.prec_mz[s];
+        const int32_t zz = a.z[s];
+        a.chg[s] = 1 > zz ? 1 : zz;
+        const double rt = a.rtime ? a.rtime[s] : 0.0;
+        a.rt[s] = (float)(0.0 > rt ? 0.0 : rt);
+        atomicMax(&a.summary[0], (uint32_t)keep);
+        atomicMax(&a.summary[1], imax);
+        atomicMax(&a.summary[2], mult);
+    }
+}
+
+// Queries.idx: exclusive prefix of the kept counts (idx[s+1] = idx[s] + cnt[s],
+// msquery.cpp:548-549), one workgroup.
+__global__ __launch_bounds__(1024) void k_pick_scan(const uint32_t *cnt, uint32_t *idx, int n) {
+    __shared__ uint32_t part[1024];
+    const int t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int b = min(n, t * per), e = min(n, b + per);
+    uint32_t sum = 0;
+    for (int s = b; s < e; s++) sum += cnt[s];
+    part[t] = sum;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const uint32_t v = t >= d ? part[t - d] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;
+    if (t == 0) idx[0] = 0;
+    for (int s = b; s < e; s++) {
+        run += cnt[s];
+        idx[s + 1] = run;
+    }
+}
+
+// staging rows -> dense CSR, one wavefront per spectrum
+__global__ __launch_bounds__(256) void k_pick_compact(const uint32_t *ub, const uint32_t *cnt, const uint32_t *idx,
+                                                      const uint32_t *st_moz, const uint32_t *st_in, uint32_t *moz,
+                                                      uint32_t *in, int n) {
+    const int s = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (s >= n) return;
+    const uint32_t o = ub[s], d = idx[s], c = cnt[s];
+    for (uint32_t t = lane; t < c; t += 64) {
+        moz[d + t] = st_moz[o + t];
+        in[d + t] = st_in[o + t];
+    }
+}
+
+}  // namespace
+}  // namespace hcx
+
+using hcx::set_error;
+using hcx::set_error_code;
+
+extern "C" int hcx_raw_upload(hcx_ctx *c, const hcx_raw_spectra_view *raw, hcx_rawbatch **out) {
+    if (!c || !raw || !out) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_raw_upload: null argument");
+    if (raw->num_specs < 0) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_raw_upload: negative count");
+    const int32_t n = raw->num_specs;
+    if (n > 0 && (!raw->idx || !raw->prec_mz || !raw->z))
+        return set_error_code(HCX_ERR_INVLD_PTR, "hcx_raw_upload: missing arrays");
+    if (n > 0 && raw->idx[0] != 0) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_raw_upload: idx[0] != 0");
+    uint32_t maxS = 0;
+    uint64_t nbig = 0;
+    std::vector<uint32_t> big(n > 0 ? n : 1, 0);
+    for (int32_t s = 0; s < n; s++) {
+        if (raw->idx[s + 1] < raw->idx[s]) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_raw_upload: idx not monotone");
+        const uint32_t S = raw->idx[s + 1] - raw->idx[s];
+        maxS = std::max(maxS, S);
+        if (S > hcx::kPickLdsMax) {
+            big[s] = (uint32_t)nbig;
+            nbig += S;
+        }
+    }
+    const uint64_t npk = n > 0 ? raw->idx[n] : 0;
+    if (npk && (!raw->mz || !raw->intensity)) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_raw_upload: missing peaks");
+    HCX_HIP(hipSetDevice(c->device));
+    hcx_rawbatch *b = new (std::nothrow) hcx_rawbatch;
+    if (!b) return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_raw_upload: out of memory");
+    b->ctx = c;
+    b->nspec = n;
+    b->file_num = raw->file_num;
+    b->npeaks = npk;
+    b->max_len = maxS;
+    b->lds_cap = std::min(hcx::kPickLdsMax, (maxS + 63) / 64 * 64);
+    b->has_rt = raw->rtime != nullptr;
+    b->h_idx.assign(raw->idx, raw->idx + (n > 0 ? n + 1 : 0));
+    if (b->mz.ensure(4 * npk + 16) || b->in.ensure(4 * npk + 16) || b->idx.ensure(4ull * (n + 1)) ||
+        b->prec.ensure(8ull * n + 16) || b->z.ensure(4ull * n + 16) || b->rt.ensure(8ull * n + 16) ||
+        b->big_off.ensure(4ull * n + 16) || b->scr.ensure(8 * nbig + 16)) {
+        delete b;
+        return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_raw_upload: device allocation failed");
+    }
+    auto up = [&](hcx::DeviceBuf &d, const void *h, size_t bytes) {
+        if (!bytes || !h) return hipSuccess;
+        return hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, c->stream);
+    };
+    if (up(b->mz, raw->mz, 4 * npk) || up(b->in, raw->intensity, 4 * npk) ||
+        up(b->idx, raw->idx, 4ull * (n > 0 ? n + 1 : 0)) || up(b->prec, raw->prec_mz, 8ull * n) ||
+        up(b->z, raw->z, 4ull * n) || up(b->rt, raw->rtime, 8ull * n) || up(b->big_off, big.data(), 4ull * n) ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+        delete b;
+        return set_error("hcx_raw_upload: copy failed");
+    }
+    *out = b;
+    return HCX_SUCCESS;
+}
+
+extern "C" int hcx_raw_free(hcx_rawbatch *rb) {
+    if (!rb) return HCX_SUCCESS;
+    (void)hipSetDevice(rb->ctx->device);
+    (void)hipStreamSynchronize(rb->ctx->stream);
+    delete rb;
+    return HCX_SUCCESS;
+}
+
+extern "C" int hcx_queries_pick(hcx_ctx *c, hcx_rawbatch *rb, const hcx_pick_params *pp, hcx_qbatch **out,
+                                float *kernel_ms, float *total_ms) {
+    if (!c || !rb || !pp || !out) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_queries_pick: null argument");
+    if (rb->ctx != c) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_queries_pick: raw batch of another context");
+    if (pp->qalen == 0 || pp->qalen > hcx::kQalenMax)
+        return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_queries_pick: qalen must be 1..4096");
+    const int32_t n = rb->nspec;
+    // staging rows: min(S, qalen) per spectrum
+    std::vector<uint32_t> ub(n > 0 ? n : 1, 0);
+    uint64_t tot = 0;
+    for (int32_t s = 0; s < n; s++) {
+        ub[s] = (uint32_t)tot;
+        tot += std::min<uint64_t>(rb->h_idx[s + 1] - rb->h_idx[s], pp->qalen);
+    }
+    if (tot > 0xFFFFFFFFull) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_queries_pick: more than 2^32 kept peaks");
+    HCX_HIP(hipSetDevice(c->device));
+    hcx_qbatch *b = new (std::nothrow) hcx_qbatch;
+    if (!b) return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_queries_pick: out of memory");
+    b->ctx = c;
+    b->nspec = n;
+    b->file_num = rb->file_num;
+    if (b->moz.ensure(4 * tot + 16) || b->inten.ensure(4 * tot + 16) || b->idx.ensure(4ull * (n + 1)) ||
+        b->prec.ensure(4ull * n + 16) || b->chg.ensure(4ull * n + 16) || b->rt.ensure(4ull * n + 16) ||
+        rb->ub.ensure(4ull * n + 16) || rb->cnt.ensure(4ull * n + 16) || rb->st.ensure(8 * tot + 16) ||
+        rb->summary.ensure(16)) {
+        delete b;
+        return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_queries_pick: device allocation failed");
+    }
+    hipStream_t st = c->stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
+    auto fail = [&](const char *msg) {
+        for (hipEvent_t e : {e0, e1, e2, e3})
+            if (e) (void)hipEventDestroy(e);
+        delete b;
+        return set_error(msg);
+    };
+    if (hipEventCreate(&e0) || hipEventCreate(&e1) || hipEventCreate(&e2) || hipEventCreate(&e3))
+        return fail("hcx_queries_pick: event creation failed");
+    uint32_t summary[4] = {0, 0, 0, 0};
+    if (hipMemcpyAsync(rb->ub.p, ub.data(), 4ull * n, hipMemcpyHostToDevice, st) ||
+        hipMemsetAsync(rb->summary.p, 0, 16, st) || hipEventRecord(e2, st))
+        return fail("hcx_queries_pick: set-up failed");
+    uint32_t p2 = 1;
+    while (p2 < pp->qalen) p2 <<= 1;
+    hcx::PickArgs a{};
+    a.raw_mz = (const uint32_t *)rb->mz.p;
+    a.raw_in = (const uint32_t *)rb->in.p;
+    a.raw_idx = (const uint32_t *)rb->idx.p;
+    a.prec_mz = (const double *)rb->prec.p;
+    a.z = (const int32_t *)rb->z.p;
+    a.rtime = rb->has_rt ? (const double *)rb->rt.p : nullptr;
+    a.big_off = (const uint32_t *)rb->big_off.p;
+    a.scr_key = (uint32_t *)rb->scr.p;
+    a.scr_val = (uint32_t *)rb->scr.p + rb->scr.bytes / 8;
+    a.ub = (const uint32_t *)rb->ub.p;
+    a.st_moz = (uint32_t *)rb->st.p;
+    a.st_in = (uint32_t *)rb->st.p + tot;
+    a.cnt = (uint32_t *)rb->cnt.p;
+    a.prec = (float *)b->prec.p;
+    a.chg = (int32_t *)b->chg.p;
+    a.rt = (float *)b->rt.p;
+    a.summary = (uint32_t *)rb->summary.p;
+    a.n = n;
+    a.qalen = pp->qalen;
+    a.base_int = pp->base_int;
+    a.min_int = pp->min_int;
+    a.span = 2 * c->params.dF;
+    a.lds_cap = rb->lds_cap;
+    a.p2 = p2;
+    const size_t lds = (2ull * a.lds_cap + p2) * 4 + hcx::kStackMax * 8;
+    if (hipEventRecord(e0, st)) return fail("hcx_queries_pick: event record failed");
+    if (n > 0) hcx::k_pickpeaks<<<(unsigned)n, 64, lds, st>>>(a);
+    if (hipGetLastError() || hipEventRecord(e1, st)) return fail("hcx_queries_pick: normalisation kernel launch failed");
+    hcx::k_pick_scan<<<1, 1024, 0, st>>>((const uint32_t *)rb->cnt.p, (uint32_t *)b->idx.p, n);
+    if (n > 0)
+        hcx::k_pick_compact<<<(unsigned)((n + 3) / 4), 256, 0, st>>>(
+            (const uint32_t *)rb->ub.p, (const uint32_t *)rb->cnt.p, (const uint32_t *)b->idx.p, a.st_moz, a.st_in,
+            (uint32_t *)b->moz.p, (uint32_t *)b->inten.p, n);
+    if (hipGetLastError() || hipEventRecord(e3, st)) return fail("hcx_queries_pick: launch failed");
+    uint32_t npk = 0;
+    if (hipMemcpyAsync(summary, rb->summary.p, 12, hipMemcpyDeviceToHost, st) ||
+        hipMemcpyAsync(&npk, (const uint32_t *)b->idx.p + n, 4, hipMemcpyDeviceToHost, st) ||
+        hipStreamSynchronize(st))
+        return fail("hcx_queries_pick: device pass failed");
+    float km = 0, tm = 0;
+    if (hipEventElapsedTime(&km, e0, e1) || hipEventElapsedTime(&tm, e2, e3))
+        return fail("hcx_queries_pick: timing failed");
+    for (hipEvent_t e : {e0, e1, e2, e3}) (void)hipEventDestroy(e);
+    if (kernel_ms) *kernel_ms = km;
+    if (total_ms) *total_ms = tm;
+    b->npeaks = npk;
+    b->max_peaks = summary[0];
+    b->max_int = summary[1];
+    b->max_mult = summary[2];
+    *out = b;
+    return HCX_SUCCESS;
+}
+
+extern "C" int hcx_queries_pickpeaks(hcx_ctx *c, const hcx_raw_spectra_view *raw, const hcx_pick_params *pp,
+                                     hcx_qbatch **out) {
+    hcx_rawbatch *rb = nullptr;
+    int rc = hcx_raw_upload(c, raw, &rb);
+    if (rc != HCX_SUCCESS) return rc;
+    rc = hcx_queries_pick(c, rb, pp, out, nullptr, nullptr);
+    hcx_raw_free(rb);
+    return rc;
+}
+
+extern "C" int hcx_queries_info(const hcx_qbatch *qb, int32_t *num_specs, uint64_t *npeaks) {
+    if (!qb) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_queries_info: null batch");
+    if (num_specs) *num_specs = qb->nspec;
+    if (npeaks) *npeaks = qb->npeaks;
+    return HCX_SUCCESS;
+}
+
+extern "C" int hcx_queries_export(const hcx_qbatch *qb, uint32_t *moz, uint32_t *intensity, uint32_t *idx,
+                                  float *precurse, int32_t *charges, float *rtimes) {
+    if (!qb) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_queries_export: null batch");
+    hcx_ctx *c = qb->ctx;
+    HCX_HIP(hipSetDevice(c->device));
+    const uint64_t n = (uint64_t)qb->nspec, k = qb->npeaks;
+    auto down = [&](void *h, const hcx::DeviceBuf &d, size_t bytes) {
+        if (!h || !bytes) return hipSuccess;
+        return hipMemcpyAsync(h, d.p, bytes, hipMemcpyDeviceToHost, c->stream);
+    };
+    if (down(moz, qb->moz, 4 * k) || down(intensity, qb->inten, 4 * k) || down(idx, qb->idx, 4 * (n + 1)) ||
+        down(precurse, qb->prec, 4 * n) || down(charges, qb->chg, 4 * n) || down(rtimes, qb->rt, 4 * n) ||
+        hipStreamSynchronize(c->stream))
+        return set_error("hcx_queries_export: copy failed");
+    return HCX_SUCCESS;
+}

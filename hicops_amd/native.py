@@ -74,6 +74,19 @@ class QueriesView(C.Structure):
     ]
 
 
+class RawSpectraView(C.Structure):
+    _fields_ = [
+        ("mz", C.c_void_p), ("intensity", C.c_void_p), ("idx", C.c_void_p),
+        ("prec_mz", C.c_void_p), ("z", C.c_void_p), ("rtime", C.c_void_p),
+        ("num_specs", C.c_int32), ("file_num", C.c_int32),
+    ]
+
+
+class PickParams(C.Structure):
+    _fields_ = [("qalen", C.c_uint32), ("base_int", C.c_uint32), ("min_int", C.c_uint32),
+                ("pad_", C.c_uint32)]
+
+
 class SearchOpts(C.Structure):
     _fields_ = [
         ("kernel", C.c_int32), ("evalue", C.c_int32), ("tile", C.c_uint32),
@@ -169,7 +182,9 @@ def hcx() -> C.CDLL:
                      "hcx_selftest_divider", "hcx_set_tuning", "hcx_get_tuning", "hcx_set_option", "hcx_get_option", "hcx_autotune",
                      "hcx_surrogate_param_count", "hcx_surrogate_init", "hcx_surrogate_set",
                      "hcx_surrogate_get", "hcx_surrogate_train", "hcx_surrogate_gradient",
-                     "hcx_surrogate_last_audit", "hcx_selftest_mfma", "hcx_last_timing"]:
+                     "hcx_surrogate_last_audit", "hcx_selftest_mfma", "hcx_last_timing",
+                     "hcx_raw_upload", "hcx_raw_free", "hcx_queries_pick", "hcx_queries_pickpeaks",
+                     "hcx_queries_info", "hcx_queries_export"]:
             getattr(lib, name).restype = C.c_int
         _hcx = lib
     return _hcx
@@ -181,7 +196,8 @@ def host() -> C.CDLL:
         lib = _load(LIBHOST, "libhcx_host.so")
         for name in ["hcx_db_create", "hcx_db_free", "hcx_db_lengths", "hcx_db_length_info",
                      "hcx_db_length_copy", "hcx_db_length_view", "hcx_spec_create",
-                     "hcx_spec_free", "hcx_spec_info", "hcx_spec_copy"]:
+                     "hcx_spec_free", "hcx_spec_info", "hcx_spec_copy", "hcx_spec_create_raw",
+                     "hcx_spec_raw_info", "hcx_spec_raw_copy"]:
             getattr(lib, name).restype = C.c_int
         _host = lib
     return _host

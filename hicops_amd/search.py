@@ -2,6 +2,8 @@
 
   Engine(params)                  gParams + DSLIM_InitializeScorecard (dslim.cpp:390-437)
   Engine.construct(L, seqs, ents) DSLIM_Construct(Index*) for one peptide length (dslim.cpp:46-138)
+  Engine.pickpeaks(raw, ...)      MSQuery::pickpeaks (msquery.cpp:509-569) over a batch, the
+                                  Queries producer (device-resident result)
   Engine.query_spectrum(queries)  DSLIM_QuerySpectrum(Queries*, Index*, nIdx) (dslim_query.cpp:515-842)
                                   including expeRT::ModelTailFit and the e-value block (:771-818)
   Engine.model_tail_fit(...)      expeRT::ModelTailFit(Results*) (expeRT.cpp:572-718) on histograms
@@ -64,20 +66,58 @@ class Index:
 
 
 class Batch:
-    """A Queries batch resident in HBM."""
+    """A Queries batch resident in HBM: uploaded from the host, or produced on
+    the device by Engine.pickpeaks."""
 
-    def __init__(self, engine: "Engine", queries):
+    def __init__(self, engine: "Engine", queries, handle: C.c_void_p | None = None):
         self.engine = engine
         self.queries = queries
-        h = C.c_void_p()
-        v = queries.view()
-        N.check(N.hcx().hcx_queries_upload(engine._h, C.byref(v), C.byref(h)))
-        self._h = h
-        self.nspec = queries.nspec
+        if handle is None:
+            h = C.c_void_p()
+            v = queries.view()
+            N.check(N.hcx().hcx_queries_upload(engine._h, C.byref(v), C.byref(h)))
+            handle = h
+        self._h = handle
+        n = C.c_int32()
+        N.check(N.hcx().hcx_queries_info(handle, C.byref(n), None))
+        self.nspec = n.value
+
+    def export(self):
+        """Host copy of the batch as a workload.Queries (no source fields)."""
+        from .workload import Queries
+        n, k = C.c_int32(), C.c_uint64()
+        N.check(N.hcx().hcx_queries_info(self._h, C.byref(n), C.byref(k)))
+        n, k = n.value, k.value
+        q = Queries(np.zeros(k, np.uint32), np.zeros(k, np.uint32), np.zeros(n + 1, np.uint32),
+                    np.zeros(n, np.float32), np.zeros(n, np.int32), np.zeros(n, np.float32),
+                    np.zeros(n, np.uint32), np.zeros(n, np.int32))
+        N.check(N.hcx().hcx_queries_export(self._h, N.ptr(q.moz), N.ptr(q.intensity), N.ptr(q.idx),
+                                           N.ptr(q.precurse), N.ptr(q.charges), N.ptr(q.rtimes)))
+        return q
 
     def free(self) -> None:
         if getattr(self, "_h", None):
             N.hcx().hcx_queries_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.free()
+
+
+class RawBatch:
+    """Raw MS2 spectra (workload.RawSpectra) resident in HBM, before pickpeaks."""
+
+    def __init__(self, engine: "Engine", raw):
+        self.engine = engine
+        h = C.c_void_p()
+        v = raw.view()
+        N.check(N.hcx().hcx_raw_upload(engine._h, C.byref(v), C.byref(h)))
+        self._h = h
+        self.nspec = raw.nspec
+
+    def free(self) -> None:
+        if getattr(self, "_h", None):
+            N.hcx().hcx_raw_free(self._h)
             self._h = None
 
     def __del__(self):
@@ -131,6 +171,27 @@ class Engine:
 
     def upload(self, queries) -> Batch:
         return Batch(self, queries)
+
+    def upload_raw(self, raw) -> RawBatch:
+        return RawBatch(self, raw)
+
+    def pickpeaks(self, raw, qalen: int, base_int: int, min_int: int) -> Batch:
+        """MSQuery::pickpeaks (msquery.cpp:509-569) over a batch of raw spectra
+        (workload.RawSpectra or a RawBatch) on the device; the Queries batch
+        stays in HBM for query_spectrum."""
+        return self.pickpeaks_timed(raw, qalen, base_int, min_int)[0]
+
+    def pickpeaks_timed(self, raw, qalen: int, base_int: int, min_int: int) -> tuple[Batch, float, float]:
+        """pickpeaks returning (batch, normalisation kernel ms, whole pass ms)."""
+        rb = raw if isinstance(raw, RawBatch) else RawBatch(self, raw)
+        pp = N.PickParams()
+        pp.qalen, pp.base_int, pp.min_int = qalen, base_int, min_int
+        h = C.c_void_p()
+        km, tm = C.c_float(), C.c_float()
+        N.check(N.hcx().hcx_queries_pick(self._h, rb._h, C.byref(pp), C.byref(h), C.byref(km), C.byref(tm)))
+        if rb is not raw:
+            rb.free()
+        return Batch(self, None, handle=h), km.value, tm.value
 
     def _idx_array(self, indices):
         indices = self.indices if indices is None else indices

@@ -187,7 +187,7 @@ class Queries:
                        self.src_len[sel].copy(), self.src_psid[sel].copy())
 
 
-def make_queries(db: Database, p: Preset, nspec: int | None = None, seed: int | None = None) -> Queries:
+def _spec_cfg(p: Preset, nspec: int | None, seed: int | None) -> N.SpecGenCfg:
     cfg = N.SpecGenCfg()
     cfg.seed = p.seed ^ 0x5EC7 if seed is None else seed
     cfg.nspec = p.nspec if nspec is None else nspec
@@ -205,18 +205,74 @@ def make_queries(db: Database, p: Preset, nspec: int | None = None, seed: int | 
     cfg.noise_mu, cfg.noise_sigma = 0.0, 1.0
     cfg.dM = p.dM
     cfg.wmin, cfg.wmax = p.wmin, p.wmax
+    return cfg
+
+
+def _copy_queries(lib, h) -> Queries:
+    ns, npk = C.c_uint32(), C.c_uint64()
+    N.check(lib.hcx_spec_info(h, C.byref(ns), C.byref(npk)), lib)
+    n, k = ns.value, npk.value
+    q = Queries(np.zeros(k, np.uint32), np.zeros(k, np.uint32), np.zeros(n + 1, np.uint32),
+                np.zeros(n, np.float32), np.zeros(n, np.int32), np.zeros(n, np.float32),
+                np.zeros(n, np.uint32), np.zeros(n, np.int32))
+    N.check(lib.hcx_spec_copy(h, N.ptr(q.moz), N.ptr(q.intensity), N.ptr(q.idx), N.ptr(q.precurse),
+                              N.ptr(q.charges), N.ptr(q.rtimes), N.ptr(q.src_len), N.ptr(q.src_psid)), lib)
+    return q
+
+
+def make_queries(db: Database, p: Preset, nspec: int | None = None, seed: int | None = None) -> Queries:
+    cfg = _spec_cfg(p, nspec, seed)
     lib = N.host()
     h = C.c_void_p()
     N.check(lib.hcx_spec_create(db._h, C.byref(cfg), C.byref(h)), lib)
     try:
-        ns, npk = C.c_uint32(), C.c_uint64()
-        N.check(lib.hcx_spec_info(h, C.byref(ns), C.byref(npk)), lib)
-        n, k = ns.value, npk.value
-        q = Queries(np.zeros(k, np.uint32), np.zeros(k, np.uint32), np.zeros(n + 1, np.uint32),
-                    np.zeros(n, np.float32), np.zeros(n, np.int32), np.zeros(n, np.float32),
-                    np.zeros(n, np.uint32), np.zeros(n, np.int32))
-        N.check(lib.hcx_spec_copy(h, N.ptr(q.moz), N.ptr(q.intensity), N.ptr(q.idx), N.ptr(q.precurse),
-                                  N.ptr(q.charges), N.ptr(q.rtimes), N.ptr(q.src_len), N.ptr(q.src_psid)), lib)
+        return _copy_queries(lib, h)
     finally:
         lib.hcx_spec_free(h)
-    return q
+
+
+@dataclass
+class RawSpectra:
+    """Raw MS2 peaks as MSQuery::readspectrum stores them (msquery.cpp:300-412):
+    mz = (uint)(m/z * scale), intensity = (uint)(I * 1000), in file (m/z) order,
+    with each spectrum's Z-line precursor and charge and its retention time."""
+    mz: np.ndarray
+    intensity: np.ndarray
+    idx: np.ndarray
+    prec_mz: np.ndarray
+    z: np.ndarray
+    rtime: np.ndarray
+
+    @property
+    def nspec(self) -> int:
+        return len(self.prec_mz)
+
+    def view(self) -> N.RawSpectraView:
+        v = N.RawSpectraView()
+        v.mz, v.intensity, v.idx = N.ptr(self.mz), N.ptr(self.intensity), N.ptr(self.idx)
+        v.prec_mz, v.z, v.rtime = N.ptr(self.prec_mz), N.ptr(self.z), N.ptr(self.rtime)
+        v.num_specs = self.nspec
+        v.file_num = 0
+        return v
+
+
+def make_raw_spectra(db: Database, p: Preset, nspec: int | None = None,
+                     seed: int | None = None) -> tuple[RawSpectra, Queries]:
+    """The spectra of make_queries (same seed, same spectra) before pickpeaks,
+    and the generator's own normalised batch (stable tie order)."""
+    cfg = _spec_cfg(p, nspec, seed)
+    lib = N.host()
+    h = C.c_void_p()
+    N.check(lib.hcx_spec_create_raw(db._h, C.byref(cfg), C.byref(h)), lib)
+    try:
+        q = _copy_queries(lib, h)
+        npk = C.c_uint64()
+        N.check(lib.hcx_spec_raw_info(h, C.byref(npk)), lib)
+        n, k = q.nspec, npk.value
+        r = RawSpectra(np.zeros(k, np.uint32), np.zeros(k, np.uint32), np.zeros(n + 1, np.uint32),
+                       np.zeros(n, np.float64), np.zeros(n, np.int32), np.zeros(n, np.float64))
+        N.check(lib.hcx_spec_raw_copy(h, N.ptr(r.mz), N.ptr(r.intensity), N.ptr(r.idx), N.ptr(r.prec_mz),
+                                      N.ptr(r.z), N.ptr(r.rtime)), lib)
+    finally:
+        lib.hcx_spec_free(h)
+    return r, q

@@ -14,6 +14,10 @@
  *   hcx_index_export              read-back of Index.ionIndex[0].{iA,bA} (slm_dsts.h:306-320)
  *   hcx_index_free                DSLIM_DeallocateIonIndex(Index*)   (dslim.h:149, dslim.cpp:644-691)
  *   hcx_queries_upload            hand-over of a Queries batch       (slm_dsts.h:422-559)
+ *   hcx_raw_upload, hcx_queries_pick, hcx_queries_pickpeaks
+ *                                 MSQuery::pickpeaks (msquery.cpp:509-569) over a batch, as
+ *                                 MSQuery::extractbatch calls it per spectrum (msquery.cpp:238-298)
+ *   hcx_queries_info, hcx_queries_export  read-back of a Queries batch (slm_dsts.h:422-468)
  *   hcx_search                    DSLIM_QuerySpectrum(Queries*, Index*, uint_t) (dslim.h:184,
  *                                 dslim_query.cpp:515-842) with expeRT::ModelTailFit(Results*)
  *                                 (expeRT.h:143, expeRT.cpp:572-718) and the e-value block
@@ -196,6 +200,54 @@ int  hcx_index_free(hcx_index *idx);
 /* ---- query batches ---- */
 int  hcx_queries_upload(hcx_ctx *ctx, const hcx_queries_view *q, hcx_qbatch **out);
 int  hcx_queries_free(hcx_qbatch *qb);
+
+/* ---- Queries producer: MSQuery::pickpeaks on the device ----
+ * Raw peaks as MSQuery::readspectrum stores them (msquery.cpp:300-412):
+ * mz = (uint)(atof(m/z) * scale), intensity = (uint)(atof(I) * 1000).      */
+typedef struct hcx_raw_spectra_view {
+    const uint32_t *mz;         /* raw peak bins                                        */
+    const uint32_t *intensity;  /* raw intensities x1000                                */
+    const uint32_t *idx;        /* num_specs+1 row offsets                              */
+    const double   *prec_mz;    /* Z-line value as read (spectrum.prec_mz)              */
+    const int32_t  *z;          /* Z-line charge (spectrum.Z)                           */
+    const double   *rtime;      /* I RTime value (spectrum.rtime); NULL: 0               */
+    int32_t         num_specs;
+    int32_t         file_num;
+} hcx_raw_spectra_view;
+
+/* pickpeaks knobs: qalen = QALEN (config.hpp, at most peaks kept per spectrum,
+ * 1..4096), base_int and min_int = gParams.base_int / min_int
+ * (slm_dsts.h:406-407, hicops.cpp:606-610).                                  */
+typedef struct hcx_pick_params {
+    uint32_t qalen;
+    uint32_t base_int;
+    uint32_t min_int;
+    uint32_t pad_;
+} hcx_pick_params;
+
+typedef struct hcx_rawbatch hcx_rawbatch;
+
+/* Raw spectra to device memory (the device copy is never modified, so one
+ * raw batch can be picked any number of times).                           */
+int  hcx_raw_upload(hcx_ctx *ctx, const hcx_raw_spectra_view *raw, hcx_rawbatch **out);
+int  hcx_raw_free(hcx_rawbatch *rb);
+/* pickpeaks over every spectrum of rb (msquery.cpp:509-569), bit-exact: the
+ * reference's quicksort (keyval.h:48-132) and its order of equal intensities,
+ * scaling to base_int, the min_int cut, the last speclen peaks in ascending-
+ * intensity order; precurse = (float)prec_mz, charges = MAX(1, z), rtimes =
+ * (float)MAX(0.0, rtime).  The batch stays in device memory, ready for
+ * hcx_search.  kernel_ms / total_ms (optional): device time of the
+ * normalisation kernel and of the whole pass (HIP events).                  */
+int  hcx_queries_pick(hcx_ctx *ctx, hcx_rawbatch *rb, const hcx_pick_params *pp, hcx_qbatch **out,
+                      float *kernel_ms, float *total_ms);
+/* hcx_raw_upload + hcx_queries_pick + hcx_raw_free.                        */
+int  hcx_queries_pickpeaks(hcx_ctx *ctx, const hcx_raw_spectra_view *raw, const hcx_pick_params *pp,
+                           hcx_qbatch **out);
+/* Size and host copy of a batch (any argument may be NULL): moz/intensity
+ * npeaks entries, idx num_specs+1, precurse/charges/rtimes num_specs.       */
+int  hcx_queries_info(const hcx_qbatch *qb, int32_t *num_specs, uint64_t *npeaks);
+int  hcx_queries_export(const hcx_qbatch *qb, uint32_t *moz, uint32_t *intensity, uint32_t *idx,
+                        float *precurse, int32_t *charges, float *rtimes);
 
 /* ---- search: DSLIM_QuerySpectrum + ModelTailFit + e-value ----
  * results:  num_specs entries (host).            Required.

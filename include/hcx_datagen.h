@@ -78,6 +78,18 @@ int  hcx_spec_copy(const hcx_specset *ss, uint32_t *moz, uint32_t *intensity, ui
                    float *precurse, int32_t *charges, float *rtimes,
                    uint32_t *src_len, int32_t *src_psid);
 
+/* The same spectra, also keeping each one's raw peaks before pickpeaks (in
+ * ascending m/z, as an MS2 file lists them): mz = (uint)(m/z * scale),
+ * intensity = (uint)(I * 1000) (msquery.cpp:403-404).  The normalised
+ * spectra of hcx_spec_copy break intensity ties by a stable sort; the raw
+ * peaks are for the device pickpeaks, which follows the reference's sort. */
+int  hcx_spec_create_raw(const hcx_db *db, const hcx_specgen_cfg *cfg, hcx_specset **out);
+int  hcx_spec_raw_info(const hcx_specset *ss, uint64_t *npeaks);
+/* idx: nspec+1; prec_mz, z, rtime: nspec (the precursor mass, charge and
+ * retention time of hcx_spec_copy, in double where the reader keeps double). */
+int  hcx_spec_raw_copy(const hcx_specset *ss, uint32_t *mz, uint32_t *intensity, uint32_t *idx,
+                       double *prec_mz, int32_t *z, double *rtime);
+
 #ifdef __cplusplus
 }
 #endif

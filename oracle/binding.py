@@ -32,7 +32,7 @@ def lib() -> C.CDLL:
         L.orc_aa_mass.argtypes = [C.c_char]
         L.orc_stat_mod.argtypes = [C.c_char]
         for f in ("orc_index_build", "orc_window", "orc_search", "orc_candidates", "orc_count_work",
-                  "orc_tailfit", "orc_heap_replay", "orc_heap_capacity"):
+                  "orc_tailfit", "orc_heap_replay", "orc_heap_capacity", "orc_pickpeaks"):
             getattr(L, f).restype = C.c_int
         _lib = L
     return _lib
@@ -161,3 +161,32 @@ def heap_replay(hs, cap: int):
 
 def heap_capacity(t: int) -> int:
     return lib().orc_heap_capacity(C.c_uint32(t))
+
+
+def keyval_sort(key: np.ndarray, val: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """keyval.h:48-132 on copies of (key, val)."""
+    k = np.ascontiguousarray(key, np.uint32).copy()
+    v = np.ascontiguousarray(val, np.uint32).copy()
+    lib().orc_keyval_sort(N.ptr(k), N.ptr(v), C.c_uint32(len(k)))
+    return k, v
+
+
+def pickpeaks(raw, qalen: int, base_int: int, min_int: int):
+    """msquery.cpp:509-569 over a batch of raw spectra (workload.RawSpectra);
+    returns a workload.Queries (no source fields)."""
+    from hicops_amd import workload as W
+    n = raw.nspec
+    cap = int(np.minimum(np.diff(raw.idx.astype(np.int64)), qalen).sum()) if n else 0
+    q = W.Queries(np.zeros(cap, np.uint32), np.zeros(cap, np.uint32), np.zeros(n + 1, np.uint32),
+                  np.zeros(n, np.float32), np.zeros(n, np.int32), np.zeros(n, np.float32),
+                  np.zeros(n, np.uint32), np.zeros(n, np.int32))
+    rt = N.ptr(raw.rtime) if raw.rtime is not None else None
+    rc = lib().orc_pickpeaks(N.ptr(raw.mz), N.ptr(raw.intensity), N.ptr(raw.idx), N.ptr(raw.prec_mz),
+                             N.ptr(raw.z), rt, C.c_int32(n), C.c_uint32(qalen), C.c_uint32(base_int),
+                             C.c_uint32(min_int), N.ptr(q.moz), N.ptr(q.intensity), N.ptr(q.idx),
+                             N.ptr(q.precurse), N.ptr(q.charges), N.ptr(q.rtimes))
+    if rc != 0:
+        raise RuntimeError(f"orc_pickpeaks failed: {rc}")
+    k = int(q.idx[n]) if n else 0
+    q.moz, q.intensity = q.moz[:k].copy(), q.intensity[:k].copy()
+    return q

@@ -690,3 +690,88 @@ int orc_count_work(const hcx_params *p, const orc_index *idx, int nidx,
     }
     return HCX_SUCCESS;
 }
+
+/* ---- Queries producer: keyval.h:48-132, msquery.cpp:509-569 ---------------- */
+
+static void keyval_rec(uint32_t *key, uint32_t *val, int64_t left, int64_t right)
+{
+    /* keyval.h:50-80 partition */
+    int64_t i = left, j = right;
+    const uint32_t pivot = key[(left + right) / 2];
+    while (i <= j) {
+        while (key[i] < pivot) i++;
+        while (key[j] > pivot) j--;
+        if (i <= j) {
+            uint32_t tk = key[i], tv = val[i];
+            key[i] = key[j];
+            val[i] = val[j];
+            key[j] = tk;
+            val[j] = tv;
+            i++;
+            j--;
+        }
+    }
+    /* keyval.h:82-105: both halves, serially or as tasks */
+    if (left < j) keyval_rec(key, val, left, j);
+    if (i < right) keyval_rec(key, val, i, right);
+}
+
+void orc_keyval_sort(uint32_t *key, uint32_t *val, uint32_t n)
+{
+    /* keyval.h:124-127 sorts [0, lenArray-1]; with lenArray 0 the partition
+     * loop never runs */
+    if (n > 0) keyval_rec(key, val, 0, (int64_t)n - 1);
+}
+
+int orc_pickpeaks(const uint32_t *raw_mz, const uint32_t *raw_in, const uint32_t *raw_idx,
+                  const double *prec_mz, const int32_t *z, const double *rtime, int32_t n,
+                  uint32_t qalen, uint32_t base_int, uint32_t min_int,
+                  uint32_t *moz, uint32_t *intensity, uint32_t *idx,
+                  float *precurse, int32_t *charges, float *rtimes)
+{
+    if (n < 0 || qalen == 0) return HCX_ERR_INVLD_PARAM;
+    uint32_t maxS = 0;
+    for (int32_t s = 0; s < n; s++) {
+        if (raw_idx[s + 1] < raw_idx[s]) return HCX_ERR_INVLD_PARAM;
+        if (raw_idx[s + 1] - raw_idx[s] > maxS) maxS = raw_idx[s + 1] - raw_idx[s];
+    }
+    uint32_t *key = (uint32_t *)malloc((size_t)(maxS ? maxS : 1) * 4);
+    uint32_t *val = (uint32_t *)malloc((size_t)(maxS ? maxS : 1) * 4);
+    if (!key || !val) { free(key); free(val); return HCX_ERR_BAD_MEM_ALLOC; }
+    idx[0] = 0;
+    for (int32_t s = 0; s < n; s++) {
+        const int64_t S = raw_idx[s + 1] - raw_idx[s];
+        memcpy(key, raw_in + raw_idx[s], (size_t)S * 4);
+        memcpy(val, raw_mz + raw_idx[s], (size_t)S * 4);
+        /* msquery.cpp:515-517 */
+        precurse[s] = (float)prec_mz[s];
+        charges[s] = 1 > z[s] ? 1 : z[s];
+        {
+            double rt = rtime ? rtime[s] : 0.0;
+            rtimes[s] = (float)(0.0 > rt ? 0.0 : rt);
+        }
+        /* msquery.cpp:519 */
+        orc_keyval_sort(key, val, (uint32_t)S);
+        uint32_t speclen = 0;
+        if (S > 0) {
+            /* msquery.cpp:524-545 */
+            const double factor = (double)base_int / key[S - 1];
+            key[S - 1] = base_int;
+            speclen = 1;
+            for (int64_t j = S - 2; j >= S - (int64_t)qalen && j >= 0; j--) {
+                double v = key[j] * factor;
+                key[j] = v == v ? (uint32_t)v : 0u;   /* 0 * inf: x86-64 gives 0 */
+                if (key[j] >= min_int) speclen++;
+            }
+        }
+        /* msquery.cpp:548-562 */
+        const uint32_t keep = speclen >= qalen ? qalen : speclen;
+        const uint32_t off = idx[s];
+        idx[s + 1] = off + keep;
+        memcpy(moz + off, val + S - keep, (size_t)keep * 4);
+        memcpy(intensity + off, key + S - keep, (size_t)keep * 4);
+    }
+    free(key);
+    free(val);
+    return HCX_SUCCESS;
+}

@@ -92,6 +92,33 @@ int orc_heap_capacity(uint32_t topmatches);
 int orc_heap_replay(const float *hs, uint32_t n, int32_t cap, int32_t *argmax,
                     int32_t *final_size);
 
+/* keyval.h:48-132 KeyVal_Parallel_Internal: Hoare quicksort of (key, val)
+ * pairs by key, pivot key[(left+right)/2], recursing on [left, j] and
+ * [i, right].  The task split of keyval.h:96-105 only chooses which thread
+ * sorts a sub-range (the sub-ranges are disjoint), never the result.  Unstable:
+ * equal keys end in the order this partition scheme leaves them.  Pinned:
+ * hand-traced tie arrays, sortedness and the pair multiset on random arrays,
+ * numpy's argsort on distinct keys (tests/test_oracle_pins.py). */
+void orc_keyval_sort(uint32_t *key, uint32_t *val, uint32_t n);
+
+/* msquery.cpp:509-569 MSQuery::pickpeaks, once per spectrum as extractbatch
+ * calls it (msquery.cpp:282-285).  Raw peaks as readspectrum stores them
+ * (msquery.cpp:403-404): mz = (uint)(m/z * scale), intensity =
+ * (uint)(I * 1000).  Per spectrum: KeyVal sort by intensity; factor =
+ * base_int / top; top -> base_int; the next up to qalen-1 peaks scaled
+ * (uint)(I * factor) and counted when >= min_int; the last speclen peaks kept
+ * in ascending-intensity order.  precurse = (float)prec_mz, charges =
+ * MAX(1, z), rtimes = (float)MAX(0.0, rtime) (msquery.cpp:515-517).  A top
+ * intensity of 0 makes the reference compute (uint)(0 * inf), undefined in C;
+ * x86-64 gives 0 and so does this function.  Output arrays hold
+ * sum(min(S, qalen)) peaks.  Pinned: hand-computed spectra (scaling,
+ * truncation, the min_int cut, qalen, ties at the cut). */
+int orc_pickpeaks(const uint32_t *raw_mz, const uint32_t *raw_in, const uint32_t *raw_idx,
+                  const double *prec_mz, const int32_t *z, const double *rtime, int32_t n,
+                  uint32_t qalen, uint32_t base_int, uint32_t min_int,
+                  uint32_t *moz, uint32_t *intensity, uint32_t *idx,
+                  float *precurse, int32_t *charges, float *rtimes);
+
 #ifdef __cplusplus
 }
 #endif

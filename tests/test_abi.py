@@ -175,3 +175,28 @@ def test_mirror_exports_reference_entry_points():
     for name in REFERENCE_SYMBOLS:
         assert kinds.get(name) in ("T", "W"), name
     assert kinds.get("params") == "U"
+
+
+def test_pickpeaks_struct_layouts_and_null_checks(tmp_path):
+    src = tmp_path / "lay2.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "hcx.h"
+int main(void) {
+  printf("%zu %zu %zu %zu\n", sizeof(hcx_raw_spectra_view), sizeof(hcx_pick_params),
+         offsetof(hcx_raw_spectra_view, num_specs), offsetof(hcx_pick_params, min_int));
+  return 0;
+}''')
+    exe = tmp_path / "lay2"
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), "-o", str(exe), str(src)], check=True)
+    out = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert out == [C.sizeof(N.RawSpectraView), C.sizeof(N.PickParams),
+                   N.RawSpectraView.num_specs.offset, N.PickParams.min_int.offset]
+    lib = N.hcx()
+    h = C.c_void_p()
+    # argument checks that return before any device call
+    assert lib.hcx_raw_upload(None, None, C.byref(h)) == N.ERR_INVLD_PTR
+    assert lib.hcx_queries_pick(None, None, None, C.byref(h), None, None) == N.ERR_INVLD_PTR
+    assert lib.hcx_queries_info(None, None, None) == N.ERR_INVLD_PTR
+    assert lib.hcx_queries_export(None, None, None, None, None, None, None) == N.ERR_INVLD_PTR

@@ -1,0 +1,129 @@
+"""The Queries producer on the GPU: MSQuery::pickpeaks (msquery.cpp:509-569)
+over raw spectra, bit-exact against the oracle -- including the reference
+quicksort's order of equal intensities (keyval.h:48-132) -- and searches of
+the device-produced batch equal to searches of the oracle's Queries."""
+import sys
+
+import numpy as np
+import pytest
+
+from hicops_amd import native as N
+from hicops_amd import workload as W
+from hicops_amd.search import Engine
+from oracle import binding as O
+from tests.test_gpu_parity import compare
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def same_queries(a, b, tag):
+    for f in ("idx", "moz", "intensity", "precurse", "charges", "rtimes"):
+        x, y = getattr(a, f), getattr(b, f)
+        assert x.shape == y.shape, (tag, f, x.shape, y.shape)
+        if x.dtype.kind == "f":
+            assert np.array_equal(x.view(np.uint32), y.view(np.uint32)), (tag, f)
+        else:
+            assert np.array_equal(x, y), (tag, f)
+
+
+def random_raw(rng, n, tie_hi):
+    """Raw spectra of 0..3000 peaks (some above the 1024 staged in LDS),
+    intensities drawn from [0, tie_hi) so equal intensities are common."""
+    sizes = rng.choice([0, 1, 2, 3, 17, 64, 65, 150, 400, 1024, 1025, 3000], n,
+                       p=[.03, .03, .03, .03, .1, .1, .1, .2, .2, .08, .06, .04])
+    idx = np.zeros(n + 1, np.uint32)
+    idx[1:] = np.cumsum(sizes)
+    k = int(idx[-1])
+    mz = np.sort(rng.integers(10000, 200000, k)).astype(np.uint32)
+    it = rng.integers(0, tie_hi, k).astype(np.uint32)
+    prec = rng.uniform(-10, 5000, n)
+    z = rng.integers(-1, 6, n).astype(np.int32)
+    rt = rng.uniform(-100, 5000, n)
+    return W.RawSpectra(mz, it, idx, prec, z, rt)
+
+
+@pytest.fixture(scope="module")
+def eng():
+    prm = W.make_params(W.PRESETS["small"])
+    e = Engine(prm)
+    yield e
+    e.close()
+
+
+@pytest.mark.parametrize("qalen,tie_hi,base_int,min_int", [
+    (100, 4, 100000, 1000),          # heavy ties, QALEN default
+    (7, 50, 100000, 0),              # short QALEN: ties straddle the cut
+    (500, 1 << 20, 100000000, 0),    # base_int x1000 (hicops.cpp:606-610), QALEN 500
+    (1, 3, 1000, 500),               # the top peak only
+    (4096, 10, 100000, 30000),       # the largest QALEN
+])
+def test_pickpeaks_random_bitexact(eng, qalen, tie_hi, base_int, min_int):
+    rng = np.random.default_rng(qalen * 7 + tie_hi)
+    raw = random_raw(rng, 600, tie_hi)
+    log(f"test_pickpeaks_random_bitexact: qalen {qalen} ties<{tie_hi} base {base_int} min {min_int}, "
+        f"{raw.nspec} spectra, {len(raw.mz)} raw peaks")
+    ref = O.pickpeaks(raw, qalen, base_int, min_int)
+    b = eng.pickpeaks(raw, qalen, base_int, min_int)
+    got = b.export()
+    same_queries(got, ref, f"qalen{qalen}")
+    b.free()
+
+
+def test_pickpeaks_raw_batch_reusable_and_empty(eng):
+    rng = np.random.default_rng(99)
+    raw = random_raw(rng, 300, 5)
+    log("test_pickpeaks_raw_batch_reusable_and_empty")
+    rb = eng.upload_raw(raw)
+    ref = O.pickpeaks(raw, 50, 100000, 100)
+    for rep in range(3):                       # the device raw copy is never modified
+        b, kms, tms = eng.pickpeaks_timed(rb, 50, 100000, 100)
+        same_queries(b.export(), ref, f"rep{rep}")
+        assert 0 < kms <= tms
+        b.free()
+    rb.free()
+    empty = W.RawSpectra(np.zeros(0, np.uint32), np.zeros(0, np.uint32), np.zeros(1, np.uint32),
+                         np.zeros(0), np.zeros(0, np.int32), np.zeros(0))
+    b = eng.pickpeaks(empty, 100, 100000, 0)
+    q = b.export()
+    assert q.nspec == 0 and len(q.moz) == 0 and q.idx.tolist() == [0]
+    b.free()
+    with pytest.raises(N.HcxError):
+        eng.pickpeaks(raw, 0, 100000, 0)
+    with pytest.raises(N.HcxError):
+        eng.pickpeaks(raw, 4097, 100000, 0)
+
+
+@pytest.mark.parametrize("preset", ["small", "rank_share"])
+def test_pickpeaks_then_search_end_to_end(preset):
+    # generated spectra before normalisation -> GPU pickpeaks -> GPU search,
+    # against oracle pickpeaks -> oracle search
+    p = W.PRESETS[preset] if preset == "small" else W.scaled("rank_share", npep=20000, nspec=2000)
+    db = W.Database(p)
+    raw, _ = W.make_raw_spectra(db, p)
+    prm = W.make_params(p)
+    qalen = p.peaks_max
+    log(f"test_pickpeaks_then_search_end_to_end[{preset}]: {raw.nspec} spectra, {len(raw.mz)} raw peaks, "
+        f"QALEN {qalen}")
+    ref_q = O.pickpeaks(raw, qalen, p.base_int, p.min_int)
+    eng = Engine(prm)
+    eng.construct_db(db)
+    b = eng.pickpeaks(raw, qalen, p.base_int, p.min_int)
+    same_queries(b.export(), ref_q, preset)
+    odb = O.OracleDB(prm, db)
+    res_o, hist_o, st_o = odb.search(ref_q)
+    for kernel in (N.KERNEL_BATCHED, N.KERNEL_PLAIN):
+        out = eng.query_spectrum(b, kernel=kernel)
+        n = compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"pick-{preset}-{kernel}")
+        log(f"  kernel {kernel}: {n} spectra with e-values match")
+        assert n > 100
+    # the device-produced batch searches exactly like the same Queries uploaded
+    up = eng.upload(ref_q)
+    a = eng.query_spectrum(up, kernel=N.KERNEL_BATCHED)
+    c = eng.query_spectrum(b, kernel=N.KERNEL_BATCHED)
+    assert np.array_equal(a.results, c.results) and np.array_equal(a.hist, c.hist)
+    eng.close()

@@ -322,3 +322,83 @@ def test_evalue_closed_form(small_db):
     # the histogram holds every candidate PSM once
     assert np.array_equal(hist.sum(axis=1), res["cpsms"])
     assert np.array_equal(r["maxhypscore"], (np.float64(r["hyperscore"] * np.float32(10)) + 0.5).astype(np.int32))
+
+
+# ---- Queries producer: keyval.h:48-132 sort, msquery.cpp:509-569 pickpeaks ----
+
+def test_keyval_sort_hand_traced_ties():
+    # traced by hand through keyval.h:50-105 (pivot key[(l+r)/2], Hoare scans,
+    # recursion on [l, j] and [i, r])
+    k, v = O.keyval_sort(np.array([2, 1, 2, 1, 2]), np.arange(5))
+    assert k.tolist() == [1, 1, 2, 2, 2] and v.tolist() == [3, 1, 4, 0, 2]
+    k, v = O.keyval_sort(np.array([3, 3, 3, 3]), np.arange(4))
+    assert k.tolist() == [3, 3, 3, 3] and v.tolist() == [2, 3, 0, 1]
+    k, v = O.keyval_sort(np.array([5, 1, 4, 1, 5, 9, 2, 6]), np.arange(8))
+    assert k.tolist() == [1, 1, 2, 4, 5, 5, 6, 9] and v.tolist() == [3, 1, 6, 2, 4, 0, 7, 5]
+    k, v = O.keyval_sort(np.array([], np.uint32), np.array([], np.uint32))
+    assert len(k) == 0 and len(v) == 0
+
+
+def test_keyval_sort_invariants():
+    rng = np.random.default_rng(17)
+    for trial in range(300):
+        n = int(rng.integers(1, 400))
+        hi = int(rng.choice([2, 4, 50, 1 << 31]))
+        key = rng.integers(0, hi, n).astype(np.uint32)
+        val = np.arange(n, dtype=np.uint32)
+        k, v = O.keyval_sort(key, val)
+        assert np.all(np.diff(k.astype(np.int64)) >= 0)              # sorted
+        assert np.array_equal(np.sort(v), val)                        # a permutation
+        assert np.array_equal(key[v], k)                              # pairs kept together
+        if len(np.unique(key)) == n:                                  # distinct: the order is fixed
+            assert np.array_equal(v, np.argsort(key, kind="stable"))
+
+
+def _raw(spectra, prec=None, z=None, rt=None):
+    idx = np.zeros(len(spectra) + 1, np.uint32)
+    idx[1:] = np.cumsum([len(m) for m, _ in spectra])
+    mz = np.concatenate([np.asarray(m, np.uint32) for m, _ in spectra]) if spectra else np.zeros(0, np.uint32)
+    it = np.concatenate([np.asarray(i, np.uint32) for _, i in spectra]) if spectra else np.zeros(0, np.uint32)
+    n = len(spectra)
+    return W.RawSpectra(mz, it, idx,
+                        np.asarray(prec if prec is not None else [1000.0] * n, np.float64),
+                        np.asarray(z if z is not None else [2] * n, np.int32),
+                        np.asarray(rt if rt is not None else [0.0] * n, np.float64))
+
+
+def test_pickpeaks_hand_computed():
+    # intensities 500..8000 distinct: factor 100000/8000 = 12.5; the two
+    # below the top scale to 62500 and 25000
+    sp = ([100, 200, 300, 400, 500], [1000, 5000, 2000, 8000, 500])
+    q = O.pickpeaks(_raw([sp]), qalen=3, base_int=100000, min_int=20000)
+    assert q.idx.tolist() == [0, 3]
+    assert q.moz.tolist() == [300, 200, 400] and q.intensity.tolist() == [25000, 62500, 100000]
+    # min_int above 25000 drops it
+    q = O.pickpeaks(_raw([sp]), qalen=3, base_int=100000, min_int=30000)
+    assert q.moz.tolist() == [200, 400] and q.intensity.tolist() == [62500, 100000]
+    # qalen 1: the top peak only
+    q = O.pickpeaks(_raw([sp]), qalen=1, base_int=100000, min_int=0)
+    assert q.moz.tolist() == [400] and q.intensity.tolist() == [100000]
+    # truncation: 3 * (100/7) = 42.86 -> 42
+    q = O.pickpeaks(_raw([([10, 20], [3, 7])]), qalen=10, base_int=100, min_int=0)
+    assert q.moz.tolist() == [10, 20] and q.intensity.tolist() == [42, 100]
+    # ties at the cut: the sort leaves m/z order [40, 20, 50, 10, 30] (the
+    # hand-traced [2,1,2,1,2] case), the last two are kept
+    q = O.pickpeaks(_raw([([10, 20, 30, 40, 50], [2, 1, 2, 1, 2])]), qalen=2, base_int=1000, min_int=0)
+    assert q.moz.tolist() == [10, 30] and q.intensity.tolist() == [1000, 1000]
+
+
+def test_pickpeaks_edges_and_precursor_fields():
+    spectra = [([], []), ([77], [0]), ([5, 6, 7], [0, 0, 0]), ([9, 8], [4, 4])]
+    raw = _raw(spectra, prec=[1234.56789, 500.25, 700.0, 800.5], z=[0, -2, 3, 2], rt=[-5.0, 12.3, 0.0, 1e9])
+    q = O.pickpeaks(raw, qalen=5, base_int=1000, min_int=1)
+    assert q.idx.tolist() == [0, 0, 1, 2, 4]
+    # a lone zero peak becomes base_int; all-zero peaks below the top scale to 0
+    # (0 * inf, x86-64's value) and fall under min_int; [0, 0, 0] at m/z
+    # [5, 6, 7] sorts to m/z [7, 6, 5] (swap 0<->2, then 1 with itself)
+    assert q.moz[0:2].tolist() == [77, 5] and q.intensity[0:2].tolist() == [1000, 1000]
+    # ties of two: [4, 4] sorts to m/z [8, 9]; 4 * 250 = 1000
+    assert q.moz[2:4].tolist() == [8, 9] and q.intensity[2:4].tolist() == [1000, 1000]
+    assert q.precurse.tolist() == [np.float32(1234.56789), np.float32(500.25), 700.0, 800.5]
+    assert q.charges.tolist() == [1, 1, 3, 2]
+    assert q.rtimes.tolist() == [0.0, np.float32(12.3), 0.0, 1e9]
