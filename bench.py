@@ -75,7 +75,7 @@ def main() -> int:
     db = W.Database(p)
     log(f"database: {db.nforms} forms over {len(db.lengths)} lengths ({time.perf_counter() - t0:.1f}s)")
     t0 = time.perf_counter()
-    q = W.make_queries(db, p)
+    raw, q = W.make_raw_spectra(db, p)      # q: the same spectra normalised by the generator
     npeaks = int(q.idx[-1])
     log(f"queries: {q.nspec} spectra, {npeaks} peaks ({time.perf_counter() - t0:.1f}s)")
     prm = W.make_params(p)
@@ -172,6 +172,7 @@ def main() -> int:
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(prm, db, idx, q, st, args.cpu_seconds)
+    pick = queries_pick(eng, raw, p, args.steps, args.warmup, not args.no_cpu)
 
     if sur_info is not None:
         sur_info["surrogate_evalue_ms"] = round(runs["batched+surrogate"]["ems"], 3)
@@ -212,6 +213,7 @@ def main() -> int:
         "index_build": {"ions": nions, "device_ms": round(build_ms, 2),
                         "ions_per_s": round(nions / (build_ms / 1e3), 1),
                         "frac_hbm": round(b_build / (build_ms / 1e3) / HBM_PEAK, 4)},
+        "queries_pick": pick,
         "psms_valid": valid,
         "tied_spectra": tied,
         "work": {"n_ia": n_ia, "n_probe": n_probe, "peaks": npeaks},
@@ -219,6 +221,53 @@ def main() -> int:
     print(json.dumps(line), flush=True)
     eng.close()
     return 0
+
+
+def queries_pick(eng, raw, p, steps: int, warmup: int, with_cpu: bool) -> dict:
+    """The Queries producer: MSQuery::pickpeaks over the benchmark's raw
+    spectra (before normalisation), raw batch resident in HBM.  Algorithmic
+    bytes of the normalisation kernel per spectrum: 8*S raw peaks read +
+    8*kept peaks written + 24 B of precursor fields read + 16 B written."""
+    qalen = p.peaks_max
+    rb = eng.upload_raw(raw)
+    for _ in range(warmup):
+        b, _k, _t = eng.pickpeaks_timed(rb, qalen, p.base_int, p.min_int)
+        b.free()
+    kms, tms = [], []
+    t0 = time.perf_counter()
+    for i in range(steps):
+        b, k, t = eng.pickpeaks_timed(rb, qalen, p.base_int, p.min_int)
+        kms.append(k)
+        tms.append(t)
+        if i < steps - 1:
+            b.free()
+    wall = time.perf_counter() - t0
+    kept = int(b.export().idx[-1])
+    b.free()
+    rb.free()
+    S = int(raw.idx[-1])
+    nbytes = 8 * S + 8 * kept + 40 * raw.nspec
+    km = float(np.mean(kms))
+    log(f"pickpeaks: {raw.nspec} spectra, {S} raw -> {kept} kept peaks; kernel {km:.3f} ms, "
+        f"pass {np.mean(tms):.3f} ms, wall {wall / steps * 1e3:.1f} ms per call")
+    out = {"spectra_per_s": round(raw.nspec / (float(np.mean(tms)) / 1e3), 1),
+           "kernel_ms": round(km, 4), "pass_ms": round(float(np.mean(tms)), 4),
+           "call_ms": round(wall / steps * 1e3, 3), "raw_peaks": S, "kept_peaks": kept, "qalen": qalen,
+           "roofline": {"bound": "hbm", "achieved": round(nbytes / (km / 1e3) / 1e9, 2),
+                        "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                        "frac": round(nbytes / (km / 1e3) / HBM_PEAK, 4), "algorithmic_bytes": nbytes}}
+    if with_cpu:
+        from oracle import binding as O
+        from hicops_amd import workload as W
+        n = min(raw.nspec, 20000)
+        sub = W.RawSpectra(raw.mz[:raw.idx[n]], raw.intensity[:raw.idx[n]], raw.idx[:n + 1].copy(),
+                           raw.prec_mz[:n], raw.z[:n], raw.rtime[:n])
+        t0 = time.perf_counter()
+        O.pickpeaks(sub, qalen, p.base_int, p.min_int)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(n / dt, 1), "unit": "spectra/s", "cores": 1, "kind": "port",
+                               "sample": f"first {n} spectra, oracle pickpeaks (serial, as extractbatch)"}
+    return out
 
 
 SCAN_KERNEL = "k_items_chunked"

@@ -6,6 +6,7 @@
   hicops_amd/libhcx_host.so  synthetic data generator (include/hcx_datagen.h)
   oracle/liboracle.so        CPU restatement used by tests and the CPU baseline
   tests/cpp/dslim_driver     test program driving include/hcx_dslim.hpp
+  tests/cpp/tsv_driver       the same, leaving PSMs to the library's TSV writer
 
 Run as ``python -m hicops_amd.build`` or through ``__graft_entry__.build()``.
 """
@@ -73,6 +74,10 @@ def build_mirror() -> Path:
     # -rdynamic: the driver's DFile_PrintScore takes precedence over the library's
     _run(["g++", "-O2", "-std=c++17", "-Wall", "-rdynamic", "-I", str(ROOT / "include"),
           "-o", str(drv), str(ROOT / "tests" / "cpp" / "dslim_driver.cpp"),
+          "-L", str(PKG), "-lhcx_dslim", "-lhcx", "-Wl,-rpath,$ORIGIN/../../hicops_amd"])
+    # the same driver leaving PSMs to the library's TSV writer
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-DHCX_TSV_DRIVER", "-I", str(ROOT / "include"),
+          "-o", str(drv.with_name("tsv_driver")), str(ROOT / "tests" / "cpp" / "dslim_driver.cpp"),
           "-L", str(PKG), "-lhcx_dslim", "-lhcx", "-Wl,-rpath,$ORIGIN/../../hicops_amd"])
     return out
 

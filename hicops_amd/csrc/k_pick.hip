@@ -4,10 +4,10 @@
 //
 // One wavefront per spectrum.  The spectrum's raw (intensity, m/z) pairs are
 // staged in LDS (up to kPickLdsMax peaks; larger spectra in a device scratch
-// copy) and sorted by one lane with the reference's own Hoare quicksort
-// (keyval.h:48-132): the sort is unstable, and which of several equal
-// intensities survive the QALEN cut, and in what order they are stored, is the
-// order that partition scheme leaves.  Sub-ranges lying wholly below the
+// copy) and sorted with the reference's own Hoare quicksort (keyval.h:48-132),
+// its pending sub-ranges partitioned side by side, one lane each: the sort is
+// unstable, and which of several equal intensities survive the QALEN cut, and
+// in what order they are stored, is the order that partition scheme leaves.  Sub-ranges lying wholly below the
 // kept top-QALEN suffix are not sorted further: partitions only move elements
 // inside their own range, so the suffix ends exactly as in a full sort.  The
 // whole wavefront then scales, counts and writes the kept peaks, and computes
@@ -58,7 +58,31 @@ struct PickArgs {
     uint32_t *summary;           // [0] most kept peaks, [1] largest intensity, [2] largest multiplicity
     int32_t n;
     uint32_t qalen, base_int, min_int, span, lds_cap, p2;
+    uint32_t tmp_words;          // max(p2, lds_cap): range list (lds_cap/2 pairs), then p2 kept bins
 };
+
+// keyval.h:50-80: one partition of [l, r] around key[(l+r)/2].  The scans
+// stop at the pivot or at a swapped element, never outside [l, r] (the bounds
+// in the loop conditions only make that explicit).
+__device__ __forceinline__ void partition(uint32_t *key, uint32_t *val, int l, int r, int &io, int &jo) {
+    const uint32_t p = key[(l + r) >> 1];
+    int i = l, j = r;
+    while (i <= j) {
+        while (i < r && key[i] < p) i++;
+        while (j > l && key[j] > p) j--;
+        if (i <= j) {
+            const uint32_t tk = key[i], tv = val[i];
+            key[i] = key[j];
+            val[i] = val[j];
+            key[j] = tk;
+            val[j] = tv;
+            i++;
+            j--;
+        }
+    }
+    io = i;
+    jo = j;
+}
 
 // keyval.h:48-105 on key[0, n), one lane.  Both halves of every partition are
 // sorted (keyval.h:82-105), the larger pushed, so the stack stays below
@@ -67,24 +91,8 @@ __device__ void keyval_sort_lane(uint32_t *key, uint32_t *val, int n, int cut, i
     if (n < 2) return;
     int sp = 0, l = 0, r = n - 1;
     for (;;) {
-        // keyval.h:50-80: pivot key[(left+right)/2]; the scans stop at the
-        // pivot or at a swapped element, never outside [l, r] (the bounds in
-        // the loop conditions only make that explicit)
-        const uint32_t p = key[(l + r) >> 1];
-        int i = l, j = r;
-        while (i <= j) {
-            while (i < r && key[i] < p) i++;
-            while (j > l && key[j] > p) j--;
-            if (i <= j) {
-                const uint32_t tk = key[i], tv = val[i];
-                key[i] = key[j];
-                val[i] = val[j];
-                key[j] = tk;
-                val[j] = tv;
-                i++;
-                j--;
-            }
-        }
+        int i, j;
+        partition(key, val, l, r, i, j);
         const bool go_l = l < j && j >= cut;
         const bool go_r = i < r;
         if (go_l && go_r) {
@@ -111,6 +119,68 @@ __device__ void keyval_sort_lane(uint32_t *key, uint32_t *val, int n, int cut, i
     }
 }
 
+// The same sort with the whole wavefront: pending ranges sit in an LDS list,
+// and each round up to 64 of them are partitioned side by side, one lane per
+// range; the halves go back on the list (a ballot prefix gives each lane its
+// slots).  Ranges never overlap, so the order in which they are partitioned
+// does not change the result.  Pending ranges are disjoint and at least two
+// long, so the list never holds more than n/2.
+__device__ void keyval_sort_wave(uint32_t *key, uint32_t *val, int n, int cut, int *list) {
+    const int lane = threadIdx.x;
+    int cnt = 0;
+    if (n >= 2) {
+        if (lane == 0) {
+            list[0] = 0;
+            list[1] = n - 1;
+        }
+        cnt = 1;
+    }
+    __syncthreads();
+    while (cnt > 0) {
+        const int take = min(cnt, 64);
+        const int base = cnt - take;
+        int l = 0, r = -1;
+        if (lane < take) {
+            l = list[2 * (base + lane)];
+            r = list[2 * (base + lane) + 1];
+        }
+        __syncthreads();
+        int nl = 0, a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+        if (lane < take) {
+            int i, j;
+            partition(key, val, l, r, i, j);
+            if (l < j && j >= cut) {
+                a0 = l;
+                a1 = j;
+                nl = 1;
+            }
+            if (i < r) {
+                if (nl) {
+                    b0 = i;
+                    b1 = r;
+                } else {
+                    a0 = i;
+                    a1 = r;
+                }
+                nl++;
+            }
+        }
+        const uint64_t m1 = __ballot(nl >= 1), m2 = __ballot(nl == 2);
+        const uint64_t lt = (1ull << lane) - 1ull;
+        const int pre = base + __popcll(m1 & lt) + __popcll(m2 & lt);
+        if (nl >= 1) {
+            list[2 * pre] = a0;
+            list[2 * pre + 1] = a1;
+        }
+        if (nl == 2) {
+            list[2 * pre + 2] = b0;
+            list[2 * pre + 3] = b1;
+        }
+        cnt = base + __popcll(m1) + __popcll(m2);
+        __syncthreads();
+    }
+}
+
 // msquery.cpp:537: dIntArr[j] *= factor, a uint times a double truncated back
 // to uint.  0 * inf (a spectrum whose top intensity is 0) is undefined in C;
 // x86-64 converts it to 0, and so does this.
@@ -124,23 +194,15 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
     return v;
 }
 
-__global__ __launch_bounds__(64) void k_pickpeaks(PickArgs a) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    uint32_t *lds = reinterpret_cast<uint32_t *>(smem_raw);
-    const int s = blockIdx.x;
+// One spectrum: key/val are its raw (intensity, m/z) arrays, in LDS or in the
+// scratch rows (the two call sites let the compiler use LDS instructions for
+// the first); tmp (the sort's range list, then the kept bins) and stack are
+// LDS.  Spectra in LDS are sorted by the whole wavefront, larger ones by one
+// lane (a range list for them would not fit).
+template <bool IN_LDS>
+__device__ __forceinline__ void pick_one(const PickArgs &a, int s, int S, uint32_t b, uint32_t *key, uint32_t *val,
+                                         uint32_t *tmp, int *stack) {
     const int lane = threadIdx.x;
-    const uint32_t b = a.raw_idx[s];
-    const int S = (int)(a.raw_idx[s + 1] - b);
-    uint32_t *tmp = lds + 2 * a.lds_cap;
-    int *stack = reinterpret_cast<int *>(tmp + a.p2);
-    uint32_t *key, *val;
-    if ((uint32_t)S <= a.lds_cap) {
-        key = lds;
-        val = lds + a.lds_cap;
-    } else {
-        key = a.scr_key + a.big_off[s];
-        val = a.scr_val + a.big_off[s];
-    }
     for (int t = lane; t < S; t += 64) {
         key[t] = a.raw_in[b + t];
         val[t] = a.raw_mz[b + t];
@@ -148,8 +210,12 @@ __global__ __launch_bounds__(64) void k_pickpeaks(PickArgs a) {
     __syncthreads();
     // msquery.cpp:519, only the part the kept suffix depends on
     const int cut = S > (int)a.qalen ? S - (int)a.qalen : 0;
-    if (lane == 0) keyval_sort_lane(key, val, S, cut, stack);
-    __syncthreads();
+    if (IN_LDS) {
+        keyval_sort_wave(key, val, S, cut, reinterpret_cast<int *>(tmp));
+    } else {
+        if (lane == 0) keyval_sort_lane(key, val, S, cut, stack);
+        __syncthreads();
+    }
 
     // msquery.cpp:524-545: factor, top -> base_int, count of scaled >= min_int
     uint32_t speclen = 0;
@@ -224,6 +290,20 @@ __global__ __launch_bounds__(64) void k_pickpeaks(PickArgs a) {
         atomicMax(&a.summary[1], imax);
         atomicMax(&a.summary[2], mult);
     }
+}
+
+__global__ __launch_bounds__(64) void k_pickpeaks(PickArgs a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    uint32_t *lds = reinterpret_cast<uint32_t *>(smem_raw);
+    const int s = blockIdx.x;
+    const uint32_t b = a.raw_idx[s];
+    const int S = (int)(a.raw_idx[s + 1] - b);
+    uint32_t *tmp = lds + 2 * a.lds_cap;
+    int *stack = reinterpret_cast<int *>(tmp + a.tmp_words);
+    if ((uint32_t)S <= a.lds_cap)
+        pick_one<true>(a, s, S, b, lds, lds + a.lds_cap, tmp, stack);
+    else
+        pick_one<false>(a, s, S, b, a.scr_key + a.big_off[s], a.scr_val + a.big_off[s], tmp, stack);
 }
 
 // Queries.idx: exclusive prefix of the kept counts (idx[s+1] = idx[s] + cnt[s],
@@ -401,7 +481,8 @@ extern "C" int hcx_queries_pick(hcx_ctx *c, hcx_rawbatch *rb, const hcx_pick_par
     a.span = 2 * c->params.dF;
     a.lds_cap = rb->lds_cap;
     a.p2 = p2;
-    const size_t lds = (2ull * a.lds_cap + p2) * 4 + hcx::kStackMax * 8;
+    a.tmp_words = std::max(p2, a.lds_cap);
+    const size_t lds = (2ull * a.lds_cap + a.tmp_words) * 4 + hcx::kStackMax * 8;
     if (hipEventRecord(e0, st)) return fail("hcx_queries_pick: event record failed");
     if (n > 0) hcx::k_pickpeaks<<<(unsigned)n, 64, lds, st>>>(a);
     if (hipGetLastError() || hipEventRecord(e1, st)) return fail("hcx_queries_pick: normalisation kernel launch failed");

@@ -7,7 +7,10 @@
 #include "../../include/hcx_dslim.hpp"
 
 #include <cstddef>
+#include <ctime>
+#include <fstream>
 #include <mutex>
+#include <new>
 #include <unordered_map>
 #include <vector>
 
@@ -68,9 +71,117 @@ status_t ensure_ctx() {
 
 }  // namespace
 
-// The library's own sink discards the PSM; an application that defines
-// DFile_PrintScore receives every reported PSM instead.
-__attribute__((weak)) status_t DFile_PrintScore(Index *, uint_t, float, hCell *, double, uint_t) {
+// The application's list of query files (hicops.cpp:28).  Weak, so that
+// applications without one still link.
+extern std::vector<std::string> queryfiles __attribute__((weak));
+
+namespace {
+
+// dslim_fileout.cpp:29-33: the open TSV files
+struct TsvFiles {
+    std::mutex mu;
+    std::ofstream *tsvs = nullptr;
+    uint_t n = 0;
+};
+TsvFiles &tsv() {
+    static TsvFiles t;
+    return t;
+}
+
+// dslim_fileout.cpp:160-176
+std::string datetime() {
+    time_t raw;
+    std::time(&raw);
+    struct tm tmv;
+    localtime_r(&raw, &tmv);
+    char buf[80];
+    std::strftime(buf, sizeof buf, "%m.%d.%Y_%H.%M.%S_", &tmv);
+    return std::string(buf) + std::to_string(params.myid);
+}
+
+}  // namespace
+
+// dslim_fileout.cpp:45-76
+status_t DFile_InitFiles() {
+    TsvFiles &t = tsv();
+    std::lock_guard<std::mutex> g(t.mu);
+    if (t.tsvs) return SLM_SUCCESS;
+    const uint_t n = params.threads ? params.threads : 1;
+    t.tsvs = new (std::nothrow) std::ofstream[n];
+    if (!t.tsvs) return ERR_BAD_MEM_ALLOC;
+    t.n = n;
+    const std::string common = params.workspace + '/' + datetime();
+    for (uint_t f = 0; f < n; f++) {
+        t.tsvs[f].open(common + "_" + std::to_string(f) + ".tsv");
+        t.tsvs[f] << "file\t" << "scan_num\t" << "prec_mass\t" << "charge\t"
+                  << "retention_time\t" << "peptide\t" << "matched_ions\t"
+                  << "total_ions\t" << "calc_pep_mass\t" << "mass_diff\t"
+                  << "mod_info\t" << "hyperscore\t" << "expectscore\t"
+                  << "num_hits" << std::endl;
+    }
+    return SLM_SUCCESS;
+}
+
+// dslim_fileout.cpp:78-88
+status_t DFile_DeinitFiles() {
+    TsvFiles &t = tsv();
+    std::lock_guard<std::mutex> g(t.mu);
+    for (uint_t i = 0; i < t.n; i++) t.tsvs[i].close();
+    delete[] t.tsvs;
+    t.tsvs = nullptr;
+    t.n = 0;
+    return SLM_SUCCESS;
+}
+
+// dslim_fileout.cpp:90-105
+status_t DFile_PrintPartials(uint_t specid, Results *resPtr) {
+    if (!resPtr) return ERR_INVLD_PTR;
+    TsvFiles &t = tsv();
+    std::lock_guard<std::mutex> g(t.mu);
+    if (!t.tsvs) return ERR_INVLD_PARAM;
+    std::ofstream &o = t.tsvs[0];
+    o << std::to_string(specid + 1);
+    o << '\t' << std::to_string(resPtr->cpsms);
+    o << '\t' << std::to_string(resPtr->mu);
+    o << '\t' << std::to_string(resPtr->beta);
+    o << '\t' << std::to_string(resPtr->minhypscore);
+    o << '\t' << std::to_string(resPtr->nexthypscore);
+    o << std::endl;
+    return SLM_SUCCESS;
+}
+
+// dslim_fileout.cpp:107-152.  While no files are open the PSM is discarded;
+// an application that defines DFile_PrintScore receives every reported PSM
+// instead.
+__attribute__((weak)) status_t DFile_PrintScore(Index *index, uint_t specid, float pmass, hCell *psm, double e_x,
+                                                uint_t npsms) {
+    TsvFiles &t = tsv();
+    std::lock_guard<std::mutex> g(t.mu);
+    if (!t.tsvs) return SLM_SUCCESS;
+    if (!index || !psm) return ERR_INVLD_PTR;
+    const Index *lcl = index + psm->idxoffset;
+    const int_t peplen = lcl->pepIndex.peplen;
+    const int_t pepid = psm->psid;
+    const std::string pep(lcl->pepIndex.seqs + (size_t)lcl->pepEntries[pepid].seqID * peplen, (size_t)peplen);
+    const std::vector<std::string> *qf = &queryfiles;
+    const std::string file = qf && psm->fileIndex < qf->size() ? (*qf)[psm->fileIndex] : std::string();
+    const float mass = lcl->pepEntries[pepid].Mass;
+    std::ofstream &o = t.tsvs[0];
+    o << file;
+    o << '\t' << std::to_string(specid + 1);
+    o << '\t' << std::to_string(pmass);
+    o << '\t' << std::to_string(psm->pchg);
+    o << '\t' << std::to_string(psm->rtime);
+    o << '\t' << pep;
+    o << '\t' << std::to_string(psm->sharedions);
+    o << '\t' << std::to_string(psm->totalions);
+    o << '\t' << std::to_string(mass);
+    o << '\t' << std::to_string((pmass - mass));
+    o << '\t';
+    o << '\t' << std::to_string(psm->hyperscore);
+    o << '\t' << std::to_string(e_x);
+    o << '\t' << std::to_string(npsms);
+    o << std::endl;
     return SLM_SUCCESS;
 }
 

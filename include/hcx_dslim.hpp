@@ -10,6 +10,8 @@
  *   DSLIM_DeallocateSC()                          dslim.h:164
  *   expeRT::ModelTailFit(Results*)                source/core/include/expeRT.h:143
  *   DFile_PrintScore(...) -- the result sink      source/core/include/dslim_fileout.h:29
+ *   DFile_InitFiles(), DFile_DeinitFiles(),        dslim_fileout.h:28-32 (the PSM TSV writer,
+ *   DFile_PrintPartials(...)                       dslim_fileout.cpp:45-152)
  * and the parameter block the core reads, `extern gParams params`, defined by
  * the application (the reference defines it in source/app/hicops.cpp:31).
  *
@@ -204,6 +206,21 @@ status_t DSLIM_QuerySpectrum(Queries *ss, Index *index, uint_t idxchunk);
 status_t DSLIM_DeallocateIonIndex(Index *index);
 status_t DSLIM_DeallocateSC();
 status_t DFile_PrintScore(Index *index, uint_t specid, float pmass, hCell *psm, double e_x, uint_t npsms);
+
+/* The PSM TSV writer (dslim_fileout.cpp:45-152).  DFile_InitFiles opens
+ * params.threads files "<workspace>/<%m.%d.%Y_%H.%M.%S_><myid>_<f>.tsv" with
+ * the reference's header; while they are open, the library's DFile_PrintScore
+ * appends one line per PSM in the reference's format (std::to_string of each
+ * field, an empty mod_info column).  The reference writes a PSM to the file of
+ * the OpenMP thread that scored it; here every PSM reaches the calling host
+ * thread, so all lines go to file 0 (the other files hold the header only):
+ * the files' concatenated lines are the reference's, in spectrum order.
+ * File names come from `queryfiles` (the application's list, hicops.cpp:28,
+ * indexed by hCell.fileIndex; an empty name if the application defines none
+ * or the index is beyond it).                                              */
+status_t DFile_InitFiles();
+status_t DFile_DeinitFiles();
+status_t DFile_PrintPartials(uint_t specid, Results *resPtr);
 
 /* expeRT.h:58-151: one per host thread in the reference (ePtrs) */
 class expeRT {

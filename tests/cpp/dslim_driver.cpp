@@ -12,6 +12,10 @@
 //   <dir>/moz.bin inten.bin idx.bin prec.bin chg.bin   CSR spectra
 // writes <dir>/psms.txt: specid psid idxoffset sharedions totalions
 //                        hyperscore-bits e_x npsms pchg fileIndex
+// Built with -DHCX_TSV_DRIVER (tests/cpp/tsv_driver) it instead leaves the
+// PSMs to the library's TSV writer: DFile_InitFiles with params.workspace =
+// <dir> and two query threads, the batches, DFile_DeinitFiles; <dir>/rt.bin
+// holds retention times.
 #include <cinttypes>
 #include <cstdio>
 #include <cstdlib>
@@ -25,6 +29,10 @@
 
 gParams params;   // the application's parameter block, as in source/app/hicops.cpp:31
 
+#ifdef HCX_TSV_DRIVER
+// the application's query file list (source/app/hicops.cpp:28)
+std::vector<std::string> queryfiles = {"q0.ms2", "q1.ms2", "q2.ms2", "sample_3.ms2"};
+#else
 static FILE *g_out = nullptr;
 
 // The application's result sink (replaces the library's own definition).
@@ -36,6 +44,7 @@ status_t DFile_PrintScore(Index *, uint_t specid, float pmass, hCell *psm, doubl
                  psm->totalions, hb, e_x, npsms, psm->pchg, psm->fileIndex);
     return SLM_SUCCESS;
 }
+#endif
 
 template <typename T>
 static std::vector<T> load(const std::string &path) {
@@ -120,8 +129,15 @@ int main(int argc, char **argv) {
     auto prec = load<float>(dir + "/prec.bin");
     auto chg = load<int32_t>(dir + "/chg.bin");
     const int nspec = (int)prec.size();
+#ifdef HCX_TSV_DRIVER
+    auto rt = load<float>(dir + "/rt.bin");
+    params.workspace = dir;
+    params.threads = 2;
+    check(DFile_InitFiles(), "DFile_InitFiles");
+#else
     g_out = std::fopen((dir + "/psms.txt").c_str(), "w");
     if (!g_out) return 2;
+#endif
     // the batches of one query file, as the I/O threads hand them over
     for (int b = 0; b < batches; b++) {
         const int s0 = (int)((long)nspec * b / batches), s1 = (int)((long)nspec * (b + 1) / batches);
@@ -134,14 +150,25 @@ int main(int argc, char **argv) {
         q.idx = bidx.data();
         q.precurse = prec.data() + s0;
         q.charges = chg.data() + s0;
+#ifdef HCX_TSV_DRIVER
+        q.rtimes = rt.data() + s0;
+#else
         q.rtimes = nullptr;
+#endif
         q.numSpecs = s1 - s0;
         q.numPeaks = (int_t)(idx[s1] - idx[s0]);
         q.batchNum = b;
         q.fileNum = 3;
         check(DSLIM_QuerySpectrum(&q, index.data(), (uint_t)index.size()), "DSLIM_QuerySpectrum");
     }
+#ifdef HCX_TSV_DRIVER
+    check(DFile_DeinitFiles(), "DFile_DeinitFiles");
+    for (auto &ix : index) check(DSLIM_DeallocateIonIndex(&ix), "DSLIM_DeallocateIonIndex");
+    check(DSLIM_DeallocateSC(), "DSLIM_DeallocateSC");
+    return 0;
+#else
     std::fclose(g_out);
+#endif
 
     // the host copy of one length's index, released with the device index
     check(DSLIM_ExportIonIndex(&index[0]), "DSLIM_ExportIonIndex");

@@ -82,3 +82,52 @@ def test_dslim_entry_points_match_oracle(tmp_path, kernel):
     H[2:11] = [5, 10, 8, 6, 3, 1, 0, 1, 1]
     mu, beta, mn, nx, _ = O.tailfit(H, 35, 10)
     assert [int(v) for v in (tmp_path / "tailfit.txt").read_text().split()] == [mu, beta, mn, nx]
+
+
+TSV_DRIVER = ROOT / "tests" / "cpp" / "tsv_driver"
+TSV_HEADER = ("file\tscan_num\tprec_mass\tcharge\tretention_time\tpeptide\tmatched_ions\ttotal_ions\t"
+              "calc_pep_mass\tmass_diff\tmod_info\thyperscore\texpectscore\tnum_hits")
+
+
+def test_tsv_writer_matches_reference_format(tmp_path):
+    # DFile_InitFiles / DFile_PrintScore / DFile_DeinitFiles (dslim_fileout.cpp:45-152)
+    assert TSV_DRIVER.exists(), "tests/cpp/tsv_driver is built by __graft_entry__.build()"
+    p = W.scaled("small", npep=4000, nspec=300, dM=100.0)
+    db = W.Database(p)
+    q = W.make_queries(db, p)
+    q.rtimes[:] = np.linspace(0.0, 4321.123, q.nspec, dtype=np.float32)
+    prm = W.make_params(p)
+    write_case(tmp_path, p, db, q, prm, N.KERNEL_BATCHED)
+    q.rtimes.astype(np.float32).tofile(tmp_path / "rt.bin")
+    print("test_tsv_writer_matches_reference_format", flush=True)
+    r = subprocess.run([str(TSV_DRIVER), str(tmp_path), "2"], capture_output=True, text=True, timeout=300)
+    sys.stdout.write(r.stdout)
+    sys.stderr.write(r.stderr)
+    assert r.returncode == 0, r.stderr
+    files = sorted(tmp_path.glob("*.tsv"))
+    assert len(files) == 2                                           # params.threads files
+    assert [f.name[-6:] for f in files] == ["_0.tsv", "_1.tsv"]
+    lines = []
+    for f in files:
+        t = f.read_text().split("\n")
+        assert t[0] == TSV_HEADER and t[-1] == ""
+        lines += t[1:-1]
+
+    res_o, _, _ = O.OracleDB(prm, db).search(q)
+    reported = np.nonzero((res_o["flags"] & N.RES_VALID) != 0)[0]
+    assert len(lines) == len(reported) and len(lines) > 100
+    for ln, s in zip(lines, reported):
+        f = ln.split("\t")
+        assert len(f) == 14, ln
+        o = res_o[s]
+        L = db.lengths[o["idxoffset"]]
+        mass = np.float32(db.entries(L)["mass"][o["psid"]])
+        pm = np.float32(q.precurse[s])
+        # std::to_string: "%f" for float and double, "%d" / "%u" for integers
+        exact = ["sample_3.ms2", str(s + 1), f"{float(pm):f}", str(int(q.charges[s])), f"{float(q.rtimes[s]):f}",
+                 db.sequence(L, int(o["psid"])), str(int(o["sharedions"])), str(int(o["totalions"])),
+                 f"{float(mass):f}", f"{float(np.float32(pm - mass)):f}", ""]
+        assert f[:11] == exact, (s, f, exact)
+        assert abs(float(f[11]) - o["hyperscore"]) <= 1e-5 * abs(o["hyperscore"]) + 5e-7, s
+        assert abs(float(f[12]) - o["e_x"]) <= 1e-3 * abs(o["e_x"]) + 5e-7, s
+        assert f[13] == str(int(o["cpsms"])), s
