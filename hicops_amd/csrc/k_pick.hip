@@ -15,8 +15,9 @@
 // most peaks of one spectrum within 2*dF+1 bins).
 //
 // Device data per pass: staging rows at ub[s] = sum over t < s of
-// min(S_t, QALEN); a one-workgroup scan of the kept counts gives Queries.idx;
-// a copy kernel packs the rows into the dense CSR arrays of the batch.
+// min(S_t, QALEN); a device scan of the kept counts gives Queries.idx; a copy
+// kernel packs the rows into the dense CSR arrays of the batch, and a small
+// reduction the batch facts.
 #include <algorithm>
 #include <new>
 #include <vector>
@@ -33,7 +34,7 @@ struct hcx_rawbatch {
     std::vector<uint32_t> h_idx;     // host copy of the row offsets
     hcx::DeviceBuf mz, in, idx, prec, z, rt;
     hcx::DeviceBuf big_off, scr;     // scratch rows of spectra above lds_cap
-    hcx::DeviceBuf ub, cnt, st, summary;   // per-pass staging
+    hcx::DeviceBuf ub, cnt, st, summary, aux, scan;   // per-pass staging
 };
 
 namespace hcx {
@@ -41,7 +42,7 @@ namespace {
 
 constexpr uint32_t kPickLdsMax = 1024;   // raw peaks of one spectrum staged in LDS
 constexpr uint32_t kQalenMax = 4096;
-constexpr int kStackMax = 40;            // > log2(2^32): ranges pushed by the sort
+constexpr int kStackMax = 48;            // > log2(2^32): ranges pushed by the one-lane sort
 
 struct PickArgs {
     const uint32_t *raw_mz, *raw_in, *raw_idx;
@@ -55,10 +56,10 @@ struct PickArgs {
     float *prec;
     int32_t *chg;
     float *rt;
-    uint32_t *summary;           // [0] most kept peaks, [1] largest intensity, [2] largest multiplicity
+    uint32_t *imax_s, *mult_s;   // per spectrum: largest kept intensity, most kept peaks within span+1 bins
     int32_t n;
-    uint32_t qalen, base_int, min_int, span, lds_cap, p2;
-    uint32_t tmp_words;          // max(p2, lds_cap): range list (lds_cap/2 pairs), then p2 kept bins
+    uint32_t qalen, base_int, min_int, span, lds_cap;
+    uint32_t tmp_words;          // range list (lds_cap/2 pairs), then the kept bins
 };
 
 // keyval.h:50-80: one partition of [l, r] around key[(l+r)/2].  The scans
@@ -119,14 +120,115 @@ __device__ void keyval_sort_lane(uint32_t *key, uint32_t *val, int n, int cut, i
     }
 }
 
-// The same sort with the whole wavefront: pending ranges sit in an LDS list,
-// and each round up to 64 of them are partitioned side by side, one lane per
-// range; the halves go back on the list (a ballot prefix gives each lane its
-// slots).  Ranges never overlap, so the order in which they are partitioned
-// does not change the result.  Pending ranges are disjoint and at least two
-// long, so the list never holds more than n/2.
-__device__ void keyval_sort_wave(uint32_t *key, uint32_t *val, int n, int cut, int *list) {
+// Index of the (n+1)-th set bit of m (m has more than n set bits).
+__device__ __forceinline__ int nth_set_bit(uint64_t m, int n) {
+    int pos = 0;
+    for (int w = 32; w >= 1; w >>= 1) {
+        const uint64_t low = m & ((1ull << w) - 1ull);
+        const int c = __popcll(low);
+        if (n >= c) {
+            n -= c;
+            m >>= w;
+            pos += w;
+        } else {
+            m = low;
+        }
+    }
+    return pos;
+}
+
+// Per-chunk masks of a cooperative partition (ranges of at most kPickLdsMax).
+struct CoopChunks {
+    uint64_t ge[kPickLdsMax / 64];
+    uint64_t le[kPickLdsMax / 64];
+    uint32_t pge[kPickLdsMax / 64];   // ge elements before the chunk
+    uint32_t ple[kPickLdsMax / 64];   // le elements before the chunk
+};
+
+static_assert(sizeof(CoopChunks) <= kStackMax * 8, "chunk masks share the one-lane sort's stack space");
+
+// Position of the (k+1)-th set element, chunks scanned from the left.
+__device__ __forceinline__ int coop_select(const uint64_t *mask, const uint32_t *pre, int nch, int l, int k) {
+    int c = 0;
+    while (c + 1 < nch && (int)pre[c + 1] <= k) c++;
+    return l + 64 * c + nth_set_bit(mask[c], k - (int)pre[c]);
+}
+
+// keyval.h:50-80 by the whole wavefront.  With L the ascending positions of
+// keys >= pivot and R the descending positions of keys <= pivot, the scans of
+// the reference swap L[k] with R[k] exactly while L[k] <= R[k]; that count is
+// K = max over x of min(#L in [l, x], #R in [x, r]), and the final scan
+// positions follow from L[K-1], R[K-1], L[K], R[K].  Every swapped pair is
+// handled by one lane, and pairs never share a position (L[k] == R[k] swaps
+// an element with itself).
+__device__ void coop_partition(uint32_t *key, uint32_t *val, int l, int r, CoopChunks *cc, int &io, int &jo) {
     const int lane = threadIdx.x;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const uint32_t p = key[(l + r) >> 1];
+    const int n = r - l + 1;
+    const int nch = (n + 63) >> 6;
+    int tge = 0, tle = 0;
+    for (int c = 0; c < nch; c++) {
+        const int x = l + 64 * c + lane;
+        const bool in = x <= r;
+        const uint32_t k = in ? key[x] : 0u;
+        const uint64_t bg = __ballot(in && k >= p), bl = __ballot(in && k <= p);
+        if (lane == 0) {
+            cc->ge[c] = bg;
+            cc->le[c] = bl;
+            cc->pge[c] = (uint32_t)tge;
+            cc->ple[c] = (uint32_t)tle;
+        }
+        tge += __popcll(bg);
+        tle += __popcll(bl);
+    }
+    __syncthreads();
+    int best = 0;
+    for (int c = 0; c < nch; c++) {
+        if (l + 64 * c + lane <= r) {
+            const int a = (int)cc->pge[c] + __popcll(cc->ge[c] & (lt | (1ull << lane)));
+            const int b = tle - (int)cc->ple[c] - __popcll(cc->le[c] & lt);
+            best = max(best, min(a, b));
+        }
+    }
+    for (int d = 32; d > 0; d >>= 1) best = max(best, __shfl_xor(best, d, 64));
+    const int K = best;
+    for (int k = lane; k < K; k += 64) {
+        const int xl = coop_select(cc->ge, cc->pge, nch, l, k);
+        const int xr = coop_select(cc->le, cc->ple, nch, l, tle - 1 - k);
+        if (xl != xr) {
+            const uint32_t kl = key[xl], vl = val[xl];
+            key[xl] = key[xr];
+            val[xl] = val[xr];
+            key[xr] = kl;
+            val[xr] = vl;
+        }
+    }
+    const int lk1 = coop_select(cc->ge, cc->pge, nch, l, K - 1);
+    const int rk1 = coop_select(cc->le, cc->ple, nch, l, tle - K);
+    const int i0 = lk1 + 1, j0 = rk1 - 1;
+    if (i0 > j0) {
+        io = i0;
+        jo = j0;
+    } else {
+        const int lk = K < tge ? coop_select(cc->ge, cc->pge, nch, l, K) : r + 1;
+        const int rk = K < tle ? coop_select(cc->le, cc->ple, nch, l, tle - 1 - K) : l - 1;
+        io = lk <= j0 ? lk : rk1;
+        jo = rk >= i0 ? rk : lk1;
+    }
+    __syncthreads();
+}
+
+// The same sort with the whole wavefront: pending ranges sit in an LDS list.
+// A range of at least 64 keys among the top 64 entries is partitioned by all
+// lanes together; otherwise up to 64 ranges are partitioned side by side, one
+// lane per range.  The halves go back on the list (a ballot prefix gives each
+// lane its slots).  Ranges never overlap, so the order in which they are
+// partitioned does not change the result.  Pending ranges are disjoint and at
+// least two long, so the list never holds more than n/2.
+__device__ void keyval_sort_wave(uint32_t *key, uint32_t *val, int n, int cut, int *list, CoopChunks *cc) {
+    const int lane = threadIdx.x;
+    const uint64_t lt = (1ull << lane) - 1ull;
     int cnt = 0;
     if (n >= 2) {
         if (lane == 0) {
@@ -144,7 +246,34 @@ __device__ void keyval_sort_wave(uint32_t *key, uint32_t *val, int n, int cut, i
             l = list[2 * (base + lane)];
             r = list[2 * (base + lane) + 1];
         }
+        const uint64_t big = __ballot(lane < take && r - l + 1 >= 64);
         __syncthreads();
+        if (big) {
+            const int src = __ffsll((unsigned long long)big) - 1;
+            const int cl = __shfl(l, src, 64), cr = __shfl(r, src, 64);
+            if (lane == 0) {            // the top entry takes the chosen one's slot
+                list[2 * (base + src)] = list[2 * (cnt - 1)];
+                list[2 * (base + src) + 1] = list[2 * (cnt - 1) + 1];
+            }
+            cnt--;
+            int i, j;
+            coop_partition(key, val, cl, cr, cc, i, j);
+            if (lane == 0) {
+                int w = cnt;
+                if (cl < j && j >= cut) {
+                    list[2 * w] = cl;
+                    list[2 * w + 1] = j;
+                    w++;
+                }
+                if (i < cr) {
+                    list[2 * w] = i;
+                    list[2 * w + 1] = cr;
+                }
+            }
+            cnt += (cl < j && j >= cut ? 1 : 0) + (i < cr ? 1 : 0);
+            __syncthreads();
+            continue;
+        }
         int nl = 0, a0 = 0, a1 = 0, b0 = 0, b1 = 0;
         if (lane < take) {
             int i, j;
@@ -166,7 +295,6 @@ __device__ void keyval_sort_wave(uint32_t *key, uint32_t *val, int n, int cut, i
             }
         }
         const uint64_t m1 = __ballot(nl >= 1), m2 = __ballot(nl == 2);
-        const uint64_t lt = (1ull << lane) - 1ull;
         const int pre = base + __popcll(m1 & lt) + __popcll(m2 & lt);
         if (nl >= 1) {
             list[2 * pre] = a0;
@@ -211,7 +339,7 @@ __device__ __forceinline__ void pick_one(const PickArgs &a, int s, int S, uint32
     // msquery.cpp:519, only the part the kept suffix depends on
     const int cut = S > (int)a.qalen ? S - (int)a.qalen : 0;
     if (IN_LDS) {
-        keyval_sort_wave(key, val, S, cut, reinterpret_cast<int *>(tmp));
+        keyval_sort_wave(key, val, S, cut, reinterpret_cast<int *>(tmp), reinterpret_cast<CoopChunks *>(stack));
     } else {
         if (lane == 0) keyval_sort_lane(key, val, S, cut, stack);
         __syncthreads();
@@ -243,52 +371,33 @@ __device__ __forceinline__ void pick_one(const PickArgs &a, int s, int S, uint32
         tmp[t] = mz;
         imax = max(imax, in);
     }
-    // most kept peaks within span+1 consecutive bins: bitonic sort of the kept
-    // bins (padded to a power of two), then an upper bound per peak
-    int p = 1;
-    while (p < keep) p <<= 1;
-    for (int t = keep + lane; t < p; t += 64) tmp[t] = 0xFFFFFFFFu;
-    for (int k = 2; k <= p; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            __syncthreads();
-            for (int t = lane; t < p; t += 64) {
-                const int u = t ^ j;
-                if (u > t) {
-                    const uint32_t x = tmp[t], y = tmp[u];
-                    const bool up = (t & k) == 0;
-                    if ((x > y) == up) {
-                        tmp[t] = y;
-                        tmp[u] = x;
-                    }
-                }
-            }
-        }
-    }
     __syncthreads();
+    // most kept peaks within span+1 consecutive bins: for each kept peak, the
+    // kept peaks in [bin, bin + span] (every lane reads the same bin per step)
     uint32_t mult = 0;
-    for (int t = lane; t < keep; t += 64) {
-        const uint64_t lim = (uint64_t)tmp[t] + a.span;
-        int lo = t + 1, hi = keep;
-        while (lo < hi) {
-            const int m = (lo + hi) >> 1;
-            if ((uint64_t)tmp[m] <= lim) lo = m + 1;
-            else hi = m;
+    for (int t0 = 0; t0 < keep; t0 += 64) {
+        const int t = t0 + lane;
+        const uint32_t mx = t < keep ? tmp[t] : 0u;
+        uint32_t c = 0;
+        for (int y = 0; y < keep; y++) {
+            const uint32_t v = tmp[y];
+            c += (v >= mx && v - mx <= a.span) ? 1u : 0u;
         }
-        mult = max(mult, (uint32_t)(lo - t));
+        if (t < keep) mult = max(mult, c);
     }
     imax = wave_max_u32(imax);
     mult = wave_max_u32(mult);
     if (lane == 0) {
         a.cnt[s] = (uint32_t)keep;
+        a.imax_s[s] = imax;
+        a.mult_s[s] = mult;
         // msquery.cpp:515-517
         a.prec[s] = (float)a.prec_mz[s];
         const int32_t zz = a.z[s];
         a.chg[s] = 1 > zz ? 1 : zz;
         const double rt = a.rtime ? a.rtime[s] : 0.0;
         a.rt[s] = (float)(0.0 > rt ? 0.0 : rt);
-        atomicMax(&a.summary[0], (uint32_t)keep);
-        atomicMax(&a.summary[1], imax);
-        atomicMax(&a.summary[2], mult);
+        if (s == 0) a.cnt[a.n] = 0;   // the scan's last term: idx[n] = total
     }
 }
 
@@ -306,28 +415,22 @@ __global__ __launch_bounds__(64) void k_pickpeaks(PickArgs a) {
         pick_one<false>(a, s, S, b, a.scr_key + a.big_off[s], a.scr_val + a.big_off[s], tmp, stack);
 }
 
-// Queries.idx: exclusive prefix of the kept counts (idx[s+1] = idx[s] + cnt[s],
-// msquery.cpp:548-549), one workgroup.
-__global__ __launch_bounds__(1024) void k_pick_scan(const uint32_t *cnt, uint32_t *idx, int n) {
-    __shared__ uint32_t part[1024];
-    const int t = threadIdx.x;
-    const int per = (n + 1023) / 1024;
-    const int b = min(n, t * per), e = min(n, b + per);
-    uint32_t sum = 0;
-    for (int s = b; s < e; s++) sum += cnt[s];
-    part[t] = sum;
-    __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {
-        const uint32_t v = t >= d ? part[t - d] : 0u;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+// Batch facts: the largest kept count, intensity and multiplicity.
+__global__ __launch_bounds__(256) void k_pick_max(const uint32_t *cnt, const uint32_t *imax_s, const uint32_t *mult_s,
+                                                  int n, uint32_t *summary) {
+    uint32_t c = 0, m = 0, u = 0;
+    for (int s = blockIdx.x * 256 + threadIdx.x; s < n; s += gridDim.x * 256) {
+        c = max(c, cnt[s]);
+        m = max(m, imax_s[s]);
+        u = max(u, mult_s[s]);
     }
-    uint32_t run = part[t] - sum;
-    if (t == 0) idx[0] = 0;
-    for (int s = b; s < e; s++) {
-        run += cnt[s];
-        idx[s + 1] = run;
+    c = wave_max_u32(c);
+    m = wave_max_u32(m);
+    u = wave_max_u32(u);
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&summary[0], c);
+        atomicMax(&summary[1], m);
+        atomicMax(&summary[2], u);
     }
 }
 
@@ -436,7 +539,8 @@ extern "C" int hcx_queries_pick(hcx_ctx *c, hcx_rawbatch *rb, const hcx_pick_par
     if (b->moz.ensure(4 * tot + 16) || b->inten.ensure(4 * tot + 16) || b->idx.ensure(4ull * (n + 1)) ||
         b->prec.ensure(4ull * n + 16) || b->chg.ensure(4ull * n + 16) || b->rt.ensure(4ull * n + 16) ||
         rb->ub.ensure(4ull * n + 16) || rb->cnt.ensure(4ull * n + 16) || rb->st.ensure(8 * tot + 16) ||
-        rb->summary.ensure(16)) {
+        rb->summary.ensure(16) || rb->aux.ensure(8ull * n + 16) ||
+        rb->scan.ensure(4 * hcx::scan_tmp_words((uint64_t)n + 1) + 64)) {
         delete b;
         return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_queries_pick: device allocation failed");
     }
@@ -454,8 +558,6 @@ extern "C" int hcx_queries_pick(hcx_ctx *c, hcx_rawbatch *rb, const hcx_pick_par
     if (hipMemcpyAsync(rb->ub.p, ub.data(), 4ull * n, hipMemcpyHostToDevice, st) ||
         hipMemsetAsync(rb->summary.p, 0, 16, st) || hipEventRecord(e2, st))
         return fail("hcx_queries_pick: set-up failed");
-    uint32_t p2 = 1;
-    while (p2 < pp->qalen) p2 <<= 1;
     hcx::PickArgs a{};
     a.raw_mz = (const uint32_t *)rb->mz.p;
     a.raw_in = (const uint32_t *)rb->in.p;
@@ -473,20 +575,32 @@ extern "C" int hcx_queries_pick(hcx_ctx *c, hcx_rawbatch *rb, const hcx_pick_par
     a.prec = (float *)b->prec.p;
     a.chg = (int32_t *)b->chg.p;
     a.rt = (float *)b->rt.p;
-    a.summary = (uint32_t *)rb->summary.p;
+    a.imax_s = (uint32_t *)rb->aux.p;
+    a.mult_s = (uint32_t *)rb->aux.p + n;
     a.n = n;
     a.qalen = pp->qalen;
     a.base_int = pp->base_int;
     a.min_int = pp->min_int;
     a.span = 2 * c->params.dF;
     a.lds_cap = rb->lds_cap;
-    a.p2 = p2;
-    a.tmp_words = std::max(p2, a.lds_cap);
+    // keeps the chunk masks after it 16-byte aligned
+    a.tmp_words = (std::max(std::min(pp->qalen, rb->max_len), a.lds_cap) + 3u) & ~3u;
     const size_t lds = (2ull * a.lds_cap + a.tmp_words) * 4 + hcx::kStackMax * 8;
     if (hipEventRecord(e0, st)) return fail("hcx_queries_pick: event record failed");
     if (n > 0) hcx::k_pickpeaks<<<(unsigned)n, 64, lds, st>>>(a);
     if (hipGetLastError() || hipEventRecord(e1, st)) return fail("hcx_queries_pick: normalisation kernel launch failed");
-    hcx::k_pick_scan<<<1, 1024, 0, st>>>((const uint32_t *)rb->cnt.p, (uint32_t *)b->idx.p, n);
+    // Queries.idx = exclusive scan of the kept counts and a trailing 0
+    // (idx[s+1] = idx[s] + speclen, msquery.cpp:548-549)
+    if (n == 0) {
+        if (hipMemsetAsync(b->idx.p, 0, 4, st)) return fail("hcx_queries_pick: set-up failed");
+    } else if (hcx::scan_exclusive_u32(st, (const uint32_t *)rb->cnt.p, (uint32_t *)b->idx.p, (uint64_t)n + 1,
+                                       (uint32_t *)rb->scan.p) != HCX_SUCCESS) {
+        return fail("hcx_queries_pick: scan failed");
+    }
+    if (n > 0) {
+        hcx::k_pick_max<<<(unsigned)std::min<int32_t>((n + 255) / 256, 1024), 256, 0, st>>>(
+            (const uint32_t *)rb->cnt.p, a.imax_s, a.mult_s, n, (uint32_t *)rb->summary.p);
+    }
     if (n > 0)
         hcx::k_pick_compact<<<(unsigned)((n + 3) / 4), 256, 0, st>>>(
             (const uint32_t *)rb->ub.p, (const uint32_t *)rb->cnt.p, (const uint32_t *)b->idx.p, a.st_moz, a.st_in,

@@ -159,6 +159,9 @@ REFERENCE_SYMBOLS = [
     "_Z18DSLIM_DeallocateSCv",
     "_ZN6expeRT12ModelTailFitEP8_Results",
     "_Z16DFile_PrintScoreP6_IndexjfP10_heapEntrydj",
+    "_Z15DFile_InitFilesv",
+    "_Z17DFile_DeinitFilesv",
+    "_Z19DFile_PrintPartialsjP8_Results",
 ]
 
 
