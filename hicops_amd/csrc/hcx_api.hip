@@ -392,8 +392,11 @@ extern "C" int hcx_queries_upload(hcx_ctx *c, const hcx_queries_view *q, hcx_qba
         if (!bytes || !h) return hipSuccess;
         return hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, c->stream);
     };
+    // a batch without retention times holds zeros (what hcx_queries_export returns)
+    const bool zero_rt = !q->rtimes && n > 0;
     if (up(b->moz, q->moz, 4 * npk) || up(b->inten, q->intensity, 4 * npk) || up(b->idx, q->idx, 4ull * (n + 1)) ||
         up(b->prec, q->precurse, 4ull * n) || up(b->chg, q->charges, 4ull * n) || up(b->rt, q->rtimes, 4ull * n) ||
+        (zero_rt && hipMemsetAsync(b->rt.p, 0, 4ull * n, c->stream) != hipSuccess) ||
         hipStreamSynchronize(c->stream) != hipSuccess) {
         delete b;
         return set_error("hcx_queries_upload: copy failed");

@@ -92,6 +92,19 @@ def test_pickpeaks_raw_batch_reusable_and_empty(eng):
     q = b.export()
     assert q.nspec == 0 and len(q.moz) == 0 and q.idx.tolist() == [0]
     b.free()
+    # no retention times: zeros (msquery.cpp:517 with rtime 0)
+    nort = W.RawSpectra(raw.mz, raw.intensity, raw.idx, raw.prec_mz, raw.z, None)
+    b = eng.pickpeaks(nort, 50, 100000, 100)
+    got = b.export()
+    same_queries(got, O.pickpeaks(nort, 50, 100000, 100), "no-rtime")
+    assert np.all(got.rtimes == 0)
+    b.free()
+    # an uploaded batch without retention times exports zeros as well
+    q0 = got
+    q0.rtimes = None
+    up = eng.upload(q0)
+    assert np.all(up.export().rtimes == 0)
+    up.free()
     with pytest.raises(N.HcxError):
         eng.pickpeaks(raw, 0, 100000, 0)
     with pytest.raises(N.HcxError):
