@@ -135,6 +135,8 @@ extern "C" int hcx_destroy(hcx_ctx *c) {
     if (c->ev3) (void)hipEventDestroy(c->ev3);
     if (c->ev4) (void)hipEventDestroy(c->ev4);
     if (c->ev5) (void)hipEventDestroy(c->ev5);
+    for (hipStream_t a : c->aux)
+        if (a) (void)hipStreamDestroy(a);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return HCX_SUCCESS;

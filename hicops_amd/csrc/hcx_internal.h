@@ -62,6 +62,7 @@ struct hcx_ctx {
     hcx_params params;
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t aux[2] = {nullptr, nullptr};   // default priority; pickpeaks size classes side by side
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     hipEvent_t ev4 = nullptr, ev5 = nullptr;   // around the e-value stage
     bool timed_ok = false;                     // the last search recorded all six events

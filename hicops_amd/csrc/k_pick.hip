@@ -646,9 +646,17 @@ extern "C" int hcx_queries_pick(hcx_ctx *c, hcx_rawbatch *rb, const hcx_pick_par
     }
     hipStream_t st = c->stream;
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
-    auto fail = [&](const char *msg) {
-        for (hipEvent_t e : {e0, e1, e2, e3})
+    hipEvent_t ef = nullptr, ej0 = nullptr, ej1 = nullptr;   // fork / joins of the class streams
+    auto drop = [&]() {
+        for (hipEvent_t e : {e0, e1, e2, e3, ef, ej0, ej1})
             if (e) (void)hipEventDestroy(e);
+    };
+    auto fail = [&](const char *msg) {
+        // nothing launched may still use the batch's buffers
+        (void)hipStreamSynchronize(st);
+        for (hipStream_t x : c->aux)
+            if (x) (void)hipStreamSynchronize(x);
+        drop();
         delete b;
         return set_error(msg);
     };
@@ -684,9 +692,18 @@ extern "C" int hcx_queries_pick(hcx_ctx *c, hcx_rawbatch *rb, const hcx_pick_par
     a.span = 2 * c->params.dF;
     if (hipEventRecord(e0, st)) return fail("hcx_queries_pick: event record failed");
     // one launch per size class, each with the LDS its largest spectrum needs
-    // (small spectra then fit more workgroups per CU; measured 4.2 -> 3.7 ms on
-    // 200 k spectra of 20-660 peaks; finer classes gain nothing more, as each
-    // launch drains before the next)
+    // (small spectra then fit more workgroups per CU).  The classes are
+    // independent: the second and third run on two auxiliary streams, so one
+    // class's tail overlaps the others' work.
+    for (int i = 0; i < 2; i++)
+        if (!c->aux[i] && hipStreamCreateWithFlags(&c->aux[i], hipStreamNonBlocking) != hipSuccess) {
+            c->aux[i] = nullptr;
+            return fail("hcx_queries_pick: stream creation failed");
+        }
+    if (hipEventCreateWithFlags(&ef, hipEventDisableTiming) || hipEventCreateWithFlags(&ej0, hipEventDisableTiming) ||
+        hipEventCreateWithFlags(&ej1, hipEventDisableTiming) || hipEventRecord(ef, st) ||
+        hipStreamWaitEvent(c->aux[0], ef, 0) || hipStreamWaitEvent(c->aux[1], ef, 0))
+        return fail("hcx_queries_pick: stream set-up failed");
     for (int k = 0; k <= hcx::kPickBins; k++) {
         const uint32_t cnt = rb->bin_off[k + 1] - rb->bin_off[k];
         if (!cnt) continue;
@@ -696,8 +713,12 @@ extern "C" int hcx_queries_pick(hcx_ctx *c, hcx_rawbatch *rb, const hcx_pick_par
         // 16-byte aligned
         a.tmp_words = k < hcx::kPickBins ? a.lds_cap : (std::min(pp->qalen, rb->max_len) + 3u) & ~3u;
         const size_t lds = (2ull * a.lds_cap + a.tmp_words) * 4 + hcx::kStackMax * 8;
-        hcx::k_pickpeaks<<<cnt, 64, lds, st>>>(a);
+        hipStream_t sk = (k == 1 || k == 2) ? c->aux[k - 1] : st;
+        hcx::k_pickpeaks<<<cnt, 64, lds, sk>>>(a);
     }
+    if (hipGetLastError() || hipEventRecord(ej0, c->aux[0]) || hipEventRecord(ej1, c->aux[1]) ||
+        hipStreamWaitEvent(st, ej0, 0) || hipStreamWaitEvent(st, ej1, 0))
+        return fail("hcx_queries_pick: normalisation kernel launch failed");
     if (hipGetLastError() || hipEventRecord(e1, st)) return fail("hcx_queries_pick: normalisation kernel launch failed");
     // Queries.idx = exclusive scan of the kept counts and a trailing 0
     // (idx[s+1] = idx[s] + speclen, msquery.cpp:548-549)
@@ -724,7 +745,7 @@ extern "C" int hcx_queries_pick(hcx_ctx *c, hcx_rawbatch *rb, const hcx_pick_par
     float km = 0, tm = 0;
     if (hipEventElapsedTime(&km, e0, e1) || hipEventElapsedTime(&tm, e2, e3))
         return fail("hcx_queries_pick: timing failed");
-    for (hipEvent_t e : {e0, e1, e2, e3}) (void)hipEventDestroy(e);
+    drop();
     if (kernel_ms) *kernel_ms = km;
     if (total_ms) *total_ms = tm;
     b->npeaks = npk;
