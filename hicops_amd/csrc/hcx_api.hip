@@ -530,6 +530,8 @@ int launch_sequence(hcx_ctx *c, ChunkArgs &ca, int kernel, bool tailfit = true) 
     int rc = launch_fill_u32(c, (uint32_t *)ca.acc, n * (sizeof(SpecAcc) / 4), 0);
     if (!rc) rc = launch_fill_u32(c, a.hist, n * c->hbins, 0);
     if (!rc) rc = launch_fill_u32(c, a.tie, 1, 0);
+    // the replay's work counter, after the list's nspec entries
+    if (!rc) rc = launch_fill_u32(c, a.tie + a.nspec + 1, 1, 0);
     if (!rc) rc = launch_fill_u32(c, ca.queue, 1, 0);
     if (rc) return rc;
     if (kernel == HCX_KERNEL_CURSOR)

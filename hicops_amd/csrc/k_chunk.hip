@@ -1120,7 +1120,15 @@ __global__ __launch_bounds__(kBT) void k_replay_chunked(ChunkArgs a) {
     }
     const uint32_t nwork = ((const AS_GLB uint32_t *)a.q.tie)[0];
     const uint32_t D = a.q.max_mass + 2;
-    for (uint32_t wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+    // tied spectra are taken one at a time from a counter after the list
+    // (zeroed before the scan): their replays differ widely in length
+    __shared__ uint32_t s_grab;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_grab = atomicAdd(&a.q.tie[a.q.nspec + 1], 1u);
+        __syncthreads();
+        const uint32_t wi = s_grab;
+        if (wi >= nwork) break;
         const uint32_t s = ((const AS_GLB uint32_t *)a.q.tie)[1 + wi];
         __syncthreads();
         const uint32_t P = load_peaks(a, sm, s);
@@ -1407,7 +1415,7 @@ int launch_replay_chunked(hcx_ctx *ctx, ChunkArgs &a) {
     r.hl_cap = 0;
     const size_t base = chunk_smem_bytes(r);
     const size_t share = (size_t)ctx->max_lds / std::max<uint32_t>(1, ctx->blocks_per_cu);
-    const size_t fixed = 0;
+    const size_t fixed = 16;   // the kernel's static LDS (the work counter's copy)
     if (share > base + fixed + 4096)
         r.hl_cap = std::min<uint32_t>(a.hl_cap, (uint32_t)(((share - base - fixed) / 2) & ~(size_t)63));
     if (chunk_smem_bytes(r) + fixed > (size_t)ctx->max_lds)
