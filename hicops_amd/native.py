@@ -18,11 +18,14 @@ HCX_MAX_MOD_TYPES = 15
 
 # status codes (slmerr.h values)
 SUCCESS = 0
-ERR_INVLD_PTR = -3001
-ERR_INVLD_MEMORY = -3002
-ERR_INVLD_PARAM = -3003
-ERR_BAD_MEM_ALLOC = -3004
-ERR_INVLD_SIZE = -3006
+# source/core/include/slmerr.h:24-43
+ERR_INVLD_MEMORY = -3001
+ERR_INVLD_PARAM = -3002
+ERR_INVLD_SIZE = -3004
+ERR_FILE_NOT_FOUND = -3006
+ERR_BAD_MEM_ALLOC = -3007
+ERR_INVLD_PTR = -3008
+ERR_NOT_ENOUGH_DATA = -3012
 ERR_DEVICE = -3100
 
 RES_VALID = 0x1

@@ -36,15 +36,22 @@
 extern "C" {
 #endif
 
-/* ---- status codes: same values as source/core/include/slmerr.h:24-43 ---- */
-#define HCX_SUCCESS            0
-#define HCX_ERR_INVLD_PTR      (-3001)
-#define HCX_ERR_INVLD_MEMORY   (-3002)
-#define HCX_ERR_INVLD_PARAM    (-3003)
-#define HCX_ERR_BAD_MEM_ALLOC  (-3004)
-#define HCX_ERR_INVLD_SIZE     (-3006)
-#define HCX_ERR_FILE_NOT_FOUND (-3007)
-#define HCX_ERR_NOT_ENOUGH_DATA (-3010)
+/* ---- status codes: same values as source/core/include/slmerr.h:24-43
+ * (SLM_ERROR_BASE -3000 minus 1 ... 13) ---- */
+#define HCX_SUCCESS               0
+#define HCX_ERR_INVLD_MEMORY      (-3001)
+#define HCX_ERR_INVLD_PARAM       (-3002)
+#define HCX_ERR_INVLD_INDEX       (-3003)
+#define HCX_ERR_INVLD_SIZE        (-3004)
+#define HCX_ERR_SLM_ALREADY_INIT  (-3005)
+#define HCX_ERR_FILE_NOT_FOUND    (-3006)
+#define HCX_ERR_BAD_MEM_ALLOC     (-3007)
+#define HCX_ERR_INVLD_PTR         (-3008)
+#define HCX_ERR_INVLD_SORT        (-3009)
+#define HCX_ERR_INVLD_MOD         (-3010)
+#define HCX_ERR_INVLD_NODE_RANK   (-3011)
+#define HCX_ERR_NOT_ENOUGH_DATA   (-3012)
+#define HCX_ERR_FILE_ERROR        (-3013)
 #define HCX_ERR_DEVICE         (-3100) /* a HIP call failed; hcx_last_error() has the text */
 
 /* ---- compile-time knobs of the reference, as run-time fields ----

@@ -50,6 +50,12 @@ typedef char     AA;
 #define ERR_INVLD_SIZE     HCX_ERR_INVLD_SIZE
 #define ERR_FILE_NOT_FOUND HCX_ERR_FILE_NOT_FOUND
 #define ERR_NOT_ENOUGH_DATA HCX_ERR_NOT_ENOUGH_DATA
+#define ERR_INVLD_INDEX    HCX_ERR_INVLD_INDEX
+#define ERR_SLM_ALREADY_INIT HCX_ERR_SLM_ALREADY_INIT
+#define ERR_INVLD_SORT     HCX_ERR_INVLD_SORT
+#define ERR_INVLD_MOD      HCX_ERR_INVLD_MOD
+#define ERR_INVLD_NODE_RANK HCX_ERR_INVLD_NODE_RANK
+#define ERR_FILE_ERROR     HCX_ERR_FILE_ERROR
 
 /* compile-time knobs of the reference (CMakeLists.txt:157-177) */
 #ifndef MAX_HYPERSCORE

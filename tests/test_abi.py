@@ -203,3 +203,22 @@ int main(void) {
     assert lib.hcx_queries_pick(None, None, None, C.byref(h), None, None) == N.ERR_INVLD_PTR
     assert lib.hcx_queries_info(None, None, None) == N.ERR_INVLD_PTR
     assert lib.hcx_queries_export(None, None, None, None, None, None, None) == N.ERR_INVLD_PTR
+
+
+def test_status_codes_are_the_reference_values(tmp_path):
+    # source/core/include/slmerr.h:24-43: SLM_ERROR_BASE (-3000) minus 1 ... 13
+    ref = {"INVLD_MEMORY": -3001, "INVLD_PARAM": -3002, "INVLD_INDEX": -3003, "INVLD_SIZE": -3004,
+           "SLM_ALREADY_INIT": -3005, "FILE_NOT_FOUND": -3006, "BAD_MEM_ALLOC": -3007, "INVLD_PTR": -3008,
+           "INVLD_SORT": -3009, "INVLD_MOD": -3010, "INVLD_NODE_RANK": -3011, "NOT_ENOUGH_DATA": -3012,
+           "FILE_ERROR": -3013}
+    src = tmp_path / "codes.cpp"
+    src.write_text("#include <cstdio>\n#include \"hcx_dslim.hpp\"\nint main() {\n" + "".join(
+        f'  std::printf("%d %d\\n", HCX_ERR_{k}, ERR_{k});\n' for k in ref) + "  return SLM_SUCCESS;\n}\n")
+    exe = tmp_path / "codes"
+    subprocess.run(["g++", "-std=c++17", "-I", str(ROOT / "include"), "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    for (k, v), line in zip(ref.items(), out):
+        assert [int(x) for x in line.split()] == [v, v], k
+    for k in ("INVLD_MEMORY", "INVLD_PARAM", "INVLD_SIZE", "FILE_NOT_FOUND", "BAD_MEM_ALLOC", "INVLD_PTR",
+              "NOT_ENOUGH_DATA"):
+        assert getattr(N, "ERR_" + k) == ref[k], k
