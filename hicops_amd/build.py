@@ -3,7 +3,8 @@
   hicops_amd/libhcx.so       HIP kernels + C ABI (include/hcx.h), gfx950
   hicops_amd/libhcx_dslim.so the reference's C++ entry points over the C ABI
                              (include/hcx_dslim.hpp)
-  hicops_amd/libhcx_host.so  synthetic data generator (include/hcx_datagen.h)
+  hicops_amd/libhcx_host.so  synthetic data generator (include/hcx_datagen.h) and the MS2
+                             reader (include/hcx_ms2.h)
   oracle/liboracle.so        CPU restatement used by tests and the CPU baseline
   tests/cpp/dslim_driver     test program driving include/hcx_dslim.hpp
   tests/cpp/tsv_driver       the same, leaving PSMs to the library's TSV writer
@@ -52,7 +53,7 @@ def build_oracle() -> Path:
 def build_host() -> Path:
     out = PKG / "libhcx_host.so"
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-Wall",
-          "-o", str(out), str(CSRC / "datagen.cpp")])
+          "-o", str(out), str(CSRC / "datagen.cpp"), str(CSRC / "ms2.cpp")])
     return out
 
 

@@ -1,4 +1,5 @@
-"""ctypes bindings of libhcx.so (include/hcx.h) and libhcx_host.so (include/hcx_datagen.h).
+"""ctypes bindings of libhcx.so (include/hcx.h) and libhcx_host.so (include/hcx_datagen.h,
+include/hcx_ms2.h).
 
 The search path runs only through libhcx.so.  If the library is missing or a
 call fails, an exception is raised: there is no fallback implementation.
@@ -200,7 +201,8 @@ def host() -> C.CDLL:
         for name in ["hcx_db_create", "hcx_db_free", "hcx_db_lengths", "hcx_db_length_info",
                      "hcx_db_length_copy", "hcx_db_length_view", "hcx_spec_create",
                      "hcx_spec_free", "hcx_spec_info", "hcx_spec_copy", "hcx_spec_create_raw",
-                     "hcx_spec_raw_info", "hcx_spec_raw_copy"]:
+                     "hcx_spec_raw_info", "hcx_spec_raw_copy", "hcx_ms2_read", "hcx_ms2_free",
+                     "hcx_ms2_info", "hcx_ms2_copy"]:
             getattr(lib, name).restype = C.c_int
         _host = lib
     return _host

@@ -7,6 +7,7 @@ import sys
 import numpy as np
 import pytest
 
+from hicops_amd import ingest
 from hicops_amd import native as N
 from hicops_amd import workload as W
 from hicops_amd.search import Engine
@@ -140,3 +141,19 @@ def test_pickpeaks_then_search_end_to_end(preset):
     c = eng.query_spectrum(b, kernel=N.KERNEL_BATCHED)
     assert np.array_equal(a.results, c.results) and np.array_equal(a.hist, c.hist)
     eng.close()
+
+
+def test_ms2_file_to_device_batch(eng, tmp_path):
+    # MS2 text -> host reader (MSQuery::readspectrum rules) -> device pickpeaks,
+    # against the oracle's pickpeaks of the same raw spectra
+    from tests.test_ms2_ingest import write_ms2
+    p = W.scaled("small", npep=3000, nspec=400)
+    db = W.Database(p)
+    raw0, _ = W.make_raw_spectra(db, p)
+    f = tmp_path / "q.ms2"
+    write_ms2(f, raw0, 100)
+    raw = ingest.read_ms2(f, 100)
+    log(f"test_ms2_file_to_device_batch: {raw.nspec} spectra, {len(raw.mz)} raw peaks")
+    b = eng.pickpeaks(raw, 100, p.base_int, p.min_int)
+    same_queries(b.export(), O.pickpeaks(raw, 100, p.base_int, p.min_int), "ms2")
+    b.free()
