@@ -2,15 +2,15 @@
 //
 // 1. k_frag: one thread per (peptide form, charge, series) walks the L-1
 //    fragment bins of that series (UTILS_GenerateSpectrum utils.cpp:231-276,
-//    UTILS_GenerateModSpectrum utils.cpp:378-505) and bumps the bin histogram
-//    (dslim.cpp:300-323).  FP64 with contraction off, so the truncated bins
-//    equal the host's.
-// 2. bA = exclusive scan of the histogram (dslim.cpp:89-126).
-// 3. iA = ion ids stably sorted by bin: LSD radix sort, 8-bit digits, tiles of
+//    UTILS_GenerateModSpectrum utils.cpp:378-505; dslim.cpp:300-323).  FP64
+//    with contraction off, so the truncated bins equal the host's.
+// 2. iA = ion ids stably sorted by bin: LSD radix sort, 8-bit digits, tiles of
 //    4096 keys, wave multi-split ranking (ballots) so equal digits keep their
 //    input order.  A stable sort of ascending ids equals the reference's
 //    quicksort followed by the per-bin ascending sort (dslim.cpp:361-388,
 //    452-482).
+// 3. bA[b] = ions in bins below b, the exclusive scan of the bin histogram
+//    (dslim.cpp:89-126), read off the sorted bins with no atomics.
 // 4. mdir[d] = first entry with mass >= d Da: a directory that shortens the
 //    precursor-window binary searches of the query.
 #include "hcx_internal.h"
@@ -51,7 +51,7 @@ __device__ inline float pep_mass_dev(const char *seq, uint32_t L) {
     return m;
 }
 
-__global__ __launch_bounds__(kBT) void k_frag(BuildArgs a, uint32_t *keys, uint32_t *cnt) {
+__global__ __launch_bounds__(kBT) void k_frag(BuildArgs a, uint32_t *keys) {
     const uint64_t t = (uint64_t)blockIdx.x * kBT + threadIdx.x;
     const uint32_t per = a.maxz * 2;
     if (t >= (uint64_t)a.n * per) return;
@@ -83,7 +83,6 @@ __global__ __launch_bounds__(kBT) void k_frag(BuildArgs a, uint32_t *keys, uint3
         // illegal form: speclen zeros into bin 0 (dslim.cpp:314-323); a legal
         // mod form the reference leaves unfilled is zero-filled too
         for (uint32_t l = 0; l < L1; l++) out[l] = 0;
-        atomicAdd(&cnt[0], L1);
         return;
     }
     const double scale = (double)a.scale;
@@ -102,7 +101,6 @@ __global__ __launch_bounds__(kBT) void k_frag(BuildArgs a, uint32_t *keys, uint3
             }
             uint32_t b = v >= NB ? NB - 1 : v;
             out[l] = b;
-            atomicAdd(&cnt[b], 1u);
         }
     } else {
         // cumulative undivided sums + prefix of mod masses, then integer divide
@@ -128,7 +126,6 @@ __global__ __launch_bounds__(kBT) void k_frag(BuildArgs a, uint32_t *keys, uint3
             uint32_t v = (cum + madd) / c;
             uint32_t b = v >= NB ? NB - 1 : v;
             out[l] = b;
-            atomicAdd(&cnt[b], 1u);
         }
     }
 }
@@ -257,6 +254,21 @@ __global__ __launch_bounds__(kBT) void k_radix_scatter(const uint32_t *kin, cons
     }
 }
 
+// bA[b] = number of ions in bins below b (dslim.cpp:89-126): the lower bound
+// of b in the bins sorted ascending, one thread per bin (neighbouring bins
+// take nearly the same search path).
+__global__ void k_bucket_offsets(const uint32_t *skeys, uint64_t n, uint32_t NB, uint32_t *bA) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > NB) return;
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint64_t m = (lo + hi) >> 1;
+        if (skeys[m] < b) lo = m + 1;
+        else hi = m;
+    }
+    bA[b] = (uint32_t)lo;
+}
+
 __global__ void k_mdir(const float *mass, uint32_t n, uint32_t *mdir, uint32_t D) {
     uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= D) return;
@@ -268,11 +280,6 @@ __global__ void k_mdir(const float *mass, uint32_t n, uint32_t *mdir, uint32_t D
         else hi = m;
     }
     mdir[d] = lo;
-}
-
-__global__ void k_zero(uint32_t *p, uint64_t n) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
 }
 
 }  // namespace
@@ -319,30 +326,27 @@ int launch_index_build(hcx_ctx *ctx, const BuildArgs &a, uint32_t *d_iA, uint32_
     const int passes = (int)((keybits + kRB - 1) / kRB);
     const uint64_t ntiles = (nions + kTile - 1) / kTile;
     const uint64_t tc_words = (uint64_t)kR * (ntiles ? ntiles : 1);
-    const uint64_t scan_words = scan_tmp_words(std::max<uint64_t>(tc_words, NB + 1));
+    const uint64_t scan_words = scan_tmp_words(tc_words);
 
-    DeviceBuf K0, K1, V1, V2, cnt, tc, tmp;
+    DeviceBuf K0, K1, V1, V2, tc, tmp;
     if (K0.ensure(nions * 4 + 4) || K1.ensure(nions * 4 + 4) || V1.ensure(nions * 4 + 4) ||
-        (passes > 2 && V2.ensure(nions * 4 + 4)) || cnt.ensure((uint64_t)(NB + 1) * 4) ||
-        tc.ensure(tc_words * 4) || tmp.ensure(scan_words * 4))
+        (passes > 2 && V2.ensure(nions * 4 + 4)) || tc.ensure(tc_words * 4) || tmp.ensure(scan_words * 4))
         return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "index build: device allocation failed");
     uint32_t *k0 = (uint32_t *)K0.p, *k1 = (uint32_t *)K1.p, *v1 = (uint32_t *)V1.p, *v2 = (uint32_t *)V2.p;
-    uint32_t *dcnt = (uint32_t *)cnt.p, *dtc = (uint32_t *)tc.p, *dtmp = (uint32_t *)tmp.p;
+    uint32_t *dtc = (uint32_t *)tc.p, *dtmp = (uint32_t *)tmp.p;
 
-    k_zero<<<(NB + 1 + 255) / 256, 256, 0, st>>>(dcnt, NB + 1);
-    HCX_HIP(hipGetLastError());
     const uint64_t nthr = (uint64_t)a.n * a.maxz * 2;
     if (nthr) {
-        k_frag<<<(unsigned)((nthr + kBT - 1) / kBT), kBT, 0, st>>>(a, k0, dcnt);
+        k_frag<<<(unsigned)((nthr + kBT - 1) / kBT), kBT, 0, st>>>(a, k0);
         HCX_HIP(hipGetLastError());
     }
-    int rc = scan_exclusive(st, dcnt, d_bA, (uint64_t)NB + 1, dtmp);
-    if (rc) return rc;
+    int rc = HCX_SUCCESS;
+    const uint32_t *sorted = k0;   // the bins in ascending order once sorted
     if (nions) {
         const uint32_t *kin = k0, *vin = nullptr;
         for (int p = 0; p < passes; p++) {
             const bool last = p == passes - 1;
-            uint32_t *kout = last ? nullptr : (kin == k0 ? k1 : k0);
+            uint32_t *kout = kin == k0 ? k1 : k0;   // the last pass keeps its keys for bA
             uint32_t *vout = last ? d_iA : (p % 2 == 0 ? v1 : v2);
             k_radix_hist<<<(unsigned)ntiles, kBT, 0, st>>>(kin, nions, p * kRB, dtc, (uint32_t)ntiles);
             HCX_HIP(hipGetLastError());
@@ -354,7 +358,10 @@ int launch_index_build(hcx_ctx *ctx, const BuildArgs &a, uint32_t *d_iA, uint32_
             kin = kout;
             vin = vout;
         }
+        sorted = kin;
     }
+    k_bucket_offsets<<<(NB + 1 + 255) / 256, 256, 0, st>>>(sorted, nions, NB, d_bA);
+    HCX_HIP(hipGetLastError());
     const uint32_t D = max_mass + 2;
     k_mdir<<<(D + 255) / 256, 256, 0, st>>>(d_mass, a.n, d_mdir, D);
     HCX_HIP(hipGetLastError());
