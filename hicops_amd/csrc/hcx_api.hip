@@ -284,20 +284,23 @@ extern "C" int hcx_index_build(hcx_ctx *c, uint32_t peplen, const char *seqs, ui
     a.min_mass = (double)p.min_mass;
     a.max_mass = (double)p.max_mass;
     std::memcpy(a.mod_mass, p.mod_mass, sizeof(a.mod_mass));
+    // every allocation before the timed device work: the (bin, chunk)
+    // refinement, the chunk-local entries and the sort's scratch
+    ix->chunk = c->chunk;
+    ix->nch = std::max<uint32_t>(1, (nentries + c->chunk - 1) / c->chunk);
+    if (ix->bA2.ensure(4ull * c->NB * (ix->nch + 1)))
+        return fail(set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_index_build: chunk offsets allocation failed"));
+    if (ix->iAc.ensure(2 * nions + 16))
+        return fail(set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_index_build: chunk-local entry allocation failed"));
+    if (int rr = reserve_index_build(c, nions, c->NB)) return fail(rr);
     if (hipEventRecord(c->ev2, c->stream) != hipSuccess) return fail(set_error("event"));
     int rc = launch_index_build(c, a, (uint32_t *)ix->iA.p, (uint32_t *)ix->bA.p, (const float *)ix->mass.p,
                                 (uint32_t *)ix->mdir.p, p.max_mass);
     if (rc) return fail(rc);
     // (bin, chunk) refinement of bA for the chunked query
-    ix->chunk = c->chunk;
-    ix->nch = std::max<uint32_t>(1, (nentries + c->chunk - 1) / c->chunk);
-    if (ix->bA2.ensure(4ull * c->NB * (ix->nch + 1)))
-        return fail(set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_index_build: chunk offsets allocation failed"));
     rc = launch_chunk_offsets(c, (const uint32_t *)ix->iA.p, (const uint32_t *)ix->bA.p, c->NB, ix->nch, ix->chunk,
                               speclen, (uint32_t *)ix->bA2.p);
     if (rc) return fail(rc);
-    if (ix->iAc.ensure(2 * nions + 16))
-        return fail(set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_index_build: chunk-local entry allocation failed"));
     rc = launch_chunk_local(c, (const uint32_t *)ix->iA.p, nions, speclen, ix->chunk, (uint16_t *)ix->iAc.p);
     if (rc) return fail(rc);
     if (hipEventRecord(c->ev1, c->stream) != hipSuccess || hipEventSynchronize(c->ev1) != hipSuccess)

@@ -85,6 +85,7 @@ struct hcx_ctx {
     uint32_t sur_step = 0;            // Adam steps taken
     hcx_audit last_audit = {};        // audit of the last surrogate search
     hcx::DeviceBuf scan_tmp;          // device scan scratch
+    hcx::DeviceBuf bld[6];            // index build scratch: keys x2, values x2, digit counts, scan
     bool sur_ready = false;
     uint32_t chunk = 16384;           // peptides per query chunk (power of two)
     uint32_t slots = 1536;            // scored-peptide slots per chunk pass (measured best)
@@ -136,6 +137,9 @@ struct BuildArgs {
     double min_mass, max_mass;
     uint32_t mod_mass[HCX_MAX_MOD_TYPES];
 };
+// Sizes the index build's scratch (kept by the context, grown geometrically)
+// for nions ions; launch_index_build then allocates nothing.
+int reserve_index_build(hcx_ctx *ctx, uint64_t nions, uint32_t NB);
 int launch_index_build(hcx_ctx *ctx, const BuildArgs &a, uint32_t *d_iA, uint32_t *d_bA,
                        const float *d_mass, uint32_t *d_mdir, uint32_t max_mass);
 

@@ -316,22 +316,59 @@ int scan_exclusive_u32(hipStream_t st, const uint32_t *in, uint32_t *out, uint64
     return HCX_SUCCESS;
 }
 
+namespace {
+
+struct BuildSizes {
+    int passes;
+    uint64_t ntiles, tc_words, scan_words;
+};
+
+BuildSizes build_sizes(uint64_t nions, uint32_t NB) {
+    uint32_t keybits = 1;
+    while (keybits < 32 && (1ull << keybits) < NB) keybits++;
+    BuildSizes z;
+    z.passes = (int)((keybits + kRB - 1) / kRB);
+    z.ntiles = (nions + kTile - 1) / kTile;
+    z.tc_words = (uint64_t)kR * (z.ntiles ? z.ntiles : 1);
+    z.scan_words = scan_tmp_words(z.tc_words);
+    return z;
+}
+
+// grow-only, with headroom, so that a run of lengths of rising size
+// reallocates only a few times
+int grow(DeviceBuf &d, uint64_t bytes) {
+    if (bytes <= d.bytes && d.p) return 0;
+    return d.ensure(std::max<uint64_t>(bytes, d.bytes + d.bytes / 2));
+}
+
+}  // namespace
+
+int reserve_index_build(hcx_ctx *ctx, uint64_t nions, uint32_t NB) {
+    const BuildSizes z = build_sizes(nions, NB);
+    const bool fits = ctx->bld[0].bytes >= nions * 4 + 4 && ctx->bld[4].bytes >= z.tc_words * 4 &&
+                      ctx->bld[5].bytes >= z.scan_words * 4 && (z.passes <= 2 || ctx->bld[3].bytes >= nions * 4 + 4);
+    if (fits) return HCX_SUCCESS;
+    // nothing queued may still use a buffer that is about to be replaced
+    HCX_HIP(hipStreamSynchronize(ctx->stream));
+    if (grow(ctx->bld[0], nions * 4 + 4) || grow(ctx->bld[1], nions * 4 + 4) || grow(ctx->bld[2], nions * 4 + 4) ||
+        (z.passes > 2 && grow(ctx->bld[3], nions * 4 + 4)) || grow(ctx->bld[4], z.tc_words * 4) ||
+        grow(ctx->bld[5], z.scan_words * 4))
+        return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "index build: device allocation failed");
+    return HCX_SUCCESS;
+}
+
 int launch_index_build(hcx_ctx *ctx, const BuildArgs &a, uint32_t *d_iA, uint32_t *d_bA, const float *d_mass,
                        uint32_t *d_mdir, uint32_t max_mass) {
     hipStream_t st = ctx->stream;
     const uint64_t nions = (uint64_t)a.n * a.speclen;
     const uint32_t NB = a.NB;
-    uint32_t keybits = 1;
-    while (keybits < 32 && (1ull << keybits) < NB) keybits++;
-    const int passes = (int)((keybits + kRB - 1) / kRB);
-    const uint64_t ntiles = (nions + kTile - 1) / kTile;
-    const uint64_t tc_words = (uint64_t)kR * (ntiles ? ntiles : 1);
-    const uint64_t scan_words = scan_tmp_words(tc_words);
-
-    DeviceBuf K0, K1, V1, V2, tc, tmp;
-    if (K0.ensure(nions * 4 + 4) || K1.ensure(nions * 4 + 4) || V1.ensure(nions * 4 + 4) ||
-        (passes > 2 && V2.ensure(nions * 4 + 4)) || tc.ensure(tc_words * 4) || tmp.ensure(scan_words * 4))
-        return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "index build: device allocation failed");
+    const BuildSizes z = build_sizes(nions, NB);
+    const int passes = z.passes;
+    const uint64_t ntiles = z.ntiles, tc_words = z.tc_words;
+    int rc0 = reserve_index_build(ctx, nions, NB);
+    if (rc0) return rc0;
+    DeviceBuf &K0 = ctx->bld[0], &K1 = ctx->bld[1], &V1 = ctx->bld[2], &V2 = ctx->bld[3], &tc = ctx->bld[4],
+              &tmp = ctx->bld[5];
     uint32_t *k0 = (uint32_t *)K0.p, *k1 = (uint32_t *)K1.p, *v1 = (uint32_t *)V1.p, *v2 = (uint32_t *)V2.p;
     uint32_t *dtc = (uint32_t *)tc.p, *dtmp = (uint32_t *)tmp.p;
 
@@ -365,8 +402,7 @@ int launch_index_build(hcx_ctx *ctx, const BuildArgs &a, uint32_t *d_iA, uint32_
     const uint32_t D = max_mass + 2;
     k_mdir<<<(D + 255) / 256, 256, 0, st>>>(d_mass, a.n, d_mdir, D);
     HCX_HIP(hipGetLastError());
-    // temporaries are freed on return; finish the stream first
-    HCX_HIP(hipStreamSynchronize(st));
+    // the scratch stays with the context: later builds on this stream reuse it in order
     return HCX_SUCCESS;
 }
 
