@@ -206,19 +206,46 @@ __global__ __launch_bounds__(kBT) void k_radix_hist(const uint32_t *keys, uint64
     for (int d = threadIdx.x; d < kR; d += kBT) tc[(uint64_t)d * ntiles + blockIdx.x] = h[d];
 }
 
+// One tile of kTile keys: stable ranks within the tile by digit (wave
+// multi-split per round of kBT keys, rounds and waves in input order), the
+// (key, value) pairs staged in LDS in digit order, then written out so that
+// each digit's run of the tile goes to consecutive addresses.
+static_assert(kBT == kR, "one thread per digit");
+
 __global__ __launch_bounds__(kBT) void k_radix_scatter(const uint32_t *kin, const uint32_t *vin,
                                                        uint32_t *kout, uint32_t *vout, uint64_t n,
                                                        int shift, const uint32_t *offs, uint32_t ntiles) {
-    __shared__ uint32_t base[kR];
+    __shared__ uint32_t start[kR];        // tile-local start of each digit's run
+    __shared__ uint32_t run[kR];          // next tile-local slot of each digit
+    __shared__ uint32_t gofs[kR];         // global start of (digit, tile)
     __shared__ uint32_t wc[kBT / 64][kR];
-    for (int d = threadIdx.x; d < kR; d += kBT) {
-        base[d] = offs[(uint64_t)d * ntiles + blockIdx.x];
-        for (int w = 0; w < kBT / 64; w++) wc[w][d] = 0;
-    }
-    __syncthreads();
+    __shared__ uint32_t sk[kTile], sv[kTile];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const uint64_t tbase = (uint64_t)blockIdx.x * kTile;
+    const uint32_t tn = (uint32_t)min<uint64_t>(kTile, n - tbase);
+    {
+        // the tile's count of digit d, from the scanned (digit, tile) offsets
+        // (digit-major: the next entry is the next tile's or the next digit's)
+        const int d = threadIdx.x;
+        const uint64_t at = (uint64_t)d * ntiles + blockIdx.x;
+        const uint32_t g = offs[at];
+        const uint32_t cnt = (at + 1 < (uint64_t)kR * ntiles ? offs[at + 1] : (uint32_t)n) - g;
+        gofs[d] = g;
+        run[d] = cnt;
+        for (int ww = 0; ww < kBT / 64; ww++) wc[ww][d] = 0;
+        __syncthreads();
+        // exclusive scan of the counts over the digits
+        for (int off = 1; off < kR; off <<= 1) {
+            const uint32_t v = d >= off ? run[d - off] : 0u;
+            __syncthreads();
+            run[d] += v;
+            __syncthreads();
+        }
+        start[d] = run[d] - cnt;
+        run[d] = start[d];
+        __syncthreads();
+    }
     for (int r = 0; r < kIPT; r++) {
         const uint64_t i = tbase + (uint64_t)r * kBT + threadIdx.x;
         const bool valid = i < n;
@@ -236,21 +263,31 @@ __global__ __launch_bounds__(kBT) void k_radix_scatter(const uint32_t *kin, cons
         if (valid && rank == 0) wc[w][d] = (uint32_t)__popcll(peers);
         __syncthreads();
         if (valid) {
-            uint32_t pos = base[d] + rank;
+            uint32_t pos = run[d] + rank;
             for (int ww = 0; ww < w; ww++) pos += wc[ww][d];
-            if (kout) kout[pos] = key;
-            vout[pos] = val;
+            sk[pos] = key;
+            sv[pos] = val;
         }
         __syncthreads();
-        for (int dd = threadIdx.x; dd < kR; dd += kBT) {
+        {
+            const int dd = threadIdx.x;
             uint32_t add = 0;
             for (int ww = 0; ww < kBT / 64; ww++) {
                 add += wc[ww][dd];
                 wc[ww][dd] = 0;
             }
-            base[dd] += add;
+            run[dd] += add;
         }
         __syncthreads();
+    }
+    // staged entry j of digit d goes to gofs[d] + (j - start[d]): each
+    // digit's run of the tile lands on consecutive addresses
+    for (uint32_t j = threadIdx.x; j < tn; j += kBT) {
+        const uint32_t key = sk[j];
+        const uint32_t d = (key >> shift) & (kR - 1);
+        const uint32_t pos = gofs[d] + (j - start[d]);
+        if (kout) kout[pos] = key;
+        vout[pos] = sv[j];
     }
 }
 
