@@ -359,3 +359,32 @@ def test_autotune_keeps_parity():
     out = eng.query_spectrum(b, kernel=N.KERNEL_BATCHED)
     compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, "autotuned")
     eng.close()
+
+
+def test_index_build_tiny_illegal_and_empty():
+    # bA read off the sorted bins: one form, forms whose precursor mass is
+    # below min_mass (every ion in bin 0, dslim.cpp:317-322), a mix, and no forms
+    prm = W.make_params(W.PRESETS["small"])
+    log("test_index_build_tiny_illegal_and_empty")
+    L = 7
+    seqs = np.frombuffer(b"GGGGGGK" + b"PEPTIDK" + b"AAAAAAR" + b"WWWWWWR", np.uint8).copy()
+
+    def entries(ids):
+        e = np.zeros(len(ids), N.PEP_DTYPE)
+        for k, sid in enumerate(ids):
+            e[k]["seq_id"] = sid
+            e[k]["mass"] = O.pep_mass(bytes(seqs[sid * L:(sid + 1) * L]).decode())
+        order = np.argsort(e["mass"], kind="stable")
+        return e[order]
+
+    eng = Engine(prm)
+    # GGGGGGK (488 Da) and AAAAAAR (~600 Da) straddle min_mass 500
+    for ids in ([1], [0], [0, 0, 0], [0, 1, 2, 3, 0, 2], []):
+        ents = entries(ids)
+        ix = eng.construct(L, seqs, ents)
+        iA, bA = ix.export()
+        oiA, obA = O.index_build(prm, L, seqs, ents)
+        assert np.array_equal(bA, obA), ids
+        assert np.array_equal(iA, oiA), ids
+        assert bA[0] == 0 and bA[-1] == len(ids) * 2 * prm.maxz * (L - 1)
+    eng.close()
