@@ -51,10 +51,31 @@ __device__ inline float pep_mass_dev(const char *seq, uint32_t L) {
     return m;
 }
 
+// One thread per (form, series, charge) row of L-1 bins; row t of the output
+// starts at t * (L-1), so a block's rows are one contiguous range: the rows go
+// to LDS (odd row stride, no bank conflicts) and out with coalesced stores.
+__device__ __forceinline__ void frag_row(const BuildArgs &a, uint64_t t, uint32_t *out);
+
 __global__ __launch_bounds__(kBT) void k_frag(BuildArgs a, uint32_t *keys) {
-    const uint64_t t = (uint64_t)blockIdx.x * kBT + threadIdx.x;
+    extern __shared__ uint32_t stage[];
     const uint32_t per = a.maxz * 2;
-    if (t >= (uint64_t)a.n * per) return;
+    const uint32_t L1 = a.peplen - 1;
+    const uint32_t S = L1 | 1u;
+    const uint64_t rows = (uint64_t)a.n * per;
+    const uint64_t t0 = (uint64_t)blockIdx.x * kBT;
+    const uint64_t t = t0 + threadIdx.x;
+    if (t < rows) frag_row(a, t, stage + threadIdx.x * S);
+    __syncthreads();
+    const uint32_t nr = (uint32_t)min<uint64_t>(kBT, rows - t0);
+    uint32_t *dst = keys + t0 * L1;
+    for (uint32_t j = threadIdx.x; j < nr * L1; j += kBT) {
+        const uint32_t r = j / L1;
+        dst[j] = stage[r * S + (j - r * L1)];
+    }
+}
+
+__device__ __forceinline__ void frag_row(const BuildArgs &a, uint64_t t, uint32_t *out) {
+    const uint32_t per = a.maxz * 2;
     const uint32_t k = (uint32_t)(t / per);
     const uint32_t r = (uint32_t)(t % per);
     const uint32_t series = r / a.maxz;      // 0 = b, 1 = y
@@ -64,7 +85,6 @@ __global__ __launch_bounds__(kBT) void k_frag(BuildArgs a, uint32_t *keys) {
     const uint32_t L1 = L - 1;
     const hcx_pep_entry e = a.entries[k];
     const char *seq = a.seqs + (size_t)e.seq_id * L;
-    uint32_t *out = keys + (uint64_t)k * a.speclen + series * a.maxz * L1 + z * L1;
 
     const bool mod = e.mod_num != 0;
     float mass;
@@ -411,7 +431,8 @@ int launch_index_build(hcx_ctx *ctx, const BuildArgs &a, uint32_t *d_iA, uint32_
 
     const uint64_t nthr = (uint64_t)a.n * a.maxz * 2;
     if (nthr) {
-        k_frag<<<(unsigned)((nthr + kBT - 1) / kBT), kBT, 0, st>>>(a, k0);
+        const size_t lds = (size_t)kBT * ((a.peplen - 1) | 1u) * 4;
+        k_frag<<<(unsigned)((nthr + kBT - 1) / kBT), kBT, lds, st>>>(a, k0);
         HCX_HIP(hipGetLastError());
     }
     int rc = HCX_SUCCESS;
