@@ -14,8 +14,11 @@
  *   orc_lgfact            closed form lgamma(n+1)/ln 10
  *   orc_pep_mass          published monoisotopic peptide masses (+ residue table values)
  *   orc_mod_mass          unmodified mass + first mod mass (Q4), float arithmetic
- *   orc_gen_*spectrum     textbook b/y m/z within the truncation bound; b/y complementarity;
- *                         hand-computed bins for short peptides
+ *   orc_gen_*spectrum     hand-traced exact bins of utils.cpp:231-276 / :378-505: full
+ *                         b/y ladders at z = 1..3, the b1 of every residue, one- and
+ *                         two-mod-type ladders; planted mistakes (ORC_MUTANT_*) are
+ *                         rejected by them; textbook b/y m/z within the truncation
+ *                         bound; b/y complementarity
  *   orc_index_build       bA prefix-sum invariants, iA permutation + per-bin order,
  *                         brute-force stable sort on tiny inputs
  *   orc_window            brute-force linear scan
@@ -52,11 +55,37 @@ static float stat_mod_of(char c) { return c == 'C' ? 57.021464f : 0.0f; }
 float orc_aa_mass(char c) { return kAAMass[c - 'A']; }
 float orc_stat_mod(char c) { return stat_mod_of(c); }
 
+/* Planted mistakes, for the pin tests only: tests/test_oracle_pins.py
+ * rebuilds this file with one ORC_MUTANT_* macro defined and checks that
+ * the hand-traced bins reject the result.  The oracle built by the Makefile
+ * defines none of them.
+ *   ORC_MUTANT_FLOAT_PROTON    PROTON as a float constant: GETAA and the ion
+ *                              arithmetic built on it in float
+ *   ORC_MUTANT_SUM_THEN_TRUNC  unmodified ions: one truncation of the running
+ *                              sum instead of one per residue term
+ *   ORC_MUTANT_MOD_PER_TERM    modified ions: each term divided by the charge
+ *                              before the sum (the unmodified rule)
+ *   ORC_MUTANT_Y_MOD_ORDER     y-ion mods taken in N-to-C nibble order
+ *   ORC_MUTANT_NO_STATMOD      the static C+57.021464 left out */
+
 /* utils.cpp:101 GETAA: float + float, then + double */
-static double getaa(char c, uint32_t z)
+#ifdef ORC_MUTANT_FLOAT_PROTON
+typedef float getaa_t;   /* a float PROTON makes GETAA and every product using it float */
+#else
+typedef double getaa_t;
+#endif
+static getaa_t getaa(char c, uint32_t z)
 {
+#ifdef ORC_MUTANT_NO_STATMOD
+    float f = kAAMass[c - 'A'];
+#else
     float f = kAAMass[c - 'A'] + stat_mod_of(c);
+#endif
+#ifdef ORC_MUTANT_FLOAT_PROTON
+    return f + (float)ORC_PROTON * (float)z;
+#else
     return (double)f + ORC_PROTON * (double)z;
+#endif
 }
 
 void orc_lgfact(double *out, uint32_t n)
@@ -101,6 +130,17 @@ float orc_gen_spectrum(const char *seq, uint32_t len, const hcx_params *p, uint3
         for (uint32_t z = 0; z < maxz; z++) {
             uint32_t bstart = z * (len - 1);
             uint32_t ystart = z * (len - 1) + maxz * (len - 1);
+#ifdef ORC_MUTANT_SUM_THEN_TRUNC
+            double bsum = getaa(seq[0], z + 1) * scale, ysum = (getaa(seq[len - 1], z + 1) + ORC_H2O) * scale;
+            S[bstart] = (uint32_t)(bsum / (z + 1));
+            S[ystart] = (uint32_t)(ysum / (z + 1));
+            for (uint32_t l = 1; l < len - 1; l++) {
+                bsum += getaa(seq[l], 0) * scale;
+                ysum += getaa(seq[len - 1 - l], 0) * scale;
+                S[bstart + l] = (uint32_t)(bsum / (z + 1));
+                S[ystart + l] = (uint32_t)(ysum / (z + 1));
+            }
+#else
             S[bstart] = (uint32_t)((getaa(seq[0], z + 1) * scale) / (z + 1));
             S[ystart] = (uint32_t)(((getaa(seq[len - 1], z + 1) + ORC_H2O) * scale) / (z + 1));
             for (uint32_t l = 1; l < len - 1; l++) {
@@ -108,6 +148,7 @@ float orc_gen_spectrum(const char *seq, uint32_t len, const hcx_params *p, uint3
                 S[ystart + l] = S[ystart + l - 1] +
                                 (uint32_t)((getaa(seq[len - 1 - l], 0) * scale) / (z + 1));
             }
+#endif
         }
     }
     return mass;
@@ -152,16 +193,26 @@ float orc_gen_mod_spectrum(const char *seq, uint32_t len, const hcx_pep_entry *e
             uint32_t bstart = z * (len - 1), counter = 0;
             for (uint32_t l = 0; l < len - 1; l++) {
                 counter += modPos[l];
+#ifdef ORC_MUTANT_MOD_PER_TERM
+                uint32_t t = 0;
+                for (uint32_t k = 0; k < counter; k++) t += p->mod_mass[modNums[k]] / (z + 1);
+                S[bstart + l] = S[bstart + l] / (z + 1) + t;
+#else
                 for (uint32_t k = 0; k < counter; k++) S[bstart + l] += p->mod_mass[modNums[k]];
                 S[bstart + l] /= (z + 1);
+#endif
             }
         }
         for (uint32_t z = 0; z < maxz; z++) {
             uint32_t ystart = z * (len - 1) + maxz * (len - 1), counter = 0;
             for (int l = (int)len - 1; l > 0; l--) {
                 counter += modPos[l];
+#ifdef ORC_MUTANT_Y_MOD_ORDER
+                for (uint32_t k = 0; k < counter; k++) S[ystart + (len - 1) - l] += p->mod_mass[modNums[k]];
+#else
                 for (uint32_t k = 0; k < counter; k++)
                     S[ystart + (len - 1) - l] += p->mod_mass[modNums[modSeen - 1 - k]];
+#endif
                 S[ystart + (len - 1) - l] /= (z + 1);
             }
         }

@@ -109,6 +109,135 @@ def test_fragment_bins_hand_computed():
     assert list(bins) == [5802, 14711]
 
 
+# ---- exact ladders traced by hand through utils.cpp:231-276 and :378-505 ----
+#
+# Unmodified ions (utils.cpp:257-270), charge Z = z + 1:
+#   b1 = (uint)((double(float AA[s0] + float StatMod[s0]) + 1.00727647*Z) * 100 / Z)
+#   b_l = b_{l-1} + (uint)(double(float AA[s_l] + float StatMod[s_l]) * 100 / Z)
+#   y1 = (uint)(((double(AA[s_last]) + 1.00727647*Z) + double(18.015f)) * 100 / Z), y_l likewise
+#   from the C-terminus; every term is truncated on its own (Q5).
+# PEPTIDEK at Z = 1, term by term: P 97.0527f = 97.05270385742..., + 1.00727647
+#   = 98.05998..., x100 -> b1 = 9805; E 129.0426f x100 = 12904.26 -> 12904, b2 = 22709;
+#   P 9705, T 10104, I 11308, D 11502, E 12904 -> b3..b7 = 32414 42518 53826 65328 78232.
+#   y1 = (128.094f + 1.00727647 + 18.015f) x100 = 14711.49 -> 14711; E 12904 -> 27615, ...
+# At Z = 2 the first b term is (97.05270 + 2.01455294) x 50 = 4953.36 -> 4953 and the
+#   others are halved before truncation (E: 6452.13 -> 6452, b2 = 11405).
+# Layout [b z=1][b z=2][b z=3][y z=1][y z=2][y z=3], each L-1 long.
+LADDERS_Z3 = {
+    "PEPTIDEK": [9805, 22709, 32414, 42518, 53826, 65328, 78232,
+                 4953, 11405, 16257, 21309, 26963, 32714, 39166,
+                 3335, 7636, 10871, 14239, 18008, 21842, 26143,
+                 14711, 27615, 39117, 50425, 60529, 70234, 83138,
+                 7406, 13858, 19609, 25263, 30315, 35167, 41619,
+                 4971, 9272, 13106, 16875, 20243, 23478, 27779],
+    # C carries the static +57.021464, summed in float with 103.00919 (utils.cpp:101)
+    "CMWHK": [16103, 29207, 47814, 61519, 8102, 14654, 23957, 30809, 5435, 9803, 16005, 20573,
+              14711, 28416, 47023, 60127, 7406, 14258, 23561, 30113, 4971, 9539, 15741, 20109],
+    "MAMWWK": [13204, 20307, 33411, 52018, 70625, 6652, 10203, 16755, 26058, 35361,
+               4468, 6835, 11203, 17405, 23607, 14711, 33318, 51925, 65029, 72132,
+               7406, 16709, 26012, 32564, 36115, 4971, 11173, 17375, 21743, 24110],
+}
+
+# b1 of every residue at Z = 1, 2, 3 (each peptide "XK"): pins each table entry
+# (utils.cpp:41-98) to the bin, C with its static mod
+B1 = {"A": [7204, 3652, 2468], "C": [16103, 8102, 5435], "D": [11603, 5852, 3935],
+      "E": [13004, 6552, 4402], "F": [14807, 7454, 5002], "G": [5802, 2951, 2001],
+      "H": [13806, 6953, 4669], "I": [11409, 5754, 3870], "K": [12910, 6505, 4370],
+      "L": [11409, 5754, 3870], "M": [13204, 6652, 4468], "N": [11505, 5802, 3902],
+      "P": [9805, 4953, 3335], "Q": [12906, 6503, 4369], "R": [15710, 7905, 5304],
+      "S": [8803, 4452, 3001], "T": [10205, 5153, 3468], "V": [10007, 5054, 3403],
+      "W": [18708, 9404, 6303], "Y": [16407, 8253, 5536]}
+
+# Modified ions (utils.cpp:441-499): undivided truncated terms are summed, the
+# mods of every site at or before the ion are added (b: k-th site from the
+# N-terminus takes nibble k; y: k-th site from the C-terminus takes nibble
+# modSeen-1-k), then the sum is integer-divided by Z.  Mod 1 = M-ox (1599),
+# mod 2 = 42.0106 Da (4201).  MAMWWK at Z = 1:
+#   sites {0, 2}, both M-ox (mod_num 0x11): b1 = 13204 + 1599 = 14803,
+#     b2 = 14803 + 7103 = 21906, b3 = 21906 + 13104 + 1599 = 36609;
+#     y4 (covers M2..K) = 65029 + 1599 = 66628, y5 (covers A1..K) = 72132 + 1599 = 73731
+#     (site 0 lies in no y ion).
+#   sites {0, 1} with nibbles 1, 2 (mod_num 0x21): b2 = 14803 + 7103 + 4201 = 26107;
+#     y5 (covers A1..K) = 72132 + 4201 = 76333 -- the C-terminal-first nibble
+#     order puts mod 2, not mod 1, on the first site met from the C-terminus.
+MOD_LADDERS_Z3 = {
+    ("MAMWWK", 0b101, 0x11): [14803, 21906, 36609, 55216, 73823, 7452, 11003, 18355, 27658, 36962,
+                              5001, 7369, 12270, 18472, 24675, 14711, 33318, 51925, 66628, 73731,
+                              7406, 16709, 26013, 33364, 36916, 4971, 11173, 17375, 22276, 24644],
+    ("MAMWWK", 0b011, 0x21): [14803, 26107, 39211, 57818, 76425, 7452, 13104, 19656, 28959, 38263,
+                              5001, 8769, 13137, 19340, 25542, 14711, 33318, 51925, 65029, 76333,
+                              7406, 16709, 26013, 32565, 38217, 4971, 11173, 17375, 21743, 25511],
+}
+TWO_MODS = [("M", 15.9949, 2), ("A", 42.0106, 1)]
+
+
+def _ladder_at(full_z3, L, maxz):
+    """The maxz < 3 layout from the maxz = 3 one (same ions, fewer charges)."""
+    n = L - 1
+    b = [full_z3[z * n:(z + 1) * n] for z in range(maxz)]
+    y = [full_z3[3 * n + z * n:3 * n + (z + 1) * n] for z in range(maxz)]
+    return [v for blk in b + y for v in blk]
+
+
+def _ladder_checks(gen):
+    """Every hand-traced ladder against `gen(seq, maxz, sites, mod_num)`; returns
+    the number of mismatching bins."""
+    bad = 0
+    for maxz in (1, 2, 3):
+        for seq, full in LADDERS_Z3.items():
+            bad += sum(int(a != b) for a, b in zip(gen(seq, maxz, 0, 0), _ladder_at(full, len(seq), maxz)))
+        for c, b1 in B1.items():
+            got = gen(c + "K", maxz, 0, 0)
+            bad += sum(int(got[z] != b1[z]) for z in range(maxz))
+        for (seq, sites, mn), full in MOD_LADDERS_Z3.items():
+            bad += sum(int(a != b) for a, b in zip(gen(seq, maxz, sites, mn), _ladder_at(full, len(seq), maxz)))
+    return bad
+
+
+def _oracle_gen(lib):
+    import ctypes as C
+
+    def gen(seq, maxz, sites, mod_num):
+        p = params(maxz=maxz, mods=TWO_MODS)
+        b = seq.encode()
+        out = np.zeros(2 * maxz * (len(b) - 1), np.uint32)
+        if mod_num == 0:
+            lib.orc_gen_spectrum(b, C.c_uint32(len(b)), C.byref(p), N.ptr(out))
+        else:
+            e = np.zeros(1, N.PEP_DTYPE)
+            e["sites"], e["mod_num"] = sites, mod_num
+            lib.orc_gen_mod_spectrum(b, C.c_uint32(len(b)), N.ptr(e), C.byref(p), N.ptr(out))
+        return [int(v) for v in out]
+    return gen
+
+
+def test_fragment_ladders_hand_traced():
+    assert _ladder_checks(_oracle_gen(O.lib())) == 0
+
+
+@pytest.mark.parametrize("mutant", ["FLOAT_PROTON", "SUM_THEN_TRUNC", "MOD_PER_TERM", "Y_MOD_ORDER",
+                                    "NO_STATMOD"])
+def test_fragment_ladders_reject_planted_mistakes(mutant, tmp_path):
+    """The oracle rebuilt with one planted mistake (ORC_MUTANT_* in
+    oracle/hcx_oracle.c) must fail the hand-traced ladders."""
+    import ctypes as C
+    import shutil
+    import subprocess
+    from pathlib import Path
+
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc is needed to rebuild the oracle")
+    src = Path(O.__file__).resolve().parent / "hcx_oracle.c"
+    so = tmp_path / f"liboracle_{mutant}.so"
+    subprocess.run(["gcc", "-O2", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off", "-std=c11",
+                    f"-DORC_MUTANT_{mutant}", "-I", str(src.parent.parent / "include"), "-o", str(so),
+                    str(src), "-lm"], check=True)
+    lib = C.CDLL(str(so))
+    lib.orc_gen_spectrum.restype = C.c_float
+    lib.orc_gen_mod_spectrum.restype = C.c_float
+    assert _ladder_checks(_oracle_gen(lib)) > 0
+
+
 def test_mod_spectrum_shift():
     # M-oxidation (+15.9949 -> 1599 after (uint)(Da*scale)) at residue 2 of MMGGMR:
     # b ions from residue 2 on and y ions containing residue 2 shift by 1599/z bins
