@@ -42,29 +42,12 @@
 // A spectrum whose maximum is shared by more than one peptide goes to the
 // in-order heap replay below, as the reference's getMax tie rules require.
 //
-// Every LDS pointer is address-space-3 qualified and every global pointer on
-// the scan path address-space-1 qualified, so no access compiles to a flat
-// instruction (a flat load counts against both the vector-memory and the LDS
-// wait counters and would serialise the LDS atomics behind HBM reads).
-#include "hcx_internal.h"
+#include "k_scan_common.h"
 
-#include <type_traits>
 
 namespace hcx {
 namespace {
 
-#define AS_LDS __attribute__((address_space(3)))
-#define AS_GLB __attribute__((address_space(1)))
-typedef AS_LDS uint32_t lu32;
-typedef AS_LDS uint16_t lu16;
-typedef AS_LDS int32_t li32;
-typedef AS_LDS unsigned long long lu64;
-typedef AS_LDS double lf64;
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-typedef AS_LDS u32x4 lu32x4;
-typedef const AS_GLB uint32_t gcu32;
-typedef AS_GLB uint32_t gu32;
-typedef AS_GLB unsigned long long gu64;
 
 constexpr int kBT = (int)kChunkThreads;   // threads per workgroup of the chunked kernels
 constexpr uint32_t kPPT = kProbesPerThread;   // run-table probes per thread
@@ -73,139 +56,7 @@ constexpr uint32_t kPPT = kProbesPerThread;   // run-table probes per thread
 #define HCX_WAVES_PER_SIMD 4
 #endif
 constexpr int kWavesPerSimd = HCX_WAVES_PER_SIMD;
-constexpr uint32_t kNoProbe = 0xFFFFFFFFu;
-#ifndef HCX_GRAB
-#define HCX_GRAB 64
-#endif
-// items taken per queue atomic, at most: a grab of consecutive items is
-// mostly one spectrum's, so fewer grabs mean fewer per-spectrum merges; near
-// the end of the queue grabs shrink so the workgroups finish together
-constexpr uint32_t kGrab = HCX_GRAB;
-constexpr unsigned long long kLow25 = (1ull << 25) - 1;
-// A group's flat entries are walked in segments of kSegEntries; the run
-// starts of the current segment are a bitmap of kSegWords 64-bit words, one
-// per 64-entry tile.
-constexpr uint32_t kSegWords = 256;
-constexpr uint32_t kSegEntries = 64u * kSegWords;
-#ifndef HCX_TILE_U
-#define HCX_TILE_U 8
-#endif
-constexpr uint32_t kTileU = HCX_TILE_U;   // tiles per wavefront step (loads in flight)
-// histogram bins kept in LDS (hyperscores below 51.15); the rare higher
-// bins go to the global histogram directly
-constexpr uint32_t kHistLds = 512;
 
-__device__ __forceinline__ uint32_t ldg(const uint32_t *p, size_t i) { return ((gcu32 *)p)[i]; }
-// raw buffer resource word 3 on gfx9-family parts (gfx950 included): byte
-// offsets, range-checked against num_records
-constexpr int kBufDword3 = 0x00020000;
-__device__ __forceinline__ uint32_t iac_load(__amdgpu_buffer_rsrc_t r, uint32_t entry) {
-    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, (int)(entry << 1), 0, 0);
-}
-__device__ __forceinline__ void lds_add(lu32 *p, uint32_t v) {
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-__device__ __forceinline__ uint32_t lds_add_rtn(lu32 *p, uint32_t v) {
-    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-__device__ __forceinline__ void lds_add64(lu64 *p, unsigned long long v) {
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-__device__ __forceinline__ void lds_and(lu32 *p, uint32_t v) {
-    __hip_atomic_fetch_and(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-__device__ __forceinline__ void lds_max(lu32 *p, uint32_t v) {
-    __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-__device__ __forceinline__ void lds_or(lu32 *p, uint32_t v) {
-    __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-template <typename T>
-__device__ __forceinline__ void glb_add(T *p, T v) {
-    __hip_atomic_fetch_add((AS_GLB T *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void glb_or(uint32_t *p, uint32_t v) {
-    __hip_atomic_fetch_or((gu32 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void glb_max64(unsigned long long *p, unsigned long long v) {
-    __hip_atomic_fetch_max((gu64 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ uint32_t udiv(uint32_t n, uint32_t m, uint32_t s) {
-    if (s == 0) return n;
-    uint32_t t = __umulhi(m, n);
-    return (t + ((n - t) >> 1)) >> (s - 1);
-}
-
-__device__ __forceinline__ uint32_t mass_lb(const float *mass, uint32_t lo, uint32_t hi, float v) {
-    while (lo < hi) {
-        uint32_t m = lo + (hi - lo) / 2;
-        if (mass[m] < v) lo = m + 1;
-        else hi = m;
-    }
-    return lo;
-}
-__device__ __forceinline__ uint32_t mass_ub(const float *mass, uint32_t lo, uint32_t hi, float v) {
-    while (lo < hi) {
-        uint32_t m = lo + (hi - lo) / 2;
-        if (v < mass[m]) hi = m;
-        else lo = m + 1;
-    }
-    return lo;
-}
-__device__ __forceinline__ void dir_range(const uint32_t *mdir, uint32_t D, uint32_t n, float v, uint32_t *lo,
-                                          uint32_t *hi) {
-    if (!(v > 0.0f)) {
-        *lo = 0;
-        *hi = mdir[1];
-        return;
-    }
-    uint32_t d = (v >= (float)(D - 1)) ? D - 1 : (uint32_t)v;
-    *lo = mdir[d];
-    *hi = (d + 1 < D) ? mdir[d + 1] : n;
-}
-// dslim_query.cpp:854-916 as lower/upper bounds over float masses
-__device__ __forceinline__ void window(const LenDesc &L, float prec, double dM, uint32_t D, int32_t *wlo,
-                                       int32_t *whi) {
-    *wlo = 1;
-    *whi = 0;
-    if (L.n == 0 || dM < 0.0 || prec != prec) return;
-    const float p1 = (float)((double)prec - dM);
-    const float p2 = (float)((double)prec + dM);
-    uint32_t lo, hi;
-    dir_range(L.mdir, D, L.n, p1, &lo, &hi);
-    const uint32_t a = mass_lb(L.mass, lo, hi, p1);
-    dir_range(L.mdir, D, L.n, p2, &lo, &hi);
-    const uint32_t b = mass_ub(L.mass, lo, hi, p2);
-    *wlo = (int32_t)a;
-    *whi = (int32_t)b - 1;
-}
-
-// Values every lane of the workgroup holds alike, moved to scalar registers
-// (an LDS or global read lands in a vector register even when uniform).
-__device__ __forceinline__ uint32_t ufl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ int32_t ufl(int32_t v) { return (int32_t)__builtin_amdgcn_readfirstlane((uint32_t)v); }
-template <typename T>
-__device__ __forceinline__ T *ufl(T *p) {
-    const uint64_t v = (uint64_t)(uintptr_t)p;
-    return (T *)(uintptr_t)(((uint64_t)ufl((uint32_t)(v >> 32)) << 32) | ufl((uint32_t)v));
-}
-struct Acc {
-    float best_hs;
-    uint32_t best_cnt;
-    uint32_t best_id;    // length << 25 | peptide id
-    uint32_t best_by;    // bc << 16 | yc
-    uint32_t cpsms, flags;
-    uint32_t nia, nprobe;   // per thread and spectrum: far below 2^32
-};
-
-__device__ __forceinline__ void acc_reset(Acc &a) {
-    a.best_hs = 0.0f;
-    a.best_cnt = 0;
-    a.best_id = a.best_by = 0;
-    a.cpsms = a.flags = 0;
-    a.nia = a.nprobe = 0;
-}
 
 // Byte offsets of the dynamic LDS arrays (host and device agree on them).
 struct Layout {
@@ -314,73 +165,6 @@ __device__ __forceinline__ Smem bind(unsigned char *raw, const Layout &l) {
     return s;
 }
 
-// Inclusive scan over the 64 lanes of a wavefront with DPP moves (no LDS
-// round trips): shifts by 1, 2, 4, 8 within each 16-lane row (lanes shifted
-// in from outside the row read 0), then row 0's last lane is added to row 1
-// and row 2's to row 3 (row_bcast:15), and lane 31's to rows 2 and 3
-// (row_bcast:31).
-__device__ __forceinline__ uint32_t wave_scan(uint32_t x) {
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, true);   // row_shr:1
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, true);   // row_shr:2
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, true);   // row_shr:4
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, true);   // row_shr:8
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);  // row_bcast:15
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);  // row_bcast:31
-    return x;
-}
-
-// Reduction over the 64 lanes of a wavefront, complete in lane 63 (the
-// DPP pattern of wave_scan with any associative op whose identity is 0).
-template <typename Op>
-__device__ __forceinline__ uint32_t wave_reduce_last(uint32_t x, Op op) {
-    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, true));
-    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, true));
-    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, true));
-    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, true));
-    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false));
-    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false));
-    return x;
-}
-__device__ __forceinline__ uint32_t wave_max_last(uint32_t x) {
-    return wave_reduce_last(x, [](uint32_t u, uint32_t v) { return max(u, v); });
-}
-
-// Block-wide exclusive scan where thread t holds tsum, the sum of its own
-// contiguous group; returns t's prefix.  Uses sm.wsum; one barrier.
-__device__ __forceinline__ uint32_t block_prefix(const Smem &sm, uint32_t tsum) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint32_t x = wave_scan(tsum);
-    if (lane == 63) sm.wsum[w] = x;
-    __syncthreads();
-    uint32_t run = x - tsum;
-    for (int ww = 0; ww < w; ww++) run += sm.wsum[ww];
-    return run;
-}
-
-// Block-wide exclusive scan of two values at once (thread t holds the sums
-// of its own contiguous group); returns t's prefixes, totals in *tot.
-__device__ __forceinline__ uint2 block_prefix2(const Smem &sm, uint32_t a0, uint32_t a1, uint2 *tot) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    constexpr int NW = kBT / 64;
-    const uint32_t x = wave_scan(a0), y = wave_scan(a1);
-    if (lane == 63) {
-        sm.wsum[w] = x;
-        sm.wsum[NW + w] = y;
-    }
-    __syncthreads();
-    uint32_t rx = x - a0, ry = y - a1, tx = 0, ty = 0;
-    for (int ww = 0; ww < NW; ww++) {
-        const uint32_t sx = sm.wsum[ww], sy = sm.wsum[NW + ww];
-        if (ww < w) {
-            rx += sx;
-            ry += sy;
-        }
-        tx += sx;
-        ty += sy;
-    }
-    *tot = make_uint2(tx, ty);
-    return make_uint2(rx, ry);
-}
 
 // Run table of the probes [g0, g1) of one item (at most kPPT * kBT probes):
 // each (peak, bin) probe's run [bA2[c], bA2[c+1]) in chunk c; the non-empty
@@ -418,7 +202,7 @@ __device__ __forceinline__ uint2 setup_runs(const ChunkArgs &a, const Smem &sm, 
     // the bitmap is cleared before the scan's barrier and filled after it
     for (uint32_t w = threadIdx.x; w < 2 * kSegWords; w += kBT) sm.seg32[w] = 0;
     uint2 tot;
-    const uint2 pre = block_prefix2(sm, nr, ne, &tot);
+    const uint2 pre = block_prefix2<kBT>(sm.wsum, nr, ne, &tot);
     uint32_t ci = pre.x, fo = pre.y;
 #pragma unroll
     for (uint32_t h = 0; h < kPPT; h++) {
@@ -439,117 +223,7 @@ __device__ __forceinline__ uint2 setup_runs(const ChunkArgs &a, const Smem &sm, 
     return make_uint2(ufl(tot.y), ufl(tot.x));
 }
 
-// The bitmap for segment [s0, s0 + kSegEntries) of the current run table.
-__device__ __forceinline__ void build_seg(const Smem &sm, uint32_t nc, uint32_t s0) {
-    __syncthreads();
-    for (uint32_t w = threadIdx.x; w < 2 * kSegWords; w += kBT) sm.seg32[w] = 0;
-    __syncthreads();
-    for (uint32_t r = threadIdx.x; r < nc; r += kBT) {
-        const uint32_t f = sm.cofs[r + 1] - 1 - s0;
-        if (f < kSegEntries) lds_or(&sm.seg32[f >> 5], 1u << (f & 31u));
-    }
-    __syncthreads();
-}
 
-// The run holding flat index x: the number of runs that end before x.
-__device__ __forceinline__ uint32_t run_at(const Smem &sm, uint32_t nc, uint32_t x) {
-    uint32_t lo = 0, hi = nc;
-    while (lo < hi) {
-        const uint32_t m = (lo + hi) >> 1;
-        if (sm.cofs[m + 1] <= x) lo = m + 1;
-        else hi = m;
-    }
-    return lo;
-}
-
-// Walk the flat entries [s0, se) of the segment in the bitmap: wavefront w
-// takes a contiguous share of its 64-entry tiles, kTileU tiles per step, lane
-// j of a tile the entry F + j.  The entry's run is the number of runs that
-// end before it: the count before the tile (C, carried from tile to tile)
-// plus the tile word's bits below j -- t.C + mbcnt(t.m), at
-// most nc.  Per step, load(t, x, ok, v, fs) fills v[] for the kTileU tiles
-// (ok[u]: x[u] is in the wavefront's share; fs: the step's first flat index)
-// and body(t, v, x, ok, fs) consumes them, so that each phase's memory
-// operations of the kTileU tiles issue back to back.
-struct Tile {
-    unsigned long long m;   // run ends in the tile, uniform
-    uint32_t C;             // runs ending before the tile, uniform
-};
-__device__ __forceinline__ uint32_t tile_below(const Tile &t) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(t.m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)t.m, 0u));
-}
-// arr[run of this lane's entry]: the tile's base run is uniform, so the lane
-// part is a scaled add to a scalar base
-__device__ __forceinline__ uint32_t tile_at(const lu32 *arr, const Tile &t) { return (arr + t.C)[tile_below(t)]; }
-template <typename Load, typename Body>
-__device__ __forceinline__ void walk_tiles(const Smem &sm, uint32_t nc, uint32_t s0, uint32_t se, Load &&load,
-                                           Body &&body) {
-    constexpr uint32_t NW = kBT / 64;
-    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-    const uint32_t T = (se - s0 + 63) >> 6;
-    const uint32_t k0 = ufl(T * w / NW), k1 = ufl(T * (w + 1) / NW);
-    if (k0 >= k1) return;
-    const uint32_t xe = min(se, s0 + 64 * k1);   // end of this wavefront's share
-    // runs ending before the share: in segment 0, the bitmap words before it
-    // (popcounts summed across the wavefront), else a binary search
-    uint32_t C;
-    if (s0 == 0) {
-        uint32_t cnt = 0;
-        for (uint32_t i = lane; i < k0; i += 64) cnt += (uint32_t)__popcll(sm.seg64[i]);
-        C = ufl((uint32_t)__builtin_amdgcn_readlane((int)wave_scan(cnt), 63));
-    } else {
-        C = ufl(run_at(sm, nc, s0 + 64 * k0));
-    }
-    // a step whose kTileU tiles lie wholly in the share (all but the last)
-    // runs with ok known true
-    auto step = [&](uint32_t k, auto whole_tag) {
-        constexpr bool WHOLE = decltype(whole_tag)::value;
-        Tile t[kTileU];
-        uint32_t x[kTileU], v[kTileU];
-        bool ok[kTileU];
-        unsigned long long mv[kTileU];
-#pragma unroll
-        for (uint32_t u = 0; u < kTileU; u++) mv[u] = sm.seg64[WHOLE ? k + u : min(k + u, kSegWords - 1)];
-        const uint32_t x0 = s0 + 64 * k + lane;
-#pragma unroll
-        for (uint32_t u = 0; u < kTileU; u++) {
-            unsigned long long m = ((unsigned long long)ufl((uint32_t)(mv[u] >> 32)) << 32) | ufl((uint32_t)mv[u]);
-            if (!WHOLE) m = k + u < k1 ? m : 0ull;
-            t[u].m = m;
-            t[u].C = C;
-            C += (uint32_t)__popcll(m);
-            x[u] = x0 + 64 * u;
-            ok[u] = WHOLE ? true : x[u] < xe;
-        }
-        load(t, x, ok, v, s0 + 64 * k);
-        body(t, v, x, ok, s0 + 64 * k);
-    };
-    uint32_t k = k0;
-    for (; k + kTileU <= k1 && s0 + 64 * (k + kTileU) <= xe; k += kTileU) step(k, std::true_type{});
-    for (; k < k1; k += kTileU) step(k, std::false_type{});
-}
-
-__device__ __forceinline__ LenDesc load_len(const ChunkArgs &a, int ixx) { return a.q.lens[ixx]; }
-
-// the descriptor of length ixx, ixx uniform
-__device__ __forceinline__ LenDesc load_len_u(const ChunkArgs &a, uint32_t ixx) {
-    LenDesc L = a.q.lens[ufl(ixx)];
-    L.mass = ufl(L.mass);
-    L.iA = ufl(L.iA);
-    L.bA = ufl(L.bA);
-    L.mdir = ufl(L.mdir);
-    L.n = ufl(L.n);
-    L.speclen = ufl(L.speclen);
-    L.half = ufl(L.half);
-    L.peplen = ufl(L.peplen);
-    L.div_m = ufl(L.div_m);
-    L.div_s = ufl(L.div_s);
-    L.nch = ufl(L.nch);
-    L.cbase = ufl(L.cbase);
-    L.bA2 = ufl(L.bA2);
-    L.iAc = ufl(L.iAc);
-    return L;
-}
 
 // ---- the reference top-K heap (minheap.h:108-177, 258-277), in LDS ----
 struct Cell {
@@ -668,11 +342,11 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
             lu16 *hl = sm.hl + hl_n;
             for (uint32_t s0 = 0; s0 < E; s0 += kSegEntries) {
                 if (s0 != seg_cur) {
-                    build_seg(sm, nc, s0);
+                    build_seg<kBT>(sm.seg32, sm.cofs, nc, s0);
                     seg_cur = s0;
                 }
-                walk_tiles(
-                    sm, nc, s0, min(E, s0 + kSegEntries),
+                walk_tiles<kBT>(
+                    sm.seg64, sm.cofs, nc, s0, min(E, s0 + kSegEntries),
                     [&](const Tile (&t)[kTileU], const uint32_t (&x)[kTileU], const bool (&)[kTileU],
                         uint32_t (&v)[kTileU], uint32_t) {
                         uint32_t gi[kTileU];
@@ -731,7 +405,7 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
         uint32_t tsum = 0;
         for (uint32_t bw = t0; bw < t1; bw++) tsum += __popc(sm.bmp[bw]);
         for (uint32_t i = threadIdx.x; i < TC / 16; i += kBT) ((lu32x4 *)sm.map)[i] = (u32x4){0u, 0u, 0u, 0u};
-        uint32_t run = block_prefix(sm, tsum);
+        uint32_t run = block_prefix<kBT>(sm.wsum, tsum);
         for (uint32_t bw = t0; bw < t1; bw++) {
             const uint32_t cnt = __popc(sm.bmp[bw]);
             sm.pre[bw] = run;
@@ -777,11 +451,11 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
             const lu16 *hl = sm.hl + hb;
             for (uint32_t s0 = 0; s0 < E; s0 += kSegEntries) {
                 if (s0 != seg_cur) {
-                    build_seg(sm, nc, s0);
+                    build_seg<kBT>(sm.seg32, sm.cofs, nc, s0);
                     seg_cur = s0;
                 }
-                walk_tiles(
-                    sm, nc, s0, min(E, s0 + kSegEntries),
+                walk_tiles<kBT>(
+                    sm.seg64, sm.cofs, nc, s0, min(E, s0 + kSegEntries),
                     [&](const Tile (&t)[kTileU], const uint32_t (&x)[kTileU], const bool (&ok)[kTileU],
                         uint32_t (&v)[kTileU], uint32_t fs) {
                         if (fs + 64 * kTileU <= lim) {   // the whole step is in the hit list
@@ -917,88 +591,6 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
         for (uint32_t bw = threadIdx.x; bw < (TC >> 5); bw += kBT) sm.bmp[bw] = 0;
 }
 
-// Merge a block's accumulators for spectrum s into the global SpecAcc.
-// s_red: [0] max bits, [1] count at max, [2] cpsms, [3] flags; s_red64: n_ia, n_probe
-__device__ __forceinline__ void flush_spectrum(const ChunkArgs &a, const Smem &sm, uint32_t s, Acc &acc, lu32 *s_red,
-                                               lu64 *s_red64) {
-    if (threadIdx.x == 0) {
-        s_red[0] = 0;
-        s_red[1] = 0;
-        s_red[2] = 0;
-        s_red[3] = 0;
-        s_red64[0] = 0;
-        s_red64[1] = 0;
-    }
-    __syncthreads();
-    // wavefront reductions first: one LDS atomic per wavefront and field
-    // (same-address atomics serialise)
-    {
-        auto add = [](uint32_t u, uint32_t v) { return u + v; };
-        const uint32_t hm = wave_max_last(acc.best_cnt ? __float_as_uint(acc.best_hs) : 0u);
-        const uint32_t cp = wave_reduce_last(acc.cpsms, add);
-        const uint32_t fl = wave_reduce_last(acc.flags, [](uint32_t u, uint32_t v) { return u | v; });
-        const uint32_t ni = wave_reduce_last(acc.nia, add);
-        const uint32_t np = wave_reduce_last(acc.nprobe, add);
-        if ((threadIdx.x & 63u) == 63u) {
-            if (hm) lds_max(&s_red[0], hm);
-            if (cp) lds_add(&s_red[2], cp);
-            if (fl) lds_or(&s_red[3], fl);
-            if (ni) lds_add64(&s_red64[0], (unsigned long long)ni);
-            if (np) lds_add64(&s_red64[1], (unsigned long long)np);
-        }
-    }
-    __syncthreads();
-    SpecAcc *g = a.acc + s;
-    const uint32_t mb = s_red[0];
-    if (acc.best_cnt && __float_as_uint(acc.best_hs) == mb) {
-        lds_add(&s_red[1], acc.best_cnt);
-        glb_max64(&g->top_a, ((unsigned long long)mb << 32) | acc.best_id);
-        glb_max64(&g->top_b, ((unsigned long long)mb << 32) | acc.best_by);
-    }
-    // histogram: non-zero bins to global, then clear
-    uint32_t *hg = a.q.hist + (size_t)s * a.q.hbins;
-    for (uint32_t h = threadIdx.x; h < min(a.q.hbins, kHistLds); h += kBT) {
-        const uint32_t v = sm.hist[h];
-        if (v) {
-            glb_add(&hg[h], v);
-            sm.hist[h] = 0;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        if (s_red[2]) glb_add(&g->cpsms, (uint32_t)s_red[2]);
-        if (s_red[3]) glb_or(&g->flags, s_red[3]);
-        if (s_red64[0]) glb_add(&g->n_ia, (unsigned long long)s_red64[0]);
-        if (s_red64[1]) glb_add(&g->n_probe, (unsigned long long)s_red64[1]);
-        const uint32_t cnt = s_red[1];
-        if (cnt) {
-            gu64 *tn = (gu64 *)&g->top_n;
-            const unsigned long long mine = ((unsigned long long)mb << 32) | cnt;
-            unsigned long long old = *tn;
-            for (;;) {
-                const uint32_t oh = (uint32_t)(old >> 32);
-                if (mb < oh) break;
-                const unsigned long long want = (mb > oh) ? mine : old + cnt;
-                if (__hip_atomic_compare_exchange_strong(tn, &old, want, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                         __HIP_MEMORY_SCOPE_AGENT))
-                    break;
-            }
-        }
-    }
-    __syncthreads();
-    acc_reset(acc);
-}
-
-__device__ __forceinline__ uint32_t load_peaks(const ChunkArgs &a, const Smem &sm, uint32_t s) {
-    gcu32 *qidx = (gcu32 *)a.q.qidx, *moz = (gcu32 *)a.q.moz, *inten = (gcu32 *)a.q.inten;
-    const uint32_t q0 = qidx[s];
-    const uint32_t P = ufl(qidx[s + 1] - q0);
-    for (uint32_t k = threadIdx.x; k < P; k += kBT) {
-        sm.pk_moz[k] = moz[q0 + k];
-        sm.pk_int[k] = inten[q0 + k];
-    }
-    return P;
-}
 
 __device__ __forceinline__ void init_smem(const ChunkArgs &a, const Smem &sm) {
     for (uint32_t i = threadIdx.x; i <= a.q.max_shdpeaks; i += kBT) sm.lgf[i] = a.q.lgf[i];
@@ -1031,7 +623,7 @@ __global__ __launch_bounds__(kBT, kWavesPerSimd) void k_spec_chunked(ChunkArgs a
         s_ctl[0] = 0;
     }
     const uint32_t s = blockIdx.x;
-    const uint32_t P = load_peaks(a, sm, s);
+    const uint32_t P = load_peaks<kBT>(a, sm.pk_moz, sm.pk_int, s);
     const uint32_t D = a.q.max_mass + 2;
     const float prec = ((const AS_GLB float *)a.q.prec)[s];
     for (int l = threadIdx.x; l < a.q.nlen; l += kBT) {
@@ -1050,7 +642,7 @@ __global__ __launch_bounds__(kBT, kWavesPerSimd) void k_spec_chunked(ChunkArgs a
         for (uint32_t c = (uint32_t)lo / a.tc; c <= (uint32_t)hi / a.tc; c++)
             process_item<false, WIDE>(a, sm, s_ctl, s, nullptr, P, ixx, c, lo, hi, acc);
     }
-    flush_spectrum(a, sm, s, acc, s_red, s_red64);
+    flush_spectrum<kBT>(a, sm.hist, s, acc, s_red, s_red64);
 }
 
 // Batched form: persistent workgroups pull (spectrum, length, chunk) items.
@@ -1091,9 +683,9 @@ __global__ __launch_bounds__(kBT, kWavesPerSimd) void k_items_chunked(ChunkArgs 
             item.ixx = (uint16_t)ufl((uint32_t)item.ixx);
             item.chunk = (uint16_t)ufl((uint32_t)item.chunk);
             if (item.spec != cur) {
-                if (cur != 0xFFFFFFFFu) flush_spectrum(a, sm, cur, acc, s_red, s_red64);
+                if (cur != 0xFFFFFFFFu) flush_spectrum<kBT>(a, sm.hist, cur, acc, s_red, s_red64);
                 cur = item.spec;
-                P = load_peaks(a, sm, cur);
+                P = load_peaks<kBT>(a, sm.pk_moz, sm.pk_int, cur);
                 __syncthreads();
             }
             process_item<false, WIDE>(a, sm, s_ctl, cur, rec ? sm.imax + (it - it0) : nullptr, P, item.ixx,
@@ -1101,7 +693,7 @@ __global__ __launch_bounds__(kBT, kWavesPerSimd) void k_items_chunked(ChunkArgs 
         }
         if (rec && threadIdx.x < it1 - it0) ((AS_GLB uint32_t *)a.item_max)[it0 + threadIdx.x] = sm.imax[threadIdx.x];
     }
-    if (cur != 0xFFFFFFFFu) flush_spectrum(a, sm, cur, acc, s_red, s_red64);
+    if (cur != 0xFFFFFFFFu) flush_spectrum<kBT>(a, sm.hist, cur, acc, s_red, s_red64);
 }
 
 // Tie replay: spectra whose maximum hyperscore is shared by several peptides.
@@ -1131,7 +723,7 @@ __global__ __launch_bounds__(kBT) void k_replay_chunked(ChunkArgs a) {
         if (wi >= nwork) break;
         const uint32_t s = ((const AS_GLB uint32_t *)a.q.tie)[1 + wi];
         __syncthreads();
-        const uint32_t P = load_peaks(a, sm, s);
+        const uint32_t P = load_peaks<kBT>(a, sm.pk_moz, sm.pk_int, s);
         const float prec = ((const AS_GLB float *)a.q.prec)[s];
         for (int l = threadIdx.x; l < a.q.nlen; l += kBT) {
             const LenDesc L = load_len(a, l);
