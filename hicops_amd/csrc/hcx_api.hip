@@ -104,7 +104,8 @@ extern "C" int hcx_create(const hcx_params *params, int device, hcx_ctx **out) {
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail("stream");
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess ||
-        hipEventCreate(&c->ev4) != hipSuccess || hipEventCreate(&c->ev5) != hipSuccess)
+        hipEventCreate(&c->ev4) != hipSuccess || hipEventCreate(&c->ev5) != hipSuccess ||
+        hipEventCreate(&c->ev6) != hipSuccess)
         return fail("events");
     // utils.h:115-131 lgfact table: val[i] = log10(i) + val[i-1]
     c->h_lgf.assign(p.max_shdpeaks + 1, 0.0);
@@ -135,6 +136,7 @@ extern "C" int hcx_destroy(hcx_ctx *c) {
     if (c->ev3) (void)hipEventDestroy(c->ev3);
     if (c->ev4) (void)hipEventDestroy(c->ev4);
     if (c->ev5) (void)hipEventDestroy(c->ev5);
+    if (c->ev6) (void)hipEventDestroy(c->ev6);
     for (hipStream_t a : c->aux)
         if (a) (void)hipStreamDestroy(a);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -166,9 +168,14 @@ extern "C" int hcx_set_option(hcx_ctx *c, int32_t option, int64_t value) {
     case HCX_OPT_HITLIST:
         c->hitlist = (int32_t)std::max<int64_t>(-1, std::min<int64_t>(value, 1 << 24));
         return HCX_SUCCESS;
-    case HCX_OPT_ITEM_ORDER:
-        if (value != 0 && value != 1) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_set_option: item order is 0 or 1");
-        if (c->item_order != (int32_t)value) c->item_order = (int32_t)value, ++c->item_gen;
+    case HCX_OPT_SCAN:
+        if (value != HCX_SCAN_AUTO && value != HCX_SCAN_TWO_PASS)
+            return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_set_option: scan is HCX_SCAN_AUTO or HCX_SCAN_TWO_PASS");
+        c->scan_mode = (int32_t)value;
+        return HCX_SUCCESS;
+    case HCX_OPT_GRAB:
+        if (value < 1 || value > (int64_t)kMaxGrab) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_set_option: grab in 1..256");
+        c->grab = (uint32_t)value;
         return HCX_SUCCESS;
     case HCX_OPT_GROUP:
         if (value != 0 && (value < 64 || value > (int64_t)(kProbesPerThread * kChunkThreads)))
@@ -184,7 +191,8 @@ extern "C" int hcx_get_option(const hcx_ctx *c, int32_t option, int64_t *value) 
     if (!c || !value) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_get_option: null argument");
     switch (option) {
     case HCX_OPT_HITLIST: *value = c->hitlist; return HCX_SUCCESS;
-    case HCX_OPT_ITEM_ORDER: *value = c->item_order; return HCX_SUCCESS;
+    case HCX_OPT_SCAN: *value = c->scan_mode; return HCX_SUCCESS;
+    case HCX_OPT_GRAB: *value = c->grab; return HCX_SUCCESS;
     case HCX_OPT_GROUP: *value = c->group; return HCX_SUCCESS;
     default: return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_get_option: unknown option");
     }
@@ -365,6 +373,7 @@ extern "C" int hcx_queries_upload(hcx_ctx *c, const hcx_queries_view *q, hcx_qba
     // largest number of one spectrum's peaks inside any 2*dF+1 bins: bounds
     // how often one fragment bin can be probed
     uint32_t mult = 0;
+    std::vector<uint32_t> mult_s((size_t)std::max(n, 0));
     {
         std::vector<uint32_t> t;
         const uint32_t span = 2 * c->params.dF;
@@ -372,10 +381,13 @@ extern "C" int hcx_queries_upload(hcx_ctx *c, const hcx_queries_view *q, hcx_qba
             t.assign(q->moz + q->idx[s], q->moz + q->idx[s + 1]);
             std::sort(t.begin(), t.end());
             size_t i = 0;
+            uint32_t m = 0;
             for (size_t j = 0; j < t.size(); j++) {
                 while (t[j] - t[i] > span) i++;
-                mult = std::max<uint32_t>(mult, (uint32_t)(j - i + 1));
+                m = std::max<uint32_t>(m, (uint32_t)(j - i + 1));
             }
+            mult_s[s] = m;
+            mult = std::max(mult, m);
         }
     }
     HCX_HIP(hipSetDevice(c->device));
@@ -389,7 +401,8 @@ extern "C" int hcx_queries_upload(hcx_ctx *c, const hcx_queries_view *q, hcx_qba
     b->max_mult = mult;
     b->max_int = maxi;
     if (b->moz.ensure(4 * npk + 16) || b->inten.ensure(4 * npk + 16) || b->idx.ensure(4ull * (n + 1)) ||
-        b->prec.ensure(4ull * n + 16) || b->chg.ensure(4ull * n + 16) || b->rt.ensure(4ull * n + 16)) {
+        b->prec.ensure(4ull * n + 16) || b->chg.ensure(4ull * n + 16) || b->rt.ensure(4ull * n + 16) ||
+        b->mult.ensure(4ull * n + 16)) {
         delete b;
         return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_queries_upload: device allocation failed");
     }
@@ -401,6 +414,7 @@ extern "C" int hcx_queries_upload(hcx_ctx *c, const hcx_queries_view *q, hcx_qba
     const bool zero_rt = !q->rtimes && n > 0;
     if (up(b->moz, q->moz, 4 * npk) || up(b->inten, q->intensity, 4 * npk) || up(b->idx, q->idx, 4ull * (n + 1)) ||
         up(b->prec, q->precurse, 4ull * n) || up(b->chg, q->charges, 4ull * n) || up(b->rt, q->rtimes, 4ull * n) ||
+        up(b->mult, mult_s.data(), 4ull * n) ||
         (zero_rt && hipMemsetAsync(b->rt.p, 0, 4ull * n, c->stream) != hipSuccess) ||
         hipStreamSynchronize(c->stream) != hipSuccess) {
         delete b;
@@ -504,15 +518,6 @@ int prepare(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, const h
     return HCX_SUCCESS;
 }
 
-uint64_t items_key(hcx_index *const *idx, int nidx, uint32_t chunk, uint32_t gen) {
-    uint64_t h = (1469598103934665603ull ^ chunk) * 1099511628211ull ^ gen;
-    for (int i = 0; i < nidx; i++) {
-        h ^= (uint64_t)(uintptr_t)idx[i];
-        h *= 1099511628211ull;
-    }
-    return h ^ (uint64_t)nidx;
-}
-
 uint64_t fnv1a(const void *p, size_t n, uint64_t h) {
     const unsigned char *b = (const unsigned char *)p;
     h ^= 1469598103934665603ull;
@@ -523,13 +528,26 @@ uint64_t fnv1a(const void *p, size_t n, uint64_t h) {
     return h;
 }
 
+// One search's scan plan: the two-pass kernel's arguments (ca), the packed
+// kernel's (pa: the same apart from its run-table group), the window kernel's
+// outputs, and which kernels run.
+struct ScanPlan {
+    ChunkArgs ca, pa;
+    WinArgs wa;
+    int kernel = HCX_KERNEL_PLAIN;
+    bool pack = false;       // packed kernel for the spectra within mult_max
+};
+
 // The device work of one search, all on the context stream with no host
 // synchronisation: zero the per-spectrum state with fill kernels (a memset
-// node is not used, so the sequence can be captured into a graph), scan,
-// replay tied spectra, tail fit.  ev0/ev1 bracket the scan kernel.
-int launch_sequence(hcx_ctx *c, ChunkArgs &ca, int kernel, bool tailfit = true) {
+// node is not used, so the sequence can be captured into a graph), windows
+// and work items, scan, replay tied spectra, tail fit.  ev0 / ev6 / ev1
+// bracket the packed and the two-pass scan kernels.
+int launch_sequence(hcx_ctx *c, ScanPlan &sp, bool tailfit = true) {
+    ChunkArgs &ca = sp.ca;
     QueryArgs &a = ca.q;
     const uint64_t n = (uint64_t)a.nspec;
+    const bool batched = sp.kernel == HCX_KERNEL_BATCHED;
     int rc = launch_fill_u32(c, (uint32_t *)ca.acc, n * (sizeof(SpecAcc) / 4), 0);
     if (!rc) rc = launch_fill_u32(c, a.hist, n * c->hbins, 0);
     if (!rc) rc = launch_fill_u32(c, a.tie, 1, 0);
@@ -537,13 +555,39 @@ int launch_sequence(hcx_ctx *c, ChunkArgs &ca, int kernel, bool tailfit = true) 
     if (!rc) rc = launch_fill_u32(c, a.tie + a.nspec + 1, 1, 0);
     if (!rc) rc = launch_fill_u32(c, ca.queue, 1, 0);
     if (rc) return rc;
-    if (kernel == HCX_KERNEL_CURSOR)
+    if (sp.kernel == HCX_KERNEL_CURSOR) {
+        if (!c->capturing) HCX_HIP(hipEventRecord(c->ev0, c->stream));
+        if (!c->capturing) HCX_HIP(hipEventRecord(c->ev6, c->stream));
         rc = launch_cursor(c, a);          // writes results + tie list itself
-    else
-        rc = launch_chunked(c, ca, kernel);
-    if (rc) return rc;
-    // the cursor path keeps its own (independent) replay; the chunked forms share one
-    rc = kernel == HCX_KERNEL_CURSOR ? launch_replay_cursor(c, a) : launch_replay_chunked(c, ca);
+        if (!rc && !c->capturing) HCX_HIP(hipEventRecord(c->ev1, c->stream));
+        if (!rc) rc = launch_replay_cursor(c, a);   // its own (independent) replay
+    } else {
+        rc = launch_fill_u32(c, sp.wa.fb_count, 1, 0);
+        if (!rc && batched) rc = launch_fill_u32(c, sp.wa.counts + n, 1, 0);
+        if (!rc) rc = launch_windows(c, ca, sp.wa);
+        if (!rc && batched) {
+            rc = scan_exclusive_u32(c->stream, sp.wa.counts, (uint32_t *)ca.item_base, n + 1, (uint32_t *)c->scan_tmp.p);
+            if (!rc) rc = launch_item_spec(c, ca.item_base, a.nspec, (uint32_t *)ca.item_spec);
+        }
+        if (rc) return rc;
+        if (!c->capturing) HCX_HIP(hipEventRecord(c->ev0, c->stream));
+        if (sp.pack) rc = launch_packed(c, sp.pa, sp.kernel);
+        if (!rc && !c->capturing) HCX_HIP(hipEventRecord(c->ev6, c->stream));
+        if (!rc) {
+            if (sp.pack) {
+                // the rest, one spectrum at a time from the list
+                ChunkArgs fa = ca;
+                fa.fb_list = sp.wa.fb_list;
+                fa.fb_count = sp.wa.fb_count;
+                rc = launch_chunked(c, fa, HCX_KERNEL_PLAIN);
+            } else {
+                rc = launch_chunked(c, ca, sp.kernel);
+            }
+        }
+        if (!rc && !c->capturing) HCX_HIP(hipEventRecord(c->ev1, c->stream));
+        if (!rc) rc = launch_finalize(c, ca);
+        if (!rc) rc = launch_replay_chunked(c, ca);
+    }
     if (rc || !tailfit) return rc;
     if (!c->capturing) HCX_HIP(hipEventRecord(c->ev4, c->stream));
     rc = launch_tailfit(c, a.res, a.hist, a.hbins, a.nspec, c->params.min_cpsm, c->params.expect_max);
@@ -552,111 +596,165 @@ int launch_sequence(hcx_ctx *c, ChunkArgs &ca, int kernel, bool tailfit = true) 
     return HCX_SUCCESS;
 }
 
-int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, const hcx_search_opts *opts,
-               bool timed, float *kernel_ms, float *total_ms) {
-    HCX_HIP(hipSetDevice(c->device));
-    ChunkArgs ca;
-    std::memset(&ca, 0, sizeof ca);
-    int rc = prepare(c, idx, nidx, qb, opts, &ca.q);
-    if (rc) return rc;
-    QueryArgs &a = ca.q;
-    const bool sur = opts && opts->evalue == HCX_EVAL_SURROGATE;
-    if (opts && opts->evalue != HCX_EVAL_TAILFIT && !sur)
-        return set_error_code(HCX_ERR_INVLD_PARAM, "search: unknown e-value method");
-    if (sur && !c->sur_ready) return set_error_code(HCX_ERR_INVLD_PARAM, "search: the surrogate is not trained");
-    int kernel = opts ? opts->kernel : HCX_KERNEL_PLAIN;
+// Largest per-spectrum peak multiplicity M for which the packed kernel's
+// words cannot overflow: a peptide series takes at most (speclen / 2) * M
+// hits, each adding at most max_int, and the count must stay below 256 and
+// the sum below 2^24.
+uint32_t packed_mult_max(uint32_t max_half, uint32_t max_int) {
+    if (max_half == 0) return 0xFFFFFFFFu;
+    uint64_t m = 255 / max_half;
+    if (max_int) m = std::min<uint64_t>(m, ((1ull << 24) - 1) / ((uint64_t)max_half * max_int));
+    return (uint32_t)m;
+}
+
+// Fill sp for a search of qb over idx (prepare() done): layouts, the scan
+// plan and every device buffer the sequence uses.
+int plan_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, int kernel, ScanPlan &sp) {
+    ChunkArgs &ca = sp.ca;
     if (c->params.min_shp > kMaxChunkShp) kernel = HCX_KERNEL_CURSOR;
+    sp.kernel = kernel;
     const uint64_t n = (uint64_t)qb->nspec;
     ca.tc = c->chunk;
     ca.slots = c->slots;
     ca.max_peaks = std::max<uint32_t>(1, qb->max_peaks);
+    ca.grab = c->grab;
     make_divider(2 * c->params.dF + 1, &ca.wdiv_m, &ca.wdiv_s);
+    uint32_t maxspec = 0;
+    for (int i = 0; i < nidx; i++) maxspec = std::max(maxspec, idx[i]->speclen);
     // 32-bit slot sums when no peptide's sum can reach 2^32: at most
     // speclen * max_mult hits per peptide, each at most the batch's largest
     // intensity
     {
-        uint32_t maxspec = 0;
-        for (int i = 0; i < nidx; i++) maxspec = std::max(maxspec, idx[i]->speclen);
         const uint64_t hits = (uint64_t)maxspec * std::max<uint32_t>(1, qb->max_mult);
         ca.wide = (uint64_t)qb->max_int * hits >= (1ull << 32) ? 1u : 0u;
     }
     ca.hl_cap = 0;
+    const uint32_t need = std::max<uint32_t>(1, ca.max_peaks * (2 * c->params.dF + 1));
     {
         const size_t share = (size_t)c->max_lds / std::max<uint32_t>(1, c->blocks_per_cu);
-        // the kernels' LDS is all in the layout; the replay kernel sizes its
-        // own hit list
-        const size_t reserve = 0;
         // run-table group: an item walks its probes group by group, and every
         // extra group costs the exact-count pass another run setup, so by
         // default the group is as large as the workgroup's LDS share allows
         // (12 bytes per probe), but not below 1024 probes
-        const uint32_t need = std::max<uint32_t>(1, ca.max_peaks * (2 * c->params.dF + 1));
         if (c->group) {
             ca.group = std::min<uint32_t>(c->group, need);
         } else {
             const uint32_t floor = std::min<uint32_t>(need, 1024);
             ca.group = std::min<uint32_t>(need, kProbesPerThread * kChunkThreads);
-            while (ca.group > floor && chunk_smem_bytes(ca) + reserve > share)
-                ca.group = std::max<uint32_t>(floor, ca.group - 128);
+            while (ca.group > floor && chunk_smem_bytes(ca) > share) ca.group = std::max<uint32_t>(floor, ca.group - 128);
         }
         // the hit list (16-bit entries) takes the LDS left in a workgroup's
         // share of the CU once blocks_per_cu workgroups are resident
         const size_t base = chunk_smem_bytes(ca);
-        if (share > base + reserve + 4096) ca.hl_cap = (uint32_t)(((share - base - reserve) / 2) & ~(size_t)63);
+        if (share > base + 4096) ca.hl_cap = (uint32_t)(((share - base) / 2) & ~(size_t)63);
         if (c->hitlist >= 0) ca.hl_cap = std::min<uint32_t>(ca.hl_cap, (uint32_t)c->hitlist & ~63u);
     }
     ca.grid = (uint32_t)std::max(1, c->num_cu) * c->blocks_per_cu;
     if (chunk_smem_bytes(ca) > (size_t)c->max_lds)
         return set_error_code(HCX_ERR_INVLD_PARAM, "search: chunk/slots do not fit in LDS");
     // hits per peptide must fit the 15-bit counters: speclen * peaks within +-dF
-    if (qb->max_mult) {
-        uint32_t maxspec = 0;
-        for (int i = 0; i < nidx; i++) maxspec = std::max(maxspec, idx[i]->speclen);
-        if ((uint64_t)maxspec * qb->max_mult >= 0x8000)
-            return set_error_code(HCX_ERR_INVLD_PARAM, "search: too many peaks within +-dF of one bin");
-    }
-    for (int i = 0; i < nidx; i++) ca.nbk += idx[i]->nch;
-    ca.by_chunk = c->item_order == 1 ? 1u : 0u;
-    if (qb->acc.ensure(sizeof(SpecAcc) * n + 16) || c->work.ensure(64) ||
-        qb->counts.ensure(8 * (std::max<uint64_t>(n, ca.nbk) + 1) + 16) ||
-        c->scan_tmp.ensure(4 * scan_tmp_words(std::max<uint64_t>(n, ca.nbk) + 1) + 64))
+    if ((uint64_t)maxspec * qb->max_mult >= 0x8000)
+        return set_error_code(HCX_ERR_INVLD_PARAM, "search: too many peaks within +-dF of one bin");
+    if (qb->acc.ensure(sizeof(SpecAcc) * n + 16) || c->work.ensure(64))
         return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "search: scratch allocation failed");
     ca.acc = (SpecAcc *)qb->acc.p;
     ca.queue = (uint32_t *)c->work.p;
-    if (kernel == HCX_KERNEL_BATCHED) {
-        const uint64_t key = items_key(idx, nidx, c->chunk, c->item_gen);
-        if (qb->items_key != key || !qb->items.p) {
-            rc = launch_items(c, ca, (uint32_t *)qb->counts.p, (uint32_t *)c->scan_tmp.p, false);
-            if (rc) return rc;
-            uint32_t total = 0;
-            const uint64_t nbuckets = ca.by_chunk ? ca.nbk : n;
-            HCX_HIP(hipMemcpyAsync(&total, (uint32_t *)qb->counts.p + nbuckets, 4, hipMemcpyDeviceToHost, c->stream));
-            HCX_HIP(hipStreamSynchronize(c->stream));
-            if (qb->items.ensure(sizeof(Item) * (size_t)total + 16))
-                return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "search: item list allocation failed");
-            ca.items_w = (Item *)qb->items.p;
-            rc = launch_items(c, ca, (uint32_t *)qb->counts.p, (uint32_t *)c->scan_tmp.p, true);
-            if (rc) return rc;
-            qb->nitems = total;
-            qb->items_key = key;
-        }
-        ca.items = (const Item *)qb->items.p;
-        ca.nitems = qb->nitems;
-        ca.items_w = nullptr;
-        // items listed by spectrum: the scan records each item's best score,
-        // which lets the tie replay skip items the full heap would reject
-        if (!ca.by_chunk) {
-            if (qb->item_max.ensure(4ull * qb->nitems + 16))
-                return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "search: item score allocation failed");
-            ca.item_max = (uint32_t *)qb->item_max.p;
-            ca.item_base = (const uint32_t *)qb->counts.p;
-        }
+    if (kernel == HCX_KERNEL_CURSOR) return HCX_SUCCESS;
+
+    // the packed kernel: the largest run-table group that fits beside the
+    // accumulators (one 1024-thread workgroup per CU)
+    const uint32_t mult_max = packed_mult_max(maxspec / 2, qb->max_int);
+    sp.pack = c->scan_mode == HCX_SCAN_AUTO && mult_max > 0;
+    if (sp.pack) {
+        ChunkArgs &pa = sp.pa;
+        pa = ca;
+        pa.group = std::min<uint32_t>(need, 2 * kPackedThreads);
+        while (pa.group > 64 && (!packed_smem_bytes(pa) || packed_smem_bytes(pa) > (size_t)c->max_lds))
+            pa.group -= 64;
+        sp.pack = packed_smem_bytes(pa) && packed_smem_bytes(pa) <= (size_t)c->max_lds;
+        pa.grid = (uint32_t)std::max(1, c->num_cu);
     }
+    // windows, work-item numbering, the two-pass list
+    uint64_t nch_all = 0;
+    for (int i = 0; i < nidx; i++) nch_all += idx[i]->nch;
+    const uint64_t spec_words = (n * nch_all) / 64 + 2;
+    if (qb->win.ensure(8ull * n * nidx + 16) || qb->packed.ensure(n + 16) || qb->fb.ensure(4 * (n + 1) + 16) ||
+        qb->counts.ensure(4 * (n + 1) + 16) || qb->item_base.ensure(4 * (n + 1) + 16) ||
+        qb->item_spec.ensure(4 * spec_words + 16) || c->scan_tmp.ensure(4 * scan_tmp_words(n + 1) + 64))
+        return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "search: scratch allocation failed");
+    if (!qb->mult.p) return set_error_code(HCX_ERR_INVLD_PARAM, "search: batch without peak multiplicities");
+    if (kernel == HCX_KERNEL_BATCHED && !qb->item_max.p) {
+        // a first estimate; grown to the item total after each search
+        qb->item_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(64 * n, 1024), 0xFFFFFFF0ull);
+        if (qb->item_max.ensure(4ull * qb->item_cap + 16)) return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "search: alloc");
+    }
+    WinArgs &wa = sp.wa;
+    std::memset(&wa, 0, sizeof wa);
+    wa.mult = (const uint32_t *)qb->mult.p;
+    wa.mult_max = mult_max;
+    wa.pack = sp.pack ? 1u : 0u;
+    wa.batched = kernel == HCX_KERNEL_BATCHED ? 1u : 0u;
+    wa.win = (int2 *)qb->win.p;
+    wa.counts = (uint32_t *)qb->counts.p;
+    wa.packed = (uint8_t *)qb->packed.p;
+    wa.fb_list = (uint32_t *)qb->fb.p + 1;
+    wa.fb_count = (uint32_t *)qb->fb.p;
+    wa.acc = ca.acc;
+    ca.win = wa.win;
+    ca.packed = wa.packed;
+    if (kernel == HCX_KERNEL_BATCHED) {
+        ca.item_base = (const uint32_t *)qb->item_base.p;
+        ca.item_spec = (const uint32_t *)qb->item_spec.p;
+        ca.item_max = (uint32_t *)qb->item_max.p;
+        ca.item_cap = qb->item_cap;
+    }
+    ChunkArgs &pa = sp.pa;
+    const uint32_t pgroup = pa.group, pgrid = pa.grid;
+    if (sp.pack) {
+        pa = ca;
+        pa.group = pgroup;
+        pa.grid = pgrid;
+    }
+    return HCX_SUCCESS;
+}
+
+// After a batched search: grow item_max to the search's item total, so that
+// the next search of the batch records (and the replay can skip) every item.
+int grow_item_max(hcx_ctx *c, hcx_qbatch *qb, const ScanPlan &sp) {
+    if (sp.kernel != HCX_KERNEL_BATCHED) return HCX_SUCCESS;
+    uint32_t total = 0;
+    HCX_HIP(hipMemcpyAsync(&total, (const uint32_t *)qb->item_base.p + qb->nspec, 4, hipMemcpyDeviceToHost, c->stream));
+    HCX_HIP(hipStreamSynchronize(c->stream));
+    if (total > qb->item_cap) {
+        qb->item_max.release();
+        if (qb->item_max.ensure(4ull * total + 16)) return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "search: alloc");
+        qb->item_cap = total;
+    }
+    return HCX_SUCCESS;
+}
+
+int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, const hcx_search_opts *opts,
+               bool timed, float *kernel_ms, float *total_ms) {
+    HCX_HIP(hipSetDevice(c->device));
+    ScanPlan sp;
+    std::memset(&sp.ca, 0, sizeof sp.ca);
+    std::memset(&sp.pa, 0, sizeof sp.pa);
+    int rc = prepare(c, idx, nidx, qb, opts, &sp.ca.q);
+    if (rc) return rc;
+    QueryArgs &a = sp.ca.q;
+    const bool sur = opts && opts->evalue == HCX_EVAL_SURROGATE;
+    if (opts && opts->evalue != HCX_EVAL_TAILFIT && !sur)
+        return set_error_code(HCX_ERR_INVLD_PARAM, "search: unknown e-value method");
+    if (sur && !c->sur_ready) return set_error_code(HCX_ERR_INVLD_PARAM, "search: the surrogate is not trained");
+    rc = plan_search(c, idx, nidx, qb, opts ? opts->kernel : HCX_KERNEL_PLAIN, sp);
+    if (rc) return rc;
+    const int kernel = sp.kernel;
+    ChunkArgs &ca = sp.ca;
     if (timed) HCX_HIP(hipEventRecord(c->ev2, c->stream));
     if (sur) {
         // scan, then predicted e-values, the audit sample's plain fits, and
         // the plain fit for everything if the audit finds a disagreement
-        rc = launch_sequence(c, ca, kernel, false);
+        rc = launch_sequence(c, sp, false);
         if (!rc) HCX_HIP(hipEventRecord(c->ev4, c->stream));
         if (!rc) rc = mlp_infer(c, qb);
         if (!rc) HCX_HIP(hipEventRecord(c->ev5, c->stream));
@@ -684,15 +782,18 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
         }
     } else if (opts && opts->use_graph) {
         // one graph per launch shape: every pointer and size the kernels see
-        // is in ca, so its bytes (zero-padded above) key the captured graph
-        const uint64_t key = fnv1a(&ca, sizeof ca, (uint64_t)kernel * 0x9E3779B97F4A7C15ull + 1);
+        // is in the plan, so its bytes (zero-padded above) key the captured
+        // graph
+        uint64_t key = fnv1a(&ca, sizeof ca, (uint64_t)kernel * 0x9E3779B97F4A7C15ull + 1);
+        key = fnv1a(&sp.pa, sizeof sp.pa, key);
+        key = fnv1a(&sp.wa, sizeof sp.wa, key + (sp.pack ? 1 : 0));
         if (!c->graph_exec || c->graph_key != key) {
             if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
             c->graph_exec = nullptr;
             c->graph_key = 0;
             HCX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             c->capturing = true;   // no event records inside the graph
-            rc = launch_sequence(c, ca, kernel);
+            rc = launch_sequence(c, sp);
             c->capturing = false;
             hipGraph_t g = nullptr;
             const hipError_t ec = hipStreamEndCapture(c->stream, &g);
@@ -713,10 +814,11 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
         HCX_HIP(hipEventRecord(c->ev0, c->stream));
         HCX_HIP(hipEventRecord(c->ev4, c->stream));
         HCX_HIP(hipGraphLaunch(c->graph_exec, c->stream));
+        HCX_HIP(hipEventRecord(c->ev6, c->stream));
         HCX_HIP(hipEventRecord(c->ev1, c->stream));
         HCX_HIP(hipEventRecord(c->ev5, c->stream));
     } else {
-        rc = launch_sequence(c, ca, kernel);
+        rc = launch_sequence(c, sp);
         if (rc) return rc;
     }
     if (timed) {
@@ -728,10 +830,12 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
         if (kernel_ms) *kernel_ms = q;
         if (total_ms) *total_ms = t;
         c->timed_ok = true;
+        c->ran_packed = sp.pack;
+        c->ran_two_pass = true;   // the two-pass kernel (or the cursor kernel) is always launched
     } else {
         HCX_HIP(hipStreamSynchronize(c->stream));
     }
-    return HCX_SUCCESS;
+    return grow_item_max(c, qb, sp);
 }
 
 }  // namespace
@@ -746,6 +850,17 @@ extern "C" int hcx_last_timing(hcx_ctx *c, float *scan_ms, float *evalue_ms, flo
     if (scan_ms) *scan_ms = q;
     if (evalue_ms) *evalue_ms = e;
     if (total_ms) *total_ms = t;
+    return HCX_SUCCESS;
+}
+
+extern "C" int hcx_last_scan_split(hcx_ctx *c, float *packed_ms, float *two_pass_ms) {
+    if (!c) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_last_scan_split: null context");
+    if (!c->timed_ok) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_last_scan_split: no timed search yet");
+    float a = 0, b = 0;
+    if (c->ran_packed) HCX_HIP(hipEventElapsedTime(&a, c->ev0, c->ev6));
+    if (c->ran_two_pass) HCX_HIP(hipEventElapsedTime(&b, c->ev6, c->ev1));
+    if (packed_ms) *packed_ms = a;
+    if (two_pass_ms) *two_pass_ms = b;
     return HCX_SUCCESS;
 }
 
@@ -826,52 +941,64 @@ extern "C" int hcx_autotune(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qba
     if (!c || !qb || !best) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_autotune: null argument");
     if (kernel != HCX_KERNEL_BATCHED && kernel != HCX_KERNEL_PLAIN)
         return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_autotune: tunes the batched or plain form");
-    const uint32_t slots0 = c->slots, bpc0 = c->blocks_per_cu;
-    const int32_t order0 = c->item_order;
-    const uint32_t slot_c[] = {512, 1024, 1536, 2048};
-    const uint32_t bpc_c[] = {1, 2, 3};
-    const int norder = kernel == HCX_KERNEL_BATCHED ? 2 : 1;
+    const uint32_t slots0 = c->slots, bpc0 = c->blocks_per_cu, grab0 = c->grab;
+    struct Cand {
+        uint32_t slots, bpc, grab;
+    };
+    std::vector<Cand> cands;
+    const bool batched = kernel == HCX_KERNEL_BATCHED;
+    if (c->scan_mode == HCX_SCAN_AUTO) {
+        // the packed kernel's layout is fixed by the chunk size: the batched
+        // form's grab size is what is left to choose
+        if (batched)
+            for (uint32_t g : {8u, 16u, 32u, 64u, 128u, 256u}) cands.push_back({slots0, bpc0, g});
+        else
+            cands.push_back({slots0, bpc0, grab0});
+    } else {
+        for (uint32_t sl : {512u, 1024u, 1536u, 2048u})
+            for (uint32_t bp : {1u, 2u, 3u}) {
+                if (batched)
+                    for (uint32_t g : {32u, 64u, 128u}) cands.push_back({sl, bp, g});
+                else
+                    cands.push_back({sl, bp, grab0});
+            }
+    }
     hcx_search_opts o = {};
     o.kernel = kernel;
     o.evalue = HCX_EVAL_TAILFIT;
     hcx_tuning b = {};
     b.kernel_ms = -1.0f;
-    for (int order = 0; order < norder; order++) {
-        if (c->item_order != order) c->item_order = order, ++c->item_gen;
-        for (uint32_t sl : slot_c) {
-            for (uint32_t bp : bpc_c) {
-                c->slots = sl;
-                c->blocks_per_cu = bp;
-                float k0 = 0, k1 = 0, t = 0;
-                // the first run builds (and caches) the work items; keep the better
-                int rc = run_search(c, idx, nidx, qb, &o, true, &k0, &t);
-                if (rc == HCX_ERR_INVLD_PARAM) continue;   // does not fit in LDS
-                if (!rc) rc = run_search(c, idx, nidx, qb, &o, true, &k1, &t);
-                if (rc) {
-                    c->slots = slots0, c->blocks_per_cu = bpc0;
-                    if (c->item_order != order0) c->item_order = order0, ++c->item_gen;
-                    return rc;
-                }
-                const float k = std::min(k0, k1);
-                if (b.kernel_ms < 0.0f || k < b.kernel_ms) {
-                    b.chunk = c->chunk;
-                    b.slots = sl;
-                    b.blocks_per_cu = bp;
-                    b.hitlist = c->hitlist;
-                    b.item_order = order;
-                    b.kernel_ms = k;
-                }
-            }
+    auto restore = [&]() { c->slots = slots0, c->blocks_per_cu = bpc0, c->grab = grab0; };
+    for (const Cand &cd : cands) {
+        c->slots = cd.slots;
+        c->blocks_per_cu = cd.bpc;
+        c->grab = cd.grab;
+        float k0 = 0, k1 = 0, t = 0;
+        // two runs (the first also sizes the batch's item records); keep the better
+        int rc = run_search(c, idx, nidx, qb, &o, true, &k0, &t);
+        if (rc == HCX_ERR_INVLD_PARAM) continue;   // does not fit in LDS
+        if (!rc) rc = run_search(c, idx, nidx, qb, &o, true, &k1, &t);
+        if (rc) {
+            restore();
+            return rc;
+        }
+        const float k = std::min(k0, k1);
+        if (b.kernel_ms < 0.0f || k < b.kernel_ms) {
+            b.chunk = c->chunk;
+            b.slots = cd.slots;
+            b.blocks_per_cu = cd.bpc;
+            b.hitlist = c->hitlist;
+            b.grab = (int32_t)cd.grab;
+            b.kernel_ms = k;
         }
     }
     if (b.kernel_ms < 0.0f) {
-        c->slots = slots0, c->blocks_per_cu = bpc0;
-        if (c->item_order != order0) c->item_order = order0, ++c->item_gen;
+        restore();
         return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_autotune: no candidate fits");
     }
     c->slots = b.slots;
     c->blocks_per_cu = b.blocks_per_cu;
-    if (c->item_order != b.item_order) c->item_order = b.item_order, ++c->item_gen;
+    c->grab = (uint32_t)b.grab;
     *best = b;
     return HCX_SUCCESS;
 }

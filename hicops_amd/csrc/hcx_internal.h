@@ -65,6 +65,8 @@ struct hcx_ctx {
     hipStream_t aux[2] = {nullptr, nullptr};   // default priority; pickpeaks size classes side by side
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     hipEvent_t ev4 = nullptr, ev5 = nullptr;   // around the e-value stage
+    hipEvent_t ev6 = nullptr;                  // between the packed and the two-pass scan kernel
+    bool ran_packed = false, ran_two_pass = false;   // scan kernels of the last timed search
     bool timed_ok = false;                     // the last search recorded all six events
     bool capturing = false;                    // launchers record no events (graph capture)
     double *d_lgf = nullptr;          // max_shdpeaks + 1
@@ -91,9 +93,9 @@ struct hcx_ctx {
     uint32_t slots = 1536;            // scored-peptide slots per chunk pass (measured best)
     uint32_t blocks_per_cu = 2;       // persistent grid of the batched kernel
     int32_t hitlist = -1;             // LDS hit-list entries: < 0 as many as fit
-    int32_t item_order = 0;           // batched work items: 0 by spectrum, 1 by (length, chunk)
     uint32_t group = 0;               // probes per run table: 0 as many as fit, else 64..kProbesPerThread x threads
-    uint32_t item_gen = 0;            // bumped when item_order changes (cached item lists)
+    int32_t scan_mode = HCX_SCAN_AUTO;   // packed scan where exact, two-pass elsewhere; or two-pass only
+    uint32_t grab = 64;               // batched kernels: items per queue grab, at most
     int max_lds = 65536;              // device limit per workgroup
     // the captured search sequence and the launch shape it was captured for
     hipGraphExec_t graph_exec = nullptr;
@@ -118,9 +120,11 @@ struct hcx_qbatch {
     uint32_t max_int = 0;    // largest peak intensity of the batch
     hcx::DeviceBuf moz, inten, idx, prec, chg, rt;
     hcx::DeviceBuf res, hist, stats, feat, pred;
-    hcx::DeviceBuf acc, items, counts, item_max;   // chunked-search state
-    uint32_t nitems = 0;
-    uint64_t items_key = 0;              // index set + chunk the items were built for
+    hcx::DeviceBuf mult;                 // per spectrum: most peaks within 2 * dF + 1 bins
+    // chunked-search state: per-spectrum accumulators, windows, work-item
+    // numbering, the two-pass kernel's list
+    hcx::DeviceBuf acc, win, packed, counts, item_base, item_spec, item_max, fb;
+    uint32_t item_cap = 0;               // item_max entries (grown to the last search's item total)
 };
 
 namespace hcx {
@@ -171,15 +175,15 @@ struct SpecAcc {
     uint32_t cpsms, flags;
     unsigned long long n_ia, n_probe;
 };
-struct Item {
-    uint32_t spec;
-    uint16_t ixx;
-    uint16_t chunk;
-    int32_t lo, hi;
-};
 // Largest min_shp the chunked kernels' 8-bit saturating counters can test;
 // a larger threshold runs on the cursor kernel.
 constexpr uint32_t kMaxChunkShp = 64;
+// Items per queue grab of the batched kernels, at most.
+constexpr uint32_t kMaxGrab = 256;
+// Work items of the batched kernels are (spectrum, length, chunk) triples,
+// numbered spectrum by spectrum (item_base), then by length and chunk; they
+// are never stored: a kernel maps an item number back to its triple through
+// item_base, item_spec and the spectrum's windows.
 struct ChunkArgs {
     QueryArgs q;
     uint32_t tc, slots, max_peaks, grid;
@@ -187,24 +191,46 @@ struct ChunkArgs {
     uint32_t wdiv_m, wdiv_s;   // magic divider for 2 * dF + 1 (probes per peak)
     uint32_t hl_cap;      // LDS hit-list entries (0: none); needs fewer than 65535 peaks
     SpecAcc *acc;
-    const Item *items;
-    Item *items_w;
-    uint32_t nitems;
     uint32_t *queue;
-    uint32_t nbk;         // (length, chunk) buckets: sum of nch
-    uint32_t by_chunk;    // items bucketed by (length, chunk) instead of spectrum
     uint32_t wide;        // 64-bit intensity sums in the slots
     uint32_t replay;      // layout of the in-order replay (slots keep their hyperscores)
-    uint32_t *item_max;         // batched, by spectrum: each item's best hyperscore (float bits), or null
-    const uint32_t *item_base;  // with item_max: first item of each spectrum
+    uint32_t grab;        // batched: items per queue grab, at most (<= kMaxGrab)
+    const int2 *win;            // nspec * nlen precursor windows (lo, hi) in pepEntries rows; lo > hi: none
+    const uint32_t *item_base;  // batched: nspec + 1, first item of each spectrum; [nspec] = item total
+    const uint32_t *item_spec;  // batched: spectrum holding item 64 * i
+    uint32_t *item_max;         // batched: each item's best hyperscore (float bits), or null
+    uint32_t item_cap;          // entries of item_max: items past it are neither recorded nor skipped
+    const uint8_t *packed;      // per spectrum: 1 = scored by the packed kernel
+    const uint32_t *fb_list;    // spectra left to the two-pass kernel (null: every spectrum)
+    const uint32_t *fb_count;   // their number (device)
+};
+// Inputs of the per-spectrum window kernel.
+struct WinArgs {
+    const uint32_t *mult;   // per spectrum: most peaks within 2 * dF + 1 bins
+    uint32_t mult_max;      // the packed kernel is exact for mult <= mult_max
+    uint32_t pack;          // 1: spectra within mult_max go to the packed kernel
+    uint32_t batched;       // 1: count each listed spectrum's work items
+    int2 *win;
+    uint32_t *counts;       // nspec + 1 (batched)
+    uint8_t *packed;
+    uint32_t *fb_list, *fb_count;
+    SpecAcc *acc;           // HCX_RES_TWO_PASS set for spectra the packed kernel leaves
 };
 size_t chunk_smem_bytes(const ChunkArgs &a);
 int launch_chunk_offsets(hcx_ctx *ctx, const uint32_t *iA, const uint32_t *bA, uint32_t NB, uint32_t nch, uint32_t TC,
                          uint32_t speclen, uint32_t *bA2);
 int launch_fill_u32(hcx_ctx *ctx, uint32_t *p, uint64_t n, uint32_t v);
 int launch_chunk_local(hcx_ctx *ctx, const uint32_t *iA, uint64_t n, uint32_t speclen, uint32_t TC, uint16_t *iAc);
-int launch_items(hcx_ctx *ctx, ChunkArgs &a, uint32_t *counts, uint32_t *scan_tmp, bool fill);
-int launch_chunked(hcx_ctx *ctx, ChunkArgs &a, int kernel);
+int launch_windows(hcx_ctx *ctx, const ChunkArgs &a, const WinArgs &w);
+int launch_item_spec(hcx_ctx *ctx, const uint32_t *item_base, int32_t nspec, uint32_t *item_spec);
+int launch_chunked(hcx_ctx *ctx, const ChunkArgs &a, int kernel);
+int launch_finalize(hcx_ctx *ctx, const ChunkArgs &a);
+// ---- packed single-pass search (k_packed.hip) ----
+constexpr uint32_t kPackedThreads = 1024;
+// LDS the packed kernels need with a.group probes per run table, or 0 when
+// a.tc peptides' accumulators do not fit beside the other arrays
+size_t packed_smem_bytes(const ChunkArgs &a);
+int launch_packed(hcx_ctx *ctx, const ChunkArgs &a, int kernel);
 int launch_replay_chunked(hcx_ctx *ctx, ChunkArgs &a);
 int scan_exclusive_u32(hipStream_t st, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *tmp);
 uint64_t scan_tmp_words(uint64_t n);

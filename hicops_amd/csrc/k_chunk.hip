@@ -75,10 +75,11 @@ struct Layout {
     uint32_t wsum;      // 2 * (kBT / 64) wave partial sums + 2 totals
     uint32_t hist;      // min(hbins, kHistLds): the low bins of the spectrum's histogram
     uint32_t lgf;       // max_shdpeaks + 1 doubles
-    uint32_t wlo, whi;  // nlen
+    uint32_t wlo, whi;  // nlen: windows of the current spectrum
+    uint32_t lfirst;    // nlen + 1: first work item of each length in the spectrum (batched)
     uint32_t hl;        // hl_cap u16: hit list of the current item
     uint32_t ctl;       // 8 words: [0..1] control, [2..5] spectrum merge, [6] grabbed item
-    uint32_t imax;      // kGrab words: best score of each item of the current grab
+    uint32_t imax;      // kMaxGrab words: best score of each item of the current grab
     uint32_t red64;     // 2 u64: spectrum merge (entries, probes)
     uint32_t heap;      // replay only: 5 x 256 words, the reference heap (hs, psid, ixx, bc, yc)
     uint32_t total;
@@ -112,9 +113,10 @@ __host__ __device__ inline Layout make_layout(const ChunkArgs &a) {
     l.lgf = take(8u * (a.q.max_shdpeaks + 1));
     l.wlo = take(4u * a.q.nlen);
     l.whi = take(4u * a.q.nlen);
+    l.lfirst = take(4u * (a.q.nlen + 1));
     l.hl = take(2u * a.hl_cap);
     l.ctl = take(32u);
-    l.imax = take(4u * kGrab);
+    l.imax = take(4u * kMaxGrab);
     l.red64 = take(16u);
     l.heap = take(a.replay ? 5u * 256u * 4u : 0u);
     l.total = off;
@@ -126,7 +128,7 @@ struct Smem {
     lu32 *seg32;
     lu64 *seg64;
     lu16 *slot_loc, *hl;
-    lu32 *ctl, *heap, *imax;
+    lu32 *ctl, *heap, *imax, *lfirst;
     lu64 *red64;
     lu64 *slot_sum64;
     lu32 *slot_sum32;
@@ -157,6 +159,7 @@ __device__ __forceinline__ Smem bind(unsigned char *raw, const Layout &l) {
     s.lgf = (lf64 *)(b + l.lgf);
     s.wlo = (li32 *)(b + l.wlo);
     s.whi = (li32 *)(b + l.whi);
+    s.lfirst = (lu32 *)(b + l.lfirst);
     s.hl = (lu16 *)(b + l.hl);
     s.ctl = (lu32 *)(b + l.ctl);
     s.imax = (lu32 *)(b + l.imax);
@@ -610,7 +613,9 @@ __device__ __forceinline__ void init_smem(const ChunkArgs &a, const Smem &sm) {
     for (uint32_t i = threadIdx.x; i < min(a.q.hbins, kHistLds); i += kBT) sm.hist[i] = 0;
 }
 
-// Plain form: one workgroup per spectrum, its items in order.
+// Two-pass form, one spectrum at a time: with a.fb_list, the listed spectra
+// (those the packed kernel leaves), taken in turn by a persistent grid;
+// otherwise one workgroup per spectrum.  Items in order.
 template <bool WIDE>
 __global__ __launch_bounds__(kBT, kWavesPerSimd) void k_spec_chunked(ChunkArgs a) {
     // all LDS is dynamic, so the hit counters start at address 0
@@ -622,30 +627,31 @@ __global__ __launch_bounds__(kBT, kWavesPerSimd) void k_spec_chunked(ChunkArgs a
     if (threadIdx.x == 0) {
         s_ctl[0] = 0;
     }
-    const uint32_t s = blockIdx.x;
-    const uint32_t P = load_peaks<kBT>(a, sm.pk_moz, sm.pk_int, s);
-    const uint32_t D = a.q.max_mass + 2;
-    const float prec = ((const AS_GLB float *)a.q.prec)[s];
-    for (int l = threadIdx.x; l < a.q.nlen; l += kBT) {
-        const LenDesc L = load_len(a, l);
-        int32_t lo, hi;
-        window(L, prec, a.q.dM, D, &lo, &hi);
-        sm.wlo[l] = lo;
-        sm.whi[l] = hi;
+    const uint32_t nwork = a.fb_list ? ufl(*(const AS_GLB uint32_t *)a.fb_count) : (uint32_t)a.q.nspec;
+    for (uint32_t wi = blockIdx.x; wi < nwork; wi += a.fb_list ? gridDim.x : nwork) {
+        const uint32_t s = a.fb_list ? ufl(((const AS_GLB uint32_t *)a.fb_list)[wi]) : wi;
+        __syncthreads();
+        const uint32_t P = load_peaks<kBT>(a, sm.pk_moz, sm.pk_int, s);
+        for (int l = threadIdx.x; l < a.q.nlen; l += kBT) {
+            const int2 w = a.win[(size_t)s * a.q.nlen + l];
+            sm.wlo[l] = w.x;
+            sm.whi[l] = w.y;
+        }
+        __syncthreads();
+        Acc acc;
+        acc_reset(acc);
+        for (int ixx = 0; ixx < a.q.nlen; ixx++) {
+            const int32_t lo = ufl((int32_t)sm.wlo[ixx]), hi = ufl((int32_t)sm.whi[ixx]);
+            if (lo > hi) continue;
+            for (uint32_t c = (uint32_t)lo / a.tc; c <= (uint32_t)hi / a.tc; c++)
+                process_item<false, WIDE>(a, sm, s_ctl, s, nullptr, P, ixx, c, lo, hi, acc);
+        }
+        flush_spectrum<kBT>(a, sm.hist, s, acc, s_red, s_red64);
     }
-    __syncthreads();
-    Acc acc;
-    acc_reset(acc);
-    for (int ixx = 0; ixx < a.q.nlen; ixx++) {
-        const int32_t lo = ufl((int32_t)sm.wlo[ixx]), hi = ufl((int32_t)sm.whi[ixx]);
-        if (lo > hi) continue;
-        for (uint32_t c = (uint32_t)lo / a.tc; c <= (uint32_t)hi / a.tc; c++)
-            process_item<false, WIDE>(a, sm, s_ctl, s, nullptr, P, ixx, c, lo, hi, acc);
-    }
-    flush_spectrum<kBT>(a, sm.hist, s, acc, s_red, s_red64);
 }
 
-// Batched form: persistent workgroups pull (spectrum, length, chunk) items.
+// Batched two-pass form: persistent workgroups pull (spectrum, length, chunk)
+// items (run_items).
 template <bool WIDE>
 __global__ __launch_bounds__(kBT, kWavesPerSimd) void k_items_chunked(ChunkArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -658,42 +664,18 @@ __global__ __launch_bounds__(kBT, kWavesPerSimd) void k_items_chunked(ChunkArgs 
     }
     Acc acc;
     acc_reset(acc);
-    uint32_t cur = 0xFFFFFFFFu, P = 0;
-    const Item *items = a.items;
-    uint32_t grab = max(1u, min(kGrab, a.nitems / (4u * gridDim.x)));
-    for (;;) {
-        __syncthreads();
-        if (threadIdx.x == 0) *s_item = __hip_atomic_fetch_add((gu32 *)a.queue, grab, __ATOMIC_RELAXED,
-                                                               __HIP_MEMORY_SCOPE_AGENT);
-        __syncthreads();
-        const uint32_t it0 = ufl((uint32_t)*s_item);
-        if (it0 >= a.nitems) break;
-        const uint32_t it1 = min(a.nitems, it0 + grab);
-        // guided: about a quarter of each workgroup's share of what is left
-        grab = max(1u, min(kGrab, (a.nitems - it1) / (4u * gridDim.x)));
-        // the grab's items' best scores collect in LDS and go out together
-        // (a global store per item would make the next barrier wait for it)
-        const bool rec = a.item_max != nullptr;
-        if (rec && threadIdx.x < it1 - it0) sm.imax[threadIdx.x] = 0;
-        for (uint32_t it = it0; it < it1; it++) {
-            Item item = items[it];
-            item.spec = ufl(item.spec);
-            item.lo = ufl(item.lo);
-            item.hi = ufl(item.hi);
-            item.ixx = (uint16_t)ufl((uint32_t)item.ixx);
-            item.chunk = (uint16_t)ufl((uint32_t)item.chunk);
-            if (item.spec != cur) {
-                if (cur != 0xFFFFFFFFu) flush_spectrum<kBT>(a, sm.hist, cur, acc, s_red, s_red64);
-                cur = item.spec;
-                P = load_peaks<kBT>(a, sm.pk_moz, sm.pk_int, cur);
-                __syncthreads();
-            }
-            process_item<false, WIDE>(a, sm, s_ctl, cur, rec ? sm.imax + (it - it0) : nullptr, P, item.ixx,
-                                      item.chunk, item.lo, item.hi, acc);
-        }
-        if (rec && threadIdx.x < it1 - it0) ((AS_GLB uint32_t *)a.item_max)[it0 + threadIdx.x] = sm.imax[threadIdx.x];
-    }
-    if (cur != 0xFFFFFFFFu) flush_spectrum<kBT>(a, sm.hist, cur, acc, s_red, s_red64);
+    uint32_t P = 0;
+    run_items<kBT>(
+        a, s_item, sm.imax, sm.wlo, sm.lfirst,
+        [&](uint32_t s) {
+            P = load_peaks<kBT>(a, sm.pk_moz, sm.pk_int, s);
+            load_windows<kBT>(a, s, sm.wlo, sm.whi, sm.lfirst);
+        },
+        [&](uint32_t s, uint32_t l, uint32_t c, lu32 *imax_out) {
+            process_item<false, WIDE>(a, sm, s_ctl, s, imax_out, P, (int)l, c, ufl((int32_t)sm.wlo[l]),
+                                      ufl((int32_t)sm.whi[l]), acc);
+        },
+        [&](uint32_t s) { flush_spectrum<kBT>(a, sm.hist, s, acc, s_red, s_red64); });
 }
 
 // Tie replay: spectra whose maximum hyperscore is shared by several peptides.
@@ -711,7 +693,6 @@ __global__ __launch_bounds__(kBT) void k_replay_chunked(ChunkArgs a) {
         s_ctl[0] = 0;
     }
     const uint32_t nwork = ((const AS_GLB uint32_t *)a.q.tie)[0];
-    const uint32_t D = a.q.max_mass + 2;
     // tied spectra are taken one at a time from a counter after the list
     // (zeroed before the scan): their replays differ widely in length
     __shared__ uint32_t s_grab;
@@ -724,18 +705,22 @@ __global__ __launch_bounds__(kBT) void k_replay_chunked(ChunkArgs a) {
         const uint32_t s = ((const AS_GLB uint32_t *)a.q.tie)[1 + wi];
         __syncthreads();
         const uint32_t P = load_peaks<kBT>(a, sm.pk_moz, sm.pk_int, s);
-        const float prec = ((const AS_GLB float *)a.q.prec)[s];
         for (int l = threadIdx.x; l < a.q.nlen; l += kBT) {
-            const LenDesc L = load_len(a, l);
-            int32_t lo, hi;
-            window(L, prec, a.q.dM, D, &lo, &hi);
-            sm.wlo[l] = lo;
-            sm.whi[l] = hi;
+            const int2 w = a.win[(size_t)s * a.q.nlen + l];
+            sm.wlo[l] = w.x;
+            sm.whi[l] = w.y;
         }
-        // the heap's size and root as thread 0 last published them; the
-        // spectrum's items follow in the item list in this loop's order
-        const bool skip_ok = a.item_max != nullptr && a.item_base != nullptr;
-        uint32_t ir = skip_ok ? ufl(((const AS_GLB uint32_t *)a.item_base)[s]) : 0u;
+        // the heap's size and root as thread 0 last published them.  A
+        // spectrum the batched kernel scored has its items numbered from
+        // item_base[s] in this loop's order (a spectrum scored elsewhere
+        // has none), and item_max holds their best scores.
+        uint32_t ir = 0;
+        bool skip_ok = false;
+        if (a.item_max != nullptr && a.item_base != nullptr) {
+            ir = ufl(((const AS_GLB uint32_t *)a.item_base)[s]);
+            const uint32_t ie = ufl(((const AS_GLB uint32_t *)a.item_base)[s + 1]);
+            skip_ok = ie > ir && ie <= a.item_cap;
+        }
         if (threadIdx.x == 0) {
             sm.ctl[6] = 0;
             sm.ctl[7] = 0;
@@ -782,50 +767,39 @@ __global__ __launch_bounds__(kBT) void k_replay_chunked(ChunkArgs a) {
     }
 }
 
-// Items: per spectrum and length with a valid window, one item per chunk,
-// bucketed by spectrum, or by (length, chunk) -- bucket LenDesc::cbase + c --
-// so that the workgroups running at any moment share a few chunks of iA in L2.
-__global__ void k_items_count(ChunkArgs a, uint32_t *bcount) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+
+// Per spectrum (one wavefront): the precursor window of every length
+// (dslim_query.cpp:854-916), whether the packed kernel scores it, and -- for
+// the batched forms -- its number of (length, chunk) work items.  Spectra the
+// packed kernel cannot score exactly go to the two-pass kernel's list.
+__global__ __launch_bounds__(256) void k_windows(ChunkArgs a, WinArgs w) {
+    const uint32_t s = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (s >= (uint32_t)a.q.nspec) return;
     const uint32_t D = a.q.max_mass + 2;
-    for (int l = 0; l < a.q.nlen; l++) {
+    const float prec = a.q.prec[s];
+    uint32_t cnt = 0;
+    for (int l = (int)lane; l < a.q.nlen; l += 64) {
         int32_t lo, hi;
-        window(a.q.lens[l], a.q.prec[s], a.q.dM, D, &lo, &hi);
-        if (lo > hi) continue;
-        if (!a.by_chunk) {
-            bcount[s] += (uint32_t)hi / a.tc - (uint32_t)lo / a.tc + 1;
-            continue;
-        }
-        const uint32_t cb = a.q.lens[l].cbase;
-        for (uint32_t c = (uint32_t)lo / a.tc; c <= (uint32_t)hi / a.tc; c++) atomicAdd(&bcount[cb + c], 1u);
+        window(a.q.lens[l], prec, a.q.dM, D, &lo, &hi);
+        w.win[(size_t)s * a.q.nlen + l] = make_int2(lo, hi);
+        if (lo <= hi) cnt += (uint32_t)hi / a.tc - (uint32_t)lo / a.tc + 1;
     }
+    cnt = (uint32_t)__builtin_amdgcn_readlane((int)wave_reduce_last(cnt, [](uint32_t u, uint32_t v) { return u + v; }),
+                                              63);
+    if (lane != 0) return;
+    const bool packed = w.pack && w.mult[s] <= w.mult_max;
+    w.packed[s] = packed ? 1 : 0;
+    if (!packed) w.acc[s].flags = HCX_RES_TWO_PASS;   // before any scan kernel ORs its flags in
+    if (w.batched) w.counts[s] = (packed || !w.pack) ? cnt : 0u;
+    if (w.pack && !packed && cnt) w.fb_list[atomicAdd(w.fb_count, 1u)] = s;
 }
 
-__global__ void k_items_fill(ChunkArgs a, uint32_t *cursor, Item *items) {
+// item_spec[i] = the spectrum holding item 64 * i (one thread per spectrum).
+__global__ void k_item_spec(const uint32_t *item_base, int32_t nspec, uint32_t *item_spec) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= (uint32_t)a.q.nspec) return;
-    const uint32_t D = a.q.max_mass + 2;
-    for (int l = 0; l < a.q.nlen; l++) {
-        int32_t lo, hi;
-        window(a.q.lens[l], a.q.prec[s], a.q.dM, D, &lo, &hi);
-        if (lo > hi) continue;
-        const uint32_t cb = a.q.lens[l].cbase;
-        for (uint32_t c = (uint32_t)lo / a.tc; c <= (uint32_t)hi / a.tc; c++) {
-            Item it;
-            it.spec = s;
-            it.ixx = (uint16_t)l;
-            it.chunk = (uint16_t)c;
-            it.lo = lo;
-            it.hi = hi;
-            items[a.by_chunk ? atomicAdd(&cursor[cb + c], 1u) : cursor[s]++] = it;
-        }
-    }
-}
-
-__global__ void k_copy_u32(const uint32_t *src, uint32_t *dst, uint64_t n) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[i];
+    if (s >= (uint32_t)nspec) return;
+    const uint32_t b0 = item_base[s], b1 = item_base[s + 1];
+    for (uint32_t i = (b0 + 63) >> 6; (i << 6) < b1; i++) item_spec[i] = s;
 }
 
 // SpecAcc -> hcx_result; tied maxima go to the replay list.
@@ -934,57 +908,50 @@ int launch_fill_u32(hcx_ctx *ctx, uint32_t *p, uint64_t n, uint32_t v) {
     return HCX_SUCCESS;
 }
 
-// counts: 2 * (nb + 1) words, nb = number of buckets (spectra, or (length,
-// chunk) pairs).  fill == false: bucket counts and their exclusive scan
-// (counts[nb] then holds the item total); fill == true: write the items,
-// bucket by bucket.
-int launch_items(hcx_ctx *ctx, ChunkArgs &a, uint32_t *counts, uint32_t *scan_tmp, bool fill) {
-    hipStream_t st = ctx->stream;
-    const int n = a.q.nspec;
-    if (n <= 0) return HCX_SUCCESS;
-    const uint64_t nb = (uint64_t)(a.by_chunk ? a.nbk : (uint32_t)n) + 1;
-    if (!fill) {
-        int rc = launch_fill_u32(ctx, counts, nb, 0);
-        if (rc) return rc;
-        k_items_count<<<(n + 255) / 256, 256, 0, st>>>(a, counts);
-        HCX_HIP(hipGetLastError());
-        return scan_exclusive_u32(st, counts, counts, nb, scan_tmp);
-    }
-    uint32_t *cursor = counts + nb;
-    k_copy_u32<<<(unsigned)((nb + 255) / 256), 256, 0, st>>>(counts, cursor, nb);
-    HCX_HIP(hipGetLastError());
-    k_items_fill<<<(n + 255) / 256, 256, 0, st>>>(a, cursor, a.items_w);
+int launch_windows(hcx_ctx *ctx, const ChunkArgs &a, const WinArgs &w) {
+    if (a.q.nspec <= 0) return HCX_SUCCESS;
+    k_windows<<<(unsigned)((a.q.nspec + 3) / 4), 256, 0, ctx->stream>>>(a, w);
     HCX_HIP(hipGetLastError());
     return HCX_SUCCESS;
 }
 
+int launch_item_spec(hcx_ctx *ctx, const uint32_t *item_base, int32_t nspec, uint32_t *item_spec) {
+    if (nspec <= 0) return HCX_SUCCESS;
+    k_item_spec<<<(unsigned)((nspec + 255) / 256), 256, 0, ctx->stream>>>(item_base, nspec, item_spec);
+    HCX_HIP(hipGetLastError());
+    return HCX_SUCCESS;
+}
+
+// kernel HCX_KERNEL_BATCHED: the persistent item kernel (grid a.grid);
+// otherwise the spectrum kernel: persistent over a.fb_list (grid a.grid), or
+// one workgroup per spectrum.
 template <bool WIDE>
-static int launch_chunked_t(hcx_ctx *ctx, ChunkArgs &a, int kernel) {
+static int launch_chunked_t(hcx_ctx *ctx, const ChunkArgs &a, int kernel) {
     hipStream_t st = ctx->stream;
     const size_t sm = chunk_smem_bytes(a);
-    const int n = a.q.nspec;
     if (kernel == HCX_KERNEL_BATCHED) {
         HCX_HIP(hipFuncSetAttribute((const void *)k_items_chunked<WIDE>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)sm));
-        if (!ctx->capturing) HCX_HIP(hipEventRecord(ctx->ev0, st));
         k_items_chunked<WIDE><<<a.grid, kBT, sm, st>>>(a);
-        HCX_HIP(hipGetLastError());
     } else {
         HCX_HIP(hipFuncSetAttribute((const void *)k_spec_chunked<WIDE>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)sm));
-        if (!ctx->capturing) HCX_HIP(hipEventRecord(ctx->ev0, st));
-        k_spec_chunked<WIDE><<<n, kBT, sm, st>>>(a);
-        HCX_HIP(hipGetLastError());
+        k_spec_chunked<WIDE><<<a.fb_list ? a.grid : (unsigned)a.q.nspec, kBT, sm, st>>>(a);
     }
-    if (!ctx->capturing) HCX_HIP(hipEventRecord(ctx->ev1, st));
-    k_finalize<<<(n + 255) / 256, 256, 0, st>>>(a);
     HCX_HIP(hipGetLastError());
     return HCX_SUCCESS;
 }
 
-int launch_chunked(hcx_ctx *ctx, ChunkArgs &a, int kernel) {
+int launch_chunked(hcx_ctx *ctx, const ChunkArgs &a, int kernel) {
     if (a.q.nspec <= 0) return HCX_SUCCESS;
     return a.wide ? launch_chunked_t<true>(ctx, a, kernel) : launch_chunked_t<false>(ctx, a, kernel);
+}
+
+int launch_finalize(hcx_ctx *ctx, const ChunkArgs &a) {
+    if (a.q.nspec <= 0) return HCX_SUCCESS;
+    k_finalize<<<(a.q.nspec + 255) / 256, 256, 0, ctx->stream>>>(a);
+    HCX_HIP(hipGetLastError());
+    return HCX_SUCCESS;
 }
 
 template <bool WIDE>

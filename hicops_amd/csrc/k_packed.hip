@@ -1,0 +1,375 @@
+// k_packed.hip -- single-pass shared-peak scan with packed per-peptide
+// accumulators: the scan for every spectrum it can score exactly.
+//
+// Work item = (spectrum, length, chunk of TC peptides), as in k_chunk.hip, on
+// the same (bin, chunk)-bucketed index (bA2 runs, 16-bit chunk-local iAc).
+// Each peptide of the chunk owns two 32-bit LDS words, one per ion series:
+// word index = the iAc entry itself (local << 1 | isY), word = count << 24 |
+// intensity sum.  For one item:
+//   R  every (peak, bin) probe reads its run [bA2[c], bA2[c+1]); the
+//      non-empty runs are compacted into one flat sequence whose run records
+//      carry (entry index minus flat offset, (1 << 24) | the probe's peak
+//      intensity), with a bitmap of run ends (as k_chunk.hip);
+//   P  one pass over the flat entries: lane j of a 64-entry tile reads entry
+//      F + j (coalesced 16-bit loads) and, if the peptide is inside the
+//      precursor window (dslim_query.cpp:637-641), adds the run's value to
+//      the entry's word -- one non-returning LDS atomic per entry, which
+//      counts the shared peak and sums its intensity at once
+//      (dslim_query.cpp:644-676);
+//   S  the window's words are read 128 bits at a time and cleared; every
+//      peptide with bc + yc >= min_shp is scored (dslim_query.cpp:679-722):
+//      hs = lgf[bc] + lgf[yc] + log10(1+ibc) + log10(1+iyc) - 6 in FP64,
+//      rounded to float, histogram bin (int)(hs*10f + 0.5), the spectrum's
+//      best.
+// Per-spectrum results merge across workgroups as in k_chunk.hip (cpsms and
+// histogram atomics, hs-keyed 64-bit maxima); tied maxima go to the in-order
+// heap replay there.
+//
+// Exactness: one ion bin is probed by at most M peaks, M = the spectrum's
+// most peaks within 2 * dF + 1 bins, so a series of a peptide takes at most
+// speclen / 2 * M hits, each adding at most the batch's largest intensity.
+// The host marks a spectrum packed only if that bound keeps the count below
+// 256 and the sum below 2^24: no field can carry into the next.  The other
+// spectra run on the two-pass kernel (k_chunk.hip), whose counts and sums
+// are separate.
+//
+// A workgroup is 1024 threads (16 wavefronts, four per SIMD) and takes most
+// of the CU's 160 KB of LDS: 16384 peptides x 8 bytes of accumulators plus the
+// run table.
+#include "k_scan_common.h"
+
+namespace hcx {
+namespace {
+
+constexpr int kPT = (int)kPackedThreads;
+// run-table probes per thread: a group holds at most kPPp * kPT probes
+constexpr uint32_t kPPp = 2;
+constexpr uint32_t kCnt = 1u << 24;   // count unit of a packed word
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef AS_LDS u32x2 lu32x2;
+
+// Byte offsets of the dynamic LDS arrays (host and device agree on them).
+struct PLayout {
+    uint32_t acc;       // 2 * tc words at offset 0: b and y word of each peptide
+    uint32_t runs;      // group + 1 uint2: (entry index - flat offset, (1 << 24) | intensity)
+    uint32_t cofs;      // group + 1: flat offset of each non-empty run (exclusive prefix)
+    uint32_t pk_moz, pk_int;   // max_peaks
+    uint32_t seg;       // kSegWords u64: run ends of the current segment
+    uint32_t wsum;      // 2 * (kPT / 64) + 2
+    uint32_t hist;      // min(hbins, kHistLds)
+    uint32_t lgf;       // max_shdpeaks + 1 doubles
+    uint32_t wlo, whi;  // nlen
+    uint32_t lfirst;    // nlen + 1
+    uint32_t ctl;       // 8 words: [2..5] spectrum merge, [6] grabbed item
+    uint32_t imax;      // kMaxGrab
+    uint32_t red64;     // 2 u64
+    uint32_t total;
+};
+
+__host__ __device__ inline uint32_t pal16(uint32_t x) { return (x + 15) & ~15u; }
+
+__host__ __device__ inline PLayout make_playout(const ChunkArgs &a) {
+    PLayout l;
+    uint32_t off = 0;
+    auto take = [&](uint32_t bytes) {
+        uint32_t o = off;
+        off = pal16(off + bytes);
+        return o;
+    };
+    l.acc = take(8u * a.tc);
+    l.runs = take(8u * (a.group + 1));
+    l.cofs = take(4u * (a.group + 1));
+    l.pk_moz = take(4u * a.max_peaks);
+    l.pk_int = take(4u * a.max_peaks);
+    l.seg = take(8u * kSegWords);
+    l.wsum = take(4u * (2 * (kPT / 64) + 2));
+    l.hist = take(4u * min(a.q.hbins, kHistLds));
+    l.lgf = take(8u * (a.q.max_shdpeaks + 1));
+    l.wlo = take(4u * a.q.nlen);
+    l.whi = take(4u * a.q.nlen);
+    l.lfirst = take(4u * (a.q.nlen + 1));
+    l.ctl = take(32u);
+    l.imax = take(4u * kMaxGrab);
+    l.red64 = take(16u);
+    l.total = off;
+    return l;
+}
+
+struct PSmem {
+    lu32 *acc, *cofs, *pk_moz, *pk_int, *wsum, *hist, *lfirst, *ctl, *imax;
+    lu32x2 *runs;
+    lu32 *seg32;
+    lu64 *seg64, *red64;
+    lf64 *lgf;
+    li32 *wlo, *whi;
+};
+
+__device__ __forceinline__ PSmem pbind(unsigned char *raw, const PLayout &l) {
+    AS_LDS unsigned char *b = (AS_LDS unsigned char *)raw;
+    PSmem s;
+    s.acc = (lu32 *)(b + l.acc);
+    s.runs = (lu32x2 *)(b + l.runs);
+    s.cofs = (lu32 *)(b + l.cofs);
+    s.pk_moz = (lu32 *)(b + l.pk_moz);
+    s.pk_int = (lu32 *)(b + l.pk_int);
+    s.seg32 = (lu32 *)(b + l.seg);
+    s.seg64 = (lu64 *)(b + l.seg);
+    s.wsum = (lu32 *)(b + l.wsum);
+    s.hist = (lu32 *)(b + l.hist);
+    s.lgf = (lf64 *)(b + l.lgf);
+    s.wlo = (li32 *)(b + l.wlo);
+    s.whi = (li32 *)(b + l.whi);
+    s.lfirst = (lu32 *)(b + l.lfirst);
+    s.ctl = (lu32 *)(b + l.ctl);
+    s.imax = (lu32 *)(b + l.imax);
+    s.red64 = (lu64 *)(b + l.red64);
+    return s;
+}
+
+// Run table of the probes [g0, g1) of one item (at most kPPp * kPT probes):
+// each (peak, bin) probe with dF < q < NB-1-dF (dslim_query.cpp:625) reads
+// its run [bA2[c], bA2[c+1]) in chunk c; the non-empty runs are compacted in
+// probe order.  Returns (E, nc): flat entries and runs, uniform.
+__device__ __forceinline__ uint2 setup_runs_p(const ChunkArgs &a, const PSmem &sm, const LenDesc &L, uint32_t c,
+                                              uint32_t g0, uint32_t g1, bool count_probes, Acc &acc) {
+    const uint32_t W = 2 * a.q.dF + 1;
+    const uint32_t n = g1 - g0;
+    uint32_t rs[kPPp], len[kPPp], pk[kPPp];
+    uint32_t nr = 0, ne = 0;
+#pragma unroll
+    for (uint32_t h = 0; h < kPPp; h++) {
+        rs[h] = len[h] = pk[h] = 0;
+        const uint32_t lp = h * kPT + threadIdx.x;
+        if (lp < n) {
+            const uint32_t p = g0 + lp;
+            const uint32_t k = udiv(p, a.wdiv_m, a.wdiv_s), j = p - k * W;
+            pk[h] = k;
+            const uint32_t q = sm.pk_moz[k];
+            if (q > a.q.dF && q < a.q.NB - 1 - a.q.dF) {
+                const size_t o = (size_t)c * a.q.NB + (q - a.q.dF + j);
+                rs[h] = ldg(L.bA2, o);
+                len[h] = ldg(L.bA2, o + a.q.NB) - rs[h];
+                if (count_probes) acc.nprobe++;
+            }
+        }
+        nr += len[h] ? 1u : 0u;
+        ne += len[h];
+    }
+    // the bitmap is cleared before the scan's barrier and filled after it
+    for (uint32_t w = threadIdx.x; w < 2 * kSegWords; w += kPT) sm.seg32[w] = 0;
+    uint2 tot;
+    const uint2 pre = block_prefix2<kPT>(sm.wsum, nr, ne, &tot);
+    uint32_t ci = pre.x, fo = pre.y;
+#pragma unroll
+    for (uint32_t h = 0; h < kPPp; h++) {
+        if (len[h]) {
+            sm.runs[ci] = (u32x2){rs[h] - fo, kCnt | sm.pk_int[pk[h]]};
+            sm.cofs[ci] = fo;
+            fo += len[h];
+            if (fo - 1 < kSegEntries) lds_or(&sm.seg32[(fo - 1) >> 5], 1u << ((fo - 1) & 31u));
+            ci++;
+        }
+    }
+    if (threadIdx.x == 0) {
+        sm.cofs[tot.x] = tot.y;
+        sm.runs[tot.x] = (u32x2){0u, 0u};
+    }
+    __syncthreads();
+    return make_uint2(ufl(tot.y), ufl(tot.x));
+}
+
+// Score one peptide from its two packed words (dslim_query.cpp:679-722).
+__device__ __forceinline__ void score_peptide(const ChunkArgs &a, const PSmem &sm, uint32_t spec, uint32_t ixx,
+                                              uint32_t pid, uint32_t wb, uint32_t wy, Acc &acc, float &imax) {
+    const uint32_t bc = wb >> 24, yc = wy >> 24;
+    if ((uint16_t)(bc + yc) < a.q.min_shp) return;
+    const uint32_t ib = wb & (kCnt - 1), iy = wy & (kCnt - 1);
+    uint32_t bi = bc, yi = yc;
+    if (bi > a.q.max_shdpeaks) { bi = a.q.max_shdpeaks; acc.flags |= HCX_RES_LGF_CLAMP; }
+    if (yi > a.q.max_shdpeaks) { yi = a.q.max_shdpeaks; acc.flags |= HCX_RES_LGF_CLAMP; }
+    double t = sm.lgf[bi] + sm.lgf[yi];
+    t = t + log10((double)(1u + ib));
+    t = t + log10((double)(1u + iy));
+    t = t - 6.0;
+    const float hs = (float)t;
+    if (!(hs > 0.0f)) return;
+    acc.cpsms++;
+    imax = fmaxf(imax, hs);
+    int bin = (int)((double)(hs * 10.0f) + 0.5);
+    if (bin >= (int)a.q.hbins) { bin = (int)a.q.hbins - 1; acc.flags |= HCX_RES_HIST_CLAMP; }
+    if ((uint32_t)bin < kHistLds) lds_add(&sm.hist[bin], 1u);
+    else glb_add(a.q.hist + (size_t)spec * a.q.hbins + bin, 1u);
+    if (hs > acc.best_hs) {
+        acc.best_hs = hs;
+        acc.best_cnt = 1;
+        acc.best_id = (ixx << 25) | pid;
+        acc.best_by = (bc << 16) | yc;
+    } else if (hs == acc.best_hs) {
+        acc.best_cnt++;
+    }
+}
+
+// One (spectrum, length, chunk) item; the spectrum's peaks are in LDS.
+// imax_out (batched form): the item's best score, LDS max of float bits.
+__device__ __forceinline__ void process_item_p(const ChunkArgs &a, const PSmem &sm, uint32_t spec, lu32 *imax_out,
+                                               uint32_t P, uint32_t ixx, uint32_t c, int32_t lo, int32_t hi,
+                                               Acc &acc) {
+    const LenDesc L = load_len_u(a, ixx);
+    const uint32_t TC = a.tc;
+    const uint32_t base = c * TC;
+    const uint32_t ilo = max((uint32_t)lo, base);
+    const uint32_t ihi = min((uint32_t)hi, base + TC - 1);
+    const uint32_t wlo = ilo - base, wspan = ihi - ilo;   // window in chunk-local ids
+    const bool first = (uint32_t)lo / TC == c;            // probes are counted once per length
+    const uint32_t NP = P * (2 * a.q.dF + 1);
+    const uint32_t GP = a.group;
+    const uint32_t ng = (NP + GP - 1) / GP;
+    // iAc through a buffer resource: 32-bit offsets, and reads past the
+    // array return 0 instead of faulting
+    const __amdgpu_buffer_rsrc_t rs_iac =
+        __builtin_amdgcn_make_buffer_rsrc((void *)L.iAc, 0, (int)(L.n * L.speclen * 2u), kBufDword3);
+    const uint32_t lane = threadIdx.x & 63u;
+    const bool full = wlo == 0 && wspan == TC - 1;
+    uint32_t nia_w = 0;
+    auto pass = [&](auto full_tag) {
+        constexpr bool FULL = decltype(full_tag)::value;
+        for (uint32_t g = 0; g < ng; g++) {
+            const uint32_t g0 = g * GP, g1 = min(NP, g0 + GP);
+            const uint2 en = setup_runs_p(a, sm, L, c, g0, g1, first, acc);
+            const uint32_t E = en.x, nc = en.y;
+            if (FULL && threadIdx.x == 0) nia_w += E;   // per wavefront otherwise
+            for (uint32_t s0 = 0; s0 < E; s0 += kSegEntries) {
+                if (s0) build_seg<kPT>(sm.seg32, sm.cofs, nc, s0);
+                uint32_t cv[kTileU];
+                walk_tiles<kPT>(
+                    sm.seg64, sm.cofs, nc, s0, min(E, s0 + kSegEntries),
+                    [&](const Tile (&t)[kTileU], const uint32_t (&x)[kTileU], const bool (&)[kTileU],
+                        uint32_t (&v)[kTileU], uint32_t) {
+                        uint32_t gi[kTileU];
+#pragma unroll
+                        for (uint32_t u = 0; u < kTileU; u++) {
+                            const u32x2 r = (sm.runs + t[u].C)[tile_below(t[u])];
+                            gi[u] = r.x + x[u];
+                            cv[u] = r.y;
+                        }
+                        // lanes outside the share read some entry or, past
+                        // the array, 0 (buffer range check); they add nothing
+#pragma unroll
+                        for (uint32_t u = 0; u < kTileU; u++) v[u] = iac_load(rs_iac, gi[u]);
+                    },
+                    [&](const Tile (&)[kTileU], const uint32_t (&v)[kTileU], const uint32_t (&)[kTileU],
+                        const bool (&ok)[kTileU], uint32_t) {
+#pragma unroll
+                        for (uint32_t u = 0; u < kTileU; u++) {
+                            const bool win = FULL ? ok[u] : ok[u] && (v[u] >> 1) - wlo <= wspan;
+                            if (!FULL) nia_w += (uint32_t)__popcll(__ballot(win));
+                            if (win) lds_add(&sm.acc[v[u]], cv[u]);
+                        }
+                    });
+            }
+            // the next group's set-up rewrites the run table and the bitmap
+            __syncthreads();
+        }
+    };
+    if (full) pass(std::true_type{});
+    else pass(std::false_type{});
+    if (lane == 0) acc.nia += nia_w;
+
+    // S: the window's words, two peptides per 128-bit read; read words are
+    // cleared for the next item (a later item's pass starts after a barrier)
+    float imax = 0.0f;
+    const uint32_t q0 = wlo >> 1, q1 = (wlo + wspan) >> 1;
+    for (uint32_t qi = q0 + threadIdx.x; qi <= q1; qi += kPT) {
+        const u32x4 w4 = ((lu32x4 *)sm.acc)[qi];
+        if ((w4.x | w4.y | w4.z | w4.w) == 0u) continue;
+        ((lu32x4 *)sm.acc)[qi] = (u32x4){0u, 0u, 0u, 0u};
+        score_peptide(a, sm, spec, ixx, base + 2 * qi, w4.x, w4.y, acc, imax);
+        score_peptide(a, sm, spec, ixx, base + 2 * qi + 1, w4.z, w4.w, acc, imax);
+    }
+    // the item's best score (one LDS atomic per wavefront: same-address
+    // atomics serialise)
+    if (imax_out) {
+        const uint32_t wm = wave_max_last(__float_as_uint(imax));
+        if (lane == 63u && wm) lds_max(imax_out, wm);
+    }
+}
+
+__device__ __forceinline__ void init_psmem(const ChunkArgs &a, const PSmem &sm) {
+    for (uint32_t i = threadIdx.x; i <= a.q.max_shdpeaks; i += kPT) sm.lgf[i] = a.q.lgf[i];
+    for (uint32_t i = threadIdx.x; i < a.tc / 2; i += kPT) ((lu32x4 *)sm.acc)[i] = (u32x4){0u, 0u, 0u, 0u};
+    for (uint32_t i = threadIdx.x; i < min(a.q.hbins, kHistLds); i += kPT) sm.hist[i] = 0;
+    if (threadIdx.x < 8) sm.ctl[threadIdx.x] = 0;
+}
+
+// Plain form: one workgroup per spectrum (those marked packed), its items in
+// order.
+__global__ __launch_bounds__(kPT, 4) void k_spec_packed(ChunkArgs a) {
+    const uint32_t s = blockIdx.x;
+    if (!a.packed[s]) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const PSmem sm = pbind(smem_raw, make_playout(a));
+    init_psmem(a, sm);
+    const uint32_t P = load_peaks<kPT>(a, sm.pk_moz, sm.pk_int, s);
+    for (int l = threadIdx.x; l < a.q.nlen; l += kPT) {
+        const int2 w = a.win[(size_t)s * a.q.nlen + l];
+        sm.wlo[l] = w.x;
+        sm.whi[l] = w.y;
+    }
+    __syncthreads();
+    Acc acc;
+    acc_reset(acc);
+    for (uint32_t ixx = 0; ixx < (uint32_t)a.q.nlen; ixx++) {
+        const int32_t lo = ufl((int32_t)sm.wlo[ixx]), hi = ufl((int32_t)sm.whi[ixx]);
+        if (lo > hi) continue;
+        for (uint32_t c = (uint32_t)lo / a.tc; c <= (uint32_t)hi / a.tc; c++)
+            process_item_p(a, sm, s, nullptr, P, ixx, c, lo, hi, acc);
+    }
+    flush_spectrum<kPT>(a, sm.hist, s, acc, sm.ctl + 2, sm.red64);
+}
+
+// Batched form: persistent workgroups pull (spectrum, length, chunk) items of
+// the packed spectra (run_items).
+__global__ __launch_bounds__(kPT, 4) void k_items_packed(ChunkArgs a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const PSmem sm = pbind(smem_raw, make_playout(a));
+    init_psmem(a, sm);
+    Acc acc;
+    acc_reset(acc);
+    uint32_t P = 0;
+    run_items<kPT>(
+        a, sm.ctl + 6, sm.imax, sm.wlo, sm.lfirst,
+        [&](uint32_t s) {
+            P = load_peaks<kPT>(a, sm.pk_moz, sm.pk_int, s);
+            load_windows<kPT>(a, s, sm.wlo, sm.whi, sm.lfirst);
+        },
+        [&](uint32_t s, uint32_t l, uint32_t c, lu32 *imax_out) {
+            process_item_p(a, sm, s, imax_out, P, l, c, ufl((int32_t)sm.wlo[l]), ufl((int32_t)sm.whi[l]), acc);
+        },
+        [&](uint32_t s) { flush_spectrum<kPT>(a, sm.hist, s, acc, sm.ctl + 2, sm.red64); });
+}
+
+}  // namespace
+
+size_t packed_smem_bytes(const ChunkArgs &a) {
+    if (a.group > kPPp * (uint32_t)kPT) return 0;
+    return make_playout(a).total;
+}
+
+int launch_packed(hcx_ctx *ctx, const ChunkArgs &a, int kernel) {
+    if (a.q.nspec <= 0) return HCX_SUCCESS;
+    const size_t sm = packed_smem_bytes(a);
+    if (!sm || sm > (size_t)ctx->max_lds)
+        return set_error_code(HCX_ERR_INVLD_PARAM, "search: the packed scan does not fit in LDS");
+    hipStream_t st = ctx->stream;
+    if (kernel == HCX_KERNEL_BATCHED) {
+        HCX_HIP(hipFuncSetAttribute((const void *)k_items_packed, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+        k_items_packed<<<a.grid, kPT, sm, st>>>(a);
+    } else {
+        HCX_HIP(hipFuncSetAttribute((const void *)k_spec_packed, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+        k_spec_packed<<<(unsigned)a.q.nspec, kPT, sm, st>>>(a);
+    }
+    HCX_HIP(hipGetLastError());
+    return HCX_SUCCESS;
+}
+
+}  // namespace hcx

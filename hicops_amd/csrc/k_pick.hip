@@ -638,7 +638,7 @@ extern "C" int hcx_queries_pick(hcx_ctx *c, hcx_rawbatch *rb, const hcx_pick_par
     b->file_num = rb->file_num;
     if (b->moz.ensure(4 * tot + 16) || b->inten.ensure(4 * tot + 16) || b->idx.ensure(4ull * (n + 1)) ||
         b->prec.ensure(4ull * n + 16) || b->chg.ensure(4ull * n + 16) || b->rt.ensure(4ull * n + 16) ||
-        rb->ub.ensure(4ull * n + 16) || rb->cnt.ensure(4ull * n + 16) || rb->st.ensure(8 * tot + 16) ||
+        b->mult.ensure(4ull * n + 16) || rb->ub.ensure(4ull * n + 16) || rb->cnt.ensure(4ull * n + 16) || rb->st.ensure(8 * tot + 16) ||
         rb->summary.ensure(16) || rb->aux.ensure(8ull * n + 16) ||
         rb->scan.ensure(4 * hcx::scan_tmp_words((uint64_t)n + 1) + 64)) {
         delete b;
@@ -684,7 +684,7 @@ extern "C" int hcx_queries_pick(hcx_ctx *c, hcx_rawbatch *rb, const hcx_pick_par
     a.chg = (int32_t *)b->chg.p;
     a.rt = (float *)b->rt.p;
     a.imax_s = (uint32_t *)rb->aux.p;
-    a.mult_s = (uint32_t *)rb->aux.p + n;
+    a.mult_s = (uint32_t *)b->mult.p;   // kept with the batch: the search needs it per spectrum
     a.n = n;
     a.qalen = pp->qalen;
     a.base_int = pp->base_int;

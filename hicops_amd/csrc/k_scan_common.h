@@ -31,13 +31,6 @@ typedef AS_GLB uint32_t gu32;
 typedef AS_GLB unsigned long long gu64;
 
 constexpr uint32_t kNoProbe = 0xFFFFFFFFu;
-#ifndef HCX_GRAB
-#define HCX_GRAB 64
-#endif
-// items taken per queue atomic, at most: a grab of consecutive items is
-// mostly one spectrum's, so fewer grabs mean fewer per-spectrum merges; near
-// the end of the queue grabs shrink so the workgroups finish together
-constexpr uint32_t kGrab = HCX_GRAB;
 constexpr unsigned long long kLow25 = (1ull << 25) - 1;
 // A group's flat entries are walked in segments of kSegEntries; the run
 // starts of the current segment are a bitmap of kSegWords 64-bit words, one
@@ -435,6 +428,105 @@ __device__ __forceinline__ uint32_t load_peaks(const ChunkArgs &a, lu32 *pk_moz,
         pk_int[k] = inten[q0 + k];
     }
     return P;
+}
+
+// ---- the batched kernels' work items (ChunkArgs) ----
+
+// Spectrum s's windows into LDS (wlo / whi, nlen each) and, per length, its
+// first work item within the spectrum (lfirst, nlen + 1 entries; lfirst[nlen]
+// = the spectrum's item count).  A length's items are the chunks its window
+// touches.  Readers must be past a barrier.
+template <int BT>
+__device__ __forceinline__ void load_windows(const ChunkArgs &a, uint32_t s, li32 *wlo, li32 *whi, lu32 *lfirst) {
+    const int nlen = a.q.nlen;
+    const int2 *win = a.win + (size_t)s * nlen;
+    if (threadIdx.x < 64) {
+        uint32_t carry = 0;
+        for (int l0 = 0; l0 < nlen; l0 += 64) {
+            const int l = l0 + (int)threadIdx.x;
+            uint32_t cnt = 0;
+            if (l < nlen) {
+                const int2 w = win[l];
+                wlo[l] = w.x;
+                whi[l] = w.y;
+                if (w.x <= w.y) cnt = (uint32_t)w.y / a.tc - (uint32_t)w.x / a.tc + 1;
+            }
+            const uint32_t inc = wave_scan(cnt);
+            if (l < nlen) lfirst[l] = carry + inc - cnt;
+            carry += (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+        }
+        if (threadIdx.x == 0) lfirst[nlen] = carry;
+    }
+}
+
+// The spectrum holding item `it`: item_spec gives the spectrum holding item
+// 64 * (it / 64), and spectra are numbered in item order.  Uniform.
+__device__ __forceinline__ uint32_t spec_of_item(const ChunkArgs &a, uint32_t it) {
+    uint32_t s = ufl(((gcu32 *)a.item_spec)[it >> 6]);
+    while (ufl(((gcu32 *)a.item_base)[s + 1]) <= it) s++;
+    return s;
+}
+
+// Item k of the current spectrum (k < lfirst[nlen]): its length *l and chunk
+// *c.  Lengths without items have lfirst[l + 1] == lfirst[l].  Uniform.
+__device__ __forceinline__ void item_at(const ChunkArgs &a, const li32 *wlo, const lu32 *lfirst, uint32_t k,
+                                        uint32_t *l, uint32_t *c) {
+    uint32_t ll = 0;
+    while (ufl((uint32_t)lfirst[ll + 1]) <= k) ll++;
+    *l = ll;
+    *c = ufl((uint32_t)wlo[ll]) / a.tc + (k - ufl((uint32_t)lfirst[ll]));
+}
+
+// The persistent loop of the batched kernels: workgroups take runs of up to
+// a.grab consecutive items from the queue (guided: about a quarter of a
+// workgroup's share of what is left, so the workgroups finish together).
+// begin(s) makes spectrum s current (after flush(previous) when the
+// spectrum changes); item(s, l, c, imax_out) processes one item and records
+// its best score in imax_out.  The grab's best scores collect in LDS (imax,
+// kMaxGrab words) and go out together; s_item: one LDS word.
+template <int BT, typename Begin, typename Item, typename Flush>
+__device__ __forceinline__ void run_items(const ChunkArgs &a, lu32 *s_item, lu32 *imax, const li32 *wlo,
+                                          const lu32 *lfirst, Begin &&begin, Item &&item, Flush &&flush) {
+    const uint32_t total = ufl(((gcu32 *)a.item_base)[a.q.nspec]);
+    const uint32_t gmax = max(1u, min(a.grab, kMaxGrab));
+    uint32_t grab = max(1u, min(gmax, total / (4u * gridDim.x)));
+    uint32_t cur = 0xFFFFFFFFu, cbase = 0, cend = 0;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0)
+            *s_item = __hip_atomic_fetch_add((gu32 *)a.queue, grab, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const uint32_t it0 = ufl((uint32_t)*s_item);
+        if (it0 >= total) break;
+        const uint32_t it1 = min(total, it0 + grab);
+        grab = max(1u, min(gmax, (total - it1) / (4u * gridDim.x)));
+        const bool rec = a.item_max != nullptr;
+        if (rec && threadIdx.x < it1 - it0) imax[threadIdx.x] = 0;
+        uint32_t s = (cur != 0xFFFFFFFFu && it0 >= cbase && it0 < cend) ? cur : spec_of_item(a, it0);
+        for (uint32_t it = it0; it < it1; it++) {
+            while (it >= cend || s != cur) {
+                if (s == cur) s++;   // past the current spectrum: the next with items
+                const uint32_t b0 = ufl(((gcu32 *)a.item_base)[s]), b1 = ufl(((gcu32 *)a.item_base)[s + 1]);
+                if (it >= b1) {
+                    s++;
+                    continue;
+                }
+                if (cur != 0xFFFFFFFFu) flush(cur);
+                cur = s;
+                cbase = b0;
+                cend = b1;
+                begin(s);
+                __syncthreads();
+            }
+            uint32_t l, c;
+            item_at(a, wlo, lfirst, it - cbase, &l, &c);
+            item(cur, l, c, rec ? imax + (it - it0) : nullptr);
+        }
+        __syncthreads();
+        if (rec && threadIdx.x < it1 - it0 && it0 + threadIdx.x < a.item_cap)
+            ((AS_GLB uint32_t *)a.item_max)[it0 + threadIdx.x] = imax[threadIdx.x];
+    }
+    if (cur != 0xFFFFFFFFu) flush(cur);
 }
 
 }  // namespace

@@ -38,13 +38,17 @@ RES_FIT_UB = 0x20
 RES_SURROGATE = 0x40
 RES_AUDITED = 0x80
 RES_COUNT_OVF = 0x100
+RES_TWO_PASS = 0x200
 
 KERNEL_PLAIN = 0
 KERNEL_BATCHED = 1
 KERNEL_CURSOR = 2
 OPT_HITLIST = 1
-OPT_ITEM_ORDER = 2
 OPT_GROUP = 3
+OPT_SCAN = 4
+OPT_GRAB = 5
+SCAN_AUTO = 0
+SCAN_TWO_PASS = 1
 EVAL_TAILFIT = 0
 EVAL_SURROGATE = 1
 
@@ -102,7 +106,7 @@ class SearchOpts(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [
         ("chunk", C.c_uint32), ("slots", C.c_uint32), ("blocks_per_cu", C.c_uint32),
-        ("hitlist", C.c_int32), ("item_order", C.c_int32), ("kernel_ms", C.c_float),
+        ("hitlist", C.c_int32), ("grab", C.c_int32), ("kernel_ms", C.c_float),
     ]
 
 
@@ -186,7 +190,7 @@ def hcx() -> C.CDLL:
                      "hcx_selftest_divider", "hcx_set_tuning", "hcx_get_tuning", "hcx_set_option", "hcx_get_option", "hcx_autotune",
                      "hcx_surrogate_param_count", "hcx_surrogate_init", "hcx_surrogate_set",
                      "hcx_surrogate_get", "hcx_surrogate_train", "hcx_surrogate_gradient",
-                     "hcx_surrogate_last_audit", "hcx_selftest_mfma", "hcx_last_timing",
+                     "hcx_surrogate_last_audit", "hcx_selftest_mfma", "hcx_last_timing", "hcx_last_scan_split",
                      "hcx_raw_upload", "hcx_raw_free", "hcx_queries_pick", "hcx_queries_pickpeaks",
                      "hcx_queries_info", "hcx_queries_export"]:
             getattr(lib, name).restype = C.c_int

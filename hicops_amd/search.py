@@ -145,9 +145,14 @@ class Engine:
         """LDS hit-list entries per work item: -1 as many as fit, 0 none."""
         N.check(N.hcx().hcx_set_option(self._h, N.OPT_HITLIST, C.c_int64(entries)))
 
-    def set_item_order(self, by_chunk: bool) -> None:
-        """Batched work items by spectrum (False) or by (length, chunk) (True)."""
-        N.check(N.hcx().hcx_set_option(self._h, N.OPT_ITEM_ORDER, C.c_int64(1 if by_chunk else 0)))
+    def set_scan(self, mode: int) -> None:
+        """N.SCAN_AUTO: the packed single-pass scan where it is exact, the
+        two-pass scan elsewhere; N.SCAN_TWO_PASS: the two-pass scan throughout."""
+        N.check(N.hcx().hcx_set_option(self._h, N.OPT_SCAN, C.c_int64(mode)))
+
+    def set_grab(self, items: int) -> None:
+        """Batched kernels: work items per queue grab, at most (1..256)."""
+        N.check(N.hcx().hcx_set_option(self._h, N.OPT_GRAB, C.c_int64(items)))
 
     def set_group(self, probes: int) -> None:
         """(peak, bin) probes per run table of the chunked kernels: 0 automatic, else 64..2560."""
@@ -157,8 +162,11 @@ class Engine:
         ch, sl, bp = C.c_uint32(), C.c_uint32(), C.c_uint32()
         cu, lds = C.c_int32(), C.c_int32()
         N.check(N.hcx().hcx_get_tuning(self._h, C.byref(ch), C.byref(sl), C.byref(bp), C.byref(cu), C.byref(lds)))
+        grab, scan = C.c_int64(), C.c_int64()
+        N.check(N.hcx().hcx_get_option(self._h, N.OPT_GRAB, C.byref(grab)))
+        N.check(N.hcx().hcx_get_option(self._h, N.OPT_SCAN, C.byref(scan)))
         return {"chunk": ch.value, "slots": sl.value, "blocks_per_cu": bp.value, "num_cu": cu.value,
-                "max_lds": lds.value}
+                "max_lds": lds.value, "grab": grab.value, "scan": "auto" if scan.value == N.SCAN_AUTO else "two-pass"}
 
     # DSLIM_Construct, one length at a time (hicops.cpp:200-300 loop)
     def construct(self, peplen: int, seqs: np.ndarray, entries: np.ndarray) -> Index:
@@ -270,7 +278,10 @@ class Engine:
         """Device ms of the last query_device: scan kernel, e-value stage, whole search."""
         q, e, t = C.c_float(), C.c_float(), C.c_float()
         N.check(N.hcx().hcx_last_timing(self._h, C.byref(q), C.byref(e), C.byref(t)))
-        return {"scan_ms": q.value, "evalue_ms": e.value, "total_ms": t.value}
+        pk, tp = C.c_float(), C.c_float()
+        N.check(N.hcx().hcx_last_scan_split(self._h, C.byref(pk), C.byref(tp)))
+        return {"scan_ms": q.value, "evalue_ms": e.value, "total_ms": t.value,
+                "packed_ms": pk.value, "two_pass_ms": tp.value}
 
     def last_audit(self) -> dict:
         a = N.Audit()

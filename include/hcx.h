@@ -125,8 +125,10 @@ typedef struct hcx_result {
 #define HCX_RES_FIT_UB       0x0020u /* tail fit hit an undefined path of the reference */
 #define HCX_RES_SURROGATE    0x0040u /* e-value from the surrogate model               */
 #define HCX_RES_AUDITED      0x0080u /* spectrum was in the surrogate audit sample      */
-#define HCX_RES_COUNT_OVF    0x0100u /* a chunk hit counter neared overflow: rerun the
-                                        spectrum with HCX_KERNEL_CURSOR                  */
+#define HCX_RES_COUNT_OVF    0x0100u /* a two-pass chunk hit counter neared overflow; the
+                                        search re-ran the spectrum on HCX_KERNEL_CURSOR  */
+#define HCX_RES_TWO_PASS     0x0200u /* scored by the two-pass kernel (outside the packed
+                                        scan's exact range, or HCX_SCAN_TWO_PASS)        */
 
 /* Per-spectrum work counters (the algorithmic-bytes inputs). */
 typedef struct hcx_stats {
@@ -175,16 +177,23 @@ int  hcx_set_tuning(hcx_ctx *ctx, uint32_t chunk, uint32_t slots, uint32_t block
  *   of each work item are kept in LDS so the exact-count pass need not read
  *   iA again.  < 0: as many as fit (default); 0: none; n > 0: at most n
  *   (rounded down to a multiple of 64).
- * HCX_OPT_ITEM_ORDER: order of the batched kernel's work items.
- *   0: by spectrum (default); 1: by (length, chunk), so that concurrent
- *   workgroups share chunks of iA in L2.
- * HCX_OPT_GROUP: (peak, bin) probes per run table of the chunked kernels.
+ * HCX_OPT_GROUP: (peak, bin) probes per run table of the two-pass kernels.
  *   0 (default): as many as the workgroup's LDS share holds beside the other
  *   arrays, up to 2560 and not below 1024; n: 64..2560.  A work item with
- *   more probes walks its entries group by group (12 bytes of LDS per probe). */
+ *   more probes walks its entries group by group (12 bytes of LDS per probe).
+ *   The packed kernels always take as many as fit (up to 2048).
+ * HCX_OPT_SCAN: HCX_SCAN_AUTO (default): the single-pass packed scan for
+ *   every spectrum whose counts and intensity sums provably fit its packed
+ *   words, the two-pass scan for the rest; HCX_SCAN_TWO_PASS: the two-pass
+ *   scan for every spectrum.  Results are identical.
+ * HCX_OPT_GRAB: batched kernels: work items per queue grab, at most (1..256,
+ *   default 64). */
 #define HCX_OPT_HITLIST    1
-#define HCX_OPT_ITEM_ORDER 2
 #define HCX_OPT_GROUP      3
+#define HCX_OPT_SCAN       4
+#define HCX_OPT_GRAB       5
+#define HCX_SCAN_AUTO      0
+#define HCX_SCAN_TWO_PASS  1
 int  hcx_set_option(hcx_ctx *ctx, int32_t option, int64_t value);
 int  hcx_get_option(const hcx_ctx *ctx, int32_t option, int64_t *value);
 int  hcx_get_tuning(const hcx_ctx *ctx, uint32_t *chunk, uint32_t *slots, uint32_t *blocks_per_cu,
@@ -264,18 +273,21 @@ int  hcx_search(hcx_ctx *ctx, hcx_index *const *idx, int nidx, hcx_qbatch *qb,
                 const hcx_search_opts *opts, hcx_result *results,
                 uint32_t *hist, hcx_stats *stats);
 
-/* Autotuner (no reference counterpart): times the scan kernel of `kernel`
- * (HCX_KERNEL_BATCHED or _PLAIN) on this batch for each candidate -- slots
- * 512/1024/1536/2048, workgroups per CU 1/2/3 and, for the batched form, both item
- * orders -- keeps the fastest in the context and returns it.  The chunk size
- * is the indices' own (they are built for it).  Runs 2 searches per candidate. */
+/* Autotuner (no reference counterpart): times the scan of `kernel`
+ * (HCX_KERNEL_BATCHED or _PLAIN) on this batch for each candidate, keeps the
+ * fastest in the context and returns it.  With HCX_SCAN_AUTO the candidates
+ * are the batched kernels' grab sizes 8..256 (the packed kernel's LDS layout
+ * is fixed by the chunk size); with HCX_SCAN_TWO_PASS, slots
+ * 512/1024/1536/2048 x workgroups per CU 1/2/3 (x grab 32/64/128 for the
+ * batched form).  The chunk size is the indices' own (they are built for
+ * it).  Runs 2 searches per candidate. */
 typedef struct hcx_tuning {
     uint32_t chunk;
     uint32_t slots;
     uint32_t blocks_per_cu;
     int32_t  hitlist;
-    int32_t  item_order;
-    float    kernel_ms;     /* the chosen candidate's scan-kernel time */
+    int32_t  grab;          /* batched: work items per queue grab */
+    float    kernel_ms;     /* the chosen candidate's scan time */
 } hcx_tuning;
 int  hcx_autotune(hcx_ctx *ctx, hcx_index *const *idx, int nidx, hcx_qbatch *qb, int32_t kernel,
                   hcx_tuning *best);
@@ -294,6 +306,10 @@ int  hcx_search_fetch(hcx_ctx *ctx, hcx_qbatch *qb, hcx_result *results,
  * replayed from a hipGraph is timed as a whole: its scan and e-value times
  * are the graph's.                                                          */
 int  hcx_last_timing(hcx_ctx *ctx, float *scan_ms, float *evalue_ms, float *total_ms);
+/* The last timed search's scan split by kernel (HIP events): the packed
+ * kernel, and the two-pass kernel over the spectra the packed one leaves (or
+ * over all, with HCX_SCAN_TWO_PASS).  Either is 0 when it did not run.      */
+int  hcx_last_scan_split(hcx_ctx *ctx, float *packed_ms, float *two_pass_ms);
 
 /* Candidate dump for parity tests: every candidate with shpk >= min_shp
  * and hs > 0, as (spectrum, idxoffset, psid, bc, yc, ibc, iyc, hs).

@@ -114,38 +114,84 @@ def test_index_parity_mods(maxz):
 KERNELS = {"plain": N.KERNEL_PLAIN, "batched": N.KERNEL_BATCHED, "cursor": N.KERNEL_CURSOR}
 
 
+SCANS = {"auto": N.SCAN_AUTO, "two-pass": N.SCAN_TWO_PASS}
+
+
+@pytest.mark.parametrize("scan", ["auto", "two-pass"])
 @pytest.mark.parametrize("kernel", ["plain", "batched", "cursor"])
-def test_query_parity_small(small_case, kernel):
+def test_query_parity_small(small_case, kernel, scan):
     p, db, q, prm = small_case
-    log(f"test_query_parity_small kernel={kernel}")
+    if kernel == "cursor" and scan == "two-pass":
+        pytest.skip("the cursor kernel has no scan modes")
+    log(f"test_query_parity_small kernel={kernel} scan={scan}")
     odb = O.OracleDB(prm, db)
     res_o, hist_o, st_o = odb.search(q)
     eng = Engine(prm)
+    eng.set_scan(SCANS[scan])
     eng.construct_db(db)
     b = eng.upload(q)
     out = eng.query_spectrum(b, kernel=KERNELS[kernel])
-    n = compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"small-{kernel}")
+    n = compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"small-{kernel}-{scan}")
     assert n > 100
-    # a second search on the same batch (cached work items) gives the same answer
+    two = (out.results["flags"] & N.RES_TWO_PASS) != 0
+    if kernel != "cursor":
+        # nearly every spectrum of this batch is within the packed scan's exact range
+        assert two.all() if scan == "two-pass" else two.sum() < 0.05 * q.nspec, int(two.sum())
+    # a second search on the same batch (work items rebuilt) gives the same answer
     out2 = eng.query_spectrum(b, kernel=KERNELS[kernel])
     assert np.array_equal(out2.results, out.results) and np.array_equal(out2.hist, out.hist)
     eng.close()
 
 
-def test_batched_items_by_chunk(small_case):
-    # work items bucketed by (length, chunk): many workgroups merge into each spectrum
+def crowd_peaks(q, every, copies):
+    """Every `every`-th spectrum gets `copies` extra peaks at the bin of its
+    first peak: more peaks within 2 dF + 1 bins than the packed scan admits,
+    so those spectra go to the two-pass kernel."""
+    moz, inten, idx = [], [], [0]
+    for s in range(q.nspec):
+        m = q.moz[q.idx[s]:q.idx[s + 1]]
+        i = q.intensity[q.idx[s]:q.idx[s + 1]]
+        if s % every == 0 and len(m):
+            m = np.concatenate([m, np.full(copies, m[0], np.uint32)])
+            i = np.concatenate([i, np.full(copies, i[0], np.uint32)])
+        moz.append(m)
+        inten.append(i)
+        idx.append(idx[-1] + len(m))
+    return W.Queries(np.concatenate(moz).astype(np.uint32), np.concatenate(inten).astype(np.uint32),
+                     np.asarray(idx, np.uint32), q.precurse.copy(), q.charges.copy(), q.rtimes.copy(),
+                     q.src_len.copy(), q.src_psid.copy())
+
+
+@pytest.mark.parametrize("kernel", ["plain", "batched"])
+def test_packed_and_two_pass_mixed(small_case, kernel):
+    # a batch split between the packed kernel and the two-pass kernel's list
+    p, db, q0, prm = small_case
+    q = crowd_peaks(q0, 3, 3)
+    log(f"test_packed_and_two_pass_mixed kernel={kernel}")
+    res_o, hist_o, st_o = O.OracleDB(prm, db).search(q)
+    eng = Engine(prm)
+    eng.construct_db(db)
+    out = eng.query_spectrum(eng.upload(q), kernel=KERNELS[kernel])
+    compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"mixed-{kernel}")
+    two = (out.results["flags"] & N.RES_TWO_PASS) != 0
+    crowded = (np.arange(q.nspec) % 3 == 0) & (np.diff(q0.idx.astype(np.int64)) > 0)
+    assert two[crowded].all() and two[~crowded].sum() < 0.05 * q.nspec
+    eng.close()
+
+
+@pytest.mark.parametrize("scan", ["auto", "two-pass"])
+@pytest.mark.parametrize("grab", [1, 7, 256])
+def test_batched_grab_sizes(small_case, scan, grab):
+    # grabs of one item (a merge per item), of an odd size and of the largest
     p, db, q, prm = small_case
-    log("test_batched_items_by_chunk")
+    log(f"test_batched_grab_sizes scan={scan} grab={grab}")
     res_o, hist_o, st_o = O.OracleDB(prm, db).search(q)
     eng = Engine(prm, chunk=1024)
-    eng.set_item_order(True)
+    eng.set_scan(SCANS[scan])
+    eng.set_grab(grab)
     eng.construct_db(db)
-    b = eng.upload(q)
-    out = eng.query_spectrum(b, kernel=N.KERNEL_BATCHED)
-    compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, "by-chunk")
-    eng.set_item_order(False)   # the cached item list is rebuilt
-    out = eng.query_spectrum(b, kernel=N.KERNEL_BATCHED)
-    compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, "by-spectrum")
+    out = eng.query_spectrum(eng.upload(q), kernel=N.KERNEL_BATCHED)
+    compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"grab-{scan}-{grab}")
     eng.close()
 
 
@@ -178,13 +224,30 @@ def test_chunked_edges_and_slot_rounds(edge_case, kernel, chunk, slots, hitlist)
     # small chunks: windows span many chunks and start/end inside chunks;
     # few slots: peptides reaching min_shp overflow the slot table (several rounds);
     # hit list off / 64 entries: the exact-count pass reads iA for all or most entries
+    # (the two-pass kernel's own knobs: the packed scan is off)
     db, q, prm, (res_o, hist_o, st_o) = edge_case
     log(f"test_chunked_edges_and_slot_rounds kernel={kernel} chunk={chunk} slots={slots} hitlist={hitlist}")
     eng = Engine(prm, chunk=chunk, slots=slots, hitlist=hitlist)
+    eng.set_scan(N.SCAN_TWO_PASS)
     eng.construct_db(db)
     out = eng.query_spectrum(eng.upload(q), kernel=KERNELS[kernel])
     n = compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"{kernel}-{chunk}-{slots}-{hitlist}")
     assert n > 50
+    eng.close()
+
+
+@pytest.mark.parametrize("kernel", ["plain", "batched"])
+@pytest.mark.parametrize("chunk", [256, 1024, 8192])
+def test_packed_chunk_edges(edge_case, kernel, chunk):
+    # the packed scan with windows that span many chunks and start / end
+    # inside chunks
+    db, q, prm, (res_o, hist_o, st_o) = edge_case
+    log(f"test_packed_chunk_edges kernel={kernel} chunk={chunk}")
+    eng = Engine(prm, chunk=chunk)
+    eng.construct_db(db)
+    out = eng.query_spectrum(eng.upload(q), kernel=KERNELS[kernel])
+    n = compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"packed-{kernel}-{chunk}")
+    assert n > 50 and ((out.results["flags"] & N.RES_TWO_PASS) != 0).sum() < 0.05 * q.nspec
     eng.close()
 
 
@@ -200,10 +263,12 @@ def test_open_search_groups_without_hit_list(hitlist):
     eng = Engine(prm, hitlist=hitlist)
     eng.construct_db(db)
     b = eng.upload(q)
-    for kernel in ("plain", "batched"):
-        out = eng.query_spectrum(b, kernel=KERNELS[kernel])
-        n = compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"groups-{hitlist}-{kernel}")
-        assert n > 50
+    for scan in ("auto", "two-pass"):
+        eng.set_scan(SCANS[scan])
+        for kernel in ("plain", "batched"):
+            out = eng.query_spectrum(b, kernel=KERNELS[kernel])
+            n = compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"groups-{hitlist}-{kernel}-{scan}")
+            assert n > 50
     eng.close()
 
 
@@ -224,14 +289,16 @@ def test_candidate_sets_bitexact(small_case):
     eng.close()
 
 
+@pytest.mark.parametrize("scan", ["auto", "two-pass"])
 @pytest.mark.parametrize("kernel", ["plain", "batched"])
-def test_ties_replayed_in_reference_order(small_case, kernel):
+def test_ties_replayed_in_reference_order(small_case, kernel, scan):
     p, db, q, prm = small_case
-    log(f"test_ties_replayed_in_reference_order kernel={kernel}")
+    log(f"test_ties_replayed_in_reference_order kernel={kernel} scan={scan}")
     twin = TwinDB(db)
     odb = O.OracleDB(prm, twin)
     res_o, hist_o, st_o = odb.search(q)
     eng = Engine(prm)
+    eng.set_scan(SCANS[scan])
     eng.construct_db(twin)
     out = eng.query_spectrum(eng.upload(q), kernel=KERNELS[kernel])
     valid = (res_o["flags"] & N.RES_VALID) != 0
@@ -341,23 +408,26 @@ def test_graph_replays_match_eager(small_case):
     eng.close()
 
 
-def test_autotune_keeps_parity():
+@pytest.mark.parametrize("scan", ["auto", "two-pass"])
+def test_autotune_keeps_parity(scan):
     p = W.scaled("open10m", npep=100000, nspec=300)
     db = W.Database(p)
     q = W.make_queries(db, p)
     prm = W.make_params(p)
-    log("test_autotune_keeps_parity")
+    log(f"test_autotune_keeps_parity scan={scan}")
     res_o, hist_o, st_o = O.OracleDB(prm, db).search(q)
     eng = Engine(prm)
+    eng.set_scan(SCANS[scan])
     eng.construct_db(db)
     b = eng.upload(q)
     best = eng.autotune(b, N.KERNEL_BATCHED)
     log(f"autotune: {best}")
-    assert best["kernel_ms"] > 0 and best["slots"] in (512, 1024, 1536, 2048) and best["blocks_per_cu"] in (1, 2, 3)
+    assert best["kernel_ms"] > 0 and best["grab"] in (8, 16, 32, 64, 128, 256)
+    assert best["slots"] in (512, 1024, 1536, 2048) and best["blocks_per_cu"] in (1, 2, 3)
     t = eng.tuning()
-    assert (t["slots"], t["blocks_per_cu"]) == (best["slots"], best["blocks_per_cu"])
+    assert (t["slots"], t["blocks_per_cu"], t["grab"]) == (best["slots"], best["blocks_per_cu"], best["grab"])
     out = eng.query_spectrum(b, kernel=N.KERNEL_BATCHED)
-    compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, "autotuned")
+    compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"autotuned-{scan}")
     eng.close()
 
 
