@@ -668,7 +668,9 @@ int plan_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, int
     if (sp.pack) {
         ChunkArgs &pa = sp.pa;
         pa = ca;
-        pa.group = std::min<uint32_t>(need, 2 * kPackedThreads);
+        // (a run table larger than the batch needs is not wasted: the scan's
+        // candidate queues use its space)
+        pa.group = 2 * kPackedThreads;
         while (pa.group > 64 && (!packed_smem_bytes(pa) || packed_smem_bytes(pa) > (size_t)c->max_lds))
             pa.group -= 64;
         sp.pack = packed_smem_bytes(pa) && packed_smem_bytes(pa) <= (size_t)c->max_lds;
@@ -1086,6 +1088,21 @@ extern "C" int hcx_tailfit_batch(hcx_ctx *c, const uint32_t *hist, uint32_t hbin
         nexthyp[s] = o[4 * s + 3];
         if (flags) flags[s] = fl[s];
     }
+    return HCX_SUCCESS;
+}
+
+// v_log_f32 of 1 .. n on the device (the packed kernel's score estimate).
+extern "C" int hcx_selftest_log2(hcx_ctx *c, uint32_t n, float *out, double *bound) {
+    if (!c || (n && !out)) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_selftest_log2: null argument");
+    if (bound) *bound = kLog2Err;
+    if (n == 0) return HCX_SUCCESS;
+    HCX_HIP(hipSetDevice(c->device));
+    hcx::DeviceBuf d;
+    if (d.ensure(4ull * n)) return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_selftest_log2: alloc");
+    int rc = selftest_log2(c, n, (float *)d.p);
+    if (rc) return rc;
+    HCX_HIP(hipMemcpyAsync(out, d.p, 4ull * n, hipMemcpyDeviceToHost, c->stream));
+    HCX_HIP(hipStreamSynchronize(c->stream));
     return HCX_SUCCESS;
 }
 

@@ -227,6 +227,11 @@ int launch_chunked(hcx_ctx *ctx, const ChunkArgs &a, int kernel);
 int launch_finalize(hcx_ctx *ctx, const ChunkArgs &a);
 // ---- packed single-pass search (k_packed.hip) ----
 constexpr uint32_t kPackedThreads = 1024;
+// Bound on |v_log_f32(n) - log2(n)| for the integers n = 1 .. 2^24 + 1 that the
+// packed kernel's score estimate assumes (the device's measured error, from
+// hcx_selftest_log2, must stay well below it).
+constexpr double kLog2Err = 2.0e-5;
+int selftest_log2(hcx_ctx *ctx, uint32_t n, float *out);
 // LDS the packed kernels need with a.group probes per run table, or 0 when
 // a.tc peptides' accumulators do not fit beside the other arrays
 size_t packed_smem_bytes(const ChunkArgs &a);

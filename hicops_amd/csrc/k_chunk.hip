@@ -348,7 +348,7 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
                     build_seg<kBT>(sm.seg32, sm.cofs, nc, s0);
                     seg_cur = s0;
                 }
-                walk_tiles<kBT>(
+                walk_tiles<kBT, kTileU>(
                     sm.seg64, sm.cofs, nc, s0, min(E, s0 + kSegEntries),
                     [&](const Tile (&t)[kTileU], const uint32_t (&x)[kTileU], const bool (&)[kTileU],
                         uint32_t (&v)[kTileU], uint32_t) {
@@ -457,7 +457,7 @@ __device__ __forceinline__ void process_item(const ChunkArgs &a, const Smem &sm,
                     build_seg<kBT>(sm.seg32, sm.cofs, nc, s0);
                     seg_cur = s0;
                 }
-                walk_tiles<kBT>(
+                walk_tiles<kBT, kTileU>(
                     sm.seg64, sm.cofs, nc, s0, min(E, s0 + kSegEntries),
                     [&](const Tile (&t)[kTileU], const uint32_t (&x)[kTileU], const bool (&ok)[kTileU],
                         uint32_t (&v)[kTileU], uint32_t fs) {

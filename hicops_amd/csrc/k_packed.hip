@@ -45,6 +45,15 @@ constexpr int kPT = (int)kPackedThreads;
 // run-table probes per thread: a group holds at most kPPp * kPT probes
 constexpr uint32_t kPPp = 2;
 constexpr uint32_t kCnt = 1u << 24;   // count unit of a packed word
+constexpr double kLog10of2 = 0.30102999566398119521;
+// tiles per wavefront step (16, every load of an item at once, measured
+// slower: register pressure)
+#ifndef HCX_PACKED_TILE_U
+#define HCX_PACKED_TILE_U 8
+#endif
+constexpr uint32_t kUp = HCX_PACKED_TILE_U;
+// accumulator reads in flight per thread in the scoring scan
+constexpr uint32_t kSr = 4;
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 typedef AS_LDS u32x2 lu32x2;
 
@@ -60,7 +69,7 @@ struct PLayout {
     uint32_t lgf;       // max_shdpeaks + 1 doubles
     uint32_t wlo, whi;  // nlen
     uint32_t lfirst;    // nlen + 1
-    uint32_t ctl;       // 8 words: [2..5] spectrum merge, [6] grabbed item
+    uint32_t ctl;       // 8 words: [0] best exact score of the spectrum, [2..5] spectrum merge, [6] grabbed item
     uint32_t imax;      // kMaxGrab
     uint32_t red64;     // 2 u64
     uint32_t total;
@@ -97,6 +106,8 @@ __host__ __device__ inline PLayout make_playout(const ChunkArgs &a) {
 
 struct PSmem {
     lu32 *acc, *cofs, *pk_moz, *pk_int, *wsum, *hist, *lfirst, *ctl, *imax;
+    uint32_t cand_cap;   // candidate list entries (12 bytes) the run table's space holds
+    lu32 *hbest;         // best exact score of the current spectrum so far (float bits)
     lu32x2 *runs;
     lu32 *seg32;
     lu64 *seg64, *red64;
@@ -123,6 +134,8 @@ __device__ __forceinline__ PSmem pbind(unsigned char *raw, const PLayout &l) {
     s.ctl = (lu32 *)(b + l.ctl);
     s.imax = (lu32 *)(b + l.imax);
     s.red64 = (lu64 *)(b + l.red64);
+    s.cand_cap = (l.pk_moz - l.runs) / 12;
+    s.hbest = s.ctl;
     return s;
 }
 
@@ -178,7 +191,19 @@ __device__ __forceinline__ uint2 setup_runs_p(const ChunkArgs &a, const PSmem &s
     return make_uint2(ufl(tot.y), ufl(tot.x));
 }
 
-// Score one peptide from its two packed words (dslim_query.cpp:679-722).
+// Score one peptide from its two packed words (dslim_query.cpp:679-722):
+//   hs = (float)(((lgf[bc] + lgf[yc]) + log10(1 + ibc)) + log10(1 + iyc) - 6)
+// in FP64 (the exact path).  Most candidates need only three facts about hs:
+// its sign, its histogram bin and that it is below the spectrum's best; an
+// estimate with a proven error bound settles them without the two FP64
+// log10: log2 of the exact integer arguments (at most 2^24, exact in float)
+// by the hardware's v_log_f32, whose largest error over 1 .. 2^24 + 1 is
+// below kLog2Err (hcx_selftest_log2 measures it on the device).  The exact
+// path runs whenever the estimate's interval [ha - E, ha + E] leaves a fact
+// open: the sign, a bin boundary within reach, or a score that could reach the
+// best exact score of the spectrum seen so far (*hbest, float bits shared by
+// the workgroup: a stale value is lower, so only safer).  The spectrum's
+// maximum and every peptide tied with it are therefore always exact.
 __device__ __forceinline__ void score_peptide(const ChunkArgs &a, const PSmem &sm, uint32_t spec, uint32_t ixx,
                                               uint32_t pid, uint32_t wb, uint32_t wy, Acc &acc, float &imax) {
     const uint32_t bc = wb >> 24, yc = wy >> 24;
@@ -187,26 +212,54 @@ __device__ __forceinline__ void score_peptide(const ChunkArgs &a, const PSmem &s
     uint32_t bi = bc, yi = yc;
     if (bi > a.q.max_shdpeaks) { bi = a.q.max_shdpeaks; acc.flags |= HCX_RES_LGF_CLAMP; }
     if (yi > a.q.max_shdpeaks) { yi = a.q.max_shdpeaks; acc.flags |= HCX_RES_LGF_CLAMP; }
-    double t = sm.lgf[bi] + sm.lgf[yi];
-    t = t + log10((double)(1u + ib));
-    t = t + log10((double)(1u + iy));
-    t = t - 6.0;
-    const float hs = (float)t;
-    if (!(hs > 0.0f)) return;
+    const double lg = sm.lgf[bi] + sm.lgf[yi];
+    // the estimate and its error bound (the FP64 roundings of both paths
+    // are below 1e-12 at these magnitudes)
+    const double l2 = (double)__builtin_amdgcn_logf((float)(1u + ib)) + (double)__builtin_amdgcn_logf((float)(1u + iy));
+    const double ha = (lg + l2 * kLog10of2) - 6.0;
+    const double E = 2.0 * kLog2Err * kLog10of2 + 1e-12;
+    const double hi = ha + E;
+    if (!(hi > 0.0)) return;   // hs <= 0: not a candidate
+    const float H = __uint_as_float(*sm.hbest);
+    // fl(D) <= D + 2^-24 D: below H for sure only if hi (1 + 2^-23) < H
+    bool exact = !(ha - E > 0.0) || hi * (1.0 + 0x1p-23) >= (double)H;
+    int bin = 0;
+    if (!exact) {
+        // bin = floor(fl(fl(hs) * 10) + 0.5); the two float roundings move
+        // 10 hs by at most 2^-23 of it
+        const double z = 10.0 * ha, M = 10.0 * E + 0x1p-22 * z;
+        const double b0 = floor(z - M + 0.5), b1 = floor(z + M + 0.5);
+        if (b0 != b1) exact = true;
+        else bin = (int)b0;
+    }
+    float hs;
+    if (exact) {
+        double t = lg;
+        t = t + log10((double)(1u + ib));
+        t = t + log10((double)(1u + iy));
+        t = t - 6.0;
+        hs = (float)t;
+        if (!(hs > 0.0f)) return;
+        bin = (int)((double)(hs * 10.0f) + 0.5);
+        if (hs > H) lds_max(sm.hbest, __float_as_uint(hs));
+        if (hs > acc.best_hs) {
+            acc.best_hs = hs;
+            acc.best_cnt = 1;
+            acc.best_id = (ixx << 25) | pid;
+            acc.best_by = (bc << 16) | yc;
+        } else if (hs == acc.best_hs) {
+            acc.best_cnt++;
+        }
+    } else {
+        // an upper bound of hs, for the item's best (the tie replay only
+        // skips items whose best is below the heap's root)
+        hs = (float)(hi * (1.0 + 1e-6));
+    }
     acc.cpsms++;
     imax = fmaxf(imax, hs);
-    int bin = (int)((double)(hs * 10.0f) + 0.5);
     if (bin >= (int)a.q.hbins) { bin = (int)a.q.hbins - 1; acc.flags |= HCX_RES_HIST_CLAMP; }
     if ((uint32_t)bin < kHistLds) lds_add(&sm.hist[bin], 1u);
     else glb_add(a.q.hist + (size_t)spec * a.q.hbins + bin, 1u);
-    if (hs > acc.best_hs) {
-        acc.best_hs = hs;
-        acc.best_cnt = 1;
-        acc.best_id = (ixx << 25) | pid;
-        acc.best_by = (bc << 16) | yc;
-    } else if (hs == acc.best_hs) {
-        acc.best_cnt++;
-    }
 }
 
 // One (spectrum, length, chunk) item; the spectrum's peaks are in LDS.
@@ -240,14 +293,14 @@ __device__ __forceinline__ void process_item_p(const ChunkArgs &a, const PSmem &
             if (FULL && threadIdx.x == 0) nia_w += E;   // per wavefront otherwise
             for (uint32_t s0 = 0; s0 < E; s0 += kSegEntries) {
                 if (s0) build_seg<kPT>(sm.seg32, sm.cofs, nc, s0);
-                uint32_t cv[kTileU];
-                walk_tiles<kPT>(
+                uint32_t cv[kUp];
+                walk_tiles<kPT, kUp>(
                     sm.seg64, sm.cofs, nc, s0, min(E, s0 + kSegEntries),
-                    [&](const Tile (&t)[kTileU], const uint32_t (&x)[kTileU], const bool (&)[kTileU],
-                        uint32_t (&v)[kTileU], uint32_t) {
-                        uint32_t gi[kTileU];
+                    [&](const Tile (&t)[kUp], const uint32_t (&x)[kUp], const bool (&)[kUp],
+                        uint32_t (&v)[kUp], uint32_t) {
+                        uint32_t gi[kUp];
 #pragma unroll
-                        for (uint32_t u = 0; u < kTileU; u++) {
+                        for (uint32_t u = 0; u < kUp; u++) {
                             const u32x2 r = (sm.runs + t[u].C)[tile_below(t[u])];
                             gi[u] = r.x + x[u];
                             cv[u] = r.y;
@@ -255,12 +308,12 @@ __device__ __forceinline__ void process_item_p(const ChunkArgs &a, const PSmem &
                         // lanes outside the share read some entry or, past
                         // the array, 0 (buffer range check); they add nothing
 #pragma unroll
-                        for (uint32_t u = 0; u < kTileU; u++) v[u] = iac_load(rs_iac, gi[u]);
+                        for (uint32_t u = 0; u < kUp; u++) v[u] = iac_load(rs_iac, gi[u]);
                     },
-                    [&](const Tile (&)[kTileU], const uint32_t (&v)[kTileU], const uint32_t (&)[kTileU],
-                        const bool (&ok)[kTileU], uint32_t) {
+                    [&](const Tile (&)[kUp], const uint32_t (&v)[kUp], const uint32_t (&)[kUp],
+                        const bool (&ok)[kUp], uint32_t) {
 #pragma unroll
-                        for (uint32_t u = 0; u < kTileU; u++) {
+                        for (uint32_t u = 0; u < kUp; u++) {
                             const bool win = FULL ? ok[u] : ok[u] && (v[u] >> 1) - wlo <= wspan;
                             if (!FULL) nia_w += (uint32_t)__popcll(__ballot(win));
                             if (win) lds_add(&sm.acc[v[u]], cv[u]);
@@ -276,15 +329,77 @@ __device__ __forceinline__ void process_item_p(const ChunkArgs &a, const PSmem &
     if (lane == 0) acc.nia += nia_w;
 
     // S: the window's words, two peptides per 128-bit read; read words are
-    // cleared for the next item (a later item's pass starts after a barrier)
+    // cleared for the next item (its pass starts after a barrier).  Peptides
+    // with bc + yc >= min_shp are few and scattered, so each wavefront queues
+    // its own in a private part of the run table (free by now; no atomics)
+    // and scores them 64 at a time, instead of taking the scoring branch for
+    // one or two lanes of every read.
+    __syncthreads();
     float imax = 0.0f;
-    const uint32_t q0 = wlo >> 1, q1 = (wlo + wspan) >> 1;
-    for (uint32_t qi = q0 + threadIdx.x; qi <= q1; qi += kPT) {
-        const u32x4 w4 = ((lu32x4 *)sm.acc)[qi];
-        if ((w4.x | w4.y | w4.z | w4.w) == 0u) continue;
-        ((lu32x4 *)sm.acc)[qi] = (u32x4){0u, 0u, 0u, 0u};
-        score_peptide(a, sm, spec, ixx, base + 2 * qi, w4.x, w4.y, acc, imax);
-        score_peptide(a, sm, spec, ixx, base + 2 * qi + 1, w4.z, w4.w, acc, imax);
+    {
+        const uint32_t w = threadIdx.x >> 6;
+        const uint32_t qcap = sm.cand_cap / (kPT / 64);   // queue entries per wavefront
+        lu32 *c_pid = (lu32 *)sm.runs + w * 3 * qcap, *c_wb = c_pid + qcap, *c_wy = c_wb + qcap;
+        uint32_t qn = 0;   // queued, uniform
+        auto drain = [&]() {
+            __builtin_amdgcn_wave_barrier();
+            for (uint32_t i = lane; i < qn; i += 64) score_peptide(a, sm, spec, ixx, base + c_pid[i], c_wb[i], c_wy[i], acc, imax);
+            __builtin_amdgcn_wave_barrier();
+            qn = 0;
+        };
+        const uint32_t q0 = wlo >> 1, q1 = (wlo + wspan) >> 1;
+        const uint32_t shp = a.q.min_shp;
+        // kSr reads in flight per thread, then their candidates
+        for (uint32_t qb = q0; qb <= q1; qb += kSr * kPT) {
+            u32x4 w4[kSr];
+#pragma unroll
+            for (uint32_t r = 0; r < kSr; r++) {
+                const uint32_t qi = qb + r * kPT + threadIdx.x;
+                w4[r] = qi <= q1 ? ((lu32x4 *)sm.acc)[qi] : (u32x4){0u, 0u, 0u, 0u};
+            }
+#pragma unroll
+            for (uint32_t r = 0; r < kSr; r++) {
+                const uint32_t qi = qb + r * kPT + threadIdx.x;
+                if ((w4[r].x | w4[r].y | w4[r].z | w4[r].w) != 0u) ((lu32x4 *)sm.acc)[qi] = (u32x4){0u, 0u, 0u, 0u};
+            }
+#pragma unroll
+            for (uint32_t r = 0; r < kSr; r++) {
+                const uint32_t qi = qb + r * kPT + threadIdx.x;
+                const u32x4 wv = w4[r];
+                // a peptide without hits scores -6 and is never a candidate
+                const bool k0 = (wv.x | wv.y) != 0u && (uint16_t)((wv.x >> 24) + (wv.y >> 24)) >= shp;
+                const bool k1 = (wv.z | wv.w) != 0u && (uint16_t)((wv.z >> 24) + (wv.w >> 24)) >= shp;
+                const unsigned long long m0 = __ballot(k0), m1 = __ballot(k1);
+                const uint32_t tot = (uint32_t)(__popcll(m0) + __popcll(m1));
+                if (tot == 0) continue;   // uniform per wavefront
+                if (qn + tot > qcap) drain();
+                // entries past the queue's end (a queue shorter than this
+                // read's candidates) are scored in place
+                uint32_t slot = qn + __builtin_amdgcn_mbcnt_hi((uint32_t)(m0 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m0, 0u)) +
+                                __builtin_amdgcn_mbcnt_hi((uint32_t)(m1 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m1, 0u));
+                if (k0) {
+                    if (slot < qcap) {
+                        c_pid[slot] = 2 * qi;
+                        c_wb[slot] = wv.x;
+                        c_wy[slot] = wv.y;
+                    } else {
+                        score_peptide(a, sm, spec, ixx, base + 2 * qi, wv.x, wv.y, acc, imax);
+                    }
+                    slot++;
+                }
+                if (k1) {
+                    if (slot < qcap) {
+                        c_pid[slot] = 2 * qi + 1;
+                        c_wb[slot] = wv.z;
+                        c_wy[slot] = wv.w;
+                    } else {
+                        score_peptide(a, sm, spec, ixx, base + 2 * qi + 1, wv.z, wv.w, acc, imax);
+                    }
+                }
+                qn = min(qn + tot, qcap);
+            }
+        }
+        if (qn) drain();
     }
     // the item's best score (one LDS atomic per wavefront: same-address
     // atomics serialise)
@@ -341,6 +456,7 @@ __global__ __launch_bounds__(kPT, 4) void k_items_packed(ChunkArgs a) {
         [&](uint32_t s) {
             P = load_peaks<kPT>(a, sm.pk_moz, sm.pk_int, s);
             load_windows<kPT>(a, s, sm.wlo, sm.whi, sm.lfirst);
+            if (threadIdx.x == 0) *sm.hbest = 0u;
         },
         [&](uint32_t s, uint32_t l, uint32_t c, lu32 *imax_out) {
             process_item_p(a, sm, s, imax_out, P, l, c, ufl((int32_t)sm.wlo[l]), ufl((int32_t)sm.whi[l]), acc);
@@ -348,7 +464,20 @@ __global__ __launch_bounds__(kPT, 4) void k_items_packed(ChunkArgs a) {
         [&](uint32_t s) { flush_spectrum<kPT>(a, sm.hist, s, acc, sm.ctl + 2, sm.red64); });
 }
 
+// log2 of the integers 1 .. n by v_log_f32, as the score estimate computes it
+__global__ void k_selftest_log2(uint32_t n, float *out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = __builtin_amdgcn_logf((float)(i + 1u));
+}
+
 }  // namespace
+
+int selftest_log2(hcx_ctx *ctx, uint32_t n, float *out) {
+    if (n == 0) return HCX_SUCCESS;
+    k_selftest_log2<<<(n + 255) / 256, 256, 0, ctx->stream>>>(n, out);
+    HCX_HIP(hipGetLastError());
+    return HCX_SUCCESS;
+}
 
 size_t packed_smem_bytes(const ChunkArgs &a) {
     if (a.group > kPPp * (uint32_t)kPT) return 0;

@@ -255,14 +255,14 @@ __device__ __forceinline__ uint32_t run_at(const lu32 *cofs, uint32_t nc, uint32
 }
 
 // Walk the flat entries [s0, se) of the segment in the bitmap: wavefront w
-// takes a contiguous share of its 64-entry tiles, kTileU tiles per step, lane
+// takes a contiguous share of its 64-entry tiles, U tiles per step, lane
 // j of a tile the entry F + j.  The entry's run is the number of runs that
 // end before it: the count before the tile (C, carried from tile to tile)
 // plus the tile word's bits below j -- t.C + mbcnt(t.m), at
-// most nc.  Per step, load(t, x, ok, v, fs) fills v[] for the kTileU tiles
+// most nc.  Per step, load(t, x, ok, v, fs) fills v[] for the U tiles
 // (ok[u]: x[u] is in the wavefront's share; fs: the step's first flat index)
 // and body(t, v, x, ok, fs) consumes them, so that each phase's memory
-// operations of the kTileU tiles issue back to back.
+// operations of the U tiles issue back to back.
 struct Tile {
     unsigned long long m;   // run ends in the tile, uniform
     uint32_t C;             // runs ending before the tile, uniform
@@ -273,7 +273,7 @@ __device__ __forceinline__ uint32_t tile_below(const Tile &t) {
 // arr[run of this lane's entry]: the tile's base run is uniform, so the lane
 // part is a scaled add to a scalar base
 __device__ __forceinline__ uint32_t tile_at(const lu32 *arr, const Tile &t) { return (arr + t.C)[tile_below(t)]; }
-template <int BT, typename Load, typename Body>
+template <int BT, uint32_t U, typename Load, typename Body>
 __device__ __forceinline__ void walk_tiles(const lu64 *seg64, const lu32 *cofs, uint32_t nc, uint32_t s0, uint32_t se,
                                            Load &&load, Body &&body) {
     constexpr uint32_t NW = BT / 64;
@@ -292,19 +292,19 @@ __device__ __forceinline__ void walk_tiles(const lu64 *seg64, const lu32 *cofs, 
     } else {
         C = ufl(run_at(cofs, nc, s0 + 64 * k0));
     }
-    // a step whose kTileU tiles lie wholly in the share (all but the last)
+    // a step whose U tiles lie wholly in the share (all but the last)
     // runs with ok known true
     auto step = [&](uint32_t k, auto whole_tag) {
         constexpr bool WHOLE = decltype(whole_tag)::value;
-        Tile t[kTileU];
-        uint32_t x[kTileU], v[kTileU];
-        bool ok[kTileU];
-        unsigned long long mv[kTileU];
+        Tile t[U];
+        uint32_t x[U], v[U];
+        bool ok[U];
+        unsigned long long mv[U];
 #pragma unroll
-        for (uint32_t u = 0; u < kTileU; u++) mv[u] = seg64[WHOLE ? k + u : min(k + u, kSegWords - 1)];
+        for (uint32_t u = 0; u < U; u++) mv[u] = seg64[WHOLE ? k + u : min(k + u, kSegWords - 1)];
         const uint32_t x0 = s0 + 64 * k + lane;
 #pragma unroll
-        for (uint32_t u = 0; u < kTileU; u++) {
+        for (uint32_t u = 0; u < U; u++) {
             unsigned long long m = ((unsigned long long)ufl((uint32_t)(mv[u] >> 32)) << 32) | ufl((uint32_t)mv[u]);
             if (!WHOLE) m = k + u < k1 ? m : 0ull;
             t[u].m = m;
@@ -317,8 +317,8 @@ __device__ __forceinline__ void walk_tiles(const lu64 *seg64, const lu32 *cofs, 
         body(t, v, x, ok, s0 + 64 * k);
     };
     uint32_t k = k0;
-    for (; k + kTileU <= k1 && s0 + 64 * (k + kTileU) <= xe; k += kTileU) step(k, std::true_type{});
-    for (; k < k1; k += kTileU) step(k, std::false_type{});
+    for (; k + U <= k1 && s0 + 64 * (k + U) <= xe; k += U) step(k, std::true_type{});
+    for (; k < k1; k += U) step(k, std::false_type{});
 }
 
 __device__ __forceinline__ LenDesc load_len(const ChunkArgs &a, int ixx) { return a.q.lens[ixx]; }
@@ -471,8 +471,15 @@ __device__ __forceinline__ uint32_t spec_of_item(const ChunkArgs &a, uint32_t it
 // *c.  Lengths without items have lfirst[l + 1] == lfirst[l].  Uniform.
 __device__ __forceinline__ void item_at(const ChunkArgs &a, const li32 *wlo, const lu32 *lfirst, uint32_t k,
                                         uint32_t *l, uint32_t *c) {
+    // the lengths whose items end at or before k are a prefix; k is in the
+    // first length after it (every wavefront counts them with ballots)
+    const int nlen = a.q.nlen;
     uint32_t ll = 0;
-    while (ufl((uint32_t)lfirst[ll + 1]) <= k) ll++;
+    for (int l0 = 0; l0 < nlen; l0 += 64) {
+        const int li = l0 + (int)(threadIdx.x & 63u);
+        ll += (uint32_t)__popcll(__ballot(li < nlen && lfirst[li + 1] <= k));
+    }
+    ll = ufl(ll);
     *l = ll;
     *c = ufl((uint32_t)wlo[ll]) / a.tc + (k - ufl((uint32_t)lfirst[ll]));
 }

@@ -374,6 +374,11 @@ int  hcx_surrogate_last_audit(const hcx_ctx *ctx, hcx_audit *out);
 /* lane-layout self-check of the BF16 MFMA tile: D = A B, 16 x 16 row-major */
 int  hcx_selftest_mfma(hcx_ctx *ctx, const float *A, const float *B, float *D);
 
+/* log2 of the integers 1 .. n as the packed scan's score estimate computes
+ * it (v_log_f32), and the error bound it assumes (*bound): a test checks the
+ * first against log2 over the whole range the estimate sees (n = 2^24 + 1). */
+int  hcx_selftest_log2(hcx_ctx *ctx, uint32_t n, float *out, double *bound);
+
 /* ---- host self-test of the raw / speclen divider the query kernels use ---- */
 int  hcx_selftest_divider(uint32_t d, const uint32_t *nums, uint32_t count, uint32_t *out);
 

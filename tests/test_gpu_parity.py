@@ -458,3 +458,20 @@ def test_index_build_tiny_illegal_and_empty():
         assert np.array_equal(iA, oiA), ids
         assert bA[0] == 0 and bA[-1] == len(ids) * 2 * prm.maxz * (L - 1)
     eng.close()
+
+
+def test_log2_estimate_error_bound():
+    # the packed scan settles most scores from log2 of the exact integer
+    # sums (1 .. 2^24 + 1) by v_log_f32, assuming its error is below a bound:
+    # measure it over the whole range
+    import ctypes as C
+    prm = W.make_params(W.PRESETS["small"])
+    eng = Engine(prm)
+    n = (1 << 24) + 1
+    out = np.zeros(n, np.float32)
+    bound = C.c_double()
+    N.check(N.hcx().hcx_selftest_log2(eng._h, C.c_uint32(n), N.ptr(out), C.byref(bound)))
+    err = np.abs(out.astype(np.float64) - np.log2(np.arange(1, n + 1, dtype=np.float64)))
+    log(f"v_log_f32 over 1..2^24+1: max error {err.max():.3e}, assumed bound {bound.value:.1e}")
+    assert err.max() < bound.value / 4
+    eng.close()
