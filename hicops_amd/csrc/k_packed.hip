@@ -348,8 +348,12 @@ __device__ __forceinline__ void process_item_p(const ChunkArgs &a, const PSmem &
             qn = 0;
         };
         const uint32_t q0 = wlo >> 1, q1 = (wlo + wspan) >> 1;
-        const uint32_t shp = a.q.min_shp;
-        // kSr reads in flight per thread, then their candidates
+        // a peptide without hits scores -6 and is never a candidate, so the
+        // count test may use max(min_shp, 1); counts are at most 255 each, so
+        // bc + yc is the reference's unsigned short sum
+        const uint32_t shp = max(a.q.min_shp, 1u);
+        // kSr reads in flight per thread, then their candidates; every read
+        // word is cleared (no test: the writes are cheaper than the branch)
         for (uint32_t qb = q0; qb <= q1; qb += kSr * kPT) {
             u32x4 w4[kSr];
 #pragma unroll
@@ -360,15 +364,14 @@ __device__ __forceinline__ void process_item_p(const ChunkArgs &a, const PSmem &
 #pragma unroll
             for (uint32_t r = 0; r < kSr; r++) {
                 const uint32_t qi = qb + r * kPT + threadIdx.x;
-                if ((w4[r].x | w4[r].y | w4[r].z | w4[r].w) != 0u) ((lu32x4 *)sm.acc)[qi] = (u32x4){0u, 0u, 0u, 0u};
+                if (qi <= q1) ((lu32x4 *)sm.acc)[qi] = (u32x4){0u, 0u, 0u, 0u};
             }
 #pragma unroll
             for (uint32_t r = 0; r < kSr; r++) {
                 const uint32_t qi = qb + r * kPT + threadIdx.x;
                 const u32x4 wv = w4[r];
-                // a peptide without hits scores -6 and is never a candidate
-                const bool k0 = (wv.x | wv.y) != 0u && (uint16_t)((wv.x >> 24) + (wv.y >> 24)) >= shp;
-                const bool k1 = (wv.z | wv.w) != 0u && (uint16_t)((wv.z >> 24) + (wv.w >> 24)) >= shp;
+                const bool k0 = (wv.x >> 24) + (wv.y >> 24) >= shp;
+                const bool k1 = (wv.z >> 24) + (wv.w >> 24) >= shp;
                 const unsigned long long m0 = __ballot(k0), m1 = __ballot(k1);
                 const uint32_t tot = (uint32_t)(__popcll(m0) + __popcll(m1));
                 if (tot == 0) continue;   // uniform per wavefront
