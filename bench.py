@@ -43,6 +43,7 @@ def main() -> int:
     ap.add_argument("--nspec", type=int, default=0)
     ap.add_argument("--kernel", default="both", choices=["plain", "batched", "both"])
     ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--chunk", type=int, default=0, help="peptides per index chunk (0: the library default)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
@@ -80,7 +81,7 @@ def main() -> int:
     log(f"queries: {q.nspec} spectra, {npeaks} peaks ({time.perf_counter() - t0:.1f}s)")
     prm = W.make_params(p)
 
-    eng = Engine(prm)
+    eng = Engine(prm, chunk=args.chunk)
     t0 = time.perf_counter()
     idx = eng.construct_db(db)
     build_wall = time.perf_counter() - t0

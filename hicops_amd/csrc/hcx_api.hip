@@ -671,10 +671,10 @@ int plan_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, int
         // (a run table larger than the batch needs is not wasted: the scan's
         // candidate queues use its space)
         pa.group = 2 * kPackedThreads;
-        while (pa.group > 64 && (!packed_smem_bytes(pa) || packed_smem_bytes(pa) > (size_t)c->max_lds))
-            pa.group -= 64;
-        sp.pack = packed_smem_bytes(pa) && packed_smem_bytes(pa) <= (size_t)c->max_lds;
-        pa.grid = (uint32_t)std::max(1, c->num_cu);
+        const size_t pshare = (size_t)c->max_lds / kPackedBlocks;
+        while (pa.group > 64 && (!packed_smem_bytes(pa) || packed_smem_bytes(pa) > pshare)) pa.group -= 64;
+        sp.pack = packed_smem_bytes(pa) && packed_smem_bytes(pa) <= pshare;
+        pa.grid = (uint32_t)std::max(1, c->num_cu) * kPackedBlocks;
     }
     // windows, work-item numbering, the two-pass list
     uint64_t nch_all = 0;

@@ -226,7 +226,16 @@ int launch_item_spec(hcx_ctx *ctx, const uint32_t *item_base, int32_t nspec, uin
 int launch_chunked(hcx_ctx *ctx, const ChunkArgs &a, int kernel);
 int launch_finalize(hcx_ctx *ctx, const ChunkArgs &a);
 // ---- packed single-pass search (k_packed.hip) ----
-constexpr uint32_t kPackedThreads = 1024;
+// threads per workgroup of the packed kernels and workgroups per CU (the
+// accumulators of a chunk of TC peptides take 8 * TC bytes of the CU's LDS)
+#ifndef HCX_PACKED_THREADS
+#define HCX_PACKED_THREADS 1024
+#endif
+#ifndef HCX_PACKED_BLOCKS
+#define HCX_PACKED_BLOCKS 1
+#endif
+constexpr uint32_t kPackedThreads = HCX_PACKED_THREADS;
+constexpr uint32_t kPackedBlocks = HCX_PACKED_BLOCKS;
 // Bound on |v_log_f32(n) - log2(n)| for the integers n = 1 .. 2^24 + 1 that the
 // packed kernel's score estimate assumes (the device's measured error, from
 // hcx_selftest_log2, must stay well below it).

@@ -421,7 +421,7 @@ __device__ __forceinline__ void init_psmem(const ChunkArgs &a, const PSmem &sm) 
 
 // Plain form: one workgroup per spectrum (those marked packed), its items in
 // order.
-__global__ __launch_bounds__(kPT, 4) void k_spec_packed(ChunkArgs a) {
+__global__ __launch_bounds__(kPT, 4 * kPackedBlocks * kPT / 1024) void k_spec_packed(ChunkArgs a) {
     const uint32_t s = blockIdx.x;
     if (!a.packed[s]) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -447,7 +447,7 @@ __global__ __launch_bounds__(kPT, 4) void k_spec_packed(ChunkArgs a) {
 
 // Batched form: persistent workgroups pull (spectrum, length, chunk) items of
 // the packed spectra (run_items).
-__global__ __launch_bounds__(kPT, 4) void k_items_packed(ChunkArgs a) {
+__global__ __launch_bounds__(kPT, 4 * kPackedBlocks * kPT / 1024) void k_items_packed(ChunkArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const PSmem sm = pbind(smem_raw, make_playout(a));
     init_psmem(a, sm);
