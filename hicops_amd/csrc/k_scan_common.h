@@ -190,15 +190,19 @@ __device__ __forceinline__ uint32_t wave_max_last(uint32_t x) {
 
 // Workgroup-wide exclusive scan where thread t holds tsum, the sum of its own
 // contiguous group; returns t's prefix.  wsum: BT / 64 words; one barrier.
+// After it each wavefront reads the wavefront sums once (a lane each) and
+// scans them across its lanes, instead of every thread adding them up.
 template <int BT>
 __device__ __forceinline__ uint32_t block_prefix(lu32 *wsum, uint32_t tsum) {
+    static_assert(BT / 64 <= 64, "one lane per wavefront sum");
+    constexpr int NW = BT / 64;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t x = wave_scan(tsum);
     if (lane == 63) wsum[w] = x;
     __syncthreads();
-    uint32_t run = x - tsum;
-    for (int ww = 0; ww < w; ww++) run += wsum[ww];
-    return run;
+    const uint32_t sx = wave_scan(lane < NW ? (uint32_t)wsum[lane] : 0u);
+    const uint32_t before = w ? (uint32_t)__builtin_amdgcn_readlane((int)sx, w - 1) : 0u;
+    return x - tsum + before;
 }
 
 // Workgroup-wide exclusive scan of two values at once (thread t holds the
@@ -206,26 +210,22 @@ __device__ __forceinline__ uint32_t block_prefix(lu32 *wsum, uint32_t tsum) {
 // wsum: 2 * BT / 64 words; one barrier.
 template <int BT>
 __device__ __forceinline__ uint2 block_prefix2(lu32 *wsum, uint32_t a0, uint32_t a1, uint2 *tot) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    static_assert(BT / 64 <= 64, "one lane per wavefront sum");
     constexpr int NW = BT / 64;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t x = wave_scan(a0), y = wave_scan(a1);
     if (lane == 63) {
         wsum[w] = x;
         wsum[NW + w] = y;
     }
     __syncthreads();
-    uint32_t rx = x - a0, ry = y - a1, tx = 0, ty = 0;
-    for (int ww = 0; ww < NW; ww++) {
-        const uint32_t sx = wsum[ww], sy = wsum[NW + ww];
-        if (ww < w) {
-            rx += sx;
-            ry += sy;
-        }
-        tx += sx;
-        ty += sy;
-    }
-    *tot = make_uint2(tx, ty);
-    return make_uint2(rx, ry);
+    const uint32_t sx = wave_scan(lane < NW ? (uint32_t)wsum[lane] : 0u);
+    const uint32_t sy = wave_scan(lane < NW ? (uint32_t)wsum[NW + lane] : 0u);
+    const uint32_t bx = w ? (uint32_t)__builtin_amdgcn_readlane((int)sx, w - 1) : 0u;
+    const uint32_t by = w ? (uint32_t)__builtin_amdgcn_readlane((int)sy, w - 1) : 0u;
+    *tot = make_uint2((uint32_t)__builtin_amdgcn_readlane((int)sx, NW - 1),
+                      (uint32_t)__builtin_amdgcn_readlane((int)sy, NW - 1));
+    return make_uint2(x - a0 + bx, y - a1 + by);
 }
 
 // The run-end bitmap (kSegWords 64-bit words) for segment [s0, s0 +
