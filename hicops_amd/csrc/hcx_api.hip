@@ -435,13 +435,98 @@ extern "C" int hcx_queries_free(hcx_qbatch *b) {
 namespace {
 
 // Fill QueryArgs for a search; also sizes the batch's result buffers.
+// Device memory one search uses beyond its inputs: its outputs (results,
+// histograms, work counters) and its scratch.  hcx_search and its variants
+// take it from grow-only buffers of the context and the batch;
+// hcx_search_noalloc carves it from the static scratch array.
+struct SearchMem {
+    LenDesc *lens;
+    hcx_result *res;
+    uint32_t *hist;
+    hcx_stats *stats;
+    uint32_t *tie, *queue, *scan_tmp;
+    SpecAcc *acc;
+    int2 *win;
+    uint8_t *packed;
+    uint32_t *fb, *counts, *item_base, *item_spec, *item_max;
+    uint32_t item_cap;
+};
+
+// the item_spec entries a search of n spectra over indices of nch_all chunks needs
+uint64_t item_spec_words(uint64_t n, uint64_t nch_all) { return (n * nch_all) / 64 + 2; }
+
+int search_mem_alloc(hcx_ctx *c, hcx_qbatch *qb, int nidx, uint64_t nch_all, int kernel, SearchMem *m) {
+    const uint64_t n = (uint64_t)qb->nspec;
+    if (c->lens.ensure(sizeof(LenDesc) * std::max(nidx, 1)) || qb->res.ensure(sizeof(hcx_result) * n + 16) ||
+        qb->hist.ensure(4ull * c->hbins * n + 16) || qb->stats.ensure(sizeof(hcx_stats) * n + 16) ||
+        c->tie.ensure(4 * (n + 2) + 16) || c->work.ensure(64) || qb->acc.ensure(sizeof(SpecAcc) * n + 16) ||
+        qb->win.ensure(8ull * n * std::max(nidx, 1) + 16) || qb->packed.ensure(n + 16) ||
+        qb->fb.ensure(4 * (n + 1) + 16) || qb->counts.ensure(4 * (n + 1) + 16) ||
+        qb->item_base.ensure(4 * (n + 1) + 16) || qb->item_spec.ensure(4 * item_spec_words(n, nch_all) + 16) ||
+        c->scan_tmp.ensure(4 * scan_tmp_words(n + 1) + 64))
+        return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "search: device allocation failed");
+    if (kernel == HCX_KERNEL_BATCHED && !qb->item_max.p) {
+        // a first estimate; grown to the item total after each search
+        qb->item_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(64 * n, 1024), 0xFFFFFFF0ull);
+        if (qb->item_max.ensure(4ull * qb->item_cap + 16))
+            return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "search: device allocation failed");
+    }
+    m->lens = (LenDesc *)c->lens.p;
+    m->res = (hcx_result *)qb->res.p;
+    m->hist = (uint32_t *)qb->hist.p;
+    m->stats = (hcx_stats *)qb->stats.p;
+    m->tie = (uint32_t *)c->tie.p;
+    m->queue = (uint32_t *)c->work.p;
+    m->scan_tmp = (uint32_t *)c->scan_tmp.p;
+    m->acc = (SpecAcc *)qb->acc.p;
+    m->win = (int2 *)qb->win.p;
+    m->packed = (uint8_t *)qb->packed.p;
+    m->fb = (uint32_t *)qb->fb.p;
+    m->counts = (uint32_t *)qb->counts.p;
+    m->item_base = (uint32_t *)qb->item_base.p;
+    m->item_spec = (uint32_t *)qb->item_spec.p;
+    m->item_max = (uint32_t *)qb->item_max.p;
+    m->item_cap = qb->item_max.p ? qb->item_cap : 0;
+    return HCX_SUCCESS;
+}
+
+// The scratch of a search of n spectra over nidx indices (nch_all chunks)
+// carved from [base, base + bytes); item_max takes what is left.  *need: the
+// bytes required before item_max.
+bool search_mem_carve(void *base, size_t bytes, uint64_t n, int nidx, uint64_t nch_all, SearchMem *m, size_t *need) {
+    size_t off = 0;
+    auto take = [&](size_t sz) {
+        const size_t o = off;
+        off = (off + sz + 255) & ~(size_t)255;
+        return (unsigned char *)base + o;
+    };
+    m->lens = (LenDesc *)take(sizeof(LenDesc) * std::max(nidx, 1));
+    m->tie = (uint32_t *)take(4 * (n + 2));
+    m->queue = (uint32_t *)take(64);
+    m->scan_tmp = (uint32_t *)take(4 * scan_tmp_words(n + 1) + 64);
+    m->acc = (SpecAcc *)take(sizeof(SpecAcc) * n);
+    m->win = (int2 *)take(8 * n * std::max(nidx, 1));
+    m->packed = (uint8_t *)take(n);
+    m->fb = (uint32_t *)take(4 * (n + 1));
+    m->counts = (uint32_t *)take(4 * (n + 1));
+    m->item_base = (uint32_t *)take(4 * (n + 1));
+    m->item_spec = (uint32_t *)take(4 * item_spec_words(n, nch_all));
+    *need = off;
+    if (off >= bytes) return false;
+    m->item_max = (uint32_t *)((unsigned char *)base + off);
+    m->item_cap = (uint32_t)std::min<uint64_t>((bytes - off) / 4, 0xFFFFFFF0ull);
+    return true;
+}
+
 int prepare(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, const hcx_search_opts *opts,
-            QueryArgs *a) {
+            QueryArgs *a, const SearchMem &m) {
     if (!c || !qb || (nidx > 0 && !idx)) return set_error_code(HCX_ERR_INVLD_PTR, "search: null argument");
     if (qb->ctx != c) return set_error_code(HCX_ERR_INVLD_PARAM, "search: batch from another context");
     if (nidx < 0 || nidx > 64) return set_error_code(HCX_ERR_INVLD_PARAM, "search: 0..64 indices");
     const hcx_params &p = c->params;
-    std::vector<LenDesc> lens(nidx > 0 ? nidx : 1);
+    LenDesc lens[64];
+    const int nlens = nidx > 0 ? nidx : 1;
+    std::memset(lens, 0, sizeof lens);
     uint32_t nbk = 0;
     for (int i = 0; i < nidx; i++) {
         const hcx_index *ix = idx[i];
@@ -470,14 +555,9 @@ int prepare(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, const h
         if ((uint64_t)ix->n * ix->speclen * 2 > 0xFFFFFFFFull)
             return set_error_code(HCX_ERR_INVLD_SIZE, "search: a length's ion array exceeds 4 GiB");
     }
-    if (c->lens.ensure(sizeof(LenDesc) * lens.size())) return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "search: alloc");
-    HCX_HIP(hipMemcpyAsync(c->lens.p, lens.data(), sizeof(LenDesc) * lens.size(), hipMemcpyHostToDevice, c->stream));
-    const uint64_t n = (uint64_t)qb->nspec;
-    if (qb->res.ensure(sizeof(hcx_result) * n + 16) || qb->hist.ensure(4ull * c->hbins * n + 16) ||
-        qb->stats.ensure(sizeof(hcx_stats) * n + 16) || c->tie.ensure(4 * (n + 1) + 16))
-        return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "search: result allocation failed");
+    HCX_HIP(hipMemcpyAsync(m.lens, lens, sizeof(LenDesc) * nlens, hipMemcpyHostToDevice, c->stream));
     std::memset(a, 0, sizeof(*a));
-    a->lens = (const LenDesc *)c->lens.p;
+    a->lens = m.lens;
     a->nlen = nidx;
     a->moz = (const uint32_t *)qb->moz.p;
     a->inten = (const uint32_t *)qb->inten.p;
@@ -492,10 +572,10 @@ int prepare(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, const h
     a->max_mass = p.max_mass;
     a->dM = p.dM;
     a->lgf = c->d_lgf;
-    a->res = (hcx_result *)qb->res.p;
-    a->hist = (uint32_t *)qb->hist.p;
-    a->stats = (hcx_stats *)qb->stats.p;
-    a->tie = (uint32_t *)c->tie.p;
+    a->res = m.res;
+    a->hist = m.hist;
+    a->stats = m.stats;
+    a->tie = m.tie;
     a->heap_cap = c->heap_cap;
     a->nprobe_max = std::max<uint32_t>(1, qb->max_peaks) * (2 * p.dF + 1);
     // tile: the largest power of two (<= 8192) whose LDS layout fits 80 KB,
@@ -536,6 +616,7 @@ struct ScanPlan {
     WinArgs wa;
     int kernel = HCX_KERNEL_PLAIN;
     bool pack = false;       // packed kernel for the spectra within mult_max
+    uint32_t *scan_tmp = nullptr;
 };
 
 // The device work of one search, all on the context stream with no host
@@ -566,7 +647,7 @@ int launch_sequence(hcx_ctx *c, ScanPlan &sp, bool tailfit = true) {
         if (!rc && batched) rc = launch_fill_u32(c, sp.wa.counts + n, 1, 0);
         if (!rc) rc = launch_windows(c, ca, sp.wa);
         if (!rc && batched) {
-            rc = scan_exclusive_u32(c->stream, sp.wa.counts, (uint32_t *)ca.item_base, n + 1, (uint32_t *)c->scan_tmp.p);
+            rc = scan_exclusive_u32(c->stream, sp.wa.counts, (uint32_t *)ca.item_base, n + 1, sp.scan_tmp);
             if (!rc) rc = launch_item_spec(c, ca.item_base, a.nspec, (uint32_t *)ca.item_spec);
         }
         if (rc) return rc;
@@ -609,7 +690,8 @@ uint32_t packed_mult_max(uint32_t max_half, uint32_t max_int) {
 
 // Fill sp for a search of qb over idx (prepare() done): layouts, the scan
 // plan and every device buffer the sequence uses.
-int plan_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, int kernel, ScanPlan &sp) {
+int plan_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, int kernel, const SearchMem &m,
+                ScanPlan &sp) {
     ChunkArgs &ca = sp.ca;
     if (c->params.min_shp > kMaxChunkShp) kernel = HCX_KERNEL_CURSOR;
     sp.kernel = kernel;
@@ -655,10 +737,8 @@ int plan_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, int
     // hits per peptide must fit the 15-bit counters: speclen * peaks within +-dF
     if ((uint64_t)maxspec * qb->max_mult >= 0x8000)
         return set_error_code(HCX_ERR_INVLD_PARAM, "search: too many peaks within +-dF of one bin");
-    if (qb->acc.ensure(sizeof(SpecAcc) * n + 16) || c->work.ensure(64))
-        return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "search: scratch allocation failed");
-    ca.acc = (SpecAcc *)qb->acc.p;
-    ca.queue = (uint32_t *)c->work.p;
+    ca.acc = m.acc;
+    ca.queue = m.queue;
     if (kernel == HCX_KERNEL_CURSOR) return HCX_SUCCESS;
 
     // the packed kernel: the largest run-table group that fits beside the
@@ -677,39 +757,28 @@ int plan_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, int
         pa.grid = (uint32_t)std::max(1, c->num_cu) * kPackedBlocks;
     }
     // windows, work-item numbering, the two-pass list
-    uint64_t nch_all = 0;
-    for (int i = 0; i < nidx; i++) nch_all += idx[i]->nch;
-    const uint64_t spec_words = (n * nch_all) / 64 + 2;
-    if (qb->win.ensure(8ull * n * nidx + 16) || qb->packed.ensure(n + 16) || qb->fb.ensure(4 * (n + 1) + 16) ||
-        qb->counts.ensure(4 * (n + 1) + 16) || qb->item_base.ensure(4 * (n + 1) + 16) ||
-        qb->item_spec.ensure(4 * spec_words + 16) || c->scan_tmp.ensure(4 * scan_tmp_words(n + 1) + 64))
-        return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "search: scratch allocation failed");
     if (!qb->mult.p) return set_error_code(HCX_ERR_INVLD_PARAM, "search: batch without peak multiplicities");
-    if (kernel == HCX_KERNEL_BATCHED && !qb->item_max.p) {
-        // a first estimate; grown to the item total after each search
-        qb->item_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(64 * n, 1024), 0xFFFFFFF0ull);
-        if (qb->item_max.ensure(4ull * qb->item_cap + 16)) return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "search: alloc");
-    }
     WinArgs &wa = sp.wa;
     std::memset(&wa, 0, sizeof wa);
     wa.mult = (const uint32_t *)qb->mult.p;
     wa.mult_max = mult_max;
     wa.pack = sp.pack ? 1u : 0u;
     wa.batched = kernel == HCX_KERNEL_BATCHED ? 1u : 0u;
-    wa.win = (int2 *)qb->win.p;
-    wa.counts = (uint32_t *)qb->counts.p;
-    wa.packed = (uint8_t *)qb->packed.p;
-    wa.fb_list = (uint32_t *)qb->fb.p + 1;
-    wa.fb_count = (uint32_t *)qb->fb.p;
+    wa.win = m.win;
+    wa.counts = m.counts;
+    wa.packed = m.packed;
+    wa.fb_list = m.fb + 1;
+    wa.fb_count = m.fb;
     wa.acc = ca.acc;
     ca.win = wa.win;
     ca.packed = wa.packed;
     if (kernel == HCX_KERNEL_BATCHED) {
-        ca.item_base = (const uint32_t *)qb->item_base.p;
-        ca.item_spec = (const uint32_t *)qb->item_spec.p;
-        ca.item_max = (uint32_t *)qb->item_max.p;
-        ca.item_cap = qb->item_cap;
+        ca.item_base = m.item_base;
+        ca.item_spec = m.item_spec;
+        ca.item_max = m.item_max;
+        ca.item_cap = m.item_cap;
     }
+    sp.scan_tmp = m.scan_tmp;
     ChunkArgs &pa = sp.pa;
     const uint32_t pgroup = pa.group, pgrid = pa.grid;
     if (sp.pack) {
@@ -735,20 +804,47 @@ int grow_item_max(hcx_ctx *c, hcx_qbatch *qb, const ScanPlan &sp) {
     return HCX_SUCCESS;
 }
 
+// The indices' total chunk count, after checking the arguments a search
+// needs before its memory is sized.
+int search_shape(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, uint64_t *nch_all) {
+    if (!c || !qb || (nidx > 0 && !idx)) return set_error_code(HCX_ERR_INVLD_PTR, "search: null argument");
+    if (nidx < 0 || nidx > 64) return set_error_code(HCX_ERR_INVLD_PARAM, "search: 0..64 indices");
+    uint64_t t = 0;
+    for (int i = 0; i < nidx; i++) {
+        if (!idx[i]) return set_error_code(HCX_ERR_INVLD_PARAM, "search: bad index");
+        t += idx[i]->nch;
+    }
+    *nch_all = t;
+    return HCX_SUCCESS;
+}
+
+// One search.  mem: the device memory to use (hcx_search_noalloc), or null:
+// the context's and the batch's own buffers.
 int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, const hcx_search_opts *opts,
-               bool timed, float *kernel_ms, float *total_ms) {
+               bool timed, float *kernel_ms, float *total_ms, const SearchMem *mem = nullptr) {
+    uint64_t nch_all = 0;
+    int rc = search_shape(c, idx, nidx, qb, &nch_all);
+    if (rc) return rc;
     HCX_HIP(hipSetDevice(c->device));
+    const int kernel0 = opts ? opts->kernel : HCX_KERNEL_PLAIN;
+    SearchMem m;
+    if (mem) {
+        m = *mem;
+    } else {
+        rc = search_mem_alloc(c, qb, nidx, nch_all, kernel0, &m);
+        if (rc) return rc;
+    }
     ScanPlan sp;
     std::memset(&sp.ca, 0, sizeof sp.ca);
     std::memset(&sp.pa, 0, sizeof sp.pa);
-    int rc = prepare(c, idx, nidx, qb, opts, &sp.ca.q);
+    rc = prepare(c, idx, nidx, qb, opts, &sp.ca.q, m);
     if (rc) return rc;
     QueryArgs &a = sp.ca.q;
     const bool sur = opts && opts->evalue == HCX_EVAL_SURROGATE;
     if (opts && opts->evalue != HCX_EVAL_TAILFIT && !sur)
         return set_error_code(HCX_ERR_INVLD_PARAM, "search: unknown e-value method");
     if (sur && !c->sur_ready) return set_error_code(HCX_ERR_INVLD_PARAM, "search: the surrogate is not trained");
-    rc = plan_search(c, idx, nidx, qb, opts ? opts->kernel : HCX_KERNEL_PLAIN, sp);
+    rc = plan_search(c, idx, nidx, qb, kernel0, m, sp);
     if (rc) return rc;
     const int kernel = sp.kernel;
     ChunkArgs &ca = sp.ca;
@@ -837,7 +933,7 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
     } else {
         HCX_HIP(hipStreamSynchronize(c->stream));
     }
-    return grow_item_max(c, qb, sp);
+    return mem ? HCX_SUCCESS : grow_item_max(c, qb, sp);
 }
 
 }  // namespace
@@ -1034,9 +1130,15 @@ extern "C" int hcx_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatc
 extern "C" int hcx_search_candidates(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, hcx_cand *out,
                                      uint64_t cap, uint64_t *n) {
     if (!c || !n || (cap && !out)) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_search_candidates: null argument");
+    uint64_t nch_all = 0;
+    int rc = search_shape(c, idx, nidx, qb, &nch_all);
+    if (rc) return rc;
     HCX_HIP(hipSetDevice(c->device));
+    SearchMem m;
+    rc = search_mem_alloc(c, qb, nidx, nch_all, HCX_KERNEL_CURSOR, &m);
+    if (rc) return rc;
     QueryArgs a;
-    int rc = prepare(c, idx, nidx, qb, nullptr, &a);
+    rc = prepare(c, idx, nidx, qb, nullptr, &a, m);
     if (rc) return rc;
     hcx::DeviceBuf dc, dn;
     if (dc.ensure(sizeof(hcx_cand) * cap + 16) || dn.ensure(16))
