@@ -59,6 +59,35 @@ extern "C" int hcx_heap_capacity(uint32_t topmatches) {
     return (1 << num) - 1;
 }
 
+// ---- scratch of the allocation-free search (hcx_search_noalloc) ----
+// A file-scope device array: every GPU the library runs on holds its own
+// copy for the life of the process.  Its name is a host-side stand-in, not a
+// device address: host code uses only the per-device pointer that
+// hipGetSymbolAddress returns (scratch_ptr), never the name.
+constexpr size_t kScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kScratchDevices = 64;
+static void *g_scratch_ptr[kScratchDevices];
+static std::mutex g_scratch_mu;
+
+namespace {
+// The device's scratch address, looked up once per device with that device
+// current (hcx_create runs it).
+int scratch_init(int device, void **ptr) {
+    if (device < 0 || device >= kScratchDevices) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_create: device id");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(std::string("hcx_create: scratch address: ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = p;
+    }
+    *ptr = g_scratch_ptr[device];
+    return HCX_SUCCESS;
+}
+}  // namespace
+
 extern "C" int hcx_create(const hcx_params *params, int device, hcx_ctx **out) {
     if (!params || !out) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_create: null argument");
     const hcx_params &p = *params;
@@ -97,6 +126,10 @@ extern "C" int hcx_create(const hcx_params *params, int device, hcx_ctx **out) {
     c->num_cu = prop.multiProcessorCount;
     if (hipDeviceGetAttribute(&c->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
         c->max_lds = 65536;
+    if (scratch_init(device, &c->scratch) != HCX_SUCCESS) {
+        hcx_destroy(c);
+        return HCX_ERR_DEVICE;
+    }
     // one byte of hit counter per peptide of a chunk; the rest of a
     // workgroup's LDS share holds slots, peaks, run table, histogram and the
     // hit list (16384 measured best on a 160 KB CU with two workgroups each)
@@ -1125,6 +1158,31 @@ extern "C" int hcx_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatc
     int rc = run_search(c, idx, nidx, qb, opts, false, nullptr, nullptr);
     if (rc) return rc;
     return hcx_search_fetch(c, qb, results, hist, stats);
+}
+
+extern "C" int hcx_search_noalloc(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, int32_t kernel,
+                                  hcx_result *d_results, uint32_t *d_hist, hcx_stats *d_stats) {
+    if (!c || !d_results || !d_hist) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_search_noalloc: null argument");
+    if (kernel != HCX_KERNEL_PLAIN && kernel != HCX_KERNEL_BATCHED && kernel != HCX_KERNEL_CURSOR)
+        return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_search_noalloc: unknown kernel");
+    if (!c->scratch) return set_error_code(HCX_ERR_DEVICE, "hcx_search_noalloc: no scratch on this device");
+    uint64_t nch_all = 0;
+    int rc = search_shape(c, idx, nidx, qb, &nch_all);
+    if (rc) return rc;
+    SearchMem m;
+    size_t need = 0;
+    if (!search_mem_carve(c->scratch, kScratchBytes, (uint64_t)qb->nspec, nidx, nch_all, &m, &need))
+        return set_error_code(HCX_ERR_INVLD_SIZE, "hcx_search_noalloc: " + std::to_string(qb->nspec) + " spectra over " +
+                                                      std::to_string(nidx) + " indices need " + std::to_string(need) +
+                                                      " bytes of scratch; the limit is " +
+                                                      std::to_string(kScratchBytes) + " (use hcx_search)");
+    m.res = d_results;
+    m.hist = d_hist;
+    m.stats = d_stats;
+    hcx_search_opts o = {};
+    o.kernel = kernel;
+    o.evalue = HCX_EVAL_TAILFIT;
+    return run_search(c, idx, nidx, qb, &o, false, nullptr, nullptr, &m);
 }
 
 extern "C" int hcx_search_candidates(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, hcx_cand *out,

@@ -97,6 +97,7 @@ struct hcx_ctx {
     int32_t scan_mode = HCX_SCAN_AUTO;   // packed scan where exact, two-pass elsewhere; or two-pass only
     uint32_t grab = 64;               // batched kernels: items per queue grab, at most
     int max_lds = 65536;              // device limit per workgroup
+    void *scratch = nullptr;          // this device's copy of the allocation-free search's scratch array
     // the captured search sequence and the launch shape it was captured for
     hipGraphExec_t graph_exec = nullptr;
     uint64_t graph_key = 0;

@@ -190,7 +190,7 @@ def hcx() -> C.CDLL:
                      "hcx_selftest_divider", "hcx_set_tuning", "hcx_get_tuning", "hcx_set_option", "hcx_get_option", "hcx_autotune",
                      "hcx_surrogate_param_count", "hcx_surrogate_init", "hcx_surrogate_set",
                      "hcx_surrogate_get", "hcx_surrogate_train", "hcx_surrogate_gradient",
-                     "hcx_surrogate_last_audit", "hcx_selftest_mfma", "hcx_last_timing", "hcx_last_scan_split", "hcx_selftest_log2",
+                     "hcx_surrogate_last_audit", "hcx_selftest_mfma", "hcx_last_timing", "hcx_last_scan_split", "hcx_selftest_log2", "hcx_search_noalloc",
                      "hcx_raw_upload", "hcx_raw_free", "hcx_queries_pick", "hcx_queries_pickpeaks",
                      "hcx_queries_info", "hcx_queries_export"]:
             getattr(lib, name).restype = C.c_int

@@ -311,6 +311,22 @@ int  hcx_last_timing(hcx_ctx *ctx, float *scan_ms, float *evalue_ms, float *tota
  * over all, with HCX_SCAN_TWO_PASS).  Either is 0 when it did not run.      */
 int  hcx_last_scan_split(hcx_ctx *ctx, float *packed_ms, float *two_pass_ms);
 
+/* Allocation-free search: no counterpart in the reference.  For callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): the same search as hcx_search
+ * (HCX_EVAL_TAILFIT, the context's tuning) with no hipMalloc, hipFree or
+ * other allocating call.  The caller passes the inputs (indices, batch) and
+ * device outputs: d_results (num_specs), d_hist (num_specs * (10 *
+ * max_hyperscore + 2) u32, required: the tail fit reads it) and d_stats
+ * (num_specs, or NULL).  The scratch is a static device array of 64 MiB that
+ * every GPU holds until the process ends; a search whose scratch exceeds it
+ * (about 70 + 8 x indices + (index chunks) / 16 bytes per spectrum, the
+ * rest holding the work items' best scores) returns HCX_ERR_INVLD_SIZE.  All calls on one GPU share that GPU's array: calls
+ * that overlap on one GPU (from several threads or contexts) race.  Returns
+ * when the results are in d_results.                                        */
+int  hcx_search_noalloc(hcx_ctx *ctx, hcx_index *const *idx, int nidx, hcx_qbatch *qb, int32_t kernel,
+                        hcx_result *d_results, uint32_t *d_hist, hcx_stats *d_stats);
+
 /* Candidate dump for parity tests: every candidate with shpk >= min_shp
  * and hs > 0, as (spectrum, idxoffset, psid, bc, yc, ibc, iyc, hs).
  * Returns the count in *n (capacity cap).                                   */
