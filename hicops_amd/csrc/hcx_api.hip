@@ -210,6 +210,10 @@ extern "C" int hcx_set_option(hcx_ctx *c, int32_t option, int64_t value) {
         if (value < 1 || value > (int64_t)kMaxGrab) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_set_option: grab in 1..256");
         c->grab = (uint32_t)value;
         return HCX_SUCCESS;
+    case HCX_OPT_FORCE_REDO:
+        if (value < 0 || value > 0x7FFFFFFF) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_set_option: force_redo >= 0");
+        c->force_redo = (uint32_t)value;
+        return HCX_SUCCESS;
     case HCX_OPT_GROUP:
         if (value != 0 && (value < 64 || value > (int64_t)(kProbesPerThread * kChunkThreads)))
             return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_set_option: group is 0 (automatic) or 64..5 x the workgroup threads");
@@ -226,6 +230,7 @@ extern "C" int hcx_get_option(const hcx_ctx *c, int32_t option, int64_t *value) 
     case HCX_OPT_HITLIST: *value = c->hitlist; return HCX_SUCCESS;
     case HCX_OPT_SCAN: *value = c->scan_mode; return HCX_SUCCESS;
     case HCX_OPT_GRAB: *value = c->grab; return HCX_SUCCESS;
+    case HCX_OPT_FORCE_REDO: *value = c->force_redo; return HCX_SUCCESS;
     case HCX_OPT_GROUP: *value = c->group; return HCX_SUCCESS;
     default: return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_get_option: unknown option");
     }
@@ -483,6 +488,7 @@ struct SearchMem {
     uint8_t *packed;
     uint32_t *fb, *counts, *item_base, *item_spec, *item_max;
     uint32_t item_cap;
+    uint32_t *redo, *tie2;   // spectra re-scored on the cursor kernel, and their ties
 };
 
 // the item_spec entries a search of n spectra over indices of nch_all chunks needs
@@ -496,7 +502,7 @@ int search_mem_alloc(hcx_ctx *c, hcx_qbatch *qb, int nidx, uint64_t nch_all, int
         qb->win.ensure(8ull * n * std::max(nidx, 1) + 16) || qb->packed.ensure(n + 16) ||
         qb->fb.ensure(4 * (n + 1) + 16) || qb->counts.ensure(4 * (n + 1) + 16) ||
         qb->item_base.ensure(4 * (n + 1) + 16) || qb->item_spec.ensure(4 * item_spec_words(n, nch_all) + 16) ||
-        c->scan_tmp.ensure(4 * scan_tmp_words(n + 1) + 64))
+        c->scan_tmp.ensure(4 * scan_tmp_words(n + 1) + 64) || qb->redo.ensure(8 * (n + 2) + 16))
         return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "search: device allocation failed");
     if (kernel == HCX_KERNEL_BATCHED && !qb->item_max.p) {
         // a first estimate; grown to the item total after each search
@@ -520,6 +526,8 @@ int search_mem_alloc(hcx_ctx *c, hcx_qbatch *qb, int nidx, uint64_t nch_all, int
     m->item_spec = (uint32_t *)qb->item_spec.p;
     m->item_max = (uint32_t *)qb->item_max.p;
     m->item_cap = qb->item_max.p ? qb->item_cap : 0;
+    m->redo = (uint32_t *)qb->redo.p;
+    m->tie2 = m->redo + (n + 2);
     return HCX_SUCCESS;
 }
 
@@ -544,6 +552,8 @@ bool search_mem_carve(void *base, size_t bytes, uint64_t n, int nidx, uint64_t n
     m->counts = (uint32_t *)take(4 * (n + 1));
     m->item_base = (uint32_t *)take(4 * (n + 1));
     m->item_spec = (uint32_t *)take(4 * item_spec_words(n, nch_all));
+    m->redo = (uint32_t *)take(4 * (n + 2));
+    m->tie2 = (uint32_t *)take(4 * (n + 2));
     *need = off;
     if (off >= bytes) return false;
     m->item_max = (uint32_t *)((unsigned char *)base + off);
@@ -650,6 +660,7 @@ struct ScanPlan {
     int kernel = HCX_KERNEL_PLAIN;
     bool pack = false;       // packed kernel for the spectra within mult_max
     uint32_t *scan_tmp = nullptr;
+    uint32_t *tie2 = nullptr;   // ties of the spectra re-scored on the cursor kernel
 };
 
 // The device work of one search, all on the context stream with no host
@@ -677,6 +688,8 @@ int launch_sequence(hcx_ctx *c, ScanPlan &sp, bool tailfit = true) {
         if (!rc) rc = launch_replay_cursor(c, a);   // its own (independent) replay
     } else {
         rc = launch_fill_u32(c, sp.wa.fb_count, 1, 0);
+        if (!rc) rc = launch_fill_u32(c, ca.redo, 1, 0);
+        if (!rc) rc = launch_fill_u32(c, sp.tie2, 1, 0);
         if (!rc && batched) rc = launch_fill_u32(c, sp.wa.counts + n, 1, 0);
         if (!rc) rc = launch_windows(c, ca, sp.wa);
         if (!rc && batched) {
@@ -701,6 +714,15 @@ int launch_sequence(hcx_ctx *c, ScanPlan &sp, bool tailfit = true) {
         if (!rc && !c->capturing) HCX_HIP(hipEventRecord(c->ev1, c->stream));
         if (!rc) rc = launch_finalize(c, ca);
         if (!rc) rc = launch_replay_chunked(c, ca);
+        // spectra whose two-pass counters neared overflow (k_finalize lists
+        // them): the cursor kernel scores them exactly and replays their ties
+        if (!rc) {
+            QueryArgs ra = a;
+            ra.redo = ca.redo;
+            ra.tie = sp.tie2;
+            rc = launch_cursor_redo(c, ra);
+            if (!rc) rc = launch_replay_cursor(c, ra);
+        }
     }
     if (rc || !tailfit) return rc;
     if (!c->capturing) HCX_HIP(hipEventRecord(c->ev4, c->stream));
@@ -772,6 +794,9 @@ int plan_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, int
         return set_error_code(HCX_ERR_INVLD_PARAM, "search: too many peaks within +-dF of one bin");
     ca.acc = m.acc;
     ca.queue = m.queue;
+    ca.redo = m.redo;
+    ca.force_redo = c->force_redo;
+    sp.tie2 = m.tie2;
     if (kernel == HCX_KERNEL_CURSOR) return HCX_SUCCESS;
 
     // the packed kernel: the largest run-table group that fits beside the

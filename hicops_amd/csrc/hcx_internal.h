@@ -96,6 +96,7 @@ struct hcx_ctx {
     uint32_t group = 0;               // probes per run table: 0 as many as fit, else 64..kProbesPerThread x threads
     int32_t scan_mode = HCX_SCAN_AUTO;   // packed scan where exact, two-pass elsewhere; or two-pass only
     uint32_t grab = 64;               // batched kernels: items per queue grab, at most
+    uint32_t force_redo = 0;          // test hook (HCX_OPT_FORCE_REDO): re-run every n-th spectrum on the cursor kernel
     int max_lds = 65536;              // device limit per workgroup
     void *scratch = nullptr;          // this device's copy of the allocation-free search's scratch array
     // the captured search sequence and the launch shape it was captured for
@@ -124,7 +125,7 @@ struct hcx_qbatch {
     hcx::DeviceBuf mult;                 // per spectrum: most peaks within 2 * dF + 1 bins
     // chunked-search state: per-spectrum accumulators, windows, work-item
     // numbering, the two-pass kernel's list
-    hcx::DeviceBuf acc, win, packed, counts, item_base, item_spec, item_max, fb;
+    hcx::DeviceBuf acc, win, packed, counts, item_base, item_spec, item_max, fb, redo;
     uint32_t item_cap = 0;               // item_max entries (grown to the last search's item total)
 };
 
@@ -167,9 +168,12 @@ struct QueryArgs {
     uint64_t cand_cap;
     int heap_cap;
     uint32_t *queue;      // batched kernel: work counter
+    const uint32_t *redo; // cursor kernel: [0] count, [1..] the spectra to (re)score; null: every spectrum
 };
 int launch_cursor(hcx_ctx *ctx, const QueryArgs &a);
 int launch_replay_cursor(hcx_ctx *ctx, const QueryArgs &a);
+// the cursor kernel over a.redo (persistent grid; the count is read on the device)
+int launch_cursor_redo(hcx_ctx *ctx, const QueryArgs &a);
 // ---- chunked search (k_chunk.hip) ----
 struct SpecAcc {
     unsigned long long top_a, top_b, top_n;
@@ -204,6 +208,8 @@ struct ChunkArgs {
     const uint8_t *packed;      // per spectrum: 1 = scored by the packed kernel
     const uint32_t *fb_list;    // spectra left to the two-pass kernel (null: every spectrum)
     const uint32_t *fb_count;   // their number (device)
+    uint32_t *redo;             // [0] count, [1..]: spectra whose saturating counters neared overflow (k_finalize)
+    uint32_t force_redo;        // test hook: every force_redo-th spectrum is listed as well (0: none)
 };
 // Inputs of the per-spectrum window kernel.
 struct WinArgs {
@@ -217,6 +223,7 @@ struct WinArgs {
     uint32_t *fb_list, *fb_count;
     SpecAcc *acc;           // HCX_RES_TWO_PASS set for spectra the packed kernel leaves
 };
+
 size_t chunk_smem_bytes(const ChunkArgs &a);
 int launch_chunk_offsets(hcx_ctx *ctx, const uint32_t *iA, const uint32_t *bA, uint32_t NB, uint32_t nch, uint32_t TC,
                          uint32_t speclen, uint32_t *bA2);

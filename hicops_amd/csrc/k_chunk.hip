@@ -819,7 +819,12 @@ __global__ void k_finalize(ChunkArgs a) {
     r.maxhypscore = r.mu = r.beta = r.minhypscore = r.nexthypscore = 0;
     r.e_x = 0.0;
     const uint32_t cnt = (uint32_t)g.top_n;
-    if (cnt >= 1) {
+    // a spectrum whose 8-bit counters neared overflow may have a wrong
+    // candidate set: it is re-scored exactly by the cursor kernel (and its
+    // ties replayed there), not by the chunked replay
+    if ((g.flags & HCX_RES_COUNT_OVF) || (a.force_redo && s % a.force_redo == 0)) {
+        a.redo[1 + atomicAdd(&a.redo[0], 1u)] = s;
+    } else if (cnt >= 1) {
         r.hyperscore = __uint_as_float((uint32_t)(g.top_n >> 32));
         if (cnt == 1) {
             const uint32_t ixx = (uint32_t)((g.top_a >> 25) & 0x7F);

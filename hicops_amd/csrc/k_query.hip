@@ -223,9 +223,9 @@ __global__ __launch_bounds__(kBT) void k_query(QueryArgs a) {
     for (uint32_t i = tid; i <= a.max_shdpeaks; i += kBT) sm.lgf[i] = a.lgf[i];
     for (uint32_t i = tid; i < 2 * a.tile; i += kBT) sm.sc[i] = 0;
 
-    const uint32_t nwork = REPLAY ? a.tie[0] : (uint32_t)a.nspec;
+    const uint32_t nwork = REPLAY ? a.tie[0] : (a.redo ? a.redo[0] : (uint32_t)a.nspec);
     for (uint32_t wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
-        const int32_t s = REPLAY ? (int32_t)a.tie[1 + wi] : (int32_t)wi;
+        const int32_t s = REPLAY ? (int32_t)a.tie[1 + wi] : (int32_t)(a.redo ? a.redo[1 + wi] : wi);
         const uint32_t q0 = a.qidx[s];
         const uint32_t P = a.qidx[s + 1] - q0;
         const float prec = a.prec[s];
@@ -399,6 +399,7 @@ __global__ __launch_bounds__(kBT) void k_query(QueryArgs a) {
                 r.bc = s_top.bc;
                 r.yc = s_top.yc;
             }
+            if (a.redo) r.flags |= HCX_RES_COUNT_OVF;   // re-scored here
             if (REPLAY) {
                 r.flags |= HCX_RES_TIE_REPLAYED;
             } else if (s_ties > 1) {
@@ -787,6 +788,19 @@ int launch_cursor(hcx_ctx *ctx, const QueryArgs &a) {
         HCX_HIP(hipGetLastError());
     }
     if (!ctx->capturing) HCX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    return HCX_SUCCESS;
+}
+
+// The cursor kernel over the spectra of a.redo, whose count only the device
+// knows: a persistent grid.
+int launch_cursor_redo(hcx_ctx *ctx, const QueryArgs &a) {
+    const size_t sm = query_smem_bytes(a, false);
+    HCX_HIP(hipFuncSetAttribute((const void *)k_query<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+    const int grid = std::max(1, std::min(a.nspec, ctx->num_cu * 2));
+    if (a.nspec > 0) {
+        k_query<false><<<grid, kBT, sm, ctx->stream>>>(a);
+        HCX_HIP(hipGetLastError());
+    }
     return HCX_SUCCESS;
 }
 

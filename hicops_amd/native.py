@@ -47,6 +47,7 @@ OPT_HITLIST = 1
 OPT_GROUP = 3
 OPT_SCAN = 4
 OPT_GRAB = 5
+OPT_FORCE_REDO = 6
 SCAN_AUTO = 0
 SCAN_TWO_PASS = 1
 EVAL_TAILFIT = 0

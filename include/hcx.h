@@ -187,11 +187,15 @@ int  hcx_set_tuning(hcx_ctx *ctx, uint32_t chunk, uint32_t slots, uint32_t block
  *   words, the two-pass scan for the rest; HCX_SCAN_TWO_PASS: the two-pass
  *   scan for every spectrum.  Results are identical.
  * HCX_OPT_GRAB: batched kernels: work items per queue grab, at most (1..256,
- *   default 64). */
+ *   default 64).
+ * HCX_OPT_FORCE_REDO: test hook: besides the spectra whose two-pass counters
+ *   neared overflow (HCX_RES_COUNT_OVF), every n-th spectrum is re-scored on
+ *   the cursor kernel (0, the default: none). */
 #define HCX_OPT_HITLIST    1
 #define HCX_OPT_GROUP      3
 #define HCX_OPT_SCAN       4
 #define HCX_OPT_GRAB       5
+#define HCX_OPT_FORCE_REDO 6
 #define HCX_SCAN_AUTO      0
 #define HCX_SCAN_TWO_PASS  1
 int  hcx_set_option(hcx_ctx *ctx, int32_t option, int64_t value);

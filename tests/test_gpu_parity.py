@@ -460,6 +460,32 @@ def test_index_build_tiny_illegal_and_empty():
     eng.close()
 
 
+@pytest.mark.parametrize("scan", ["auto", "two-pass"])
+@pytest.mark.parametrize("kernel", ["plain", "batched"])
+def test_forced_redo_on_cursor_kernel(small_case, kernel, scan):
+    # spectra listed for an exact re-run (what k_finalize does for spectra
+    # whose 8-bit counters neared overflow) are re-scored by the cursor kernel
+    # and their ties replayed there; forced here for every third spectrum,
+    # on a database whose every top hit is tied
+    import ctypes as C
+    p, db, q, prm = small_case
+    log(f"test_forced_redo_on_cursor_kernel kernel={kernel} scan={scan}")
+    twin = TwinDB(db)
+    res_o, hist_o, st_o = O.OracleDB(prm, twin).search(q)
+    eng = Engine(prm)
+    eng.set_scan(SCANS[scan])
+    N.check(N.hcx().hcx_set_option(eng._h, N.OPT_FORCE_REDO, C.c_int64(3)))
+    eng.construct_db(twin)
+    out = eng.query_spectrum(eng.upload(q), kernel=KERNELS[kernel])
+    n = compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"redo-{kernel}-{scan}")
+    assert n > 100
+    redone = (out.results["flags"] & N.RES_COUNT_OVF) != 0
+    assert np.array_equal(redone, np.arange(q.nspec) % 3 == 0)
+    valid = (res_o["flags"] & N.RES_VALID) != 0
+    assert np.all((out.results["flags"][valid] & N.RES_TIE_REPLAYED) != 0)
+    eng.close()
+
+
 def test_log2_estimate_error_bound():
     # the packed scan settles most scores from log2 of the exact integer
     # sums (1 .. 2^24 + 1) by v_log_f32, assuming its error is below a bound:
