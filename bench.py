@@ -44,8 +44,10 @@ def main() -> int:
     ap.add_argument("--kernel", default="both", choices=["plain", "batched", "both"])
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0, help="peptides per index chunk (0: the library default)")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=6.0,
+                    help="CPU work per timed oracle query run (three runs, median)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-only", action="store_true", help="the CPU baseline alone, no GPU")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batched form with the library defaults instead of the autotuned tuning")
     ap.add_argument("--pmc-probe", action="store_true",
@@ -56,6 +58,9 @@ def main() -> int:
 
     if args.traffic:
         return traffic_from_pmc(args.traffic[0], args.traffic[1], args.preset)
+
+    if args.cpu_only:
+        return cpu_only(args)
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -191,7 +196,7 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(prm, db, idx, q, st, args.cpu_seconds)
+        cpu = cpu_baseline(prm, db, q, st, args.cpu_seconds)
     pick = queries_pick(eng, raw, p, args.steps, args.warmup, not args.no_cpu)
 
     if sur_info is not None:
@@ -363,21 +368,44 @@ def traffic_from_pmc(fetch_csv: str, write_csv: str, preset: str) -> int:
     return 0
 
 
-def cpu_baseline(prm, db, idx, q, gpu_stats, seconds: float):
-    """The oracle (a C restatement of the reference loop, OpenMP over spectra)
-    on a bounded sample of the same spectra, against the same index."""
+def cpu_facts() -> dict:
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    share = len(os.sched_getaffinity(0))
+    threads = min(share, int(os.environ.get("OMP_NUM_THREADS", share) or share))
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity": share, "threads": threads}
+
+
+def cpu_baseline(prm, db, q, gpu_stats, seconds: float, runs: int = 3):
+    """The oracle (a C restatement of the reference's DSLIM_Construct and
+    query loop, OpenMP over lengths and over spectra) on the GPU box's host
+    cores: index build ions/s over the whole database and query spectra/s
+    over a bounded random sample of the same batch against that index, each
+    the median of `runs` runs.  gpu_stats (or None): the GPU search's work
+    counts, compared with the oracle's for the sample."""
+    # thread placement as the reference's OpenMP runs are set up; read by
+    # libgomp when the oracle library is first loaded
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     from oracle import binding as O
 
-    threads = len(os.sched_getaffinity(0))
-    lengths = db.lengths
-    orc = O.OracleDB.__new__(O.OracleDB)
-    orc.params, orc.lengths, orc.keep = prm, lengths, []
-    orc.idx = (O.OrcIndex * len(lengths))()
-    for i, (L, ix) in enumerate(zip(lengths, idx)):
-        iA, bA = ix.export()               # bit-identical to the oracle's own build (tests)
-        mass = np.ascontiguousarray(db.entries(L)["mass"])
-        orc.keep += [iA, bA, mass]
-        orc.idx[i] = O.OrcIndex(L, len(mass), mass.ctypes.data, iA.ctypes.data, bA.ctypes.data)
+    facts = cpu_facts()
+    threads = facts["threads"]
+    built, tb = None, []
+    for r in range(runs):
+        built, dt = O.index_build_all(prm, db, threads, out=built)
+        tb.append(dt)
+    nions = sum(len(a) for a, _ in built.values())
+    t_build = float(np.median(tb))
+    log(f"cpu index build: {nions} ions, {['%.2f' % t for t in tb]} s on {threads} threads")
+    orc = O.OracleDB.from_built(prm, db, built)
+
     rng = np.random.default_rng(1)
     order = rng.permutation(q.nspec)
     probe = q.subset(np.sort(order[:64]))
@@ -387,15 +415,46 @@ def cpu_baseline(prm, db, idx, q, gpu_stats, seconds: float):
     n = int(min(q.nspec, max(64, rate * seconds)))
     sel = np.sort(order[:n])
     sample = q.subset(sel)
-    t0 = time.perf_counter()
-    _, _, st = orc.search(sample, threads)
-    dt = time.perf_counter() - t0
-    match = bool(np.array_equal(st["n_ia"], gpu_stats["n_ia"][sel]) and
-                 np.array_equal(st["n_probe"], gpu_stats["n_probe"][sel]))
-    log(f"cpu baseline: {n} spectra in {dt:.1f}s on {threads} threads; work counts match GPU: {match}")
-    return {"value": round(n / dt, 2), "unit": "spectra/s", "cores": threads, "kind": "port",
-            "sample": f"{n} random spectra of the same batch, same index, OpenMP over spectra",
+    ts = []
+    for r in range(runs):
+        t0 = time.perf_counter()
+        _, _, st = orc.search(sample, threads)
+        ts.append(time.perf_counter() - t0)
+    t_q = float(np.median(ts))
+    match = None
+    if gpu_stats is not None:
+        match = bool(np.array_equal(st["n_ia"], gpu_stats["n_ia"][sel]) and
+                     np.array_equal(st["n_probe"], gpu_stats["n_probe"][sel]))
+    log(f"cpu query: {n} spectra, {['%.2f' % t for t in ts]} s on {threads} threads; "
+        f"work counts match GPU: {match}")
+    return {"value": round(n / t_q, 2), "unit": "spectra/s", "cores": threads, "kind": "port",
+            "sample": f"{n} random spectra of the same batch against the oracle's own index of the whole "
+                      f"database, OpenMP over spectra; median of {runs}",
+            "runs_s": [round(t, 3) for t in ts],
+            "index_build": {"ions": nions, "ions_per_s": round(nions / t_build, 1), "median_s": round(t_build, 3),
+                            "runs_s": [round(t, 3) for t in tb], "sample": "the whole database, OpenMP over lengths"},
+            **facts, "omp_proc_bind": os.environ["OMP_PROC_BIND"], "omp_places": os.environ["OMP_PLACES"],
             "work_counts_match_gpu": match}
+
+
+def cpu_only(args) -> int:
+    """The CPU baseline alone (no GPU): the same synthetic database and
+    spectra as the GPU run of this preset."""
+    from hicops_amd import workload as W
+
+    kw = {}
+    if args.npep:
+        kw["npep"] = args.npep
+    if args.nspec:
+        kw["nspec"] = args.nspec
+    p = W.scaled(args.preset, **kw)
+    db = W.Database(p)
+    _raw, q = W.make_raw_spectra(db, p)
+    prm = W.make_params(p)
+    log(f"cpu-only: {db.nforms} forms, {q.nspec} spectra")
+    cpu = cpu_baseline(prm, db, q, None, args.cpu_seconds)
+    print(json.dumps({"preset": p.name, "peptides": db.nforms, "spectra": q.nspec, "cpu_baseline": cpu}), flush=True)
+    return 0
 
 
 if __name__ == "__main__":

@@ -60,6 +60,9 @@ class Index:
         if getattr(self, "_h", None):
             N.hcx().hcx_index_free(self._h)
             self._h = None
+            eng = getattr(self, "engine", None)
+            if eng is not None and self in getattr(eng, "indices", []):
+                eng.indices.remove(self)
 
     def __del__(self):
         self.free()

@@ -7,6 +7,7 @@ The product package (hicops_amd) never imports it.
 from __future__ import annotations
 
 import ctypes as C
+import time
 from pathlib import Path
 
 import numpy as np
@@ -31,7 +32,7 @@ def lib() -> C.CDLL:
         L.orc_stat_mod.restype = C.c_float
         L.orc_aa_mass.argtypes = [C.c_char]
         L.orc_stat_mod.argtypes = [C.c_char]
-        for f in ("orc_index_build", "orc_window", "orc_search", "orc_candidates", "orc_count_work",
+        for f in ("orc_index_build", "orc_index_build_all", "orc_window", "orc_search", "orc_candidates", "orc_count_work",
                   "orc_tailfit", "orc_heap_replay", "orc_heap_capacity", "orc_pickpeaks"):
             getattr(L, f).restype = C.c_int
         _lib = L
@@ -93,6 +94,33 @@ def window(mass: np.ndarray, prec: float, dM: float):
     return bool(ok), lo.value, hi.value
 
 
+def index_build_all(params: N.Params, db, threads: int = 0, out=None):
+    """orc_index_build for every length of db at once (OpenMP over the
+    lengths).  Returns ({L: (iA, bA)}, seconds inside the C call); `out`, a
+    previous result for the same db, is filled again instead of allocating."""
+    Ls = list(db.lengths)
+    k = len(Ls)
+    seqs = [np.ascontiguousarray(db.seqs(L), np.uint8) for L in Ls]
+    ents = [np.ascontiguousarray(db.entries(L), N.PEP_DTYPE) for L in Ls]
+    if out is None:
+        out = {L: (np.zeros(len(e) * 2 * params.maxz * (L - 1), np.uint32),
+                   np.zeros(params.max_mass * params.scale + 1, np.uint32)) for L, e in zip(Ls, ents)}
+    iA = [out[L][0] for L in Ls]
+    bA = [out[L][1] for L in Ls]
+    P = C.c_void_p * max(1, k)
+    lens = np.asarray(Ls, np.uint32)
+    counts = np.asarray([len(e) for e in ents], np.uint32)
+    t0 = time.perf_counter()
+    rc = lib().orc_index_build_all(C.byref(params), C.c_uint32(k), N.ptr(lens),
+                                   P(*[x.ctypes.data for x in seqs]), P(*[x.ctypes.data for x in ents]),
+                                   N.ptr(counts), P(*[x.ctypes.data for x in iA]), P(*[x.ctypes.data for x in bA]),
+                                   C.c_int(threads))
+    dt = time.perf_counter() - t0
+    if rc != 0:
+        raise RuntimeError(f"orc_index_build_all failed: {rc}")
+    return out, dt
+
+
 class OracleDB:
     """The oracle's view of a database: per-length iA/bA built on the CPU."""
 
@@ -109,6 +137,22 @@ class OracleDB:
             self.iA[L], self.bA[L] = iA, bA
             self.keep += [iA, bA, mass]
             self.idx[i] = OrcIndex(L, len(ents), mass.ctypes.data, iA.ctypes.data, bA.ctypes.data)
+
+    @classmethod
+    def from_built(cls, params: N.Params, db, built):
+        """An OracleDB over indices already built ({L: (iA, bA)}, as
+        index_build_all returns them)."""
+        o = cls.__new__(cls)
+        o.params, o.lengths, o.keep = params, list(db.lengths), []
+        o.idx = (OrcIndex * max(1, len(o.lengths)))()
+        o.iA, o.bA = {}, {}
+        for i, L in enumerate(o.lengths):
+            iA, bA = built[L]
+            mass = np.ascontiguousarray(db.entries(L)["mass"])
+            o.iA[L], o.bA[L] = iA, bA
+            o.keep += [iA, bA, mass]
+            o.idx[i] = OrcIndex(L, len(mass), mass.ctypes.data, iA.ctypes.data, bA.ctypes.data)
+        return o
 
     def search(self, queries, threads: int = 0):
         n = queries.nspec

@@ -265,6 +265,30 @@ int orc_index_build(const hcx_params *p, uint32_t peplen, const char *seqs,
     return (bA[NB] == total) ? HCX_SUCCESS : HCX_ERR_INVLD_SIZE;
 }
 
+/* DSLIM_Construct over several lengths at once (dslim.cpp:46-138 runs one
+ * length after another; each is independent): OpenMP over the lengths, for
+ * the CPU baseline.  threads <= 0: the OpenMP default. */
+int orc_index_build_all(const hcx_params *p, uint32_t nlen, const uint32_t *peplen, const char *const *seqs,
+                        const hcx_pep_entry *const *entries, const uint32_t *n, uint32_t *const *iA,
+                        uint32_t *const *bA, int threads)
+{
+    int rc = HCX_SUCCESS;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int64_t i = 0; i < (int64_t)nlen; i++) {
+        const int r = orc_index_build(p, peplen[i], seqs[i], entries[i], n[i], iA[i], bA[i]);
+        if (r != HCX_SUCCESS) {
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+            rc = r;
+        }
+    }
+    return rc;
+}
+
 /* dslim_query.cpp:919-955 */
 static int32_t bin_find_min(const float *m, float pmass1, int32_t min, int32_t max)
 {

@@ -47,6 +47,10 @@ float orc_gen_mod_spectrum(const char *seq, uint32_t len, const hcx_pep_entry *e
 int orc_index_build(const hcx_params *p, uint32_t peplen, const char *seqs,
                     const hcx_pep_entry *entries, uint32_t n,
                     uint32_t *iA, uint32_t *bA);
+/* orc_index_build over nlen lengths, OpenMP over the lengths (CPU baseline) */
+int orc_index_build_all(const hcx_params *p, uint32_t nlen, const uint32_t *peplen, const char *const *seqs,
+                        const hcx_pep_entry *const *entries, const uint32_t *n, uint32_t *const *iA,
+                        uint32_t *const *bA, int threads);
 
 /* dslim_query.cpp:854-994.  Window [minl, maxl] over masses sorted
  * ascending; returns 1 if valid.  Pinned: brute-force linear scan. */

@@ -501,3 +501,39 @@ def test_log2_estimate_error_bound():
     log(f"v_log_f32 over 1..2^24+1: max error {err.max():.3e}, assumed bound {bound.value:.1e}")
     assert err.max() < bound.value / 4
     eng.close()
+
+
+class HalfDB:
+    """The first half of every length's pepEntries (the same sequences)."""
+
+    def __init__(self, db):
+        self.lengths = db.lengths
+        self.db = db
+
+    def seqs(self, L):
+        return self.db.seqs(L)
+
+    def entries(self, L):
+        e = self.db.entries(L)
+        return e[:max(1, len(e) // 2)]
+
+
+@pytest.mark.parametrize("use_graph", [0, 1])
+def test_rebuilt_index_same_batch(small_case, use_graph):
+    # search a batch, free the index, rebuild it with fewer peptides (the
+    # allocator may hand back the same addresses), search the same uploaded
+    # batch again: both searches equal the oracle on their own index
+    p, db, q, prm = small_case
+    log(f"test_rebuilt_index_same_batch use_graph={use_graph}")
+    eng = Engine(prm)
+    b = eng.upload(q)
+    for d in (db, HalfDB(db), db):
+        ix = eng.construct_db(d)
+        res_o, hist_o, st_o = O.OracleDB(prm, d).search(q)
+        for kernel in (N.KERNEL_BATCHED, N.KERNEL_PLAIN):
+            out = eng.query_spectrum(b, indices=ix, kernel=kernel, use_graph=use_graph)
+            compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"rebuilt-{type(d).__name__}-{kernel}")
+        for x in ix:
+            x.free()
+        assert not eng.indices
+    eng.close()
