@@ -144,6 +144,14 @@ extern "C" int hcx_create(const hcx_params *params, int device, hcx_ctx **out) {
     c->h_lgf.assign(p.max_shdpeaks + 1, 0.0);
     for (uint32_t i = 2; i <= p.max_shdpeaks; i++) c->h_lgf[i] = std::log10((double)i) + c->h_lgf[i - 1];
     if (hipMalloc(&c->d_lgf, sizeof(double) * c->h_lgf.size()) != hipSuccess) return fail("lgfact alloc");
+    // staging for the host tail fits (fix_host_fits), allocated once here so
+    // that no search allocates for it
+    if (c->fix_dev.ensure((size_t)kFixRows * c->hbins * 4 + (size_t)kFixRows * sizeof(hcx_result)))
+        return fail("host fit staging alloc");
+    if (hipHostMalloc(&c->fix_host, fix_host_bytes(c->hbins), hipHostMallocDefault) != hipSuccess) {
+        c->fix_host = nullptr;
+        return fail("host fit staging alloc");
+    }
     if (hipMemcpy(c->d_lgf, c->h_lgf.data(), sizeof(double) * c->h_lgf.size(), hipMemcpyHostToDevice) !=
         hipSuccess)
         return fail("lgfact copy");
@@ -162,6 +170,8 @@ extern "C" int hcx_destroy(hcx_ctx *c) {
     c->sur.release();
     c->sur_tmp.release();
     c->mlp_buf.release();
+    c->fix_dev.release();
+    if (c->fix_host) (void)hipHostFree(c->fix_host);
     if (c->d_lgf) (void)hipFree(c->d_lgf);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -214,6 +224,11 @@ extern "C" int hcx_set_option(hcx_ctx *c, int32_t option, int64_t value) {
         if (value < 0 || value > 0x7FFFFFFF) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_set_option: force_redo >= 0");
         c->force_redo = (uint32_t)value;
         return HCX_SUCCESS;
+    case HCX_OPT_FORCE_HOST_FIT:
+        if (value < 0 || value > 0x7FFFFFFF)
+            return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_set_option: force_host_fit >= 0");
+        c->force_host_fit = (uint32_t)value;
+        return HCX_SUCCESS;
     case HCX_OPT_GROUP:
         if (value != 0 && (value < 64 || value > (int64_t)(kProbesPerThread * kChunkThreads)))
             return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_set_option: group is 0 (automatic) or 64..5 x the workgroup threads");
@@ -231,6 +246,7 @@ extern "C" int hcx_get_option(const hcx_ctx *c, int32_t option, int64_t *value) 
     case HCX_OPT_SCAN: *value = c->scan_mode; return HCX_SUCCESS;
     case HCX_OPT_GRAB: *value = c->grab; return HCX_SUCCESS;
     case HCX_OPT_FORCE_REDO: *value = c->force_redo; return HCX_SUCCESS;
+    case HCX_OPT_FORCE_HOST_FIT: *value = c->force_host_fit; return HCX_SUCCESS;
     case HCX_OPT_GROUP: *value = c->group; return HCX_SUCCESS;
     default: return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_get_option: unknown option");
     }
@@ -726,7 +742,11 @@ int launch_sequence(hcx_ctx *c, ScanPlan &sp, bool tailfit = true) {
     }
     if (rc || !tailfit) return rc;
     if (!c->capturing) HCX_HIP(hipEventRecord(c->ev4, c->stream));
-    rc = launch_tailfit(c, a.res, a.hist, a.hbins, a.nspec, c->params.min_cpsm, c->params.expect_max);
+    // the redo list is done with: it now lists the fits for the host
+    rc = launch_fill_u32(c, ca.redo, 1, 0);
+    if (!rc)
+        rc = launch_tailfit(c, a.res, a.hist, a.hbins, a.nspec, c->params.min_cpsm, c->params.expect_max, ca.redo,
+                            c->force_host_fit);
     if (rc) return rc;
     if (!c->capturing) HCX_HIP(hipEventRecord(c->ev5, c->stream));
     return HCX_SUCCESS;
@@ -916,11 +936,12 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
         if (!rc) HCX_HIP(hipEventRecord(c->ev5, c->stream));
         uint32_t *cnt = (uint32_t *)c->work.p + 4;
         if (!rc) rc = launch_fill_u32(c, cnt, 4, 0);
+        if (!rc) rc = launch_fill_u32(c, ca.redo, 1, 0);
         const uint32_t every = opts->audit_every ? opts->audit_every : 64;
         const double tol = opts->audit_tol > 0.0 ? opts->audit_tol : 1e-3;
         if (!rc)
             rc = launch_tailfit_audit(c, a.res, a.hist, a.hbins, a.nspec, c->params.min_cpsm, c->params.expect_max,
-                                      every, tol, cnt);
+                                      every, tol, cnt, ca.redo, c->force_host_fit);
         if (rc) return rc;
         uint32_t h[4];
         HCX_HIP(hipMemcpyAsync(h, cnt, 16, hipMemcpyDeviceToHost, c->stream));
@@ -933,7 +954,10 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
         au.fell_back = h[1] > 0 ? 1 : 0;
         c->last_audit = au;
         if (au.fell_back) {
-            rc = launch_tailfit(c, a.res, a.hist, a.hbins, a.nspec, c->params.min_cpsm, c->params.expect_max);
+            rc = launch_fill_u32(c, ca.redo, 1, 0);
+            if (!rc)
+                rc = launch_tailfit(c, a.res, a.hist, a.hbins, a.nspec, c->params.min_cpsm, c->params.expect_max,
+                                    ca.redo, c->force_host_fit);
             if (rc) return rc;
         }
     } else if (opts && opts->use_graph) {
@@ -977,6 +1001,9 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
         rc = launch_sequence(c, sp);
         if (rc) return rc;
     }
+    // fits the device could not decide: on the host, before the search ends
+    rc = fix_host_fits(c, ca.redo, a.res, a.hist, a.hbins, c->params.expect_max);
+    if (rc) return rc;
     if (timed) {
         HCX_HIP(hipEventRecord(c->ev3, c->stream));
         HCX_HIP(hipEventSynchronize(c->ev3));
@@ -1259,14 +1286,23 @@ extern "C" int hcx_tailfit_batch(hcx_ctx *c, const uint32_t *hist, uint32_t hbin
     HCX_HIP(hipMemcpyAsync(dc.p, cpsms, 4ull * n, hipMemcpyHostToDevice, c->stream));
     HCX_HIP(hipMemcpyAsync(dm.p, maxhyp, 4ull * n, hipMemcpyHostToDevice, c->stream));
     int rc = launch_tailfit_raw(c, (const uint32_t *)dh.p, hbins, n, (const uint32_t *)dc.p, (const int32_t *)dm.p,
-                                (int32_t *)dout.p, (uint32_t *)dfl.p);
+                                (int32_t *)dout.p, (uint32_t *)dfl.p, c->force_host_fit);
     if (rc) return rc;
     HCX_HIP(hipStreamSynchronize(c->stream));
     std::vector<int32_t> o(4ull * n);
     HCX_HIP(hipMemcpy(o.data(), dout.p, 16ull * n, hipMemcpyDeviceToHost));
     std::vector<uint32_t> fl(n);
     HCX_HIP(hipMemcpy(fl.data(), dfl.p, 4ull * n, hipMemcpyDeviceToHost));
+    std::vector<double> sx;
     for (uint32_t s = 0; s < n; s++) {
+        if (fl[s] & HCX_RES_FIT_HOST) {
+            // too close to a boundary for the device's log10: the host's
+            // FP64 path decides
+            if (sx.empty()) sx.resize(hbins);
+            uint32_t f = 0;
+            host_tailfit(hist + (size_t)s * hbins, hbins, cpsms[s], maxhyp[s], sx.data(), &o[4 * s], &f);
+            fl[s] = f | HCX_RES_FIT_HOST;
+        }
         mu[s] = o[4 * s];
         beta[s] = o[4 * s + 1];
         minhyp[s] = o[4 * s + 2];

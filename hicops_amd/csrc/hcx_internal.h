@@ -97,8 +97,11 @@ struct hcx_ctx {
     int32_t scan_mode = HCX_SCAN_AUTO;   // packed scan where exact, two-pass elsewhere; or two-pass only
     uint32_t grab = 64;               // batched kernels: items per queue grab, at most
     uint32_t force_redo = 0;          // test hook (HCX_OPT_FORCE_REDO): re-run every n-th spectrum on the cursor kernel
+    uint32_t force_host_fit = 0;      // test hook (HCX_OPT_FORCE_HOST_FIT): fit every n-th spectrum on the host
     int max_lds = 65536;              // device limit per workgroup
     void *scratch = nullptr;          // this device's copy of the allocation-free search's scratch array
+    hcx::DeviceBuf fix_dev;           // host re-fit staging: kFixRows histogram rows, then kFixRows results
+    void *fix_host = nullptr;         // pinned host copy of the same (+ 256 B for the count)
     // the captured search sequence and the launch shape it was captured for
     hipGraphExec_t graph_exec = nullptr;
     uint64_t graph_key = 0;
@@ -248,6 +251,15 @@ constexpr uint32_t kPackedBlocks = HCX_PACKED_BLOCKS;
 // packed kernel's score estimate assumes (the device's measured error, from
 // hcx_selftest_log2, must stay well below it).
 constexpr double kLog2Err = 2.0e-5;
+// The tail fit's log10 values: two correct-to-a-few-ulp log10s (the device's
+// and a host libm's) differ by less than kFitYErr on the fit's range
+// (|log10 v| < 10 for v in [2^-32, 1]: 4 ulp = 7.1e-15, with margin).
+constexpr double kFitYErr = 3.0e-14;
+// and a window threshold comparison is trusted when the value is further
+// than kFitWinEps from the threshold
+constexpr double kFitWinEps = 1.0e-12;
+// spectra fitted again on the host per round (staging rows)
+constexpr uint32_t kFixRows = 256;
 int selftest_log2(hcx_ctx *ctx, uint32_t n, float *out);
 // LDS the packed kernels need with a.group probes per run table, or 0 when
 // a.tc peptides' accumulators do not fit beside the other arrays
@@ -257,13 +269,36 @@ int launch_replay_chunked(hcx_ctx *ctx, ChunkArgs &a);
 int scan_exclusive_u32(hipStream_t st, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *tmp);
 uint64_t scan_tmp_words(uint64_t n);
 
+// fix: a list (count, then spectrum ids; room for n) that receives the
+// spectra whose device fit sat too close to a truncation or window boundary
+// to be trusted (HCX_RES_FIT_HOST); the host fits them again (fix_host_fits).
 int launch_tailfit(hcx_ctx *ctx, hcx_result *res, const uint32_t *hist, uint32_t hbins, int32_t n,
-                   uint32_t min_cpsm, double expect_max);
+                   uint32_t min_cpsm, double expect_max, uint32_t *fix, uint32_t force = 0);
 // audit of surrogate e-values: the plain fit for every stride-th spectrum,
 // compared with the surrogate's e-value; cnt[0] audited, cnt[1] disagreeing
 // (relative error > tol), cnt[2..3] largest relative error (double bits).
 int launch_tailfit_audit(hcx_ctx *ctx, hcx_result *res, const uint32_t *hist, uint32_t hbins, int32_t n,
-                         uint32_t min_cpsm, double expect_max, uint32_t stride, double tol, uint32_t *cnt);
+                         uint32_t min_cpsm, double expect_max, uint32_t stride, double tol, uint32_t *cnt,
+                         uint32_t *fix, uint32_t force = 0);
+// ---- host tail fit (fit_host.hip) ----
+// expeRT::ModelTailFit on the host in FP64 with the host libm's log10, for
+// the spectra the device lists as fragile.  sx: hbins doubles of workspace;
+// out4: mu, beta, stt1, end1.
+void host_tailfit(const uint32_t *H, uint32_t hbins, uint32_t cpsms, int32_t hyp, double *sx, int32_t out4[4],
+                  uint32_t *flags);
+// bytes of the pinned host staging: count, kFixRows rows and results, and
+// one fit's workspace
+inline size_t fix_host_bytes(uint32_t hbins) {
+    return 256 + (size_t)kFixRows * hbins * 4 + (size_t)kFixRows * sizeof(hcx_result) + (size_t)hbins * 8;
+}
+// Fits the spectra of the device list `fix` again on the host and writes
+// their results back (kFixRows at a time through the context's staging
+// buffers); ends with the stream synchronised.
+int fix_host_fits(hcx_ctx *c, const uint32_t *fix, hcx_result *res, const uint32_t *hist, uint32_t hbins,
+                  double expect_max);
+int launch_fix_gather(hcx_ctx *c, const uint32_t *ids, uint32_t k, const uint32_t *hist, uint32_t hbins,
+                      const hcx_result *res, uint32_t *rows, hcx_result *rstage);
+int launch_fix_scatter(hcx_ctx *c, const uint32_t *ids, uint32_t k, const hcx_result *rstage, hcx_result *res);
 // ---- surrogate (k_mlp.hip) ----
 uint32_t mlp_param_count();
 int mlp_alloc(hcx_ctx *c);
@@ -275,6 +310,6 @@ int mlp_train(hcx_ctx *c, hcx_qbatch *qb, uint32_t steps, uint32_t batch, float 
 int selftest_mfma(hcx_ctx *c, const float *A, const float *B, float *D);
 
 int launch_tailfit_raw(hcx_ctx *ctx, const uint32_t *hist, uint32_t hbins, uint32_t n, const uint32_t *cpsms,
-                       const int32_t *maxhyp, int32_t *out4, uint32_t *flags);
+                       const int32_t *maxhyp, int32_t *out4, uint32_t *flags, uint32_t force = 0);
 
 }  // namespace hcx

@@ -24,6 +24,8 @@
 // Tail fit (expeRT.cpp:572-718) runs one wavefront per spectrum in FP64; the
 // prefix sums are of integer counts (exact in any order), the regression sums
 // run on one lane in the reference order.
+#include <cfloat>
+
 #include "hcx_internal.h"
 
 namespace hcx {
@@ -525,12 +527,19 @@ __global__ __launch_bounds__(kBT) void k_candidates(QueryArgs a) {
 struct FitOut {
     int32_t mu, beta, stt1, end1;
     uint32_t flags;
+    bool fragile;   // another correct log10 could give other ints: fit it again on the host
 };
+
+// v is within tol (+ its own rounding) of an integer, so truncating another
+// value within tol of it could give a different int
+__device__ __forceinline__ bool near_int(double v, double tol) {
+    return fabs(v - rint(v)) <= tol + 4.0 * DBL_EPSILON * fabs(v);
+}
 
 // sx: LDS scratch of this wave (>= hbins doubles).  Lane 0's return is valid.
 __device__ FitOut tailfit_wave(const uint32_t *H, uint32_t hbins, uint32_t cpsms, int32_t hyp, double *sx) {
     const int lane = threadIdx.x & 63;
-    FitOut o = {0, 0, 0, 0, 0};
+    FitOut o = {0, 0, 0, 0, 0, false};
     if (hyp < 1 || hyp >= (int)hbins) {
         // the reference reads yy[-1] or past the histogram (UB): a zero fit
         o.flags = HCX_RES_FIT_UB;
@@ -629,6 +638,17 @@ __device__ FitOut tailfit_wave(const uint32_t *H, uint32_t hbins, uint32_t cpsms
             }
             return n - 1;
         };
+        // a log10 value this close to a window threshold could compare the
+        // other way under another log10 within kFitYErr
+        auto near_thr = [&](double v) {
+            for (int b = 0; b < n; b += 64) {
+                const int i = b + lane;
+                if (__ballot(i < n && fabs(sx[i] - v) <= kFitWinEps)) return true;
+            }
+            return false;
+        };
+        bool fragile = false;
+        if (n >= 3) fragile = near_thr(s0 + hgt * 0.22) || (n > 3 && near_thr(s0 + hgt * 0.87));
         if (n > 3) {
             mark = largmax(s0 + hgt * 0.22) - 1;
             mark2 = rargmax(s0 + hgt * 0.87);
@@ -649,11 +669,15 @@ __device__ FitOut tailfit_wave(const uint32_t *H, uint32_t hbins, uint32_t cpsms
         } else if (lane == 0) {
             const int m = mark2 - mark + 1;
             const double *y = sx + mark;
+            // e_mu, e_beta: bounds on how far mu_t and beta_t can move when
+            // every y moves by up to kFitYErr (first order through the sums,
+            // plus one rounding difference per operation)
+            double e_mu = 0.0, e_beta = kFitYErr;
             if (m == 1) {
                 mu_t = 0.0;
                 beta_t = y[0];
             } else {
-                double xbar = 0.0, ybar = 0.0, top = 0.0, bot = 0.0;
+                double xbar = 0.0, ybar = 0.0, top = 0.0, bot = 0.0, sdx = 0.0, sab = 0.0;
                 for (int i = 0; i < m; i++) {
                     xbar = xbar + (double)(stt1 + mark + i);
                     ybar = ybar + y[i];
@@ -664,24 +688,42 @@ __device__ FitOut tailfit_wave(const uint32_t *H, uint32_t hbins, uint32_t cpsms
                     const double dx = (double)(stt1 + mark + i) - xbar;
                     top = top + dx * (y[i] - ybar);
                     bot = bot + dx * dx;
+                    sdx += fabs(dx);
+                    sab += fabs(dx * (y[i] - ybar));
                 }
                 mu_t = top / bot;
                 beta_t = ybar - mu_t * xbar;
+                e_mu = (2.0 * kFitYErr * sdx + (double)(m + 2) * DBL_EPSILON * sab) / bot +
+                       2.0 * DBL_EPSILON * fabs(mu_t);
+                e_beta = kFitYErr + (double)m * DBL_EPSILON * fabs(ybar) + e_mu * fabs(xbar) +
+                         2.0 * DBL_EPSILON * (fabs(mu_t * xbar) + fabs(beta_t));
             }
+            // the ints are the values x 1e6 truncated (margin 4x); a single
+            // point's slope is an exact 0
+            if ((m > 1 && near_int(mu_t * 1e6, 4e6 * e_mu)) || near_int(beta_t * 1e6, 4e6 * e_beta) ||
+                fabs(fabs(mu_t) - 2.0e3) <= 4.0 * e_mu || fabs(fabs(beta_t) - 2.0e3) <= 4.0 * e_beta)
+                fragile = true;
             if (!(fabs(mu_t) < 2.0e3) || !(fabs(beta_t) < 2.0e3)) {
                 o.flags |= HCX_RES_FIT_UB;
                 mu_t = 0.0;
                 beta_t = 0.0;
             }
         }
+        o.fragile = fragile;
     }
     o.mu = (int32_t)(mu_t * 1e6);
     o.beta = (int32_t)(beta_t * 1e6);
     return o;
 }
 
+__device__ __forceinline__ void list_fragile(uint16_t &flags, uint32_t *fix, uint32_t s) {
+    flags |= HCX_RES_FIT_HOST;
+    if (fix) fix[1 + atomicAdd(fix, 1u)] = s;
+}
+
 __global__ __launch_bounds__(kBT) void k_tailfit(hcx_result *res, const uint32_t *hist, uint32_t hbins, int32_t n,
-                                                 uint32_t min_cpsm, double expect_max) {
+                                                 uint32_t min_cpsm, double expect_max, uint32_t *fix,
+                                                 uint32_t force) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int32_t s = blockIdx.x * (kBT / 64) + w;
@@ -710,6 +752,7 @@ __global__ __launch_bounds__(kBT) void k_tailfit(hcx_result *res, const uint32_t
         r.e_x = e;
         flags |= (uint16_t)f.flags | HCX_RES_VALID;
         if (e < expect_max) flags |= HCX_RES_REPORTED;
+        if (f.fragile || (force && s % force == 0)) list_fragile(flags, fix, (uint32_t)s);
         r.flags = flags;
         res[s] = r;
     }
@@ -720,7 +763,8 @@ __global__ __launch_bounds__(kBT) void k_tailfit(hcx_result *res, const uint32_t
 // (flag HCX_RES_AUDITED).
 __global__ __launch_bounds__(kBT) void k_tailfit_audit(hcx_result *res, const uint32_t *hist, uint32_t hbins,
                                                        int32_t n, uint32_t min_cpsm, double expect_max,
-                                                       uint32_t stride, double tol, uint32_t *cnt) {
+                                                       uint32_t stride, double tol, uint32_t *cnt, uint32_t *fix,
+                                                       uint32_t force) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t s = ((int64_t)blockIdx.x * (kBT / 64) + w) * stride;
@@ -749,6 +793,7 @@ __global__ __launch_bounds__(kBT) void k_tailfit_audit(hcx_result *res, const ui
         uint16_t flags = r.flags & ~(HCX_RES_REPORTED | HCX_RES_SURROGATE);
         flags |= (uint16_t)f.flags | HCX_RES_VALID | HCX_RES_AUDITED;
         if (e < expect_max) flags |= HCX_RES_REPORTED;
+        if (f.fragile || (force && s % force == 0)) list_fragile(flags, fix, (uint32_t)s);
         r.flags = flags;
         res[s] = r;
     }
@@ -756,7 +801,7 @@ __global__ __launch_bounds__(kBT) void k_tailfit_audit(hcx_result *res, const ui
 
 __global__ __launch_bounds__(kBT) void k_tailfit_raw(const uint32_t *hist, uint32_t hbins, uint32_t n,
                                                      const uint32_t *cpsms, const int32_t *maxhyp, int32_t *out4,
-                                                     uint32_t *flags) {
+                                                     uint32_t *flags, uint32_t force) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t s = blockIdx.x * (kBT / 64) + w;
@@ -768,7 +813,7 @@ __global__ __launch_bounds__(kBT) void k_tailfit_raw(const uint32_t *hist, uint3
         out4[4 * s + 1] = f.beta;
         out4[4 * s + 2] = f.stt1;
         out4[4 * s + 3] = f.end1;
-        if (flags) flags[s] = f.flags;
+        if (flags) flags[s] = f.flags | ((f.fragile || (force && s % force == 0)) ? HCX_RES_FIT_HOST : 0u);
     }
 }
 
@@ -826,36 +871,71 @@ int launch_candidates(hcx_ctx *ctx, const QueryArgs &a) {
 }
 
 int launch_tailfit(hcx_ctx *ctx, hcx_result *res, const uint32_t *hist, uint32_t hbins, int32_t n,
-                   uint32_t min_cpsm, double expect_max) {
+                   uint32_t min_cpsm, double expect_max, uint32_t *fix, uint32_t force) {
     if (n <= 0) return HCX_SUCCESS;
     const size_t sm = (size_t)(kBT / 64) * hbins * sizeof(double);
     HCX_HIP(hipFuncSetAttribute((const void *)k_tailfit, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
     const int per = kBT / 64;
-    k_tailfit<<<(n + per - 1) / per, kBT, sm, ctx->stream>>>(res, hist, hbins, n, min_cpsm, expect_max);
+    k_tailfit<<<(n + per - 1) / per, kBT, sm, ctx->stream>>>(res, hist, hbins, n, min_cpsm, expect_max, fix,
+                                                             force);
     HCX_HIP(hipGetLastError());
     return HCX_SUCCESS;
 }
 
 int launch_tailfit_audit(hcx_ctx *ctx, hcx_result *res, const uint32_t *hist, uint32_t hbins, int32_t n,
-                         uint32_t min_cpsm, double expect_max, uint32_t stride, double tol, uint32_t *cnt) {
+                         uint32_t min_cpsm, double expect_max, uint32_t stride, double tol, uint32_t *cnt,
+                         uint32_t *fix, uint32_t force) {
     if (n <= 0) return HCX_SUCCESS;
     const size_t sm = (size_t)(kBT / 64) * hbins * sizeof(double);
     HCX_HIP(hipFuncSetAttribute((const void *)k_tailfit_audit, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
     const int per = kBT / 64;
     const int64_t m = ((int64_t)n + stride - 1) / stride;
     k_tailfit_audit<<<(unsigned)((m + per - 1) / per), kBT, sm, ctx->stream>>>(res, hist, hbins, n, min_cpsm,
-                                                                             expect_max, stride, tol, cnt);
+                                                                             expect_max, stride, tol, cnt, fix, force);
     HCX_HIP(hipGetLastError());
     return HCX_SUCCESS;
 }
 
 int launch_tailfit_raw(hcx_ctx *ctx, const uint32_t *hist, uint32_t hbins, uint32_t n, const uint32_t *cpsms,
-                       const int32_t *maxhyp, int32_t *out4, uint32_t *flags) {
+                       const int32_t *maxhyp, int32_t *out4, uint32_t *flags, uint32_t force) {
     if (n == 0) return HCX_SUCCESS;
     const size_t sm = (size_t)(kBT / 64) * hbins * sizeof(double);
     HCX_HIP(hipFuncSetAttribute((const void *)k_tailfit_raw, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
     const uint32_t per = kBT / 64;
-    k_tailfit_raw<<<(n + per - 1) / per, kBT, sm, ctx->stream>>>(hist, hbins, n, cpsms, maxhyp, out4, flags);
+    k_tailfit_raw<<<(n + per - 1) / per, kBT, sm, ctx->stream>>>(hist, hbins, n, cpsms, maxhyp, out4, flags,
+                                                                 force);
+    HCX_HIP(hipGetLastError());
+    return HCX_SUCCESS;
+}
+
+// Histogram rows and results of the listed spectra into staging (one
+// workgroup per spectrum), and the corrected results back.
+__global__ __launch_bounds__(256) void k_fix_gather(const uint32_t *ids, const uint32_t *hist, uint32_t hbins,
+                                                    const hcx_result *res, uint32_t *rows, hcx_result *rstage) {
+    const uint32_t s = ids[blockIdx.x];
+    const uint32_t *src = hist + (size_t)s * hbins;
+    uint32_t *dst = rows + (size_t)blockIdx.x * hbins;
+    for (uint32_t i = threadIdx.x; i < hbins; i += 256) dst[i] = src[i];
+    if (threadIdx.x == 0) rstage[blockIdx.x] = res[s];
+}
+
+__global__ __launch_bounds__(256) void k_fix_scatter(const uint32_t *ids, uint32_t k, const hcx_result *rstage,
+                                                     hcx_result *res) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < k) res[ids[i]] = rstage[i];
+}
+
+int launch_fix_gather(hcx_ctx *c, const uint32_t *ids, uint32_t k, const uint32_t *hist, uint32_t hbins,
+                      const hcx_result *res, uint32_t *rows, hcx_result *rstage) {
+    if (k == 0) return HCX_SUCCESS;
+    k_fix_gather<<<k, 256, 0, c->stream>>>(ids, hist, hbins, res, rows, rstage);
+    HCX_HIP(hipGetLastError());
+    return HCX_SUCCESS;
+}
+
+int launch_fix_scatter(hcx_ctx *c, const uint32_t *ids, uint32_t k, const hcx_result *rstage, hcx_result *res) {
+    if (k == 0) return HCX_SUCCESS;
+    k_fix_scatter<<<(k + 255) / 256, 256, 0, c->stream>>>(ids, k, rstage, res);
     HCX_HIP(hipGetLastError());
     return HCX_SUCCESS;
 }

@@ -39,6 +39,7 @@ RES_SURROGATE = 0x40
 RES_AUDITED = 0x80
 RES_COUNT_OVF = 0x100
 RES_TWO_PASS = 0x200
+RES_FIT_HOST = 0x400
 
 KERNEL_PLAIN = 0
 KERNEL_BATCHED = 1
@@ -48,6 +49,7 @@ OPT_GROUP = 3
 OPT_SCAN = 4
 OPT_GRAB = 5
 OPT_FORCE_REDO = 6
+OPT_FORCE_HOST_FIT = 7
 SCAN_AUTO = 0
 SCAN_TWO_PASS = 1
 EVAL_TAILFIT = 0

@@ -148,6 +148,9 @@ class Engine:
         """LDS hit-list entries per work item: -1 as many as fit, 0 none."""
         N.check(N.hcx().hcx_set_option(self._h, N.OPT_HITLIST, C.c_int64(entries)))
 
+    def set_option(self, option: int, value: int) -> None:
+        N.check(N.hcx().hcx_set_option(self._h, option, C.c_int64(value)))
+
     def set_scan(self, mode: int) -> None:
         """N.SCAN_AUTO: the packed single-pass scan where it is exact, the
         two-pass scan elsewhere; N.SCAN_TWO_PASS: the two-pass scan throughout."""
@@ -330,7 +333,7 @@ class Engine:
         return out
 
     def close(self) -> None:
-        for ix in self.indices:
+        for ix in list(self.indices):
             ix.free()
         self.indices = []
         if getattr(self, "_h", None):

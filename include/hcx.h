@@ -129,6 +129,10 @@ typedef struct hcx_result {
                                         search re-ran the spectrum on HCX_KERNEL_CURSOR  */
 #define HCX_RES_TWO_PASS     0x0200u /* scored by the two-pass kernel (outside the packed
                                         scan's exact range, or HCX_SCAN_TWO_PASS)        */
+#define HCX_RES_FIT_HOST     0x0400u /* tail fit recomputed on the host: the device's slope
+                                        or intercept (x 1e6) sat within the log10 error
+                                        bound of an integer, or a log10 value of a window
+                                        threshold, so the host's FP64 path decided it    */
 
 /* Per-spectrum work counters (the algorithmic-bytes inputs). */
 typedef struct hcx_stats {
@@ -190,12 +194,16 @@ int  hcx_set_tuning(hcx_ctx *ctx, uint32_t chunk, uint32_t slots, uint32_t block
  *   default 64).
  * HCX_OPT_FORCE_REDO: test hook: besides the spectra whose two-pass counters
  *   neared overflow (HCX_RES_COUNT_OVF), every n-th spectrum is re-scored on
- *   the cursor kernel (0, the default: none). */
+ *   the cursor kernel (0, the default: none).
+ * HCX_OPT_FORCE_HOST_FIT: test hook: besides the fits the device lists as
+ *   too close to a boundary (HCX_RES_FIT_HOST), every n-th spectrum's tail
+ *   fit is recomputed on the host (0, the default: none). */
 #define HCX_OPT_HITLIST    1
 #define HCX_OPT_GROUP      3
 #define HCX_OPT_SCAN       4
 #define HCX_OPT_GRAB       5
 #define HCX_OPT_FORCE_REDO 6
+#define HCX_OPT_FORCE_HOST_FIT 7
 #define HCX_SCAN_AUTO      0
 #define HCX_SCAN_TWO_PASS  1
 int  hcx_set_option(hcx_ctx *ctx, int32_t option, int64_t value);

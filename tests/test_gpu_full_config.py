@@ -113,3 +113,79 @@ def test_open10m_benchmark_workload_full_size():
         assert np.array_equal(getattr(got, f), getattr(ref, f)), f
     log("  pickpeaks: bit-exact")
     eng.close()
+
+
+@pytest.mark.timeout(900)
+def test_large40m_full_size_index_properties_and_sample():
+    # the large workload: 12.7M tryptic peptides (7-40) with Met-oxidation
+    # (<= 3 per peptide), ~40M forms, ~4.2G fragment ions, charge 1-2
+    # fragments; 1,000,000 spectra at 0.01 Da fragment bins (dF = 1), open
+    # search +-500 Da.  Every length's iA/bA is checked by its defining
+    # properties (a bin-sorted permutation of the ion ids, ascending inside
+    # each bin, bA the bin offsets, sampled peptides' ions in their own
+    # bins); then 5,000 random spectra are compared field by field with the
+    # oracle searching the exported index, and the same spectra inside the
+    # full 1M-spectrum search equal that.
+    p = W.PRESETS["large40m"]
+    t0 = time.perf_counter()
+    db = W.Database(p)
+    q = W.make_queries(db, p)
+    prm = W.make_params(p)
+    log(f"test_large40m: {db.nforms} forms, {q.nspec} spectra ({time.perf_counter() - t0:.1f}s)")
+    assert db.nforms > 38_000_000 and q.nspec == 1_000_000
+    eng = Engine(prm)
+    idx = eng.construct_db(db)
+    NB = eng.NB
+    nions = sum(ix.nions for ix in idx)
+    log(f"  GPU index: {nions} ions, {sum(ix.build_ms for ix in idx):.1f} ms")
+    assert nions > 4_000_000_000
+    rng = np.random.default_rng(5)
+    built = {}
+    for L, ix in zip(db.lengths, idx):
+        ents = db.entries(L)
+        seqs = db.seqs(L)
+        n = len(ents)
+        speclen = 2 * prm.maxz * (L - 1)
+        total = n * speclen
+        iA, bA = ix.export()
+        assert len(iA) == total and bA[0] == 0 and int(bA[NB]) == total, L
+        assert np.all(bA[1:] >= bA[:-1]), L
+        inv = np.full(total, 0xFFFFFFFF, np.uint32)
+        inv[iA] = np.arange(total, dtype=np.uint32)
+        assert not np.any(inv == 0xFFFFFFFF), L          # every ion id exactly once
+        desc = np.nonzero(iA[1:] <= iA[:-1])[0] + 1
+        assert np.all(np.isin(desc, bA)), L               # a descent only where a bin starts
+        for k in rng.choice(n, min(n, 64), replace=False):
+            e = ents[k]
+            sid = int(e["seq_id"])
+            seq = bytes(seqs[sid * L:(sid + 1) * L]).decode()
+            m, bins = O.gen_spectrum(seq, prm, int(e["sites"]), int(e["mod_num"]))
+            if m < prm.min_mass or m > prm.max_mass:
+                bins[:] = 0
+            else:
+                bins = np.minimum(bins, NB - 1)
+            pos = inv[int(k) * speclen + np.arange(speclen)].astype(np.int64)
+            assert np.all((bA[bins] <= pos) & (pos < bA[bins + 1])), (L, int(k))
+        del inv, desc
+        built[L] = (iA, bA)
+        log(f"  length {L}: {n} forms, {total} ions: properties hold")
+
+    sel = np.sort(rng.choice(q.nspec, 5000, replace=False))
+    sample = q.subset(sel)
+    odb = O.OracleDB.from_built(prm, db, built)
+    t0 = time.perf_counter()
+    res_o, hist_o, st_o = odb.search(sample)
+    log(f"  oracle search of 5,000 spectra {time.perf_counter() - t0:.1f}s")
+    del odb, built
+    bs = eng.upload(sample)
+    out = eng.query_spectrum(bs, kernel=N.KERNEL_BATCHED)
+    n = compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, "large40m-sample")
+    log(f"  sample: {n} spectra with e-values match the oracle")
+    assert n > 4000
+    bs.free()
+    b = eng.upload(q)
+    full = eng.query_spectrum(b, kernel=N.KERNEL_BATCHED, want_hist=False)
+    assert np.array_equal(full.results[sel], out.results)
+    assert np.array_equal(full.stats[sel], out.stats)
+    log(f"  full 1M-spectrum search: the sample's rows equal the sample search")
+    eng.close()
