@@ -514,7 +514,7 @@ int search_mem_alloc(hcx_ctx *c, hcx_qbatch *qb, int nidx, uint64_t nch_all, int
     const uint64_t n = (uint64_t)qb->nspec;
     if (c->lens.ensure(sizeof(LenDesc) * std::max(nidx, 1)) || qb->res.ensure(sizeof(hcx_result) * n + 16) ||
         qb->hist.ensure(4ull * c->hbins * n + 16) || qb->stats.ensure(sizeof(hcx_stats) * n + 16) ||
-        c->tie.ensure(4 * (n + 2) + 16) || c->work.ensure(64) || qb->acc.ensure(sizeof(SpecAcc) * n + 16) ||
+        c->tie.ensure(4 * (n + 2) + 16) || c->work.ensure(64 + 4ull * kSurSlots * kSurStride) || qb->acc.ensure(sizeof(SpecAcc) * n + 16) ||
         qb->win.ensure(8ull * n * std::max(nidx, 1) + 16) || qb->packed.ensure(n + 16) ||
         qb->fb.ensure(4 * (n + 1) + 16) || qb->counts.ensure(4 * (n + 1) + 16) ||
         qb->item_base.ensure(4 * (n + 1) + 16) || qb->item_spec.ensure(4 * item_spec_words(n, nch_all) + 16) ||
@@ -931,24 +931,32 @@ int run_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, cons
         // scan, then predicted e-values, the audit sample's plain fits, and
         // the plain fit for everything if the audit finds a disagreement
         rc = launch_sequence(c, sp, false);
-        if (!rc) HCX_HIP(hipEventRecord(c->ev4, c->stream));
-        if (!rc) rc = mlp_infer(c, qb);
-        if (!rc) HCX_HIP(hipEventRecord(c->ev5, c->stream));
+        // audit counters (4 words); spectra located / plain-fitted, counted
+        // in kSurSlots slots a cache line apart
         uint32_t *cnt = (uint32_t *)c->work.p + 4;
+        uint32_t *scnt = (uint32_t *)c->work.p + 16;
         if (!rc) rc = launch_fill_u32(c, cnt, 4, 0);
+        if (!rc) rc = launch_fill_u32(c, scnt, kSurSlots * kSurStride, 0);
         if (!rc) rc = launch_fill_u32(c, ca.redo, 1, 0);
+        if (!rc) HCX_HIP(hipEventRecord(c->ev4, c->stream));
+        if (!rc) rc = mlp_infer(c, qb, ca.redo, scnt);
+        if (!rc) HCX_HIP(hipEventRecord(c->ev5, c->stream));
         const uint32_t every = opts->audit_every ? opts->audit_every : 64;
         const double tol = opts->audit_tol > 0.0 ? opts->audit_tol : 1e-3;
         if (!rc)
             rc = launch_tailfit_audit(c, a.res, a.hist, a.hbins, a.nspec, c->params.min_cpsm, c->params.expect_max,
                                       every, tol, cnt, ca.redo, c->force_host_fit);
         if (rc) return rc;
-        uint32_t h[4];
-        HCX_HIP(hipMemcpyAsync(h, cnt, 16, hipMemcpyDeviceToHost, c->stream));
+        uint32_t h[12 + kSurSlots * kSurStride];   // cnt (work word 4) through the slots (word 16)
+        HCX_HIP(hipMemcpyAsync(h, cnt, sizeof h, hipMemcpyDeviceToHost, c->stream));
         HCX_HIP(hipStreamSynchronize(c->stream));
         hcx_audit au = {};
         au.audited = h[0];
         au.disagreed = h[1];
+        for (uint32_t k = 0; k < kSurSlots; k++) {
+            au.located += h[12 + k * kSurStride];
+            au.plain_fit += h[12 + k * kSurStride + 1];
+        }
         uint64_t eb = (uint64_t)h[2] | ((uint64_t)h[3] << 32);
         std::memcpy(&au.max_rel_err, &eb, 8);
         au.fell_back = h[1] > 0 ? 1 : 0;

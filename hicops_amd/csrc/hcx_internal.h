@@ -258,6 +258,10 @@ constexpr double kFitYErr = 3.0e-14;
 // and a window threshold comparison is trusted when the value is further
 // than kFitWinEps from the threshold
 constexpr double kFitWinEps = 1.0e-12;
+// the surrogate's located / plain-fitted counters: kSurSlots slots, each
+// kSurStride words (a cache line) apart, so that one atomic add per spectrum
+// does not queue on one address
+constexpr uint32_t kSurSlots = 64, kSurStride = 32;
 // spectra fitted again on the host per round (staging rows)
 constexpr uint32_t kFixRows = 256;
 int selftest_log2(hcx_ctx *ctx, uint32_t n, float *out);
@@ -280,6 +284,14 @@ int launch_tailfit(hcx_ctx *ctx, hcx_result *res, const uint32_t *hist, uint32_t
 int launch_tailfit_audit(hcx_ctx *ctx, hcx_result *res, const uint32_t *hist, uint32_t hbins, int32_t n,
                          uint32_t min_cpsm, double expect_max, uint32_t stride, double tol, uint32_t *cnt,
                          uint32_t *fix, uint32_t force = 0);
+// plain-fit window ends (absolute bins, 2 per spectrum, -1: no regular fit)
+int launch_fit_windows(hcx_ctx *ctx, const hcx_result *res, const uint32_t *hist, uint32_t hbins, int32_t n,
+                       uint32_t min_cpsm, int32_t *win);
+// surrogate e-values: predicted window ends located exactly, else the plain
+// fit; cnt: kSurSlots slots of (located, plain) kSurStride words apart
+int launch_sur_fit(hcx_ctx *ctx, const float *Y, uint32_t ystride, const float *norm, hcx_result *res,
+                   const uint32_t *hist, uint32_t hbins, int32_t n, uint32_t min_cpsm, double expect_max,
+                   uint32_t *fix, uint32_t force, uint32_t *cnt);
 // ---- host tail fit (fit_host.hip) ----
 // expeRT::ModelTailFit on the host in FP64 with the host libm's log10, for
 // the spectra the device lists as fragile.  sx: hbins doubles of workspace;
@@ -304,7 +316,9 @@ uint32_t mlp_param_count();
 int mlp_alloc(hcx_ctx *c);
 int mlp_set(hcx_ctx *c, const float *params, const float *norm6);
 int mlp_get(hcx_ctx *c, float *params, float *norm6);
-int mlp_infer(hcx_ctx *c, hcx_qbatch *qb);
+// features, network, then e-values by k_sur_fit (fix: the host re-fit list;
+// cnt: the located / plain-fit counter slots)
+int mlp_infer(hcx_ctx *c, hcx_qbatch *qb, uint32_t *fix, uint32_t *cnt);
 int mlp_gradient(hcx_ctx *c, const float *X, const float *T, uint32_t n, float *grad, float *loss);
 int mlp_train(hcx_ctx *c, hcx_qbatch *qb, uint32_t steps, uint32_t batch, float lr, uint64_t seed, float *loss_out);
 int selftest_mfma(hcx_ctx *c, const float *A, const float *B, float *D);

@@ -3,12 +3,17 @@
 // backward and Adam kernels.
 //
 // Input per spectrum (128 features, from its score histogram): the log10
-// survival function S(b) = 1 - C(b)/cpsms (C = cumulative histogram, the
-// survival function of expeRT.cpp:600-640) at the 96 bins b = hyp-1 ... hyp-96
-// below the top score's bin hyp = (int)(hs*10 + 0.5), plus log10(cpsms)/5 and
-// hyp/1000.  Only the 97 bins up to hyp are read.  Outputs (3 of 16 used), standardised: the plain fit's slope
-// mu/1e6, intercept beta/1e6, and log10 s(hyp) = slope*hyp + intercept, from
-// which e = 10^log10 s(hyp) * cpsms (dslim_query.cpp:786-799).
+// survival function S(b) = 1 - C(b)/cpsms (C: the counts through bin b, the
+// survival function of expeRT.cpp:600-640), clipped at 1e-7, at the 124
+// bins b_f = f * hyp / 124 below the top score's bin hyp = (int)(hs*10 +
+// 0.5), then log10(cpsms)/5, hyp/1000 and the first and last occupied bin
+// below hyp over hyp.  Outputs (3 of 16 used), standardised: the plain fit's
+// window ends over hyp -- the first bin at or below 22 % of the log-survival
+// drop and the last at or above 87 % (expeRT.cpp:657-681) -- and the window's
+// width over hyp.  The e-value is not predicted: k_sur_fit locates the
+// predicted ends exactly among their 64 neighbours and fits the line through
+// the window, giving the plain fit's ints, or runs the plain fit where the
+// prediction misses.
 //
 // One wavefront handles 16 spectra (the MFMA's M); activations stay in LDS.
 // MFMA operand layout (16x16x16, wave64): lane l supplies A[m = l%16][k =
@@ -31,7 +36,7 @@ constexpr int kH = 128;          // hidden width
 constexpr int kO = 16;           // outputs (kOut used)
 constexpr int kOut = 3;
 constexpr int kRows = 16;        // spectra per wavefront
-constexpr int kWin = 96;         // survival-function bins below hyp
+constexpr int kPts = 124;        // survival-function points below hyp
 
 // parameter block (FP32 master copy): W1[kH][kF] b1[kH] W2[kH][kH] b2[kH] W3[kO][kH] b3[kO]
 constexpr int oW1 = 0, ob1 = oW1 + kH * kF, oW2 = ob1 + kH, ob2 = oW2 + kH * kH, oW3 = ob2 + kH,
@@ -108,11 +113,12 @@ struct MlpW {
     const float *p;       // kParams f32 (biases read from here)
 };
 
-// Features and (when res holds a plain fit) training targets, one wave per
-// spectrum.  X: [npad][kF] bf16; T: [npad][4] f32 (t0, t1, t2, weight).
+// Features and (with win, the plain fit's window ends) training targets,
+// one wave per spectrum.  X: [npad][kF] bf16; T: [npad][4] f32 (t0, t1, t2,
+// weight).
 __global__ __launch_bounds__(256) void k_sur_features(const hcx_result *res, const uint32_t *hist, uint32_t hbins,
-                                                      int32_t n, int32_t npad, uint32_t min_cpsm, uint16_t *X,
-                                                      float *T) {
+                                                      int32_t n, int32_t npad, uint32_t min_cpsm, const int32_t *win,
+                                                      uint16_t *X, float *T) {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int32_t s = blockIdx.x * 4 + w;
     if (s >= npad) return;
@@ -127,41 +133,53 @@ __global__ __launch_bounds__(256) void k_sur_features(const hcx_result *res, con
         if (t && lane < 4) t[lane] = 0.0f;
         return;
     }
-    // Every candidate scores at or below the top hit, so the histogram is zero
-    // above hyp and S(b) = (sum of h over (b, hyp]) / cpsms: a suffix sum over
-    // the 96 bins below hyp and bin hyp itself.  Feature f is S(hyp - 1 - f).
+    // running counts over bins [0, hyp) into this wave's LDS row, then
+    // feature f reads C(b_f) (lane f % 64)
+    extern __shared__ uint32_t cs_raw[];
+    uint32_t *cs = cs_raw + (size_t)w * hbins;
     const uint32_t *h = hist + (size_t)s * hbins;
-    const double inv = 1.0 / (double)r.cpsms;
     uint32_t carry = 0;
-#pragma unroll
-    for (int part = 0; part < 2; part++) {
-        const int j = part * 64 + lane;             // bin hyp - j
-        uint32_t v = (j < kWin && hyp - j >= 0) ? h[hyp - j] : 0u;
+    int first = -1, last = -1;
+    for (int B = 0; B < hyp; B += 64) {
+        const int b = B + lane;
+        const uint32_t v0 = b < hyp ? h[b] : 0u;
+        const unsigned long long occ = __ballot(v0 != 0u);
+        if (occ) {
+            if (first < 0) first = B + (__ffsll((long long)occ) - 1);
+            last = B + 63 - __clzll((long long)occ);
+        }
+        uint32_t v = v0;
         for (int off = 1; off < 64; off <<= 1) {
             const uint32_t y = __shfl_up(v, off, 64);
             if (lane >= off) v += y;
         }
-        const uint32_t suf = carry + v;             // sum of h over [hyp - j, hyp]
-        if (j < kWin) {
-            const int32_t b = hyp - 1 - j;
-            const double S = (double)suf * inv;
-            const double lg = b < 0 ? 0.0 : log10(S > 1e-7 ? S : 1e-7);
-            x[j] = f2bf((float)lg);
-        }
+        if (b < hyp) cs[b] = carry + v;
         carry += __shfl(v, 63, 64);
     }
-    if (lane < kF - kWin) {
+    __syncwarp();
+    const uint32_t mine0 = cs[(lane * hyp) / kPts];
+    const uint32_t mine1 = lane + 64 < kPts ? cs[((lane + 64) * hyp) / kPts] : 0u;
+    const double inv = 1.0 / (double)r.cpsms;
+    auto feat = [&](uint32_t c) {
+        const double S = 1.0 - (double)c * inv;
+        return f2bf((float)log10(S > 1e-7 ? S : 1e-7));
+    };
+    x[lane] = feat(mine0);
+    if (lane + 64 < kPts) x[lane + 64] = feat(mine1);
+    if (lane < kF - kPts) {
         float v = 0.0f;
         if (lane == 0) v = (float)(log10((double)r.cpsms) / 5.0);
         if (lane == 1) v = (float)hyp / 1000.0f;
-        x[kWin + lane] = f2bf(v);
+        if (lane == 2) v = (float)(first < 0 ? 0 : first) / (float)hyp;
+        if (lane == 3) v = (float)(last < 0 ? 0 : last) / (float)hyp;
+        x[kPts + lane] = f2bf(v);
     }
     if (t && lane == 0) {
-        const bool ok = (r.flags & HCX_RES_FIT_UB) == 0;
-        const double t0 = r.mu / 1e6, t1 = r.beta / 1e6;
-        t[0] = (float)t0;
-        t[1] = (float)t1;
-        t[2] = (float)(t0 * hyp + t1);
+        const int32_t lo = win[2 * s], hi = win[2 * s + 1];
+        const bool ok = lo >= 0 && hi >= 0;
+        t[0] = ok ? (float)lo / (float)hyp : 0.0f;
+        t[1] = ok ? (float)hi / (float)hyp : 0.0f;
+        t[2] = ok ? (float)(hi - lo) / (float)hyp : 0.0f;
         t[3] = ok ? 1.0f : 0.0f;
     }
 }
@@ -343,34 +361,6 @@ __global__ void k_mlp_adam(float *P, float *G, float *M, float *V, uint16_t *bf,
     }
 }
 
-// Surrogate e-values for the spectra with cpsms >= min_cpsm.
-__global__ void k_sur_apply(const float *Y, hcx_result *res, int32_t n, uint32_t min_cpsm, double expect_max,
-                            const float *norm) {
-    const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    hcx_result r = res[s];
-    uint16_t flags = r.flags & ~(HCX_RES_VALID | HCX_RES_REPORTED | HCX_RES_SURROGATE | HCX_RES_AUDITED);
-    if (r.cpsms < min_cpsm || r.cpsms == 0) {
-        res[s].flags = flags;
-        return;
-    }
-    const int32_t hyp = (int32_t)((double)(r.hyperscore * 10.0f) + 0.5);
-    const float *y = Y + (size_t)s * kO;
-    const double t0 = (double)y[0] * norm[3] + norm[0];
-    const double t1 = (double)y[1] * norm[4] + norm[1];
-    const double t2 = (double)y[2] * norm[5] + norm[2];
-    r.maxhypscore = hyp;
-    r.mu = (int32_t)(t0 * 1e6);
-    r.beta = (int32_t)(t1 * 1e6);
-    r.minhypscore = 0;
-    r.nexthypscore = 0;
-    r.e_x = pow(10.0, t2) * (double)r.cpsms;
-    flags |= HCX_RES_VALID | HCX_RES_SURROGATE;
-    if (r.e_x < expect_max) flags |= HCX_RES_REPORTED;
-    r.flags = flags;
-    res[s] = r;
-}
-
 // MFMA lane-layout self-check: D = A B for 16x16 operands (row-major f32,
 // rounded to bf16), through wave_gemm (B supplied transposed, as W).
 __global__ void k_selftest_mfma(const float *A, const float *B, float *D) {
@@ -444,22 +434,31 @@ int mlp_get(hcx_ctx *c, float *params, float *norm6) {
     return HCX_SUCCESS;
 }
 
-// features (+ targets) of a searched batch into the context's scratch
+// features (+ targets) of a searched batch into the context's scratch:
+// X [npad][kF] bf16 | T [npad][4] f32 | Y [npad][kO] f32 | [npad][kH] f32 | win [npad][2] i32
 static int features(hcx_ctx *c, hcx_qbatch *qb, bool targets, int32_t *npad_out) {
     const int32_t n = qb->nspec, npad = (n + kRows - 1) / kRows * kRows;
-    const size_t need = 2ull * npad * kF + 4ull * npad * 4 + 4ull * npad * kO + 4ull * npad * kH + 256;
+    const size_t need = 2ull * npad * kF + 4ull * npad * 4 + 4ull * npad * kO + 4ull * npad * kH + 8ull * npad + 256;
     if (c->sur_tmp.ensure(need)) return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "surrogate: scratch allocation failed");
     uint16_t *X = (uint16_t *)c->sur_tmp.p;
     float *T = (float *)(X + (size_t)npad * kF);
-    k_sur_features<<<(npad + 3) / 4, 256, 0, c->stream>>>((const hcx_result *)qb->res.p, (const uint32_t *)qb->hist.p,
-                                                         c->hbins, n, npad, c->params.min_cpsm, X,
-                                                         targets ? T : nullptr);
+    int32_t *win = (int32_t *)(T + (size_t)npad * 4 + (size_t)npad * kO + (size_t)npad * kH);
+    if (targets) {
+        int rc = launch_fit_windows(c, (const hcx_result *)qb->res.p, (const uint32_t *)qb->hist.p, c->hbins, n,
+                                    c->params.min_cpsm, win);
+        if (rc) return rc;
+    }
+    const size_t sm = 4ull * 4 * c->hbins;   // four waves' running counts
+    HCX_HIP(hipFuncSetAttribute((const void *)k_sur_features, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+    k_sur_features<<<(npad + 3) / 4, 256, sm, c->stream>>>((const hcx_result *)qb->res.p, (const uint32_t *)qb->hist.p,
+                                                         c->hbins, n, npad, c->params.min_cpsm, targets ? win : nullptr,
+                                                         X, targets ? T : nullptr);
     HCX_HIP(hipGetLastError());
     *npad_out = npad;
     return HCX_SUCCESS;
 }
 
-int mlp_infer(hcx_ctx *c, hcx_qbatch *qb) {
+int mlp_infer(hcx_ctx *c, hcx_qbatch *qb, uint32_t *fix, uint32_t *cnt) {
     if (!c->sur_ready) return set_error_code(HCX_ERR_INVLD_PARAM, "surrogate: not trained");
     int32_t npad = 0;
     int rc = features(c, qb, false, &npad);
@@ -469,10 +468,8 @@ int mlp_infer(hcx_ctx *c, hcx_qbatch *qb) {
     const MlpW w = {surB(c), surP(c)};
     k_mlp_fwd<false><<<npad / kRows, 64, 0, c->stream>>>(X, w, npad, Y, nullptr, nullptr);
     HCX_HIP(hipGetLastError());
-    k_sur_apply<<<(qb->nspec + 255) / 256, 256, 0, c->stream>>>(Y, (hcx_result *)qb->res.p, qb->nspec,
-                                                               c->params.min_cpsm, c->params.expect_max, surN(c));
-    HCX_HIP(hipGetLastError());
-    return HCX_SUCCESS;
+    return launch_sur_fit(c, Y, kO, surN(c), (hcx_result *)qb->res.p, (const uint32_t *)qb->hist.p, c->hbins,
+                          qb->nspec, c->params.min_cpsm, c->params.expect_max, fix, c->force_host_fit, cnt);
 }
 
 // one forward + backward over rows [0, npad) of X/T in scratch; G += grads
@@ -529,8 +526,8 @@ int mlp_gradient(hcx_ctx *c, const float *Xh, const float *Th, uint32_t n, float
     return HCX_SUCCESS;
 }
 
-// Train on a batch searched with the plain fit: targets are its (mu, beta)
-// and log10 s(hyp).  Mini-batches of `batch` rows (multiple of 16), `steps`
+// Train on a batch searched with the plain fit: targets are its window ends
+// and width over hyp.  Mini-batches of `batch` rows (multiple of 16), `steps`
 // Adam steps at learning rate lr.  *loss_out: mean loss of the last 10% of steps.
 int mlp_train(hcx_ctx *c, hcx_qbatch *qb, uint32_t steps, uint32_t batch, float lr, uint64_t seed, float *loss_out) {
     if (!c->sur_ready) return set_error_code(HCX_ERR_INVLD_PARAM, "surrogate: not initialised");

@@ -528,6 +528,7 @@ struct FitOut {
     int32_t mu, beta, stt1, end1;
     uint32_t flags;
     bool fragile;   // another correct log10 could give other ints: fit it again on the host
+    int32_t lm, rm; // n > 3: first index at or below the 22 % threshold, last at or above the 87 % one
 };
 
 // v is within tol (+ its own rounding) of an integer, so truncating another
@@ -536,38 +537,130 @@ __device__ __forceinline__ bool near_int(double v, double tol) {
     return fabs(v - rint(v)) <= tol + 4.0 * DBL_EPSILON * fabs(v);
 }
 
-// sx: LDS scratch of this wave (>= hbins doubles).  Lane 0's return is valid.
-__device__ FitOut tailfit_wave(const uint32_t *H, uint32_t hbins, uint32_t cpsms, int32_t hyp, double *sx) {
+// The fit's range: end1 = rargmax(yy, 0, hyp-1, 1.0), the last p <= hyp-1
+// with yy[p] >= 1 (else hyp-1); stt1 = argmax(yy, 0, end1, 1.0), the first p
+// in [0, end1] with yy[p] >= 1 (else 0).  Wave-uniform.
+__device__ void fit_range(const uint32_t *H, int32_t hyp, int *stt1_out, int *end1_out) {
     const int lane = threadIdx.x & 63;
-    FitOut o = {0, 0, 0, 0, 0, false};
-    if (hyp < 1 || hyp >= (int)hbins) {
-        // the reference reads yy[-1] or past the histogram (UB): a zero fit
-        o.flags = HCX_RES_FIT_UB;
-        return o;
-    }
-    // end1 = rargmax(yy, 0, hyp-1, 1.0): last p <= hyp-1 with yy[p] >= 1, else hyp-1
     int end1 = hyp - 1;
     for (int top = hyp - 1; top >= 0; top -= 64) {
         const int p = top - lane;
-        const bool ok = p >= 0 && H[p] >= 1u;
-        const unsigned long long m = __ballot(ok);
+        const unsigned long long m = __ballot(p >= 0 && H[p] >= 1u);
         if (m) {
             end1 = top - (__ffsll((long long)m) - 1);
             break;
         }
     }
-    // stt1 = argmax(yy, 0, end1, 1.0): first p in [0, end1] with yy[p] >= 1, else 0
     int stt1 = 0;
     for (int b = 0; b <= end1; b += 64) {
         const int p = b + lane;
-        const bool ok = p <= end1 && H[p] >= 1u;
-        const unsigned long long m = __ballot(ok);
+        const unsigned long long m = __ballot(p <= end1 && H[p] >= 1u);
         if (m) {
             stt1 = b + (__ffsll((long long)m) - 1);
             break;
         }
     }
     if (stt1 == end1) end1 += 1;
+    *stt1_out = stt1;
+    *end1_out = end1;
+}
+
+// The survival function over the window before its log10: sx[i] = 1 -
+// (inclusive count through bin stt1 + i) / cpsms, values above 1 set to
+// 0.999; returns the replacement for values <= 0, sx at rargmax(sx, 0, n-1,
+// 1e-4).  Wave-uniform; sx complete on return.
+__device__ double fit_survival(const uint32_t *H, int stt1, int n, uint32_t cpsms, double *sx) {
+    const int lane = threadIdx.x & 63;
+    const double vaa = (double)(int)cpsms;
+    // inclusive prefix sums of integer counts: exact in any order
+    double carry = 0.0;
+    for (int b = 0; b < n; b += 64) {
+        const int i = b + lane;
+        double x = (i < n) ? (double)H[stt1 + i] : 0.0;
+        for (int off = 1; off < 64; off <<= 1) {
+            double y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        if (i < n) {
+            double t = (carry + x) / vaa;
+            t = t + (-1.0);
+            sx[i] = -t;
+        }
+        carry += __shfl(x, 63, 64);
+    }
+    __syncwarp();
+    for (int i = lane; i < n; i += 64)
+        if (sx[i] > 1.0) sx[i] = 0.999;
+    __syncwarp();
+    int rep = n - 1;
+    for (int top = n - 1; top >= 0; top -= 64) {
+        const int p = top - lane;
+        const unsigned long long m = __ballot(p >= 0 && sx[p] >= 1e-4);
+        if (m) {
+            rep = top - (__ffsll((long long)m) - 1);
+            break;
+        }
+    }
+    const double rv = sx[rep];
+    __syncwarp();
+    return rv;
+}
+
+// Lane 0: the least-squares line through the log10 values y[0, m) at bins
+// stt1 + mark + i (expeRT.cpp:686-704), with the bound of how far slope and
+// intercept move under another log10 within kFitYErr; fragile when the ints
+// (values x 1e6 truncated) could then differ.
+__device__ void fit_line(const double *y, int m, int x0, double &mu_t, double &beta_t, bool &fragile,
+                         uint32_t &flags) {
+    // e_mu, e_beta: first order through the sums, plus one rounding
+    // difference per operation
+    double e_mu = 0.0, e_beta = kFitYErr;
+    if (m == 1) {
+        mu_t = 0.0;
+        beta_t = y[0];
+    } else {
+        double xbar = 0.0, ybar = 0.0, top = 0.0, bot = 0.0, sdx = 0.0, sab = 0.0;
+        for (int i = 0; i < m; i++) {
+            xbar = xbar + (double)(x0 + i);
+            ybar = ybar + y[i];
+        }
+        xbar = xbar / (double)m;
+        ybar = ybar / (double)m;
+        for (int i = 0; i < m; i++) {
+            const double dx = (double)(x0 + i) - xbar;
+            top = top + dx * (y[i] - ybar);
+            bot = bot + dx * dx;
+            sdx += fabs(dx);
+            sab += fabs(dx * (y[i] - ybar));
+        }
+        mu_t = top / bot;
+        beta_t = ybar - mu_t * xbar;
+        e_mu = (2.0 * kFitYErr * sdx + (double)(m + 2) * DBL_EPSILON * sab) / bot + 2.0 * DBL_EPSILON * fabs(mu_t);
+        e_beta = kFitYErr + (double)m * DBL_EPSILON * fabs(ybar) + e_mu * fabs(xbar) +
+                 2.0 * DBL_EPSILON * (fabs(mu_t * xbar) + fabs(beta_t));
+    }
+    // margin 4x; a single point's slope is an exact 0
+    if ((m > 1 && near_int(mu_t * 1e6, 4e6 * e_mu)) || near_int(beta_t * 1e6, 4e6 * e_beta) ||
+        fabs(fabs(mu_t) - 2.0e3) <= 4.0 * e_mu || fabs(fabs(beta_t) - 2.0e3) <= 4.0 * e_beta)
+        fragile = true;
+    if (!(fabs(mu_t) < 2.0e3) || !(fabs(beta_t) < 2.0e3)) {
+        flags |= HCX_RES_FIT_UB;
+        mu_t = 0.0;
+        beta_t = 0.0;
+    }
+}
+
+// sx: LDS scratch of this wave (>= hbins doubles).  Lane 0's return is valid.
+__device__ FitOut tailfit_wave(const uint32_t *H, uint32_t hbins, uint32_t cpsms, int32_t hyp, double *sx) {
+    const int lane = threadIdx.x & 63;
+    FitOut o = {0, 0, 0, 0, 0, false, -1, -1};
+    if (hyp < 1 || hyp >= (int)hbins) {
+        // the reference reads yy[-1] or past the histogram (UB): a zero fit
+        o.flags = HCX_RES_FIT_UB;
+        return o;
+    }
+    int stt1, end1;
+    fit_range(H, hyp, &stt1, &end1);
     const int n = end1 - stt1 + 1;
     o.stt1 = stt1;
     o.end1 = end1;
@@ -580,39 +673,7 @@ __device__ FitOut tailfit_wave(const uint32_t *H, uint32_t hbins, uint32_t cpsms
         mu_t = 0.0;
         beta_t = 100.0;
     } else {
-        const double vaa = (double)(int)cpsms;
-        // inclusive prefix sums of integer counts: exact in any order
-        double carry = 0.0;
-        for (int b = 0; b < n; b += 64) {
-            const int i = b + lane;
-            double x = (i < n) ? (double)H[stt1 + i] : 0.0;
-            for (int off = 1; off < 64; off <<= 1) {
-                double y = __shfl_up(x, off, 64);
-                if (lane >= off) x += y;
-            }
-            if (i < n) {
-                double t = (carry + x) / vaa;
-                t = t + (-1.0);
-                sx[i] = -t;
-            }
-            carry += __shfl(x, 63, 64);
-        }
-        __syncwarp();
-        for (int i = lane; i < n; i += 64)
-            if (sx[i] > 1.0) sx[i] = 0.999;
-        __syncwarp();
-        // replacement = rargmax(sx, 0, n-1, 1e-4)
-        int rep = n - 1;
-        for (int top = n - 1; top >= 0; top -= 64) {
-            const int p = top - lane;
-            const unsigned long long m = __ballot(p >= 0 && sx[p] >= 1e-4);
-            if (m) {
-                rep = top - (__ffsll((long long)m) - 1);
-                break;
-            }
-        }
-        const double rv = sx[rep];
-        __syncwarp();
+        const double rv = fit_survival(H, stt1, n, cpsms, sx);
         for (int i = lane; i < n; i += 64) {
             double v = sx[i];
             if (v <= 0.0) v = rv;
@@ -650,8 +711,10 @@ __device__ FitOut tailfit_wave(const uint32_t *H, uint32_t hbins, uint32_t cpsms
         bool fragile = false;
         if (n >= 3) fragile = near_thr(s0 + hgt * 0.22) || (n > 3 && near_thr(s0 + hgt * 0.87));
         if (n > 3) {
-            mark = largmax(s0 + hgt * 0.22) - 1;
-            mark2 = rargmax(s0 + hgt * 0.87);
+            o.lm = largmax(s0 + hgt * 0.22);
+            o.rm = rargmax(s0 + hgt * 0.87);
+            mark = o.lm - 1;
+            mark2 = o.rm;
             if (mark2 == n) mark2 -= 1;
             if (mark >= mark2) mark = mark2 - 1;
         } else if (n == 3) {
@@ -667,53 +730,130 @@ __device__ FitOut tailfit_wave(const uint32_t *H, uint32_t hbins, uint32_t cpsms
             mu_t = 0.0;
             beta_t = 0.0;
         } else if (lane == 0) {
-            const int m = mark2 - mark + 1;
-            const double *y = sx + mark;
-            // e_mu, e_beta: bounds on how far mu_t and beta_t can move when
-            // every y moves by up to kFitYErr (first order through the sums,
-            // plus one rounding difference per operation)
-            double e_mu = 0.0, e_beta = kFitYErr;
-            if (m == 1) {
-                mu_t = 0.0;
-                beta_t = y[0];
-            } else {
-                double xbar = 0.0, ybar = 0.0, top = 0.0, bot = 0.0, sdx = 0.0, sab = 0.0;
-                for (int i = 0; i < m; i++) {
-                    xbar = xbar + (double)(stt1 + mark + i);
-                    ybar = ybar + y[i];
-                }
-                xbar = xbar / (double)m;
-                ybar = ybar / (double)m;
-                for (int i = 0; i < m; i++) {
-                    const double dx = (double)(stt1 + mark + i) - xbar;
-                    top = top + dx * (y[i] - ybar);
-                    bot = bot + dx * dx;
-                    sdx += fabs(dx);
-                    sab += fabs(dx * (y[i] - ybar));
-                }
-                mu_t = top / bot;
-                beta_t = ybar - mu_t * xbar;
-                e_mu = (2.0 * kFitYErr * sdx + (double)(m + 2) * DBL_EPSILON * sab) / bot +
-                       2.0 * DBL_EPSILON * fabs(mu_t);
-                e_beta = kFitYErr + (double)m * DBL_EPSILON * fabs(ybar) + e_mu * fabs(xbar) +
-                         2.0 * DBL_EPSILON * (fabs(mu_t * xbar) + fabs(beta_t));
-            }
-            // the ints are the values x 1e6 truncated (margin 4x); a single
-            // point's slope is an exact 0
-            if ((m > 1 && near_int(mu_t * 1e6, 4e6 * e_mu)) || near_int(beta_t * 1e6, 4e6 * e_beta) ||
-                fabs(fabs(mu_t) - 2.0e3) <= 4.0 * e_mu || fabs(fabs(beta_t) - 2.0e3) <= 4.0 * e_beta)
-                fragile = true;
-            if (!(fabs(mu_t) < 2.0e3) || !(fabs(beta_t) < 2.0e3)) {
-                o.flags |= HCX_RES_FIT_UB;
-                mu_t = 0.0;
-                beta_t = 0.0;
-            }
+            fit_line(sx + mark, mark2 - mark + 1, stt1 + mark, mu_t, beta_t, fragile, o.flags);
         }
         o.fragile = fragile;
     }
     o.mu = (int32_t)(mu_t * 1e6);
     o.beta = (int32_t)(beta_t * 1e6);
     return o;
+}
+
+// an index whose log10 value is the constant tail's (z when the tail
+// exists; otherwise any index, the value is then unused)
+__device__ __forceinline__ int rep_of_tail(const double *, int z, int n) { return z < n ? z : n - 1; }
+
+// The same fit with the window ends taken from a prediction (absolute bins
+// p_lo ~ stt1 + lm, p_hi ~ stt1 + rm): the survival function is
+// non-increasing up to its first value <= 0 (index z) and constant after
+// (the replacement), so each end is the single threshold crossing of the
+// monotone part, or, for the upper end, the constant tail itself.  The 64
+// lanes test the 64 indices around each predicted end; a crossing found
+// there is exactly the plain fit's, and the line is then fitted through the
+// window alone.  Returns false (wave-uniform) when a crossing lies outside
+// the tested indices or the histogram needs the plain fit's special cases
+// (n <= 3, an undefined path); *o is then unset.
+__device__ bool tailfit_window(const uint32_t *H, uint32_t hbins, uint32_t cpsms, int32_t hyp, double p_lo,
+                               double p_hi, double *sx, FitOut *o) {
+    const int lane = threadIdx.x & 63;
+    if (hyp < 1 || hyp >= (int)hbins || (int)cpsms < 1 || !(p_lo == p_lo) || !(p_hi == p_hi)) return false;
+    int stt1, end1;
+    fit_range(H, hyp, &stt1, &end1);
+    const int n = end1 - stt1 + 1;
+    if (n <= 3 || end1 >= (int)hbins) return false;
+    const double rv = fit_survival(H, stt1, n, cpsms, sx);
+    int z = n;   // first value <= 0
+    for (int b = 0; b < n; b += 64) {
+        const int i = b + lane;
+        const unsigned long long m = __ballot(i < n && sx[i] <= 0.0);
+        if (m) {
+            z = b + (__ffsll((long long)m) - 1);
+            break;
+        }
+    }
+    if (z == 0 || !(rv > 0.0)) return false;
+    auto lg = [&](int i) {
+        double v = sx[i];
+        if (v <= 0.0) v = rv;
+        return log10(v);
+    };
+    // lanes 0, 1, 2: the first and last values and the constant tail's
+    // (indices >= z), in one log10 for the wave
+    const double l3 = lg(lane == 0 ? 0 : lane == 1 ? n - 1 : rep_of_tail(sx, z, n));
+    const double s0 = __shfl(l3, 0, 64);
+    const double hgt = __shfl(l3, 1, 64) - s0;
+    const double lr = __shfl(l3, 2, 64);
+    const double th1 = s0 + hgt * 0.22, th2 = s0 + hgt * 0.87;
+    if (!(th1 == th1) || !(th2 == th2) || !(lr == lr)) return false;
+    // lower end: first index with value <= th1, a crossing inside [1, z);
+    // lane l holds index c1 + l and tests it against lane l - 1's
+    const double plo = fmin(fmax(rint(p_lo) - (double)stt1, 1.0), (double)(n - 1));
+    const int c1 = (int)plo - 33;
+    int lm = -1;
+    bool frag = false;
+    {
+        const int q = c1 + lane;
+        const double a = (q >= 0 && q < n) ? lg(q) : 0.0;
+        const double b = __shfl_up(a, 1, 64);
+        bool hit = false, near = false;
+        if (lane >= 1 && q >= 1 && q < z) {
+            hit = a <= th1 && b > th1;
+            near = fabs(a - th1) <= kFitWinEps || fabs(b - th1) <= kFitWinEps;
+        }
+        const unsigned long long m = __ballot(hit);
+        if (m) lm = c1 + (__ffsll((long long)m) - 1);
+        frag = __ballot(hit && near) != 0;
+    }
+    if (lm < 0) return false;
+    // upper end: last index with value >= th2: the end of the tail when the
+    // tail's value reaches th2, else the crossing inside [0, z)
+    int rm = -1;
+    if (z < n) {
+        if (fabs(lr - th2) <= kFitWinEps) frag = true;
+        if (lr >= th2) rm = n - 1;
+    }
+    if (rm < 0) {
+        // lane l holds index c2 + l and tests it against lane l + 1's
+        const double phi = fmin(fmax(rint(p_hi) - (double)stt1, 0.0), (double)(n - 1));
+        const int c2 = (int)phi - 31;
+        const int q = c2 + lane;
+        const double a = (q >= 0 && q < n) ? lg(q) : 0.0;
+        const double b = __shfl_down(a, 1, 64);
+        bool hit = false, near = false;
+        if (q >= 0 && q < z) {
+            if (q + 1 < z) {
+                if (lane < 63) {
+                    hit = a >= th2 && b < th2;
+                    near = fabs(a - th2) <= kFitWinEps || fabs(b - th2) <= kFitWinEps;
+                }
+            } else {
+                // q = z - 1: next is the tail (below th2 here) or the end
+                hit = a >= th2;
+                near = fabs(a - th2) <= kFitWinEps;
+            }
+        }
+        const unsigned long long m = __ballot(hit);
+        if (m) rm = c2 + (__ffsll((long long)m) - 1);
+        if (__ballot(hit && near)) frag = true;
+    }
+    if (rm < 0) return false;
+    FitOut r = {0, 0, stt1, end1, 0, frag, lm, rm};
+    int mark = lm - 1, mark2 = rm;
+    if (mark2 == n) mark2 -= 1;
+    if (mark >= mark2) mark = mark2 - 1;
+    double mu_t = 0.0, beta_t = 0.0;
+    if (mark < 0) {
+        r.flags |= HCX_RES_FIT_UB;
+    } else {
+        __syncwarp();
+        for (int i = mark + lane; i <= mark2; i += 64) sx[i] = lg(i);   // each lane its own entries
+        __syncwarp();
+        if (lane == 0) fit_line(sx + mark, mark2 - mark + 1, stt1 + mark, mu_t, beta_t, r.fragile, r.flags);
+    }
+    r.mu = (int32_t)(mu_t * 1e6);
+    r.beta = (int32_t)(beta_t * 1e6);
+    *o = r;
+    return true;
 }
 
 __device__ __forceinline__ void list_fragile(uint16_t &flags, uint32_t *fix, uint32_t s) {
@@ -726,7 +866,7 @@ __global__ __launch_bounds__(kBT) void k_tailfit(hcx_result *res, const uint32_t
                                                  uint32_t force) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int32_t s = blockIdx.x * (kBT / 64) + w;
+    const int32_t s = blockIdx.x * (int32_t)(blockDim.x >> 6) + w;
     if (s >= n) return;
     double *sx = (double *)smem_raw + (size_t)w * hbins;
     hcx_result r = res[s];
@@ -767,7 +907,7 @@ __global__ __launch_bounds__(kBT) void k_tailfit_audit(hcx_result *res, const ui
                                                        uint32_t force) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t s = ((int64_t)blockIdx.x * (kBT / 64) + w) * stride;
+    const int64_t s = ((int64_t)blockIdx.x * (blockDim.x >> 6) + w) * stride;
     if (s >= n) return;
     double *sx = (double *)smem_raw + (size_t)w * hbins;
     hcx_result r = res[s];
@@ -804,7 +944,7 @@ __global__ __launch_bounds__(kBT) void k_tailfit_raw(const uint32_t *hist, uint3
                                                      uint32_t *flags, uint32_t force) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint32_t s = blockIdx.x * (kBT / 64) + w;
+    const uint32_t s = blockIdx.x * (blockDim.x >> 6) + w;
     if (s >= n) return;
     double *sx = (double *)smem_raw + (size_t)w * hbins;
     FitOut f = tailfit_wave(hist + (size_t)s * hbins, hbins, cpsms[s], maxhyp[s], sx);
@@ -814,6 +954,86 @@ __global__ __launch_bounds__(kBT) void k_tailfit_raw(const uint32_t *hist, uint3
         out4[4 * s + 2] = f.stt1;
         out4[4 * s + 3] = f.end1;
         if (flags) flags[s] = f.flags | ((f.fragile || (force && s % force == 0)) ? HCX_RES_FIT_HOST : 0u);
+    }
+}
+
+// The plain fit's window ends (absolute bins stt1 + lm, stt1 + rm) of every
+// spectrum with a regular fit (n > 3, defined path), -1 otherwise: the
+// surrogate's training targets.  win: 2 per spectrum.
+__global__ __launch_bounds__(kBT) void k_fit_windows(const hcx_result *res, const uint32_t *hist, uint32_t hbins,
+                                                     int32_t n, uint32_t min_cpsm, int32_t *win) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int32_t s = blockIdx.x * (int32_t)(blockDim.x >> 6) + w;
+    if (s >= n) return;
+    double *sx = (double *)smem_raw + (size_t)w * hbins;
+    const hcx_result r = res[s];
+    int32_t lo = -1, hi = -1;
+    if (r.cpsms >= min_cpsm && r.cpsms > 0) {
+        const int32_t hyp = (int32_t)((double)(r.hyperscore * 10.0f) + 0.5);
+        const FitOut f = tailfit_wave(hist + (size_t)s * hbins, hbins, r.cpsms, hyp, sx);
+        if (f.lm >= 1 && f.rm >= 0 && !(f.flags & HCX_RES_FIT_UB)) {
+            lo = f.stt1 + f.lm;
+            hi = f.stt1 + f.rm;
+        }
+    }
+    if (lane == 0) {
+        win[2 * s] = lo;
+        win[2 * s + 1] = hi;
+    }
+}
+
+// The surrogate's e-values: per spectrum, the predicted window ends (rows of
+// Y, fractions of the top score's bin, standardised by norm) located
+// exactly by tailfit_window, else the plain fit (tailfit_wave); either way
+// the plain fit's ints.  cnt: per slot (workgroup modulo kSurSlots, slots
+// kSurStride words apart) the spectra whose window the prediction located
+// and those that took the plain fit.
+__global__ __launch_bounds__(kBT) void k_sur_fit(const float *Y, uint32_t ystride, const float *norm,
+                                                 hcx_result *res, const uint32_t *hist, uint32_t hbins, int32_t n,
+                                                 uint32_t min_cpsm, double expect_max, uint32_t *fix, uint32_t force,
+                                                 uint32_t *cnt) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int32_t s = blockIdx.x * (int32_t)(blockDim.x >> 6) + w;
+    if (s >= n) return;
+    double *sx = (double *)smem_raw + (size_t)w * hbins;
+    hcx_result r = res[s];
+    uint16_t flags = r.flags & ~(HCX_RES_VALID | HCX_RES_REPORTED | HCX_RES_SURROGATE | HCX_RES_AUDITED);
+    if (r.cpsms < min_cpsm || r.cpsms == 0) {
+        if (lane == 0) res[s].flags = flags;
+        return;
+    }
+    const int32_t hyp = (int32_t)((double)(r.hyperscore * 10.0f) + 0.5);
+    const float *y = Y + (size_t)s * ystride;
+    const double p_lo = ((double)y[0] * norm[3] + norm[0]) * hyp;
+    const double p_hi = ((double)y[1] * norm[4] + norm[1]) * hyp;
+    const uint32_t *H = hist + (size_t)s * hbins;
+    FitOut f;
+    const bool located = tailfit_window(H, hbins, r.cpsms, hyp, p_lo, p_hi, sx, &f);
+    if (!located) {
+        __syncwarp();
+        f = tailfit_wave(H, hbins, r.cpsms, hyp, sx);
+    }
+    if (lane == 0) {
+        atomicAdd(&cnt[(blockIdx.x % kSurSlots) * kSurStride + (located ? 0 : 1)], 1u);
+        r.maxhypscore = hyp;
+        r.mu = f.mu;
+        r.beta = f.beta;
+        r.minhypscore = f.stt1;
+        r.nexthypscore = f.end1;
+        double w_ = r.mu, b_ = r.beta;
+        w_ /= 1e6;
+        b_ /= 1e6;
+        const double lgs = (w_ * r.maxhypscore) + b_;
+        double e = pow(10.0, lgs);
+        e *= (double)r.cpsms;
+        r.e_x = e;
+        flags |= (uint16_t)f.flags | HCX_RES_VALID | (located ? HCX_RES_SURROGATE : 0);
+        if (e < expect_max) flags |= HCX_RES_REPORTED;
+        if (f.fragile || (force && s % force == 0)) list_fragile(flags, fix, (uint32_t)s);
+        r.flags = flags;
+        res[s] = r;
     }
 }
 
@@ -870,13 +1090,21 @@ int launch_candidates(hcx_ctx *ctx, const QueryArgs &a) {
     return HCX_SUCCESS;
 }
 
+// Wavefronts per workgroup of the fit kernels: each holds hbins doubles of
+// LDS, up to four per workgroup within the device's limit.
+static int fit_waves(const hcx_ctx *ctx, uint32_t hbins) {
+    const size_t per_wave = (size_t)hbins * sizeof(double);
+    const int w = (int)std::min<size_t>(kBT / 64, (size_t)ctx->max_lds / per_wave);
+    return std::max(1, w);
+}
+
 int launch_tailfit(hcx_ctx *ctx, hcx_result *res, const uint32_t *hist, uint32_t hbins, int32_t n,
                    uint32_t min_cpsm, double expect_max, uint32_t *fix, uint32_t force) {
     if (n <= 0) return HCX_SUCCESS;
-    const size_t sm = (size_t)(kBT / 64) * hbins * sizeof(double);
+    const int per = fit_waves(ctx, hbins);
+    const size_t sm = (size_t)per * hbins * sizeof(double);
     HCX_HIP(hipFuncSetAttribute((const void *)k_tailfit, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
-    const int per = kBT / 64;
-    k_tailfit<<<(n + per - 1) / per, kBT, sm, ctx->stream>>>(res, hist, hbins, n, min_cpsm, expect_max, fix,
+    k_tailfit<<<(n + per - 1) / per, per * 64, sm, ctx->stream>>>(res, hist, hbins, n, min_cpsm, expect_max, fix,
                                                              force);
     HCX_HIP(hipGetLastError());
     return HCX_SUCCESS;
@@ -886,11 +1114,11 @@ int launch_tailfit_audit(hcx_ctx *ctx, hcx_result *res, const uint32_t *hist, ui
                          uint32_t min_cpsm, double expect_max, uint32_t stride, double tol, uint32_t *cnt,
                          uint32_t *fix, uint32_t force) {
     if (n <= 0) return HCX_SUCCESS;
-    const size_t sm = (size_t)(kBT / 64) * hbins * sizeof(double);
+    const int per = fit_waves(ctx, hbins);
+    const size_t sm = (size_t)per * hbins * sizeof(double);
     HCX_HIP(hipFuncSetAttribute((const void *)k_tailfit_audit, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
-    const int per = kBT / 64;
     const int64_t m = ((int64_t)n + stride - 1) / stride;
-    k_tailfit_audit<<<(unsigned)((m + per - 1) / per), kBT, sm, ctx->stream>>>(res, hist, hbins, n, min_cpsm,
+    k_tailfit_audit<<<(unsigned)((m + per - 1) / per), per * 64, sm, ctx->stream>>>(res, hist, hbins, n, min_cpsm,
                                                                              expect_max, stride, tol, cnt, fix, force);
     HCX_HIP(hipGetLastError());
     return HCX_SUCCESS;
@@ -899,11 +1127,35 @@ int launch_tailfit_audit(hcx_ctx *ctx, hcx_result *res, const uint32_t *hist, ui
 int launch_tailfit_raw(hcx_ctx *ctx, const uint32_t *hist, uint32_t hbins, uint32_t n, const uint32_t *cpsms,
                        const int32_t *maxhyp, int32_t *out4, uint32_t *flags, uint32_t force) {
     if (n == 0) return HCX_SUCCESS;
-    const size_t sm = (size_t)(kBT / 64) * hbins * sizeof(double);
+    const uint32_t per = (uint32_t)fit_waves(ctx, hbins);
+    const size_t sm = (size_t)per * hbins * sizeof(double);
     HCX_HIP(hipFuncSetAttribute((const void *)k_tailfit_raw, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
-    const uint32_t per = kBT / 64;
-    k_tailfit_raw<<<(n + per - 1) / per, kBT, sm, ctx->stream>>>(hist, hbins, n, cpsms, maxhyp, out4, flags,
+    k_tailfit_raw<<<(n + per - 1) / per, per * 64, sm, ctx->stream>>>(hist, hbins, n, cpsms, maxhyp, out4, flags,
                                                                  force);
+    HCX_HIP(hipGetLastError());
+    return HCX_SUCCESS;
+}
+
+int launch_fit_windows(hcx_ctx *ctx, const hcx_result *res, const uint32_t *hist, uint32_t hbins, int32_t n,
+                       uint32_t min_cpsm, int32_t *win) {
+    if (n <= 0) return HCX_SUCCESS;
+    const int per = fit_waves(ctx, hbins);
+    const size_t sm = (size_t)per * hbins * sizeof(double);
+    HCX_HIP(hipFuncSetAttribute((const void *)k_fit_windows, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+    k_fit_windows<<<(n + per - 1) / per, per * 64, sm, ctx->stream>>>(res, hist, hbins, n, min_cpsm, win);
+    HCX_HIP(hipGetLastError());
+    return HCX_SUCCESS;
+}
+
+int launch_sur_fit(hcx_ctx *ctx, const float *Y, uint32_t ystride, const float *norm, hcx_result *res,
+                   const uint32_t *hist, uint32_t hbins, int32_t n, uint32_t min_cpsm, double expect_max,
+                   uint32_t *fix, uint32_t force, uint32_t *cnt) {
+    if (n <= 0) return HCX_SUCCESS;
+    const int per = fit_waves(ctx, hbins);
+    const size_t sm = (size_t)per * hbins * sizeof(double);
+    HCX_HIP(hipFuncSetAttribute((const void *)k_sur_fit, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+    k_sur_fit<<<(n + per - 1) / per, per * 64, sm, ctx->stream>>>(Y, ystride, norm, res, hist, hbins, n, min_cpsm,
+                                                                  expect_max, fix, force, cnt);
     HCX_HIP(hipGetLastError());
     return HCX_SUCCESS;
 }

@@ -116,7 +116,7 @@ class Tuning(C.Structure):
 class Audit(C.Structure):
     _fields_ = [
         ("audited", C.c_uint32), ("disagreed", C.c_uint32), ("max_rel_err", C.c_double),
-        ("fell_back", C.c_int32), ("pad_", C.c_int32),
+        ("fell_back", C.c_int32), ("located", C.c_uint32), ("plain_fit", C.c_uint32), ("pad_", C.c_uint32),
     ]
 
 

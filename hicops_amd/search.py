@@ -293,7 +293,7 @@ class Engine:
         a = N.Audit()
         N.check(N.hcx().hcx_surrogate_last_audit(self._h, C.byref(a)))
         return {"audited": a.audited, "disagreed": a.disagreed, "max_rel_err": a.max_rel_err,
-                "fell_back": bool(a.fell_back)}
+                "fell_back": bool(a.fell_back), "located": a.located, "plain_fit": a.plain_fit}
 
     def selftest_mfma(self, A: np.ndarray, B: np.ndarray) -> np.ndarray:
         A = np.ascontiguousarray(A, np.float32)

@@ -366,19 +366,27 @@ int  hcx_tailfit_batch(hcx_ctx *ctx, const uint32_t *hist, uint32_t hbins, uint3
 
 /* ---- learned tail-parameter surrogate (no reference counterpart) ----
  * A 128-128-128-3 MLP (BF16 MFMA, FP32 accumulate) maps a spectrum's score
- * histogram to the tail-fit slope, intercept and log10 s(maxhyp); it is
- * trained on pairs the plain fit (expeRT::ModelTailFit) produced.  With
- * opts.evalue = HCX_EVAL_SURROGATE, hcx_search predicts every e-value, runs
- * the plain fit on every audit_every-th spectrum (0: 64), reports the plain
- * values for those (HCX_RES_AUDITED), and falls back to the plain fit for the
- * whole batch when any audited e-value differs by more than audit_tol
- * (relative; 0: 1e-3).  hcx_surrogate_last_audit reports what happened.   */
+ * histogram (its log-survival function at 124 points below the top score)
+ * to the window of expeRT::ModelTailFit's line fit: the first bin at or
+ * below 22 % of the log-survival drop and the last at or above 87 %
+ * (expeRT.cpp:657-681).  It is trained on the windows the plain fit
+ * (expeRT::ModelTailFit) chose.  With opts.evalue = HCX_EVAL_SURROGATE,
+ * hcx_search tests the 64 bins around each predicted end for the exact
+ * threshold crossing; where both are found the line is fitted through that
+ * window (HCX_RES_SURROGATE), which gives the plain fit's ints; elsewhere
+ * the spectrum takes the plain fit.  The plain fit also runs on every
+ * audit_every-th spectrum (0: 64) and replaces the values there
+ * (HCX_RES_AUDITED); if any audited e-value differs by more than audit_tol
+ * (relative; 0: 1e-3) the whole batch falls back to the plain fit.
+ * hcx_surrogate_last_audit reports what happened.                         */
 typedef struct hcx_audit {
     uint32_t audited;       /* spectra with a plain fit in the sample          */
     uint32_t disagreed;     /* of those, relative e-value error > audit_tol     */
     double   max_rel_err;   /* largest relative error in the sample            */
     int32_t  fell_back;     /* 1: the batch got plain e-values throughout      */
-    int32_t  pad_;
+    uint32_t located;       /* spectra whose predicted window was located      */
+    uint32_t plain_fit;     /* spectra the prediction missed: plain fit        */
+    uint32_t pad_;
 } hcx_audit;
 /* number of float parameters (weights + biases) of the model */
 int  hcx_surrogate_param_count(void);

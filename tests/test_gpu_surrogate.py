@@ -71,54 +71,39 @@ def test_train_predict_audit_and_fallback():
     bs = eng.upload(q_test)
     plain = eng.query_spectrum(bs, kernel=N.KERNEL_BATCHED)
     valid = (plain.results["flags"] & N.RES_VALID) != 0
-    ok = valid & ((plain.results["flags"] & N.RES_FIT_UB) == 0)
     log(f"test_train_predict_audit_and_fallback: {int(valid.sum())} valid test spectra")
-    assert ok.sum() > 300 and ((plain_train.results["flags"] & N.RES_VALID) != 0).sum() > 600
+    assert valid.sum() > 300 and ((plain_train.results["flags"] & N.RES_VALID) != 0).sum() > 600
 
-    def held_out_error():
+    def held_out():
         out = eng.query_spectrum(bs, kernel=N.KERNEL_BATCHED, evalue=N.EVAL_SURROGATE,
                                  audit_every=1 << 30, audit_tol=1e300)
         r = out.results
-        # the top hits never depend on the e-value method
-        for f in ("psid", "idxoffset", "sharedions", "cpsms", "hyperscore"):
-            assert np.array_equal(r[f], plain.results[f]), f
+        # the top hits never depend on the e-value method, and located or
+        # not, every tail fit equals the plain fit's ints and e-value
+        for f in ("psid", "idxoffset", "sharedions", "cpsms", "hyperscore", "mu", "beta", "minhypscore",
+                  "nexthypscore", "maxhypscore"):
+            assert np.array_equal(r[f][valid], plain.results[f][valid]), f
+        assert np.array_equal(r["e_x"][valid], plain.results["e_x"][valid])
         assert np.array_equal(out.hist, plain.hist)
-        sel = ok.copy()
-        sel[0] = False                               # spectrum 0 is the audit sample
-        assert np.all((r["flags"][sel] & N.RES_SURROGATE) != 0)
-        d = np.abs(np.log10(r["e_x"][sel]) - np.log10(plain.results["e_x"][sel]))
-        return float(np.median(d)), float(np.mean(d))
+        a = eng.last_audit()
+        sur = (r["flags"] & N.RES_SURROGATE) != 0
+        # (the audited spectrum carries the plain fit's flags)
+        assert 0 <= a["located"] - int(sur.sum()) <= a["audited"]
+        assert a["located"] + a["plain_fit"] == int(valid.sum())
+        return a["located"] / max(1, a["located"] + a["plain_fit"])
 
-    # what training gains: the same network untrained, and the best predictor
-    # without learning -- the training set's median tail slope and intercept
-    # for every spectrum, e = 10^(mu/1e6 * hyp + beta/1e6) * cpsms
+    # what training gains: the share of spectra whose fit window the
+    # network's prediction locates (the rest take the plain fit), untrained
+    # against trained
     eng.surrogate_init(seed=3)
-    med0, _ = held_out_error()
-    rt = plain_train.results
-    okt = ((rt["flags"] & N.RES_VALID) != 0) & ((rt["flags"] & N.RES_FIT_UB) == 0)
-    mu_c, beta_c = np.median(rt["mu"][okt]) / 1e6, np.median(rt["beta"][okt]) / 1e6
-    rp = plain.results
-    sel = ok.copy()
-    sel[0] = False
-    log_e_const = mu_c * rp["maxhypscore"][sel] + beta_c + np.log10(rp["cpsms"][sel].astype(np.float64))
-    med_const = float(np.median(np.abs(log_e_const - np.log10(rp["e_x"][sel]))))
-
+    rate0 = held_out()
     loss = eng.surrogate_train(bt, steps=3000, batch_size=256, lr=2e-3)
-    med, mean = held_out_error()
-    log(f"  trained: final loss {loss:.4f}, held-out |dlog10 e| median {med:.4f} mean {mean:.4f}; "
-        f"untrained {med0:.4f}, constant tail parameters {med_const:.4f}")
-    # measured: untrained 1.7 (its standardised outputs sit near the training
-    # means, i.e. the constant predictor), constant parameters 1.7, trained
-    # 0.25-0.35: training cuts the error about five-fold
-    assert med < 0.5 * med_const and med < 0.5 * med0
-    # measured: loss ~0.5 (standardised).  The fit extrapolates its line from
-    # the bulk of the score distribution to the top score, so a slope error is
-    # amplified by that distance; the learned model gets e within a small
-    # factor, not within 1e-3 (hence the audit)
-    assert loss < 0.7 and med < 0.45
+    rate = held_out()
+    log(f"  trained: final loss {loss:.4f}; windows located on held-out spectra {rate:.3f} (untrained {rate0:.3f})")
+    assert rate > 0.8 and rate > rate0 + 0.3
 
-    # audit sample every 8th spectrum at 1e-3: audited spectra report the plain
-    # fit; any disagreement sends the whole batch back to the plain fit
+    # audit sample every 8th spectrum at 1e-3: the audited spectra carry the
+    # plain fit, and nothing disagrees
     out = eng.query_spectrum(bs, kernel=N.KERNEL_BATCHED, evalue=N.EVAL_SURROGATE, audit_every=8, audit_tol=1e-3)
     a = eng.last_audit()
     log(f"  audit: {a}")
@@ -126,20 +111,9 @@ def test_train_predict_audit_and_fallback():
     audited = (r["flags"] & N.RES_AUDITED) != 0
     assert a["audited"] == int(audited.sum()) and a["audited"] > 0
     assert np.all(np.arange(len(r))[audited] % 8 == 0)
+    assert a["disagreed"] == 0 and not a["fell_back"] and a["max_rel_err"] == 0.0
     e_p = plain.results["e_x"]
-    assert np.all(np.abs(r["e_x"][audited] - e_p[audited]) <= 1e-12 * np.abs(e_p[audited]))
-    if a["fell_back"]:
-        assert not np.any((r["flags"] & N.RES_SURROGATE) != 0)
-        assert np.all(np.abs(r["e_x"][valid] - e_p[valid]) <= 1e-12 * np.abs(e_p[valid]))
-        for f in ("mu", "beta", "minhypscore", "nexthypscore"):
-            assert np.array_equal(r[f][valid], plain.results[f][valid]), f
-    else:
-        assert a["disagreed"] == 0
-        assert np.all((r["flags"][valid & ~audited] & N.RES_SURROGATE) != 0)
-    # with no tolerance at all the predictions stay (outliers reach e x 1e9)
-    out = eng.query_spectrum(bs, kernel=N.KERNEL_BATCHED, evalue=N.EVAL_SURROGATE, audit_every=8, audit_tol=1e300)
-    a = eng.last_audit()
-    assert not a["fell_back"] and ((out.results["flags"] & N.RES_SURROGATE) != 0).sum() > 0
+    assert np.array_equal(r["e_x"][valid], e_p[valid])
 
     # parameters round-trip through the host
     params, nrm = eng.surrogate_params()
@@ -148,4 +122,56 @@ def test_train_predict_audit_and_fallback():
     p2, n2 = eng2.surrogate_params()
     assert np.array_equal(p2, params) and np.array_equal(n2, nrm)
     eng2.close()
+    eng.close()
+
+
+@pytest.mark.timeout(900)
+def test_surrogate_config3_500k_pairs():
+    # trained on 500,000 (histogram, plain fit) pairs from a search of the
+    # open-search workload (10M peptides, +-500 Da) with spectra of their own
+    # seed, evaluated on 50,000 held-out spectra of another seed: the
+    # e-value error against the plain fit, and the share of spectra whose
+    # window the prediction missed (they take the plain fit)
+    p = W.PRESETS["open10m"]
+    db = W.Database(p)
+    prm = W.make_params(p)
+    eng = Engine(prm)
+    eng.construct_db(db)
+    qt = W.make_queries(db, p, nspec=500_000, seed=p.seed ^ 0x3A11)
+    qh = W.make_queries(db, p, nspec=50_000, seed=p.seed ^ 0x3B22)
+    log(f"test_surrogate_config3_500k_pairs: {db.nforms} forms, {qt.nspec} training and {qh.nspec} held-out spectra")
+    bt = eng.upload(qt)
+    eng.query_device(bt, kernel=N.KERNEL_BATCHED)
+    bh = eng.upload(qh)
+    plain = eng.query_spectrum(bh, kernel=N.KERNEL_BATCHED)
+    k, t = eng.query_device(bh, kernel=N.KERNEL_BATCHED)
+    plain_ms = eng.last_timing()["evalue_ms"]
+    valid = (plain.results["flags"] & N.RES_VALID) != 0
+
+    def evaluate():
+        out = eng.query_spectrum(bh, kernel=N.KERNEL_BATCHED, evalue=N.EVAL_SURROGATE, audit_every=64,
+                                 audit_tol=1e-3)
+        a = eng.last_audit()
+        e_s, e_p = out.results["e_x"][valid], plain.results["e_x"][valid]
+        rel = np.abs(e_s - e_p) / np.abs(e_p)
+        miss = a["plain_fit"] / max(1, a["located"] + a["plain_fit"])
+        return out, a, rel, miss
+
+    eng.surrogate_init(seed=3)
+    _, a0, _, miss0 = evaluate()
+    loss = eng.surrogate_train(bt, steps=6000, batch_size=256, lr=2e-3)
+    bt.free()
+    out, a, rel, miss = evaluate()
+    k, t = eng.query_device(bh, kernel=N.KERNEL_BATCHED, evalue=N.EVAL_SURROGATE, audit_every=64, audit_tol=1e-3)
+    sur_ms = eng.last_timing()["evalue_ms"]
+    q = np.quantile(rel, [0.5, 0.95, 0.99, 1.0])
+    log(f"  {int(valid.sum())} valid held-out spectra; loss {loss:.4f}; relative e-value error vs the plain fit: "
+        f"median {q[0]:.3g}, 95% {q[1]:.3g}, 99% {q[2]:.3g}, max {q[3]:.3g}; within 1e-3: {(rel <= 1e-3).mean():.4f}")
+    log(f"  prediction missed (plain fit): {miss:.4f} trained, {miss0:.4f} untrained; audit {a}")
+    log(f"  e-value stage: surrogate {sur_ms:.3f} ms, plain fit {plain_ms:.3f} ms for {qh.nspec} spectra")
+    assert (rel <= 1e-3).mean() >= 0.95
+    assert a["disagreed"] == 0 and not a["fell_back"]
+    assert miss < 0.1 and miss < miss0
+    for f in ("psid", "mu", "beta", "minhypscore", "nexthypscore"):
+        assert np.array_equal(out.results[f][valid], plain.results[f][valid]), f
     eng.close()
