@@ -139,6 +139,107 @@ void host_tailfit(const uint32_t *H, uint32_t hbins, uint32_t cpsms, int32_t hyp
     *flags = fl;
 }
 
+double host_tailfit_merged(const double *pd, uint32_t hbins, int32_t pN, int32_t hyp, double *sx) {
+    if (hyp < 1 || hyp >= (int)hbins) return INFINITY;
+    int end1 = hyp - 1;
+    for (int p = hyp - 1; p >= 0; p--)
+        if (pd[p] >= 1.0) {
+            end1 = p;
+            break;
+        }
+    int stt1 = 0;
+    for (int p = 0; p <= end1; p++)
+        if (pd[p] >= 1.0) {
+            stt1 = p;
+            break;
+        }
+    const int n = end1 - stt1 + 1;
+    const int vaa = pN;
+    double mu_t = 0.0, beta_t = 4.0;
+    if (vaa < 1) {
+        mu_t = 0.0;
+        beta_t = 100.0;
+    } else {
+        int acc = 1;
+        for (int i = 0; i < n; i++) acc = (int)((double)acc + pd[stt1 + i]);
+        const int vaa2 = acc > vaa ? acc : vaa;
+        double run = 0.0;
+        for (int i = 0; i < n; i++) {
+            const double q = pd[stt1 + i] / (double)vaa2;
+            run = (i == 0) ? q : run + q;
+            const double t = run + (-1.0);
+            sx[i] = -t;
+        }
+        for (int i = 0; i < n; i++)
+            if (sx[i] > 1.0) sx[i] = 0.999;
+        int rep = n - 1;
+        for (int p = n - 1; p >= 0; p--)
+            if (sx[p] >= 1e-4) {
+                rep = p;
+                break;
+            }
+        const double rv = sx[rep];
+        for (int i = 0; i < n; i++) {
+            double v = sx[i];
+            if (v <= 0.0) v = rv;
+            sx[i] = std::log10(v);
+        }
+        const double s0 = sx[0];
+        const double hgt = sx[n - 1] - s0;
+        auto largmax = [&](double v) {
+            for (int i = 0; i < n; i++)
+                if (sx[i] <= v) return i;
+            return 0;
+        };
+        auto rargmax = [&](double v) {
+            for (int p = n - 1; p >= 0; p--)
+                if (sx[p] >= v) return p;
+            return n - 1;
+        };
+        int mark = 0, mark2 = 0;
+        if (n > 3) {
+            mark = largmax(s0 + hgt * 0.22) - 1;
+            mark2 = rargmax(s0 + hgt * 0.87);
+            if (mark2 == n) mark2 -= 1;
+            if (mark >= mark2) mark = mark2 - 1;
+        } else if (n == 3) {
+            mark = largmax(s0 + hgt * 0.22) - 1;
+            mark2 = n - 1;
+            if (mark >= mark2) mark = mark2 - 1;
+        } else {
+            mark = 0;
+            mark2 = n - 1;
+        }
+        if (mark < 0) {
+            mu_t = 0.0;
+            beta_t = 0.0;
+        } else {
+            const int m = mark2 - mark + 1;
+            const double *y = sx + mark;
+            if (m == 1) {
+                mu_t = 0.0;
+                beta_t = y[0];
+            } else {
+                double xbar = 0.0, ybar = 0.0, top = 0.0, bot = 0.0;
+                for (int i = 0; i < m; i++) {
+                    xbar = xbar + (double)(stt1 + mark + i);
+                    ybar = ybar + y[i];
+                }
+                xbar = xbar / (double)m;
+                ybar = ybar / (double)m;
+                for (int i = 0; i < m; i++) {
+                    const double dx = (double)(stt1 + mark + i) - xbar;
+                    top = top + dx * (y[i] - ybar);
+                    bot = bot + dx * dx;
+                }
+                mu_t = top / bot;
+                beta_t = ybar - mu_t * xbar;
+            }
+        }
+    }
+    return std::pow(10, hyp * mu_t + beta_t) * vaa;
+}
+
 int fix_host_fits(hcx_ctx *c, const uint32_t *fix, hcx_result *res, const uint32_t *hist, uint32_t hbins,
                   double expect_max) {
     if (!c->fix_host || !c->fix_dev.p || hbins > c->hbins) return set_error("host fit: staging not set up");

@@ -1320,6 +1320,116 @@ extern "C" int hcx_tailfit_batch(hcx_ctx *c, const uint32_t *hist, uint32_t hbin
     return HCX_SUCCESS;
 }
 
+extern "C" int hcx_part_export(hcx_ctx *c, hcx_qbatch *qb, int32_t qid0, uint32_t slot, hcx_part_res *packs,
+                               uint16_t *slices) {
+    if (!c || !qb || (qb->nspec > 0 && (!packs || (slot && !slices))))
+        return set_error_code(HCX_ERR_INVLD_PTR, "hcx_part_export: null argument");
+    if (slot == 0 || slot > 65536) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_part_export: slot in 1..65536");
+    const int32_t n = qb->nspec;
+    if (n == 0) return HCX_SUCCESS;
+    if (!qb->res.p || !qb->hist.p) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_part_export: batch not searched");
+    HCX_HIP(hipSetDevice(c->device));
+    hcx::DeviceBuf dp, ds, dov;
+    if (dp.ensure(sizeof(hcx_part_res) * (size_t)n) || ds.ensure(2ull * slot * (size_t)n) || dov.ensure(4))
+        return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_part_export: alloc");
+    int rc = launch_fill_u32(c, (uint32_t *)dov.p, 1, 0);
+    if (!rc)
+        rc = launch_part_export(c, (const hcx_result *)qb->res.p, (const uint32_t *)qb->hist.p, c->hbins, n, qid0,
+                                slot, (hcx_part_res *)dp.p, (uint16_t *)ds.p, (uint32_t *)dov.p);
+    if (rc) return rc;
+    uint32_t ov = 0;
+    HCX_HIP(hipMemcpyAsync(&ov, dov.p, 4, hipMemcpyDeviceToHost, c->stream));
+    HCX_HIP(hipMemcpyAsync(packs, dp.p, sizeof(hcx_part_res) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    HCX_HIP(hipMemcpyAsync(slices, ds.p, 2ull * slot * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    HCX_HIP(hipStreamSynchronize(c->stream));
+    if (ov) return set_error_code(HCX_ERR_INVLD_SIZE, "hcx_part_export: a histogram's occupied range exceeds slot");
+    return HCX_SUCCESS;
+}
+
+extern "C" int hcx_combine(hcx_ctx *c, uint32_t nshards, uint32_t nspec, uint32_t slot,
+                           const hcx_part_res *const *packs, const uint16_t *const *slices, hcx_fresult *out,
+                           int32_t *keys) {
+    if (!c || (nspec && (!packs || !slices || !out || !keys)))
+        return set_error_code(HCX_ERR_INVLD_PTR, "hcx_combine: null argument");
+    if (nshards == 0 || nshards > 4096 || slot == 0 || slot > 65536 || nspec > 0x7FFFFFFFu)
+        return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_combine: 1..4096 shards, slot in 1..65536");
+    if (nspec == 0) return HCX_SUCCESS;
+    for (uint32_t g = 0; g < nshards; g++) {
+        if (!packs[g] || !slices[g]) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_combine: null shard");
+        for (uint32_t s = 0; s < nspec; s++) {
+            const hcx_part_res &r = packs[g][s];
+            if (r.N >= 1 && (r.max2 < r.min || r.max2 >= c->hbins || (uint32_t)(r.max2 - r.min) >= slot))
+                return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_combine: a slice outside the histogram or its slot");
+        }
+    }
+    HCX_HIP(hipSetDevice(c->device));
+    const size_t n = nspec;
+    hcx::DeviceBuf dp, ds, dout, dkeys, dfix;
+    if (dp.ensure(sizeof(hcx_part_res) * n * nshards) || ds.ensure(2ull * slot * n * nshards) ||
+        dout.ensure(sizeof(hcx_fresult) * n) || dkeys.ensure(4 * n) || dfix.ensure(4 * (n + 1)))
+        return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_combine: alloc");
+    for (uint32_t g = 0; g < nshards; g++) {
+        HCX_HIP(hipMemcpyAsync((hcx_part_res *)dp.p + g * n, packs[g], sizeof(hcx_part_res) * n,
+                               hipMemcpyHostToDevice, c->stream));
+        HCX_HIP(hipMemcpyAsync((uint16_t *)ds.p + g * n * slot, slices[g], 2ull * slot * n, hipMemcpyHostToDevice,
+                               c->stream));
+    }
+    int rc = launch_fill_u32(c, (uint32_t *)dfix.p, 1, 0);
+    if (!rc)
+        rc = launch_combine(c, (const hcx_part_res *)dp.p, (const uint16_t *)ds.p, nshards, (int32_t)nspec, slot,
+                            (hcx_fresult *)dout.p, (int32_t *)dkeys.p, (uint32_t *)dfix.p);
+    if (rc) return rc;
+    uint32_t nfix = 0;
+    HCX_HIP(hipMemcpyAsync(out, dout.p, sizeof(hcx_fresult) * n, hipMemcpyDeviceToHost, c->stream));
+    HCX_HIP(hipMemcpyAsync(keys, dkeys.p, 4 * n, hipMemcpyDeviceToHost, c->stream));
+    HCX_HIP(hipMemcpyAsync(&nfix, dfix.p, 4, hipMemcpyDeviceToHost, c->stream));
+    HCX_HIP(hipStreamSynchronize(c->stream));
+    if (nfix == 0) return HCX_SUCCESS;
+    // fits too close to a boundary for the device's log10: the host merges
+    // and fits those spectra again from the same inputs
+    std::vector<uint32_t> ids(nfix);
+    HCX_HIP(hipMemcpy(ids.data(), (uint32_t *)dfix.p + 1, 4ull * nfix, hipMemcpyDeviceToHost));
+    std::vector<double> pd(c->hbins), sx(c->hbins);
+    for (uint32_t s : ids) {
+        std::fill(pd.begin(), pd.end(), 0.0);
+        int32_t cpsms = 0, pN = 0, key = (int32_t)nshards;
+        float maxhyp = -1.0f;
+        for (uint32_t g = 0; g < nshards; g++) {
+            const hcx_part_res &r = packs[g][s];
+            if (r.min == 0 && r.max == 0.0f && r.N == 0 && r.max2 == 0) continue;
+            cpsms += r.N;
+            if (r.N >= 1) {
+                pN += r.N;
+                const uint16_t *buf = slices[g] + (size_t)s * slot;
+                for (int jj = r.min; jj <= (int)r.max2; jj++) {
+                    double v = buf[jj - r.min];
+                    if (r.N > 65500) v = (v / 65500) * r.N;
+                    pd[jj] = pd[jj] + v;
+                }
+                if (r.max > 0 && r.max > maxhyp) {
+                    maxhyp = r.max;
+                    key = (int32_t)g;
+                }
+            }
+        }
+        const int32_t hyp = (int32_t)((double)(maxhyp * 10.0f) + 0.5);
+        const double e = hcx::host_tailfit_merged(pd.data(), c->hbins, pN, hyp, sx.data());
+        // (int) as x86-64 converts: out of range gives INT_MIN
+        auto i32_of = [](double v) { return (v > -2147483649.0 && v < 2147483648.0) ? (int32_t)v : INT32_MIN; };
+        hcx_fresult f = {i32_of(c->params.expect_max * 1e6), -1, 0};
+        int32_t k = (int32_t)nshards;
+        if (e < c->params.expect_max) {
+            f.eValue = i32_of(e * 1e6);
+            f.specID = packs[key][s].qID;
+            f.npsms = cpsms;
+            k = key;
+        }
+        out[s] = f;
+        keys[s] = k;
+    }
+    return HCX_SUCCESS;
+}
+
 // v_log_f32 of 1 .. n on the device (the packed kernel's score estimate).
 extern "C" int hcx_selftest_log2(hcx_ctx *c, uint32_t n, float *out, double *bound) {
     if (!c || (n && !out)) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_selftest_log2: null argument");

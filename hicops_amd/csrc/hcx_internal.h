@@ -292,6 +292,13 @@ int launch_fit_windows(hcx_ctx *ctx, const hcx_result *res, const uint32_t *hist
 int launch_sur_fit(hcx_ctx *ctx, const float *Y, uint32_t ystride, const float *norm, hcx_result *res,
                    const uint32_t *hist, uint32_t hbins, int32_t n, uint32_t min_cpsm, double expect_max,
                    uint32_t *fix, uint32_t force, uint32_t *cnt);
+// shard partial results (one wave per spectrum; *overflow set when a
+// histogram's occupied range exceeds slot) and their merge (fix: the host
+// re-fit list, count then ids)
+int launch_part_export(hcx_ctx *ctx, const hcx_result *res, const uint32_t *hist, uint32_t hbins, int32_t n,
+                       int32_t qid0, uint32_t slot, hcx_part_res *packs, uint16_t *slices, uint32_t *overflow);
+int launch_combine(hcx_ctx *ctx, const hcx_part_res *packs, const uint16_t *slices, uint32_t G, int32_t n,
+                   uint32_t slot, hcx_fresult *out, int32_t *keys, uint32_t *fix);
 // ---- host tail fit (fit_host.hip) ----
 // expeRT::ModelTailFit on the host in FP64 with the host libm's log10, for
 // the spectra the device lists as fragile.  sx: hbins doubles of workspace;
@@ -303,6 +310,9 @@ void host_tailfit(const uint32_t *H, uint32_t hbins, uint32_t cpsms, int32_t hyp
 inline size_t fix_host_bytes(uint32_t hbins) {
     return 256 + (size_t)kFixRows * hbins * 4 + (size_t)kFixRows * sizeof(hcx_result) + (size_t)hbins * 8;
 }
+// expeRT::ModelTailFit(double&, int) on a merged histogram, on the host
+// (sx: hbins doubles of workspace); returns the e-value.
+double host_tailfit_merged(const double *pd, uint32_t hbins, int32_t pN, int32_t hyp, double *sx);
 // Fits the spectra of the device list `fix` again on the host and writes
 // their results back (kFixRows at a time through the context's staging
 // buffers); ends with the stream synchronised.

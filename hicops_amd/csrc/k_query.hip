@@ -610,11 +610,12 @@ __device__ double fit_survival(const uint32_t *H, int stt1, int n, uint32_t cpsm
 // stt1 + mark + i (expeRT.cpp:686-704), with the bound of how far slope and
 // intercept move under another log10 within kFitYErr; fragile when the ints
 // (values x 1e6 truncated) could then differ.
-__device__ void fit_line(const double *y, int m, int x0, double &mu_t, double &beta_t, bool &fragile,
-                         uint32_t &flags) {
+__device__ void line_fit(const double *y, int m, int x0, double &mu_t, double &beta_t, double &e_mu,
+                         double &e_beta) {
     // e_mu, e_beta: first order through the sums, plus one rounding
     // difference per operation
-    double e_mu = 0.0, e_beta = kFitYErr;
+    e_mu = 0.0;
+    e_beta = kFitYErr;
     if (m == 1) {
         mu_t = 0.0;
         beta_t = y[0];
@@ -639,6 +640,14 @@ __device__ void fit_line(const double *y, int m, int x0, double &mu_t, double &b
         e_beta = kFitYErr + (double)m * DBL_EPSILON * fabs(ybar) + e_mu * fabs(xbar) +
                  2.0 * DBL_EPSILON * (fabs(mu_t * xbar) + fabs(beta_t));
     }
+}
+
+// The one-rank fit's line: slope and intercept become ints x 1e6, so the
+// fit is fragile when either lies within its bound of an integer there.
+__device__ void fit_line(const double *y, int m, int x0, double &mu_t, double &beta_t, bool &fragile,
+                         uint32_t &flags) {
+    double e_mu, e_beta;
+    line_fit(y, m, x0, mu_t, beta_t, e_mu, e_beta);
     // margin 4x; a single point's slope is an exact 0
     if ((m > 1 && near_int(mu_t * 1e6, 4e6 * e_mu)) || near_int(beta_t * 1e6, 4e6 * e_beta) ||
         fabs(fabs(mu_t) - 2.0e3) <= 4.0 * e_mu || fabs(fabs(beta_t) - 2.0e3) <= 4.0 * e_beta)
@@ -1037,6 +1046,271 @@ __global__ __launch_bounds__(kBT) void k_sur_fit(const float *Y, uint32_t ystrid
     }
 }
 
+// ---- shard partial results and their merge (dslim_query.cpp:733-763,
+// expeRT.cpp:720-936, dslim_score.cpp:244-345) ----
+
+// (ushort) of a double as the reference's x86-64 code converts it: a 32-bit
+// truncation, then the low 16 bits
+__device__ __forceinline__ uint16_t u16_of(double v) { return (uint16_t)(uint32_t)(int32_t)v; }
+// (int) of a double as x86-64's cvttsd2si converts it: out of range (or
+// NaN) gives INT_MIN, where the device's conversion would saturate
+__device__ __forceinline__ int32_t i32_of(double v) {
+    return (v > -2147483649.0 && v < 2147483648.0) ? (int32_t)v : INT32_MIN;
+}
+
+// One wave per spectrum: the partRes and the u16 counts of the occupied
+// histogram range (expeRT::StoreIResults), the rest of the slot zero.
+__global__ __launch_bounds__(kBT) void k_part_export(const hcx_result *res, const uint32_t *hist, uint32_t hbins,
+                                                     int32_t n, int32_t qid0, uint32_t slot, hcx_part_res *packs,
+                                                     uint16_t *slices, uint32_t *overflow) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int32_t s = blockIdx.x * (int32_t)(blockDim.x >> 6) + w;
+    if (s >= n) return;
+    const hcx_result r = res[s];
+    const int32_t cpsms = (int32_t)r.cpsms;
+    const uint32_t *yy = hist + (size_t)s * hbins;
+    hcx_part_res pr;
+    pr.min = 0;
+    pr.max2 = 0;
+    pr.max = 0.0f;
+    pr.N = 0;
+    pr.qID = qid0 + s;
+    int stt = 0, ends = -1;
+    if (cpsms >= 1) {
+        // ends = rargmax(yy, 0, hbins-1, 0.99), stt = argmax(yy, 0, ends, 0.99)
+        ends = (int)hbins - 1;
+        for (int top = (int)hbins - 1; top >= 0; top -= 64) {
+            const int p = top - lane;
+            const unsigned long long m = __ballot(p >= 0 && yy[p] >= 1u);
+            if (m) {
+                ends = top - (__ffsll((long long)m) - 1);
+                break;
+            }
+        }
+        for (int b = 0; b <= ends; b += 64) {
+            const int p = b + lane;
+            const unsigned long long m = __ballot(p <= ends && yy[p] >= 1u);
+            if (m) {
+                stt = b + (__ffsll((long long)m) - 1);
+                break;
+            }
+        }
+        if ((uint32_t)(ends - stt + 1) > slot) {
+            if (lane == 0) atomicOr(overflow, 1u);
+            ends = -1;
+        } else {
+            pr.min = (uint16_t)stt;
+            pr.max2 = (uint16_t)ends;
+            pr.max = r.hyperscore;
+            pr.N = cpsms;
+        }
+    }
+    uint16_t *out = slices + (size_t)s * slot;
+    for (uint32_t i = lane; i < slot; i += 64) {
+        uint16_t k = 0;
+        if ((int)i <= ends - stt) {
+            k = (uint16_t)yy[stt + i];
+            if (cpsms > 65500) k = u16_of((double)(int32_t)((uint32_t)k * 65500u) / (double)cpsms);
+        }
+        out[i] = k;
+    }
+    if (lane == 0) packs[s] = pr;
+}
+
+// expeRT::ModelTailFit(double&, int) on the merged histogram pd (LDS, this
+// wave's hbins doubles, overwritten): returns the e-value (lane 0's is
+// valid); *fragile when the window or the e-value's truncation x 1e6 or
+// its comparison with expect_max lies within the device log10's error
+// bound of a boundary.
+__device__ double tailfit_merged_wave(double *pd, uint32_t hbins, int32_t pN, int32_t hyp, double expect_max,
+                                      bool *fragile, uint32_t *flags) {
+    const int lane = threadIdx.x & 63;
+    *fragile = false;
+    if (hyp < 1 || hyp >= (int)hbins) {
+        *flags |= HCX_RES_FIT_UB;
+        return INFINITY;
+    }
+    int end1 = hyp - 1;   // rargmax(pd, 0, hyp-1, 1.0)
+    for (int top = hyp - 1; top >= 0; top -= 64) {
+        const int p = top - lane;
+        const unsigned long long m = __ballot(p >= 0 && pd[p] >= 1.0);
+        if (m) {
+            end1 = top - (__ffsll((long long)m) - 1);
+            break;
+        }
+    }
+    int stt1 = 0;         // argmax(pd, 0, end1, 1.0)
+    for (int b = 0; b <= end1; b += 64) {
+        const int p = b + lane;
+        const unsigned long long m = __ballot(p <= end1 && pd[p] >= 1.0);
+        if (m) {
+            stt1 = b + (__ffsll((long long)m) - 1);
+            break;
+        }
+    }
+    const int n = end1 - stt1 + 1;
+    const int vaa = pN;
+    double mu_t = 0.0, beta_t = 4.0, e_mu = 0.0, e_beta = 0.0;
+    bool frag = false;
+    if (vaa < 1) {
+        mu_t = 0.0;
+        beta_t = 100.0;
+    } else {
+        double *sx = pd + stt1;
+        if (lane == 0) {
+            // std::accumulate(range, 1) into an int; then divide and
+            // partial_sum in order (one lane: the order fixes the rounding)
+            int acc = 1;
+            for (int i = 0; i < n; i++) acc = (int)((double)acc + sx[i]);
+            const int vaa2 = acc > vaa ? acc : vaa;
+            double run = 0.0;
+            for (int i = 0; i < n; i++) {
+                const double q = sx[i] / (double)vaa2;
+                run = (i == 0) ? q : run + q;
+                const double t = run + (-1.0);
+                sx[i] = -t;
+            }
+        }
+        __syncwarp();
+        for (int i = lane; i < n; i += 64)
+            if (sx[i] > 1.0) sx[i] = 0.999;
+        __syncwarp();
+        int rep = n - 1;
+        for (int top = n - 1; top >= 0; top -= 64) {
+            const int p = top - lane;
+            const unsigned long long m = __ballot(p >= 0 && sx[p] >= 1e-4);
+            if (m) {
+                rep = top - (__ffsll((long long)m) - 1);
+                break;
+            }
+        }
+        const double rv = sx[rep];
+        __syncwarp();
+        for (int i = lane; i < n; i += 64) {
+            double v = sx[i];
+            if (v <= 0.0) v = rv;
+            sx[i] = log10(v);
+        }
+        __syncwarp();
+        const double s0 = sx[0];
+        const double hgt = sx[n - 1] - s0;
+        auto largmax = [&](double v) {
+            for (int b = 0; b < n; b += 64) {
+                const int i = b + lane;
+                const unsigned long long m = __ballot(i < n && sx[i] <= v);
+                if (m) return b + (__ffsll((long long)m) - 1);
+            }
+            return 0;
+        };
+        auto rargmax = [&](double v) {
+            for (int top = n - 1; top >= 0; top -= 64) {
+                const int p = top - lane;
+                const unsigned long long m = __ballot(p >= 0 && sx[p] >= v);
+                if (m) return top - (__ffsll((long long)m) - 1);
+            }
+            return n - 1;
+        };
+        auto near_thr = [&](double v) {
+            for (int b = 0; b < n; b += 64) {
+                const int i = b + lane;
+                if (__ballot(i < n && fabs(sx[i] - v) <= kFitWinEps)) return true;
+            }
+            return false;
+        };
+        if (n >= 3) frag = near_thr(s0 + hgt * 0.22) || (n > 3 && near_thr(s0 + hgt * 0.87));
+        int mark = 0, mark2 = 0;
+        if (n > 3) {
+            mark = largmax(s0 + hgt * 0.22) - 1;
+            mark2 = rargmax(s0 + hgt * 0.87);
+            if (mark2 == n) mark2 -= 1;
+            if (mark >= mark2) mark = mark2 - 1;
+        } else if (n == 3) {
+            mark = largmax(s0 + hgt * 0.22) - 1;
+            mark2 = n - 1;
+            if (mark >= mark2) mark = mark2 - 1;
+        } else {
+            mark = 0;
+            mark2 = n - 1;
+        }
+        if (mark < 0) {
+            *flags |= HCX_RES_FIT_UB;   // the reference reads sx[-1]: a zero line
+            mu_t = 0.0;
+            beta_t = 0.0;
+        } else if (lane == 0) {
+            line_fit(sx + mark, mark2 - mark + 1, stt1 + mark, mu_t, beta_t, e_mu, e_beta);
+        }
+    }
+    const double e = pow(10, hyp * mu_t + beta_t) * vaa;
+    if (lane == 0) {
+        // relative bound on e under another log10 and pow (a few ulp each)
+        const double eb = 2.302585092994046 * (fabs((double)hyp) * e_mu + e_beta) + 8.0 * DBL_EPSILON;
+        if (near_int(e * 1e6, 4.0 * eb * e * 1e6) || fabs(e - expect_max) <= 4.0 * eb * e) frag = true;
+    }
+    *fragile = frag;
+    return e;
+}
+
+// One wave per spectrum: CombineResults over G shards' partial results
+// (packs [G][n], slices [G][n][slot]) -- histograms summed in shard order,
+// the key shard of the top hyperscore, the merged tail fit; spectra whose
+// fit is fragile are listed in fix (count, then ids) for the host.
+__global__ __launch_bounds__(kBT) void k_combine(const hcx_part_res *packs, const uint16_t *slices, uint32_t G,
+                                                 int32_t n, uint32_t slot, uint32_t hbins, uint32_t min_cpsm,
+                                                 double expect_max, hcx_fresult *out, int32_t *keys, uint32_t *fix) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int32_t s = blockIdx.x * (int32_t)(blockDim.x >> 6) + w;
+    if (s >= n) return;
+    double *pd = (double *)smem_raw + (size_t)w * hbins;
+    for (uint32_t i = lane; i < hbins; i += 64) pd[i] = 0.0;
+    __syncwarp();
+    int32_t cpsms = 0, pN = 0, key = (int32_t)G;
+    float maxhyp = -1.0f;
+    for (uint32_t g = 0; g < G; g++) {
+        const hcx_part_res r = packs[(size_t)g * n + s];
+        if (r.min == 0 && r.max == 0.0f && r.N == 0 && r.max2 == 0) continue;
+        cpsms += r.N;
+        if (r.N >= 1) {
+            pN += r.N;
+            const uint16_t *buf = slices + ((size_t)g * n + s) * slot;
+            for (int jj = r.min + lane; jj <= (int)r.max2; jj += 64) {
+                double v = buf[jj - r.min];
+                if (r.N > 65500) v = (v / 65500) * r.N;
+                pd[jj] = pd[jj] + v;
+            }
+            if (r.max > 0 && r.max > maxhyp) {
+                maxhyp = r.max;
+                key = (int32_t)g;
+            }
+        }
+    }
+    __syncwarp();
+    hcx_fresult f;
+    f.eValue = i32_of(expect_max * 1e6);
+    f.specID = -1;
+    f.npsms = 0;
+    int32_t k = (int32_t)G;
+    if (key < (int32_t)G && cpsms >= (int32_t)min_cpsm) {
+        const int32_t hyp = (int32_t)((double)(maxhyp * 10.0f) + 0.5);
+        bool fragile = false;
+        uint32_t fl = 0;
+        const double e = tailfit_merged_wave(pd, hbins, pN, hyp, expect_max, &fragile, &fl);
+        if (lane == 0) {
+            if (e < expect_max) {
+                f.eValue = i32_of(e * 1e6);
+                f.specID = packs[(size_t)key * n + s].qID;
+                f.npsms = cpsms;
+                k = key;
+            }
+            if (fragile && fix) fix[1 + atomicAdd(fix, 1u)] = (uint32_t)s;
+        }
+    }
+    if (lane == 0) {
+        out[s] = f;
+        keys[s] = k;
+    }
+}
+
 }  // namespace
 
 size_t query_smem_bytes(const QueryArgs &a, bool replay) { return smem_layout(a, replay, nullptr, nullptr); }
@@ -1156,6 +1430,29 @@ int launch_sur_fit(hcx_ctx *ctx, const float *Y, uint32_t ystride, const float *
     HCX_HIP(hipFuncSetAttribute((const void *)k_sur_fit, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
     k_sur_fit<<<(n + per - 1) / per, per * 64, sm, ctx->stream>>>(Y, ystride, norm, res, hist, hbins, n, min_cpsm,
                                                                   expect_max, fix, force, cnt);
+    HCX_HIP(hipGetLastError());
+    return HCX_SUCCESS;
+}
+
+int launch_part_export(hcx_ctx *ctx, const hcx_result *res, const uint32_t *hist, uint32_t hbins, int32_t n,
+                       int32_t qid0, uint32_t slot, hcx_part_res *packs, uint16_t *slices, uint32_t *overflow) {
+    if (n <= 0) return HCX_SUCCESS;
+    const int per = kBT / 64;
+    k_part_export<<<(n + per - 1) / per, kBT, 0, ctx->stream>>>(res, hist, hbins, n, qid0, slot, packs, slices,
+                                                                overflow);
+    HCX_HIP(hipGetLastError());
+    return HCX_SUCCESS;
+}
+
+int launch_combine(hcx_ctx *ctx, const hcx_part_res *packs, const uint16_t *slices, uint32_t G, int32_t n,
+                   uint32_t slot, hcx_fresult *out, int32_t *keys, uint32_t *fix) {
+    if (n <= 0) return HCX_SUCCESS;
+    const int per = fit_waves(ctx, ctx->hbins);
+    const size_t sm = (size_t)per * ctx->hbins * sizeof(double);
+    HCX_HIP(hipFuncSetAttribute((const void *)k_combine, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+    k_combine<<<(n + per - 1) / per, per * 64, sm, ctx->stream>>>(packs, slices, G, n, slot, ctx->hbins,
+                                                                  ctx->params.min_cpsm, ctx->params.expect_max, out,
+                                                                  keys, fix);
     HCX_HIP(hipGetLastError());
     return HCX_SUCCESS;
 }

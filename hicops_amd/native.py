@@ -156,7 +156,10 @@ CAND_DTYPE = np.dtype([
     ("spec", "<u4"), ("idxoffset", "<u2"), ("bc", "<u2"), ("psid", "<i4"), ("yc", "<u2"),
     ("pad_", "<u2"), ("ibc", "<u8"), ("iyc", "<u8"), ("hyperscore", "<f4"), ("pad2_", "<u4")])
 
+PART_DTYPE = np.dtype([("min", "<u2"), ("max2", "<u2"), ("max", "<f4"), ("N", "<i4"), ("qID", "<i4")])
+FRES_DTYPE = np.dtype([("eValue", "<i4"), ("specID", "<i4"), ("npsms", "<i4")])
 assert PEP_DTYPE.itemsize == 24 and RESULT_DTYPE.itemsize == 56
+assert PART_DTYPE.itemsize == 16 and FRES_DTYPE.itemsize == 12
 assert CAND_DTYPE.itemsize == 40 and C.sizeof(Params) == 120
 
 
@@ -195,7 +198,7 @@ def hcx() -> C.CDLL:
                      "hcx_surrogate_get", "hcx_surrogate_train", "hcx_surrogate_gradient",
                      "hcx_surrogate_last_audit", "hcx_selftest_mfma", "hcx_last_timing", "hcx_last_scan_split", "hcx_selftest_log2", "hcx_search_noalloc",
                      "hcx_raw_upload", "hcx_raw_free", "hcx_queries_pick", "hcx_queries_pickpeaks",
-                     "hcx_queries_info", "hcx_queries_export"]:
+                     "hcx_queries_info", "hcx_queries_export", "hcx_part_export", "hcx_combine"]:
             getattr(lib, name).restype = C.c_int
         _hcx = lib
     return _hcx

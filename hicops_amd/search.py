@@ -332,6 +332,33 @@ class Engine:
         out["flags"] = flags
         return out
 
+    # partial results of index shards and their merge (the reference's
+    # multi-rank wire format, dslim_query.cpp:733-763 / dslim_score.cpp:198-386)
+    def part_export(self, batch: Batch, qid0: int = 0, slot: int = 0) -> tuple[np.ndarray, np.ndarray]:
+        """The last search's partial result: (packs, slices [nspec, slot] u16);
+        slot 0: one histogram's bins."""
+        slot = slot or self.hbins
+        packs = np.zeros(batch.nspec, N.PART_DTYPE)
+        slices = np.zeros((batch.nspec, slot), np.uint16)
+        N.check(N.hcx().hcx_part_export(self._h, batch._h, C.c_int32(qid0), C.c_uint32(slot), N.ptr(packs),
+                                        N.ptr(slices)))
+        return packs, slices
+
+    def combine(self, packs: list, slices: list) -> tuple[np.ndarray, np.ndarray]:
+        """CombineResults over shards in order: (fResult rows, keys)."""
+        g = len(packs)
+        n = len(packs[0])
+        slot = slices[0].shape[1]
+        packs = [np.ascontiguousarray(x, N.PART_DTYPE) for x in packs]
+        slices = [np.ascontiguousarray(x, np.uint16) for x in slices]
+        P = C.c_void_p * g
+        out = np.zeros(n, N.FRES_DTYPE)
+        keys = np.zeros(n, np.int32)
+        N.check(N.hcx().hcx_combine(self._h, C.c_uint32(g), C.c_uint32(n), C.c_uint32(slot),
+                                    P(*[x.ctypes.data for x in packs]), P(*[x.ctypes.data for x in slices]),
+                                    N.ptr(out), N.ptr(keys)))
+        return out, keys
+
     def close(self) -> None:
         for ix in list(self.indices):
             ix.free()

@@ -24,6 +24,11 @@
  *                                 (dslim_query.cpp:771-818); results in place of DFile_PrintScore
  *                                 (dslim_fileout.h:29)
  *   hcx_tailfit_batch             expeRT::ModelTailFit(Results*) over a batch of histograms
+ *   hcx_part_export               the multi-rank query's partial result (dslim_query.cpp:733-763,
+ *                                 expeRT::StoreIResults expeRT.cpp:870-906) of one index shard
+ *   hcx_combine                   DSLIM_Score::CombineResults (dslim_score.cpp:198-386) with
+ *                                 expeRT::Reconstruct (expeRT.cpp:909-936) and
+ *                                 expeRT::ModelTailFit(double&, int) (expeRT.cpp:720-852)
  *   hcx_surrogate_*               no counterpart: learned tail-parameter model with audit fallback
  */
 #ifndef HCX_H
@@ -363,6 +368,48 @@ int  hcx_tailfit_batch(hcx_ctx *ctx, const uint32_t *hist, uint32_t hbins, uint3
                        const uint32_t *cpsms, const int32_t *maxhyp,
                        int32_t *mu, int32_t *beta, int32_t *minhyp, int32_t *nexthyp,
                        uint32_t *flags);
+
+/* ---- partial results of index shards and their merge ----
+ * A rank of the reference searches its share of the database and writes, per
+ * spectrum, a partRes and the u16 counts of its score histogram from the
+ * first to the last occupied bin (slm_dsts.h:757-854, 913-950); one rank
+ * then merges the shares (DSLIM_Score::CombineResults).  Here the shards are
+ * index sets searched one after another on one GPU.                         */
+typedef struct hcx_part_res {   /* partRes, slm_dsts.h:757-854 */
+    uint16_t min;    /* first occupied histogram bin */
+    uint16_t max2;   /* last occupied histogram bin */
+    float    max;    /* the top hit's hyperscore */
+    int32_t  N;      /* candidates scored (cpsms) */
+    int32_t  qID;    /* spectrum id */
+} hcx_part_res;
+typedef struct hcx_fresult {    /* fResult, slm_dsts.h:869-905 */
+    int32_t eValue;  /* e-value x 1e6, truncated */
+    int32_t specID;  /* the winning shard's spectrum id; -1: not reported */
+    int32_t npsms;   /* candidates over all shards; 0: not reported */
+} hcx_fresult;
+/* The partial result of the indices last searched with qb (any kernel, any
+ * e-value method): spectrum s gets qID = qid0 + s; spectra with no candidate
+ * get a zero hcx_part_res.  The counts of bins min..max2 go to slices + s *
+ * slot (the rest of the slot zero); above 65500 candidates each count k is
+ * stored as (u16)((double)(int)(k * 65500) / cpsms) with the int product
+ * wrapping as the reference's does (expeRT.cpp:880-890).  A histogram with
+ * more than `slot` occupied-range bins is HCX_ERR_INVLD_SIZE (the reference,
+ * with slot = psize = 128, writes past its spectrum's slot).  packs: nspec,
+ * slices: nspec * slot u16, host memory. */
+int  hcx_part_export(hcx_ctx *ctx, hcx_qbatch *qb, int32_t qid0, uint32_t slot, hcx_part_res *packs,
+                     uint16_t *slices);
+/* The merge of nshards partial results of the same nspec spectra, shards in
+ * order: counts summed into one histogram per spectrum (decoded from 65500
+ * levels above 65500 candidates), the shard of the highest hyperscore as the
+ * key (strictly greater: the first shard wins a tie), and for spectra with
+ * a key and >= min_cpsm candidates the tail fit of the sum with its slope and
+ * intercept kept in FP64; e < expect_max is reported (eValue = e x 1e6,
+ * specID = the key shard's qID, npsms, keys[s] = key), anything else gets
+ * eValue = expect_max x 1e6, specID -1, npsms 0, keys[s] = nshards.  A fit
+ * whose window or truncation lies within the device log10's error bound of
+ * a boundary is done again on the host.  Host memory. */
+int  hcx_combine(hcx_ctx *ctx, uint32_t nshards, uint32_t nspec, uint32_t slot, const hcx_part_res *const *packs,
+                 const uint16_t *const *slices, hcx_fresult *out, int32_t *keys);
 
 /* ---- learned tail-parameter surrogate (no reference counterpart) ----
  * A 128-128-128-3 MLP (BF16 MFMA, FP32 accumulate) maps a spectrum's score

@@ -32,7 +32,9 @@ def lib() -> C.CDLL:
         L.orc_stat_mod.restype = C.c_float
         L.orc_aa_mass.argtypes = [C.c_char]
         L.orc_stat_mod.argtypes = [C.c_char]
-        for f in ("orc_index_build", "orc_index_build_all", "orc_window", "orc_search", "orc_candidates", "orc_count_work",
+        L.orc_tailfit_merged.restype = C.c_double
+        L.orc_tailfit_merged.argtypes = [C.c_void_p, C.c_uint32, C.c_int32, C.c_int32, C.c_void_p]
+        for f in ("orc_part_export", "orc_combine", "orc_index_build", "orc_index_build_all", "orc_window", "orc_search", "orc_candidates", "orc_count_work",
                   "orc_tailfit", "orc_heap_replay", "orc_heap_capacity", "orc_pickpeaks"):
             getattr(L, f).restype = C.c_int
         _lib = L
@@ -192,6 +194,44 @@ def tailfit(hist: np.ndarray, cpsms: int, maxhyp: int):
     lib().orc_tailfit(N.ptr(h), C.c_uint32(len(h)), C.c_uint32(cpsms), C.c_int32(maxhyp), C.byref(mu), C.byref(beta),
                       C.byref(mn), C.byref(nx), C.byref(fl))
     return mu.value, beta.value, mn.value, nx.value, fl.value
+
+
+def part_export(res: np.ndarray, hist: np.ndarray, qid0: int, slot: int):
+    """One shard's partRes + u16 slices of searched spectra (res, hist)."""
+    res = np.ascontiguousarray(res, N.RESULT_DTYPE)
+    hist = np.ascontiguousarray(hist, np.uint32)
+    n, hb = hist.shape
+    packs = np.zeros(n, N.PART_DTYPE)
+    slices = np.zeros((n, slot), np.uint16)
+    rc = lib().orc_part_export(N.ptr(res), N.ptr(hist), C.c_uint32(hb), C.c_uint32(n), C.c_int32(qid0),
+                               C.c_uint32(slot), N.ptr(packs), N.ptr(slices))
+    if rc != 0:
+        raise RuntimeError(f"orc_part_export failed: {rc}")
+    return packs, slices
+
+
+def tailfit_merged(pdata: np.ndarray, pN: int, hyp: int):
+    pdata = np.ascontiguousarray(pdata, np.float64)
+    fl = C.c_uint32(0)
+    e = lib().orc_tailfit_merged(pdata.ctypes.data, len(pdata), pN, hyp, C.byref(fl))
+    return e, fl.value
+
+
+def combine(params: N.Params, packs: list, slices: list, slot: int):
+    """CombineResults over shards (lists of per-shard arrays, shard order)."""
+    g = len(packs)
+    n = len(packs[0])
+    packs = [np.ascontiguousarray(x, N.PART_DTYPE) for x in packs]
+    slices = [np.ascontiguousarray(x, np.uint16) for x in slices]
+    P = C.c_void_p * max(1, g)
+    out = np.zeros(n, N.FRES_DTYPE)
+    keys = np.zeros(n, np.int32)
+    rc = lib().orc_combine(C.byref(params), C.c_uint32(g), C.c_uint32(n), C.c_uint32(slot),
+                           P(*[x.ctypes.data for x in packs]), P(*[x.ctypes.data for x in slices]),
+                           N.ptr(out), N.ptr(keys))
+    if rc != 0:
+        raise RuntimeError(f"orc_combine failed: {rc}")
+    return out, keys
 
 
 def heap_replay(hs, cap: int):

@@ -525,6 +525,180 @@ int orc_tailfit(const double *yy, uint32_t hbins, uint32_t cpsms, int32_t hyp,
     return HCX_SUCCESS;
 }
 
+/* ---------------- multi-rank partial results and merge ---------------- */
+/* The reference's (ushort) conversions of doubles outside 0..65535 and its
+ * int product k * 65500 are undefined in C++; x86-64 g++ code converts
+ * through a 32-bit cvttsd2si and keeps the low 16 bits, and multiplies in
+ * 32 bits with wraparound.  Both are restated that way. */
+static uint16_t u16_of(double v) { return (uint16_t)(uint32_t)(int32_t)v; }
+
+/* dslim_query.cpp:733-763 and expeRT::StoreIResults (expeRT.cpp:870-906) */
+int orc_part_export(const hcx_result *res, const uint32_t *hist, uint32_t hbins, uint32_t nspec, int32_t qid0,
+                    uint32_t slot, hcx_part_res *packs, uint16_t *slices)
+{
+    for (uint32_t s = 0; s < nspec; s++) {
+        const hcx_result *r = res + s;
+        hcx_part_res *pr = packs + s;
+        uint16_t *out = slices + (size_t)s * slot;
+        memset(pr, 0, sizeof *pr);
+        memset(out, 0, sizeof(uint16_t) * slot);
+        if ((int32_t)r->cpsms >= 1) {
+            const uint32_t *yy = hist + (size_t)s * hbins;
+            int ends = (int)hbins - 1, stt = 0;          /* rargmax / argmax at 0.99 */
+            for (int p = (int)hbins - 1; p >= 0; p--) if ((double)yy[p] >= 0.99) { ends = p; break; }
+            for (int p = 0; p <= ends; p++) if ((double)yy[p] >= 0.99) { stt = p; break; }
+            if ((uint32_t)(ends - stt + 1) > slot) return HCX_ERR_INVLD_SIZE;
+            for (int ii = stt; ii <= ends; ii++) {
+                uint16_t k = u16_of((double)yy[ii]);
+                if ((int32_t)r->cpsms > 65500)
+                    k = u16_of((double)(int32_t)((uint32_t)k * 65500u) / (double)(int32_t)r->cpsms);
+                out[ii - stt] = k;
+            }
+            pr->min = (uint16_t)stt;
+            pr->max2 = (uint16_t)ends;
+            pr->max = r->hyperscore;
+            pr->N = (int32_t)r->cpsms;
+        }
+        pr->qID = qid0 + (int32_t)s;
+    }
+    return HCX_SUCCESS;
+}
+
+/* expeRT::ModelTailFit(double&, int) (expeRT.cpp:720-852) on the merged
+ * histogram pdata with pN candidates: unlike the one-rank fit, no widening
+ * of a one-bin range, vaa2 = max(std::accumulate(range, 1) in an int, pN),
+ * the counts divided before the running sum, and the slope and intercept
+ * used unrounded.  Undefined paths of the reference (hyp outside the
+ * histogram, mark < 0) are a zero line, flagged. */
+double orc_tailfit_merged(const double *pdata, uint32_t hbins, int32_t pN, int32_t hyp, uint32_t *flags)
+{
+    double mu_t = 0.0, beta_t = 4.0;
+    if (hyp < 1 || hyp >= (int32_t)hbins) {
+        /* the reference reads outside pdata: not reported */
+        if (flags) *flags |= HCX_RES_FIT_UB;
+        return INFINITY;
+    }
+    int end1 = rargmax_d(pdata, 0, hyp - 1, 1.0);
+    int stt1 = argmax_d(pdata, 0, end1, 1.0);
+    int n = end1 - stt1 + 1;
+    int vaa = pN;
+    if (vaa < 1) {
+        mu_t = 0.0; beta_t = 100.0;
+    } else {
+        int acc = 1;
+        for (int i = 0; i < n; i++) acc = (int)((double)acc + pdata[stt1 + i]);
+        const int vaa2 = acc > vaa ? acc : vaa;
+        double *sx = (double *)malloc(sizeof(double) * (size_t)n);
+        double run = 0.0;
+        for (int i = 0; i < n; i++) {            /* divide, partial_sum, add(-1), negative */
+            run = (i == 0) ? pdata[stt1] / (double)vaa2 : run + pdata[stt1 + i] / (double)vaa2;
+            double t = run + (-1.0);
+            sx[i] = -t;
+        }
+        for (int i = 0; i < n; i++) if (sx[i] > 1.0) sx[i] = 0.999;
+        int rep = rargmax_d(sx, 0, n - 1, 1e-4);
+        double rv = sx[rep];
+        for (int i = 0; i < n; i++) if (sx[i] <= 0.0) sx[i] = rv;
+        for (int i = 0; i < n; i++) sx[i] = log10(sx[i]);
+        int mark = 0, mark2 = 0;
+        double hgt = sx[n - 1] - sx[0];
+        if (n > 3) {
+            mark = largmax_d(sx, 0, n - 1, sx[0] + hgt * 0.22) - 1;
+            mark2 = rargmax_d(sx, 0, n - 1, sx[0] + hgt * 0.87);
+            if (mark2 == n) mark2 -= 1;
+            if (mark >= mark2) mark = mark2 - 1;
+        } else if (n == 3) {
+            mark = largmax_d(sx, 0, n - 1, sx[0] + hgt * 0.22) - 1;
+            mark2 = n - 1;
+            if (mark >= mark2) mark = mark2 - 1;
+        } else {
+            mark = 0;
+            mark2 = n - 1;
+        }
+        if (mark < 0) {
+            if (flags) *flags |= HCX_RES_FIT_UB;
+            mu_t = 0.0; beta_t = 0.0;
+        } else {
+            int m = mark2 - mark + 1;
+            const double *y = sx + mark;
+            if (m == 1) {                         /* expeRT.cpp:1308-1313 */
+                mu_t = 0.0; beta_t = y[0];
+            } else {                              /* expeRT.cpp:1317-1342 */
+                double xbar = 0.0, ybar = 0.0, top = 0.0, bot = 0.0;
+                for (int i = 0; i < m; i++) {
+                    xbar = xbar + (double)(stt1 + mark + i);
+                    ybar = ybar + y[i];
+                }
+                xbar = xbar / (double)m;
+                ybar = ybar / (double)m;
+                for (int i = 0; i < m; i++) {
+                    double dx = (double)(stt1 + mark + i) - xbar;
+                    top = top + dx * (y[i] - ybar);
+                    bot = bot + dx * dx;
+                }
+                mu_t = top / bot;
+                beta_t = ybar - mu_t * xbar;
+            }
+        }
+        free(sx);
+    }
+    return pow(10, hyp * mu_t + beta_t) * vaa;
+}
+
+/* DSLIM_Score::CombineResults, dslim_score.cpp:244-345, with
+ * expeRT::Reconstruct (expeRT.cpp:909-936).  eValue = e * 1e6 into an int
+ * (dslim_score.cpp:300, 318): out of the int range that is undefined in C++;
+ * x86-64's cvttsd2si gives INT_MIN, restated explicitly. */
+static int32_t i32_of(double v) { return (v > -2147483649.0 && v < 2147483648.0) ? (int32_t)v : INT32_MIN; }
+int orc_combine(const hcx_params *p, uint32_t nshards, uint32_t nspec, uint32_t slot,
+                const hcx_part_res *const *packs, const uint16_t *const *slices, hcx_fresult *out, int32_t *keys)
+{
+    const uint32_t hbins = 10 * p->max_hyperscore + 2;
+    double *pdata = (double *)malloc(sizeof(double) * hbins);
+    if (!pdata) return HCX_ERR_BAD_MEM_ALLOC;
+    for (uint32_t spec = 0; spec < nspec; spec++) {
+        memset(pdata, 0, sizeof(double) * hbins);
+        int32_t cpsms = 0, pN = 0, key = (int32_t)nshards;
+        float maxhyp = -1;
+        for (uint32_t sno = 0; sno < nshards; sno++) {
+            const hcx_part_res *r = packs[sno] + spec;
+            if (r->min == 0 && r->max == 0 && r->N == 0 && r->max2 == 0) continue;   /* *sResult == 0 */
+            cpsms += r->N;
+            if (r->N >= 1) {
+                const uint16_t *buf = slices[sno] + (size_t)spec * slot;
+                pN += r->N;
+                for (int jj = r->min; jj <= (int)r->max2; jj++) {
+                    double v = buf[jj - r->min];
+                    if (r->N > 65500) v = (v / 65500) * r->N;
+                    pdata[jj] = pdata[jj] + v;
+                }
+                if (r->max > 0 && r->max > maxhyp) { maxhyp = r->max; key = (int32_t)sno; }
+            }
+        }
+        hcx_fresult *f = out + spec;
+        int reported = 0;
+        if (key < (int32_t)nshards && cpsms >= (int32_t)p->min_cpsm) {
+            const int32_t hyp = (int32_t)((double)(maxhyp * 10.0f) + 0.5);
+            const double e_x = orc_tailfit_merged(pdata, hbins, pN, hyp, NULL);
+            if (e_x < p->expect_max) {
+                f->eValue = i32_of(e_x * 1e6);
+                f->specID = packs[key][spec].qID;
+                f->npsms = cpsms;
+                keys[spec] = key;
+                reported = 1;
+            }
+        }
+        if (!reported) {
+            f->eValue = i32_of(p->expect_max * 1e6);
+            f->specID = -1;
+            f->npsms = 0;
+            keys[spec] = (int32_t)nshards;
+        }
+    }
+    free(pdata);
+    return HCX_SUCCESS;
+}
+
 /* ---------------- dslim_query.cpp:515-842 ---------------- */
 typedef struct orc_byc { int16_t bc, yc; int64_t ibc, iyc; } orc_byc;
 

@@ -52,6 +52,22 @@ int orc_index_build_all(const hcx_params *p, uint32_t nlen, const uint32_t *pepl
                         const hcx_pep_entry *const *entries, const uint32_t *n, uint32_t *const *iA,
                         uint32_t *const *bA, int threads);
 
+/* dslim_query.cpp:733-763 + expeRT.cpp:870-906: one shard's partRes and
+ * u16 histogram slices of searched spectra (res, hist).  Pinned: hand-traced
+ * slices and the 65500-level encoding with its int wrap
+ * (tests/test_oracle_pins.py). */
+int orc_part_export(const hcx_result *res, const uint32_t *hist, uint32_t hbins, uint32_t nspec, int32_t qid0,
+                    uint32_t slot, hcx_part_res *packs, uint16_t *slices);
+/* expeRT.cpp:720-852: the merged-histogram tail fit; returns the e-value.
+ * Pinned: a histogram whose survival function is exactly geometric (closed
+ * form line), the one-bin range, the accumulate-from-1 denominator. */
+double orc_tailfit_merged(const double *pdata, uint32_t hbins, int32_t pN, int32_t hyp, uint32_t *flags);
+/* dslim_score.cpp:244-345 + expeRT.cpp:909-936.  Pinned: hand-built
+ * shard sets (tie to the first shard, skipped empty shards, cpsms sum,
+ * decoding above 65500 candidates). */
+int orc_combine(const hcx_params *p, uint32_t nshards, uint32_t nspec, uint32_t slot,
+                const hcx_part_res *const *packs, const uint16_t *const *slices, hcx_fresult *out, int32_t *keys);
+
 /* dslim_query.cpp:854-994.  Window [minl, maxl] over masses sorted
  * ascending; returns 1 if valid.  Pinned: brute-force linear scan. */
 int orc_window(const float *mass, uint32_t n, float precursor, double dM,

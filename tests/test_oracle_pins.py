@@ -531,3 +531,116 @@ def test_pickpeaks_edges_and_precursor_fields():
     assert q.precurse.tolist() == [np.float32(1234.56789), np.float32(500.25), 700.0, 800.5]
     assert q.charges.tolist() == [1, 1, 3, 2]
     assert q.rtimes.tolist() == [0.0, np.float32(12.3), 0.0, 1e9]
+
+
+# ---- partial results of index shards and their merge (dslim_query.cpp:733-763,
+# expeRT.cpp:720-936, dslim_score.cpp:244-345), traced by hand ----
+
+def _res(cpsms, hs):
+    r = np.zeros(1, N.RESULT_DTYPE)
+    r["cpsms"], r["hyperscore"] = cpsms, hs
+    return r
+
+
+def test_part_export_hand_traced():
+    hb = 1002
+    res = np.concatenate([_res(100, 4.25), _res(131000, 9.5), _res(80000, 7.0), _res(0, 0.0)])
+    hist = np.zeros((4, hb), np.uint32)
+    hist[0, 3], hist[0, 5], hist[0, 9] = 2, 7, 1
+    hist[1, 20], hist[1, 22] = 1000, 70000
+    hist[2, 40] = 40000
+    packs, slices = O.part_export(res, hist, qid0=500, slot=8)
+    # spectrum 0: bins 3..9, counts as they are
+    assert tuple(packs[0]) == (3, 9, np.float32(4.25), 100, 500)
+    assert list(slices[0]) == [2, 0, 7, 0, 0, 0, 1, 0]
+    # spectrum 1 (> 65500 candidates): 1000 * 65500 / 131000 = 500; the
+    # count 70000 is first cut to 16 bits (4464), then 4464 * 65500 / 131000 = 2232
+    assert tuple(packs[1])[:4] == (20, 22, np.float32(9.5), 131000)
+    assert list(slices[1][:3]) == [500, 0, 2232]
+    # spectrum 2: 40000 * 65500 wraps in 32 bits to -1674967296; / 80000 =
+    # -20937.09, truncated to -20937, kept as its low 16 bits: 44599
+    assert list(slices[2][:1]) == [44599]
+    # spectrum 3: no candidate -> a zero record with its id
+    assert tuple(packs[3]) == (0, 0, 0.0, 0, 503) and not slices[3].any()
+    with pytest.raises(RuntimeError):
+        O.part_export(res[:1], hist[:1], qid0=0, slot=6)   # bins 3..9 need 7 entries
+
+
+def test_tailfit_merged_geometric_closed_form():
+    # counts 512, 256, ..., 1 in bins 10..19: with the denominator
+    # max(1 + 1023, pN) = 1024 the survival function is exactly 2^-(i+1);
+    # the 22 % / 87 % thresholds of its log drop select bins 11..17, whose
+    # line is log10 s = -(x - 9) log10 2, so e = 2^-(hyp - 9) * pN
+    pd = np.zeros(1002)
+    pd[10:20] = [512, 256, 128, 64, 32, 16, 8, 4, 2, 1]
+    for pN in (1000, 1024):
+        e, fl = O.tailfit_merged(pd, pN, 50)
+        assert fl == 0
+        assert abs(e - pN * 2.0 ** -41) <= 1e-9 * pN * 2.0 ** -41, (pN, e)
+    # the one-rank fit would truncate slope and intercept to 1e-6 first
+    mu, beta = int(-math.log10(2) * 1e6) / 1e6, int(9 * math.log10(2) * 1e6) / 1e6
+    assert abs(10 ** (50 * mu + beta) * 1000 - 1000 * 2.0 ** -41) > 1e-5 * 1000 * 2.0 ** -41
+
+
+def test_tailfit_merged_one_bin_range():
+    # one occupied bin below the top score: the merged fit keeps the one-bin
+    # range (no widening to the top score's bin): s = 1 - 5/10, a single
+    # point, slope 0, e = 0.5 * 10
+    pd = np.zeros(1002)
+    pd[10], pd[11] = 5, 1
+    e, fl = O.tailfit_merged(pd, 10, 11)
+    assert abs(e - 5.0) <= 1e-12 and fl == 0
+
+
+def test_combine_hand_traced():
+    prm = params()
+    prm.min_cpsm, prm.expect_max = 4, 20.0
+    slot = 8
+    n = 3
+    packs = [np.zeros(n, N.PART_DTYPE) for _ in range(3)]
+    slices = [np.zeros((n, slot), np.uint16) for _ in range(3)]
+    for g in range(3):
+        packs[g]["qID"] = np.arange(n) + 100 * g
+    # spectrum 0: shards 0 and 1 tie on the top hyperscore -> shard 0; shard 2 empty
+    packs[0][0] = (10, 14, 5.0, 300, 0)
+    slices[0][0, :5] = [100, 80, 60, 40, 19]
+    packs[1][0] = (11, 13, 5.0, 200, 100)
+    slices[1][0, :3] = [90, 70, 39]
+    # spectrum 1: shard 2 strictly higher; shard 1 encoded at 65500 levels
+    packs[0][1] = (10, 12, 3.0, 50, 1)
+    slices[0][1, :3] = [30, 15, 5]
+    packs[1][1] = (10, 11, 2.0, 70000, 101)
+    slices[1][1, :2] = [40000, 25500]
+    packs[2][1] = (12, 12, 4.0, 10, 201)
+    slices[2][1, :1] = [10]
+    # spectrum 2: fewer than min_cpsm candidates overall
+    packs[0][2] = (10, 10, 2.0, 3, 2)
+    slices[0][2, :1] = [3]
+    out, keys = O.combine(prm, packs, slices, slot)
+    assert keys[2] == 3
+    assert out[2]["specID"] == -1 and out[2]["npsms"] == 0 and out[2]["eValue"] == int(20.0 * 1e6)
+    # the merged histograms, summed in shard order and decoded
+    pd0 = np.zeros(prm.hbins)
+    pd0[10:15] += [100, 80, 60, 40, 19]
+    pd0[11:14] += [90, 70, 39]
+    pd1 = np.zeros(prm.hbins)
+    pd1[10:13] += [30, 15, 5]
+    pd1[10:12] += [40000 / 65500 * 70000, 25500 / 65500 * 70000]
+    pd1[12] += 10
+    for s, pd, pN, hs, key in ((0, pd0, 500, 5.0, 0), (1, pd1, 70060, 4.0, 2)):
+        e, _ = O.tailfit_merged(pd, pN, int(np.float32(hs) * np.float32(10) + 0.5))
+        assert e < 20.0
+        assert keys[s] == key and out[s]["eValue"] == int(e * 1e6)
+        assert out[s]["specID"] == packs[key][s]["qID"] and out[s]["npsms"] == pN
+
+
+def test_combine_evalue_out_of_int_range():
+    # eValue = e x 1e6 into an int: beyond the int range x86-64 gives INT_MIN
+    prm = params()
+    prm.min_cpsm, prm.expect_max = 4, 1e300
+    packs = [np.zeros(1, N.PART_DTYPE)]
+    slices = [np.zeros((1, 4), np.uint16)]
+    packs[0][0] = (10, 10, 2.0, 3, 7)
+    slices[0][0, 0] = 3
+    out, keys = O.combine(prm, packs, slices, 4)
+    assert out[0]["eValue"] == -2 ** 31 and out[0]["specID"] == -1 and keys[0] == 1
