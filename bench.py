@@ -246,6 +246,7 @@ def main() -> int:
         "queries_pick": pick,
         "psms_valid": valid,
         "tied_spectra": tied,
+        "host_fits": int(((res["flags"] & N.RES_FIT_HOST) != 0).sum()),
         "work": {"n_ia": n_ia, "n_probe": n_probe, "peaks": npeaks},
     }
     print(json.dumps(line), flush=True)
