@@ -22,7 +22,7 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 
-HIP_SOURCES = ["hcx_api.hip", "k_index.hip", "k_query.hip", "fit_host.hip", "k_chunk.hip", "k_packed.hip", "k_mlp.hip", "k_pick.hip"]
+HIP_SOURCES = ["hcx_api.hip", "k_index.hip", "k_query.hip", "fit_host.hip", "k_chunk.hip", "k_packed.hip", "k_mlp.hip", "k_pick.hip", "k_lbe.hip"]
 ARCH = os.environ.get("HCX_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -304,6 +304,22 @@ extern "C" int hcx_index_build(hcx_ctx *c, uint32_t peplen, const char *seqs, ui
         if (!(entries[k - 1].mass <= entries[k].mass))
             return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_index_build: entries not sorted by mass");
     if (hipSetDevice(c->device) != hipSuccess) return set_error("hipSetDevice");
+    std::vector<float> mass(nentries);
+    for (uint32_t k = 0; k < nentries; k++) mass[k] = entries[k].mass;
+    return hcx::index_build_from(c, peplen, seqs, nseqs, entries, mass.data(), nentries, hipMemcpyHostToDevice, out);
+}
+
+namespace hcx {
+
+// DSLIM_Construct over entries, masses and sequences copied from host or
+// device memory (kind), into a new index.
+int index_build_from(hcx_ctx *c, uint32_t peplen, const char *seqs, uint32_t nseqs, const hcx_pep_entry *entries,
+                     const float *mass, uint32_t nentries, hipMemcpyKind kind, hcx_index **out) {
+    const hcx_params &p = c->params;
+    const uint32_t speclen = 2 * p.maxz * (peplen - 1);
+    const uint64_t nions = (uint64_t)nentries * speclen;
+    if (nions >= 0xFFFFFFF0ull)
+        return set_error_code(HCX_ERR_INVLD_SIZE, "hcx_index_build: more than 2^32 ions in one length");
     hcx_index *ix = new (std::nothrow) hcx_index;
     if (!ix) return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_index_build: out of memory");
     ix->ctx = c;
@@ -320,18 +336,15 @@ extern "C" int hcx_index_build(hcx_ctx *c, uint32_t peplen, const char *seqs, ui
         delete ix;
         return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_index_build: device allocation failed");
     }
-    std::vector<float> mass(nentries);
-    for (uint32_t k = 0; k < nentries; k++) mass[k] = entries[k].mass;
     auto fail = [&](int rc) {
+        (void)hipStreamSynchronize(c->stream);
         delete ix;
         return rc;
     };
     if (nentries) {
-        if (hipMemcpyAsync(ix->entries.p, entries, sizeof(hcx_pep_entry) * nentries, hipMemcpyHostToDevice,
-                           c->stream) != hipSuccess ||
-            hipMemcpyAsync(ix->mass.p, mass.data(), 4ull * nentries, hipMemcpyHostToDevice, c->stream) !=
-                hipSuccess ||
-            hipMemcpyAsync(ix->seqs.p, seqs, (size_t)nseqs * peplen, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        if (hipMemcpyAsync(ix->entries.p, entries, sizeof(hcx_pep_entry) * nentries, kind, c->stream) != hipSuccess ||
+            hipMemcpyAsync(ix->mass.p, mass, 4ull * nentries, kind, c->stream) != hipSuccess ||
+            hipMemcpyAsync(ix->seqs.p, seqs, (size_t)nseqs * peplen, kind, c->stream) != hipSuccess)
             return fail(set_error("hcx_index_build: upload failed"));
     }
     BuildArgs a;
@@ -373,6 +386,14 @@ extern "C" int hcx_index_build(hcx_ctx *c, uint32_t peplen, const char *seqs, ui
     *out = ix;
     return HCX_SUCCESS;
 }
+
+int index_build_device(hcx_ctx *c, uint32_t peplen, const char *d_seqs, uint32_t nseqs, const hcx_pep_entry *d_entries,
+                       const float *d_mass, uint32_t nentries, hcx_index **out) {
+    if (hipSetDevice(c->device) != hipSuccess) return set_error("hipSetDevice");
+    return index_build_from(c, peplen, d_seqs, nseqs, d_entries, d_mass, nentries, hipMemcpyDeviceToDevice, out);
+}
+
+}  // namespace hcx
 
 extern "C" int hcx_index_info(const hcx_index *ix, uint32_t *peplen, uint32_t *speclen, uint32_t *nentries,
                               uint64_t *nions) {

@@ -151,6 +151,16 @@ struct BuildArgs {
 int reserve_index_build(hcx_ctx *ctx, uint64_t nions, uint32_t NB);
 int launch_index_build(hcx_ctx *ctx, const BuildArgs &a, uint32_t *d_iA, uint32_t *d_bA,
                        const float *d_mass, uint32_t *d_mdir, uint32_t max_mass);
+// DSLIM_Construct into a new index from entries, masses and sequences in host
+// (kind hipMemcpyHostToDevice) or device memory (hcx_api.hip).
+int index_build_from(hcx_ctx *c, uint32_t peplen, const char *seqs, uint32_t nseqs, const hcx_pep_entry *entries,
+                     const float *mass, uint32_t nentries, hipMemcpyKind kind, hcx_index **out);
+int index_build_device(hcx_ctx *c, uint32_t peplen, const char *d_seqs, uint32_t nseqs, const hcx_pep_entry *d_entries,
+                       const float *d_mass, uint32_t nentries, hcx_index **out);
+// ---- peptide database stage (k_lbe.hip) ----
+// Sorts the segments [seg[i].x, seg[i].y) of (K, V) by key into the order
+// libstdc++'s std::sort leaves them; ends with the stream synchronised.
+int sort_mass_segments(hcx_ctx *c, float *K, uint32_t *V, uint64_t total, const std::vector<uint2> &seg);
 
 // ---- query (k_query.hip) ----
 struct QueryArgs {

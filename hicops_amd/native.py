@@ -54,6 +54,11 @@ SCAN_AUTO = 0
 SCAN_TWO_PASS = 1
 EVAL_TAILFIT = 0
 EVAL_SURROGATE = 1
+POLICY_CHUNK = 0
+POLICY_CYCLIC = 1
+ERR_INVLD_SORT = -3009
+ERR_INVLD_MOD = -3010
+ERR_INVLD_NODE_RANK = -3011
 
 
 class HcxError(RuntimeError):
@@ -118,6 +123,11 @@ class Audit(C.Structure):
         ("audited", C.c_uint32), ("disagreed", C.c_uint32), ("max_rel_err", C.c_double),
         ("fell_back", C.c_int32), ("located", C.c_uint32), ("plain_fit", C.c_uint32), ("pad_", C.c_uint32),
     ]
+
+
+class LbeCfg(C.Structure):
+    _fields_ = [("modconditions", C.c_char_p), ("nodes", C.c_uint32), ("myid", C.c_uint32),
+                ("policy", C.c_int32), ("sort", C.c_int32)]
 
 
 class DbGenCfg(C.Structure):
@@ -198,7 +208,9 @@ def hcx() -> C.CDLL:
                      "hcx_surrogate_get", "hcx_surrogate_train", "hcx_surrogate_gradient",
                      "hcx_surrogate_last_audit", "hcx_selftest_mfma", "hcx_last_timing", "hcx_last_scan_split", "hcx_selftest_log2", "hcx_search_noalloc",
                      "hcx_raw_upload", "hcx_raw_free", "hcx_queries_pick", "hcx_queries_pickpeaks",
-                     "hcx_queries_info", "hcx_queries_export", "hcx_part_export", "hcx_combine"]:
+                     "hcx_queries_info", "hcx_queries_export", "hcx_part_export", "hcx_combine",
+                     "hcx_pepdb_create", "hcx_pepdb_counts", "hcx_pepdb_generate", "hcx_pepdb_export",
+                     "hcx_pepdb_ms", "hcx_pepdb_free", "hcx_index_build_pepdb", "hcx_sort_mass"]:
             getattr(lib, name).restype = C.c_int
         _hcx = lib
     return _hcx

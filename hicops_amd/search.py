@@ -1,6 +1,11 @@
 """Python face of the search core, mirroring the reference's entry points.
 
   Engine(params)                  gParams + DSLIM_InitializeScorecard (dslim.cpp:390-437)
+  Engine.count_peps(L, lines, ...) LBE_CountPeps + MODS_ModCounter for one peptide length
+                                  (lbe_internal.cpp:369-467, mods.cpp:397-431)
+  Engine.initialize(dbs)          LBE_Initialize (lbe_internal.cpp:119-206) for several lengths at
+                                  once: LBE_GeneratePeps, MODS_GenerateMods, std::sort by mass
+  Engine.construct_pepdb(db)      DSLIM_Construct over a device-resident PepDb
   Engine.construct(L, seqs, ents) DSLIM_Construct(Index*) for one peptide length (dslim.cpp:46-138)
   Engine.pickpeaks(raw, ...)      MSQuery::pickpeaks (msquery.cpp:509-569) over a batch, the
                                   Queries producer (device-resident result)
@@ -63,6 +68,49 @@ class Index:
             eng = getattr(self, "engine", None)
             if eng is not None and self in getattr(eng, "indices", []):
                 eng.indices.remove(self)
+
+    def __del__(self):
+        self.free()
+
+
+class PepDb:
+    """One peptide length's database (the reference's per-length Index before
+    DSLIM_Construct: sequences and pepEntries), resident in HBM."""
+
+    def __init__(self, engine: "Engine", peplen: int, lines: np.ndarray, modconditions: str | None,
+                 nodes: int = 1, myid: int = 0, policy: int = N.POLICY_CYCLIC, sort: bool = True):
+        self.engine = engine
+        self.peplen = peplen
+        lines = np.ascontiguousarray(lines, dtype=np.uint8)
+        cfg = N.LbeCfg()
+        cfg.modconditions = modconditions.encode() if modconditions is not None else None
+        cfg.nodes, cfg.myid, cfg.policy, cfg.sort = nodes, myid, policy, int(bool(sort))
+        h = C.c_void_p()
+        N.check(N.hcx().hcx_pepdb_create(engine._h, C.byref(cfg), C.c_uint32(peplen), N.ptr(lines),
+                                         C.c_uint32(len(lines) // peplen), C.byref(h)))
+        self._h = h
+        cnt = np.zeros(6, np.uint32)
+        N.check(N.hcx().hcx_pepdb_counts(h, N.ptr(cnt)))
+        (self.pep_count, self.mod_count, self.total_count, self.lclpep, self.lclmod,
+         self.lcltot) = (int(x) for x in cnt)
+
+    def timing(self) -> tuple[float, float]:
+        """(create ms, generate ms) on the device."""
+        a, b = C.c_float(), C.c_float()
+        N.check(N.hcx().hcx_pepdb_ms(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def export(self) -> tuple[np.ndarray, np.ndarray]:
+        """(kept sequences as pep_count * peplen bytes, pepEntries)."""
+        seqs = np.zeros(self.pep_count * self.peplen, np.uint8)
+        ents = np.zeros(self.lcltot, N.PEP_DTYPE)
+        N.check(N.hcx().hcx_pepdb_export(self._h, N.ptr(seqs), N.ptr(ents)))
+        return seqs, ents
+
+    def free(self) -> None:
+        if getattr(self, "_h", None):
+            N.hcx().hcx_pepdb_free(self._h)
+            self._h = None
 
     def __del__(self):
         self.free()
@@ -182,6 +230,35 @@ class Engine:
 
     def construct_db(self, db) -> list[Index]:
         return [self.construct(L, db.seqs(L), db.entries(L)) for L in db.lengths]
+
+    # the peptide database stage (LBE + MODS) on the device
+    def count_peps(self, peplen: int, lines: np.ndarray, modconditions: str | None, nodes: int = 1,
+                   myid: int = 0, policy: int = N.POLICY_CYCLIC, sort: bool = True) -> PepDb:
+        return PepDb(self, peplen, lines, modconditions, nodes, myid, policy, sort)
+
+    def initialize(self, dbs: list) -> None:
+        arr = (C.c_void_p * max(1, len(dbs)))(*[d._h for d in dbs])
+        N.check(N.hcx().hcx_pepdb_generate(arr, C.c_uint32(len(dbs))))
+
+    def construct_pepdb(self, db: PepDb) -> Index:
+        ix = Index.__new__(Index)
+        ix.engine = self
+        ix.peplen = db.peplen
+        h = C.c_void_p()
+        N.check(N.hcx().hcx_index_build_pepdb(self._h, db._h, C.byref(h)))
+        ix._h = h
+        pl, sl, ne, ni = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint64()
+        N.check(N.hcx().hcx_index_info(h, C.byref(pl), C.byref(sl), C.byref(ne), C.byref(ni)))
+        ix.speclen, ix.n, ix.nions = sl.value, ne.value, ni.value
+        self.indices.append(ix)
+        return ix
+
+    def sort_mass(self, keys: np.ndarray, vals: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+        """The device mass sort alone (libstdc++ std::sort order by key)."""
+        k = np.ascontiguousarray(keys, np.float32).copy()
+        v = np.ascontiguousarray(vals, np.uint32).copy()
+        N.check(N.hcx().hcx_sort_mass(self._h, N.ptr(k), N.ptr(v), C.c_uint32(len(k))))
+        return k, v
 
     def upload(self, queries) -> Batch:
         return Batch(self, queries)

@@ -93,6 +93,14 @@ def make_params(p: Preset, topmatches: int = 10, expect_max: float = 1e300) -> N
     return prm
 
 
+def modconditions(p: Preset) -> str:
+    """gParams.modconditions for the preset's variable mods, as hicops.cpp:636-666
+    builds it: the per-form limit, then residues and per-form cap per mod."""
+    if not p.mods or p.max_mods == 0:
+        return "0 M 0"
+    return " ".join([str(p.max_mods)] + [f"{res} {mx}" for res, _, mx in p.mods])
+
+
 class Database:
     """Per-length sequence tables and mass-sorted pepEntries."""
 

@@ -13,6 +13,12 @@
  *   hcx_index_build               DSLIM_Construct(Index*)            (dslim.h:49, dslim.cpp:46-138)
  *   hcx_index_export              read-back of Index.ionIndex[0].{iA,bA} (slm_dsts.h:306-320)
  *   hcx_index_free                DSLIM_DeallocateIonIndex(Index*)   (dslim.h:149, dslim.cpp:644-691)
+ *   hcx_pepdb_create              LBE_CountPeps(char*, Index*, uint_t) (lbe.h:90, lbe_internal.cpp:369-467)
+ *                                 with MODS_Initialize / MODS_ModCounter (mods.cpp:63-101, 397-431)
+ *   hcx_pepdb_generate            LBE_Initialize(Index*) (lbe.h:45, lbe_internal.cpp:119-206) with
+ *                                 LBE_GeneratePeps, MODS_GenerateMods (mods.cpp:447-496)
+ *   hcx_pepdb_counts              LBE_CreatePartitions(Index*) (lbe_internal.cpp:308-352)
+ *   hcx_index_build_pepdb         DSLIM_Construct(Index*) on the device-resident pepEntries
  *   hcx_queries_upload            hand-over of a Queries batch       (slm_dsts.h:422-559)
  *   hcx_raw_upload, hcx_queries_pick, hcx_queries_pickpeaks
  *                                 MSQuery::pickpeaks (msquery.cpp:509-569) over a batch, as
@@ -229,6 +235,57 @@ int  hcx_index_export(const hcx_index *idx, uint32_t *iA, uint32_t *bA);
 /* last build's device time in ms (fragment generation + radix sort + bA scan) */
 int  hcx_index_build_ms(const hcx_index *idx, float *ms);
 int  hcx_index_free(hcx_index *idx);
+
+/* ---- peptide database: LBE + MODS on the device ----
+ * One hcx_pepdb per peptide length, as the reference keeps one Index per
+ * length (a FASTA file of equal-length peptides each).  Masses use the
+ * context's residue table and mod masses (hcx_params.mod_mass, nmods);
+ * min_mass / max_mass are the legality bounds of LBE_CountPeps.          */
+#define HCX_POLICY_CHUNK  0   /* DistPolicy _chunk  (slm_dsts.h:33-39)              */
+#define HCX_POLICY_CYCLIC 1   /* DistPolicy _cyclic, gParams' default (slm_dsts.h:416) */
+
+typedef struct hcx_lbe_cfg {
+    const char *modconditions; /* gParams.modconditions, "limit AAs n [AAs n ...]" as
+                                  hicops.cpp:636-666 builds it; NULL or a limit of 0:
+                                  no variants.  At most 8 mods per form (4-bit
+                                  modNum nibbles), residues A-Z, each named once,
+                                  types 1..nmods of the context.                      */
+    uint32_t nodes;            /* ranks sharing the database (gParams.nodes)          */
+    uint32_t myid;             /* this rank (gParams.myid)                             */
+    int32_t  policy;           /* HCX_POLICY_*; nodes > 1 needs HCX_POLICY_CYCLIC     */
+    int32_t  sort;             /* gParams.dM > 0: sort by mass (lbe_internal.cpp:184-187) */
+} hcx_lbe_cfg;
+
+typedef struct hcx_pepdb hcx_pepdb;
+/* LBE_CountPeps (lbe_internal.cpp:369-467) with MODS_Initialize (mods.cpp:63-101)
+ * and MODS_ModCounter (mods.cpp:397-431): lines holds nlines sequences of peplen
+ * characters back to back (lower case is upper-cased, as std::toupper does); the
+ * mass-legal ones are kept in order and their variants counted on the device.
+ * Returns when the counts are known. */
+int  hcx_pepdb_create(hcx_ctx *ctx, const hcx_lbe_cfg *cfg, uint32_t peplen, const char *lines,
+                      uint32_t nlines, hcx_pepdb **out);
+/* counts[6]: pepCount, modCount, totalCount, lclpepCnt, lclmodCnt, lcltotCnt
+ * (the Index fields, slm_dsts.h:322-357; LBE_CreatePartitions lbe_internal.cpp:308-352) */
+int  hcx_pepdb_counts(const hcx_pepdb *db, uint32_t *counts);
+/* LBE_Initialize (lbe_internal.cpp:119-206) for n lengths at once: LBE_GeneratePeps
+ * (lbe_internal.cpp:209-232), MODS_GenerateMods (mods.cpp:447-496, each sequence's
+ * variants in MODS_ModList order, mods.cpp:333-384, then glibc qsort with
+ * cmpvarEntries), and std::sort by mass with CmpPepEntries -- the tie order of
+ * libstdc++'s introsort, reproduced exactly. */
+int  hcx_pepdb_generate(hcx_pepdb *const *dbs, uint32_t n);
+/* seqs: pepCount * peplen characters (Index.pepIndex.seqs), entries: lcltotCnt
+ * (Index.pepEntries); either may be NULL. */
+int  hcx_pepdb_export(const hcx_pepdb *db, char *seqs, hcx_pep_entry *entries);
+/* device time of hcx_pepdb_create (upload, filter, counts) and of the
+ * hcx_pepdb_generate call that produced db (all its lengths together) */
+int  hcx_pepdb_ms(const hcx_pepdb *db, float *create_ms, float *generate_ms);
+int  hcx_pepdb_free(hcx_pepdb *db);
+/* DSLIM_Construct (dslim.cpp:46-138) over a generated database, entries and
+ * sequences taken on the device (no host round trip). */
+int  hcx_index_build_pepdb(hcx_ctx *ctx, const hcx_pepdb *db, hcx_index **out);
+/* The device mass sort alone on n (key, value) pairs: the order libstdc++'s
+ * std::sort leaves pepEntries in when compared by key alone (test hook). */
+int  hcx_sort_mass(hcx_ctx *ctx, float *keys, uint32_t *vals, uint32_t n);
 
 /* ---- query batches ---- */
 int  hcx_queries_upload(hcx_ctx *ctx, const hcx_queries_view *q, hcx_qbatch **out);

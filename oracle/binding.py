@@ -274,3 +274,70 @@ def pickpeaks(raw, qalen: int, base_int: int, min_int: int):
     k = int(q.idx[n]) if n else 0
     q.moz, q.intensity = q.moz[:k].copy(), q.intensity[:k].copy()
     return q
+
+
+# ---- peptide database stage (LBE + MODS) ----
+
+def lbe_count(seq: str, modconditions: str) -> int:
+    """mods.cpp:270-296 count(): forms of one sequence, unmodified included."""
+    b = seq.encode()
+    out = C.c_uint64()
+    rc = lib().orc_lbe_count(b, C.c_uint32(len(b)), modconditions.encode(), C.byref(out))
+    if rc != 0:
+        raise RuntimeError(f"orc_lbe_count failed: {rc}")
+    return out.value
+
+
+def lbe_filter(lines: np.ndarray, L: int, min_mass: int, max_mass: int) -> np.ndarray:
+    """LBE_CountPeps' upper-casing and mass filter: the kept sequences."""
+    lines = np.ascontiguousarray(lines, np.uint8)
+    n = len(lines) // L
+    out = np.zeros(n * L, np.uint8)
+    k = C.c_uint32()
+    rc = lib().orc_lbe_filter(N.ptr(lines), C.c_uint32(n), C.c_uint32(L), C.c_uint32(min_mass),
+                              C.c_uint32(max_mass), N.ptr(out), C.byref(k))
+    if rc != 0:
+        raise RuntimeError(f"orc_lbe_filter failed: {rc}")
+    return out[:k.value * L].copy()
+
+
+def lbe_generate(params: N.Params, seqs: np.ndarray, L: int, modconditions: str | None, nodes: int = 1,
+                 myid: int = 0, policy: int = N.POLICY_CYCLIC, sort: bool = True):
+    """LBE_Initialize of one length: (pepEntries, (pepCount, modCount, lclpep, lclmod))."""
+    seqs = np.ascontiguousarray(seqs, np.uint8)
+    n = len(seqs) // L
+    mc = (modconditions if modconditions is not None else "0 M 0").encode()
+    cnt = np.zeros(4, np.uint32)
+    args = (C.byref(params), N.ptr(seqs), C.c_uint32(n), C.c_uint32(L), mc, C.c_uint32(nodes), C.c_uint32(myid),
+            C.c_int(policy), C.c_int(int(bool(sort))))
+    rc = lib().orc_lbe_generate(*args, None, N.ptr(cnt))
+    if rc != 0:
+        raise RuntimeError(f"orc_lbe_generate failed: {rc}")
+    out = np.zeros(int(cnt[2]) + int(cnt[3]), N.PEP_DTYPE)
+    rc = lib().orc_lbe_generate(*args, N.ptr(out), N.ptr(cnt))
+    if rc != 0:
+        raise RuntimeError(f"orc_lbe_generate failed: {rc}")
+    return out, tuple(int(x) for x in cnt)
+
+
+def std_sort_mass(keys: np.ndarray, vals: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """std::sort of (key, value) records by key, as LBE_Initialize sorts pepEntries."""
+    k = np.ascontiguousarray(keys, np.float32).copy()
+    v = np.ascontiguousarray(vals, np.uint32).copy()
+    lib().orc_std_sort_mass(N.ptr(k), N.ptr(v), C.c_uint32(len(k)))
+    return k, v
+
+
+def qsort_sites(sites: np.ndarray, vals: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """glibc qsort with cmpvarEntries on (sites, value) records."""
+    s = np.ascontiguousarray(sites, np.uint64).copy()
+    v = np.ascontiguousarray(vals, np.uint32).copy()
+    lib().orc_qsort_sites(N.ptr(s), N.ptr(v), C.c_uint32(len(s)))
+    return s, v
+
+
+def antiqsort(n: int) -> np.ndarray:
+    """Keys on which std::sort falls back to its heap sort (McIlroy's adversary)."""
+    k = np.zeros(n, np.float32)
+    lib().orc_antiqsort(C.c_uint32(n), N.ptr(k))
+    return k

@@ -139,6 +139,35 @@ int orc_pickpeaks(const uint32_t *raw_mz, const uint32_t *raw_in, const uint32_t
                   uint32_t *moz, uint32_t *intensity, uint32_t *idx,
                   float *precurse, int32_t *charges, float *rtimes);
 
+/* ---- peptide database stage (hcx_oracle_lbe.cpp) ---- */
+
+/* mods.cpp:270-296 count() with MODS_Initialize's parse (mods.cpp:63-101):
+ * forms of one sequence, the unmodified one included.  Pinned: the number of
+ * forms MODS_ModList emits plus one, counted by brute force over all site
+ * subsets (tests/test_oracle_pins.py). */
+int orc_lbe_count(const char *seq, uint32_t len, const char *modconditions, uint64_t *forms);
+/* lbe_internal.cpp:397-418: upper case, keep lines whose float mass lies in
+ * [min_mass, max_mass].  Pinned: published peptide masses at the bounds. */
+int orc_lbe_filter(const char *lines, uint32_t nlines, uint32_t len, uint32_t min_mass, uint32_t max_mass,
+                   char *seqs, uint32_t *nseqs);
+/* LBE_Initialize (lbe_internal.cpp:119-206) of one length after
+ * LBE_CountPeps: MODS_ModCounter, LBE_CreatePartitions, LBE_GeneratePeps,
+ * MODS_GenerateMods (MODS_ModList order, glibc qsort with cmpvarEntries) and,
+ * when sort != 0, std::sort by mass.  counts: pepCount, modCount, lclpepCnt,
+ * lclmodCnt; out (lclpep + lclmod entries) may be NULL to get the counts.
+ * Pinned: hand-traced variant lists and orders (tests/test_oracle_pins.py);
+ * the sorts are the library calls the reference makes. */
+int orc_lbe_generate(const hcx_params *p, const char *seqs, uint32_t nseqs, uint32_t len,
+                     const char *modconditions, uint32_t nodes, uint32_t myid, int policy, int sort,
+                     hcx_pep_entry *out, uint32_t *counts);
+/* std::sort with CmpPepEntries on (key, value) records compared by key. */
+void orc_std_sort_mass(float *keys, uint32_t *vals, uint32_t n);
+/* glibc qsort with cmpvarEntries on (sites, value) records. */
+void orc_qsort_sites(uint64_t *sites, uint32_t *vals, uint32_t n);
+/* McIlroy's adversary against std::sort: n keys that drive it to its
+ * heap-sort fallback. */
+void orc_antiqsort(uint32_t n, float *keys);
+
 #ifdef __cplusplus
 }
 #endif
