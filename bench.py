@@ -54,6 +54,8 @@ def main() -> int:
                     help="one batched search, then a 1 GiB device copy (run under rocprofv3 --pmc)")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="turn two rocprofv3 --pmc runs of --pmc-probe into the traffic figure")
+    ap.add_argument("--pepdb-only", action="store_true",
+                    help="time the peptide database stage (LBE + MODS) alone and print its JSON")
     args = ap.parse_args()
 
     if args.traffic:
@@ -87,6 +89,11 @@ def main() -> int:
     prm = W.make_params(p)
 
     eng = Engine(prm, chunk=args.chunk)
+    if args.pepdb_only:
+        print(json.dumps({"preset": p.name, "pepdb": pepdb_stage(eng, db, p, prm, args.steps, args.warmup,
+                                                                 not args.no_cpu)}), flush=True)
+        eng.close()
+        return 0
     t0 = time.perf_counter()
     idx = eng.construct_db(db)
     build_wall = time.perf_counter() - t0
@@ -198,6 +205,7 @@ def main() -> int:
     if not args.no_cpu:
         cpu = cpu_baseline(prm, db, q, st, args.cpu_seconds)
     pick = queries_pick(eng, raw, p, args.steps, args.warmup, not args.no_cpu)
+    pdb = pepdb_stage(eng, db, p, prm, args.steps, args.warmup, not args.no_cpu)
 
     if sur_info is not None:
         sur_info["surrogate_evalue_ms"] = round(runs["batched+surrogate"]["ems"], 3)
@@ -244,6 +252,7 @@ def main() -> int:
                         "ions_per_s": round(nions / (build_ms / 1e3), 1),
                         "frac_hbm": round(b_build / (build_ms / 1e3) / HBM_PEAK, 4)},
         "queries_pick": pick,
+        "pepdb": pdb,
         "psms_valid": valid,
         "tied_spectra": tied,
         "host_fits": int(((res["flags"] & N.RES_FIT_HOST) != 0).sum()),
@@ -252,6 +261,76 @@ def main() -> int:
     print(json.dumps(line), flush=True)
     eng.close()
     return 0
+
+
+def pepdb_stage(eng, db, p, prm, steps: int, warmup: int, with_cpu: bool) -> dict:
+    """The peptide database stage (LBE_CountPeps + LBE_Initialize: mass
+    filter, variant counts, entries, the exact std::sort order) for every
+    length of the benchmark database, from sequence lines to sorted
+    pepEntries in HBM.  Device time is the filter/count pass of each length
+    plus one generate call over all lengths, with the lines in HBM; the
+    lines' upload over PCIe is reported beside it.
+    Algorithmic bytes: the sequence characters read once, 24-byte entries
+    written once."""
+    from hicops_amd import workload as W
+    mc = W.modconditions(p)
+    lines = {L: db.seqs(L) for L in db.lengths}
+
+    def one():
+        t0 = time.perf_counter()
+        dbs = [eng.count_peps(L, lines[L], mc) for L in db.lengths]
+        eng.initialize(dbs)
+        wall = time.perf_counter() - t0
+        um = sum(d.timing()[0] for d in dbs)
+        cm = sum(d.timing()[1] for d in dbs)
+        gm = dbs[0].timing()[2]
+        forms = sum(d.lcltot for d in dbs)
+        for d in dbs:
+            d.free()
+        return wall, cm, gm, forms, um
+
+    for _ in range(warmup):
+        one()
+    rs = [one() for _ in range(steps)]
+    wall = float(np.median([r[0] for r in rs]))
+    cm = float(np.median([r[1] for r in rs]))
+    gm = float(np.median([r[2] for r in rs]))
+    um = float(np.median([r[4] for r in rs]))
+    forms = rs[0][3]
+    chars = sum(len(v) for v in lines.values())
+    nbytes = chars + 24 * forms
+    dev_ms = cm + gm
+    log(f"pepdb: {forms} forms over {len(lines)} lengths; device {dev_ms:.2f} ms "
+        f"(filter/count {cm:.2f} + generate/sort {gm:.2f}), upload {um:.2f} ms, wall {wall * 1e3:.1f} ms")
+    out = {"forms": forms, "lengths": len(lines), "modconditions": mc, "device_ms": round(dev_ms, 3),
+           "count_ms": round(cm, 3), "generate_ms": round(gm, 3), "upload_ms": round(um, 3),
+           "with_upload_forms_per_s": round(forms / ((dev_ms + um) / 1e3), 1), "wall_ms": round(wall * 1e3, 2),
+           "forms_per_s": round(forms / (dev_ms / 1e3), 1),
+           "roofline": {"bound": "hbm", "achieved": round(nbytes / (dev_ms / 1e3) / 1e9, 2),
+                        "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                        "frac": round(nbytes / (dev_ms / 1e3) / HBM_PEAK, 5), "algorithmic_bytes": nbytes}}
+    if with_cpu:
+        from concurrent.futures import ThreadPoolExecutor
+        from oracle import binding as O
+        threads = cpu_facts()["threads"]
+
+        def orc(L):
+            s = O.lbe_filter(lines[L], L, prm.min_mass, prm.max_mass)
+            return len(O.lbe_generate(prm, s, L, mc)[0])
+
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            with ThreadPoolExecutor(threads) as ex:
+                n = sum(ex.map(orc, db.lengths))
+            ts.append(time.perf_counter() - t0)
+        t = float(np.median(ts))
+        assert n == forms
+        out["cpu_baseline"] = {"value": round(forms / t, 1), "unit": "forms/s", "cores": threads, "kind": "port",
+                               "sample": "the whole database: the oracle's LBE (filter, counts, variants, glibc "
+                                         "qsort, std::sort) per length, lengths side by side; median of 3",
+                               "runs_s": [round(x, 3) for x in ts]}
+    return out
 
 
 def queries_pick(eng, raw, p, steps: int, warmup: int, with_cpu: bool) -> dict:

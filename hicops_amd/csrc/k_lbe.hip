@@ -815,7 +815,7 @@ struct hcx_pepdb {
     hcx::LbeMods mods{};
     uint32_t peplen = 0, nlines = 0, nseqs = 0, mod_count = 0, lclpep = 0, lclmod = 0;
     bool generated = false;
-    float create_ms = 0.0f, generate_ms = 0.0f;
+    float upload_ms = 0.0f, count_ms = 0.0f, generate_ms = 0.0f;
     hcx::DeviceBuf seqs, var_off, entries, mass;
 };
 
@@ -918,7 +918,7 @@ extern "C" int hcx_pepdb_create(hcx_ctx *c, const hcx_lbe_cfg *cfg, uint32_t pep
         return fail(set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_pepdb_create: device allocation failed"));
     if (hipEventRecord(c->ev2, st) != hipSuccess) return fail(set_error("event"));
     if ((nchars && hipMemcpyAsync(dl.p, lines, nchars, hipMemcpyHostToDevice, st) != hipSuccess) ||
-        hipMemsetAsync(err.p, 0, 64, st) != hipSuccess)
+        hipMemsetAsync(err.p, 0, 64, st) != hipSuccess || hipEventRecord(c->ev3, st) != hipSuccess)
         return fail(set_error("hcx_pepdb_create: upload failed"));
     hcx::k_lbe_filter<<<(nlines + 256) / 256, 256, 0, st>>>((char *)dl.p, nlines, peplen, (float)c->params.min_mass,
                                                             (float)c->params.max_mass, (uint32_t *)flag.p,
@@ -960,7 +960,8 @@ extern "C" int hcx_pepdb_create(hcx_ctx *c, const hcx_lbe_cfg *cfg, uint32_t pep
     db->lclmod = db->mod_count / p + ((db->mod_count % p) > id ? 1u : 0u);
     if ((uint64_t)db->lclpep + db->lclmod > 0xFFFFFFF0ull)
         return fail(set_error_code(HCX_ERR_INVLD_SIZE, "hcx_pepdb_create: more than 2^32 forms"));
-    (void)hipEventElapsedTime(&db->create_ms, c->ev2, c->ev1);
+    (void)hipEventElapsedTime(&db->upload_ms, c->ev2, c->ev3);
+    (void)hipEventElapsedTime(&db->count_ms, c->ev3, c->ev1);
     *out = db;
     return HCX_SUCCESS;
 }
@@ -1074,9 +1075,10 @@ extern "C" int hcx_pepdb_export(const hcx_pepdb *db, char *seqs, hcx_pep_entry *
     return HCX_SUCCESS;
 }
 
-extern "C" int hcx_pepdb_ms(const hcx_pepdb *db, float *create_ms, float *generate_ms) {
+extern "C" int hcx_pepdb_ms(const hcx_pepdb *db, float *upload_ms, float *count_ms, float *generate_ms) {
     if (!db) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_pepdb_ms: null");
-    if (create_ms) *create_ms = db->create_ms;
+    if (upload_ms) *upload_ms = db->upload_ms;
+    if (count_ms) *count_ms = db->count_ms;
     if (generate_ms) *generate_ms = db->generate_ms;
     return HCX_SUCCESS;
 }

@@ -6,6 +6,7 @@
 // (dslim.cpp:390-437).  Device indices live in a side table keyed by Index*.
 #include "../../include/hcx_dslim.hpp"
 
+#include <algorithm>
 #include <cstddef>
 #include <ctime>
 #include <fstream>
@@ -23,10 +24,16 @@ static_assert(sizeof(pepEntry) == sizeof(hcx_pep_entry), "pepEntry and hcx_pep_e
 
 namespace {
 
+struct PepDbRef {
+    hcx_pepdb *db = nullptr;
+    const pepEntry *entries = nullptr;   // the host array LBE_Initialize filled
+};
+
 struct State {
     std::mutex mu;
     hcx_ctx *ctx = nullptr;
     std::unordered_map<const Index *, hcx_index *> idx;
+    std::unordered_map<const Index *, PepDbRef> pdb;   // LBE_CountPeps .. LBE_Deinitialize
     uint_t max_hyperscore = MAX_HYPERSCORE;
     uint_t max_shdpeaks = MAX_SHDPEAKS;
     int_t kernel = HCX_KERNEL_BATCHED;
@@ -204,6 +211,164 @@ status_t DSLIM_SetKernel(int_t kernel) {
 
 hcx_ctx *DSLIM_Context() { return st().ctx; }
 
+namespace {
+
+// DistPolicy -> the C ABI's policy codes
+int32_t policy_code() { return params.policy == _chunk ? HCX_POLICY_CHUNK : HCX_POLICY_CYCLIC; }
+
+}  // namespace
+
+// mods.cpp:63-101: the mod conditions are parsed (and checked) where they are
+// used, by hcx_pepdb_create; here only that params.modconditions reads.
+status_t MODS_Initialize() {
+    State &s = st();
+    std::lock_guard<std::mutex> g(s.mu);
+    status_t rc = ensure_ctx();
+    if (rc != SLM_SUCCESS) return rc;
+    const std::string mc = params.modconditions.empty() ? std::string("0 M 0") : params.modconditions;
+    hcx_lbe_cfg cfg = {mc.c_str(), 1, 0, HCX_POLICY_CYCLIC, 1};
+    const char probe[] = "GGGGGGGGGK";
+    hcx_pepdb *db = nullptr;
+    rc = hcx_pepdb_create(s.ctx, &cfg, 10, probe, 1, &db);
+    if (db) hcx_pepdb_free(db);
+    return rc;
+}
+
+// lbe_internal.cpp:369-467: read one length's peptides (lines not starting
+// with '>', a trailing '\r' dropped), keep the mass-legal ones and count their
+// variants (on the device).  A line of another length is ERR_INVLD_SIZE, as
+// in the reference (which reads an empty line out of range instead).
+status_t LBE_CountPeps(char *filename, Index *index, uint_t explen) {
+    if (!filename || !index) return ERR_INVLD_PTR;
+    if (explen < 2 || explen > 60) return ERR_INVLD_SIZE;
+    std::ifstream f(filename);
+    if (!f.is_open()) return ERR_INVLD_PARAM;
+    std::vector<char> lines;
+    std::string line;
+    uint_t n = 0;
+    while (std::getline(f, line)) {
+        if (!line.empty() && line[0] == '>') continue;
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        if (line.size() != explen) return ERR_INVLD_SIZE;
+        lines.insert(lines.end(), line.begin(), line.end());
+        n++;
+    }
+    State &s = st();
+    std::lock_guard<std::mutex> g(s.mu);
+    status_t rc = ensure_ctx();
+    if (rc != SLM_SUCCESS) return rc;
+    const std::string mc = params.modconditions.empty() ? std::string("0 M 0") : params.modconditions;
+    hcx_lbe_cfg cfg = {mc.c_str(), params.nodes ? params.nodes : 1, params.myid, policy_code(), params.dM > 0.0};
+    hcx_pepdb *db = nullptr;
+    rc = hcx_pepdb_create(s.ctx, &cfg, explen, lines.data(), n, &db);
+    if (rc != SLM_SUCCESS) return rc;
+    auto it = s.pdb.find(index);
+    if (it != s.pdb.end()) hcx_pepdb_free(it->second.db);
+    s.pdb[index] = PepDbRef{db, nullptr};
+    uint32_t c[6];
+    hcx_pepdb_counts(db, c);
+    index->pepCount = c[0];
+    index->modCount = c[1];
+    index->totalCount = c[2];
+    index->pepIndex.peplen = (ushort_t)explen;
+    index->pepIndex.AAs = c[0] * explen;
+    return SLM_SUCCESS;
+}
+
+// lbe_internal.cpp:308-352
+status_t LBE_CreatePartitions(Index *index) {
+    if (!index) return ERR_INVLD_PTR;
+    State &s = st();
+    std::lock_guard<std::mutex> g(s.mu);
+    auto it = s.pdb.find(index);
+    if (it == s.pdb.end()) return ERR_INVLD_PARAM;   // LBE_CountPeps first
+    uint32_t c[6];
+    hcx_pepdb_counts(it->second.db, c);
+    index->lclpepCnt = c[3];
+    index->lclmodCnt = c[4];
+    index->lcltotCnt = c[5];
+    return SLM_SUCCESS;
+}
+
+// lbe_internal.cpp:119-206: the entries of this rank's share, sorted by mass
+// when params.dM > 0, into new host arrays as LBE_AllocateMem makes them.
+status_t LBE_Initialize(Index *index) {
+    if (!index) return ERR_INVLD_PTR;
+    State &s = st();
+    std::lock_guard<std::mutex> g(s.mu);
+    auto it = s.pdb.find(index);
+    if (it == s.pdb.end()) return ERR_INVLD_PARAM;
+    hcx_pepdb *db = it->second.db;
+    uint32_t c[6];
+    hcx_pepdb_counts(db, c);
+    if (c[0] == 0) return ERR_INVLD_PARAM;             // lbe_internal.cpp:134-141
+    status_t rc = hcx_pepdb_generate(&db, 1);
+    if (rc != SLM_SUCCESS) return rc;
+    AA *seqs = new (std::nothrow) AA[(size_t)c[0] * index->pepIndex.peplen];
+    pepEntry *ents = new (std::nothrow) pepEntry[c[5] ? c[5] : 1];
+    if (!seqs || !ents) {
+        delete[] seqs;
+        delete[] ents;
+        return ERR_BAD_MEM_ALLOC;
+    }
+    rc = hcx_pepdb_export(db, seqs, reinterpret_cast<hcx_pep_entry *>(ents));
+    if (rc != SLM_SUCCESS) {
+        delete[] seqs;
+        delete[] ents;
+        return rc;
+    }
+    index->pepIndex.seqs = seqs;
+    index->pepEntries = ents;
+    index->lclpepCnt = c[3];
+    index->lclmodCnt = c[4];
+    index->lcltotCnt = c[5];
+    it->second.entries = ents;
+    return SLM_SUCCESS;
+}
+
+// lbe_internal.cpp:256-295: the reference's chunk arithmetic (MAX_IONS
+// 0x80000000 ions per chunk, a scorecard of spadmem bytes per thread of
+// 24-byte BYC cells).  The device index keeps one chunk per length whatever
+// these fields say.
+status_t LBE_Distribute(Index *index) {
+    if (!index) return ERR_INVLD_PTR;
+    const uint_t N = index->lcltotCnt;
+    const uint_t speclen = (index->pepIndex.peplen - 1) * params.maxz * 2;
+    if (speclen == 0 || N == 0) return ERR_INVLD_PARAM;
+    const uint_t maxchunk = 0x80000000u / speclen;
+    const uint_t maxchunk2 = params.spadmem / (24u * (params.threads ? params.threads : 1));
+    uint_t chunksize = std::min(N, maxchunk);
+    chunksize = std::min(chunksize, maxchunk2);
+    if (chunksize == 0) return ERR_INVLD_PARAM;
+    uint_t nchunks = N / chunksize + ((N % chunksize) != 0);
+    index->nChunks = nchunks;
+    index->chunksize = chunksize;
+    index->lastchunksize = (N % chunksize == 0) ? chunksize : N - chunksize * (N / chunksize);
+    return SLM_SUCCESS;
+}
+
+// lbe_internal.cpp:244 (DSLIM_Deinitialize, dslim.cpp:644-720): the ion
+// index, the peptide arrays and the device database of this Index.
+status_t LBE_Deinitialize(Index *index) {
+    if (!index) return ERR_INVLD_PTR;
+    status_t rc = DSLIM_DeallocateIonIndex(index);
+    State &s = st();
+    std::lock_guard<std::mutex> g(s.mu);
+    auto it = s.pdb.find(index);
+    if (it != s.pdb.end()) {
+        hcx_pepdb_free(it->second.db);
+        s.pdb.erase(it);
+    }
+    delete[] index->pepEntries;
+    index->pepEntries = nullptr;
+    delete[] index->pepIndex.seqs;
+    index->pepIndex.seqs = nullptr;
+    index->pepCount = index->modCount = index->totalCount = 0;
+    index->pepIndex.AAs = 0;
+    index->pepIndex.peplen = 0;
+    return rc;
+}
+
 // dslim.cpp:46-138: build the ion index of one peptide length from its
 // sequences and mass-sorted pepEntries.
 status_t DSLIM_Construct(Index *index) {
@@ -230,7 +395,11 @@ status_t DSLIM_Construct(Index *index) {
         s.idx.erase(it);
     }
     hcx_index *ix = nullptr;
-    rc = hcx_index_build(s.ctx, L, index->pepIndex.seqs, index->pepIndex.AAs / L, e.data(), n, &ix);
+    auto pd = s.pdb.find(index);
+    if (pd != s.pdb.end() && pd->second.entries == index->pepEntries && params.dM > 0.0)
+        rc = hcx_index_build_pepdb(s.ctx, pd->second.db, &ix);   // the entries LBE_Initialize left on the device
+    else
+        rc = hcx_index_build(s.ctx, L, index->pepIndex.seqs, index->pepIndex.AAs / L, e.data(), n, &ix);
     if (rc != SLM_SUCCESS) return rc;
     s.idx[index] = ix;
     return SLM_SUCCESS;
@@ -362,6 +531,8 @@ status_t DSLIM_DeallocateSC() {
     std::lock_guard<std::mutex> g(s.mu);
     for (auto &kv : s.idx) hcx_index_free(kv.second);
     s.idx.clear();
+    for (auto &kv : s.pdb) hcx_pepdb_free(kv.second.db);
+    s.pdb.clear();
     if (s.ctx) hcx_destroy(s.ctx);
     s.ctx = nullptr;
     s.spectrumID = 0;

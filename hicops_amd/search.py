@@ -94,11 +94,11 @@ class PepDb:
         (self.pep_count, self.mod_count, self.total_count, self.lclpep, self.lclmod,
          self.lcltot) = (int(x) for x in cnt)
 
-    def timing(self) -> tuple[float, float]:
-        """(create ms, generate ms) on the device."""
-        a, b = C.c_float(), C.c_float()
-        N.check(N.hcx().hcx_pepdb_ms(self._h, C.byref(a), C.byref(b)))
-        return a.value, b.value
+    def timing(self) -> tuple[float, float, float]:
+        """(upload ms, filter/count ms, generate ms) on the device."""
+        a, b, g = C.c_float(), C.c_float(), C.c_float()
+        N.check(N.hcx().hcx_pepdb_ms(self._h, C.byref(a), C.byref(b), C.byref(g)))
+        return a.value, b.value, g.value
 
     def export(self) -> tuple[np.ndarray, np.ndarray]:
         """(kept sequences as pep_count * peplen bytes, pepEntries)."""

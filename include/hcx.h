@@ -276,9 +276,10 @@ int  hcx_pepdb_generate(hcx_pepdb *const *dbs, uint32_t n);
 /* seqs: pepCount * peplen characters (Index.pepIndex.seqs), entries: lcltotCnt
  * (Index.pepEntries); either may be NULL. */
 int  hcx_pepdb_export(const hcx_pepdb *db, char *seqs, hcx_pep_entry *entries);
-/* device time of hcx_pepdb_create (upload, filter, counts) and of the
- * hcx_pepdb_generate call that produced db (all its lengths together) */
-int  hcx_pepdb_ms(const hcx_pepdb *db, float *create_ms, float *generate_ms);
+/* device time of hcx_pepdb_create (the lines' upload over PCIe, then the
+ * filter and counts in HBM) and of the hcx_pepdb_generate call that produced
+ * db (all its lengths together); any pointer may be NULL */
+int  hcx_pepdb_ms(const hcx_pepdb *db, float *upload_ms, float *count_ms, float *generate_ms);
 int  hcx_pepdb_free(hcx_pepdb *db);
 /* DSLIM_Construct (dslim.cpp:46-138) over a generated database, entries and
  * sequences taken on the device (no host round trip). */

@@ -10,6 +10,12 @@
 //   <dir>/lengths.txt  one peptide length per line
 //   <dir>/seqs_<L>.bin, <dir>/ents_<L>.bin   sequences, pepEntry[24 B]
 //   <dir>/moz.bin inten.bin idx.bin prec.bin chg.bin   CSR spectra
+// With "lbe 1" in params.txt the database comes from the peptide database
+// stage instead: <dir>/fasta_<L>.txt (one peptide per line, '>' headers) and
+// <dir>/mods.txt (params.modconditions) through MODS_Initialize,
+// LBE_CountPeps, LBE_CreatePartitions, LBE_Initialize and LBE_Distribute, as
+// hicops.cpp:200-260 calls them; <dir>/lbe_ents_<L>.bin and lbe_seqs_<L>.bin
+// receive the resulting pepEntries and sequences.
 // writes <dir>/psms.txt: specid psid idxoffset sharedions totalions
 //                        hyperscore-bits e_x npsms pchg fileIndex
 // Built with -DHCX_TSV_DRIVER (tests/cpp/tsv_driver) it instead leaves the
@@ -106,7 +112,33 @@ int main(int argc, char **argv) {
     std::vector<Index> index(lengths.size());
     std::vector<std::vector<char>> seqs(lengths.size());
     std::vector<std::vector<pepEntry>> ents(lengths.size());
-    for (size_t i = 0; i < lengths.size(); i++) {
+    const bool lbe = kv.count("lbe") && kv["lbe"] != 0;
+    if (lbe) {
+        std::ifstream mf(dir + "/mods.txt");
+        std::getline(mf, params.modconditions);
+        params.nodes = 1;
+        params.myid = 0;
+        check(MODS_Initialize(), "MODS_Initialize");
+        for (size_t i = 0; i < lengths.size(); i++) {
+            const std::string L = std::to_string(lengths[i]);
+            Index &ix = index[i];
+            std::memset(&ix, 0, sizeof ix);
+            std::string fasta = dir + "/fasta_" + L + ".txt";
+            check(LBE_CountPeps(&fasta[0], &ix, (uint_t)lengths[i]), "LBE_CountPeps");
+            check(LBE_CreatePartitions(&ix), "LBE_CreatePartitions");
+            check(LBE_Initialize(&ix), "LBE_Initialize");
+            check(LBE_Distribute(&ix), "LBE_Distribute");
+            FILE *f = std::fopen((dir + "/lbe_ents_" + L + ".bin").c_str(), "wb");
+            std::fwrite(ix.pepEntries, sizeof(pepEntry), ix.lcltotCnt, f);
+            std::fclose(f);
+            f = std::fopen((dir + "/lbe_seqs_" + L + ".bin").c_str(), "wb");
+            std::fwrite(ix.pepIndex.seqs, 1, ix.pepIndex.AAs, f);
+            std::fclose(f);
+            ents[i].assign(ix.pepEntries, ix.pepEntries + ix.lcltotCnt);
+            check(DSLIM_Construct(&ix), "DSLIM_Construct");
+        }
+    }
+    for (size_t i = 0; i < lengths.size() && !lbe; i++) {
         const std::string L = std::to_string(lengths[i]);
         seqs[i] = load<char>(dir + "/seqs_" + L + ".bin");
         ents[i] = load<pepEntry>(dir + "/ents_" + L + ".bin");
@@ -179,7 +211,7 @@ int main(int argc, char **argv) {
         std::fwrite(index[0].ionIndex[0].bA, 4, (size_t)params.max_mass * params.scale + 1, f);
         std::fclose(f);
     }
-    for (auto &ix : index) check(DSLIM_DeallocateIonIndex(&ix), "DSLIM_DeallocateIonIndex");
+    for (auto &ix : index) check(lbe ? LBE_Deinitialize(&ix) : DSLIM_DeallocateIonIndex(&ix), "deallocate");
 
     // expeRT::ModelTailFit on a histogram given as survival counts
     {

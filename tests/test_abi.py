@@ -150,7 +150,8 @@ def test_datagen_mod_variants():
 
 # The reference's own C++ entry points, by their mangled names, so that an
 # application built against source/core/include/dslim.h and expeRT.h links to
-# libhcx_dslim.so (dslim.h:49,53,149,164,184; expeRT.h:143; dslim_fileout.h:29).
+# libhcx_dslim.so (dslim.h:49,53,149,164,184; expeRT.h:143; dslim_fileout.h:29;
+# the peptide database stage of lbe.h and mods.h).
 REFERENCE_SYMBOLS = [
     "_Z15DSLIM_ConstructP6_Index",
     "_Z25DSLIM_InitializeScorecardP6_Indexj",
@@ -162,6 +163,13 @@ REFERENCE_SYMBOLS = [
     "_Z15DFile_InitFilesv",
     "_Z17DFile_DeinitFilesv",
     "_Z19DFile_PrintPartialsjP8_Results",
+    # lbe.h:45,74,90,92; mods.h:63; lbe_internal.cpp:244
+    "_Z14LBE_InitializeP6_Index",
+    "_Z14LBE_DistributeP6_Index",
+    "_Z13LBE_CountPepsPcP6_Indexj",
+    "_Z20LBE_CreatePartitionsP6_Index",
+    "_Z16LBE_DeinitializeP6_Index",
+    "_Z15MODS_Initializev",
 ]
 
 

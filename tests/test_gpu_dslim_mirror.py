@@ -131,3 +131,68 @@ def test_tsv_writer_matches_reference_format(tmp_path):
         assert abs(float(f[11]) - o["hyperscore"]) <= 1e-5 * abs(o["hyperscore"]) + 5e-7, s
         assert abs(float(f[12]) - o["e_x"]) <= 1e-3 * abs(o["e_x"]) + 5e-7, s
         assert f[13] == str(int(o["cpsms"])), s
+
+
+class _LbeDb:
+    """The oracle's database (sequences and pepEntries per length) in the
+    shape OracleDB reads."""
+
+    def __init__(self, lengths, tables):
+        self.lengths = lengths
+        self.tables = tables
+
+    def seqs(self, L):
+        return self.tables[L][0]
+
+    def entries(self, L):
+        return self.tables[L][1]
+
+
+def test_lbe_entry_points_match_oracle(tmp_path):
+    # MODS_Initialize, LBE_CountPeps (FASTA per length), LBE_CreatePartitions,
+    # LBE_Initialize, LBE_Distribute, then DSLIM_Construct from the entries the
+    # stage left on the device, DSLIM_QuerySpectrum, LBE_Deinitialize
+    p = W.scaled("rank_share", npep=3000, nspec=200)
+    db = W.Database(p)
+    q = W.make_queries(db, p)
+    prm = W.make_params(p)
+    mc = W.modconditions(p)
+    write_case(tmp_path, p, db, q, prm, N.KERNEL_BATCHED)
+    with open(tmp_path / "params.txt", "a") as f:
+        f.write("lbe 1\n")
+    (tmp_path / "mods.txt").write_text(mc + "\n")
+    rng = np.random.default_rng(3)
+    for L in db.lengths:
+        s = db.seqs(L).reshape(-1, L)
+        with open(tmp_path / f"fasta_{L}.txt", "wb") as f:
+            for i, row in enumerate(s):
+                line = bytes(row)
+                if rng.random() < 0.2:
+                    line = line.lower()
+                f.write(b">p%d\n" % i + line + (b"\r\n" if i % 3 == 0 else b"\n"))
+    print("test_lbe_entry_points_match_oracle", flush=True)
+    r = subprocess.run([str(DRIVER), str(tmp_path), "2"], capture_output=True, text=True, timeout=300)
+    sys.stdout.write(r.stdout)
+    sys.stderr.write(r.stderr)
+    assert r.returncode == 0, r.stderr
+
+    tables = {}
+    for L in db.lengths:
+        so = O.lbe_filter(db.seqs(L), L, prm.min_mass, prm.max_mass)
+        eo, _ = O.lbe_generate(prm, so, L, mc)
+        eg = np.fromfile(tmp_path / f"lbe_ents_{L}.bin", N.PEP_DTYPE)
+        sg = np.fromfile(tmp_path / f"lbe_seqs_{L}.bin", np.uint8)
+        assert np.array_equal(sg, so), L
+        for f in ("seq_id", "sites", "mod_num"):
+            assert np.array_equal(eg[f], eo[f]), (L, f)
+        assert np.array_equal(eg["mass"].view(np.uint32), eo["mass"].view(np.uint32)), L
+        tables[L] = (so, eo)
+    res_o, _, _ = O.OracleDB(prm, _LbeDb(db.lengths, tables)).search(q)
+    reported = np.nonzero((res_o["flags"] & N.RES_VALID) != 0)[0]
+    rows = [ln.split() for ln in (tmp_path / "psms.txt").read_text().splitlines()]
+    assert len(rows) == len(reported) and len(rows) > 50
+    got = {int(x[0]): x for x in rows}
+    for s in reported:
+        x, o = got[int(s)], res_o[s]
+        assert int(x[1]) == o["psid"] and int(x[2]) == o["idxoffset"] and int(x[3]) == o["sharedions"], s
+        assert abs(float(x[6]) - o["e_x"]) <= 1e-3 * abs(o["e_x"]), s

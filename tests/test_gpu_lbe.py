@@ -177,3 +177,54 @@ def test_pepdb_errors(eng):
     s, e = d.export()
     assert len(s) == 0 and len(e) == 0
     d.free()
+
+
+def _full_size(name, min_forms):
+    from concurrent.futures import ThreadPoolExecutor
+    import time
+    p = W.PRESETS[name]
+    t0 = time.perf_counter()
+    db = W.Database(p)
+    prm = W.make_params(p)
+    mc = W.modconditions(p)
+    print(f"[lbe] {name}: {db.nforms} generator forms over {len(db.lengths)} lengths "
+          f"({time.perf_counter() - t0:.1f}s)", flush=True)
+    eng = Engine(prm)
+    dbs = [eng.count_peps(L, db.seqs(L), mc) for L in db.lengths]
+    eng.initialize(dbs)
+    dev_ms = sum(d.timing()[1] for d in dbs) + dbs[0].timing()[2]
+    total = sum(d.lcltot for d in dbs)
+    print(f"[lbe] device: {total} forms in {dev_ms:.1f} ms", flush=True)
+    assert total == db.nforms and total > min_forms
+
+    def orc(L):
+        s = O.lbe_filter(db.seqs(L), L, prm.min_mass, prm.max_mass)
+        return O.lbe_generate(prm, s, L, mc)
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(16) as ex:
+        ref = list(ex.map(orc, db.lengths))
+    print(f"[lbe] oracle ({time.perf_counter() - t0:.1f}s)", flush=True)
+    ties = 0
+    for L, d, (ents_o, cnt_o) in zip(db.lengths, dbs, ref):
+        seqs_g, ents_g = d.export()
+        assert np.array_equal(seqs_g, db.seqs(L))
+        assert (d.pep_count, d.mod_count, d.lclpep, d.lclmod) == cnt_o, L
+        _same_entries(ents_g, ents_o)
+        m = ents_o["mass"]
+        ties += int(np.count_nonzero(m[1:] == m[:-1]))
+        d.free()
+    print(f"[lbe] {name}: every length's pepEntries bit-exact ({ties} adjacent equal masses)", flush=True)
+    assert ties > 0
+    eng.close()
+
+
+def test_rank_share_pepdb_full_size():
+    # the one-rank share: 2M tryptic peptides (7-40) + Met-oxidation <= 2
+    _full_size("rank_share", 5_000_000)
+
+
+@pytest.mark.timeout(600)
+def test_large40m_pepdb_full_size():
+    # 12.7M tryptic peptides (7-40) + Met-oxidation <= 3: ~40M forms
+    _full_size("large40m", 38_000_000)
