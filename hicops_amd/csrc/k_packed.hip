@@ -328,8 +328,8 @@ __device__ __forceinline__ void process_item_p(const ChunkArgs &a, const PSmem &
     else pass(std::false_type{});
     if (lane == 0) acc.nia += nia_w;
 
-    // S: the window's words, two peptides per 128-bit read; read words are
-    // cleared for the next item (its pass starts after a barrier).  Peptides
+    // S: the window's words, two peptides per 128-bit exchange with zeros
+    // (read and cleared for the next item, whose pass starts after a barrier).  Peptides
     // with bc + yc >= min_shp are few and scattered, so each wavefront queues
     // its own in a private part of the run table (free by now; no atomics)
     // and scores them 64 at a time, instead of taking the scoring branch for
@@ -359,12 +359,16 @@ __device__ __forceinline__ void process_item_p(const ChunkArgs &a, const PSmem &
 #pragma unroll
             for (uint32_t r = 0; r < kSr; r++) {
                 const uint32_t qi = qb + r * kPT + threadIdx.x;
-                w4[r] = qi <= q1 ? ((lu32x4 *)sm.acc)[qi] : (u32x4){0u, 0u, 0u, 0u};
-            }
-#pragma unroll
-            for (uint32_t r = 0; r < kSr; r++) {
-                const uint32_t qi = qb + r * kPT + threadIdx.x;
-                if (qi <= q1) ((lu32x4 *)sm.acc)[qi] = (u32x4){0u, 0u, 0u, 0u};
+                w4[r] = (u32x4){0u, 0u, 0u, 0u};
+                if (qi <= q1) {
+                    // read and clear in one instruction (ds_wrxchg_rtn_b64)
+                    lu64 *pw = (lu64 *)sm.acc + 2 * qi;
+                    const unsigned long long lo2 =
+                        __hip_atomic_exchange(pw, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    const unsigned long long hi2 =
+                        __hip_atomic_exchange(pw + 1, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    w4[r] = (u32x4){(uint32_t)lo2, (uint32_t)(lo2 >> 32), (uint32_t)hi2, (uint32_t)(hi2 >> 32)};
+                }
             }
 #pragma unroll
             for (uint32_t r = 0; r < kSr; r++) {
