@@ -1,8 +1,8 @@
 // k_lbe.hip -- the peptide database stage on the GPU (LBE + MODS): the mass
 // filter of LBE_CountPeps (lbe_internal.cpp:369-467), the variant counts of
-// MODS_ModCounter (mods.cpp:397-431), the rank share of LBE_CreatePartitions
+// MODS_ModCounter (mods.cpp:399-432), the rank share of LBE_CreatePartitions
 // (lbe_internal.cpp:308-352), the entries of LBE_GeneratePeps
-// (lbe_internal.cpp:209-232) and MODS_GenerateMods (mods.cpp:447-496), and
+// (lbe_internal.cpp:204-229) and MODS_GenerateMods (mods.cpp:448-496), and
 // LBE_Initialize's sort by mass (lbe_internal.cpp:184-187).
 //
 // Orders that decide which row a form gets (and so every iA id and the tie
@@ -41,7 +41,7 @@
 
 namespace hcx {
 
-// Parsed modconditions (MODS_Initialize, mods.cpp:63-101).
+// Parsed modconditions (MODS_Initialize, mods.cpp:64-103).
 struct LbeMods {
     int32_t limit;                        // mods per form, at most
     uint32_t ntypes;
@@ -67,7 +67,7 @@ constexpr int kSW = kST / 64;
 constexpr uint32_t kSmallList = kSmall / (kLeaf + 1) + 2;   // ranges > 16 at one level, at most
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 
-__constant__ unsigned long long c_comb[64][64];   // mods.cpp:140-164
+__constant__ unsigned long long c_comb[64][64];   // mods.cpp:137-160
 
 // ---- filter, counts, entries ----
 
@@ -103,14 +103,18 @@ __global__ void k_lbe_compact(const char *lines, const uint32_t *flag, const uin
     if (flag[i]) seqs[(size_t)pos[i] * L + l] = lines[g];
 }
 
-// count() of mods.cpp:270-296 in closed form: partition3 equals the sum, over
+// count() of mods.cpp:290-315 in closed form: partition3 equals the sum, over
 // per-type counts k_t <= cap_t with sum k_t <= limit, of prod C(N_t, k_t),
 // N_t the residues of type t in the sequence (Vandermonde on partition2).
-__global__ void k_lbe_count(const char *seqs, uint32_t n, uint32_t L, LbeMods m, uint32_t *vc,
-                            unsigned long long *total, uint32_t *err) {
+// The kept count n is read on the device (the filter's scan total), so the
+// host does not wait between the filter and the counts; the grid covers every
+// input line and entries n..nmax are cleared for the scan that follows.
+__global__ void k_lbe_count(const char *seqs, const uint32_t *np, uint32_t nmax, uint32_t L, LbeMods m,
+                            uint32_t *vc, unsigned long long *total, uint32_t *err) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = *np;
     unsigned long long v = 0;
-    if (i == n) vc[n] = 0;
+    if (i >= n && i <= nmax) vc[i] = 0;
     if (i < n) {
         const char *s = seqs + (size_t)i * L;
         uint32_t N[HCX_MAX_MOD_TYPES];
@@ -242,7 +246,7 @@ __global__ __launch_bounds__(kModT) void k_lbe_mods(const char *seqs, uint32_t n
     for (uint32_t l = 0; l < L; l++) ty[l] = m.type_of[s[l] - 'A'];
     for (int k = 0; k < 64; k++) hist[k][t] = 0;
     auto keep = [&](uint32_t g) {
-        // LBE_ApplyPolicy (lbe_internal.cpp:76-102) for modified forms
+        // LBE_ApplyPolicy (lbe_internal.cpp:75-103) for modified forms
         return policy == HCX_POLICY_CHUNK ? (g / lclmod) == myid : (g % nodes) == myid;
     };
     uint32_t g = g0;
@@ -824,7 +828,7 @@ using hcx::set_error_code;
 
 namespace {
 
-// MODS_Initialize (mods.cpp:63-101) with the checks the reference leaves to
+// MODS_Initialize (mods.cpp:64-103) with the checks the reference leaves to
 // its input: residues A-Z, each in one condition only (the count and the
 // enumeration would otherwise disagree), caps >= 0, at most 8 mods per form
 // (modNum holds 8 nibbles), types within the context's mod table.
@@ -866,7 +870,7 @@ int comb_table_upload(int device) {
     static bool done[64] = {false};
     if (device < 0 || device >= 64) return set_error("comb table: device id");
     if (done[device]) return HCX_SUCCESS;
-    // mods.cpp:140-164 MODS_GenCombinations
+    // mods.cpp:137-160 MODS_GenCombinations
     static unsigned long long C[64][64];
     std::memset(C, 0, sizeof C);
     C[0][0] = 1;
@@ -913,8 +917,11 @@ extern "C" int hcx_pepdb_create(hcx_ctx *c, const hcx_lbe_cfg *cfg, uint32_t pep
     hipStream_t st = c->stream;
     const uint64_t nchars = (uint64_t)nlines * peplen;
     hcx::DeviceBuf dl, flag, fpos, scan, err;
+    // the kept sequences and their variant offsets are sized for every line,
+    // so nothing is allocated between the upload and the counts
     if (dl.ensure(nchars + 16) || flag.ensure(4ull * nlines + 8) || fpos.ensure(4ull * nlines + 8) ||
-        scan.ensure(4 * hcx::scan_tmp_words((uint64_t)nlines + 1) + 64) || err.ensure(64))
+        scan.ensure(4 * hcx::scan_tmp_words((uint64_t)nlines + 1) + 64) || err.ensure(64) ||
+        db->seqs.ensure(nchars + 16) || db->var_off.ensure(4ull * nlines + 8))
         return fail(set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_pepdb_create: device allocation failed"));
     if (hipEventRecord(c->ev2, st) != hipSuccess) return fail(set_error("event"));
     if ((nchars && hipMemcpyAsync(dl.p, lines, nchars, hipMemcpyHostToDevice, st) != hipSuccess) ||
@@ -927,29 +934,25 @@ extern "C" int hcx_pepdb_create(hcx_ctx *c, const hcx_lbe_cfg *cfg, uint32_t pep
     if (int rc = hcx::scan_exclusive_u32(st, (const uint32_t *)flag.p, (uint32_t *)fpos.p, (uint64_t)nlines + 1,
                                          (uint32_t *)scan.p))
         return fail(rc);
-    uint32_t hk[2] = {0, 0};
-    if (hipMemcpyAsync(&hk[0], (const uint32_t *)fpos.p + nlines, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipMemcpyAsync(&hk[1], err.p, 4, hipMemcpyDeviceToHost, st) != hipSuccess || hipStreamSynchronize(st))
-        return fail(set_error("hcx_pepdb_create: filter failed"));
-    if (hk[1]) return fail(set_error_code(HCX_ERR_INVLD_PARAM, "hcx_pepdb_create: sequence characters must be letters"));
-    const uint32_t n = hk[0];
-    db->nseqs = n;
-    if (db->seqs.ensure((size_t)n * peplen + 16) || db->var_off.ensure(4ull * n + 8))
-        return fail(set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_pepdb_create: device allocation failed"));
     if (nchars)
         hcx::k_lbe_compact<<<(unsigned)((nchars + 255) / 256), 256, 0, st>>>(
             (const char *)dl.p, (const uint32_t *)flag.p, (const uint32_t *)fpos.p, nchars, peplen, (char *)db->seqs.p);
-    hcx::k_lbe_count<<<(n + 256) / 256, 256, 0, st>>>((const char *)db->seqs.p, n, peplen, db->mods,
-                                                      (uint32_t *)flag.p, (unsigned long long *)((char *)err.p + 8),
-                                                      (uint32_t *)err.p);
+    hcx::k_lbe_count<<<(nlines + 256) / 256, 256, 0, st>>>((const char *)db->seqs.p, (const uint32_t *)fpos.p + nlines,
+                                                           nlines, peplen, db->mods, (uint32_t *)flag.p,
+                                                           (unsigned long long *)((char *)err.p + 8), (uint32_t *)err.p);
     if (hipGetLastError() != hipSuccess) return fail(set_error("hcx_pepdb_create: count launch failed"));
-    if (int rc = hcx::scan_exclusive_u32(st, (const uint32_t *)flag.p, (uint32_t *)db->var_off.p, (uint64_t)n + 1,
-                                         (uint32_t *)scan.p))
+    if (int rc = hcx::scan_exclusive_u32(st, (const uint32_t *)flag.p, (uint32_t *)db->var_off.p,
+                                         (uint64_t)nlines + 1, (uint32_t *)scan.p))
         return fail(rc);
     if (hipEventRecord(c->ev1, st) != hipSuccess) return fail(set_error("event"));
-    uint32_t he[4] = {0, 0, 0, 0};
-    if (hipMemcpyAsync(he, err.p, 16, hipMemcpyDeviceToHost, st) != hipSuccess || hipStreamSynchronize(st))
+    uint32_t he[4] = {0, 0, 0, 0}, n = 0;
+    if (hipMemcpyAsync(he, err.p, 16, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(&n, (const uint32_t *)fpos.p + nlines, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st))
         return fail(set_error("hcx_pepdb_create: count failed"));
+    if (he[0] & 1u)
+        return fail(set_error_code(HCX_ERR_INVLD_PARAM, "hcx_pepdb_create: sequence characters must be letters"));
+    db->nseqs = n;
     const uint64_t mods = (uint64_t)he[2] | ((uint64_t)he[3] << 32);
     if (he[0] & 2u || mods > 0xFFFFFFFFull)
         return fail(set_error_code(HCX_ERR_INVLD_SIZE, "hcx_pepdb_create: more than 2^32 variants"));

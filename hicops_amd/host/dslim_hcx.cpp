@@ -218,7 +218,7 @@ int32_t policy_code() { return params.policy == _chunk ? HCX_POLICY_CHUNK : HCX_
 
 }  // namespace
 
-// mods.cpp:63-101: the mod conditions are parsed (and checked) where they are
+// mods.cpp:64-103: the mod conditions are parsed (and checked) where they are
 // used, by hcx_pepdb_create; here only that params.modconditions reads.
 status_t MODS_Initialize() {
     State &s = st();
@@ -290,7 +290,7 @@ status_t LBE_CreatePartitions(Index *index) {
     return SLM_SUCCESS;
 }
 
-// lbe_internal.cpp:119-206: the entries of this rank's share, sorted by mass
+// lbe_internal.cpp:119-202: the entries of this rank's share, sorted by mass
 // when params.dM > 0, into new host arrays as LBE_AllocateMem makes them.
 status_t LBE_Initialize(Index *index) {
     if (!index) return ERR_INVLD_PTR;
@@ -347,7 +347,7 @@ status_t LBE_Distribute(Index *index) {
     return SLM_SUCCESS;
 }
 
-// lbe_internal.cpp:244 (DSLIM_Deinitialize, dslim.cpp:644-720): the ion
+// lbe_internal.cpp:241 (DSLIM_Deinitialize, dslim.cpp:644-720): the ion
 // index, the peptide arrays and the device database of this Index.
 status_t LBE_Deinitialize(Index *index) {
     if (!index) return ERR_INVLD_PTR;

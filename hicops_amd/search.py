@@ -2,8 +2,8 @@
 
   Engine(params)                  gParams + DSLIM_InitializeScorecard (dslim.cpp:390-437)
   Engine.count_peps(L, lines, ...) LBE_CountPeps + MODS_ModCounter for one peptide length
-                                  (lbe_internal.cpp:369-467, mods.cpp:397-431)
-  Engine.initialize(dbs)          LBE_Initialize (lbe_internal.cpp:119-206) for several lengths at
+                                  (lbe_internal.cpp:369-467, mods.cpp:399-432)
+  Engine.initialize(dbs)          LBE_Initialize (lbe_internal.cpp:119-202) for several lengths at
                                   once: LBE_GeneratePeps, MODS_GenerateMods, std::sort by mass
   Engine.construct_pepdb(db)      DSLIM_Construct over a device-resident PepDb
   Engine.construct(L, seqs, ents) DSLIM_Construct(Index*) for one peptide length (dslim.cpp:46-138)

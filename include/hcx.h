@@ -14,9 +14,9 @@
  *   hcx_index_export              read-back of Index.ionIndex[0].{iA,bA} (slm_dsts.h:306-320)
  *   hcx_index_free                DSLIM_DeallocateIonIndex(Index*)   (dslim.h:149, dslim.cpp:644-691)
  *   hcx_pepdb_create              LBE_CountPeps(char*, Index*, uint_t) (lbe.h:90, lbe_internal.cpp:369-467)
- *                                 with MODS_Initialize / MODS_ModCounter (mods.cpp:63-101, 397-431)
- *   hcx_pepdb_generate            LBE_Initialize(Index*) (lbe.h:45, lbe_internal.cpp:119-206) with
- *                                 LBE_GeneratePeps, MODS_GenerateMods (mods.cpp:447-496)
+ *                                 with MODS_Initialize / MODS_ModCounter (mods.cpp:64-103, 397-431)
+ *   hcx_pepdb_generate            LBE_Initialize(Index*) (lbe.h:45, lbe_internal.cpp:119-202) with
+ *                                 LBE_GeneratePeps, MODS_GenerateMods (mods.cpp:448-496)
  *   hcx_pepdb_counts              LBE_CreatePartitions(Index*) (lbe_internal.cpp:308-352)
  *   hcx_index_build_pepdb         DSLIM_Construct(Index*) on the device-resident pepEntries
  *   hcx_queries_upload            hand-over of a Queries batch       (slm_dsts.h:422-559)
@@ -257,8 +257,8 @@ typedef struct hcx_lbe_cfg {
 } hcx_lbe_cfg;
 
 typedef struct hcx_pepdb hcx_pepdb;
-/* LBE_CountPeps (lbe_internal.cpp:369-467) with MODS_Initialize (mods.cpp:63-101)
- * and MODS_ModCounter (mods.cpp:397-431): lines holds nlines sequences of peplen
+/* LBE_CountPeps (lbe_internal.cpp:369-467) with MODS_Initialize (mods.cpp:64-103)
+ * and MODS_ModCounter (mods.cpp:399-432): lines holds nlines sequences of peplen
  * characters back to back (lower case is upper-cased, as std::toupper does); the
  * mass-legal ones are kept in order and their variants counted on the device.
  * Returns when the counts are known. */
@@ -267,8 +267,8 @@ int  hcx_pepdb_create(hcx_ctx *ctx, const hcx_lbe_cfg *cfg, uint32_t peplen, con
 /* counts[6]: pepCount, modCount, totalCount, lclpepCnt, lclmodCnt, lcltotCnt
  * (the Index fields, slm_dsts.h:322-357; LBE_CreatePartitions lbe_internal.cpp:308-352) */
 int  hcx_pepdb_counts(const hcx_pepdb *db, uint32_t *counts);
-/* LBE_Initialize (lbe_internal.cpp:119-206) for n lengths at once: LBE_GeneratePeps
- * (lbe_internal.cpp:209-232), MODS_GenerateMods (mods.cpp:447-496, each sequence's
+/* LBE_Initialize (lbe_internal.cpp:119-202) for n lengths at once: LBE_GeneratePeps
+ * (lbe_internal.cpp:204-229), MODS_GenerateMods (mods.cpp:448-496, each sequence's
  * variants in MODS_ModList order, mods.cpp:333-384, then glibc qsort with
  * cmpvarEntries), and std::sort by mass with CmpPepEntries -- the tie order of
  * libstdc++'s introsort, reproduced exactly. */

@@ -211,12 +211,12 @@ extern gParams params;
  * are kept beside the Index, so DSLIM_Construct after LBE_Initialize builds
  * from them without a second upload (the Index's host arrays are filled as
  * the reference fills them and must not be changed in between). */
-status_t MODS_Initialize();                                           /* mods.cpp:63-101          */
+status_t MODS_Initialize();                                           /* mods.cpp:64-103          */
 status_t LBE_CountPeps(char *filename, Index *index, uint_t explen);  /* lbe_internal.cpp:369-467 */
 status_t LBE_CreatePartitions(Index *index);                          /* lbe_internal.cpp:308-352 */
-status_t LBE_Initialize(Index *index);                                /* lbe_internal.cpp:119-206 */
+status_t LBE_Initialize(Index *index);                                /* lbe_internal.cpp:119-202 */
 status_t LBE_Distribute(Index *index);                                /* lbe_internal.cpp:256-295 */
-status_t LBE_Deinitialize(Index *index);                              /* lbe_internal.cpp:244     */
+status_t LBE_Deinitialize(Index *index);                              /* lbe_internal.cpp:241     */
 
 status_t DSLIM_Construct(Index *index);
 status_t DSLIM_InitializeScorecard(Index *index, uint_t idxs);

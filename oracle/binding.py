@@ -279,7 +279,7 @@ def pickpeaks(raw, qalen: int, base_int: int, min_int: int):
 # ---- peptide database stage (LBE + MODS) ----
 
 def lbe_count(seq: str, modconditions: str) -> int:
-    """mods.cpp:270-296 count(): forms of one sequence, unmodified included."""
+    """mods.cpp:290-315 count(): forms of one sequence, unmodified included."""
     b = seq.encode()
     out = C.c_uint64()
     rc = lib().orc_lbe_count(b, C.c_uint32(len(b)), modconditions.encode(), C.byref(out))

@@ -141,7 +141,7 @@ int orc_pickpeaks(const uint32_t *raw_mz, const uint32_t *raw_in, const uint32_t
 
 /* ---- peptide database stage (hcx_oracle_lbe.cpp) ---- */
 
-/* mods.cpp:270-296 count() with MODS_Initialize's parse (mods.cpp:63-101):
+/* mods.cpp:290-315 count() with MODS_Initialize's parse (mods.cpp:64-103):
  * forms of one sequence, the unmodified one included.  Pinned: the number of
  * forms MODS_ModList emits plus one, counted by brute force over all site
  * subsets (tests/test_oracle_pins.py). */
@@ -150,7 +150,7 @@ int orc_lbe_count(const char *seq, uint32_t len, const char *modconditions, uint
  * [min_mass, max_mass].  Pinned: published peptide masses at the bounds. */
 int orc_lbe_filter(const char *lines, uint32_t nlines, uint32_t len, uint32_t min_mass, uint32_t max_mass,
                    char *seqs, uint32_t *nseqs);
-/* LBE_Initialize (lbe_internal.cpp:119-206) of one length after
+/* LBE_Initialize (lbe_internal.cpp:119-202) of one length after
  * LBE_CountPeps: MODS_ModCounter, LBE_CreatePartitions, LBE_GeneratePeps,
  * MODS_GenerateMods (MODS_ModList order, glibc qsort with cmpvarEntries) and,
  * when sort != 0, std::sort by mass.  counts: pepCount, modCount, lclpepCnt,

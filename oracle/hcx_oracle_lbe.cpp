@@ -28,7 +28,7 @@ namespace {
 constexpr int kMaxSeqLen = 60;   // CMakeLists.txt:156-157 (MAX_SEQ_LEN)
 constexpr int kMaxCombs = 64;    // mods.h:30 (MAX_COMBS)
 
-// mods.cpp:140-164 MODS_GenCombinations: Pascal's triangle, filled the
+// mods.cpp:137-160 MODS_GenCombinations: Pascal's triangle, filled the
 // reference's way (left half by the recurrence, right half mirrored).
 struct Comb {
     uint64_t c[kMaxCombs][kMaxCombs];
@@ -50,7 +50,7 @@ const Comb &comb() {
     return C;
 }
 
-// mods.cpp:63-101 MODS_Initialize: "limit AAs n AAs n ..." split on white
+// mods.cpp:64-103 MODS_Initialize: "limit AAs n AAs n ..." split on white
 // space; a residue's type is the last condition naming it.
 struct Mods {
     int limit = 0;
@@ -75,7 +75,7 @@ int parse_mods(const char *text, Mods *m) {
     return HCX_SUCCESS;
 }
 
-// mods.cpp:180-218 partition2: ways to pick up to b elements from the
+// mods.cpp:192-231 partition2: ways to pick up to b elements from the
 // multiset whose class sizes are a.
 int64_t partition2(const std::vector<int> &a, int b) {
     int64_t sum = 0;
@@ -89,7 +89,7 @@ int64_t partition2(const std::vector<int> &a, int b) {
     return sum;
 }
 
-// mods.cpp:233-258 partition3: the per-condition caps and the per-peptide limit.
+// mods.cpp:246-276 partition3: the per-condition caps and the per-peptide limit.
 int64_t partition3(const std::vector<std::vector<int>> &A, const std::vector<int> &B, int limit) {
     if (A.size() == 1) return B[0] <= limit ? partition2(A[0], B[0]) : partition2(A[0], limit);
     std::vector<std::vector<int>> A2(A.begin() + 1, A.end());
@@ -100,7 +100,7 @@ int64_t partition3(const std::vector<std::vector<int>> &A, const std::vector<int
     return sum;
 }
 
-// mods.cpp:270-296 count(): forms of one sequence, the unmodified one included.
+// mods.cpp:290-315 count(): forms of one sequence, the unmodified one included.
 uint64_t count_forms(const char *s, uint32_t L, const Mods &m) {
     int cnt[256] = {0};
     for (uint32_t i = 0; i < L; i++) cnt[(unsigned char)s[i]]++;
@@ -119,7 +119,7 @@ uint64_t count_forms(const char *s, uint32_t L, const Mods &m) {
 // mods.cpp:333-384 MODS_ModList: depth-first, the branch that modifies the
 // current residue before the one that skips it; a form is emitted on entry
 // to every modifying branch.  keep(global) is LBE_ApplyPolicy
-// (lbe_internal.cpp:76-102) for modified forms.
+// (lbe_internal.cpp:75-103) for modified forms.
 template <class Keep>
 void mod_list(const char *pep, uint32_t L, std::vector<int> &cond, int total, hcx_pep_entry cont, uint32_t letter,
               bool novel, int seen, uint32_t refid, uint32_t &global, std::vector<hcx_pep_entry> &out, const Mods &m,
@@ -209,7 +209,7 @@ extern "C" int orc_lbe_generate(const hcx_params *p, const char *seqs, uint32_t 
     Mods m;
     int rc = parse_mods(modconditions, &m);
     if (rc) return rc;
-    // mods.cpp:397-431 MODS_ModCounter: forms - 1 per sequence, exclusive scan
+    // mods.cpp:399-432 MODS_ModCounter: forms - 1 per sequence, exclusive scan
     std::vector<uint32_t> var_count(nseqs + 1, 0);
     uint64_t mod_count = 0;
     if (m.limit > 0) {
@@ -230,7 +230,7 @@ extern "C" int orc_lbe_generate(const hcx_params *p, const char *seqs, uint32_t 
     counts[2] = lclpep;
     counts[3] = lclmod;
     if (!out) return HCX_SUCCESS;
-    // lbe_internal.cpp:209-232 LBE_GeneratePeps
+    // lbe_internal.cpp:204-229 LBE_GeneratePeps
     for (uint32_t f = 0; f < lclpep; f++) {
         uint32_t idd = generate_index(f, lclpep, nodes, myid, policy);
         hcx_pep_entry e;
@@ -239,7 +239,7 @@ extern "C" int orc_lbe_generate(const hcx_params *p, const char *seqs, uint32_t 
         e.seq_id = idd;
         out[f] = e;
     }
-    // mods.cpp:447-496 MODS_GenerateMods
+    // mods.cpp:448-496 MODS_GenerateMods
     if (lclmod > 0) {
         hcx_pep_entry *mod_entries = out + lclpep;
         auto keep = [&](uint32_t key) {

@@ -163,7 +163,7 @@ REFERENCE_SYMBOLS = [
     "_Z15DFile_InitFilesv",
     "_Z17DFile_DeinitFilesv",
     "_Z19DFile_PrintPartialsjP8_Results",
-    # lbe.h:45,74,90,92; mods.h:63; lbe_internal.cpp:244
+    # lbe.h:45,74,90,92; mods.h:63; lbe_internal.cpp:241
     "_Z14LBE_InitializeP6_Index",
     "_Z14LBE_DistributeP6_Index",
     "_Z13LBE_CountPepsPcP6_Indexj",

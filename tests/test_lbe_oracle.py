@@ -1,13 +1,13 @@
 """Pins of the oracle's peptide database stage (LBE + MODS) on the CPU.
 
-  * count() (mods.cpp:270-296) against brute force: every subset of the
+  * count() (mods.cpp:290-315) against brute force: every subset of the
     modifiable residues within the per-type caps and the per-form limit;
   * MODS_ModList's emission order and glibc qsort with cmpvarEntries
     (mods.cpp:333-384, 105-126, 491) on hand-traced sequences;
   * cmpvarEntries under qsort is a stable sort by ascending span (last
     modified residue minus first), the rule the device applies;
   * the rank share of LBE_CreatePartitions / LBE_GeneratePeps /
-    MODS_GenerateMods (lbe_internal.cpp:308-352, 209-232; mods.cpp:447-496)
+    MODS_GenerateMods (lbe_internal.cpp:308-352, 209-232; mods.cpp:448-496)
     on a hand-worked three-rank split;
   * masses: the unmodified mass and the first mod only (utils.cpp:290-362);
   * the mass sort is std::sort (lbe_internal.cpp:184-187).
