@@ -54,6 +54,8 @@ def main() -> int:
                     help="one batched search, then a 1 GiB device copy (run under rocprofv3 --pmc)")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="turn two rocprofv3 --pmc runs of --pmc-probe into the traffic figure")
+    ap.add_argument("--no-mirror", action="store_true",
+                    help="skip the reference-shaped entry point leg (DSLIM_QuerySpectrum through the C++ driver)")
     ap.add_argument("--pepdb-only", action="store_true",
                     help="time the peptide database stage (LBE + MODS) alone and print its JSON")
     args = ap.parse_args()
@@ -206,6 +208,7 @@ def main() -> int:
         cpu = cpu_baseline(prm, db, q, st, args.cpu_seconds)
     pick = queries_pick(eng, raw, p, args.steps, args.warmup, not args.no_cpu)
     pdb = pepdb_stage(eng, db, p, prm, args.steps, args.warmup, not args.no_cpu)
+    mirror = None if args.no_mirror else mirror_leg(db, q, prm, args.steps, args.warmup)
 
     if sur_info is not None:
         sur_info["surrogate_evalue_ms"] = round(runs["batched+surrogate"]["ems"], 3)
@@ -253,6 +256,7 @@ def main() -> int:
                         "frac_hbm": round(b_build / (build_ms / 1e3) / HBM_PEAK, 4)},
         "queries_pick": pick,
         "pepdb": pdb,
+        "mirror": mirror,
         "psms_valid": valid,
         "tied_spectra": tied,
         "host_fits": int(((res["flags"] & N.RES_FIT_HOST) != 0).sum()),
@@ -261,6 +265,44 @@ def main() -> int:
     print(json.dumps(line), flush=True)
     eng.close()
     return 0
+
+
+def mirror_leg(db, q, prm, steps: int, warmup: int) -> dict:
+    """DSLIM_QuerySpectrum, the reference-shaped entry point, timed the way an
+    application calls it: the C++ driver (tests/cpp/dslim_driver, linked to
+    libhcx_dslim.so) builds each length's index with DSLIM_Construct from host
+    pepEntries, then calls DSLIM_QuerySpectrum on the benchmark's batch from
+    host memory, warmup + steps times.  Each call uploads the batch over PCIe,
+    runs the batched search with tail-fit e-values, copies the results back
+    and passes every PSM to the application's DFile_PrintScore (here a
+    counter).  Wall time per call, from the driver's own clock."""
+    import shutil
+    import subprocess
+    import tempfile
+    from hicops_amd import native as N
+    from hicops_amd import workload as W
+    drv = ROOT / "tests" / "cpp" / "dslim_driver"
+    if not drv.exists():
+        return {"skipped": "tests/cpp/dslim_driver is not built"}
+    d = Path(tempfile.mkdtemp(prefix="hcx_mirror_"))
+    try:
+        W.write_driver_case(d, db, q, prm, N.KERNEL_BATCHED, repeat=warmup + steps, quiet=1)
+        t0 = time.perf_counter()
+        r = subprocess.run([str(drv), str(d), "1"], capture_output=True, text=True, timeout=900)
+        wall = time.perf_counter() - t0
+        if r.returncode:
+            return {"error": f"driver exit {r.returncode}: {r.stderr[-400:]}"}
+        lines = [x for x in (d / "timing.txt").read_text().splitlines() if x]
+        ms = [float(x) for x in lines if not x.startswith("psms")]
+        psms = int(next(x for x in lines if x.startswith("psms")).split()[1])
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    t = float(np.median(ms[warmup:])) if len(ms) > warmup else float(ms[-1])
+    log(f"mirror: DSLIM_QuerySpectrum {t:.1f} ms per {q.nspec}-spectrum call ({psms} PSMs), driver wall {wall:.1f}s")
+    return {"entry_point": "DSLIM_QuerySpectrum(Queries*, Index*, uint_t)", "kernel": "batched", "evalue": "tailfit",
+            "spectra": q.nspec, "calls_ms": [round(x, 2) for x in ms], "ms_per_call": round(t, 2),
+            "spectra_per_s": round(q.nspec / (t / 1e3), 1), "psms_per_call": psms,
+            "includes": "Queries upload from host memory, search, results to host, PSM sink"}
 
 
 def pepdb_stage(eng, db, p, prm, steps: int, warmup: int, with_cpu: bool) -> dict:

@@ -438,33 +438,12 @@ extern "C" int hcx_queries_upload(hcx_ctx *c, const hcx_queries_view *q, hcx_qba
     const uint64_t npk = n > 0 ? q->idx[n] : 0;
     if (npk && (!q->moz || !q->intensity))
         return set_error_code(HCX_ERR_INVLD_PTR, "hcx_queries_upload: missing peaks");
-    uint32_t maxp = 0, maxi = 0;
-    for (uint64_t i = 0; i < npk; i++) maxi = std::max(maxi, q->intensity[i]);
+    uint32_t maxp = 0;
     for (int32_t s = 0; s < n; s++) {
         if (q->idx[s + 1] < q->idx[s]) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_queries_upload: idx not monotone");
         maxp = std::max(maxp, q->idx[s + 1] - q->idx[s]);
     }
     if (n > 0 && q->idx[0] != 0) return set_error_code(HCX_ERR_INVLD_PARAM, "hcx_queries_upload: idx[0] != 0");
-    // largest number of one spectrum's peaks inside any 2*dF+1 bins: bounds
-    // how often one fragment bin can be probed
-    uint32_t mult = 0;
-    std::vector<uint32_t> mult_s((size_t)std::max(n, 0));
-    {
-        std::vector<uint32_t> t;
-        const uint32_t span = 2 * c->params.dF;
-        for (int32_t s = 0; s < n; s++) {
-            t.assign(q->moz + q->idx[s], q->moz + q->idx[s + 1]);
-            std::sort(t.begin(), t.end());
-            size_t i = 0;
-            uint32_t m = 0;
-            for (size_t j = 0; j < t.size(); j++) {
-                while (t[j] - t[i] > span) i++;
-                m = std::max<uint32_t>(m, (uint32_t)(j - i + 1));
-            }
-            mult_s[s] = m;
-            mult = std::max(mult, m);
-        }
-    }
     HCX_HIP(hipSetDevice(c->device));
     hcx_qbatch *b = new (std::nothrow) hcx_qbatch;
     if (!b) return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_queries_upload: out of memory");
@@ -473,8 +452,6 @@ extern "C" int hcx_queries_upload(hcx_ctx *c, const hcx_queries_view *q, hcx_qba
     b->file_num = q->file_num;
     b->npeaks = npk;
     b->max_peaks = maxp;
-    b->max_mult = mult;
-    b->max_int = maxi;
     if (b->moz.ensure(4 * npk + 16) || b->inten.ensure(4 * npk + 16) || b->idx.ensure(4ull * (n + 1)) ||
         b->prec.ensure(4ull * n + 16) || b->chg.ensure(4ull * n + 16) || b->rt.ensure(4ull * n + 16) ||
         b->mult.ensure(4ull * n + 16)) {
@@ -489,11 +466,15 @@ extern "C" int hcx_queries_upload(hcx_ctx *c, const hcx_queries_view *q, hcx_qba
     const bool zero_rt = !q->rtimes && n > 0;
     if (up(b->moz, q->moz, 4 * npk) || up(b->inten, q->intensity, 4 * npk) || up(b->idx, q->idx, 4ull * (n + 1)) ||
         up(b->prec, q->precurse, 4ull * n) || up(b->chg, q->charges, 4ull * n) || up(b->rt, q->rtimes, 4ull * n) ||
-        up(b->mult, mult_s.data(), 4ull * n) ||
-        (zero_rt && hipMemsetAsync(b->rt.p, 0, 4ull * n, c->stream) != hipSuccess) ||
-        hipStreamSynchronize(c->stream) != hipSuccess) {
+        (zero_rt && hipMemsetAsync(b->rt.p, 0, 4ull * n, c->stream) != hipSuccess)) {
         delete b;
         return set_error("hcx_queries_upload: copy failed");
+    }
+    // largest intensity and, per spectrum, most peaks inside any 2*dF+1 bins
+    // (bounds how often one fragment bin can be probed), on the device
+    if (int rc = hcx::upload_facts(c, b, maxp)) {
+        delete b;
+        return rc;
     }
     *out = b;
     return HCX_SUCCESS;

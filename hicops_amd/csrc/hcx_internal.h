@@ -134,6 +134,11 @@ struct hcx_qbatch {
 
 namespace hcx {
 
+// The batch facts of a batch uploaded by hcx_queries_upload, on the device
+// (k_pick.hip): per-spectrum multiplicities into b->mult, b->max_int,
+// b->max_mult.  Ends with the stream synchronised.
+int upload_facts(hcx_ctx *c, hcx_qbatch *b, uint32_t maxp);
+
 // Integer division n / d for all 32-bit n (round-up method with a 33-bit
 // effective multiplier): q = (t + ((n - t) >> 1)) >> (s - 1), t = mulhi(m, n).
 void make_divider(uint32_t d, uint32_t *m, uint32_t *s);

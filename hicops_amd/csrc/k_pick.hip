@@ -536,7 +536,78 @@ __global__ __launch_bounds__(256) void k_pick_compact(const uint32_t *ub, const 
     }
 }
 
+// Batch facts of a Queries batch uploaded as it is (hcx_queries_upload):
+// per spectrum the most peaks within span + 1 consecutive bins (the broadcast
+// count of pick_one over all of the spectrum's peaks) and its largest
+// intensity.  One wavefront per spectrum; the bins are staged in LDS when the
+// spectrum has at most `cap` peaks, else read from HBM.
+__global__ __launch_bounds__(64) void k_upload_facts(const uint32_t *moz, const uint32_t *inten, const uint32_t *idx,
+                                                     uint32_t span, uint32_t cap, uint32_t *imax_s, uint32_t *mult_s) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    uint32_t *tmp = reinterpret_cast<uint32_t *>(smem_raw);
+    const int s = (int)blockIdx.x;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t b = idx[s], P = idx[s + 1] - b;
+    const bool staged = P <= cap;
+    uint32_t imax = 0;
+    for (uint32_t t = lane; t < P; t += 64) {
+        imax = max(imax, inten[b + t]);
+        if (staged) tmp[t] = moz[b + t];
+    }
+    __syncthreads();
+    const uint32_t *bins = staged ? tmp : moz + b;
+    uint32_t mult = 0;
+    for (uint32_t t0 = 0; t0 < P; t0 += 64) {
+        const uint32_t t = t0 + lane;
+        const uint32_t mx = t < P ? bins[t] : 0u;
+        uint32_t c = 0;
+        for (uint32_t y = 0; y < P; y++) {
+            const uint32_t v = bins[y];
+            c += (v >= mx && v - mx <= span) ? 1u : 0u;
+        }
+        if (t < P) mult = max(mult, c);
+    }
+    imax = wave_max_u32(imax);
+    mult = wave_max_u32(mult);
+    if (lane == 0) {
+        imax_s[s] = imax;
+        mult_s[s] = mult;
+    }
+}
+
 }  // namespace
+
+// The batch facts of an uploaded batch (b->moz, inten, idx resident): per
+// spectrum multiplicities into b->mult, the batch's largest intensity and
+// multiplicity into b->max_int, b->max_mult.  maxp: the largest spectrum.
+// Ends with the stream synchronised.
+int upload_facts(hcx_ctx *c, hcx_qbatch *b, uint32_t maxp) {
+    const int n = b->nspec;
+    b->max_int = b->max_mult = 0;
+    if (n <= 0) return HCX_SUCCESS;
+    hipStream_t st = c->stream;
+    const uint32_t cap = std::min<uint32_t>(maxp, 4096);
+    DeviceBuf imx, summary;
+    if (imx.ensure(4ull * n + 16) || summary.ensure(64))
+        return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_queries_upload: device allocation failed");
+    HCX_HIP(hipMemsetAsync(summary.p, 0, 16, st));
+    k_upload_facts<<<(unsigned)n, 64, 4u * cap + 16, st>>>((const uint32_t *)b->moz.p, (const uint32_t *)b->inten.p,
+                                                           (const uint32_t *)b->idx.p, 2 * c->params.dF, cap,
+                                                           (uint32_t *)imx.p, (uint32_t *)b->mult.p);
+    HCX_HIP(hipGetLastError());
+    // the count field of the summary is not needed here: the intensities
+    // stand in for it
+    k_pick_max<<<(unsigned)std::min<int32_t>((n + 255) / 256, 1024), 256, 0, st>>>(
+        (const uint32_t *)imx.p, (const uint32_t *)imx.p, (const uint32_t *)b->mult.p, n, (uint32_t *)summary.p);
+    HCX_HIP(hipGetLastError());
+    uint32_t h[4] = {0, 0, 0, 0};
+    HCX_HIP(hipMemcpyAsync(h, summary.p, 12, hipMemcpyDeviceToHost, st));
+    HCX_HIP(hipStreamSynchronize(st));
+    b->max_int = h[1];
+    b->max_mult = h[2];
+    return HCX_SUCCESS;
+}
+
 }  // namespace hcx
 
 using hcx::set_error;
@@ -770,6 +841,20 @@ extern "C" int hcx_queries_info(const hcx_qbatch *qb, int32_t *num_specs, uint64
     if (!qb) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_queries_info: null batch");
     if (num_specs) *num_specs = qb->nspec;
     if (npeaks) *npeaks = qb->npeaks;
+    return HCX_SUCCESS;
+}
+
+extern "C" int hcx_queries_facts(const hcx_qbatch *qb, uint32_t *max_peaks, uint32_t *max_int, uint32_t *max_mult,
+                                 uint32_t *mult) {
+    if (!qb) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_queries_facts: null batch");
+    if (max_peaks) *max_peaks = qb->max_peaks;
+    if (max_int) *max_int = qb->max_int;
+    if (max_mult) *max_mult = qb->max_mult;
+    if (mult && qb->nspec > 0) {
+        HCX_HIP(hipSetDevice(qb->ctx->device));
+        HCX_HIP(hipStreamSynchronize(qb->ctx->stream));
+        HCX_HIP(hipMemcpy(mult, qb->mult.p, 4ull * qb->nspec, hipMemcpyDeviceToHost));
+    }
     return HCX_SUCCESS;
 }
 

@@ -208,7 +208,7 @@ def hcx() -> C.CDLL:
                      "hcx_surrogate_get", "hcx_surrogate_train", "hcx_surrogate_gradient",
                      "hcx_surrogate_last_audit", "hcx_selftest_mfma", "hcx_last_timing", "hcx_last_scan_split", "hcx_selftest_log2", "hcx_search_noalloc",
                      "hcx_raw_upload", "hcx_raw_free", "hcx_queries_pick", "hcx_queries_pickpeaks",
-                     "hcx_queries_info", "hcx_queries_export", "hcx_part_export", "hcx_combine",
+                     "hcx_queries_info", "hcx_queries_export", "hcx_queries_facts", "hcx_part_export", "hcx_combine",
                      "hcx_pepdb_create", "hcx_pepdb_counts", "hcx_pepdb_generate", "hcx_pepdb_export",
                      "hcx_pepdb_ms", "hcx_pepdb_free", "hcx_index_build_pepdb", "hcx_sort_mass"]:
             getattr(lib, name).restype = C.c_int

@@ -284,3 +284,29 @@ def make_raw_spectra(db: Database, p: Preset, nspec: int | None = None,
     finally:
         lib.hcx_spec_free(h)
     return r, q
+
+
+def write_driver_case(d, db: Database, q: Queries, prm: N.Params, kernel: int, **extra) -> None:
+    """The input directory of the C++ driver of the reference-shaped entry
+    points (tests/cpp/dslim_driver.cpp): params.txt (gParams fields, limits,
+    kernel and any extra keys), lengths.txt, each length's sequences and
+    pepEntries, and the CSR spectra."""
+    from pathlib import Path
+    d = Path(d)
+    kv = {"maxz": prm.maxz, "scale": prm.scale, "min_mass": prm.min_mass, "max_mass": prm.max_mass,
+          "dF": prm.dF, "min_shp": prm.min_shp, "min_cpsm": prm.min_cpsm, "topmatches": prm.topmatches,
+          "dM": prm.dM, "expect_max": 1e300, "nmods": prm.nmods, "max_hyperscore": prm.max_hyperscore,
+          "max_shdpeaks": prm.max_shdpeaks, "kernel": kernel}
+    for i in range(prm.nmods):
+        kv[f"mod{i}"] = prm.mod_mass[i]
+    kv.update(extra)
+    (d / "params.txt").write_text("".join(f"{k} {v!r}\n" for k, v in kv.items()))
+    (d / "lengths.txt").write_text("".join(f"{L}\n" for L in db.lengths))
+    for L in db.lengths:
+        np.ascontiguousarray(db.seqs(L)).tofile(d / f"seqs_{L}.bin")
+        np.ascontiguousarray(db.entries(L)).tofile(d / f"ents_{L}.bin")
+    q.moz.astype(np.uint32).tofile(d / "moz.bin")
+    q.intensity.astype(np.uint32).tofile(d / "inten.bin")
+    q.idx.astype(np.uint32).tofile(d / "idx.bin")
+    q.precurse.astype(np.float32).tofile(d / "prec.bin")
+    q.charges.astype(np.int32).tofile(d / "chg.bin")

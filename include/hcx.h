@@ -339,6 +339,13 @@ int  hcx_queries_pickpeaks(hcx_ctx *ctx, const hcx_raw_spectra_view *raw, const 
 int  hcx_queries_info(const hcx_qbatch *qb, int32_t *num_specs, uint64_t *npeaks);
 int  hcx_queries_export(const hcx_qbatch *qb, uint32_t *moz, uint32_t *intensity, uint32_t *idx,
                         float *precurse, int32_t *charges, float *rtimes);
+/* The batch facts the search sizes its exact scan by (no reference
+ * counterpart; computed on the device at upload or pick time): the largest
+ * spectrum, the largest intensity, the most peaks of one spectrum within any
+ * 2*dF+1 consecutive bins, and that count per spectrum (mult: num_specs
+ * entries, or NULL).  Any argument may be NULL.                              */
+int  hcx_queries_facts(const hcx_qbatch *qb, uint32_t *max_peaks, uint32_t *max_int, uint32_t *max_mult,
+                       uint32_t *mult);
 
 /* ---- search: DSLIM_QuerySpectrum + ModelTailFit + e-value ----
  * results:  num_specs entries (host).            Required.

@@ -18,10 +18,16 @@
 // receive the resulting pepEntries and sequences.
 // writes <dir>/psms.txt: specid psid idxoffset sharedions totalions
 //                        hyperscore-bits e_x npsms pchg fileIndex
+// Timing: "repeat R" in params.txt runs the batches R times, "quiet 1" makes
+// the sink count PSMs instead of printing them; <dir>/timing.txt receives the
+// wall time of every DSLIM_QuerySpectrum call (ms, one per line) and the
+// PSMs reported per pass.
 // Built with -DHCX_TSV_DRIVER (tests/cpp/tsv_driver) it instead leaves the
 // PSMs to the library's TSV writer: DFile_InitFiles with params.workspace =
 // <dir> and two query threads, the batches, DFile_DeinitFiles; <dir>/rt.bin
 // holds retention times.
+#include <algorithm>
+#include <chrono>
 #include <cinttypes>
 #include <cstdio>
 #include <cstdlib>
@@ -34,15 +40,19 @@
 #include "../../include/hcx_dslim.hpp"
 
 gParams params;   // the application's parameter block, as in source/app/hicops.cpp:31
+static uint64_t g_psms = 0;   // PSMs reported through this program's sink
 
 #ifdef HCX_TSV_DRIVER
 // the application's query file list (source/app/hicops.cpp:28)
 std::vector<std::string> queryfiles = {"q0.ms2", "q1.ms2", "q2.ms2", "sample_3.ms2"};
 #else
 static FILE *g_out = nullptr;
+static bool g_quiet = false;
 
 // The application's result sink (replaces the library's own definition).
 status_t DFile_PrintScore(Index *, uint_t specid, float pmass, hCell *psm, double e_x, uint_t npsms) {
+    g_psms++;
+    if (g_quiet) return SLM_SUCCESS;
     uint32_t hb;
     std::memcpy(&hb, &psm->hyperscore, 4);
     (void)pmass;
@@ -169,9 +179,13 @@ int main(int argc, char **argv) {
 #else
     g_out = std::fopen((dir + "/psms.txt").c_str(), "w");
     if (!g_out) return 2;
+    g_quiet = kv.count("quiet") && kv["quiet"] != 0;
 #endif
+    const int repeat = kv.count("repeat") ? std::max(1, (int)kv["repeat"]) : 1;
+    std::vector<double> call_ms;
     // the batches of one query file, as the I/O threads hand them over
-    for (int b = 0; b < batches; b++) {
+    for (int rb = 0; rb < repeat * batches; rb++) {
+        const int b = rb % batches;
         const int s0 = (int)((long)nspec * b / batches), s1 = (int)((long)nspec * (b + 1) / batches);
         std::vector<uint32_t> bidx(s1 - s0 + 1);
         for (int s = s0; s <= s1; s++) bidx[s - s0] = idx[s] - idx[s0];
@@ -191,7 +205,15 @@ int main(int argc, char **argv) {
         q.numPeaks = (int_t)(idx[s1] - idx[s0]);
         q.batchNum = b;
         q.fileNum = 3;
+        const auto t0 = std::chrono::steady_clock::now();
         check(DSLIM_QuerySpectrum(&q, index.data(), (uint_t)index.size()), "DSLIM_QuerySpectrum");
+        call_ms.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    }
+    {
+        FILE *f = std::fopen((dir + "/timing.txt").c_str(), "w");
+        for (double m : call_ms) std::fprintf(f, "%.4f\n", m);
+        std::fprintf(f, "psms_per_pass %" PRIu64 "\n", g_psms / (uint64_t)repeat);
+        std::fclose(f);
     }
 #ifdef HCX_TSV_DRIVER
     check(DFile_DeinitFiles(), "DFile_DeinitFiles");

@@ -157,3 +157,48 @@ def test_ms2_file_to_device_batch(eng, tmp_path):
     b = eng.pickpeaks(raw, 100, p.base_int, p.min_int)
     same_queries(b.export(), O.pickpeaks(raw, 100, p.base_int, p.min_int), "ms2")
     b.free()
+
+
+def _mult_ref(moz, idx, span):
+    """Most peaks of each spectrum inside span + 1 consecutive bins: a sort and
+    a sliding window (the definition the exact scan's bound uses)."""
+    out = np.zeros(len(idx) - 1, np.uint32)
+    for s in range(len(idx) - 1):
+        t = np.sort(moz[idx[s]:idx[s + 1]].astype(np.int64))
+        if len(t):
+            out[s] = int((np.searchsorted(t, t + span, side="right") - np.arange(len(t))).max())
+    return out
+
+
+def test_uploaded_batch_facts(eng):
+    # spectra of 0..5000 peaks (above the 4096 staged in LDS too), bins drawn
+    # from narrow ranges so that duplicates and dense windows are common
+    rng = np.random.default_rng(11)
+    sizes = rng.choice([0, 1, 2, 7, 64, 65, 300, 4096, 4097, 5000], 60)
+    idx = np.zeros(len(sizes) + 1, np.uint32)
+    idx[1:] = np.cumsum(sizes)
+    k = int(idx[-1])
+    moz = np.concatenate([rng.integers(1000, 1000 + int(rng.choice([5, 50, 5000])), n) for n in sizes]
+                         ).astype(np.uint32) if k else np.zeros(0, np.uint32)
+    inten = rng.integers(0, 1 << 20, k).astype(np.uint32)
+    n = len(sizes)
+    q = W.Queries(moz, inten, idx, np.full(n, 1500.0, np.float32), np.full(n, 2, np.int32),
+                  np.zeros(n, np.float32), np.zeros(n, np.uint32), np.zeros(n, np.int32))
+    b = eng.upload(q)
+    f = b.facts()
+    span = 2 * eng.params.dF
+    ref = _mult_ref(moz, idx, span)
+    log(f"[facts] {n} spectra, {k} peaks, span {span}: max_mult {f['max_mult']} (ref {ref.max()})")
+    assert np.array_equal(f["mult"], ref)
+    assert f["max_mult"] == ref.max()
+    assert f["max_int"] == (int(inten.max()) if k else 0)
+    assert f["max_peaks"] == int(sizes.max())
+    b.free()
+    # the device pickpeaks path computes the same facts for its own batch
+    raw = W.make_raw_spectra(W.Database(W.PRESETS["small"]), W.PRESETS["small"], nspec=300)[0]
+    pb = eng.pickpeaks(raw, 100, 1000000, 10)
+    pf = pb.facts()
+    pq = pb.export()
+    assert np.array_equal(pf["mult"], _mult_ref(pq.moz, pq.idx, span))
+    assert pf["max_int"] == int(pq.intensity.max())
+    pb.free()
