@@ -56,6 +56,8 @@ def main() -> int:
                     help="turn two rocprofv3 --pmc runs of --pmc-probe into the traffic figure")
     ap.add_argument("--no-mirror", action="store_true",
                     help="skip the reference-shaped entry point leg (DSLIM_QuerySpectrum through the C++ driver)")
+    ap.add_argument("--no-merge", action="store_true",
+                    help="skip the shard merge leg (two cyclic index shards: export and CombineResults)")
     ap.add_argument("--pepdb-only", action="store_true",
                     help="time the peptide database stage (LBE + MODS) alone and print its JSON")
     args = ap.parse_args()
@@ -209,6 +211,7 @@ def main() -> int:
     pick = queries_pick(eng, raw, p, args.steps, args.warmup, not args.no_cpu)
     pdb = pepdb_stage(eng, db, p, prm, args.steps, args.warmup, not args.no_cpu)
     mirror = None if args.no_mirror else mirror_leg(db, q, prm, args.steps, args.warmup)
+    merge = None if args.no_merge else merge_leg(eng, db, batch, prm, args.steps, args.warmup, not args.no_cpu)
 
     if sur_info is not None:
         sur_info["surrogate_evalue_ms"] = round(runs["batched+surrogate"]["ems"], 3)
@@ -257,6 +260,7 @@ def main() -> int:
         "queries_pick": pick,
         "pepdb": pdb,
         "mirror": mirror,
+        "merge": merge,
         "psms_valid": valid,
         "tied_spectra": tied,
         "host_fits": int(((res["flags"] & N.RES_FIT_HOST) != 0).sum()),
@@ -265,6 +269,79 @@ def main() -> int:
     print(json.dumps(line), flush=True)
     eng.close()
     return 0
+
+
+def merge_leg(eng, db, batch, prm, steps: int, warmup: int, with_cpu: bool) -> dict:
+    """The one-GPU part of the multi-rank path on the benchmark workload: the
+    database dealt cyclically to G = 2 index shards (every 2nd pepEntry of each
+    length, lbe_internal.cpp:75-103), the batch searched against each shard,
+    each shard's partial result exported to host memory (hcx_part_export: the
+    partRes records and u16 histogram slices a rank writes, one slot of hbins
+    entries per spectrum), then CombineResults over the two (hcx_combine:
+    the records and slices uploaded, merged, the merged tail fit, the fResult
+    rows and keys back).  Timed: export per shard and the merge, host buffers
+    on both sides (they stand for the reference's per-rank files)."""
+    from hicops_amd import native as N
+
+    class Shard:
+        def __init__(self, g, G):
+            self.g, self.G = g, G
+            self.lengths = [L for L in db.lengths if len(db.entries(L)) > g]
+
+        def seqs(self, L):
+            return db.seqs(L)
+
+        def entries(self, L):
+            return np.ascontiguousarray(db.entries(L)[self.g::self.G])
+
+    G = 2
+    packs, slices, exp_ms = [], [], []
+    for g in range(G):
+        ix = eng.construct_db(Shard(g, G))
+        eng.query_spectrum(batch, indices=ix, want_hist=False, want_stats=False, kernel=N.KERNEL_BATCHED)
+        for ixx in ix:
+            ixx.free()
+        ts = []
+        for _ in range(warmup + steps):
+            t0 = time.perf_counter()
+            pk, sl = eng.part_export(batch, qid0=0)
+            ts.append(time.perf_counter() - t0)
+        exp_ms.append(float(np.median(ts[warmup:])) * 1e3)
+        packs.append(pk)
+        slices.append(sl)
+    ts = []
+    for _ in range(warmup + steps):
+        t0 = time.perf_counter()
+        out, keys = eng.combine(packs, slices)
+        ts.append(time.perf_counter() - t0)
+    comb_ms = float(np.median(ts[warmup:])) * 1e3
+    n = batch.nspec
+    reported = int(np.count_nonzero(keys < G))   # keys[s] == G: not reported
+    wire = sum(p.nbytes + s.nbytes for p, s in zip(packs, slices))
+    log(f"merge: {G} shards, export {', '.join(f'{x:.1f}' for x in exp_ms)} ms, combine {comb_ms:.1f} ms "
+        f"({wire / 1e9:.2f} GB of records), {reported} spectra reported")
+    res = {"shards": G, "spectra": n, "slot": int(slices[0].shape[1]), "wire_bytes": int(wire),
+           "export_ms": [round(x, 2) for x in exp_ms], "combine_ms": round(comb_ms, 2),
+           "combine_spectra_per_s": round(n / (comb_ms / 1e3), 1), "reported": reported,
+           "includes": "export: device search state to host records; combine: records from host, merge, "
+                       "merged tail fit, rows and keys to host"}
+    if with_cpu:
+        from oracle import binding as O
+        k = min(n, 20000)
+        sub_p = [np.ascontiguousarray(p[:k]) for p in packs]
+        sub_s = [np.ascontiguousarray(s[:k]) for s in slices]
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            out_o, keys_o = O.combine(prm, sub_p, sub_s, int(slices[0].shape[1]))
+            ts.append(time.perf_counter() - t0)
+        t = float(np.median(ts))
+        same = bool(np.array_equal(keys_o, keys[:k]) and np.array_equal(out_o, out[:k]))
+        res["cpu_baseline"] = {"value": round(k / t, 1), "unit": "spectra/s", "cores": 1, "kind": "port",
+                               "sample": f"the oracle's CombineResults over the first {k} spectra of the same "
+                                         f"two shards' records; median of 3",
+                               "same_rows_and_keys": same}
+    return res
 
 
 def mirror_leg(db, q, prm, steps: int, warmup: int) -> dict:

@@ -117,6 +117,25 @@ def test_tsv_writer_matches_reference_format(tmp_path):
         assert abs(float(f[12]) - o["e_x"]) <= 1e-3 * abs(o["e_x"]) + 5e-7, s
         assert f[13] == str(int(o["cpsms"])), s
 
+    # the reference's post-processing (tools/runtime/psm2tsv.py): every partial
+    # TSV of the directory read with pandas (tab-separated, header row), the
+    # frames concatenated in the directory's listing order, written as one
+    # results.tsv with a fresh index column.  The partial files must go through
+    # it unchanged: one row per reported PSM, the 14 columns by header name.
+    pd = pytest.importorskip("pandas")
+    frames = [pd.read_csv(f, sep="\t", index_col=None, header=0) for f in tmp_path.glob("*.tsv")]
+    frame = pd.concat(frames, axis=0, ignore_index=True)
+    assert list(frame.columns) == TSV_HEADER.split("\t")
+    assert len(frame) == len(reported)
+    frame.to_csv(tmp_path / "results.tsv", sep="\t")
+    merged = pd.read_csv(tmp_path / "results.tsv", sep="\t", index_col=0, header=0)
+    got = merged.sort_values("scan_num").reset_index(drop=True)
+    assert list(got["scan_num"]) == [int(s) + 1 for s in reported]
+    assert list(got["matched_ions"]) == [int(res_o[s]["sharedions"]) for s in reported]
+    assert list(got["num_hits"]) == [int(res_o[s]["cpsms"]) for s in reported]
+    assert list(got["peptide"]) == [db.sequence(db.lengths[res_o[s]["idxoffset"]], int(res_o[s]["psid"]))
+                                    for s in reported]
+
 
 class _LbeDb:
     """The oracle's database (sequences and pepEntries per length) in the
