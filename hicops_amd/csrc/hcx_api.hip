@@ -453,8 +453,7 @@ extern "C" int hcx_queries_upload(hcx_ctx *c, const hcx_queries_view *q, hcx_qba
     b->npeaks = npk;
     b->max_peaks = maxp;
     if (b->moz.ensure(4 * npk + 16) || b->inten.ensure(4 * npk + 16) || b->idx.ensure(4ull * (n + 1)) ||
-        b->prec.ensure(4ull * n + 16) || b->chg.ensure(4ull * n + 16) || b->rt.ensure(4ull * n + 16) ||
-        b->mult.ensure(4ull * n + 16)) {
+        b->prec.ensure(4ull * n + 16) || b->chg.ensure(4ull * n + 16) || b->rt.ensure(4ull * n + 16)) {
         delete b;
         return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_queries_upload: device allocation failed");
     }
@@ -470,9 +469,10 @@ extern "C" int hcx_queries_upload(hcx_ctx *c, const hcx_queries_view *q, hcx_qba
         delete b;
         return set_error("hcx_queries_upload: copy failed");
     }
-    // largest intensity and, per spectrum, most peaks inside any 2*dF+1 bins
-    // (bounds how often one fragment bin can be probed), on the device
-    if (int rc = hcx::upload_facts(c, b, maxp)) {
+    // largest intensity and, per spectrum, the most peaks and the largest
+    // intensity sum inside any 2*dF+1 bins (they bound what one fragment bin
+    // can collect), on the device
+    if (int rc = hcx::batch_facts(c, b, maxp)) {
         delete b;
         return rc;
     }
@@ -676,7 +676,7 @@ struct ScanPlan {
     ChunkArgs ca, pa;
     WinArgs wa;
     int kernel = HCX_KERNEL_PLAIN;
-    bool pack = false;       // packed kernel for the spectra within mult_max
+    bool pack = false;       // packed kernel for the spectra it scores exactly
     uint32_t *scan_tmp = nullptr;
     uint32_t *tie2 = nullptr;   // ties of the spectra re-scored on the cursor kernel
 };
@@ -754,17 +754,6 @@ int launch_sequence(hcx_ctx *c, ScanPlan &sp, bool tailfit = true) {
     return HCX_SUCCESS;
 }
 
-// Largest per-spectrum peak multiplicity M for which the packed kernel's
-// words cannot overflow: a peptide series takes at most (speclen / 2) * M
-// hits, each adding at most max_int, and the count must stay below 256 and
-// the sum below 2^24.
-uint32_t packed_mult_max(uint32_t max_half, uint32_t max_int) {
-    if (max_half == 0) return 0xFFFFFFFFu;
-    uint64_t m = 255 / max_half;
-    if (max_int) m = std::min<uint64_t>(m, ((1ull << 24) - 1) / ((uint64_t)max_half * max_int));
-    return (uint32_t)m;
-}
-
 // Fill sp for a search of qb over idx (prepare() done): layouts, the scan
 // plan and every device buffer the sequence uses.
 int plan_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, int kernel, const SearchMem &m,
@@ -823,8 +812,9 @@ int plan_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, int
 
     // the packed kernel: the largest run-table group that fits beside the
     // accumulators (one 1024-thread workgroup per CU)
-    const uint32_t mult_max = packed_mult_max(maxspec / 2, qb->max_int);
-    sp.pack = c->scan_mode == HCX_SCAN_AUTO && mult_max > 0;
+    // (each spectrum is checked against the packed words' exactness bound in
+    // k_windows; the rest go to the two-pass kernel)
+    sp.pack = c->scan_mode == HCX_SCAN_AUTO;
     if (sp.pack) {
         ChunkArgs &pa = sp.pa;
         pa = ca;
@@ -837,11 +827,13 @@ int plan_search(hcx_ctx *c, hcx_index *const *idx, int nidx, hcx_qbatch *qb, int
         pa.grid = (uint32_t)std::max(1, c->num_cu) * kPackedBlocks;
     }
     // windows, work-item numbering, the two-pass list
-    if (!qb->mult.p) return set_error_code(HCX_ERR_INVLD_PARAM, "search: batch without peak multiplicities");
+    if (!qb->mult.p || !qb->wsum.p)
+        return set_error_code(HCX_ERR_INVLD_PARAM, "search: batch without peak multiplicities");
     WinArgs &wa = sp.wa;
     std::memset(&wa, 0, sizeof wa);
     wa.mult = (const uint32_t *)qb->mult.p;
-    wa.mult_max = mult_max;
+    wa.wsum = (const uint32_t *)qb->wsum.p;
+    wa.half = maxspec / 2;
     wa.pack = sp.pack ? 1u : 0u;
     wa.batched = kernel == HCX_KERNEL_BATCHED ? 1u : 0u;
     wa.win = m.win;

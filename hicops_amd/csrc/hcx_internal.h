@@ -126,6 +126,7 @@ struct hcx_qbatch {
     hcx::DeviceBuf moz, inten, idx, prec, chg, rt;
     hcx::DeviceBuf res, hist, stats, feat, pred;
     hcx::DeviceBuf mult;                 // per spectrum: most peaks within 2 * dF + 1 bins
+    hcx::DeviceBuf wsum;                 // per spectrum: largest intensity sum within 2 * dF + 1 bins (saturating)
     // chunked-search state: per-spectrum accumulators, windows, work-item
     // numbering, the two-pass kernel's list
     hcx::DeviceBuf acc, win, packed, counts, item_base, item_spec, item_max, fb, redo;
@@ -134,10 +135,11 @@ struct hcx_qbatch {
 
 namespace hcx {
 
-// The batch facts of a batch uploaded by hcx_queries_upload, on the device
-// (k_pick.hip): per-spectrum multiplicities into b->mult, b->max_int,
-// b->max_mult.  Ends with the stream synchronised.
-int upload_facts(hcx_ctx *c, hcx_qbatch *b, uint32_t maxp);
+// The batch facts of a batch in HBM (uploaded by hcx_queries_upload or made by
+// pickpeaks), on the device (k_pick.hip): per-spectrum multiplicities and
+// windowed intensity sums into b->mult and b->wsum, b->max_int, b->max_mult.
+// Allocates b->mult and b->wsum.  Ends with the stream synchronised.
+int batch_facts(hcx_ctx *c, hcx_qbatch *b, uint32_t maxp);
 
 // Integer division n / d for all 32-bit n (round-up method with a 33-bit
 // effective multiplier): q = (t + ((n - t) >> 1)) >> (s - 1), t = mulhi(m, n).
@@ -232,8 +234,9 @@ struct ChunkArgs {
 // Inputs of the per-spectrum window kernel.
 struct WinArgs {
     const uint32_t *mult;   // per spectrum: most peaks within 2 * dF + 1 bins
-    uint32_t mult_max;      // the packed kernel is exact for mult <= mult_max
-    uint32_t pack;          // 1: spectra within mult_max go to the packed kernel
+    const uint32_t *wsum;   // per spectrum: largest intensity sum within 2 * dF + 1 bins
+    uint32_t half;          // ions per series of the longest length (speclen / 2)
+    uint32_t pack;          // 1: spectra the packed kernel scores exactly go to it
     uint32_t batched;       // 1: count each listed spectrum's work items
     int2 *win;
     uint32_t *counts;       // nspec + 1 (batched)

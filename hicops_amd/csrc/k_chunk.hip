@@ -787,7 +787,11 @@ __global__ __launch_bounds__(256) void k_windows(ChunkArgs a, WinArgs w) {
     cnt = (uint32_t)__builtin_amdgcn_readlane((int)wave_reduce_last(cnt, [](uint32_t u, uint32_t v) { return u + v; }),
                                               63);
     if (lane != 0) return;
-    const bool packed = w.pack && w.mult[s] <= w.mult_max;
+    // a series takes at most half hits per peak window of 2 * dF + 1 bins
+    // (one ion per entry, the hits of one ion from one window): its count
+    // must stay below 256 and its intensity sum below 2^24
+    const bool packed = w.pack && (uint64_t)w.half * w.mult[s] <= 255u &&
+                        (uint64_t)w.half * w.wsum[s] < (1ull << 24);
     w.packed[s] = packed ? 1 : 0;
     if (!packed) w.acc[s].flags = HCX_RES_TWO_PASS;   // before any scan kernel ORs its flags in
     if (w.batched) w.counts[s] = (packed || !w.pack) ? cnt : 0u;

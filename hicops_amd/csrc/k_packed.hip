@@ -25,11 +25,13 @@
 // histogram atomics, hs-keyed 64-bit maxima); tied maxima go to the in-order
 // heap replay there.
 //
-// Exactness: one ion bin is probed by at most M peaks, M = the spectrum's
-// most peaks within 2 * dF + 1 bins, so a series of a peptide takes at most
-// speclen / 2 * M hits, each adding at most the batch's largest intensity.
-// The host marks a spectrum packed only if that bound keeps the count below
-// 256 and the sum below 2^24: no field can carry into the next.  The other
+// Exactness: the peaks that hit one ion bin lie within dF of it, in one
+// window of 2 * dF + 1 bins, so they number at most M (the spectrum's most
+// peaks within 2 * dF + 1 bins) and their intensities sum to at most W (its
+// largest intensity sum within 2 * dF + 1 bins); a series of speclen / 2 ions
+// takes at most speclen / 2 * M hits summing to at most speclen / 2 * W.  A
+// spectrum is marked packed (k_windows) only if those stay below 256 and
+// 2^24 for the longest length: no field can carry into the next.  The other
 // spectra run on the two-pass kernel (k_chunk.hip), whose counts and sums
 // are separate.
 //

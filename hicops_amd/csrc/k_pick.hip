@@ -65,11 +65,10 @@ struct PickArgs {
     float *prec;
     int32_t *chg;
     float *rt;
-    uint32_t *imax_s, *mult_s;   // per spectrum: largest kept intensity, most kept peaks within span+1 bins
     int32_t n;
-    uint32_t qalen, base_int, min_int, span;
+    uint32_t qalen, base_int, min_int;
     uint32_t lds_cap;            // spectra of this launch have at most lds_cap peaks (0: scratch rows)
-    uint32_t tmp_words;          // range list (lds_cap/2 pairs), then the kept bins
+    uint32_t tmp_words;          // the sort's range list (lds_cap/2 pairs)
 };
 
 // keyval.h:50-80: one partition of [l, r] around key[(l+r)/2].  The scans
@@ -441,44 +440,13 @@ __device__ __forceinline__ void pick_one(const PickArgs &a, int s, int S, uint32
     // msquery.cpp:548-562: the last min(speclen, QALEN) peaks, ascending intensity
     const int keep = (int)min(speclen, a.qalen);
     const uint32_t o = a.ub[s];
-    uint32_t imax = 0;
     for (int t = lane; t < keep; t += 64) {
         const int pos = S - keep + t;
-        const uint32_t in = pos == S - 1 ? a.base_int : scale_int(key[pos], factor);
-        const uint32_t mz = val[pos];
-        a.st_in[o + t] = in;
-        a.st_moz[o + t] = mz;
-        tmp[t] = mz;
-        imax = max(imax, in);
+        a.st_in[o + t] = pos == S - 1 ? a.base_int : scale_int(key[pos], factor);
+        a.st_moz[o + t] = val[pos];
     }
-    __syncthreads();
-    // most kept peaks within span+1 consecutive bins: for each kept peak, the
-    // kept peaks in [bin, bin + span] (every lane reads the same bin per step)
-    uint32_t mult = 0;
-    for (int t0 = 0; t0 < keep; t0 += 64) {
-        const int t = t0 + lane;
-        const uint32_t mx = t < keep ? tmp[t] : 0u;
-        uint32_t c = 0;
-        const int k4 = keep & ~3;   // tmp is 16-byte aligned
-        for (int y = 0; y < k4; y += 4) {
-            const uint4 v = *reinterpret_cast<const uint4 *>(tmp + y);
-            c += (v.x >= mx && v.x - mx <= a.span) ? 1u : 0u;
-            c += (v.y >= mx && v.y - mx <= a.span) ? 1u : 0u;
-            c += (v.z >= mx && v.z - mx <= a.span) ? 1u : 0u;
-            c += (v.w >= mx && v.w - mx <= a.span) ? 1u : 0u;
-        }
-        for (int y = k4; y < keep; y++) {
-            const uint32_t v = tmp[y];
-            c += (v >= mx && v - mx <= a.span) ? 1u : 0u;
-        }
-        if (t < keep) mult = max(mult, c);
-    }
-    imax = wave_max_u32(imax);
-    mult = wave_max_u32(mult);
     if (lane == 0) {
         a.cnt[s] = (uint32_t)keep;
-        a.imax_s[s] = imax;
-        a.mult_s[s] = mult;
         // msquery.cpp:515-517
         a.prec[s] = (float)a.prec_mz[s];
         const int32_t zz = a.z[s];
@@ -536,73 +504,119 @@ __global__ __launch_bounds__(256) void k_pick_compact(const uint32_t *ub, const 
     }
 }
 
-// Batch facts of a Queries batch uploaded as it is (hcx_queries_upload):
-// per spectrum the most peaks within span + 1 consecutive bins (the broadcast
-// count of pick_one over all of the spectrum's peaks) and its largest
-// intensity.  One wavefront per spectrum; the bins are staged in LDS when the
-// spectrum has at most `cap` peaks, else read from HBM.
-__global__ __launch_bounds__(64) void k_upload_facts(const uint32_t *moz, const uint32_t *inten, const uint32_t *idx,
-                                                     uint32_t span, uint32_t cap, uint32_t *imax_s, uint32_t *mult_s) {
+// Batch facts of a Queries batch in HBM (uploaded as it is, or made by
+// pickpeaks): per spectrum its peak count, its largest intensity, the most
+// peaks within span + 1 consecutive bins and the largest intensity sum within
+// span + 1 bins (saturating at 2^32 - 1), by a broadcast count over its peaks
+// (for each peak, the peaks in [bin, bin + span]).  One wavefront per
+// spectrum; bins and intensities are staged in LDS when the spectrum has at
+// most `cap` peaks (a multiple of 4), else read from HBM.
+// STAGED: bins and ints are 16-byte aligned LDS arrays, read four at a time
+template <bool STAGED>
+__device__ __forceinline__ void window_facts(const uint32_t *bins, const uint32_t *ints, uint32_t P, uint32_t span,
+                                             uint32_t lane, uint32_t *mult, uint32_t *wsum) {
+    uint32_t m = 0;
+    unsigned long long wm = 0;
+    const uint32_t P4 = STAGED ? P & ~3u : 0u;
+    for (uint32_t t0 = 0; t0 < P; t0 += 64) {
+        const uint32_t t = t0 + lane;
+        const uint32_t mx = t < P ? bins[t] : 0u;
+        uint32_t c = 0;
+        unsigned long long w = 0;
+        auto add = [&](uint32_t v, uint32_t i) {
+            const bool in = v >= mx && v - mx <= span;
+            c += in ? 1u : 0u;
+            w += in ? i : 0u;
+        };
+        for (uint32_t y = 0; y < P4; y += 4) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(bins + y);
+            const uint4 i = *reinterpret_cast<const uint4 *>(ints + y);
+            add(v.x, i.x);
+            add(v.y, i.y);
+            add(v.z, i.z);
+            add(v.w, i.w);
+        }
+        for (uint32_t y = P4; y < P; y++) add(bins[y], ints[y]);
+        if (t < P) {
+            m = max(m, c);
+            wm = max(wm, w);
+        }
+    }
+    *mult = m;
+    *wsum = (uint32_t)min(wm, 0xFFFFFFFFull);
+}
+
+__global__ __launch_bounds__(64) void k_batch_facts(const uint32_t *moz, const uint32_t *inten, const uint32_t *idx,
+                                                    uint32_t span, uint32_t cap, uint32_t *cnt_s, uint32_t *imax_s,
+                                                    uint32_t *mult_s, uint32_t *wsum_s) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    uint32_t *tmp = reinterpret_cast<uint32_t *>(smem_raw);
+    uint32_t *tb = reinterpret_cast<uint32_t *>(smem_raw);
+    uint32_t *ti = tb + cap;
     const int s = (int)blockIdx.x;
     const uint32_t lane = threadIdx.x;
     const uint32_t b = idx[s], P = idx[s + 1] - b;
     const bool staged = P <= cap;
     uint32_t imax = 0;
     for (uint32_t t = lane; t < P; t += 64) {
-        imax = max(imax, inten[b + t]);
-        if (staged) tmp[t] = moz[b + t];
+        const uint32_t in = inten[b + t];
+        imax = max(imax, in);
+        if (staged) {
+            tb[t] = moz[b + t];
+            ti[t] = in;
+        }
     }
     __syncthreads();
-    const uint32_t *bins = staged ? tmp : moz + b;
-    uint32_t mult = 0;
-    for (uint32_t t0 = 0; t0 < P; t0 += 64) {
-        const uint32_t t = t0 + lane;
-        const uint32_t mx = t < P ? bins[t] : 0u;
-        uint32_t c = 0;
-        for (uint32_t y = 0; y < P; y++) {
-            const uint32_t v = bins[y];
-            c += (v >= mx && v - mx <= span) ? 1u : 0u;
-        }
-        if (t < P) mult = max(mult, c);
-    }
+    uint32_t mult, ws;
+    if (staged) window_facts<true>(tb, ti, P, span, lane, &mult, &ws);
+    else window_facts<false>(moz + b, inten + b, P, span, lane, &mult, &ws);
     imax = wave_max_u32(imax);
     mult = wave_max_u32(mult);
+    ws = wave_max_u32(ws);
     if (lane == 0) {
+        cnt_s[s] = P;
         imax_s[s] = imax;
         mult_s[s] = mult;
+        wsum_s[s] = ws;
     }
 }
 
 }  // namespace
 
-// The batch facts of an uploaded batch (b->moz, inten, idx resident): per
-// spectrum multiplicities into b->mult, the batch's largest intensity and
-// multiplicity into b->max_int, b->max_mult.  maxp: the largest spectrum.
-// Ends with the stream synchronised.
-int upload_facts(hcx_ctx *c, hcx_qbatch *b, uint32_t maxp) {
+// The batch facts of b (moz, inten, idx resident): per spectrum into b->mult
+// and b->wsum (allocated by the caller), and the maxima of the peak counts,
+// intensities and multiplicities into summary[0..2] (16 bytes, zeroed here).
+// tmp: 2 * nspec words.  maxp bounds the largest spectrum.  Enqueues only.
+int batch_facts_launch(hcx_ctx *c, hcx_qbatch *b, uint32_t maxp, uint32_t *tmp, uint32_t *summary) {
     const int n = b->nspec;
-    b->max_int = b->max_mult = 0;
     if (n <= 0) return HCX_SUCCESS;
     hipStream_t st = c->stream;
-    const uint32_t cap = std::min<uint32_t>(maxp, 4096);
-    DeviceBuf imx, summary;
-    if (imx.ensure(4ull * n + 16) || summary.ensure(64))
-        return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_queries_upload: device allocation failed");
-    HCX_HIP(hipMemsetAsync(summary.p, 0, 16, st));
-    k_upload_facts<<<(unsigned)n, 64, 4u * cap + 16, st>>>((const uint32_t *)b->moz.p, (const uint32_t *)b->inten.p,
-                                                           (const uint32_t *)b->idx.p, 2 * c->params.dF, cap,
-                                                           (uint32_t *)imx.p, (uint32_t *)b->mult.p);
+    // a multiple of 4, so the intensities after the bins stay 16-byte aligned
+    const uint32_t cap = (std::min<uint32_t>(maxp, 4096) + 3u) & ~3u;
+    HCX_HIP(hipMemsetAsync(summary, 0, 16, st));
+    k_batch_facts<<<(unsigned)n, 64, 8u * cap + 16, st>>>(
+        (const uint32_t *)b->moz.p, (const uint32_t *)b->inten.p, (const uint32_t *)b->idx.p, 2 * c->params.dF, cap,
+        tmp, tmp + n, (uint32_t *)b->mult.p, (uint32_t *)b->wsum.p);
     HCX_HIP(hipGetLastError());
-    // the count field of the summary is not needed here: the intensities
-    // stand in for it
-    k_pick_max<<<(unsigned)std::min<int32_t>((n + 255) / 256, 1024), 256, 0, st>>>(
-        (const uint32_t *)imx.p, (const uint32_t *)imx.p, (const uint32_t *)b->mult.p, n, (uint32_t *)summary.p);
+    k_pick_max<<<(unsigned)std::min<int32_t>((n + 255) / 256, 1024), 256, 0, st>>>(tmp, tmp + n, (const uint32_t *)b->mult.p,
+                                                                                  n, summary);
     HCX_HIP(hipGetLastError());
+    return HCX_SUCCESS;
+}
+
+int batch_facts(hcx_ctx *c, hcx_qbatch *b, uint32_t maxp) {
+    const int n = b->nspec;
+    b->max_peaks = b->max_int = b->max_mult = 0;
+    if (b->mult.ensure(4ull * std::max(n, 0) + 16) || b->wsum.ensure(4ull * std::max(n, 0) + 16))
+        return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "batch facts: device allocation failed");
+    if (n <= 0) return HCX_SUCCESS;
+    DeviceBuf tmp, summary;
+    if (tmp.ensure(8ull * n + 16) || summary.ensure(64))
+        return set_error_code(HCX_ERR_BAD_MEM_ALLOC, "batch facts: device allocation failed");
+    if (int rc = batch_facts_launch(c, b, maxp, (uint32_t *)tmp.p, (uint32_t *)summary.p)) return rc;
     uint32_t h[4] = {0, 0, 0, 0};
-    HCX_HIP(hipMemcpyAsync(h, summary.p, 12, hipMemcpyDeviceToHost, st));
-    HCX_HIP(hipStreamSynchronize(st));
+    HCX_HIP(hipMemcpyAsync(h, summary.p, 12, hipMemcpyDeviceToHost, c->stream));
+    HCX_HIP(hipStreamSynchronize(c->stream));
+    b->max_peaks = h[0];
     b->max_int = h[1];
     b->max_mult = h[2];
     return HCX_SUCCESS;
@@ -709,7 +723,7 @@ extern "C" int hcx_queries_pick(hcx_ctx *c, hcx_rawbatch *rb, const hcx_pick_par
     b->file_num = rb->file_num;
     if (b->moz.ensure(4 * tot + 16) || b->inten.ensure(4 * tot + 16) || b->idx.ensure(4ull * (n + 1)) ||
         b->prec.ensure(4ull * n + 16) || b->chg.ensure(4ull * n + 16) || b->rt.ensure(4ull * n + 16) ||
-        b->mult.ensure(4ull * n + 16) || rb->ub.ensure(4ull * n + 16) || rb->cnt.ensure(4ull * n + 16) || rb->st.ensure(8 * tot + 16) ||
+        b->mult.ensure(4ull * n + 16) || b->wsum.ensure(4ull * n + 16) || rb->ub.ensure(4ull * n + 16) || rb->cnt.ensure(4ull * n + 16) || rb->st.ensure(8 * tot + 16) ||
         rb->summary.ensure(16) || rb->aux.ensure(8ull * n + 16) ||
         rb->scan.ensure(4 * hcx::scan_tmp_words((uint64_t)n + 1) + 64)) {
         delete b;
@@ -754,13 +768,10 @@ extern "C" int hcx_queries_pick(hcx_ctx *c, hcx_rawbatch *rb, const hcx_pick_par
     a.prec = (float *)b->prec.p;
     a.chg = (int32_t *)b->chg.p;
     a.rt = (float *)b->rt.p;
-    a.imax_s = (uint32_t *)rb->aux.p;
-    a.mult_s = (uint32_t *)b->mult.p;   // kept with the batch: the search needs it per spectrum
     a.n = n;
     a.qalen = pp->qalen;
     a.base_int = pp->base_int;
     a.min_int = pp->min_int;
-    a.span = 2 * c->params.dF;
     if (hipEventRecord(e0, st)) return fail("hcx_queries_pick: event record failed");
     // one launch per size class, each with the LDS its largest spectrum needs
     // (small spectra then fit more workgroups per CU).  The classes are
@@ -780,9 +791,10 @@ extern "C" int hcx_queries_pick(hcx_ctx *c, hcx_rawbatch *rb, const hcx_pick_par
         if (!cnt) continue;
         a.order = (const uint32_t *)rb->order.p + rb->bin_off[k];
         a.lds_cap = k < hcx::kPickBins ? 256u << k : 0u;
-        // the range list and the kept bins; keeps the chunk masks after it
-        // 16-byte aligned
-        a.tmp_words = k < hcx::kPickBins ? a.lds_cap : (std::min(pp->qalen, rb->max_len) + 3u) & ~3u;
+        // the sort's range list (a multiple of 4 words: the chunk masks after
+        // it stay 16-byte aligned); the one-lane sort of the scratch rows
+        // needs none
+        a.tmp_words = k < hcx::kPickBins ? a.lds_cap : 0u;
         const size_t lds = (2ull * a.lds_cap + a.tmp_words) * 4 + hcx::kStackMax * 8;
         hipStream_t sk = (k == 1 || k == 2) ? c->aux[k - 1] : st;
         hcx::k_pickpeaks<<<cnt, 64, lds, sk>>>(a);
@@ -799,15 +811,16 @@ extern "C" int hcx_queries_pick(hcx_ctx *c, hcx_rawbatch *rb, const hcx_pick_par
                                        (uint32_t *)rb->scan.p) != HCX_SUCCESS) {
         return fail("hcx_queries_pick: scan failed");
     }
-    if (n > 0) {
-        hcx::k_pick_max<<<(unsigned)std::min<int32_t>((n + 255) / 256, 1024), 256, 0, st>>>(
-            (const uint32_t *)rb->cnt.p, a.imax_s, a.mult_s, n, (uint32_t *)rb->summary.p);
-    }
     if (n > 0)
         hcx::k_pick_compact<<<(unsigned)((n + 3) / 4), 256, 0, st>>>(
             (const uint32_t *)rb->ub.p, (const uint32_t *)rb->cnt.p, (const uint32_t *)b->idx.p, a.st_moz, a.st_in,
             (uint32_t *)b->moz.p, (uint32_t *)b->inten.p, n);
-    if (hipGetLastError() || hipEventRecord(e3, st)) return fail("hcx_queries_pick: launch failed");
+    if (hipGetLastError()) return fail("hcx_queries_pick: launch failed");
+    // the batch facts the search sizes its exact scan by, from the batch itself
+    if (hcx::batch_facts_launch(c, b, std::min<uint32_t>(pp->qalen, rb->max_len), (uint32_t *)rb->aux.p,
+                                (uint32_t *)rb->summary.p) != HCX_SUCCESS ||
+        hipEventRecord(e3, st))
+        return fail("hcx_queries_pick: launch failed");
     uint32_t npk = 0;
     if (hipMemcpyAsync(summary, rb->summary.p, 12, hipMemcpyDeviceToHost, st) ||
         hipMemcpyAsync(&npk, (const uint32_t *)b->idx.p + n, 4, hipMemcpyDeviceToHost, st) ||
@@ -845,7 +858,7 @@ extern "C" int hcx_queries_info(const hcx_qbatch *qb, int32_t *num_specs, uint64
 }
 
 extern "C" int hcx_queries_facts(const hcx_qbatch *qb, uint32_t *max_peaks, uint32_t *max_int, uint32_t *max_mult,
-                                 uint32_t *mult) {
+                                 uint32_t *mult, uint32_t *wsum) {
     if (!qb) return set_error_code(HCX_ERR_INVLD_PTR, "hcx_queries_facts: null batch");
     if (max_peaks) *max_peaks = qb->max_peaks;
     if (max_int) *max_int = qb->max_int;
@@ -854,6 +867,11 @@ extern "C" int hcx_queries_facts(const hcx_qbatch *qb, uint32_t *max_peaks, uint
         HCX_HIP(hipSetDevice(qb->ctx->device));
         HCX_HIP(hipStreamSynchronize(qb->ctx->stream));
         HCX_HIP(hipMemcpy(mult, qb->mult.p, 4ull * qb->nspec, hipMemcpyDeviceToHost));
+    }
+    if (wsum && qb->nspec > 0) {
+        HCX_HIP(hipSetDevice(qb->ctx->device));
+        HCX_HIP(hipStreamSynchronize(qb->ctx->stream));
+        HCX_HIP(hipMemcpy(wsum, qb->wsum.p, 4ull * qb->nspec, hipMemcpyDeviceToHost));
     }
     return HCX_SUCCESS;
 }

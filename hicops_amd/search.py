@@ -149,11 +149,13 @@ class Batch:
     def facts(self) -> dict:
         """The batch facts the search sizes its exact scan by: largest
         spectrum, largest intensity, most peaks within 2*dF+1 bins (batch and
-        per spectrum)."""
+        per spectrum) and, per spectrum, the largest intensity sum within
+        2*dF+1 bins."""
         mp, mi, mm = C.c_uint32(), C.c_uint32(), C.c_uint32()
         mult = np.zeros(self.nspec, np.uint32)
-        N.check(N.hcx().hcx_queries_facts(self._h, C.byref(mp), C.byref(mi), C.byref(mm), N.ptr(mult)))
-        return {"max_peaks": mp.value, "max_int": mi.value, "max_mult": mm.value, "mult": mult}
+        wsum = np.zeros(self.nspec, np.uint32)
+        N.check(N.hcx().hcx_queries_facts(self._h, C.byref(mp), C.byref(mi), C.byref(mm), N.ptr(mult), N.ptr(wsum)))
+        return {"max_peaks": mp.value, "max_int": mi.value, "max_mult": mm.value, "mult": mult, "wsum": wsum}
 
     def free(self) -> None:
         if getattr(self, "_h", None):

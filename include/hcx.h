@@ -342,10 +342,13 @@ int  hcx_queries_export(const hcx_qbatch *qb, uint32_t *moz, uint32_t *intensity
 /* The batch facts the search sizes its exact scan by (no reference
  * counterpart; computed on the device at upload or pick time): the largest
  * spectrum, the largest intensity, the most peaks of one spectrum within any
- * 2*dF+1 consecutive bins, and that count per spectrum (mult: num_specs
- * entries, or NULL).  Any argument may be NULL.                              */
+ * 2*dF+1 consecutive bins, and per spectrum that count (mult) and the largest
+ * intensity sum within 2*dF+1 bins (wsum, saturating at 2^32-1): a spectrum
+ * is scored by the packed single-pass scan when (speclen/2) * mult < 256 and
+ * (speclen/2) * wsum < 2^24 for the longest length.  mult, wsum: num_specs
+ * entries.  Any argument may be NULL.                                        */
 int  hcx_queries_facts(const hcx_qbatch *qb, uint32_t *max_peaks, uint32_t *max_int, uint32_t *max_mult,
-                       uint32_t *mult);
+                       uint32_t *mult, uint32_t *wsum);
 
 /* ---- search: DSLIM_QuerySpectrum + ModelTailFit + e-value ----
  * results:  num_specs entries (host).            Required.

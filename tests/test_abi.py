@@ -211,7 +211,7 @@ int main(void) {
     assert lib.hcx_queries_pick(None, None, None, C.byref(h), None, None) == N.ERR_INVLD_PTR
     assert lib.hcx_queries_info(None, None, None) == N.ERR_INVLD_PTR
     assert lib.hcx_queries_export(None, None, None, None, None, None, None) == N.ERR_INVLD_PTR
-    assert lib.hcx_queries_facts(None, None, None, None, None) == N.ERR_INVLD_PTR
+    assert lib.hcx_queries_facts(None, None, None, None, None, None) == N.ERR_INVLD_PTR
 
 
 def test_status_codes_are_the_reference_values(tmp_path):

@@ -135,16 +135,17 @@ def test_query_parity_small(small_case, kernel, scan):
 
 
 def crowd_peaks(q, every, copies):
-    """Every `every`-th spectrum gets `copies` extra peaks at the bin of its
-    first peak: more peaks within 2 dF + 1 bins than the packed scan admits,
-    so those spectra go to the two-pass kernel."""
+    """Every `every`-th spectrum gets `copies` extra copies of its most intense
+    peak (the last: peaks are stored in ascending intensity): more peaks and
+    more intensity within 2 dF + 1 bins than the packed scan admits, so those
+    spectra go to the two-pass kernel."""
     moz, inten, idx = [], [], [0]
     for s in range(q.nspec):
         m = q.moz[q.idx[s]:q.idx[s + 1]]
         i = q.intensity[q.idx[s]:q.idx[s + 1]]
         if s % every == 0 and len(m):
-            m = np.concatenate([m, np.full(copies, m[0], np.uint32)])
-            i = np.concatenate([i, np.full(copies, i[0], np.uint32)])
+            m = np.concatenate([m, np.full(copies, m[-1], np.uint32)])
+            i = np.concatenate([i, np.full(copies, i[-1], np.uint32)])
         moz.append(m)
         inten.append(i)
         idx.append(idx[-1] + len(m))
@@ -157,7 +158,7 @@ def crowd_peaks(q, every, copies):
 def test_packed_and_two_pass_mixed(small_case, kernel):
     # a batch split between the packed kernel and the two-pass kernel's list
     p, db, q0, prm = small_case
-    q = crowd_peaks(q0, 3, 3)
+    q = crowd_peaks(q0, 3, 4)
     log(f"test_packed_and_two_pass_mixed kernel={kernel}")
     res_o, hist_o, st_o = O.OracleDB(prm, db).search(q)
     eng = Engine(prm)

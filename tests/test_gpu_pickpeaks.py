@@ -159,15 +159,25 @@ def test_ms2_file_to_device_batch(eng, tmp_path):
     b.free()
 
 
-def _mult_ref(moz, idx, span):
-    """Most peaks of each spectrum inside span + 1 consecutive bins: a sort and
-    a sliding window (the definition the exact scan's bound uses)."""
-    out = np.zeros(len(idx) - 1, np.uint32)
-    for s in range(len(idx) - 1):
-        t = np.sort(moz[idx[s]:idx[s + 1]].astype(np.int64))
+def _facts_ref(moz, inten, idx, span):
+    """Per spectrum, the most peaks and the largest intensity sum inside
+    span + 1 consecutive bins: a sort and a sliding window (the definitions
+    the exact scan's bound uses)."""
+    n = len(idx) - 1
+    mult = np.zeros(n, np.uint32)
+    wsum = np.zeros(n, np.uint64)
+    for s in range(n):
+        a, b = int(idx[s]), int(idx[s + 1])
+        o = np.argsort(moz[a:b], kind="stable")
+        t = moz[a:b][o].astype(np.int64)
+        w = inten[a:b][o].astype(np.int64)
         if len(t):
-            out[s] = int((np.searchsorted(t, t + span, side="right") - np.arange(len(t))).max())
-    return out
+            e = np.searchsorted(t, t + span, side="right")
+            c = np.concatenate([[0], np.cumsum(w)])
+            i = np.arange(len(t))
+            mult[s] = int((e - i).max())
+            wsum[s] = int((c[e] - c[i]).max())
+    return mult, np.minimum(wsum, 0xFFFFFFFF).astype(np.uint32)
 
 
 def test_uploaded_batch_facts(eng):
@@ -180,16 +190,19 @@ def test_uploaded_batch_facts(eng):
     k = int(idx[-1])
     moz = np.concatenate([rng.integers(1000, 1000 + int(rng.choice([5, 50, 5000])), n) for n in sizes]
                          ).astype(np.uint32) if k else np.zeros(0, np.uint32)
-    inten = rng.integers(0, 1 << 20, k).astype(np.uint32)
+    # 1 % of the intensities near 2^31: dense windows saturate the sums
+    inten = np.where(rng.random(k) < 0.01, rng.integers(1 << 30, 1 << 31, k),
+                     rng.integers(0, 1 << 20, k)).astype(np.uint32)
     n = len(sizes)
     q = W.Queries(moz, inten, idx, np.full(n, 1500.0, np.float32), np.full(n, 2, np.int32),
                   np.zeros(n, np.float32), np.zeros(n, np.uint32), np.zeros(n, np.int32))
     b = eng.upload(q)
     f = b.facts()
     span = 2 * eng.params.dF
-    ref = _mult_ref(moz, idx, span)
+    ref, wref = _facts_ref(moz, inten, idx, span)
     log(f"[facts] {n} spectra, {k} peaks, span {span}: max_mult {f['max_mult']} (ref {ref.max()})")
     assert np.array_equal(f["mult"], ref)
+    assert np.array_equal(f["wsum"], wref)
     assert f["max_mult"] == ref.max()
     assert f["max_int"] == (int(inten.max()) if k else 0)
     assert f["max_peaks"] == int(sizes.max())
@@ -199,6 +212,9 @@ def test_uploaded_batch_facts(eng):
     pb = eng.pickpeaks(raw, 100, 1000000, 10)
     pf = pb.facts()
     pq = pb.export()
-    assert np.array_equal(pf["mult"], _mult_ref(pq.moz, pq.idx, span))
+    pm, pw = _facts_ref(pq.moz, pq.intensity, pq.idx, span)
+    assert np.array_equal(pf["mult"], pm)
+    assert np.array_equal(pf["wsum"], pw)
     assert pf["max_int"] == int(pq.intensity.max())
+    assert pf["max_peaks"] == int(np.diff(pq.idx.astype(np.int64)).max())
     pb.free()
