@@ -34,6 +34,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <sstream>
 #include <string>
@@ -847,6 +848,8 @@ int parse_mods(const char *text, const hcx_params &p, hcx::LbeMods *m) {
     if (!end || *end || lim < 0 || lim > 8)
         return set_error_code(HCX_ERR_INVLD_MOD, "modconditions: the mod limit must be 0..8");
     m->limit = (int32_t)lim;
+    // pairs of (residues, count) after the limit; like MODS_Initialize
+    // (mods.cpp:89-100), a trailing residue token without a count is ignored
     const size_t nt = (tok.size() - 1) / 2;
     if (nt > HCX_MAX_MOD_TYPES) return set_error_code(HCX_ERR_INVLD_MOD, "modconditions: too many mod types");
     m->ntypes = (uint32_t)nt;
@@ -866,13 +869,18 @@ int parse_mods(const char *text, const hcx_params &p, hcx::LbeMods *m) {
     return HCX_SUCCESS;
 }
 
+// The binomial table into c_comb of the current device, once per device
+// (callers on several threads may race to it: the flag is taken under a lock,
+// the table is built on the caller's stack).
 int comb_table_upload(int device) {
+    static std::mutex mu;
     static bool done[64] = {false};
     if (device < 0 || device >= 64) return set_error("comb table: device id");
+    std::lock_guard<std::mutex> g(mu);
     if (done[device]) return HCX_SUCCESS;
     // mods.cpp:137-160 MODS_GenCombinations
-    static unsigned long long C[64][64];
-    std::memset(C, 0, sizeof C);
+    std::vector<unsigned long long> tab(64 * 64, 0ull);
+    auto C = reinterpret_cast<unsigned long long(*)[64]>(tab.data());
     C[0][0] = 1;
     C[1][0] = 1;
     C[1][1] = 1;
@@ -882,7 +890,7 @@ int comb_table_upload(int device) {
         for (int j = i / 2; j < i; j++) C[i][j] = C[i][i - j];
         C[i][i] = 1;
     }
-    HCX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(hcx::c_comb), C, sizeof C));
+    HCX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(hcx::c_comb), tab.data(), sizeof(unsigned long long) * 64 * 64));
     done[device] = true;
     return HCX_SUCCESS;
 }
