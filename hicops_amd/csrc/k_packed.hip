@@ -335,8 +335,11 @@ __device__ __forceinline__ void process_item_p(const ChunkArgs &a, const PSmem &
     // with bc + yc >= min_shp are few and scattered, so each wavefront queues
     // its own in a private part of the run table (free by now; no atomics)
     // and scores them 64 at a time, instead of taking the scoring branch for
-    // one or two lanes of every read.
-    __syncthreads();
+    // one or two lanes of every read.  The last probe group's barrier has
+    // every wavefront's atomics done; an item without probes had no pass, and
+    // needs a barrier of its own before its words are read (a wavefront may
+    // still be clearing them in the previous item's scan).
+    if (ng == 0) __syncthreads();
     float imax = 0.0f;
     {
         const uint32_t w = threadIdx.x >> 6;
