@@ -592,3 +592,49 @@ def test_rebuilt_index_same_batch(small_case, use_graph):
             x.free()
         assert not eng.indices
     eng.close()
+
+
+@pytest.mark.parametrize("kernel", ["plain", "batched"])
+def test_packed_bound_at_its_limit(kernel):
+    # Spectra made of one longest-length peptide's own ion bins, peaks more
+    # than 2 dF apart, every intensity I: each ion takes one hit of I, so its
+    # b series sums to half * I with half = speclen / 2 of the longest length.
+    # At I = (2^24 - 1) // half the packed words sit one step below a carry
+    # into the count field and the spectrum must still be packed and exact;
+    # at I + 1 the bound sends it to the two-pass kernel.
+    p = W.scaled("small", npep=3000, nspec=10, dM=100.0)
+    db = W.Database(p)
+    prm = W.make_params(p)
+    L = max(db.lengths)
+    half = prm.maxz * (L - 1)
+    seq = db.sequence(L, 0)
+    mass, bins = O.gen_spectrum(seq, prm)
+    span = 2 * prm.dF
+    kept = []
+    for b in sorted(set(int(x) for x in bins)):
+        if prm.dF < b < prm.max_mass * prm.scale - 1 - prm.dF and (not kept or b - kept[-1] > span):
+            kept.append(b)
+    I = ((1 << 24) - 1) // half
+    k = len(kept)
+    moz = np.array(kept * 2, np.uint32)
+    inten = np.array([I] * k + [I + 1] * k, np.uint32)
+    q = W.Queries(moz, inten, np.array([0, k, 2 * k], np.uint32), np.full(2, mass, np.float32),
+                  np.full(2, 2, np.int32), np.zeros(2, np.float32), np.zeros(2, np.uint32), np.zeros(2, np.int32))
+    log(f"test_packed_bound_at_its_limit kernel={kernel}: L={L} half={half} I={I} peaks={k}")
+    res_o, hist_o, st_o = O.OracleDB(prm, db).search(q)
+    eng = Engine(prm)
+    eng.construct_db(db)
+    b = eng.upload(q)
+    f = b.facts()
+    assert list(f["mult"]) == [1, 1] and list(f["wsum"]) == [I, I + 1]
+    out = eng.query_spectrum(b, kernel=KERNELS[kernel])
+    compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"bound-{kernel}")
+    two = (out.results["flags"] & N.RES_TWO_PASS) != 0
+    assert list(two) == [False, True]
+    # the peptide's b series in the packed spectrum: nearly every ion hit once,
+    # its sum at most one count step (I) short of 2^24
+    kept_a = np.array(kept, np.int64)
+    hit_b = sum(1 for x in bins[: len(bins) // 2] if np.any(np.abs(kept_a - int(x)) <= prm.dF))
+    log(f"  b ions hit {hit_b} of {half}: b sum {hit_b * I}, limit {(1 << 24) - 1}")
+    assert hit_b >= half - 4 and hit_b * I <= (1 << 24) - 1
+    eng.close()
