@@ -218,3 +218,15 @@ def test_uploaded_batch_facts(eng):
     assert pf["max_int"] == int(pq.intensity.max())
     assert pf["max_peaks"] == int(np.diff(pq.idx.astype(np.int64)).max())
     pb.free()
+    # an empty batch: no facts, and a search of it returns nothing
+    z = W.Queries(np.zeros(0, np.uint32), np.zeros(0, np.uint32), np.zeros(1, np.uint32), np.zeros(0, np.float32),
+                  np.zeros(0, np.int32), np.zeros(0, np.float32), np.zeros(0, np.uint32), np.zeros(0, np.int32))
+    e2 = Engine(eng.params)
+    e2.construct_db(W.Database(W.scaled("small", npep=500)))
+    zb = e2.upload(z)
+    zf = zb.facts()
+    assert (zf["max_peaks"], zf["max_int"], zf["max_mult"]) == (0, 0, 0) and len(zf["mult"]) == 0
+    out = e2.query_spectrum(zb, kernel=N.KERNEL_BATCHED)
+    assert len(out.results) == 0
+    zb.free()
+    e2.close()
