@@ -363,7 +363,7 @@ int index_build_from(hcx_ctx *c, uint32_t peplen, const char *seqs, uint32_t nse
     // refinement, the chunk-local entries and the sort's scratch
     ix->chunk = c->chunk;
     ix->nch = std::max<uint32_t>(1, (nentries + c->chunk - 1) / c->chunk);
-    if (ix->bA2.ensure(4ull * c->NB * (ix->nch + 1)))
+    if (ix->bA2.ensure(4ull * ba2_words(c->NB, ix->nch)))
         return fail(set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_index_build: chunk offsets allocation failed"));
     if (ix->iAc.ensure(2 * nions + 16))
         return fail(set_error_code(HCX_ERR_BAD_MEM_ALLOC, "hcx_index_build: chunk-local entry allocation failed"));

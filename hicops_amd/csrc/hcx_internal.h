@@ -44,9 +44,24 @@ struct LenDesc {
     uint32_t div_s;         // shift
     uint32_t nch;           // peptide chunks (of ctx->chunk peptides)
     uint32_t cbase;         // first (length, chunk) work bucket of this length
-    const uint32_t *bA2;    // (nch + 1) * NB, chunk-major: bA2[c * NB + b] = start of chunk c inside bin b
+    const uint32_t *bA2;    // start of chunk c inside bin b, at ba2_index(b, c, nch)
     const uint16_t *iAc;    // n * speclen: (peptide mod chunk) << 1 | isY, iA order
 };
+
+// Word of bA2 holding where chunk c (0 .. nch) starts inside bin b.  Bins
+// are taken in groups of eight and a group's words are chunk-major, so the
+// two words of a probe's run (chunks c and c + 1) are 32 bytes apart, one
+// 128-byte line holds four consecutive chunks of eight bins (a peak's 2dF + 1
+// bins touch one or two groups), and a spectrum's items on consecutive chunks
+// probe lines an earlier item brought into L2.  Groups of 2, 4 and 16 bins
+// and the plain chunk-major layout measured slower.
+constexpr uint32_t kBa2Group = 8;
+__host__ __device__ inline size_t ba2_index(uint32_t b, uint32_t c, uint32_t nch) {
+    return ((size_t)(b / kBa2Group) * (nch + 1) + c) * kBa2Group + b % kBa2Group;
+}
+__host__ __device__ inline size_t ba2_words(uint32_t NB, uint32_t nch) {
+    return (size_t)(NB + kBa2Group - 1) / kBa2Group * kBa2Group * (nch + 1);
+}
 
 struct DeviceBuf {
     void *p = nullptr;

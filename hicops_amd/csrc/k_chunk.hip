@@ -3,9 +3,11 @@
 // Index refinement: each length's peptides (sorted by mass) are cut into
 // chunks of TC peptides.  Inside a bin the reference iA is ascending in ion
 // id, hence in peptide id, so the entries of one (bin, chunk) pair are a
-// contiguous run; bA2[c * NB + bin] is where chunk c starts inside bin `bin`
-// (bA2[0 * NB + bin] = bA[bin], bA2[nch * NB + bin] = bA[bin+1]): chunk-major,
-// so one item's probes of neighbouring bins share cache lines.  iA itself is the
+// contiguous run; bA2 holds where chunk c starts inside bin `bin` (chunk 0 at
+// bA[bin], chunk nch at bA[bin+1]), at ba2_index(bin, c, nch): bins in groups
+// of eight, each group chunk-major, so a probe's two words share a line, a
+// peak's neighbouring bins share one or two, and consecutive chunks of one
+// spectrum reuse them from L2.  iA itself is the
 // reference array, unchanged; the scan reads iAc, its 16-bit chunk-local
 // form (peptide mod TC) << 1 | isY, which needs neither a division nor the
 // ion offset at query time and halves the bytes read.
@@ -193,9 +195,9 @@ __device__ __forceinline__ uint2 setup_runs(const ChunkArgs &a, const Smem &sm, 
             pk[h] = k;
             const uint32_t q = sm.pk_moz[k];
             if (q > a.q.dF && q < a.q.NB - 1 - a.q.dF) {
-                const size_t o = (size_t)c * a.q.NB + (q - a.q.dF + j);
+                const size_t o = ba2_index(q - a.q.dF + j, c, L.nch);
                 rs[h] = ldg(L.bA2, o);
-                len[h] = ldg(L.bA2, o + a.q.NB) - rs[h];
+                len[h] = ldg(L.bA2, o + kBa2Group) - rs[h];
                 if (count_probes) acc.nprobe++;
             }
         }
@@ -852,10 +854,16 @@ __global__ void k_finalize(ChunkArgs a) {
 
 __global__ void k_chunk_offsets(const uint32_t *iA, const uint32_t *bA, uint32_t NB, uint32_t nch, uint32_t TC,
                                 uint32_t speclen, uint32_t *bA2) {
+    // one thread per word, in storage order (ba2_index)
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t stride = (uint64_t)nch + 1;
-    if (t >= (uint64_t)NB * stride) return;
-    const uint32_t c = (uint32_t)(t / NB), bin = (uint32_t)(t % NB);   // chunk-major
+    if (t >= ba2_words(NB, nch)) return;
+    const uint64_t gw = (uint64_t)kBa2Group * (nch + 1);   // words of a bin group
+    const uint32_t c = (uint32_t)(t % gw / kBa2Group);
+    const uint32_t bin = (uint32_t)(t / gw * kBa2Group + t % kBa2Group);
+    if (bin >= NB) {   // the last group's padding
+        bA2[t] = 0;
+        return;
+    }
     const uint32_t s0 = bA[bin], e0 = bA[bin + 1];
     uint32_t r;
     if (c == 0) {
@@ -897,7 +905,7 @@ size_t chunk_smem_bytes(const ChunkArgs &a) { return make_layout(a).total; }
 
 int launch_chunk_offsets(hcx_ctx *ctx, const uint32_t *iA, const uint32_t *bA, uint32_t NB, uint32_t nch, uint32_t TC,
                          uint32_t speclen, uint32_t *bA2) {
-    const uint64_t n = (uint64_t)NB * (nch + 1);
+    const uint64_t n = ba2_words(NB, nch);
     k_chunk_offsets<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(iA, bA, NB, nch, TC, speclen, bA2);
     HCX_HIP(hipGetLastError());
     return HCX_SUCCESS;

@@ -161,9 +161,9 @@ __device__ __forceinline__ uint2 setup_runs_p(const ChunkArgs &a, const PSmem &s
             pk[h] = k;
             const uint32_t q = sm.pk_moz[k];
             if (q > a.q.dF && q < a.q.NB - 1 - a.q.dF) {
-                const size_t o = (size_t)c * a.q.NB + (q - a.q.dF + j);
+                const size_t o = ba2_index(q - a.q.dF + j, c, L.nch);
                 rs[h] = ldg(L.bA2, o);
-                len[h] = ldg(L.bA2, o + a.q.NB) - rs[h];
+                len[h] = ldg(L.bA2, o + kBa2Group) - rs[h];
                 if (count_probes) acc.nprobe++;
             }
         }
