@@ -48,14 +48,14 @@ constexpr int kPT = (int)kPackedThreads;
 constexpr uint32_t kPPp = 2;
 constexpr uint32_t kCnt = 1u << 24;   // count unit of a packed word
 constexpr double kLog10of2 = 0.30102999566398119521;
-// tiles per wavefront step (16, every load of an item at once, measured
-// slower: register pressure)
+// tiles per wavefront step and accumulator reads in flight per thread in the
+// scoring scan: 4 and 2 measured fastest (tiles 2, 3, 6, 8, 12, 16 and reads
+// 1, 4, 8 slower on the 10 M-peptide open search)
 #ifndef HCX_PACKED_TILE_U
-#define HCX_PACKED_TILE_U 8
+#define HCX_PACKED_TILE_U 4
 #endif
 constexpr uint32_t kUp = HCX_PACKED_TILE_U;
-// accumulator reads in flight per thread in the scoring scan
-constexpr uint32_t kSr = 4;
+constexpr uint32_t kSr = 2;
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 typedef AS_LDS u32x2 lu32x2;
 
