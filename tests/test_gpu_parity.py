@@ -337,6 +337,29 @@ def test_open_search_uneven_windows():
     eng.close()
 
 
+@pytest.mark.parametrize("scan", ["auto", "two-pass"])
+def test_bins_not_a_whole_number_of_offset_groups(scan):
+    # NB = max_mass * scale = 499 900 bins: the (bin, chunk) offsets are stored
+    # in groups of 8 bins, so the last group holds 4 bins and 4 padding words
+    p = W.scaled("small", npep=4000, nspec=300, dM=100.0)
+    db = W.Database(p)
+    q = W.make_queries(db, p)
+    prm = W.make_params(p)
+    prm.max_mass = 4999
+    assert (prm.max_mass * prm.scale) % 8 == 4
+    log(f"test_bins_not_a_whole_number_of_offset_groups[{scan}]: NB {prm.max_mass * prm.scale}")
+    res_o, hist_o, st_o = O.OracleDB(prm, db).search(q)
+    eng = Engine(prm)
+    eng.set_scan(SCANS[scan])
+    eng.construct_db(db)
+    b = eng.upload(q)
+    for kernel in ("plain", "batched"):
+        out = eng.query_spectrum(b, kernel=KERNELS[kernel])
+        n = compare(out.results, out.hist, out.stats, res_o, hist_o, st_o, f"nb-odd-{scan}-{kernel}")
+        assert n > 100
+    eng.close()
+
+
 def test_tailfit_batch_hand_traced():
     prm = W.make_params(W.PRESETS["small"])
     eng = Engine(prm)
