@@ -583,8 +583,8 @@ def cpu_facts() -> dict:
 
 def cpu_baseline(prm, db, q, gpu_stats, seconds: float, runs: int = 3):
     """The oracle (a C restatement of the reference's DSLIM_Construct and
-    query loop, OpenMP over lengths and over spectra) on the GPU box's host
-    cores: index build ions/s over the whole database and query spectra/s
+    query loop, OpenMP over lengths and over spectra) on the host cores of
+    the machine that holds the GPU: index build ions/s over the whole database and query spectra/s
     over a bounded random sample of the same batch against that index, each
     the median of `runs` runs.  gpu_stats (or None): the GPU search's work
     counts, compared with the oracle's for the sample."""
